@@ -1,0 +1,164 @@
+/*
+ * slepc_amd.h -- C ABI of libslepc_amd.so, an MI355X (gfx950) Krylov-Schur hot path.
+ *
+ * Plain pointers and sizes only.  Every function returns 0 on success and a
+ * nonzero code on failure; sa_last_error() returns the message (the analogue
+ * of PetscErrorCode + PetscCheck).  "_dev" pointers are device (HBM) addresses,
+ * "_host" pointers are host memory.  All work is ordered on the library's HIP
+ * stream (sa_stream()).
+ *
+ * The functions map onto the SLEPc/PETSc interfaces the reference's Krylov-Schur
+ * calls (paths relative to the SLEPc 3.22.2 source tree); the registration
+ * shims that bind them as a BV type and a Mat type are in
+ * slepc_amd/shim/ (see the integration notes).
+ */
+#ifndef SLEPC_AMD_H
+#define SLEPC_AMD_H
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct sa_mat_s *sa_mat;   /* CSR operator resident in HBM            */
+typedef struct sa_bv_s  *sa_bv;    /* n x m column-major basis in HBM         */
+typedef struct sa_eps_s *sa_eps;   /* Krylov-Schur eigensolver context        */
+typedef struct sa_mlp_s *sa_mlp;   /* learned warm-start network (BF16 MFMA)  */
+
+enum { SA_FP64 = 0, SA_FP32 = 1 };
+enum { SA_LARGEST_MAGNITUDE = 1, SA_SMALLEST_MAGNITUDE = 2, SA_LARGEST_REAL = 3, SA_SMALLEST_REAL = 4 };
+enum { SA_ERROR_ABSOLUTE = 0, SA_ERROR_RELATIVE = 1 };
+
+/* ---- library ----------------------------------------------------------- */
+/* SlepcInitialize analogue: selects the device and creates the stream. */
+int         sa_init(int device);
+const char *sa_last_error(void);
+void       *sa_stream(void);                    /* hipStream_t */
+int         sa_synchronize(void);
+/* Device memory helpers for hosts without their own allocator. */
+int sa_malloc(void **ptr_dev, int64_t bytes);
+int sa_free(void *ptr_dev);
+int sa_memcpy_h2d(void *dst_dev, const void *src_host, int64_t bytes);
+int sa_memcpy_d2h(void *dst_host, const void *src_dev, int64_t bytes);
+/* HIP events on the library stream around a timed region (milliseconds). */
+int sa_timer_start(void);
+int sa_timer_stop(double *ms);
+
+/* ---- Mat: CSR (MATAIJ) ----------------------------------------------------
+ * Replaces MatCreateSeqAIJWithArrays/MatAssembly + MatMult of PETSc's AIJ
+ * (called at src/sys/classes/bv/interface/bvops.c:879 and
+ * src/sys/classes/st/interface/stsolve.c:22).  Arrays are host CSR with
+ * ascending column indices per row; prec selects the stored value type. */
+int sa_mat_create_csr(int n, int64_t nnz, const int32_t *rowptr_host, const int32_t *col_host,
+                      const double *val_host, int prec, sa_mat *A);
+int sa_mat_destroy(sa_mat A);
+int sa_mat_get_info(sa_mat A, int *n, int64_t *nnz, int *prec);
+/* y = A x with x, y device vectors of the matrix precision (MatMult). */
+int sa_mat_mult(sa_mat A, const void *x_dev, void *y_dev);
+
+/* ---- MatSetValues-style assembly and the stencil families ----------------
+ * COO triplets -> CSR with ascending columns and INSERT_VALUES semantics
+ * (the last value given for a duplicate (row,col) wins), as PETSc's
+ * MatSetValues + MatAssemblyEnd produce.  Returns nnz; fills the outputs when
+ * non-NULL. */
+int64_t sa_csr_assemble(int n, int64_t ntrip, const int32_t *rows, const int32_t *cols,
+                        const double *vals, int32_t *rowptr, int32_t *col, double *val);
+/* Generators used by the workloads; each returns nnz and fills when non-NULL. */
+int64_t sa_gen_lap2d(int nx, int ny, double pot_scale, uint64_t seed,
+                     int32_t *rowptr, int32_t *col, double *val);
+int64_t sa_gen_convdiff3d(int nx, int ny, int nz, int32_t *rowptr, int32_t *col, double *val);
+int64_t sa_gen_stencil27(int nx, int ny, int nz, uint64_t seed,
+                         int32_t *rowptr, int32_t *col, double *val);
+int64_t sa_gen_fv2d(int nx, int ny, const double *theta8, double *sigma,
+                    int32_t *rowptr, int32_t *col, double *val);
+int64_t sa_gen_markov(int m, int32_t *rowptr, int32_t *col, double *val);
+void    sa_fv2d_theta(uint64_t seed, int sample, double *theta8);
+void    sa_rand_uniform(uint64_t seed, int64_t n, double *out_host); /* U[-1,1) */
+double  sa_u01(uint64_t seed, uint64_t i);
+uint64_t sa_sub_seed(uint64_t seed, uint64_t p);
+
+/* ---- BV: basis vectors ------------------------------------------------------
+ * Replaces BVCreate_Mat storage (src/sys/classes/bv/impls/mat/bvmat.c:385-524)
+ * and the ops table slots of struct _BVOps (include/slepc/private/bvimpl.h:25-61).
+ * Storage: m columns of n rows, leading dimension *ld (n rounded up to 64). */
+int sa_bv_create(int64_t n, int m, int prec, sa_bv *V);
+int sa_bv_destroy(sa_bv V);
+int sa_bv_get_info(sa_bv V, int64_t *n, int *m, int64_t *ld, int *prec);
+int sa_bv_get_array(sa_bv V, void **data_dev);                 /* BVGetArray   */
+int sa_bv_set_active_columns(sa_bv V, int l, int k);           /* BVSetActiveColumns bvbasic.c */
+int sa_bv_get_active_columns(sa_bv V, int *l, int *k);
+int sa_bv_set_column(sa_bv V, int j, const double *x_host);    /* BVInsertVec without orth. */
+int sa_bv_get_column(sa_bv V, int j, double *x_host);          /* BVCopyVec to host        */
+/* ops->dotvec (bvmat.c:110-130): q = V(:,l:k)^T x; q_host==NULL stores into
+   column 0 of the coefficient buffer (the q==NULL convention, bvimpl.h:289-415). */
+int sa_bv_dotvec(sa_bv V, const void *x_dev, double *q_host);
+/* ops->multvec (bvmat.c:38-53): y = beta*y + alpha*V(:,l:k) q */
+int sa_bv_multvec(sa_bv V, double alpha, double beta, void *y_dev, const double *q_host);
+/* ops->multinplace (bvmat.c:55-71 -> bvblas.c:74-106): V(:,s:e) = V(:,l:k) Q(l:k,s:e),
+   Q host column-major with leading dimension ldq (MATSEQDENSE); MFMA GEMM. */
+int sa_bv_multinplace(sa_bv V, const double *Q_host, int ldq, int s, int e);
+/* ops->scale (bvmat.c:151-163): column j (j<0: all active columns) times alpha */
+int sa_bv_scale_column(sa_bv V, int j, double alpha);
+/* BVNormColumn NORM_2 (bvglobal.c:662-685) and BVNorm NORM_FROBENIUS over active columns */
+int sa_bv_norm_column(sa_bv V, int j, double *nrm);
+int sa_bv_norm(sa_bv V, double *nrm_fro);
+/* ops->copycolumn (bvmat.c:250-260) */
+int sa_bv_copy_column(sa_bv V, int src, int dst);
+/* BVMatMultColumn (bvops.c:862-885): V(:,j+1) = A V(:,j) */
+int sa_bv_mat_mult_column(sa_bv V, sa_mat A, int j);
+/* BVOrthonormalizeColumn (bvorthog.c:380-427) with classical Gram-Schmidt and
+   DGKS refinement "if needed" (bvorthog.c:91-217), eta as in BVSetOrthogonalization.
+   Coefficients land in column j of the coefficient buffer (norm at row j). */
+int sa_bv_orthonormalize_column(sa_bv V, int j, double eta, double *norm, int *lindep);
+/* Copy columns [c0,c1) of the m x m coefficient buffer to the host (ld m). */
+int sa_bv_get_coefficients(sa_bv V, int c0, int c1, double *buf_host);
+/* The same SpMV + CGS/DGKS step fused for an Arnoldi/Lanczos sweep
+   (BVMatArnoldi/BVMatLanczos bvkrylov.c:56-113,165-226): columns k+1..m are
+   produced from V(:,k); *m is reduced on breakdown. */
+int sa_bv_mat_arnoldi(sa_bv V, sa_mat A, int k, int *m, double eta, double *beta, int *breakdown);
+
+/* ---- EPS: Krylov-Schur --------------------------------------------------------
+ * Mirrors EPSCreate/EPSSetOperators/EPSSetProblemType/EPSSetWhichEigenpairs/
+ * EPSSetDimensions/EPSSetTolerances/EPSSetInitialSpace/EPSSolve/EPSGetConverged/
+ * EPSGetEigenpair/EPSComputeError (src/eps/interface/*.c) for the default
+ * Krylov-Schur variant (src/eps/impls/krylov/krylovschur/krylovschur.c:93-348). */
+int sa_eps_create(sa_eps *eps);
+int sa_eps_destroy(sa_eps eps);
+int sa_eps_set_operators(sa_eps eps, sa_mat A);
+/* Optional second operator in FP32 used by the mixed-precision bulk phase. */
+int sa_eps_set_operators_fp32(sa_eps eps, sa_mat A32);
+int sa_eps_set_problem_type(sa_eps eps, int hermitian);
+int sa_eps_set_which(sa_eps eps, int which);
+int sa_eps_set_dimensions(sa_eps eps, int nev, int ncv, int mpd);
+int sa_eps_set_tolerances(sa_eps eps, double tol, int max_it);
+/* EPSSetInitialSpace (epssetup.c:613-626): nvec host vectors of length n,
+   stored consecutively.  As in the reference, Krylov-Schur starts from the
+   first one after orthonormalisation. */
+int sa_eps_set_initial_space(sa_eps eps, int nvec, const double *vecs_host);
+/* Precision: SA_FP64 plain, or mixed (bulk FP32 to tol32, then FP64 polish
+   seeded with the FP32 Ritz vectors). */
+int sa_eps_set_mixed(sa_eps eps, int mixed, double tol32);
+int sa_eps_solve(sa_eps eps);
+int sa_eps_get_converged(sa_eps eps, int *nconv);
+int sa_eps_get_iteration_number(sa_eps eps, int *its);
+int sa_eps_get_converged_reason(sa_eps eps, int *reason);
+/* Eigenpair i in the order EPSGetEigenpair returns them; vectors may be NULL. */
+int sa_eps_get_eigenpair(sa_eps eps, int i, double *re, double *im, double *xr_host, double *xi_host);
+int sa_eps_get_error_estimate(sa_eps eps, int i, double *errest);
+/* EPSComputeError (epssolve.c:742-815) in FP64 on the device. */
+int sa_eps_compute_error(sa_eps eps, int i, int type, double *error);
+/* Counters of the last solve: matvecs, Gram-Schmidt passes, restarts of each
+   phase, seconds of each phase, algorithmic HBM bytes moved by the SpMV/BV
+   kernels (per the byte model of the kernels). */
+typedef struct {
+  int64_t matvecs, gs_passes, matvecs32, gs_passes32;
+  int     its, its32, nconv, reason;
+  double  seconds, seconds32, seconds_ds;
+  double  bytes_spmv, bytes_bv, bytes_restart;
+} sa_eps_stats;
+int sa_eps_get_stats(sa_eps eps, sa_eps_stats *st);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

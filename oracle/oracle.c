@@ -1,0 +1,960 @@
+/*
+ * oracle.c -- CPU FP64 restatement of the SLEPc 3.22.2 Krylov-Schur hot path.
+ *
+ * TEST INFRASTRUCTURE ONLY: the checker for the HIP product in slepc_amd/.
+ * Only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg load it.
+ *
+ * Structure follows the reference call stack (paths relative to the SLEPc tree):
+ *   EPSSolve_KrylovSchur_Default   src/eps/impls/krylov/krylovschur/krylovschur.c:227-348
+ *   BVMatLanczos / BVMatArnoldi    src/sys/classes/bv/interface/bvkrylov.c:56-113,165-226
+ *   BVOrthonormalizeColumn         src/sys/classes/bv/interface/bvorthog.c:91-217,380-427
+ *   DS HEP                         src/sys/classes/ds/impls/hep/dshep.c:137-426,643-670
+ *   DS NHEP                        src/sys/classes/ds/impls/dsutil.c:21-166, nhep/dsnhep.c:101-425
+ *   EPSKrylovConvergence           src/eps/impls/krylov/epskrylov.c:207-295
+ *   EPSConvergedRelative           src/eps/interface/epsdefault.c:224-232
+ *   EPSStoppingBasic               src/eps/interface/epsdefault.c:290-302
+ *   SlepcSortEigenvalues           src/sys/slepcsc.c:89-140
+ * The dense projected problems call host LAPACK exactly where the reference does
+ * (steqr, gehrd, orghr, hseqr, trexc, trevc, lartg, rot); the LAPACK library is
+ * resolved at run time (scipy's bundled OpenBLAS or MKL).
+ */
+#define _GNU_SOURCE
+#include "oracle.h"
+#include <dlfcn.h>
+#include <glob.h>
+#include <math.h>
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+#include <time.h>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+
+/* ======================================================================== */
+/* Random numbers: counter-based splitmix64 so CPU and GPU draw equal bits.   */
+/* ======================================================================== */
+uint64_t orc_splitmix64(uint64_t z)
+{
+  z += 0x9E3779B97F4A7C15ULL;
+  z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ULL;
+  z = (z ^ (z >> 27)) * 0x94D049BB133111EBULL;
+  return z ^ (z >> 31);
+}
+double orc_u01(uint64_t seed, uint64_t i)
+{
+  uint64_t z = orc_splitmix64(orc_splitmix64(seed) + i);
+  return (double)(z >> 11) * 0x1.0p-53;
+}
+void orc_rand_uniform(uint64_t seed, int64_t n, double *out)
+{
+  for (int64_t i = 0; i < n; i++) out[i] = 2.0 * orc_u01(seed, (uint64_t)i) - 1.0;
+}
+double orc_gauss(uint64_t seed, uint64_t i)
+{
+  double u1 = 1.0 - orc_u01(seed, 2 * i), u2 = orc_u01(seed, 2 * i + 1);
+  return sqrt(-2.0 * log(u1)) * cos(6.283185307179586 * u2);
+}
+uint64_t orc_sub_seed(uint64_t seed, uint64_t p)
+{
+  return orc_splitmix64(seed ^ (0xD1B54A32D192ED03ULL * (p + 1)));
+}
+
+/* ======================================================================== */
+/* CSR generators.  Columns of every row ascending, as PETSc AIJ stores them  */
+/* after MatAssemblyEnd (INSERT_VALUES, no duplicates).                       */
+/* ======================================================================== */
+
+/* 2-D 5-point Laplacian, row II = i*nx + j: src/eps/tutorials/ex2.c:46-53 */
+int64_t orc_gen_lap2d(int nx, int ny, int32_t *rp, int32_t *ci, double *v)
+{
+  return orc_gen_lap2d_pot(nx, ny, 0, 0.0, rp, ci, v);
+}
+
+/* Same stencil plus diagonal potential scale*U[0,1) (batched family). */
+int64_t orc_gen_lap2d_pot(int nx, int ny, uint64_t seed, double scale, int32_t *rp, int32_t *ci, double *v)
+{
+  int64_t N = (int64_t)nx * ny, nz = 0;
+  for (int64_t II = 0; II < N; II++) {
+    int64_t i = II / nx, j = II - i * nx;
+    if (rp) rp[II] = (int32_t)nz;
+#define PUT(c, x) do { if (ci) { ci[nz] = (int32_t)(c); v[nz] = (x); } nz++; } while (0)
+    if (i > 0) PUT(II - nx, -1.0);
+    if (j > 0) PUT(II - 1, -1.0);
+    PUT(II, scale != 0.0 ? 4.0 + scale * orc_u01(seed, (uint64_t)II) : 4.0);
+    if (j < nx - 1) PUT(II + 1, -1.0);
+    if (i < ny - 1) PUT(II + nx, -1.0);
+  }
+  if (rp) rp[N] = (int32_t)nz;
+  return nz;
+}
+
+/* 3-D 7-point convection-diffusion: diag 6, neighbour at -e_d: -1-0.1*b_d,
+   at +e_d: -1+0.1*b_d, b = (1, 0.5, 0.25); row r = i + nx*(j + ny*k). */
+int64_t orc_gen_convdiff3d(int nx, int ny, int nz_, int32_t *rp, int32_t *ci, double *v)
+{
+  const double b[3] = {1.0, 0.5, 0.25};
+  double lo[3], hi[3];
+  for (int d = 0; d < 3; d++) { lo[d] = -1.0 - 0.1 * b[d]; hi[d] = -1.0 + 0.1 * b[d]; }
+  int64_t sxy = (int64_t)nx * ny, N = sxy * nz_, nz = 0;
+  for (int64_t r = 0; r < N; r++) {
+    int64_t k = r / sxy, rem = r - k * sxy, j = rem / nx, i = rem - j * nx;
+    if (rp) rp[r] = (int32_t)nz;
+    if (k > 0) PUT(r - sxy, lo[2]);
+    if (j > 0) PUT(r - nx, lo[1]);
+    if (i > 0) PUT(r - 1, lo[0]);
+    PUT(r, 6.0);
+    if (i < nx - 1) PUT(r + 1, hi[0]);
+    if (j < ny - 1) PUT(r + nx, hi[1]);
+    if (k < nz_ - 1) PUT(r + sxy, hi[2]);
+  }
+  if (rp) rp[N] = (int32_t)nz;
+  return nz;
+}
+
+/* 3-D 27-point stencil: off-diagonal -1, diagonal 26 + U[0,1)(seed, r). */
+int64_t orc_gen_stencil27(int nx, int ny, int nz_, uint64_t seed, int32_t *rp, int32_t *ci, double *v)
+{
+  int64_t sxy = (int64_t)nx * ny, N = sxy * nz_, nz = 0;
+  for (int64_t r = 0; r < N; r++) {
+    int64_t k = r / sxy, rem = r - k * sxy, j = rem / nx, i = rem - j * nx;
+    if (rp) rp[r] = (int32_t)nz;
+    for (int dk = -1; dk <= 1; dk++) {
+      if (k + dk < 0 || k + dk >= nz_) continue;
+      for (int dj = -1; dj <= 1; dj++) {
+        if (j + dj < 0 || j + dj >= ny) continue;
+        for (int di = -1; di <= 1; di++) {
+          if (i + di < 0 || i + di >= nx) continue;
+          int64_t c = r + di + (int64_t)nx * dj + sxy * dk;
+          if (c == r) PUT(c, 26.0 + orc_u01(seed, (uint64_t)r));
+          else PUT(c, -1.0);
+        }
+      }
+    }
+  }
+  if (rp) rp[N] = (int32_t)nz;
+  return nz;
+}
+
+/* Parameterised finite-volume diffusion -div(k grad u) on the unit square,
+   nx x ny cells, homogeneous Dirichlet boundary, harmonic-mean face
+   conductivities.  log k = sum_m theta_m sqrt(lambda_m) phi_p(x) phi_q(y) over
+   the 8 leading separable cosine modes (p,q) of a Gaussian covariance with
+   correlation length 0.2.  Returns sigma I - L with sigma the Gershgorin
+   bound max_r sum_c |L_rc|, so the smallest eigenvalues of L become the
+   largest of the returned matrix. */
+static const int FV_P[8] = {1, 0, 1, 2, 0, 2, 1, 2};
+static const int FV_Q[8] = {0, 1, 1, 0, 2, 1, 2, 2};
+static double fv_lambda1(int p)
+{
+  const double ell = 0.2, PI = 3.141592653589793;
+  double a = p * PI * ell;
+  return sqrt(2.0 * PI) * ell * exp(-0.5 * a * a);
+}
+static double fv_phi(int p, double x)
+{
+  const double PI = 3.141592653589793;
+  return p == 0 ? 1.0 : sqrt(2.0) * cos(p * PI * x);
+}
+void orc_fv2d_theta(uint64_t seed, int sample, double *theta8)
+{
+  for (int m = 0; m < 8; m++) theta8[m] = orc_gauss(seed, (uint64_t)sample * 8 + m);
+}
+int64_t orc_gen_fv2d(int nx, int ny, const double *theta, double *sigma_out, int32_t *rp, int32_t *ci, double *v)
+{
+  int64_t N = (int64_t)nx * ny, nz = 0;
+  double *kc = (double *)malloc(sizeof(double) * N);
+  double sq[8];
+  for (int m = 0; m < 8; m++) sq[m] = theta[m] * sqrt(fv_lambda1(FV_P[m]) * fv_lambda1(FV_Q[m]));
+  for (int64_t r = 0; r < N; r++) {
+    int64_t j = r / nx, i = r - j * nx;
+    double x = (i + 0.5) / nx, y = (j + 0.5) / ny, lk = 0.0;
+    for (int m = 0; m < 8; m++) lk += sq[m] * fv_phi(FV_P[m], x) * fv_phi(FV_Q[m], y);
+    kc[r] = exp(lk);
+  }
+  /* pass 1: Gershgorin bound of L (diag + |off| = 2*diag - boundary share) */
+  double sigma = 0.0;
+  for (int64_t r = 0; r < N; r++) {
+    int64_t j = r / nx, i = r - j * nx;
+    double k0 = kc[r], t[4], diag = 0.0, off = 0.0;
+    int has[4] = {j > 0, i > 0, i < nx - 1, j < ny - 1};
+    int64_t nb[4] = {r - nx, r - 1, r + 1, r + nx};
+    for (int f = 0; f < 4; f++) {
+      if (has[f]) { double k1 = kc[nb[f]]; t[f] = (2.0 * k0 * k1) / (k0 + k1); off += t[f]; }
+      else t[f] = 2.0 * k0;
+      diag += t[f];
+    }
+    double s = diag + off;
+    if (s > sigma) sigma = s;
+  }
+  if (sigma_out) *sigma_out = sigma;
+  for (int64_t r = 0; r < N; r++) {
+    int64_t j = r / nx, i = r - j * nx;
+    double k0 = kc[r], t[4], diag = 0.0;
+    int has[4] = {j > 0, i > 0, i < nx - 1, j < ny - 1};
+    int64_t nb[4] = {r - nx, r - 1, r + 1, r + nx};
+    for (int f = 0; f < 4; f++) {
+      if (has[f]) { double k1 = kc[nb[f]]; t[f] = (2.0 * k0 * k1) / (k0 + k1); }
+      else t[f] = 2.0 * k0;
+      diag += t[f];
+    }
+    if (rp) rp[r] = (int32_t)nz;
+    if (has[0]) PUT(nb[0], t[0]);
+    if (has[1]) PUT(nb[1], t[1]);
+    PUT(r, sigma - diag);
+    if (has[2]) PUT(nb[2], t[2]);
+    if (has[3]) PUT(nb[3], t[3]);
+  }
+  if (rp) rp[N] = (int32_t)nz;
+  free(kc);
+  return nz;
+}
+
+/* Markov walk on a triangular grid, src/eps/tutorials/ex5.c:146-178
+   (row ix-1 holds the transition probabilities into state ix-1). */
+int64_t orc_gen_markov(int m, int32_t *rp, int32_t *ci, double *v)
+{
+  const double cst = 0.5 / (double)(m - 1);
+  int64_t N = (int64_t)m * (m + 1) / 2, nz = 0, ix = 0;
+  for (int i = 1; i <= m; i++) {
+    int jmax = m - i + 1;
+    for (int j = 1; j <= jmax; j++) {
+      ix++;
+      int64_t row = ix - 1;
+      int64_t c[4]; double x[4]; int cnt = 0;
+      double pu = 0.5 - cst * (double)(i + j - 3);
+      if (i > 1) { c[cnt] = ix - jmax - 2; x[cnt++] = pu; }          /* west  */
+      if (j > 1) { c[cnt] = ix - 2; x[cnt++] = pu; }                 /* south */
+      if (j != jmax) {
+        double pd = cst * (double)(i + j - 1);
+        c[cnt] = ix; x[cnt++] = (i == 1) ? 2 * pd : pd;               /* north */
+        c[cnt] = ix + jmax - 1; x[cnt++] = (j == 1) ? 2 * pd : pd;    /* east  */
+      }
+      /* sort ascending (at most 4 entries) */
+      for (int a = 1; a < cnt; a++)
+        for (int b2 = a; b2 > 0 && c[b2 - 1] > c[b2]; b2--) {
+          int64_t tc = c[b2]; c[b2] = c[b2 - 1]; c[b2 - 1] = tc;
+          double tx = x[b2]; x[b2] = x[b2 - 1]; x[b2 - 1] = tx;
+        }
+      if (rp) rp[row] = (int32_t)nz;
+      for (int a = 0; a < cnt; a++) PUT(c[a], x[a]);
+    }
+  }
+  if (rp) rp[N] = (int32_t)nz;
+  return nz;
+}
+#undef PUT
+
+/* ======================================================================== */
+/* LAPACK / BLAS resolved at run time                                        */
+/* ======================================================================== */
+typedef void (*f_lartg)(const double *, const double *, double *, double *, double *);
+typedef void (*f_rot)(const int *, double *, const int *, double *, const int *, const double *, const double *);
+typedef void (*f_steqr)(const char *, const int *, double *, double *, double *, const int *, double *, int *, size_t);
+typedef void (*f_gehrd)(const int *, const int *, const int *, double *, const int *, double *, double *, const int *, int *);
+typedef void (*f_orghr)(const int *, const int *, const int *, double *, const int *, const double *, double *, const int *, int *);
+typedef void (*f_hseqr)(const char *, const char *, const int *, const int *, const int *, double *, const int *, double *, double *,
+                        double *, const int *, double *, const int *, int *, size_t, size_t);
+typedef void (*f_trexc)(const char *, const int *, double *, const int *, double *, const int *, int *, int *, double *, int *, size_t);
+typedef void (*f_trevc)(const char *, const char *, int *, const int *, const double *, const int *, double *, const int *, double *,
+                        const int *, const int *, int *, double *, int *, size_t, size_t);
+static struct {
+  int tried, ok;
+  char path[512];
+  f_lartg lartg; f_rot rot; f_steqr steqr; f_gehrd gehrd; f_orghr orghr; f_hseqr hseqr; f_trexc trexc; f_trevc trevc;
+} LP;
+
+static void *lp_sym(void *h, const char *name)
+{
+  char buf[64];
+  snprintf(buf, sizeof buf, "%s_", name);
+  void *p = dlsym(h, buf);
+  if (!p) { snprintf(buf, sizeof buf, "scipy_%s_", name); p = dlsym(h, buf); }
+  return p;
+}
+static int lp_try(const char *path)
+{
+  void *h = dlopen(path, RTLD_NOW | RTLD_LOCAL);
+  if (!h) return 0;
+  LP.lartg = (f_lartg)lp_sym(h, "dlartg"); LP.rot = (f_rot)lp_sym(h, "drot");
+  LP.steqr = (f_steqr)lp_sym(h, "dsteqr"); LP.gehrd = (f_gehrd)lp_sym(h, "dgehrd");
+  LP.orghr = (f_orghr)lp_sym(h, "dorghr"); LP.hseqr = (f_hseqr)lp_sym(h, "dhseqr");
+  LP.trexc = (f_trexc)lp_sym(h, "dtrexc"); LP.trevc = (f_trevc)lp_sym(h, "dtrevc");
+  if (!(LP.lartg && LP.rot && LP.steqr && LP.gehrd && LP.orghr && LP.hseqr && LP.trexc && LP.trevc)) { dlclose(h); return 0; }
+  void (*setlayer)(int) = (void (*)(int))dlsym(h, "MKL_Set_Threading_Layer");
+  if (setlayer) setlayer(1); /* MKL_THREADING_SEQUENTIAL */
+  snprintf(LP.path, sizeof LP.path, "%s", path);
+  return 1;
+}
+static int lp_load(void)
+{
+  if (LP.tried) return LP.ok;
+  LP.tried = 1;
+  const char *env = getenv("SLEPC_AMD_LAPACK");
+  if (env && lp_try(env)) return LP.ok = 1;
+  glob_t g;
+  if (glob("/usr/local/lib/python3*/dist-packages/scipy.libs/libscipy_openblas*.so", 0, NULL, &g) == 0) {
+    for (size_t i = 0; i < g.gl_pathc; i++)
+      if (lp_try(g.gl_pathv[i])) { globfree(&g); return LP.ok = 1; }
+    globfree(&g);
+  }
+  const char *cands[] = {"/opt/conda/lib/libmkl_rt.so.1", "/opt/conda/lib/libmkl_rt.so", "libopenblas.so.0", "liblapack.so.3", NULL};
+  for (int i = 0; cands[i]; i++) if (lp_try(cands[i])) return LP.ok = 1;
+  return LP.ok = 0;
+}
+int orc_lapack_status(char *buf, int len)
+{
+  int ok = lp_load();
+  if (buf && len > 0) snprintf(buf, (size_t)len, "%s", ok ? LP.path : "not found");
+  return ok;
+}
+
+/* ======================================================================== */
+/* SpMV and BV kernels (bvblas.c / bvmat.c semantics, OpenMP over rows)      */
+/* ======================================================================== */
+void orc_spmv(int n, const int32_t *rp, const int32_t *ci, const double *val, const double *x, double *y)
+{
+#pragma omp parallel for schedule(static)
+  for (int r = 0; r < n; r++) {
+    double s = 0.0;
+    for (int32_t p = rp[r]; p < rp[r + 1]; p++) s += val[p] * x[ci[p]];
+    y[r] = s;
+  }
+}
+
+/* q = V(:,0:k)^T x : BVDotVec_BLAS_Private (bvblas.c:240-261) */
+void orc_bv_dotvec(int64_t n, int k, const double *V, int64_t ld, const double *x, double *q)
+{
+  if (k <= 0) return;
+  int nt = 1;
+#ifdef _OPENMP
+  nt = omp_get_max_threads();
+#endif
+  double *part = (double *)calloc((size_t)nt * k, sizeof(double));
+#pragma omp parallel num_threads(nt)
+  {
+    int t = 0;
+#ifdef _OPENMP
+    t = omp_get_thread_num();
+#endif
+    int64_t r0 = n * t / nt, r1 = n * (t + 1) / nt;
+    double *p = part + (size_t)t * k;
+    for (int c = 0; c < k; c++) {
+      const double *vc = V + (int64_t)c * ld;
+      double s = 0.0;
+      for (int64_t r = r0; r < r1; r++) s += vc[r] * x[r];
+      p[c] = s;
+    }
+  }
+  for (int c = 0; c < k; c++) {
+    double s = 0.0;
+    for (int t = 0; t < nt; t++) s += part[(size_t)t * k + c];
+    q[c] = s;
+  }
+  free(part);
+}
+
+/* y = beta*y + alpha*V(:,0:k) q : BVMultVec_BLAS_Private (bvblas.c:56-67) */
+void orc_bv_multvec(int64_t n, int k, double alpha, const double *V, int64_t ld, const double *q, double beta, double *y)
+{
+#pragma omp parallel for schedule(static)
+  for (int64_t r = 0; r < n; r++) {
+    double s = 0.0;
+    for (int c = 0; c < k; c++) s += V[r + (int64_t)c * ld] * q[c];
+    y[r] = beta * y[r] + alpha * s;
+  }
+}
+
+/* V(:,s:e) = V(:,l:k) * Q(l:k, s:e) : BVMultInPlace_BLAS_Private (bvblas.c:74-106) */
+void orc_bv_multinplace(int64_t n, double *V, int64_t ld, int l, int k, int s, int e, const double *Q, int ldq)
+{
+  int ncol = e - s, kk = k - l;
+  if (ncol <= 0 || n == 0) return;
+  const int BS = 64;
+#pragma omp parallel
+  {
+    double *tmp = (double *)malloc(sizeof(double) * BS * ncol);
+#pragma omp for schedule(static)
+    for (int64_t r0 = 0; r0 < n; r0 += BS) {
+      int rows = (int)((n - r0) < BS ? (n - r0) : BS);
+      for (int j = 0; j < ncol; j++)
+        for (int r = 0; r < rows; r++) {
+          double acc = 0.0;
+          for (int c = 0; c < kk; c++) acc += V[r0 + r + (int64_t)(l + c) * ld] * Q[(l + c) + (int64_t)(s + j) * ldq];
+          tmp[r + j * BS] = acc;
+        }
+      for (int j = 0; j < ncol; j++)
+        for (int r = 0; r < rows; r++) V[r0 + r + (int64_t)(s + j) * ld] = tmp[r + j * BS];
+    }
+    free(tmp);
+  }
+}
+
+void orc_bv_scale(int64_t n, double *x, double alpha)
+{
+#pragma omp parallel for schedule(static)
+  for (int64_t r = 0; r < n; r++) x[r] *= alpha;
+}
+
+double orc_bv_norm2(int64_t n, const double *x)
+{
+  double s;
+  orc_bv_dotvec(n, 1, x, n, x, &s);
+  return sqrt(s);
+}
+
+/* ======================================================================== */
+/* Gram-Schmidt: BVOrthogonalizeCGS1 (bvorthog.c:91-132) and                  */
+/* BVOrthogonalizeGS REFINE_IFNEEDED (bvorthog.c:145-217).  The coefficient   */
+/* buffer is m x m column-major; column 0 is the scratch vector c and column  */
+/* j accumulates h_j with the norm stored at row j (bvimpl.h:289-415).        */
+/* ======================================================================== */
+static double safe_sqrt(double a) { return a < 0.0 ? 0.0 : sqrt(a); } /* BV_SafeSqrt, bvimpl.h:121-141 */
+
+static void cgs1(int64_t n, double *V, int64_t ld, int j, double *buf, int m, double *onorm, double *norm, int64_t *passes)
+{
+  double *c = buf;               /* scratch column 0 */
+  double *h = buf + (int64_t)j * m;
+  double *vj = V + (int64_t)j * ld;
+  (*passes)++;
+  orc_bv_dotvec(n, j + 1, V, ld, vj, c);           /* BVDotColumnInc: includes v_j^T v_j */
+  double beta = safe_sqrt(c[j]);                   /* BV_SquareRoot */
+  orc_bv_multvec(n, j, -1.0, V, ld, c, 1.0, vj);    /* BVMultColumn(-1,1) */
+  if (onorm) *onorm = beta;
+  if (norm) {
+    double sum = 0.0;
+    for (int i = 0; i < j; i++) sum += c[i] * c[i];  /* BV_SquareSum */
+    double nn = beta * beta - sum;
+    if (nn <= 0.0) *norm = orc_bv_norm2(n, vj);
+    else *norm = sqrt(nn);
+  }
+  for (int i = 0; i < j; i++) h[i] += c[i];        /* BV_AddCoefficients */
+}
+
+static void orthogonalize_gs(int64_t n, double *V, int64_t ld, int j, double *buf, int m, double eta,
+                             double *norm, int *lindep, int64_t *passes)
+{
+  double onrm = 0.0, nrm = 0.0;
+  double *h = buf + (int64_t)j * m;
+  for (int i = 0; i < j; i++) h[i] = 0.0;           /* BV_CleanCoefficients */
+  cgs1(n, V, ld, j, buf, m, &onrm, &nrm, passes);
+  int l = 1;
+  while (l < 3 && nrm != 0.0 && fabs(nrm) < eta * fabs(onrm)) {
+    l++;
+    cgs1(n, V, ld, j, buf, m, &onrm, &nrm, passes);
+  }
+  *lindep = !(nrm != 0.0 && fabs(nrm) >= eta * fabs(onrm));
+  *norm = nrm;
+  h[j] = *lindep ? 0.0 : nrm;                       /* BV_SetValue(k,k,h,nrm) */
+}
+
+int orc_bv_orthonormalize_column(int64_t n, double *V, int64_t ld, int j, double *buffer, int m, double eta,
+                                 double *norm, int *lindep)
+{
+  int64_t passes = 0;
+  double nrm;
+  int ld_ = 0;
+  orthogonalize_gs(n, V, ld, j, buffer, m, eta, &nrm, &ld_, &passes);
+  if (nrm != 1.0 && nrm != 0.0) orc_bv_scale(n, V + (int64_t)j * ld, 1.0 / nrm);
+  if (norm) *norm = nrm;
+  if (lindep) *lindep = ld_;
+  return (int)passes;
+}
+
+/* ======================================================================== */
+/* Dense projected problem (DS)                                               */
+/* ======================================================================== */
+enum { ST_RAW = 0, ST_INTERMEDIATE = 1, ST_CONDENSED = 2, ST_TRUNCATED = 3 };
+typedef struct {
+  int ld, n, l, k, t, state, hermitian, which;
+  double *A, *Q, *T, *X, *work;
+  int *perm, *iwork;
+} DSo;
+
+/* SlepcCompare* (slepcsc.c:152-200): result>0 means (ar,ai) goes after (br,bi). */
+static int sc_compare(int which, double ar, double ai, double br, double bi)
+{
+  double a, b;
+  switch (which) {
+    case ORC_LM: a = hypot(ar, ai); b = hypot(br, bi); return a < b ? 1 : (a > b ? -1 : 0);
+    case ORC_SM: a = hypot(ar, ai); b = hypot(br, bi); return a > b ? 1 : (a < b ? -1 : 0);
+    case ORC_LR: a = ar; b = br; return a < b ? 1 : (a > b ? -1 : 0);
+    case ORC_SR: a = ar; b = br; return a > b ? 1 : (a < b ? -1 : 0);
+  }
+  return 0;
+}
+
+/* DSArrowTridiag (dshep.c:221-265), Rutishauser's Givens chase */
+void orc_arrow_tridiag(int n, double *d, double *e, double *Q, int ld)
+{
+  int one = 1;
+  double c, s, p, off, temp;
+  if (n <= 2) return;
+  lp_load();
+  for (int j = 0; j < n - 2; j++) {
+    temp = e[j + 1];
+    LP.lartg(&temp, &e[j], &c, &s, &e[j + 1]);
+    s = -s;
+    temp = d[j + 1];
+    e[j] = c * s * (temp - d[j]);
+    d[j + 1] = s * s * d[j] + c * c * temp;
+    d[j] = c * c * d[j] + s * s * temp;
+    int j2 = j + 2;
+    LP.rot(&j2, Q + (int64_t)j * ld, &one, Q + (int64_t)(j + 1) * ld, &one, &c, &s);
+    for (int i = j - 1; i >= 0; i--) {
+      off = -s * e[i];
+      e[i] = c * e[i];
+      temp = e[i + 1];
+      LP.lartg(&temp, &off, &c, &s, &e[i + 1]);
+      s = -s;
+      temp = (d[i] - d[i + 1]) * s - 2.0 * c * e[i];
+      p = s * temp;
+      d[i + 1] += p;
+      d[i] -= p;
+      e[i] = -e[i] - c * temp;
+      LP.rot(&j2, Q + (int64_t)i * ld, &one, Q + (int64_t)(i + 1) * ld, &one, &c, &s);
+    }
+  }
+}
+
+/* DSSetIdentity (dspriv.c:407-426) */
+static void ds_set_identity(DSo *ds, double *M)
+{
+  memset(M + (int64_t)ds->l * ds->ld, 0, sizeof(double) * (size_t)ds->ld * (ds->n - ds->l));
+  for (int i = ds->l; i < ds->n; i++) M[i + (int64_t)i * ds->ld] = 1.0;
+}
+
+/* DSSolve_HEP_QR with DSIntermediate_HEP, compact storage (dshep.c:267-321,383-426) */
+static int ds_solve_hep(DSo *ds, double *wr, double *wi)
+{
+  int n = ds->n, l = ds->l, ld = ds->ld, n1 = n - l, info = 0;
+  double *d = ds->T, *e = ds->T + ld;
+  int na = ds->k - l + 1; if (na < 0) na = 0;
+  ds_set_identity(ds, ds->Q);
+  int off = l + l * ld;
+  if (ds->state < ST_INTERMEDIATE) orc_arrow_tridiag(na, d + l, e + l, ds->Q + off, ld);
+  for (int i = 0; i < l; i++) wr[i] = d[i];
+  if (n1 > 0) LP.steqr("V", &n1, d + l, e + l, ds->Q + off, &ld, ds->work, &info, 1);
+  for (int i = l; i < n; i++) wr[i] = d[i];
+  for (int i = 0; i < n - 1; i++) e[i] = 0.0;
+  if (wi) for (int i = l; i < n; i++) wi[i] = 0.0;
+  ds->state = ST_CONDENSED;
+  return info;
+}
+
+/* DSSortEigenvaluesReal_Private (dspriv.c:254-273) + DSPermuteColumns_Private (:278-300) */
+static void ds_sort_hep(DSo *ds, double *wr)
+{
+  int n = ds->n, l = ds->l, ld = ds->ld, *perm = ds->perm;
+  double *d = ds->T;
+  for (int i = 0; i < n; i++) perm[i] = i;
+  for (int i = l + 1; i < ds->t; i++) {
+    double re = d[perm[i]];
+    int j = i - 1;
+    int res = sc_compare(ds->which, re, 0.0, d[perm[j]], 0.0);
+    while (res < 0 && j >= l) {
+      int tmp = perm[j]; perm[j] = perm[j + 1]; perm[j + 1] = tmp; j--;
+      if (j >= l) res = sc_compare(ds->which, re, 0.0, d[perm[j]], 0.0);
+    }
+  }
+  for (int i = l; i < n; i++) wr[i] = d[perm[i]];
+  for (int i = l; i < n; i++) {
+    int p = perm[i];
+    if (p != i) {
+      int j = i + 1;
+      while (perm[j] != i) j++;
+      perm[j] = p; perm[i] = i;
+      for (int k = 0; k < n; k++) { double t = ds->Q[k + p * ld]; ds->Q[k + p * ld] = ds->Q[k + i * ld]; ds->Q[k + i * ld] = t; }
+    }
+  }
+  for (int i = l; i < n; i++) d[i] = wr[i];
+}
+
+/* DSUpdateExtraRow_HEP compact (dshep.c:349-381) */
+static void ds_update_extra_row_hep(DSo *ds)
+{
+  int n = ds->n, ld = ds->ld;
+  double *e = ds->T + ld, beta = e[n - 1];
+  for (int i = 0; i < n; i++) e[i] = beta * ds->Q[n - 1 + i * ld];
+  ds->k = n;
+}
+
+/* DSSolve_NHEP_Private (dsutil.c:21-90) */
+static int ds_solve_nhep(DSo *ds, double *wr, double *wi)
+{
+  int n = ds->n, ld = ds->ld, ilo = ds->l + 1, info = 0, lwork = 6 * ld;
+  double *A = ds->A, *Q = ds->Q, *tau = ds->work, *work = ds->work + ld;
+  memset(Q, 0, sizeof(double) * (size_t)ld * ld);
+  for (int i = 0; i < n; i++) Q[i + i * ld] = 1.0;
+  if (n == 1) { wr[0] = A[0]; wi[0] = 0.0; ds->state = ST_CONDENSED; return 0; }
+  if (ds->state < ST_INTERMEDIATE) {
+    LP.gehrd(&n, &ilo, &n, A, &ld, tau, work, &lwork, &info);
+    if (info) return info;
+    for (int j = 0; j < n - 1; j++)
+      for (int i = j + 2; i < n; i++) { Q[i + j * ld] = A[i + j * ld]; A[i + j * ld] = 0.0; }
+    LP.orghr(&n, &ilo, &n, Q, &ld, tau, work, &lwork, &info);
+    if (info) return info;
+  }
+  LP.hseqr("S", "V", &n, &ilo, &n, A, &ld, wr, wi, Q, &ld, work, &lwork, &info, 1, 1);
+  for (int j = 0; j < ds->l; j++) {
+    if (j == n - 1 || A[j + 1 + j * ld] == 0.0) { wr[j] = A[j + j * ld]; wi[j] = 0.0; }
+    else {
+      wr[j] = A[j + j * ld]; wr[j + 1] = A[j + j * ld];
+      wi[j] = sqrt(fabs(A[j + 1 + j * ld])) * sqrt(fabs(A[j + (j + 1) * ld]));
+      wi[j + 1] = -wi[j];
+      j++;
+    }
+  }
+  ds->state = ST_CONDENSED;
+  return info;
+}
+
+/* DSSort_NHEP_Total (dsutil.c:93-166): selection sort with trexc */
+static int ds_sort_nhep(DSo *ds, double *wr, double *wi)
+{
+  int n = ds->n, ld = ds->ld, info = 0;
+  double *T = ds->A;
+  for (int i = ds->l; i < n - 1; i++) {
+    double re = wr[i], im = wi[i];
+    int pos = 0, j = i + 1;
+    if (im != 0) j = i + 2;
+    for (; j < n; j++) {
+      if (sc_compare(ds->which, re, im, wr[j], wi[j]) > 0) { re = wr[j]; im = wi[j]; pos = j; }
+      if (wi[j] != 0) j++;
+    }
+    if (pos) {
+      int ifst = pos + 1, ilst = i + 1;
+      LP.trexc("V", &n, T, &ld, ds->Q, &ld, &ifst, &ilst, ds->work, &info, 1);
+      if (info) return info;
+      for (j = i; j < n; j++) {
+        wr[j] = T[j + j * ld];
+        if (j < n - 1 && T[j + 1 + j * ld] != 0.0) {
+          wi[j] = sqrt(fabs(T[j + 1 + j * ld])) * sqrt(fabs(T[j + (j + 1) * ld]));
+          wr[j + 1] = wr[j]; wi[j + 1] = -wi[j]; j++;
+        } else wi[j] = 0.0;
+      }
+    }
+    if (wi[i] != 0) i++;
+  }
+  return 0;
+}
+
+/* DSUpdateExtraRow_NHEP (dsnhep.c:318-340): b := Q^T b */
+static void ds_update_extra_row_nhep(DSo *ds)
+{
+  int n = ds->n, ld = ds->ld;
+  double *x = ds->work, *y = ds->work + ld;
+  for (int i = 0; i < n; i++) x[i] = ds->A[n + i * ld];
+  for (int i = 0; i < n; i++) {
+    double s = 0.0;
+    for (int r = 0; r < n; r++) s += ds->Q[r + i * ld] * x[r];
+    y[i] = s;
+  }
+  for (int i = 0; i < n; i++) ds->A[n + i * ld] = y[i];
+  ds->k = n;
+}
+
+/* DSVectors_NHEP_Eigen_Some (dsnhep.c:101-166): returns |last comp| and flag for complex pair */
+static int ds_vector_nhep_some(DSo *ds, int k, double *rnorm, int *iscomplex)
+{
+  int n = ds->n, ld = ds->ld, mm, mout, info = 0;
+  int *select = ds->iwork;
+  for (int i = 0; i < n; i++) select[i] = 0;
+  double *Y = ds->X + (int64_t)k * ld;
+  select[k] = 1;
+  *iscomplex = (k < n - 1 && ds->A[k + 1 + k * ld] != 0.0);
+  mm = *iscomplex ? 2 : 1;
+  if (*iscomplex) select[k + 1] = 1;
+  LP.trevc("R", "S", select, &n, ds->A, &ld, Y, &ld, Y, &ld, &mm, &mout, ds->work, &info, 1, 1);
+  if (info) return info;
+  double *tmp = ds->work;
+  memcpy(tmp, Y, sizeof(double) * (size_t)mout * ld);
+  for (int c = 0; c < mout; c++)
+    for (int r = 0; r < n; r++) {
+      double s = 0.0;
+      for (int q = 0; q < n; q++) s += ds->Q[r + q * ld] * tmp[q + c * ld];
+      Y[r + c * ld] = s;
+    }
+  double nr = 0.0, ni = 0.0;
+  for (int r = 0; r < n; r++) nr += Y[r] * Y[r];
+  nr = sqrt(nr);
+  double norm = nr;
+  if (*iscomplex) { for (int r = 0; r < n; r++) ni += Y[r + ld] * Y[r + ld]; norm = hypot(nr, sqrt(ni)); }
+  for (int c = 0; c < mout; c++) for (int r = 0; r < n; r++) Y[r + c * ld] /= norm;
+  *rnorm = *iscomplex ? hypot(Y[n - 1], Y[n - 1 + ld]) : fabs(Y[n - 1]);
+  return 0;
+}
+
+/* DSVectors_NHEP_Eigen_All (dsnhep.c:168-226): X = eigvecs(T), back-transformed
+   with Q only while the DS is condensed (a trimmed DS is RAW: dsops.c:250) */
+static int ds_vectors_nhep_all(DSo *ds, int back_q)
+{
+  int n = ds->n, ld = ds->ld, mout, info = 0;
+  if (back_q) memcpy(ds->X, ds->Q, sizeof(double) * (size_t)ld * ld);
+  LP.trevc("R", back_q ? "B" : "A", NULL, &n, ds->A, &ld, NULL, &ld, ds->X, &ld, &n, &mout, ds->work, &info, 1, 1);
+  if (info) return info;
+  for (int i = 0; i < n; i++) {
+    int cplx = (i < n - 1 && ds->A[i + 1 + i * ld] != 0.0);
+    double s = 0.0, s2 = 0.0;
+    for (int r = 0; r < n; r++) s += ds->X[r + i * ld] * ds->X[r + i * ld];
+    double norm = sqrt(s);
+    if (cplx) { for (int r = 0; r < n; r++) s2 += ds->X[r + (i + 1) * ld] * ds->X[r + (i + 1) * ld]; norm = hypot(norm, sqrt(s2)); }
+    for (int c = 0; c < (cplx ? 2 : 1); c++) for (int r = 0; r < n; r++) ds->X[r + (i + c) * ld] /= norm;
+    if (cplx) i++;
+  }
+  return 0;
+}
+
+/* ======================================================================== */
+/* Krylov-Schur driver                                                        */
+/* ======================================================================== */
+static double now_s(void)
+{
+  struct timespec ts;
+  clock_gettime(CLOCK_MONOTONIC, &ts);
+  return ts.tv_sec + 1e-9 * ts.tv_nsec;
+}
+
+typedef struct {
+  int n;
+  const int32_t *rp, *ci;
+  const double *val;
+  double *V; int64_t ld; int m;  /* BV: n x m, m = ncv+1 */
+  double *buf;                   /* m x m coefficient buffer */
+  double eta;
+  int64_t matvecs, passes;
+} KSo;
+
+/* EPSGetStartVector (epssolve.c:841-873) with a caller-provided vector for i==0
+   or a deterministic random vector (seed 0x5EED + i) otherwise. */
+static int ks_start_vector(KSo *ks, int i, int nini, int *breakdown)
+{
+  double *vi = ks->V + (int64_t)i * ks->ld;
+  if (i > 0 || nini == 0) orc_rand_uniform(0x5EEDULL + (uint64_t)i, ks->n, vi);
+  double norm; int lindep;
+  orthogonalize_gs(ks->n, ks->V, ks->ld, i, ks->buf, ks->m, ks->eta, &norm, &lindep, &ks->passes);
+  if (breakdown) *breakdown = lindep;
+  else if (lindep || norm == 0.0) return -1;
+  orc_bv_scale(ks->n, vi, 1.0 / norm);
+  return 0;
+}
+
+/* BVMatArnoldi / BVMatLanczos (bvkrylov.c:56-113,165-226) */
+static void ks_expand(KSo *ks, int k, int *m, double *beta, int *breakdown)
+{
+  int lindep = 0;
+  for (int j = k; j < *m; j++) {
+    orc_spmv(ks->n, ks->rp, ks->ci, ks->val, ks->V + (int64_t)j * ks->ld, ks->V + (int64_t)(j + 1) * ks->ld);
+    ks->matvecs++;
+    double nrm;
+    orthogonalize_gs(ks->n, ks->V, ks->ld, j + 1, ks->buf, ks->m, ks->eta, &nrm, &lindep, &ks->passes);
+    if (nrm != 1.0 && nrm != 0.0) orc_bv_scale(ks->n, ks->V + (int64_t)(j + 1) * ks->ld, 1.0 / nrm);
+    *beta = nrm;
+    if (lindep) { *m = j + 1; break; }
+  }
+  *breakdown = lindep;
+}
+
+int orc_eps_solve(int n, const int32_t *rp, const int32_t *ci, const double *val, const orc_opts *o, const double *v0, orc_result *res)
+{
+  if (!lp_load()) return -10;
+#ifdef _OPENMP
+  int saved_threads = omp_get_max_threads();
+  if (o->nthreads > 0) omp_set_num_threads(o->nthreads);
+#endif
+  int nev = o->nev > 0 ? o->nev : 1;
+  int ncv = o->ncv > 0 ? o->ncv : (nev + 15 > 2 * nev ? nev + 15 : 2 * nev);
+  if (ncv > n) ncv = n;
+  int mpd = o->mpd > 0 ? o->mpd : ncv;
+  int max_it = o->max_it > 0 ? o->max_it : (100 > 2 * n / ncv ? 100 : 2 * n / ncv);
+  double tol = o->tol > 0 ? o->tol : 1e-8, keep = o->keep > 0 ? o->keep : 0.5;
+  int herm = o->hermitian, which = o->which ? o->which : ORC_LM, lock = o->lock;
+  KSo ks = {0};
+  ks.n = n; ks.rp = rp; ks.ci = ci; ks.val = val; ks.m = ncv + 1; ks.ld = n;
+  ks.eta = o->eta > 0 ? o->eta : 0.7071;
+  ks.V = (double *)calloc((size_t)n * ks.m, sizeof(double));
+  ks.buf = (double *)calloc((size_t)ks.m * ks.m, sizeof(double));
+  int ld = ncv + 1;
+  DSo ds = {0};
+  ds.ld = ld; ds.hermitian = herm; ds.which = which;
+  ds.A = (double *)calloc((size_t)ld * ld, sizeof(double));
+  ds.Q = (double *)calloc((size_t)ld * ld, sizeof(double));
+  ds.X = (double *)calloc((size_t)ld * ld, sizeof(double));
+  ds.T = (double *)calloc((size_t)3 * ld, sizeof(double));
+  ds.work = (double *)calloc((size_t)8 * ld * ld + 16 * ld, sizeof(double));
+  ds.perm = (int *)calloc(ld, sizeof(int));
+  ds.iwork = (int *)calloc(ld, sizeof(int));
+  double *eigr = (double *)calloc(ld, sizeof(double)), *eigi = (double *)calloc(ld, sizeof(double));
+  double *errest = (double *)calloc(ld, sizeof(double));
+  int *perm = (int *)calloc(ld, sizeof(int));
+  for (int i = 0; i < ncv; i++) perm[i] = i;
+  int rc = 0, nini = 0;
+  double t0 = now_s();
+
+  /* EPSSetInitialSpace -> BVInsertVecs(V,0,&k,IS,TRUE) (epssetup.c:421-427, bvfunc.c:331-374) */
+  if (v0) {
+    memcpy(ks.V, v0, sizeof(double) * (size_t)n);
+    double norm; int lindep;
+    orthogonalize_gs(n, ks.V, ks.ld, 0, ks.buf, ks.m, ks.eta, &norm, &lindep, &ks.passes);
+    if (norm == 0.0 || lindep) { rc = -3; goto done; }
+    orc_bv_scale(n, ks.V, 1.0 / norm);
+    nini = 1;
+  }
+  if (ks_start_vector(&ks, 0, nini, NULL)) { rc = -3; goto done; }
+
+  int l = 0, nconv = 0, its = 0, reason = 0;
+  double *d = ds.T, *e = ds.T + ld;
+  while (reason == 0) {
+    its++;
+    int nv = nconv + mpd < ncv ? nconv + mpd : ncv;
+    ds.n = nv; ds.t = nv; ds.l = nconv; ds.k = nconv + l;          /* DSSetDimensions */
+    double beta = 0.0; int breakdown = 0, k0 = nconv + l;
+    ks_expand(&ks, k0, &nv, &beta, &breakdown);
+    int bm = ks.m;
+    if (herm) {
+      for (int j = k0; j < nv; j++) { d[j] = ks.buf[j + (int64_t)(j + 1) * bm]; e[j] = ks.buf[j + 1 + (int64_t)(j + 1) * bm]; }
+    } else {
+      double *H = ds.A;
+      for (int j = k0; j < nv - 1; j++) memcpy(H + (int64_t)j * ld, ks.buf + (int64_t)(j + 1) * bm, sizeof(double) * (j + 2));
+      memcpy(H + (int64_t)(nv - 1) * ld, ks.buf + (int64_t)nv * bm, sizeof(double) * nv);
+      if (ld > nv) H[nv + (nv - 1) * ld] = ks.buf[nv + (int64_t)nv * bm];
+    }
+    ds.n = nv; ds.t = nv; ds.l = nconv; ds.k = nconv + l;
+    ds.state = l ? ST_RAW : ST_INTERMEDIATE;
+    int info = herm ? ds_solve_hep(&ds, eigr, eigi) : ds_solve_nhep(&ds, eigr, eigi);
+    if (info) { rc = -20 - info; goto done; }
+    if (herm) { ds_sort_hep(&ds, eigr); ds_update_extra_row_hep(&ds); }
+    else {
+      if ((info = ds_sort_nhep(&ds, eigr, eigi))) { rc = -40; goto done; }
+      ds_update_extra_row_nhep(&ds);
+    }
+    /* EPSKrylovConvergence (epskrylov.c:207-295), relative criterion */
+    int k, marker = -1;
+    for (k = nconv; k < nv; k++) {
+      double re = eigr[k], im = eigi[k], resnorm;
+      int cplx = 0;
+      if (herm) resnorm = fabs(ds.Q[nv - 1 + k * ld]);
+      else if ((info = ds_vector_nhep_some(&ds, k, &resnorm, &cplx))) { rc = -50; goto done; }
+      resnorm *= beta;
+      double w = hypot(re, im);
+      errest[k] = (w != 0.0) ? resnorm / w : 1.79e308;
+      if (marker == -1 && errest[k] >= tol) marker = k;
+      if (cplx) { errest[k + 1] = errest[k]; k++; }
+      if (marker != -1) break;
+    }
+    if (marker != -1) k = marker;
+    /* EPSStoppingBasic (epsdefault.c:290-302) */
+    if (k >= nev) reason = 1;
+    else if (its >= max_it) reason = -1;
+    int newl;
+    if (reason != 0 || breakdown || k == nv) newl = 0;
+    else {
+      newl = (int)((nv - k) * keep); if (newl < 1) newl = 1;
+      if (!herm) {                                           /* DSGetTruncateSize_Default (dsops.c:329-344) */
+        if (ds.A[(k + newl) + (k + newl - 1) * ld] != 0.0) { if (k + newl < nv - 1) newl++; else newl--; }
+      }
+    }
+    if (!lock && newl > 0) { newl += k; k = 0; }
+    l = newl;
+    if (reason == 0) {
+      if (breakdown || k == nv) {
+        if (k < nev) {
+          int bd = 0;
+          ks_start_vector(&ks, k, nini, &bd);
+          if (bd) reason = -2;
+        }
+      } else {                                                /* DSTruncate(k+l,FALSE) */
+        int nn = k + l;
+        if (!herm && ds.k == ds.n) {
+          for (int i = ds.l; i < nn; i++) ds.A[nn + i * ld] = ds.A[ds.n + i * ld];
+          for (int i = ds.l; i < ds.n; i++) ds.A[ds.n + i * ld] = 0.0;
+        }
+        ds.k = nn; ds.t = ds.n; ds.n = nn; ds.state = ST_TRUNCATED;
+      }
+    }
+    /* BVMultInPlace(V,Q,nconv,k+l) with active columns [nconv,nv) */
+    orc_bv_multinplace(n, ks.V, ks.ld, nconv, nv, nconv, k + l, ds.Q, ld);
+    if (reason == 0 && !breakdown) memcpy(ks.V + (int64_t)(k + l) * ks.ld, ks.V + (int64_t)nv * ks.ld, sizeof(double) * (size_t)n);
+    nconv = k;
+  }
+  /* DSTruncate(nconv,TRUE); EPSComputeVectors; SlepcSortEigenvalues */
+  if (!herm) for (int i = ds.l; i < ds.n; i++) ds.A[ds.n + i * ld] = 0.0; /* clean extra row */
+  ds.n = nconv; ds.l = 0; ds.k = 0; ds.t = nconv;
+  if (!herm && nconv > 0) {
+    if (ds_vectors_nhep_all(&ds, 0)) { rc = -60; goto done; }
+    orc_bv_multinplace(n, ks.V, ks.ld, 0, nconv, 0, nconv, ds.X, ld);
+  }
+  /* reorder conjugate pairs, positive imaginary part first (epssolve.c:159-176) */
+  for (int i = 0; i < nconv - 1; i++) {
+    if (eigi[i] != 0) {
+      if (eigi[i] < 0) {
+        eigi[i] = -eigi[i]; eigi[i + 1] = -eigi[i + 1];
+        orc_bv_scale(n, ks.V + (int64_t)(i + 1) * ks.ld, -1.0);
+      }
+      i++;
+    }
+  }
+  for (int i = 0; i < nconv; i++) perm[i] = i;
+  for (int i = nconv - 1; i >= 0; i--) {                     /* SlepcSortEigenvalues (slepcsc.c:89-140) */
+    double re = eigr[perm[i]], im = eigi[perm[i]];
+    int j = i + 1;
+    if (im != 0) { i--; im = eigi[perm[i]]; }
+    while (j < nconv) {
+      if (sc_compare(which, re, im, eigr[perm[j]], eigi[perm[j]]) <= 0) break;
+      int tmp;
+      if (!im) {
+        if (eigi[perm[j]] == 0.0) { tmp = perm[j - 1]; perm[j - 1] = perm[j]; perm[j] = tmp; j++; }
+        else { tmp = perm[j - 1]; perm[j - 1] = perm[j]; perm[j] = perm[j + 1]; perm[j + 1] = tmp; j += 2; }
+      } else {
+        if (eigi[perm[j]] == 0.0) { tmp = perm[j - 2]; perm[j - 2] = perm[j]; perm[j] = perm[j - 1]; perm[j - 1] = tmp; j++; }
+        else {
+          tmp = perm[j - 2]; perm[j - 2] = perm[j]; perm[j] = tmp;
+          tmp = perm[j - 1]; perm[j - 1] = perm[j + 1]; perm[j + 1] = tmp; j += 2;
+        }
+      }
+    }
+  }
+  res->seconds = now_s() - t0;
+  res->nconv = nconv; res->its = its; res->reason = reason; res->ncv_used = ncv;
+  res->matvecs = ks.matvecs; res->orth_passes = ks.passes;
+  for (int i = 0; i < nconv; i++) {
+    if (res->eigr) res->eigr[i] = eigr[perm[i]];
+    if (res->eigi) res->eigi[i] = eigi[perm[i]];
+    if (res->errest) res->errest[i] = errest[perm[i]];
+  }
+  if (res->vecs) {
+    int nv = res->nvec < nconv ? res->nvec : nconv;
+    for (int i = 0; i < nv; i++) memcpy(res->vecs + (int64_t)i * n, ks.V + (int64_t)perm[i] * ks.ld, sizeof(double) * (size_t)n);
+  }
+done:
+#ifdef _OPENMP
+  omp_set_num_threads(saved_threads);
+#endif
+  free(ks.V); free(ks.buf); free(ds.A); free(ds.Q); free(ds.X); free(ds.T); free(ds.work); free(ds.perm); free(ds.iwork);
+  free(eigr); free(eigi); free(errest); free(perm);
+  return rc;
+}
+
+/* EPSComputeResidualNorm_Private + EPS_ERROR_RELATIVE (epssolve.c:666-815) */
+double orc_residual_rel(int n, const int32_t *rp, const int32_t *ci, const double *val, double kr, double ki,
+                        const double *xr, const double *xi)
+{
+  double *u = (double *)malloc(sizeof(double) * n);
+  double nr, ni = 0.0;
+  const double eps = 2.220446049250313e-16;
+  if (ki == 0 || fabs(ki) < fabs(kr * eps) || !xi) {
+    orc_spmv(n, rp, ci, val, xr, u);
+    if (fabs(kr) > eps) for (int i = 0; i < n; i++) u[i] -= kr * xr[i];
+    nr = orc_bv_norm2(n, u);
+    free(u);
+    return nr / fabs(kr);
+  }
+  orc_spmv(n, rp, ci, val, xr, u);
+  for (int i = 0; i < n; i++) u[i] += -kr * xr[i] + ki * xi[i];
+  nr = orc_bv_norm2(n, u);
+  orc_spmv(n, rp, ci, val, xi, u);
+  for (int i = 0; i < n; i++) u[i] += -kr * xi[i] - ki * xr[i];
+  ni = orc_bv_norm2(n, u);
+  free(u);
+  return hypot(nr, ni) / hypot(kr, ki);
+}

@@ -1,0 +1,90 @@
+/*
+ * oracle.h -- CPU FP64 restatement of the SLEPc 3.22.2 Krylov-Schur hot path.
+ *
+ * TEST INFRASTRUCTURE ONLY.  This library is the checker for the HIP product
+ * in slepc_amd/: only tests/, __graft_entry__.smoke() and bench.py's
+ * cpu_baseline leg may load it.  The product never links or calls it.
+ *
+ * Every function cites the reference file:line it restates (paths relative
+ * to the SLEPc source tree).  Problem generators follow the stencil
+ * definitions of the workloads (2-D/3-D Laplacians, convection-diffusion,
+ * 27-point stencil, finite-volume diffusion, Markov walk of ex5.c).
+ */
+#ifndef SLEPC_AMD_ORACLE_H
+#define SLEPC_AMD_ORACLE_H
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- deterministic random numbers (counter based, splitmix64) ---------- */
+uint64_t orc_splitmix64(uint64_t z);
+double   orc_u01(uint64_t seed, uint64_t i);           /* uniform [0,1)  */
+void     orc_rand_uniform(uint64_t seed, int64_t n, double *out); /* U[-1,1) */
+double   orc_gauss(uint64_t seed, uint64_t i);         /* N(0,1) Box-Muller */
+uint64_t orc_sub_seed(uint64_t seed, uint64_t p);
+
+/* ---- CSR generators: return nnz; fill arrays when non-NULL ------------- */
+int64_t orc_gen_lap2d(int nx, int ny, int32_t *rowptr, int32_t *col, double *val);
+int64_t orc_gen_lap2d_pot(int nx, int ny, uint64_t seed, double scale,
+                          int32_t *rowptr, int32_t *col, double *val);
+int64_t orc_gen_convdiff3d(int nx, int ny, int nz, int32_t *rowptr, int32_t *col, double *val);
+int64_t orc_gen_stencil27(int nx, int ny, int nz, uint64_t seed,
+                          int32_t *rowptr, int32_t *col, double *val);
+int64_t orc_gen_fv2d(int nx, int ny, const double *theta, double *sigma_out,
+                     int32_t *rowptr, int32_t *col, double *val);
+int64_t orc_gen_markov(int m, int32_t *rowptr, int32_t *col, double *val);
+void    orc_fv2d_theta(uint64_t seed, int sample, double *theta8);
+
+/* ---- SpMV and BV kernels ----------------------------------------------- */
+void orc_spmv(int n, const int32_t *rowptr, const int32_t *col, const double *val,
+              const double *x, double *y);
+void orc_bv_dotvec(int64_t n, int k, const double *V, int64_t ld, const double *x, double *q);
+void orc_bv_multvec(int64_t n, int k, double alpha, const double *V, int64_t ld,
+                    const double *q, double beta, double *y);
+void orc_bv_multinplace(int64_t n, double *V, int64_t ld, int l, int k, int s, int e,
+                        const double *Q, int ldq);
+void orc_bv_scale(int64_t n, double *x, double alpha);
+double orc_bv_norm2(int64_t n, const double *x);
+/* BVOrthonormalizeColumn with CGS + DGKS(if needed): buffer is m x m */
+int  orc_bv_orthonormalize_column(int64_t n, double *V, int64_t ld, int j, double *buffer,
+                                  int m, double eta, double *norm, int *lindep);
+
+/* ---- dense projected problems (host LAPACK) ---------------------------- */
+int  orc_lapack_status(char *buf, int len);
+void orc_arrow_tridiag(int n, double *d, double *e, double *Q, int ld);
+
+/* ---- Krylov-Schur solve ------------------------------------------------ */
+enum { ORC_LM = 1, ORC_SM = 2, ORC_LR = 3, ORC_SR = 4 };
+typedef struct {
+  int     hermitian;   /* 1: Lanczos + DSHEP, 0: Arnoldi + DSNHEP */
+  int     which;       /* ORC_LM ... */
+  int     nev, ncv, mpd, max_it;
+  double  tol;
+  double  keep;        /* restart keep ratio, default 0.5 */
+  double  eta;         /* DGKS threshold, default 0.7071 */
+  int     lock;        /* locking variant (default 1) */
+  int     nthreads;    /* OpenMP threads, 0 = default */
+} orc_opts;
+
+typedef struct {
+  int     nconv, its, reason;   /* reason: 1 converged, -1 max_it, -2 breakdown */
+  int     ncv_used;
+  double  *eigr, *eigi;         /* [ncv] sorted as EPSGetEigenpair returns them */
+  double  *errest;              /* [ncv] */
+  double  *vecs;                /* [n * nvec] eigenvectors (real parts), may be NULL */
+  int     nvec;                 /* number of vectors to return */
+  int64_t matvecs, orth_passes; /* counters */
+  double  seconds;              /* wall time of the solve loop */
+} orc_result;
+
+int orc_eps_solve(int n, const int32_t *rowptr, const int32_t *col, const double *val,
+                  const orc_opts *o, const double *v0, orc_result *r);
+double orc_residual_rel(int n, const int32_t *rowptr, const int32_t *col, const double *val,
+                        double lr, double li, const double *xr, const double *xi);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

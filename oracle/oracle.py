@@ -1,0 +1,261 @@
+"""ctypes wrapper over liboracle.so -- the CPU FP64 restatement of SLEPc's
+Krylov-Schur hot path.
+
+TEST INFRASTRUCTURE ONLY: used by tests/, __graft_entry__.smoke() and the
+cpu_baseline leg of bench.py as the checker.  Never imported by slepc_amd/.
+"""
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB = None
+
+LM, SM, LR, SR = 1, 2, 3, 4
+WHICH = {"LM": LM, "SM": SM, "LR": LR, "SR": SR}
+
+
+class Opts(C.Structure):
+    _fields_ = [("hermitian", C.c_int), ("which", C.c_int), ("nev", C.c_int),
+                ("ncv", C.c_int), ("mpd", C.c_int), ("max_it", C.c_int),
+                ("tol", C.c_double), ("keep", C.c_double), ("eta", C.c_double),
+                ("lock", C.c_int), ("nthreads", C.c_int)]
+
+
+class Result(C.Structure):
+    _fields_ = [("nconv", C.c_int), ("its", C.c_int), ("reason", C.c_int),
+                ("ncv_used", C.c_int), ("eigr", C.POINTER(C.c_double)),
+                ("eigi", C.POINTER(C.c_double)), ("errest", C.POINTER(C.c_double)),
+                ("vecs", C.POINTER(C.c_double)), ("nvec", C.c_int),
+                ("matvecs", C.c_int64), ("orth_passes", C.c_int64),
+                ("seconds", C.c_double)]
+
+
+def build():
+    subprocess.run(["make", "-s", "-C", _HERE], check=True)
+
+
+def lib():
+    global _LIB
+    if _LIB is None:
+        path = os.path.join(_HERE, "liboracle.so")
+        if not os.path.exists(path):
+            build()
+        L = C.CDLL(path)
+        i32p = np.ctypeslib.ndpointer(np.int32, flags="C")
+        f64p = np.ctypeslib.ndpointer(np.float64, flags="C")
+        L.orc_u01.restype = C.c_double
+        L.orc_u01.argtypes = [C.c_uint64, C.c_uint64]
+        L.orc_gauss.restype = C.c_double
+        L.orc_gauss.argtypes = [C.c_uint64, C.c_uint64]
+        L.orc_sub_seed.restype = C.c_uint64
+        L.orc_sub_seed.argtypes = [C.c_uint64, C.c_uint64]
+        L.orc_rand_uniform.argtypes = [C.c_uint64, C.c_int64, f64p]
+        L.orc_lapack_status.argtypes = [C.c_char_p, C.c_int]
+        for name in ("orc_gen_lap2d", "orc_gen_lap2d_pot", "orc_gen_convdiff3d",
+                     "orc_gen_stencil27", "orc_gen_fv2d", "orc_gen_markov"):
+            getattr(L, name).restype = C.c_int64
+        L.orc_spmv.argtypes = [C.c_int, i32p, i32p, f64p, f64p, f64p]
+        L.orc_bv_dotvec.argtypes = [C.c_int64, C.c_int, f64p, C.c_int64, f64p, f64p]
+        L.orc_bv_multvec.argtypes = [C.c_int64, C.c_int, C.c_double, f64p, C.c_int64,
+                                     f64p, C.c_double, f64p]
+        L.orc_bv_multinplace.argtypes = [C.c_int64, f64p, C.c_int64, C.c_int, C.c_int,
+                                         C.c_int, C.c_int, f64p, C.c_int]
+        L.orc_bv_norm2.restype = C.c_double
+        L.orc_bv_norm2.argtypes = [C.c_int64, f64p]
+        L.orc_bv_orthonormalize_column.argtypes = [C.c_int64, f64p, C.c_int64, C.c_int, f64p,
+                                                   C.c_int, C.c_double,
+                                                   C.POINTER(C.c_double), C.POINTER(C.c_int)]
+        L.orc_arrow_tridiag.argtypes = [C.c_int, f64p, f64p, f64p, C.c_int]
+        L.orc_eps_solve.argtypes = [C.c_int, i32p, i32p, f64p, C.POINTER(Opts),
+                                    C.c_void_p, C.POINTER(Result)]
+        L.orc_residual_rel.restype = C.c_double
+        L.orc_residual_rel.argtypes = [C.c_int, i32p, i32p, f64p, C.c_double, C.c_double,
+                                       f64p, C.c_void_p]
+        _LIB = L
+    return _LIB
+
+
+def lapack_path():
+    buf = C.create_string_buffer(512)
+    ok = lib().orc_lapack_status(buf, 512)
+    return buf.value.decode() if ok else None
+
+
+# ---------------------------------------------------------------- generators
+class CSR:
+    def __init__(self, n, rowptr, col, val, sigma=None):
+        self.n, self.rowptr, self.col, self.val = n, rowptr, col, val
+        self.sigma = sigma
+
+    @property
+    def nnz(self):
+        return int(self.rowptr[-1])
+
+    def to_dense(self):
+        A = np.zeros((self.n, self.n))
+        for r in range(self.n):
+            s, e = self.rowptr[r], self.rowptr[r + 1]
+            A[r, self.col[s:e]] = self.val[s:e]
+        return A
+
+    def matvec(self, x):
+        y = np.empty(self.n)
+        lib().orc_spmv(self.n, self.rowptr, self.col, self.val, np.ascontiguousarray(x, np.float64), y)
+        return y
+
+
+def _gen(fn, n, args, sigma=False):
+    L = lib()
+    f = getattr(L, fn)
+    nul = C.c_void_p(0)
+    sig = C.c_double(0.0)
+    extra = [C.byref(sig)] if sigma else []
+    nnz = f(*args, *extra, nul, nul, nul)
+    rp = np.zeros(n + 1, np.int32)
+    ci = np.zeros(nnz, np.int32)
+    v = np.zeros(nnz, np.float64)
+    f(*args, *extra, rp.ctypes.data_as(C.c_void_p), ci.ctypes.data_as(C.c_void_p),
+      v.ctypes.data_as(C.c_void_p))
+    return CSR(n, rp, ci, v, sig.value if sigma else None)
+
+
+def lap2d(nx, ny=None):
+    ny = nx if ny is None else ny
+    return _gen("orc_gen_lap2d", nx * ny, [C.c_int(nx), C.c_int(ny)])
+
+
+def lap2d_pot(nx, ny, seed, scale=10.0):
+    return _gen("orc_gen_lap2d_pot", nx * ny, [C.c_int(nx), C.c_int(ny), C.c_uint64(seed),
+                                               C.c_double(scale)])
+
+
+def convdiff3d(nx, ny=None, nz=None):
+    ny = nx if ny is None else ny
+    nz = nx if nz is None else nz
+    return _gen("orc_gen_convdiff3d", nx * ny * nz, [C.c_int(nx), C.c_int(ny), C.c_int(nz)])
+
+
+def stencil27(nx, ny=None, nz=None, seed=1):
+    ny = nx if ny is None else ny
+    nz = nx if nz is None else nz
+    return _gen("orc_gen_stencil27", nx * ny * nz, [C.c_int(nx), C.c_int(ny), C.c_int(nz),
+                                                    C.c_uint64(seed)])
+
+
+def fv_theta(seed, sample):
+    th = np.zeros(8)
+    lib().orc_fv2d_theta(C.c_uint64(seed), C.c_int(sample), th.ctypes.data_as(C.c_void_p))
+    return th
+
+
+def fv2d(nx, ny, theta):
+    th = np.ascontiguousarray(theta, np.float64)
+    return _gen("orc_gen_fv2d", nx * ny, [C.c_int(nx), C.c_int(ny), th.ctypes.data_as(C.c_void_p)],
+                sigma=True)
+
+
+def markov(m):
+    return _gen("orc_gen_markov", m * (m + 1) // 2, [C.c_int(m)])
+
+
+def rand_uniform(seed, n):
+    out = np.empty(n)
+    lib().orc_rand_uniform(seed, n, out)
+    return out
+
+
+def u01(seed, i):
+    return lib().orc_u01(seed, i)
+
+
+def sub_seed(seed, p):
+    return lib().orc_sub_seed(seed, p)
+
+
+# ---------------------------------------------------------------- BV ops
+def bv_dotvec(V, x):
+    n, k = V.shape
+    Vf = np.asfortranarray(V)
+    q = np.empty(k)
+    lib().orc_bv_dotvec(n, k, Vf.ravel(order="F"), n, np.ascontiguousarray(x, np.float64), q)
+    return q
+
+
+def bv_multvec(V, alpha, q, beta, y):
+    n, k = V.shape
+    Vf = np.asfortranarray(V).ravel(order="F")
+    y = np.array(y, np.float64)
+    lib().orc_bv_multvec(n, k, alpha, Vf, n, np.ascontiguousarray(q, np.float64), beta, y)
+    return y
+
+
+def bv_multinplace(V, Q, s, e, l=0, k=None):
+    n, m = V.shape
+    k = m if k is None else k
+    flat = np.array(V, order="F").ravel(order="F")
+    Qf = np.asfortranarray(Q)
+    lib().orc_bv_multinplace(n, flat, n, l, k, s, e, Qf.ravel(order="F"), Qf.shape[0])
+    return flat.reshape((n, m), order="F")
+
+
+def bv_orthonormalize_column(V, j, eta=0.7071):
+    """Returns (V', norm, lindep, h) -- h the coefficient column incl. norm."""
+    n, m = V.shape
+    flat = np.array(V, order="F").ravel(order="F")
+    buf = np.zeros(m * m)
+    nrm = C.c_double()
+    lind = C.c_int()
+    lib().orc_bv_orthonormalize_column(n, flat, n, j, buf, m, eta, C.byref(nrm), C.byref(lind))
+    return flat.reshape((n, m), order="F"), nrm.value, lind.value, buf[j * m:j * m + j + 1].copy()
+
+
+def arrow_tridiag(d, e):
+    n = len(d)
+    d = np.array(d, np.float64)
+    e = np.array(e, np.float64)
+    Q = np.eye(n).ravel(order="F").copy()
+    lib().orc_arrow_tridiag(n, d, e, Q, n)
+    return d, e, Q.reshape((n, n), order="F")
+
+
+# ---------------------------------------------------------------- solve
+def eps_solve(A, nev, ncv=0, tol=1e-8, which="LM", hermitian=True, v0=None, max_it=0,
+              mpd=0, nvec=None, nthreads=0, keep=0.5, lock=1):
+    o = Opts(hermitian=1 if hermitian else 0, which=WHICH[which], nev=nev, ncv=ncv, mpd=mpd,
+             max_it=max_it, tol=tol, keep=keep, eta=0.7071, lock=lock, nthreads=nthreads)
+    ncv_eff = ncv if ncv else min(A.n, max(2 * nev, nev + 15))
+    eigr = np.zeros(ncv_eff + 1)
+    eigi = np.zeros(ncv_eff + 1)
+    err = np.zeros(ncv_eff + 1)
+    nvec = ncv_eff if nvec is None else nvec
+    vecs = np.zeros(A.n * max(nvec, 1))
+    r = Result()
+    r.eigr = eigr.ctypes.data_as(C.POINTER(C.c_double))
+    r.eigi = eigi.ctypes.data_as(C.POINTER(C.c_double))
+    r.errest = err.ctypes.data_as(C.POINTER(C.c_double))
+    r.vecs = vecs.ctypes.data_as(C.POINTER(C.c_double)) if nvec else None
+    r.nvec = nvec
+    v0p = None
+    if v0 is not None:
+        v0 = np.ascontiguousarray(v0, np.float64)
+        v0p = v0.ctypes.data_as(C.c_void_p)
+    rc = lib().orc_eps_solve(A.n, A.rowptr, A.col, A.val, C.byref(o), v0p, C.byref(r))
+    if rc != 0:
+        raise RuntimeError("oracle eps_solve failed rc=%d" % rc)
+    nc = r.nconv
+    X = vecs[:A.n * min(nc, nvec)].reshape((min(nc, nvec), A.n)).T.copy() if nvec else None
+    return dict(nconv=nc, its=r.its, reason=r.reason, eigr=eigr[:nc].copy(), eigi=eigi[:nc].copy(),
+                errest=err[:nc].copy(), vecs=X, matvecs=r.matvecs, passes=r.orth_passes,
+                seconds=r.seconds)
+
+
+def residual_rel(A, lr, li, xr, xi=None):
+    xr = np.ascontiguousarray(xr, np.float64)
+    xip = None
+    if xi is not None:
+        xi = np.ascontiguousarray(xi, np.float64)
+        xip = xi.ctypes.data_as(C.c_void_p)
+    return lib().orc_residual_rel(A.n, A.rowptr, A.col, A.val, lr, li, xr, xip)

@@ -1,0 +1,359 @@
+"""slepc_amd -- MI355X-native Krylov-Schur hot path.
+
+Python host mirror of the reference's operator interface (names follow
+SLEPc/slepc4py: ``Mat``, ``BV``, ``EPS`` with ``setOperators``,
+``setDimensions``, ``setInitialSpace``, ``solve``, ``getEigenpair``,
+``computeError``).  Every numeric operation runs in libslepc_amd.so on the GPU;
+this module only moves arguments across the C ABI.
+"""
+import ctypes as C
+
+import numpy as np
+
+from . import _lib
+from ._lib import (FP32, FP64, LARGEST_MAGNITUDE, LARGEST_REAL, SMALLEST_MAGNITUDE, SMALLEST_REAL,
+                   ERROR_ABSOLUTE, ERROR_RELATIVE, SlepcAmdError, check, lib)
+
+__all__ = ["Mat", "BV", "EPS", "CSR", "init", "lap2d", "convdiff3d", "stencil27", "fv2d", "markov",
+           "fv_theta", "rand_uniform", "FP32", "FP64", "SlepcAmdError"]
+
+WHICH = {"LM": LARGEST_MAGNITUDE, "SM": SMALLEST_MAGNITUDE, "LR": LARGEST_REAL, "SR": SMALLEST_REAL}
+
+
+def init(device=0):
+    check(lib().sa_init(device))
+
+
+# ------------------------------------------------------------------ CSR host data
+class CSR:
+    """Host CSR arrays (int32 row pointers / columns, float64 values)."""
+
+    def __init__(self, n, rowptr, col, val, sigma=None):
+        self.n, self.rowptr, self.col, self.val, self.sigma = n, rowptr, col, val, sigma
+
+    @property
+    def nnz(self):
+        return int(self.rowptr[-1])
+
+
+def _gen(fname, n, args, with_sigma=False):
+    f = getattr(lib(), fname)
+    sig = C.c_double(0.0)
+    extra = [C.byref(sig)] if with_sigma else []
+    nnz = f(*args, *extra, None, None, None)
+    rp = np.zeros(n + 1, np.int32)
+    ci = np.zeros(nnz, np.int32)
+    v = np.zeros(nnz, np.float64)
+    f(*args, *extra, rp.ctypes.data, ci.ctypes.data, v.ctypes.data)
+    return CSR(n, rp, ci, v, sig.value if with_sigma else None)
+
+
+def lap2d(nx, ny=None, pot_scale=0.0, seed=0):
+    ny = nx if ny is None else ny
+    return _gen("sa_gen_lap2d", nx * ny, [nx, ny, pot_scale, seed])
+
+
+def convdiff3d(nx, ny=None, nz=None):
+    ny = nx if ny is None else ny
+    nz = nx if nz is None else nz
+    return _gen("sa_gen_convdiff3d", nx * ny * nz, [nx, ny, nz])
+
+
+def stencil27(nx, ny=None, nz=None, seed=1):
+    ny = nx if ny is None else ny
+    nz = nx if nz is None else nz
+    return _gen("sa_gen_stencil27", nx * ny * nz, [nx, ny, nz, seed])
+
+
+def fv2d(nx, ny, theta):
+    th = np.ascontiguousarray(theta, np.float64)
+    return _gen("sa_gen_fv2d", nx * ny, [nx, ny, th], with_sigma=True)
+
+
+def markov(m):
+    return _gen("sa_gen_markov", m * (m + 1) // 2, [m])
+
+
+def fv_theta(seed, sample):
+    th = np.zeros(8)
+    lib().sa_fv2d_theta(seed, sample, th)
+    return th
+
+
+def rand_uniform(seed, n):
+    out = np.empty(n)
+    lib().sa_rand_uniform(seed, n, out)
+    return out
+
+
+def csr_assemble(n, rows, cols, vals):
+    rows = np.ascontiguousarray(rows, np.int32)
+    cols = np.ascontiguousarray(cols, np.int32)
+    vals = np.ascontiguousarray(vals, np.float64)
+    L = lib()
+    nnz = L.sa_csr_assemble(n, len(rows), rows, cols, vals, None, None, None)
+    if nnz < 0:
+        raise SlepcAmdError("index out of range in csr_assemble")
+    rp = np.zeros(n + 1, np.int32)
+    ci = np.zeros(nnz, np.int32)
+    v = np.zeros(nnz, np.float64)
+    L.sa_csr_assemble(n, len(rows), rows, cols, vals, rp.ctypes.data, ci.ctypes.data, v.ctypes.data)
+    return CSR(n, rp, ci, v)
+
+
+# ------------------------------------------------------------------ device objects
+class DeviceArray:
+    """A device buffer owned by the library allocator."""
+
+    def __init__(self, nbytes):
+        self.ptr = C.c_void_p()
+        self.nbytes = int(nbytes)
+        check(lib().sa_malloc(C.byref(self.ptr), self.nbytes))
+
+    @classmethod
+    def from_host(cls, arr):
+        arr = np.ascontiguousarray(arr)
+        d = cls(arr.nbytes)
+        check(lib().sa_memcpy_h2d(d.ptr, arr.ctypes.data, arr.nbytes))
+        return d
+
+    def to_host(self, dtype, count):
+        out = np.empty(count, dtype)
+        check(lib().sa_memcpy_d2h(out.ctypes.data, self.ptr, out.nbytes))
+        return out
+
+    def free(self):
+        if self.ptr:
+            lib().sa_free(self.ptr)
+            self.ptr = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+
+class Mat:
+    """CSR operator in HBM (MATAIJ analogue); MatMult runs the HIP SpMV."""
+
+    def __init__(self, csr, prec=FP64):
+        self.h = C.c_void_p()
+        self.n, self.prec = csr.n, prec
+        check(lib().sa_mat_create_csr(csr.n, csr.nnz, csr.rowptr, csr.col, csr.val, prec, C.byref(self.h)))
+
+    def mult(self, x_dev, y_dev):
+        check(lib().sa_mat_mult(self.h, x_dev.ptr if isinstance(x_dev, DeviceArray) else x_dev,
+                                y_dev.ptr if isinstance(y_dev, DeviceArray) else y_dev))
+
+    def mult_host(self, x):
+        dt = np.float64 if self.prec == FP64 else np.float32
+        xd = DeviceArray.from_host(np.asarray(x, dt))
+        yd = DeviceArray(self.n * np.dtype(dt).itemsize)
+        self.mult(xd, yd)
+        return yd.to_host(dt, self.n)
+
+    def destroy(self):
+        if self.h:
+            lib().sa_mat_destroy(self.h)
+            self.h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.destroy()
+        except Exception:
+            pass
+
+
+class BV:
+    """Basis of m columns of length n in HBM (BVMAT analogue)."""
+
+    def __init__(self, n, m, prec=FP64):
+        self.h = C.c_void_p()
+        self.n, self.m, self.prec = n, m, prec
+        check(lib().sa_bv_create(n, m, prec, C.byref(self.h)))
+        ld = C.c_int64()
+        check(lib().sa_bv_get_info(self.h, None, None, C.byref(ld), None))
+        self.ld = ld.value
+
+    def set_active_columns(self, l, k):
+        check(lib().sa_bv_set_active_columns(self.h, l, k))
+
+    def set_column(self, j, x):
+        check(lib().sa_bv_set_column(self.h, j, np.ascontiguousarray(x, np.float64)))
+
+    def get_column(self, j):
+        out = np.empty(self.n)
+        check(lib().sa_bv_get_column(self.h, j, out))
+        return out
+
+    def set_matrix(self, X):
+        for j in range(X.shape[1]):
+            self.set_column(j, X[:, j])
+
+    def get_matrix(self, ncols=None):
+        ncols = self.m if ncols is None else ncols
+        return np.stack([self.get_column(j) for j in range(ncols)], axis=1)
+
+    def column_ptr(self, j):
+        base = C.c_void_p()
+        check(lib().sa_bv_get_array(self.h, C.byref(base)))
+        es = 8 if self.prec == FP64 else 4
+        return C.c_void_p(base.value + j * self.ld * es)
+
+    def dotvec(self, x_dev):
+        l, k = C.c_int(), C.c_int()
+        check(lib().sa_bv_get_active_columns(self.h, C.byref(l), C.byref(k)))
+        q = np.zeros(max(k.value - l.value, 1))
+        check(lib().sa_bv_dotvec(self.h, x_dev, q.ctypes.data))
+        return q[:k.value - l.value]
+
+    def multvec(self, alpha, beta, y_dev, q):
+        q = np.ascontiguousarray(q, np.float64)
+        check(lib().sa_bv_multvec(self.h, alpha, beta, y_dev, q.ctypes.data))
+
+    def multinplace(self, Q, s, e):
+        Qf = np.asfortranarray(Q, np.float64)
+        check(lib().sa_bv_multinplace(self.h, Qf.ravel(order="F"), Qf.shape[0], s, e))
+
+    def scale_column(self, j, alpha):
+        check(lib().sa_bv_scale_column(self.h, j, alpha))
+
+    def norm_column(self, j):
+        v = C.c_double()
+        check(lib().sa_bv_norm_column(self.h, j, C.byref(v)))
+        return v.value
+
+    def norm(self):
+        v = C.c_double()
+        check(lib().sa_bv_norm(self.h, C.byref(v)))
+        return v.value
+
+    def copy_column(self, src, dst):
+        check(lib().sa_bv_copy_column(self.h, src, dst))
+
+    def mat_mult_column(self, A, j):
+        check(lib().sa_bv_mat_mult_column(self.h, A.h, j))
+
+    def orthonormalize_column(self, j, eta=0.7071):
+        nrm, lind = C.c_double(), C.c_int()
+        check(lib().sa_bv_orthonormalize_column(self.h, j, eta, C.byref(nrm), C.byref(lind)))
+        return nrm.value, bool(lind.value)
+
+    def coefficients(self, c0, c1):
+        buf = np.zeros(self.m * max(c1 - c0, 1))
+        check(lib().sa_bv_get_coefficients(self.h, c0, c1, buf))
+        return buf[:self.m * (c1 - c0)].reshape((c1 - c0, self.m)).T
+
+    def mat_arnoldi(self, A, k, m, eta=0.7071):
+        mm, beta, bd = C.c_int(m), C.c_double(), C.c_int()
+        check(lib().sa_bv_mat_arnoldi(self.h, A.h, k, C.byref(mm), eta, C.byref(beta), C.byref(bd)))
+        return mm.value, beta.value, bool(bd.value)
+
+    def destroy(self):
+        if self.h:
+            lib().sa_bv_destroy(self.h)
+            self.h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.destroy()
+        except Exception:
+            pass
+
+
+class EPS:
+    """Krylov-Schur eigensolver (EPSKRYLOVSCHUR default variant)."""
+
+    def __init__(self):
+        self.h = C.c_void_p()
+        check(lib().sa_eps_create(C.byref(self.h)))
+        self.A = self.A32 = None
+
+    def setOperators(self, A, A32=None):
+        self.A = A
+        check(lib().sa_eps_set_operators(self.h, A.h))
+        if A32 is not None:
+            self.A32 = A32
+            check(lib().sa_eps_set_operators_fp32(self.h, A32.h))
+
+    def setProblemType(self, hermitian):
+        check(lib().sa_eps_set_problem_type(self.h, 1 if hermitian else 0))
+
+    def setWhichEigenpairs(self, which):
+        check(lib().sa_eps_set_which(self.h, WHICH[which] if isinstance(which, str) else which))
+
+    def setDimensions(self, nev, ncv=0, mpd=0):
+        check(lib().sa_eps_set_dimensions(self.h, nev, ncv, mpd))
+
+    def setTolerances(self, tol=0.0, max_it=0):
+        check(lib().sa_eps_set_tolerances(self.h, tol, max_it))
+
+    def setInitialSpace(self, vecs):
+        V = np.ascontiguousarray(np.atleast_2d(np.asarray(vecs, np.float64)))
+        if V.shape[1] != self.A.n:
+            V = np.ascontiguousarray(V.T)
+        check(lib().sa_eps_set_initial_space(self.h, V.shape[0], V.ravel()))
+
+    def setMixedPrecision(self, mixed=True, tol32=1e-5):
+        check(lib().sa_eps_set_mixed(self.h, 1 if mixed else 0, tol32))
+
+    def solve(self):
+        check(lib().sa_eps_solve(self.h))
+
+    def getConverged(self):
+        v = C.c_int()
+        check(lib().sa_eps_get_converged(self.h, C.byref(v)))
+        return v.value
+
+    def getIterationNumber(self):
+        v = C.c_int()
+        check(lib().sa_eps_get_iteration_number(self.h, C.byref(v)))
+        return v.value
+
+    def getConvergedReason(self):
+        v = C.c_int()
+        check(lib().sa_eps_get_converged_reason(self.h, C.byref(v)))
+        return v.value
+
+    def getEigenpair(self, i, vectors=True):
+        re, im = C.c_double(), C.c_double()
+        xr = np.empty(self.A.n) if vectors else None
+        xi = np.empty(self.A.n) if vectors else None
+        check(lib().sa_eps_get_eigenpair(self.h, i, C.byref(re), C.byref(im),
+                                         xr.ctypes.data if vectors else None,
+                                         xi.ctypes.data if vectors else None))
+        return re.value, im.value, xr, xi
+
+    def getEigenvalues(self):
+        out = []
+        for i in range(self.getConverged()):
+            re, im, _, _ = self.getEigenpair(i, vectors=False)
+            out.append(complex(re, im) if im else re)
+        return out
+
+    def getErrorEstimate(self, i):
+        v = C.c_double()
+        check(lib().sa_eps_get_error_estimate(self.h, i, C.byref(v)))
+        return v.value
+
+    def computeError(self, i, etype=ERROR_RELATIVE):
+        v = C.c_double()
+        check(lib().sa_eps_compute_error(self.h, i, etype, C.byref(v)))
+        return v.value
+
+    def getStats(self):
+        s = _lib.EpsStats()
+        check(lib().sa_eps_get_stats(self.h, C.byref(s)))
+        return {f: getattr(s, f) for f, _ in s._fields_}
+
+    def destroy(self):
+        if self.h:
+            lib().sa_eps_destroy(self.h)
+            self.h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.destroy()
+        except Exception:
+            pass

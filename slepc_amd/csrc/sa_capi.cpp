@@ -1,0 +1,672 @@
+// extern "C" entry points of libslepc_amd.so (declared in include/slepc_amd.h).
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstring>
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "sa_ds.h"
+#include "sa_eps.h"
+#include "sa_internal.h"
+
+namespace sa {
+
+namespace {
+thread_local std::string g_err;
+hipStream_t g_stream = nullptr;
+int g_device = -1;
+int g_cus = 0;
+std::mutex g_init_mu;
+double now() {
+  return std::chrono::duration<double>(std::chrono::steady_clock::now().time_since_epoch()).count();
+}
+}  // namespace
+
+void set_error(const std::string &msg) { g_err = msg; }
+hipStream_t stream() { return g_stream; }
+int num_cus() { return g_cus; }
+
+int ensure_init() {
+  if (g_stream) return 0;
+  return sa_init(0);
+}
+
+}  // namespace sa
+
+using namespace sa;
+
+struct sa_mat_s {
+  Mat A;
+};
+struct sa_bv_s {
+  BV V;
+};
+struct sa_eps_s {
+  sa_mat A = nullptr, A32 = nullptr;
+  KsParams p;
+  bool ncv_set = false, mpd_set = false, maxit_set = false;
+  std::vector<double> init;   // initial space, nini x n
+  int nini = 0;
+  bool mixed = false;
+  double tol32 = 1e-5;
+  BV V;                        // FP64 basis holding the solution
+  KsResult res, res32;
+  sa_eps_stats st{};
+  bool solved = false;
+};
+
+#define API_INIT() SA_TRY(ensure_init())
+
+extern "C" {
+
+int sa_init(int device) {
+  std::lock_guard<std::mutex> lk(g_init_mu);
+  if (g_stream && device == g_device) return 0;
+  int ndev = 0;
+  SA_HIP(hipGetDeviceCount(&ndev));
+  SA_CHECK(ndev > 0, "no HIP device found");
+  SA_CHECK(device >= 0 && device < ndev, "device index out of range");
+  SA_HIP(hipSetDevice(device));
+  hipDeviceProp_t prop;
+  SA_HIP(hipGetDeviceProperties(&prop, device));
+  SA_CHECK(std::string(prop.gcnArchName).rfind("gfx950", 0) == 0,
+           std::string("libslepc_amd is built for gfx950, found ") + prop.gcnArchName);
+  g_cus = prop.multiProcessorCount;
+  if (g_stream) (void)hipStreamDestroy(g_stream);
+  SA_HIP(hipStreamCreateWithFlags(&g_stream, hipStreamNonBlocking));
+  g_device = device;
+  return 0;
+}
+
+const char *sa_last_error(void) { return g_err.c_str(); }
+void *sa_stream(void) { return (void *)g_stream; }
+int sa_synchronize(void) {
+  API_INIT();
+  SA_HIP(hipStreamSynchronize(g_stream));
+  return 0;
+}
+int sa_malloc(void **p, int64_t bytes) {
+  API_INIT();
+  SA_HIP(hipMalloc(p, (size_t)std::max<int64_t>(bytes, 1)));
+  return 0;
+}
+int sa_free(void *p) {
+  SA_HIP(hipFree(p));
+  return 0;
+}
+int sa_memcpy_h2d(void *dst, const void *src, int64_t bytes) {
+  API_INIT();
+  SA_HIP(hipMemcpyAsync(dst, src, (size_t)bytes, hipMemcpyHostToDevice, g_stream));
+  SA_HIP(hipStreamSynchronize(g_stream));
+  return 0;
+}
+int sa_memcpy_d2h(void *dst, const void *src, int64_t bytes) {
+  API_INIT();
+  SA_HIP(hipMemcpyAsync(dst, src, (size_t)bytes, hipMemcpyDeviceToHost, g_stream));
+  SA_HIP(hipStreamSynchronize(g_stream));
+  return 0;
+}
+
+// ---------------------------------------------------------------- Mat
+int sa_mat_create_csr(int n, int64_t nnz, const int32_t *rp, const int32_t *ci, const double *val, int prec,
+                      sa_mat *out) {
+  API_INIT();
+  SA_CHECK(out && n >= 0 && nnz >= 0, "invalid arguments to sa_mat_create_csr");
+  SA_CHECK(prec == SA_FP64 || prec == SA_FP32, "invalid precision");
+  SA_CHECK(rp && (nnz == 0 || (ci && val)), "null CSR arrays");
+  SA_CHECK(rp[0] == 0 && rp[n] == nnz, "row pointer does not match nnz");
+  for (int r = 0; r < n; r++) {
+    SA_CHECK(rp[r + 1] >= rp[r], "row pointer not monotone");
+    for (int32_t p = rp[r]; p < rp[r + 1]; p++) {
+      SA_CHECK(ci[p] >= 0 && ci[p] < n, "column index out of range");
+      SA_CHECK(p == rp[r] || ci[p] > ci[p - 1], "column indices must ascend within a row");
+    }
+  }
+  auto *M = new sa_mat_s();
+  Mat &A = M->A;
+  A.n = n;
+  A.nnz = nnz;
+  A.prec = prec;
+  const double avg = n ? (double)nnz / n : 1.0;
+  A.lanes = avg <= 4.5 ? 4 : (avg <= 9 ? 8 : (avg <= 18 ? 16 : 32));
+  const size_t es = prec_size(prec);
+  hipStream_t s = g_stream;
+  if (hipMalloc(&A.rowptr, sizeof(int32_t) * (n + 1)) != hipSuccess ||
+      hipMalloc(&A.col, sizeof(int32_t) * std::max<int64_t>(1, nnz)) != hipSuccess ||
+      hipMalloc(&A.val, es * std::max<int64_t>(1, nnz)) != hipSuccess) {
+    sa_mat_destroy(M);
+    set_error("hipMalloc failed for the CSR arrays");
+    return 1;
+  }
+  SA_HIP(hipMemcpyAsync(A.rowptr, rp, sizeof(int32_t) * (n + 1), hipMemcpyHostToDevice, s));
+  if (nnz) {
+    SA_HIP(hipMemcpyAsync(A.col, ci, sizeof(int32_t) * nnz, hipMemcpyHostToDevice, s));
+    if (prec == SA_FP64) {
+      SA_HIP(hipMemcpyAsync(A.val, val, sizeof(double) * nnz, hipMemcpyHostToDevice, s));
+    } else {
+      std::vector<float> f(val, val + nnz);
+      SA_HIP(hipMemcpyAsync(A.val, f.data(), sizeof(float) * nnz, hipMemcpyHostToDevice, s));
+      SA_HIP(hipStreamSynchronize(s));
+    }
+  }
+  SA_HIP(hipStreamSynchronize(s));
+  *out = M;
+  return 0;
+}
+
+int sa_mat_destroy(sa_mat M) {
+  if (!M) return 0;
+  if (M->A.rowptr) (void)hipFree(M->A.rowptr);
+  if (M->A.col) (void)hipFree(M->A.col);
+  if (M->A.val) (void)hipFree(M->A.val);
+  delete M;
+  return 0;
+}
+
+int sa_mat_get_info(sa_mat M, int *n, int64_t *nnz, int *prec) {
+  SA_CHECK(M, "null matrix");
+  if (n) *n = M->A.n;
+  if (nnz) *nnz = M->A.nnz;
+  if (prec) *prec = M->A.prec;
+  return 0;
+}
+
+int sa_mat_mult(sa_mat M, const void *x, void *y) {
+  API_INIT();
+  SA_CHECK(M && x && y, "null argument to sa_mat_mult");
+  return launch_spmv(M->A, x, y, g_stream);
+}
+
+// ---------------------------------------------------------------- BV
+int sa_bv_create(int64_t n, int m, int prec, sa_bv *out) {
+  API_INIT();
+  SA_CHECK(out, "null output");
+  SA_CHECK(m < GS_MAXC, "too many columns (the fused kernels handle up to 127)");
+  auto *B = new sa_bv_s();
+  int rc = bv_alloc(B->V, n, m, prec);
+  if (rc) {
+    bv_free(B->V);
+    delete B;
+    return rc;
+  }
+  *out = B;
+  return 0;
+}
+
+int sa_bv_destroy(sa_bv B) {
+  if (!B) return 0;
+  bv_free(B->V);
+  delete B;
+  return 0;
+}
+
+int sa_bv_get_info(sa_bv B, int64_t *n, int *m, int64_t *ld, int *prec) {
+  SA_CHECK(B, "null BV");
+  if (n) *n = B->V.n;
+  if (m) *m = B->V.m;
+  if (ld) *ld = B->V.ld;
+  if (prec) *prec = B->V.prec;
+  return 0;
+}
+
+int sa_bv_get_array(sa_bv B, void **data) {
+  SA_CHECK(B && data, "null argument");
+  *data = B->V.data;
+  return 0;
+}
+
+int sa_bv_set_active_columns(sa_bv B, int l, int k) {
+  SA_CHECK(B && l >= 0 && l <= k && k <= B->V.m, "invalid active columns");
+  B->V.l = l;
+  B->V.k = k;
+  return 0;
+}
+
+int sa_bv_get_active_columns(sa_bv B, int *l, int *k) {
+  SA_CHECK(B, "null BV");
+  if (l) *l = B->V.l;
+  if (k) *k = B->V.k;
+  return 0;
+}
+
+int sa_bv_set_column(sa_bv B, int j, const double *x) {
+  API_INIT();
+  SA_CHECK(B && x && j >= 0 && j < B->V.m, "invalid column");
+  return bv_upload_column(B->V, j, x);
+}
+
+int sa_bv_get_column(sa_bv B, int j, double *x) {
+  API_INIT();
+  SA_CHECK(B && x && j >= 0 && j < B->V.m, "invalid column");
+  return bv_download_column(B->V, j, x);
+}
+
+int sa_bv_dotvec(sa_bv B, const void *x, double *q) {
+  API_INIT();
+  SA_CHECK(B && x, "null argument");
+  BV &V = B->V;
+  const int nc = V.k - V.l;
+  double *qd = q ? V.dq : V.dbuf;   // q == NULL: column 0 of the coefficient buffer
+  SA_TRY(launch_dotvec(V, V.l, V.k, x, qd, g_stream));
+  if (q && nc > 0) SA_HIP(hipMemcpyAsync(q, V.dq, sizeof(double) * nc, hipMemcpyDeviceToHost, g_stream));
+  SA_HIP(hipStreamSynchronize(g_stream));
+  return 0;
+}
+
+int sa_bv_multvec(sa_bv B, double alpha, double beta, void *y, const double *q) {
+  API_INIT();
+  SA_CHECK(B && y, "null argument");
+  BV &V = B->V;
+  const int nc = V.k - V.l;
+  const double *qd = V.dbuf;
+  if (q) {
+    if (nc > 0) {
+      std::memcpy(V.hq, q, sizeof(double) * nc);
+      SA_HIP(hipMemcpyAsync(V.dq, V.hq, sizeof(double) * nc, hipMemcpyHostToDevice, g_stream));
+    }
+    qd = V.dq;
+  }
+  SA_TRY(launch_multvec(V, V.l, V.k, alpha, qd, beta, y, g_stream));
+  SA_HIP(hipStreamSynchronize(g_stream));
+  return 0;
+}
+
+int sa_bv_multinplace(sa_bv B, const double *Q, int ldq, int s_, int e) {
+  API_INIT();
+  SA_CHECK(B && Q, "null argument");
+  BV &V = B->V;
+  SA_CHECK(s_ >= V.l && e <= V.k && s_ <= e, "columns out of the active range");
+  SA_CHECK(ldq >= V.k, "leading dimension of Q too small");
+  const int64_t bytes = sizeof(double) * (int64_t)ldq * e;   // Q holds at least e columns
+  SA_CHECK(bytes <= (int64_t)sizeof(double) * V.m * V.m + 64 * (int64_t)sizeof(double), "Q too large");
+  std::memcpy(V.hq, Q, bytes);
+  SA_HIP(hipMemcpyAsync(V.dq, V.hq, bytes, hipMemcpyHostToDevice, g_stream));
+  SA_TRY(launch_multinplace(V, V.l, V.k, s_, e, V.dq, ldq, g_stream));
+  SA_HIP(hipStreamSynchronize(g_stream));
+  return 0;
+}
+
+int sa_bv_scale_column(sa_bv B, int j, double alpha) {
+  API_INIT();
+  SA_CHECK(B && j < B->V.m, "invalid column");
+  BV &V = B->V;
+  if (j >= 0) {
+    SA_TRY(launch_scale(V.prec, col_ptr(V, j), V.n, alpha, g_stream));
+  } else {
+    for (int c = V.l; c < V.k; c++) SA_TRY(launch_scale(V.prec, col_ptr(V, c), V.n, alpha, g_stream));
+  }
+  SA_HIP(hipStreamSynchronize(g_stream));
+  return 0;
+}
+
+int sa_bv_norm_column(sa_bv B, int j, double *nrm) {
+  API_INIT();
+  SA_CHECK(B && nrm && j >= 0 && j < B->V.m, "invalid column");
+  BV &V = B->V;
+  double *out = V.dq;
+  SA_TRY(launch_dot_self(V.prec, col_ptr(V, j), V.n, out, V.ws.part, g_stream));
+  double h = 0.0;
+  SA_HIP(hipMemcpyAsync(&h, out, sizeof(double), hipMemcpyDeviceToHost, g_stream));
+  SA_HIP(hipStreamSynchronize(g_stream));
+  *nrm = std::sqrt(h);
+  return 0;
+}
+
+int sa_bv_norm(sa_bv B, double *nrm) {
+  API_INIT();
+  SA_CHECK(B && nrm, "null argument");
+  BV &V = B->V;
+  double tot = 0.0;
+  for (int c = V.l; c < V.k; c++) {
+    double v = 0.0;
+    SA_TRY(sa_bv_norm_column(B, c, &v));
+    tot += v * v;
+  }
+  *nrm = std::sqrt(tot);
+  return 0;
+}
+
+int sa_bv_copy_column(sa_bv B, int src, int dst) {
+  API_INIT();
+  SA_CHECK(B && src >= 0 && dst >= 0 && src < B->V.m && dst < B->V.m, "invalid column");
+  BV &V = B->V;
+  SA_HIP(hipMemcpyAsync(col_ptr(V, dst), col_ptr(V, src), prec_size(V.prec) * V.n, hipMemcpyDeviceToDevice,
+                        g_stream));
+  SA_HIP(hipStreamSynchronize(g_stream));
+  return 0;
+}
+
+int sa_bv_mat_mult_column(sa_bv B, sa_mat M, int j) {
+  API_INIT();
+  SA_CHECK(B && M && j >= 0 && j + 1 < B->V.m, "invalid column");
+  SA_CHECK(M->A.prec == B->V.prec && M->A.n == B->V.n, "matrix and basis differ in size or precision");
+  SA_TRY(launch_spmv(M->A, col_ptr(B->V, j), col_ptr(B->V, j + 1), g_stream));
+  SA_HIP(hipStreamSynchronize(g_stream));
+  return 0;
+}
+
+int sa_bv_orthonormalize_column(sa_bv B, int j, double eta, double *norm, int *lindep) {
+  API_INIT();
+  SA_CHECK(B && j >= 0 && j < B->V.m && j < GS_MAXC, "invalid column");
+  return bv_orthonormalize(B->V, j, eta > 0 ? eta : 0.7071, norm, lindep);
+}
+
+int sa_bv_get_coefficients(sa_bv B, int c0, int c1, double *buf) {
+  API_INIT();
+  SA_CHECK(B && buf && c0 >= 0 && c1 <= B->V.m && c0 <= c1, "invalid coefficient range");
+  const int m = B->V.m;
+  if (c1 > c0)
+    SA_HIP(hipMemcpyAsync(buf, B->V.dbuf + (size_t)c0 * m, sizeof(double) * m * (c1 - c0), hipMemcpyDeviceToHost,
+                          g_stream));
+  SA_HIP(hipStreamSynchronize(g_stream));
+  return 0;
+}
+
+int sa_bv_mat_arnoldi(sa_bv B, sa_mat M, int k, int *m, double eta, double *beta, int *breakdown) {
+  API_INIT();
+  SA_CHECK(B && M && m && beta && breakdown, "null argument");
+  SA_CHECK(M->A.prec == B->V.prec && M->A.n == B->V.n, "matrix and basis differ in size or precision");
+  return bv_sweep(B->V, M->A, k, m, eta > 0 ? eta : 0.7071, beta, breakdown);
+}
+
+// ---------------------------------------------------------------- EPS
+int sa_eps_create(sa_eps *out) {
+  SA_CHECK(out, "null output");
+  *out = new sa_eps_s();
+  return 0;
+}
+
+int sa_eps_destroy(sa_eps E) {
+  if (!E) return 0;
+  bv_free(E->V);
+  delete E;
+  return 0;
+}
+
+int sa_eps_set_operators(sa_eps E, sa_mat A) {
+  SA_CHECK(E && A, "null argument");
+  SA_CHECK(A->A.prec == SA_FP64, "the operator of EPS must be FP64 (use sa_eps_set_operators_fp32 for the bulk phase)");
+  E->A = A;
+  E->solved = false;
+  return 0;
+}
+
+int sa_eps_set_operators_fp32(sa_eps E, sa_mat A32) {
+  SA_CHECK(E && A32 && A32->A.prec == SA_FP32, "expected an FP32 matrix");
+  E->A32 = A32;
+  return 0;
+}
+
+int sa_eps_set_problem_type(sa_eps E, int hermitian) {
+  SA_CHECK(E, "null EPS");
+  E->p.hermitian = hermitian != 0;
+  return 0;
+}
+
+int sa_eps_set_which(sa_eps E, int which) {
+  SA_CHECK(E && which >= 1 && which <= 4, "invalid which");
+  E->p.which = which;
+  return 0;
+}
+
+int sa_eps_set_dimensions(sa_eps E, int nev, int ncv, int mpd) {
+  SA_CHECK(E && nev > 0, "nev must be positive");
+  E->p.nev = nev;
+  E->ncv_set = ncv > 0;
+  E->mpd_set = mpd > 0;
+  if (ncv > 0) E->p.ncv = ncv;
+  if (mpd > 0) E->p.mpd = mpd;
+  return 0;
+}
+
+int sa_eps_set_tolerances(sa_eps E, double tol, int max_it) {
+  SA_CHECK(E, "null EPS");
+  if (tol > 0) E->p.tol = tol;
+  E->maxit_set = max_it > 0;
+  if (max_it > 0) E->p.max_it = max_it;
+  return 0;
+}
+
+int sa_eps_set_initial_space(sa_eps E, int nvec, const double *vecs) {
+  SA_CHECK(E && E->A, "set the operator first");
+  SA_CHECK(nvec >= 0 && (nvec == 0 || vecs), "invalid initial space");
+  const int n = E->A->A.n;
+  E->init.assign(vecs, vecs + (size_t)nvec * n);
+  E->nini = nvec;
+  return 0;
+}
+
+int sa_eps_set_mixed(sa_eps E, int mixed, double tol32) {
+  SA_CHECK(E, "null EPS");
+  E->mixed = mixed != 0;
+  if (tol32 > 0) E->tol32 = tol32;
+  return 0;
+}
+
+// EPSSetUp + EPSSetDimensions_Default (epssetup.c:669-694, krylovschur.c:93-120)
+static int eps_setup_dims(sa_eps E, KsParams &p) {
+  const int n = E->A->A.n;
+  p = E->p;
+  if (!E->ncv_set && !E->mpd_set) p.ncv = std::min(n, std::max(2 * p.nev, p.nev + 15));
+  else if (!E->ncv_set) p.ncv = std::min(n, p.nev + p.mpd);
+  if (!E->mpd_set) p.mpd = p.ncv;
+  SA_CHECK(p.ncv >= p.nev + 1 || (p.ncv == p.nev && p.ncv == n), "The value of ncv must be at least nev+1");
+  SA_CHECK(p.ncv <= p.nev + p.mpd, "The value of ncv must not be larger than nev+mpd");
+  if (!E->maxit_set) p.max_it = std::max(100, 2 * n / p.ncv);
+  return 0;
+}
+
+int sa_eps_solve(sa_eps E) {
+  API_INIT();
+  SA_CHECK(E && E->A, "EPS has no operator");
+  KsParams p;
+  SA_TRY(eps_setup_dims(E, p));
+  const Mat &A = E->A->A;
+  if (E->V.m != p.ncv + 1 || E->V.n != A.n || E->V.prec != SA_FP64) {
+    bv_free(E->V);
+    SA_TRY(bv_alloc(E->V, A.n, p.ncv + 1, SA_FP64));
+  }
+  E->st = sa_eps_stats{};
+  E->res32 = KsResult();
+  const double t0 = now();
+  bool have_init = E->nini > 0;
+  if (E->mixed) {
+    // Bulk phase in FP32 (basis, SpMV and BV products), projected problem FP64.
+    SA_CHECK(E->A32 && E->A32->A.n == A.n, "mixed precision needs the FP32 operator (sa_eps_set_operators_fp32)");
+    BV V32;
+    SA_TRY(bv_alloc(V32, A.n, p.ncv + 1, SA_FP32));
+    KsParams p32 = p;
+    p32.tol = E->tol32;
+    int rc = 0;
+    if (have_init) rc = bv_upload_column(V32, 0, E->init.data());
+    if (!rc) rc = ks_solve(p32, E->A32->A, V32, have_init, E->res32);
+    if (!rc) {
+      // Seed for the FP64 polish: the sum of the first nev FP32 Ritz vectors,
+      // each of unit norm, promoted to FP64 (EPSSetInitialSpace of the polish).
+      const int nr = std::min(E->res32.nconv, p.nev);
+      SA_CHECK(nr > 0 || E->res32.nconv == 0, "internal");
+      if (nr > 0) {
+        std::vector<double> q(p.ncv, 0.0);
+        for (int i = 0; i < nr; i++) q[E->res32.perm[i]] = 1.0;
+        // V64(:,0) = V32(:,perm[0..nr)) * ones, computed in FP64 after promotion.
+        rc = launch_convert(SA_FP32, V32.data, V32.ld, SA_FP64, E->V.data, E->V.ld, A.n, p.ncv + 1, stream());
+        if (!rc) {
+          std::memcpy(E->V.hq, q.data(), sizeof(double) * p.ncv);
+          if (hipMemcpyAsync(E->V.dq, E->V.hq, sizeof(double) * p.ncv, hipMemcpyHostToDevice, stream()) !=
+              hipSuccess)
+            rc = 1;
+        }
+        // V(:,ncv) = V(:,0:ncv) q, then moved to column 0
+        if (!rc) rc = launch_multvec(E->V, 0, p.ncv, 1.0, E->V.dq, 0.0, col_ptr(E->V, p.ncv), stream());
+        if (!rc && hipMemcpyAsync(col_ptr(E->V, 0), col_ptr(E->V, p.ncv), sizeof(double) * A.n,
+                                  hipMemcpyDeviceToDevice, stream()) != hipSuccess)
+          rc = 1;
+        if (!rc && hipStreamSynchronize(stream()) != hipSuccess) rc = 1;
+        have_init = true;
+      }
+    }
+    bv_free(V32);
+    if (rc) return rc;
+  } else if (have_init) {
+    SA_TRY(bv_upload_column(E->V, 0, E->init.data()));
+  }
+  SA_TRY(ks_solve(p, A, E->V, have_init, E->res));
+  E->solved = true;
+  sa_eps_stats &s = E->st;
+  s.matvecs = E->res.matvecs;
+  s.gs_passes = E->res.gs_passes;
+  s.matvecs32 = E->res32.matvecs;
+  s.gs_passes32 = E->res32.gs_passes;
+  s.its = E->res.its;
+  s.its32 = E->res32.its;
+  s.nconv = E->res.nconv;
+  s.reason = E->res.reason;
+  s.seconds = now() - t0;
+  s.seconds32 = E->res32.seconds;
+  s.seconds_ds = E->res.seconds_ds + E->res32.seconds_ds;
+  s.bytes_spmv = E->res.bytes_spmv + E->res32.bytes_spmv;
+  s.bytes_bv = E->res.bytes_bv + E->res32.bytes_bv;
+  s.bytes_restart = E->res.bytes_restart + E->res32.bytes_restart;
+  return 0;
+}
+
+int sa_eps_get_converged(sa_eps E, int *nconv) {
+  SA_CHECK(E && nconv && E->solved, "EPS not solved");
+  *nconv = E->res.nconv;
+  return 0;
+}
+
+int sa_eps_get_iteration_number(sa_eps E, int *its) {
+  SA_CHECK(E && its && E->solved, "EPS not solved");
+  *its = E->res.its;
+  return 0;
+}
+
+int sa_eps_get_converged_reason(sa_eps E, int *reason) {
+  SA_CHECK(E && reason && E->solved, "EPS not solved");
+  *reason = E->res.reason;
+  return 0;
+}
+
+int sa_eps_get_eigenpair(sa_eps E, int i, double *re, double *im, double *xr, double *xi) {
+  API_INIT();
+  SA_CHECK(E && E->solved && i >= 0 && i < E->res.nconv, "eigenpair index out of range");
+  const KsResult &r = E->res;
+  const int k = r.perm[i];
+  if (re) *re = r.eigr[k];
+  if (im) *im = r.eigi[k];
+  // EPSGetEigenvector: real part in column k, imaginary part in column k+1 for a pair
+  if (xr || xi) {
+    if (r.eigi[k] == 0.0) {
+      if (xr) SA_TRY(bv_download_column(E->V, k, xr));
+      if (xi) std::memset(xi, 0, sizeof(double) * E->V.n);
+    } else if (r.eigi[k] > 0) {
+      if (xr) SA_TRY(bv_download_column(E->V, k, xr));
+      if (xi) SA_TRY(bv_download_column(E->V, k + 1, xi));
+    } else {
+      if (xr) SA_TRY(bv_download_column(E->V, k - 1, xr));
+      if (xi) {
+        SA_TRY(bv_download_column(E->V, k, xi));
+        for (int64_t q = 0; q < E->V.n; q++) xi[q] = -xi[q];
+      }
+    }
+  }
+  return 0;
+}
+
+int sa_eps_get_error_estimate(sa_eps E, int i, double *errest) {
+  SA_CHECK(E && E->solved && errest && i >= 0 && i < E->res.nconv, "eigenpair index out of range");
+  *errest = E->res.errest[E->res.perm[i]];
+  return 0;
+}
+
+// EPSComputeError (epssolve.c:742-815): ||A x - lambda x|| / |lambda| on the device in FP64.
+int sa_eps_compute_error(sa_eps E, int i, int type, double *error) {
+  API_INIT();
+  SA_CHECK(E && E->solved && error && i >= 0 && i < E->res.nconv, "eigenpair index out of range");
+  const KsResult &r = E->res;
+  const int k = r.perm[i];
+  BV &V = E->V;
+  const Mat &A = E->A->A;
+  hipStream_t s = g_stream;
+  double *tmp = nullptr, *out = nullptr;
+  SA_HIP(hipMalloc((void **)&tmp, sizeof(double) * std::max<int64_t>(1, V.n) * 2));
+  SA_HIP(hipMalloc((void **)&out, sizeof(double) * 2));
+  const double kr = r.eigr[k], ki = r.eigi[k];
+  double res = 0.0;
+  int rc = 0;
+  if (ki == 0.0) {
+    rc = launch_spmv(A, col_ptr(V, k), tmp, s);
+    if (!rc) rc = launch_axpy_norm(SA_FP64, tmp, col_ptr(V, k), std::fabs(kr) > 2.220446049250313e-16 ? kr : 0.0,
+                                   V.n, V.ws.part, out, s);
+    double h = 0.0;
+    if (!rc && hipMemcpyAsync(&h, out, sizeof(double), hipMemcpyDeviceToHost, s) != hipSuccess) rc = 1;
+    if (!rc && hipStreamSynchronize(s) != hipSuccess) rc = 1;
+    res = std::sqrt(h);
+  } else {
+    // complex pair: columns (xr, xi) = (k, k+1) when ki > 0
+    std::vector<double> xr(V.n), xi(V.n), u(V.n);
+    rc = sa_eps_get_eigenpair(E, i, nullptr, nullptr, xr.data(), xi.data());
+    double kre = kr, kim = std::fabs(ki) * (ki > 0 ? 1 : -1);
+    double *dxr = tmp, *dxi = tmp + V.n;
+    double *dy = nullptr;
+    if (!rc && hipMalloc((void **)&dy, sizeof(double) * V.n) != hipSuccess) rc = 1;
+    if (!rc) rc = sa_memcpy_h2d(dxr, xr.data(), sizeof(double) * V.n);
+    if (!rc) rc = sa_memcpy_h2d(dxi, xi.data(), sizeof(double) * V.n);
+    double nr = 0.0, ni = 0.0;
+    if (!rc) {
+      rc = launch_spmv(A, dxr, dy, s);
+      std::vector<double> y(V.n);
+      if (!rc) rc = sa_memcpy_d2h(y.data(), dy, sizeof(double) * V.n);
+      for (int64_t q = 0; q < V.n; q++) { double t = y[q] - kre * xr[q] + kim * xi[q]; nr += t * t; }
+      if (!rc) rc = launch_spmv(A, dxi, dy, s);
+      if (!rc) rc = sa_memcpy_d2h(y.data(), dy, sizeof(double) * V.n);
+      for (int64_t q = 0; q < V.n; q++) { double t = y[q] - kre * xi[q] - kim * xr[q]; ni += t * t; }
+    }
+    if (dy) (void)hipFree(dy);
+    res = std::hypot(std::sqrt(nr), std::sqrt(ni));
+  }
+  (void)hipFree(tmp);
+  (void)hipFree(out);
+  if (rc) return rc;
+  if (type == SA_ERROR_RELATIVE) res /= std::hypot(kr, ki);
+  *error = res;
+  return 0;
+}
+
+int sa_eps_get_stats(sa_eps E, sa_eps_stats *st) {
+  SA_CHECK(E && st && E->solved, "EPS not solved");
+  *st = E->st;
+  return 0;
+}
+
+}  // extern "C"
+
+// ---------------------------------------------------------------- timing
+namespace {
+hipEvent_t g_ev0 = nullptr, g_ev1 = nullptr;
+}
+extern "C" {
+// HIP events recorded on the library stream around a timed region.
+int sa_timer_start(void) {
+  API_INIT();
+  if (!g_ev0) {
+    SA_HIP(hipEventCreate(&g_ev0));
+    SA_HIP(hipEventCreate(&g_ev1));
+  }
+  SA_HIP(hipEventRecord(g_ev0, g_stream));
+  return 0;
+}
+int sa_timer_stop(double *ms) {
+  API_INIT();
+  SA_CHECK(g_ev0 && ms, "timer not started");
+  SA_HIP(hipEventRecord(g_ev1, g_stream));
+  SA_HIP(hipEventSynchronize(g_ev1));
+  float f = 0.f;
+  SA_HIP(hipEventElapsedTime(&f, g_ev0, g_ev1));
+  *ms = f;
+  return 0;
+}
+}  // extern "C"

@@ -1,0 +1,46 @@
+// Projected dense eigenproblem (the DS object of the reference) on the host in
+// FP64 with LAPACK: DSHEP (compact arrowhead/tridiagonal, steqr) and DSNHEP
+// (Hessenberg, hseqr/trexc/trevc).
+#pragma once
+#include <string>
+#include <vector>
+
+namespace sa {
+
+bool lapack_ready(std::string *path = nullptr);
+
+enum DsState { DS_RAW = 0, DS_INTERMEDIATE = 1, DS_CONDENSED = 2, DS_TRUNCATED = 3 };
+
+// SlepcCompare* (src/sys/slepcsc.c:152-200): >0 when (ar,ai) must follow (br,bi)
+int sc_compare(int which, double ar, double ai, double br, double bi);
+
+class DenseProblem {
+ public:
+  void allocate(int ld, bool hermitian, int which);
+  // DSSetDimensions (dsops.c:130-165)
+  void set_dimensions(int n, int l, int k) { n_ = n; l_ = l; k_ = k; t_ = n; }
+  void set_state(int s) { state_ = s; }
+  int n() const { return n_; }
+  int ld() const { return ld_; }
+  double *d() { return T_.data(); }          // HEP diagonal
+  double *e() { return T_.data() + ld_; }    // HEP off-diagonal / arrow
+  double *A() { return A_.data(); }          // NHEP matrix (ld x ld, extra row at n)
+  const double *Q() const { return Q_.data(); }
+  const double *X() const { return X_.data(); }
+  int solve(double *wr, double *wi);         // DSSolve
+  int sort(double *wr, double *wi);          // DSSort
+  void update_extra_row();                   // DSUpdateExtraRow
+  // Residual weight of Ritz pair i: |last row of the eigenvector| (DSVectors with
+  // index); sets *cplx when i starts a complex pair.
+  int ritz_residual(int i, double *rnorm, int *cplx);
+  int truncate_size(int k, int nv, int l) const;   // DSGetTruncateSize_Default
+  void truncate(int n);                            // DSTruncate(n,FALSE)
+  int trim_and_vectors(int nconv);                 // DSTruncate(n,TRUE) + DSVectors(X) (NHEP)
+ private:
+  int ld_ = 0, n_ = 0, l_ = 0, k_ = 0, t_ = 0, state_ = DS_RAW, which_ = 1;
+  bool herm_ = true;
+  std::vector<double> A_, Q_, X_, T_, work_;
+  std::vector<int> perm_, iwork_;
+};
+
+}  // namespace sa

@@ -1,0 +1,272 @@
+// Krylov-Schur eigensolver driver: the restart loop of
+// EPSSolve_KrylovSchur_Default (src/eps/impls/krylov/krylovschur/krylovschur.c:227-348)
+// over device-resident BV/Mat objects, with the projected problem on the host.
+#include "sa_eps.h"
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstring>
+
+#include "sa_ds.h"
+
+namespace sa {
+
+namespace {
+double now() {
+  return std::chrono::duration<double>(std::chrono::steady_clock::now().time_since_epoch()).count();
+}
+}  // namespace
+
+// Algorithmic HBM bytes of one SpMV: values + column indices + row pointers + x + y.
+double spmv_bytes(const Mat &A) {
+  const double s = (double)prec_size(A.prec);
+  return (double)A.nnz * (s + 4.0) + 4.0 * (A.n + 1) + 2.0 * A.n * s;
+}
+
+// Uploads host vector x (FP64) into column j, converting to the BV precision.
+int bv_upload_column(BV &V, int j, const double *x) {
+  hipStream_t s = stream();
+  if (V.prec == SA_FP64) {
+    SA_HIP(hipMemcpyAsync(col_ptr(V, j), x, sizeof(double) * V.n, hipMemcpyHostToDevice, s));
+  } else {
+    std::vector<float> f(x, x + V.n);
+    SA_HIP(hipMemcpyAsync(col_ptr(V, j), f.data(), sizeof(float) * V.n, hipMemcpyHostToDevice, s));
+    SA_HIP(hipStreamSynchronize(s));
+  }
+  SA_HIP(hipStreamSynchronize(s));
+  return 0;
+}
+
+int bv_download_column(BV &V, int j, double *x) {
+  hipStream_t s = stream();
+  if (V.prec == SA_FP64) {
+    SA_HIP(hipMemcpyAsync(x, col_ptr(V, j), sizeof(double) * V.n, hipMemcpyDeviceToHost, s));
+    SA_HIP(hipStreamSynchronize(s));
+    return 0;
+  }
+  std::vector<float> f(V.n);
+  SA_HIP(hipMemcpyAsync(f.data(), col_ptr(V, j), sizeof(float) * V.n, hipMemcpyDeviceToHost, s));
+  SA_HIP(hipStreamSynchronize(s));
+  for (int64_t i = 0; i < V.n; i++) x[i] = f[i];
+  return 0;
+}
+
+// EPSGetStartVector (epssolve.c:841-873): column i is a deterministic random
+// vector unless an initial vector is already in place (i == 0 && nini > 0).
+static int start_vector(BV &V, int i, bool have_initial, double eta, int *breakdown, std::vector<double> &scratch) {
+  if (i > 0 || !have_initial) {
+    scratch.resize(V.n);
+    sa_rand_uniform(0x5EEDULL + (uint64_t)i, V.n, scratch.data());
+    SA_TRY(bv_upload_column(V, i, scratch.data()));
+  }
+  double norm = 0.0;
+  int lindep = 0;
+  SA_TRY(bv_orthonormalize(V, i, eta, &norm, &lindep));
+  if (breakdown) *breakdown = lindep;
+  else SA_CHECK(!(lindep || norm == 0.0), "Initial vector is zero or belongs to the deflation space");
+  return 0;
+}
+
+int ks_solve(const KsParams &p, const Mat &A, BV &V, bool have_initial, KsResult &r) {
+  SA_CHECK(lapack_ready(), "no LAPACK library found for the projected problem");
+  const int n = A.n, ncv = p.ncv, nev = p.nev, mpd = p.mpd, ld = ncv + 1;
+  SA_CHECK(V.m == ncv + 1 && V.n == n && V.prec == A.prec, "basis does not match the operator/ncv");
+  SA_CHECK(ncv + 1 < GS_MAXC, "ncv too large for the fused Gram-Schmidt kernels");
+  hipStream_t s = stream();
+  DenseProblem ds;
+  ds.allocate(ld, p.hermitian, p.which);
+  std::vector<double> eigr(ld, 0.0), eigi(ld, 0.0), errest(ld, 0.0), buf, scratch;
+  const double t0 = now();
+  double tds = 0.0;
+  r = KsResult();
+  const int64_t passes0 = V.passes_done;
+  const double bsp = spmv_bytes(A), es = (double)prec_size(V.prec);
+
+  if (have_initial) {  // BVInsertVecs(V,0,&k,IS,TRUE) (bvfunc.c:331-374)
+    double norm = 0.0;
+    int lindep = 0;
+    SA_TRY(bv_orthonormalize(V, 0, p.eta, &norm, &lindep));
+    SA_CHECK(!(norm == 0.0 || lindep), "initial vector is linearly dependent");
+    r.bytes_bv += 3.0 * V.n * es;
+  }
+  SA_TRY(start_vector(V, 0, have_initial, p.eta, nullptr, scratch));
+  r.bytes_bv += 3.0 * V.n * es;
+
+  int l = 0, nconv = 0, its = 0, reason = 0;
+  while (reason == 0) {
+    its++;
+    int nv = std::min(nconv + mpd, ncv);
+    const int k0 = nconv + l;
+    ds.set_dimensions(nv, nconv, k0);
+    double beta = 0.0;
+    int breakdown = 0;
+    const int64_t pbefore = V.passes_done;
+    SA_TRY(bv_sweep(V, A, k0, &nv, p.eta, &beta, &breakdown));
+    r.matvecs += nv - k0;
+    // Algorithmic bytes of the sweep: SpMV, then per Gram-Schmidt pass the dot
+    // products read columns 0..c-1 and the update reads them again plus v and
+    // writes v, i.e. (2c+2) n s per pass; passes averaged over the sweep.
+    if (nv > k0) {
+      const double avgp = (double)(V.passes_done - pbefore) / (double)(nv - k0);
+      for (int c = k0 + 1; c <= nv; c++) r.bytes_bv += avgp * (2.0 * c + 2.0) * V.n * es;
+    }
+    r.bytes_spmv += (nv - k0) * bsp;
+    // coefficients of columns k0+1..nv (BVMatArnoldi/BVMatLanczos copy-out, bvkrylov.c:100-109,210-222)
+    const int m = V.m;
+    buf.resize((size_t)m * (nv - k0));
+    SA_HIP(hipMemcpyAsync(buf.data(), V.dbuf + (size_t)(k0 + 1) * m, sizeof(double) * m * (nv - k0),
+                          hipMemcpyDeviceToHost, s));
+    SA_HIP(hipStreamSynchronize(s));
+    const double td0 = now();
+    auto B = [&](int row, int colc) { return buf[(size_t)(colc - k0 - 1) * m + row]; };
+    if (p.hermitian) {
+      double *d = ds.d(), *e = ds.e();
+      for (int j = k0; j < nv; j++) {
+        d[j] = B(j, j + 1);
+        e[j] = B(j + 1, j + 1);
+      }
+    } else {
+      double *H = ds.A();
+      for (int j = k0; j < nv - 1; j++)
+        for (int i = 0; i < j + 2; i++) H[i + (size_t)j * ld] = B(i, j + 1);
+      for (int i = 0; i < nv; i++) H[i + (size_t)(nv - 1) * ld] = B(i, nv);
+      if (ld > nv) H[nv + (size_t)(nv - 1) * ld] = B(nv, nv);
+    }
+    ds.set_dimensions(nv, nconv, k0);
+    ds.set_state(l ? DS_RAW : DS_INTERMEDIATE);
+    int info = ds.solve(eigr.data(), eigi.data());
+    SA_CHECK(info == 0, "LAPACK failure in the projected problem (solve)");
+    info = ds.sort(eigr.data(), eigi.data());
+    SA_CHECK(info == 0, "LAPACK failure in the projected problem (sort)");
+    ds.update_extra_row();
+    // EPSKrylovConvergence (epskrylov.c:207-295) with EPSConvergedRelative (epsdefault.c:224-232)
+    int k = nconv, marker = -1;
+    for (k = nconv; k < nv; k++) {
+      double resnorm = 0.0;
+      int cplx = 0;
+      info = ds.ritz_residual(k, &resnorm, &cplx);
+      SA_CHECK(info == 0, "LAPACK failure in the projected problem (vectors)");
+      resnorm *= beta;
+      const double w = std::hypot(eigr[k], eigi[k]);
+      errest[k] = (w != 0.0) ? resnorm / w : 1.79e308;
+      if (marker == -1 && errest[k] >= p.tol) marker = k;
+      if (cplx) {
+        errest[k + 1] = errest[k];
+        k++;
+      }
+      if (marker != -1) break;
+    }
+    if (marker != -1) k = marker;
+    // EPSStoppingBasic (epsdefault.c:290-302)
+    if (k >= nev) reason = 1;
+    else if (its >= p.max_it) reason = -1;
+    int newl = 0;
+    if (!(reason != 0 || breakdown || k == nv)) {
+      newl = std::max(1, (int)((nv - k) * p.keep));
+      if (!p.hermitian) newl = ds.truncate_size(k, nv, newl);
+    }
+    l = newl;
+    tds += now() - td0;
+    if (reason == 0) {
+      if (breakdown || k == nv) {
+        if (k < nev) {
+          int bd = 0;
+          SA_TRY(start_vector(V, k, false, p.eta, &bd, scratch));
+          if (bd) reason = -2;
+        }
+      } else {
+        ds.truncate(k + l);
+      }
+    }
+    // BVMultInPlace(V,Q,nconv,k+l) over active columns [nconv,nv) (bvops.c:220-245)
+    if (k + l > nconv) {
+      std::memcpy(V.hq, ds.Q(), sizeof(double) * (size_t)ld * ld);
+      SA_HIP(hipMemcpyAsync(V.dq, V.hq, sizeof(double) * (size_t)ld * ld, hipMemcpyHostToDevice, s));
+      SA_TRY(launch_multinplace(V, nconv, nv, nconv, k + l, V.dq, ld, s));
+      r.bytes_restart += ((double)(nv - nconv) + (double)(k + l - nconv)) * V.n * es;
+    }
+    if (reason == 0 && !breakdown) {  // BVCopyColumn(V,nv,k+l)
+      SA_HIP(hipMemcpyAsync(col_ptr(V, k + l), col_ptr(V, nv), es * V.n, hipMemcpyDeviceToDevice, s));
+      r.bytes_bv += 2.0 * V.n * es;
+    }
+    nconv = k;
+    if (p.monitor) p.monitor(p.monitor_ctx, its, nconv, eigr.data(), errest.data(), nv);
+  }
+  // DSTruncate(nconv,TRUE); EPSComputeVectors (Schur: V <- V X)
+  const double td1 = now();
+  int info = ds.trim_and_vectors(nconv);
+  SA_CHECK(info == 0, "LAPACK failure computing eigenvectors of the projected problem");
+  tds += now() - td1;
+  if (!p.hermitian && nconv > 0) {
+    // the last restart may still be reading the pinned staging buffer
+    SA_HIP(hipStreamSynchronize(s));
+    std::memcpy(V.hq, ds.X(), sizeof(double) * (size_t)ld * ld);
+    SA_HIP(hipMemcpyAsync(V.dq, V.hq, sizeof(double) * (size_t)ld * ld, hipMemcpyHostToDevice, s));
+    SA_TRY(launch_multinplace(V, 0, nconv, 0, nconv, V.dq, ld, s));
+  }
+  // reorder conjugate pairs, positive imaginary part first (epssolve.c:159-176)
+  for (int i = 0; i < nconv - 1; i++) {
+    if (eigi[i] != 0) {
+      if (eigi[i] < 0) {
+        eigi[i] = -eigi[i];
+        eigi[i + 1] = -eigi[i + 1];
+        SA_TRY(launch_scale(V.prec, col_ptr(V, i + 1), V.n, -1.0, s));
+      }
+      i++;
+    }
+  }
+  // SlepcSortEigenvalues (slepcsc.c:89-140)
+  std::vector<int> perm(nconv);
+  for (int i = 0; i < nconv; i++) perm[i] = i;
+  for (int i = nconv - 1; i >= 0; i--) {
+    const double re = eigr[perm[i]];
+    double im = eigi[perm[i]];
+    int j = i + 1;
+    if (im != 0) {
+      i--;
+      im = eigi[perm[i]];
+    }
+    while (j < nconv) {
+      if (sc_compare(p.which, re, im, eigr[perm[j]], eigi[perm[j]]) <= 0) break;
+      if (!im) {
+        if (eigi[perm[j]] == 0.0) {
+          std::swap(perm[j - 1], perm[j]);
+          j++;
+        } else {
+          const int tmp = perm[j - 1];
+          perm[j - 1] = perm[j];
+          perm[j] = perm[j + 1];
+          perm[j + 1] = tmp;
+          j += 2;
+        }
+      } else {
+        if (eigi[perm[j]] == 0.0) {
+          const int tmp = perm[j - 2];
+          perm[j - 2] = perm[j];
+          perm[j] = perm[j - 1];
+          perm[j - 1] = tmp;
+          j++;
+        } else {
+          std::swap(perm[j - 2], perm[j]);
+          std::swap(perm[j - 1], perm[j + 1]);
+          j += 2;
+        }
+      }
+    }
+  }
+  SA_HIP(hipStreamSynchronize(s));
+  r.seconds = now() - t0;
+  r.seconds_ds = tds;
+  r.nconv = nconv;
+  r.its = its;
+  r.reason = reason;
+  r.gs_passes = V.passes_done - passes0;
+  r.perm = perm;
+  r.eigr.assign(eigr.begin(), eigr.begin() + nconv);
+  r.eigi.assign(eigi.begin(), eigi.begin() + nconv);
+  r.errest.assign(errest.begin(), errest.begin() + nconv);
+  return 0;
+}
+
+}  // namespace sa

@@ -1,0 +1,34 @@
+// Krylov-Schur driver interface shared by the C API.
+#pragma once
+#include <vector>
+
+#include "sa_internal.h"
+
+namespace sa {
+
+struct KsParams {
+  bool hermitian = true;
+  int which = SA_LARGEST_MAGNITUDE;
+  int nev = 1, ncv = 16, mpd = 16, max_it = 100;
+  double tol = 1e-8, keep = 0.5, eta = 0.7071;
+  void (*monitor)(void *, int, int, const double *, const double *, int) = nullptr;
+  void *monitor_ctx = nullptr;
+};
+
+struct KsResult {
+  int nconv = 0, its = 0, reason = 0;
+  int64_t matvecs = 0, gs_passes = 0;
+  double seconds = 0.0, seconds_ds = 0.0;
+  double bytes_spmv = 0.0, bytes_bv = 0.0, bytes_restart = 0.0;
+  std::vector<int> perm;           // sorted position -> column of V
+  std::vector<double> eigr, eigi, errest;
+};
+
+double spmv_bytes(const Mat &A);
+int bv_upload_column(BV &V, int j, const double *x);
+int bv_download_column(BV &V, int j, double *x);
+// Runs Krylov-Schur on A with basis V (V.m == ncv+1).  If have_initial, column
+// 0 of V holds the initial vector (EPSSetInitialSpace).
+int ks_solve(const KsParams &p, const Mat &A, BV &V, bool have_initial, KsResult &r);
+
+}  // namespace sa

@@ -1,0 +1,117 @@
+// Internal declarations shared by the HIP kernels and the host driver.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <cstdint>
+#include <string>
+#include <vector>
+
+#include "../../include/slepc_amd.h"
+
+namespace sa {
+
+// ---- errors ---------------------------------------------------------------
+void set_error(const std::string &msg);
+#define SA_HIP(call)                                                              \
+  do {                                                                            \
+    hipError_t e_ = (call);                                                       \
+    if (e_ != hipSuccess) {                                                       \
+      ::sa::set_error(std::string(#call) + ": " + hipGetErrorString(e_));         \
+      return 1;                                                                   \
+    }                                                                             \
+  } while (0)
+#define SA_CHECK(cond, msg)                                                       \
+  do {                                                                            \
+    if (!(cond)) { ::sa::set_error(msg); return 2; }                              \
+  } while (0)
+#define SA_TRY(call)                                                              \
+  do { int r_ = (call); if (r_) return r_; } while (0)
+
+hipStream_t stream();
+int ensure_init();
+int num_cus();
+
+// ---- Gram-Schmidt step machine ---------------------------------------------
+// Column c is orthogonalised against columns 0..c-1 by up to five kernels
+// (slots 1..5).  Kernel slot i reads mode[i]; the last workgroup of a kernel
+// that reduces dot products writes the decision for slot i+1.
+constexpr int GS_MAXC = 128;        // largest column index + 1 handled
+constexpr int GS_SLOTS = 7;
+enum GsMode : int { GS_IDLE = 0, GS_MULT_SCALE = 1, GS_MULT_DOTS = 2, GS_SCALE_ONLY = 3 };
+
+struct GsState {
+  int mode[GS_SLOTS];
+  int pass;       // Gram-Schmidt passes done for the current column
+  int pending;    // estimate was <= 0: next reduction evaluates the true norm
+  int lindep;     // final linear-dependence flag of the current column
+  int stop;       // breakdown happened: later steps of the sweep are no-ops
+  int stop_col;
+  int passes_total;
+  double onrm, nrm, beta;
+  double scale[GS_SLOTS];
+  double coef[GS_SLOTS][GS_MAXC];
+};
+
+struct Workspace {           // device scratch for partial sums of one BV
+  double *part = nullptr;    // [maxG][GS_MAXC+1]
+  double *gpart = nullptr;   // [maxGroups][GS_MAXC+1]
+  unsigned *cnt = nullptr;   // [1 + maxGroups]
+  int maxG = 0;
+};
+
+struct BV {
+  int64_t n = 0, ld = 0;
+  int m = 0, prec = SA_FP64;
+  int l = 0, k = 0;
+  void *data = nullptr;        // T[ld * m]
+  double *dbuf = nullptr;      // coefficient buffer m x m (FP64) on device
+  GsState *st = nullptr;       // device
+  GsState *st_host = nullptr;  // pinned host mirror
+  Workspace ws;
+  double *dq = nullptr;        // device staging for Q / q vectors (m*m doubles)
+  double *hq = nullptr;        // pinned host staging
+  int G = 0, rows_per_block = 0;
+  int64_t passes_done = 0;     // host-side counter of orthogonalisation passes
+};
+
+struct Mat {
+  int n = 0;
+  int64_t nnz = 0;
+  int prec = SA_FP64;
+  int32_t *rowptr = nullptr, *col = nullptr;
+  void *val = nullptr;
+  int lanes = 8;               // lanes per row of the vector-CSR kernels
+};
+
+// ---- kernel launchers (sa_kernels.hip) -------------------------------------
+int launch_spmv(const Mat &A, const void *x, void *y, hipStream_t s);
+// One Arnoldi/orthonormalisation step for column c of V; spmv_from >= 0 first
+// computes V(:,c) = A V(:,spmv_from).  eta: DGKS threshold.
+int launch_gs_column(BV &V, const Mat *A, int c, double eta, hipStream_t s);
+int launch_dotvec(BV &V, int l, int k, const void *x, double *q_dev, hipStream_t s);
+int launch_multvec(BV &V, int l, int k, double alpha, const double *q_dev, double beta, void *y,
+                   hipStream_t s);
+int launch_multinplace(BV &V, int l, int k, int s_, int e, const double *Q_dev, int ldq,
+                       hipStream_t s);
+int launch_scale(int prec, void *x, int64_t n, double alpha, hipStream_t s);
+int launch_dot_self(int prec, const void *x, int64_t n, double *out_dev, double *part, hipStream_t s);
+int launch_convert(int src_prec, const void *src, int64_t src_ld, int dst_prec, void *dst,
+                   int64_t dst_ld, int64_t n, int ncols, hipStream_t s);
+int launch_axpy_norm(int prec, const void *Ax, const void *x, double lambda, int64_t n,
+                     double *part, double *out_dev, hipStream_t s);
+int launch_fill_random(int prec, void *x, int64_t n, uint64_t seed, hipStream_t s);
+
+// ---- generators (sa_problems.cpp) ------------------------------------------
+uint64_t splitmix64(uint64_t z);
+double u01(uint64_t seed, uint64_t i);
+
+// ---- BV helpers (sa_bv.cpp) ---------------------------------------------------
+int bv_alloc(BV &V, int64_t n, int m, int prec);
+void bv_free(BV &V);
+int bv_sweep(BV &V, const Mat &A, int k, int *m, double eta, double *beta, int *breakdown);
+int bv_orthonormalize(BV &V, int j, double eta, double *norm, int *lindep);
+inline size_t prec_size(int prec) { return prec == SA_FP64 ? 8 : 4; }
+inline char *col_ptr(const BV &V, int j) {
+  return static_cast<char *>(V.data) + (size_t)j * V.ld * prec_size(V.prec);
+}
+
+}  // namespace sa

@@ -1,0 +1,609 @@
+// HIP kernels for gfx950 (MI355X): CSR SpMV, the fused Gram-Schmidt step
+// (SpMV + classical Gram-Schmidt + DGKS refinement), BV block kernels and the
+// restart GEMM on MFMA.  Wave size 64; 256-thread workgroups.
+#include "sa_internal.h"
+
+#include <algorithm>
+#include <cmath>
+
+namespace sa {
+
+constexpr int BLK = 256;
+constexpr int CH = 64;          // rows per chunk (one row per lane of a wave)
+constexpr int LDP = CH + 1;     // padded LDS column stride
+constexpr int GROUP = 32;       // first level of the cross-workgroup reduction
+
+__device__ __forceinline__ double coherent_load(const double *p) {
+  return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ double safe_sqrt(double a) { return a < 0.0 ? 0.0 : sqrt(a); }
+
+// ---------------------------------------------------------------------------
+// Cross-workgroup reduction of per-workgroup partial sums part[b][0..ncol).
+// Two levels: the last workgroup of each group of GROUP sums its group, the
+// last group sums the group totals.  Summation order is fixed (deterministic).
+// Returns true in the single workgroup that holds the totals in out_lds.
+// ---------------------------------------------------------------------------
+__device__ bool reduce_partials(const double *acc_t, int ncol, double *part, double *gpart,
+                                unsigned *cnt, int G, double *out_lds, int *s_flag) {
+  const int t = threadIdx.x, b = blockIdx.x;
+  if (t < ncol) part[(size_t)b * ncol + t] = *acc_t;
+  __threadfence();
+  __syncthreads();
+  const int g = b / GROUP, ng = (G + GROUP - 1) / GROUP, gsz = min(GROUP, G - g * GROUP);
+  if (t == 0) *s_flag = (atomicAdd(&cnt[1 + g], 1u) == (unsigned)(gsz - 1));
+  __syncthreads();
+  if (!*s_flag) return false;
+  __threadfence();
+  if (t < ncol) {
+    double s = 0.0;
+    for (int bb = g * GROUP; bb < g * GROUP + gsz; bb++) s += coherent_load(&part[(size_t)bb * ncol + t]);
+    gpart[(size_t)g * ncol + t] = s;
+  }
+  if (t == 0) cnt[1 + g] = 0u;
+  __threadfence();
+  __syncthreads();
+  if (t == 0) *s_flag = (atomicAdd(&cnt[0], 1u) == (unsigned)(ng - 1));
+  __syncthreads();
+  if (!*s_flag) return false;
+  __threadfence();
+  if (t < ncol) {
+    double s = 0.0;
+    for (int gg = 0; gg < ng; gg++) s += coherent_load(&gpart[(size_t)gg * ncol + t]);
+    out_lds[t] = s;
+  }
+  if (t == 0) cnt[0] = 0u;
+  __syncthreads();
+  return true;
+}
+
+// ---------------------------------------------------------------------------
+// Gram-Schmidt step kernel.
+//   first (slot 1): v = A V(:,c-1) (if rp) or v = V(:,c); dots V(:,0..c)^T v
+//   slot i > 1    : mode[i] decides: v = (v - V(:,0..c-1) coef) * scale
+//                   and/or dots of the updated v for the next pass.
+// The last workgroup evaluates the CGS1/DGKS rules of the reference
+// (bvorthog.c:91-217) and writes the decision for slot i+1.
+// ---------------------------------------------------------------------------
+template <typename T>
+struct GsArgs {
+  int64_t n, ld;
+  T *V;
+  int c, slot, first;
+  const int32_t *rp, *ci;
+  const T *val;
+  GsState *st;
+  double *dbuf;
+  int m;
+  double *part, *gpart;
+  unsigned *cnt;
+  int G, rpb;
+  double eta;
+};
+
+template <typename T, int L>
+__global__ __launch_bounds__(BLK) void gs_kernel(GsArgs<T> a) {
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  const int c = a.c;
+  T *Vs = reinterpret_cast<T *>(smem_raw);   // [c][LDP]
+  T *vs = Vs + (size_t)c * LDP;              // [CH]
+  T *cf = vs + CH;                           // [c]
+  T *rt = cf + c;                            // [BLK]
+  __shared__ double s_dots[GS_MAXC + 1];
+  __shared__ double s_red[BLK];
+  __shared__ int s_mode, s_flag, s_record, s_out;
+  __shared__ double s_scale;
+  const int t = threadIdx.x;
+
+  if (t == 0) {
+    int md;
+    const GsState *st = a.st;
+    if (st->stop && (a.first || c > st->stop_col)) md = GS_IDLE;
+    else md = a.first ? 100 : st->mode[a.slot];
+    s_mode = md;
+    s_scale = (md == GS_MULT_SCALE || md == GS_SCALE_ONLY) ? st->scale[a.slot] : 1.0;
+  }
+  __syncthreads();
+  const int mode = s_mode;
+  if (mode == GS_IDLE) return;
+  const bool do_mult = (mode == GS_MULT_SCALE || mode == GS_MULT_DOTS);
+  const bool do_upd = do_mult || mode == GS_SCALE_ONLY;
+  const bool do_dots = (mode == 100 || mode == GS_MULT_DOTS);
+  const bool need_tile = (do_mult || do_dots) && c > 0;
+  const T scl = (T)s_scale;
+  if (do_mult)
+    for (int i = t; i < c; i += BLK) cf[i] = (T)a.st->coef[a.slot][i];
+
+  double acc = 0.0;
+  const int64_t rb = (int64_t)blockIdx.x * a.rpb;
+  const int64_t re = min(a.n, rb + a.rpb);
+  T *vc = a.V + (size_t)c * a.ld;
+  for (int64_t r0 = rb; r0 < re; r0 += CH) {
+    const int rows = (int)min((int64_t)CH, re - r0);
+    // 1. the vector chunk
+    if (mode == 100 && a.rp) {
+      const T *x = a.V + (size_t)(c - 1) * a.ld;
+      constexpr int RPS = BLK / L;
+      const int g = t / L, lg = t % L;
+      for (int rr = g; rr < CH; rr += RPS) {
+        T s = (T)0;
+        if (rr < rows) {
+          const int64_t row = r0 + rr;
+          const int32_t pe = a.rp[row + 1];
+          for (int32_t p = a.rp[row] + lg; p < pe; p += L) s += a.val[p] * x[a.ci[p]];
+        }
+#pragma unroll
+        for (int off = L / 2; off > 0; off >>= 1) s += __shfl_xor(s, off, L);
+        if (lg == 0) {
+          vs[rr] = (rr < rows) ? s : (T)0;
+          if (rr < rows) vc[r0 + rr] = s;
+        }
+      }
+    } else if (t < CH) {
+      vs[t] = (t < rows) ? vc[r0 + t] : (T)0;
+    }
+    // 2. the tile V(r0:r0+CH, 0:c) into LDS, coalesced along rows
+    if (need_tile) {
+      const int row = t & (CH - 1), cg = t >> 6;
+      const bool in = row < rows;
+      const T *src = a.V + r0 + row;
+      int i = cg;
+      for (; i + 12 < c; i += 16) {
+        T x0 = in ? src[(size_t)i * a.ld] : (T)0;
+        T x1 = in ? src[(size_t)(i + 4) * a.ld] : (T)0;
+        T x2 = in ? src[(size_t)(i + 8) * a.ld] : (T)0;
+        T x3 = in ? src[(size_t)(i + 12) * a.ld] : (T)0;
+        Vs[i * LDP + row] = x0;
+        Vs[(i + 4) * LDP + row] = x1;
+        Vs[(i + 8) * LDP + row] = x2;
+        Vs[(i + 12) * LDP + row] = x3;
+      }
+      for (; i < c; i += 4) Vs[i * LDP + row] = in ? src[(size_t)i * a.ld] : (T)0;
+    }
+    __syncthreads();
+    // 3. update v = (v - V coef) * scale
+    if (do_upd) {
+      const int row = t & (CH - 1), cg = t >> 6;
+      T s = (T)0;
+      if (do_mult)
+        for (int i = cg; i < c; i += 4) s += Vs[i * LDP + row] * cf[i];
+      rt[t] = s;
+      __syncthreads();
+      if (t < CH) {
+        const T tot = (rt[t] + rt[t + 64]) + (rt[t + 128] + rt[t + 192]);
+        const T vn = (vs[t] - tot) * scl;
+        vs[t] = (t < rows) ? vn : (T)0;
+        if (t < rows) vc[r0 + t] = vn;
+      }
+      __syncthreads();
+    }
+    // 4. dot products with the (updated) vector; thread c takes v^T v
+    if (do_dots && t <= c) {
+      const T *colp = (t < c) ? (Vs + t * LDP) : vs;
+      T s = (T)0;
+      for (int r = 0; r < rows; r++) s += colp[r] * vs[r];
+      acc += (double)s;
+    }
+    __syncthreads();
+  }
+  if (!do_dots) return;
+
+  if (!reduce_partials(&acc, c + 1, a.part, a.gpart, a.cnt, a.G, s_dots, &s_flag)) return;
+
+  // ---- decision (single workgroup) ----
+  s_red[t] = (t < c) ? s_dots[t] * s_dots[t] : 0.0;
+  __syncthreads();
+  for (int w = BLK / 2; w > 0; w >>= 1) {
+    if (t < w) s_red[t] += s_red[t + w];
+    __syncthreads();
+  }
+  if (t == 0) {
+    GsState *st = a.st;
+    const int out = a.slot + 1;
+    const double self = s_dots[c];
+    int mode_out = GS_IDLE, record = 0, fin = 0;
+    double scale_out = 1.0, nrm_f = 0.0, onrm_f = 0.0;
+    if (a.first) { st->pass = 0; st->pending = 0; st->lindep = 0; }
+    if (st->pending) {                      // true norm of the previous update
+      const double nrm = safe_sqrt(self);
+      st->pending = 0;
+      if (!(nrm != 0.0 && nrm < a.eta * st->onrm && st->pass < 3)) {
+        fin = 1; nrm_f = nrm; onrm_f = st->onrm; mode_out = GS_SCALE_ONLY;
+      }
+    }
+    if (!fin) {                              // one CGS1 pass (bvorthog.c:91-132)
+      st->pass++;
+      st->passes_total++;
+      record = 1;
+      const double onrm = safe_sqrt(self);
+      const double est = onrm * onrm - s_red[0];
+      if (est <= 0.0) {
+        st->pending = 1; st->onrm = onrm; mode_out = GS_MULT_DOTS;
+      } else {
+        const double nrm = sqrt(est);
+        if (nrm != 0.0 && nrm < a.eta * onrm && st->pass < 3) {
+          st->onrm = onrm; mode_out = GS_MULT_DOTS;
+        } else {
+          fin = 1; nrm_f = nrm; onrm_f = onrm; mode_out = GS_MULT_SCALE;
+        }
+      }
+    }
+    if (fin) {                               // bvorthog.c:175-216 + 417-422
+      const int lind = !(nrm_f != 0.0 && fabs(nrm_f) >= a.eta * fabs(onrm_f));
+      st->lindep = lind;
+      st->nrm = nrm_f;
+      st->beta = nrm_f;
+      a.dbuf[(size_t)c * a.m + c] = lind ? 0.0 : nrm_f;
+      scale_out = (nrm_f != 0.0 && nrm_f != 1.0) ? 1.0 / nrm_f : 1.0;
+      if (lind) { st->stop = 1; st->stop_col = c; }
+    }
+    st->mode[out] = mode_out;
+    st->scale[out] = scale_out;
+    if (a.first)
+      for (int s = out + 1; s < GS_SLOTS; s++) st->mode[s] = GS_IDLE;
+    s_record = record;
+    s_out = out;
+  }
+  __syncthreads();
+  if (s_record && t < c) {
+    double *h = a.dbuf + (size_t)c * a.m;
+    h[t] = (a.first ? 0.0 : h[t]) + s_dots[t];   // BV_CleanCoefficients + BV_AddCoefficients
+    a.st->coef[s_out][t] = s_dots[t];
+  }
+}
+
+__global__ void gs_reset_kernel(GsState *st) {
+  if (threadIdx.x == 0) {
+    st->stop = 0;
+    st->stop_col = 0;
+    st->pending = 0;
+    st->lindep = 0;
+    st->beta = 0.0;
+    for (int s = 0; s < GS_SLOTS; s++) st->mode[s] = GS_IDLE;
+  }
+}
+
+template <typename T>
+static int gs_launch(BV &V, const Mat *A, int c, double eta, hipStream_t s) {
+  GsArgs<T> a;
+  a.n = V.n; a.ld = V.ld; a.V = static_cast<T *>(V.data); a.c = c;
+  a.rp = A ? A->rowptr : nullptr; a.ci = A ? A->col : nullptr;
+  a.val = A ? static_cast<const T *>(A->val) : nullptr;
+  a.st = V.st; a.dbuf = V.dbuf; a.m = V.m;
+  a.part = V.ws.part; a.gpart = V.ws.gpart; a.cnt = V.ws.cnt;
+  a.G = V.G; a.rpb = V.rows_per_block; a.eta = eta;
+  const size_t shm = ((size_t)c * LDP + CH + c + BLK) * sizeof(T);
+  SA_CHECK(shm + 4096 <= 65536, "basis too wide for the LDS tile of the Gram-Schmidt kernel");
+  const int lanes = A ? A->lanes : 8;
+  for (int slot = 1; slot <= 5; slot++) {
+    a.slot = slot;
+    a.first = (slot == 1);
+    if (slot > 1) { a.rp = nullptr; a.ci = nullptr; a.val = nullptr; }
+    switch (lanes) {
+      case 4: hipLaunchKernelGGL((gs_kernel<T, 4>), dim3(V.G), dim3(BLK), shm, s, a); break;
+      case 8: hipLaunchKernelGGL((gs_kernel<T, 8>), dim3(V.G), dim3(BLK), shm, s, a); break;
+      case 16: hipLaunchKernelGGL((gs_kernel<T, 16>), dim3(V.G), dim3(BLK), shm, s, a); break;
+      default: hipLaunchKernelGGL((gs_kernel<T, 32>), dim3(V.G), dim3(BLK), shm, s, a); break;
+    }
+  }
+  SA_HIP(hipGetLastError());
+  return 0;
+}
+
+int launch_gs_column(BV &V, const Mat *A, int c, double eta, hipStream_t s) {
+  SA_CHECK(c >= 0 && c < GS_MAXC && c < V.m, "column index out of range for the Gram-Schmidt kernel");
+  SA_CHECK(!A || (A->prec == V.prec && A->n == V.n), "matrix and basis sizes or precisions differ");
+  SA_CHECK(!A || c > 0, "SpMV step needs a previous column");
+  if (V.prec == SA_FP64) return gs_launch<double>(V, A, c, eta, s);
+  return gs_launch<float>(V, A, c, eta, s);
+}
+
+int launch_gs_reset(BV &V, hipStream_t s) {
+  hipLaunchKernelGGL(gs_reset_kernel, dim3(1), dim3(64), 0, s, V.st);
+  SA_HIP(hipGetLastError());
+  return 0;
+}
+
+// ---------------------------------------------------------------------------
+// Standalone vector-CSR SpMV: L lanes per row, shuffle reduction.
+// ---------------------------------------------------------------------------
+template <typename T, int L>
+__global__ __launch_bounds__(BLK) void spmv_kernel(int n, const int32_t *__restrict__ rp,
+                                                   const int32_t *__restrict__ ci,
+                                                   const T *__restrict__ val,
+                                                   const T *__restrict__ x, T *__restrict__ y) {
+  const int lg = threadIdx.x % L;
+  const int64_t row = ((int64_t)blockIdx.x * BLK + threadIdx.x) / L;
+  T s = (T)0;
+  if (row < n) {
+    const int32_t pe = rp[row + 1];
+    for (int32_t p = rp[row] + lg; p < pe; p += L) s += val[p] * x[ci[p]];
+  }
+#pragma unroll
+  for (int off = L / 2; off > 0; off >>= 1) s += __shfl_xor(s, off, L);
+  if (lg == 0 && row < n) y[row] = s;
+}
+
+template <typename T>
+static int spmv_t(const Mat &A, const T *x, T *y, hipStream_t s) {
+  const int64_t threads = (int64_t)A.n * A.lanes;
+  const int grid = (int)((threads + BLK - 1) / BLK);
+  switch (A.lanes) {
+    case 4: hipLaunchKernelGGL((spmv_kernel<T, 4>), dim3(grid), dim3(BLK), 0, s, A.n, A.rowptr, A.col, (const T *)A.val, x, y); break;
+    case 8: hipLaunchKernelGGL((spmv_kernel<T, 8>), dim3(grid), dim3(BLK), 0, s, A.n, A.rowptr, A.col, (const T *)A.val, x, y); break;
+    case 16: hipLaunchKernelGGL((spmv_kernel<T, 16>), dim3(grid), dim3(BLK), 0, s, A.n, A.rowptr, A.col, (const T *)A.val, x, y); break;
+    default: hipLaunchKernelGGL((spmv_kernel<T, 32>), dim3(grid), dim3(BLK), 0, s, A.n, A.rowptr, A.col, (const T *)A.val, x, y); break;
+  }
+  SA_HIP(hipGetLastError());
+  return 0;
+}
+
+int launch_spmv(const Mat &A, const void *x, void *y, hipStream_t s) {
+  if (A.n == 0) return 0;
+  if (A.prec == SA_FP64) return spmv_t<double>(A, (const double *)x, (double *)y, s);
+  return spmv_t<float>(A, (const float *)x, (float *)y, s);
+}
+
+// ---------------------------------------------------------------------------
+// General BVDotVec: q = V(:,l:k)^T x  (partials per workgroup, then reduce).
+// ---------------------------------------------------------------------------
+template <typename T>
+__global__ __launch_bounds__(BLK) void dotvec_kernel(int64_t n, int64_t ld, const T *__restrict__ V, int ncol,
+                                                     const T *__restrict__ x, int rpb, double *part,
+                                                     double *gpart, unsigned *cnt, int G, double *q_out) {
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  T *Vs = reinterpret_cast<T *>(smem_raw);
+  T *xs = Vs + (size_t)ncol * LDP;
+  __shared__ double s_tot[GS_MAXC + 1];
+  __shared__ int s_flag;
+  const int t = threadIdx.x;
+  double acc = 0.0;
+  const int64_t rb = (int64_t)blockIdx.x * rpb, re = min(n, rb + rpb);
+  for (int64_t r0 = rb; r0 < re; r0 += CH) {
+    const int rows = (int)min((int64_t)CH, re - r0);
+    const int row = t & (CH - 1), cg = t >> 6;
+    if (t < CH) xs[t] = (t < rows) ? x[r0 + t] : (T)0;
+    for (int i = cg; i < ncol; i += 4) Vs[i * LDP + row] = (row < rows) ? V[(size_t)i * ld + r0 + row] : (T)0;
+    __syncthreads();
+    if (t < ncol) {
+      T s = (T)0;
+      for (int r = 0; r < rows; r++) s += Vs[t * LDP + r] * xs[r];
+      acc += (double)s;
+    }
+    __syncthreads();
+  }
+  if (!reduce_partials(&acc, ncol, part, gpart, cnt, G, s_tot, &s_flag)) return;
+  if (t < ncol) q_out[t] = s_tot[t];
+}
+
+int launch_dotvec(BV &V, int l, int k, const void *x, double *q_dev, hipStream_t s) {
+  const int ncol = k - l;
+  if (ncol <= 0) return 0;
+  SA_CHECK(ncol <= GS_MAXC, "too many active columns for BVDotVec");
+  const size_t es = prec_size(V.prec);
+  const size_t shm = ((size_t)ncol * LDP + CH) * es;
+  if (V.prec == SA_FP64)
+    hipLaunchKernelGGL(dotvec_kernel<double>, dim3(V.G), dim3(BLK), shm, s, V.n, V.ld,
+                       (const double *)col_ptr(V, l), ncol, (const double *)x, V.rows_per_block,
+                       V.ws.part, V.ws.gpart, V.ws.cnt, V.G, q_dev);
+  else
+    hipLaunchKernelGGL(dotvec_kernel<float>, dim3(V.G), dim3(BLK), shm, s, V.n, V.ld,
+                       (const float *)col_ptr(V, l), ncol, (const float *)x, V.rows_per_block,
+                       V.ws.part, V.ws.gpart, V.ws.cnt, V.G, q_dev);
+  SA_HIP(hipGetLastError());
+  return 0;
+}
+
+// y = beta*y + alpha*V(:,0:ncol) q
+template <typename T>
+__global__ __launch_bounds__(BLK) void multvec_kernel(int64_t n, int64_t ld, const T *__restrict__ V, int ncol,
+                                                      const double *__restrict__ q, double alpha,
+                                                      double beta, T *__restrict__ y) {
+  __shared__ T qs[GS_MAXC];
+  for (int i = threadIdx.x; i < ncol; i += BLK) qs[i] = (T)q[i];
+  __syncthreads();
+  const int64_t r = (int64_t)blockIdx.x * BLK + threadIdx.x;
+  if (r >= n) return;
+  T s = (T)0;
+  for (int i = 0; i < ncol; i++) s += V[(size_t)i * ld + r] * qs[i];
+  y[r] = (T)beta * y[r] + (T)alpha * s;
+}
+
+int launch_multvec(BV &V, int l, int k, double alpha, const double *q_dev, double beta, void *y,
+                   hipStream_t s) {
+  const int ncol = std::max(0, k - l);
+  SA_CHECK(ncol <= GS_MAXC, "too many active columns for BVMultVec");
+  const int grid = (int)((V.n + BLK - 1) / BLK);
+  if (V.n == 0) return 0;
+  if (V.prec == SA_FP64)
+    hipLaunchKernelGGL(multvec_kernel<double>, dim3(grid), dim3(BLK), 0, s, V.n, V.ld,
+                       (const double *)col_ptr(V, l), ncol, q_dev, alpha, beta, (double *)y);
+  else
+    hipLaunchKernelGGL(multvec_kernel<float>, dim3(grid), dim3(BLK), 0, s, V.n, V.ld,
+                       (const float *)col_ptr(V, l), ncol, q_dev, alpha, beta, (float *)y);
+  SA_HIP(hipGetLastError());
+  return 0;
+}
+
+// ---------------------------------------------------------------------------
+// Restart GEMM on MFMA: V(:,s:e) = V(:,l:k) * Q(l:k,s:e), in place.
+// A workgroup owns 64 rows: it stages V(rows, l:k) in LDS, then each wave
+// computes a 16-row slab of the output with 16x16x4 MFMA tiles.
+// ---------------------------------------------------------------------------
+constexpr int MIP_NT = 8;   // up to 8 output tiles of 16 columns (128 columns)
+typedef double dbl4 __attribute__((ext_vector_type(4)));
+typedef float flt4 __attribute__((ext_vector_type(4)));
+
+__device__ __forceinline__ dbl4 mfma16x16x4(double a, double b, dbl4 c) {
+  return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
+}
+__device__ __forceinline__ flt4 mfma16x16x4(float a, float b, flt4 c) {
+  return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
+}
+template <typename T> struct Vec4;
+template <> struct Vec4<double> { typedef dbl4 type; };
+template <> struct Vec4<float> { typedef flt4 type; };
+
+template <typename T>
+__global__ __launch_bounds__(BLK) void multinplace_kernel(int64_t n, int64_t ld, T *V, int l, int K,
+                                                          int s, int N, const double *__restrict__ Q,
+                                                          int ldq) {
+  typedef typename Vec4<T>::type V4;
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  T *As = reinterpret_cast<T *>(smem_raw);       // [K][LDP]
+  const int t = threadIdx.x, lane = t & 63, w = t >> 6;
+  const int64_t r0 = (int64_t)blockIdx.x * CH;
+  const int rows = (int)min((int64_t)CH, n - r0);
+  {
+    const int row = t & (CH - 1), cg = t >> 6;
+    for (int i = cg; i < K; i += 4)
+      As[i * LDP + row] = (row < rows) ? V[(size_t)(l + i) * ld + r0 + row] : (T)0;
+  }
+  __syncthreads();
+  const int ntiles = (N + 15) / 16;
+  V4 acc[MIP_NT];
+#pragma unroll
+  for (int q = 0; q < MIP_NT; q++) acc[q] = (V4){0, 0, 0, 0};
+  const int ai = w * 16 + (lane & 15);     // A row within the block
+  const int kq = lane >> 4;                // k index within the 4-wide step
+  const int bj = lane & 15;                // B column within the tile
+  for (int k0 = 0; k0 < K; k0 += 4) {
+    const int kk = k0 + kq;
+    const T av = (kk < K) ? As[kk * LDP + ai] : (T)0;
+#pragma unroll
+    for (int q = 0; q < MIP_NT; q++) {
+      if (q < ntiles) {
+        const int j = q * 16 + bj;
+        const T bv = (kk < K && j < N) ? (T)Q[(size_t)(l + kk) + (size_t)(s + j) * ldq] : (T)0;
+        acc[q] = mfma16x16x4(av, bv, acc[q]);
+      }
+    }
+  }
+  // Output layout: column j = lane % 16.  f32 16x16x4: element v of lane
+  // group g = lane/16 is row 4g + v; f64 16x16x4: row g + 4v.
+  constexpr bool kF64 = sizeof(T) == 8;
+#pragma unroll
+  for (int q = 0; q < MIP_NT; q++) {
+    if (q < ntiles) {
+      const int j = q * 16 + (lane & 15);
+#pragma unroll
+      for (int v = 0; v < 4; v++) {
+        const int i = w * 16 + (kF64 ? ((lane >> 4) + 4 * v) : (4 * (lane >> 4) + v));
+        if (j < N && i < rows) V[(size_t)(s + j) * ld + r0 + i] = acc[q][v];
+      }
+    }
+  }
+}
+
+int launch_multinplace(BV &V, int l, int k, int s_, int e, const double *Q_dev, int ldq, hipStream_t s) {
+  const int K = k - l, N = e - s_;
+  if (K <= 0 || N <= 0 || V.n == 0) return 0;
+  SA_CHECK(N <= MIP_NT * 16 && K <= GS_MAXC, "BVMultInPlace block too large");
+  const int grid = (int)((V.n + CH - 1) / CH);
+  const size_t shm = (size_t)K * LDP * prec_size(V.prec);
+  if (V.prec == SA_FP64)
+    hipLaunchKernelGGL(multinplace_kernel<double>, dim3(grid), dim3(BLK), shm, s, V.n, V.ld, (double *)V.data,
+                       l, K, s_, N, Q_dev, ldq);
+  else
+    hipLaunchKernelGGL(multinplace_kernel<float>, dim3(grid), dim3(BLK), shm, s, V.n, V.ld, (float *)V.data,
+                       l, K, s_, N, Q_dev, ldq);
+  SA_HIP(hipGetLastError());
+  return 0;
+}
+
+// ---------------------------------------------------------------------------
+// Small elementwise / reduction kernels
+// ---------------------------------------------------------------------------
+template <typename T>
+__global__ void scale_kernel(T *x, int64_t n, T alpha) {
+  const int64_t i = (int64_t)blockIdx.x * BLK + threadIdx.x;
+  if (i < n) x[i] *= alpha;
+}
+
+int launch_scale(int prec, void *x, int64_t n, double alpha, hipStream_t s) {
+  if (n == 0) return 0;
+  const int grid = (int)((n + BLK - 1) / BLK);
+  if (prec == SA_FP64) hipLaunchKernelGGL(scale_kernel<double>, dim3(grid), dim3(BLK), 0, s, (double *)x, n, alpha);
+  else hipLaunchKernelGGL(scale_kernel<float>, dim3(grid), dim3(BLK), 0, s, (float *)x, n, (float)alpha);
+  SA_HIP(hipGetLastError());
+  return 0;
+}
+
+// Sum of squares: block partials then a single-block finish (FP64 totals).
+template <typename T>
+__global__ __launch_bounds__(BLK) void sumsq_partial_kernel(const T *x, const T *y, double lambda, int64_t n,
+                                                            double *part) {
+  __shared__ double red[BLK];
+  double s = 0.0;
+  for (int64_t i = (int64_t)blockIdx.x * BLK + threadIdx.x; i < n; i += (int64_t)gridDim.x * BLK) {
+    double v = y ? (double)x[i] - lambda * (double)y[i] : (double)x[i];
+    s += v * v;
+  }
+  red[threadIdx.x] = s;
+  __syncthreads();
+  for (int w = BLK / 2; w > 0; w >>= 1) {
+    if (threadIdx.x < w) red[threadIdx.x] += red[threadIdx.x + w];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) part[blockIdx.x] = red[0];
+}
+__global__ void sum_finish_kernel(const double *part, int np, double *out) {
+  __shared__ double red[BLK];
+  double s = 0.0;
+  for (int i = threadIdx.x; i < np; i += BLK) s += part[i];
+  red[threadIdx.x] = s;
+  __syncthreads();
+  for (int w = BLK / 2; w > 0; w >>= 1) {
+    if (threadIdx.x < w) red[threadIdx.x] += red[threadIdx.x + w];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) *out = red[0];
+}
+
+static int sumsq(int prec, const void *x, const void *y, double lambda, int64_t n, double *part, double *out,
+                 hipStream_t s) {
+  const int grid = (int)std::max<int64_t>(1, std::min<int64_t>(1024, (n + BLK - 1) / BLK));
+  if (prec == SA_FP64)
+    hipLaunchKernelGGL(sumsq_partial_kernel<double>, dim3(grid), dim3(BLK), 0, s, (const double *)x,
+                       (const double *)y, lambda, n, part);
+  else
+    hipLaunchKernelGGL(sumsq_partial_kernel<float>, dim3(grid), dim3(BLK), 0, s, (const float *)x,
+                       (const float *)y, lambda, n, part);
+  hipLaunchKernelGGL(sum_finish_kernel, dim3(1), dim3(BLK), 0, s, part, grid, out);
+  SA_HIP(hipGetLastError());
+  return 0;
+}
+
+int launch_dot_self(int prec, const void *x, int64_t n, double *out_dev, double *part, hipStream_t s) {
+  return sumsq(prec, x, nullptr, 0.0, n, part, out_dev, s);
+}
+
+int launch_axpy_norm(int prec, const void *Ax, const void *x, double lambda, int64_t n, double *part,
+                     double *out_dev, hipStream_t s) {
+  return sumsq(prec, Ax, x, lambda, n, part, out_dev, s);
+}
+
+template <typename S, typename D>
+__global__ void convert_kernel(const S *src, int64_t sld, D *dst, int64_t dld, int64_t n) {
+  const int64_t i = (int64_t)blockIdx.x * BLK + threadIdx.x;
+  const int j = blockIdx.y;
+  if (i < n) dst[(size_t)j * dld + i] = (D)src[(size_t)j * sld + i];
+}
+
+int launch_convert(int sp, const void *src, int64_t sld, int dp, void *dst, int64_t dld, int64_t n, int ncols,
+                   hipStream_t s) {
+  if (n == 0 || ncols == 0) return 0;
+  dim3 grid((unsigned)((n + BLK - 1) / BLK), (unsigned)ncols);
+  if (sp == SA_FP64 && dp == SA_FP32)
+    hipLaunchKernelGGL((convert_kernel<double, float>), grid, dim3(BLK), 0, s, (const double *)src, sld, (float *)dst, dld, n);
+  else if (sp == SA_FP32 && dp == SA_FP64)
+    hipLaunchKernelGGL((convert_kernel<float, double>), grid, dim3(BLK), 0, s, (const float *)src, sld, (double *)dst, dld, n);
+  else if (sp == SA_FP64)
+    hipLaunchKernelGGL((convert_kernel<double, double>), grid, dim3(BLK), 0, s, (const double *)src, sld, (double *)dst, dld, n);
+  else
+    hipLaunchKernelGGL((convert_kernel<float, float>), grid, dim3(BLK), 0, s, (const float *)src, sld, (float *)dst, dld, n);
+  SA_HIP(hipGetLastError());
+  return 0;
+}
+
+}  // namespace sa

@@ -1,0 +1,302 @@
+"""GPU parity tests: HIP SpMV / BV kernels / Krylov-Schur against the CPU oracle
+and the reference's golden outputs.  All calls go through the C ABI."""
+import json
+import os
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+GOLD = json.load(open(os.path.join(os.path.dirname(__file__), "golden", "reference_outputs.json")))
+LIBRES = json.load(open(os.path.join(os.path.dirname(__file__), "golden", "library_results.json")))
+
+
+def _rel(a, b):
+    a, b = np.asarray(a, float), np.asarray(b, float)
+    return np.max(np.abs(a - b) / np.maximum(np.abs(b), 1e-300))
+
+
+# ------------------------------------------------------------------ SpMV
+@pytest.mark.parametrize("gen", ["lap2d", "convdiff3d", "stencil27", "markov"])
+def test_spmv_fp64_matches_oracle(sa, orc, gen):
+    A = {"lap2d": lambda m: m.lap2d(37, 23), "convdiff3d": lambda m: m.convdiff3d(11, 9, 7),
+         "stencil27": lambda m: m.stencil27(9, 8, 7), "markov": lambda m: m.markov(20)}[gen]
+    Ah, Ao = A(sa), A(orc)
+    M = sa.Mat(Ah, sa.FP64)
+    # integer-valued x on an integer stencil: products and sums are exact
+    xi = np.arange(Ah.n, dtype=np.float64) % 7 - 3
+    y = M.mult_host(xi)
+    yo = Ao.matvec(xi)
+    if gen == "lap2d":
+        assert np.array_equal(y, yo)
+    x = orc.rand_uniform(3, Ah.n)
+    y, yo = M.mult_host(x), Ao.matvec(x)
+    # one rounding per product and per addition: |err| <= nnz_row * eps * sum|a||x|
+    absA = np.abs(Ao.to_dense()) if Ah.n <= 2000 else None
+    bound = 32 * 2.2e-16 * (absA @ np.abs(x) if absA is not None else np.abs(yo).max() + 1)
+    assert np.all(np.abs(y - yo) <= bound + 1e-300)
+
+
+def test_spmv_fp32(sa, orc):
+    Ah = sa.stencil27(16, 15, 14, seed=1)
+    Ao = orc.stencil27(16, 15, 14, seed=1)
+    M = sa.Mat(Ah, sa.FP32)
+    x = orc.rand_uniform(4, Ah.n).astype(np.float32)
+    y = M.mult_host(x).astype(np.float64)
+    yo = Ao.matvec(x.astype(np.float64))
+    assert np.max(np.abs(y - yo)) <= 27 * 6e-8 * 60
+
+
+# ------------------------------------------------------------------ BV golden (test1.c)
+def test_bv_test1_golden(sa):
+    g = GOLD["bv_test1"]
+    X0 = np.array(g["X0"])
+    Ymult = np.array(g["Y_after_mult"])
+    for prec in (sa.FP64, sa.FP32):
+        X = sa.BV(10, 5, prec)
+        X.set_matrix(X0)
+        # BVMultVec(X,-1,1,Y[0],z), z = 2, -1, .5, -.25, .125 (test1.c:104-110)
+        z = np.array([2.0, -1.0, 0.5, -0.25, 0.125])
+        Y = sa.BV(10, 1, prec)
+        Y.set_column(0, Ymult[:, 0])
+        X.multvec(-1.0, 1.0, Y.column_ptr(0), z)
+        y0 = Y.get_column(0)
+        assert np.array_equal(y0, np.array(g["Y_after_multvec"])[:, 0])
+        # BVDotVec(X, Y[0]) (test1.c:130-134)
+        q = X.dotvec(Y.column_ptr(0))
+        assert np.array_equal(q, np.array(g["z_after_dotvec"]))
+        # BVMultInPlace(X,Q,1,l=3) then BVScale(X,2) (test1.c:146-147)
+        Q = np.array(g["Q"])
+        X.multinplace(Q, 1, 3)
+        X.scale_column(-1, 2.0)
+        assert np.array_equal(X.get_matrix(5), np.array(g["X_after_multinplace_scale"]))
+        assert abs(X.norm_column(0) - g["norm2_X0"]) < 5e-5
+        assert abs(X.norm() - g["normF_X"]) < 5e-5
+
+
+def test_bv_orthogonality_level(sa):
+    """test2.c criterion: CGS with refinement gives ||X^T X - I||_1 < 100 eps."""
+    n, k = 20, 8
+    rng = np.random.default_rng(7)
+    X0 = rng.standard_normal((n, k))
+    X = sa.BV(n, k, sa.FP64)
+    X.set_matrix(X0)
+    for j in range(k):
+        X.orthonormalize_column(j)
+    Q = X.get_matrix(k)
+    assert np.linalg.norm(Q.T @ Q - np.eye(k), 1) < 100 * 2.2e-16
+    Qo = np.linalg.qr(X0)[0]
+    assert np.linalg.norm(np.abs(Q.T @ Qo) - np.eye(k)) < 1e-12
+
+
+# ------------------------------------------------------------------ Gram-Schmidt vs oracle
+@pytest.mark.parametrize("n", [1000, 70001])
+def test_orthonormalize_matches_oracle(sa, orc, n):
+    """Well-conditioned columns: GPU CGS/DGKS equals the oracle to rounding."""
+    k = 12
+    rng = np.random.default_rng(11)
+    X0 = rng.standard_normal((n, k))
+    V = sa.BV(n, k, sa.FP64)
+    V.set_matrix(X0)
+    Vo = X0.copy(order="F")
+    for j in range(k):
+        nrm, lind = V.orthonormalize_column(j)
+        Vo, nrmo, lindo, ho = orc.bv_orthonormalize_column(Vo, j)
+        assert lind == bool(lindo)
+        assert abs(nrm - nrmo) <= 1e-12 * abs(nrmo)
+        h = V.coefficients(j, j + 1)[:j + 1, 0]
+        assert np.max(np.abs(h - ho)) <= 1e-12 * np.abs(ho).max()
+    Q = V.get_matrix(k)
+    assert np.max(np.abs(Q - Vo)) < 1e-12
+    assert np.linalg.norm(Q.T @ Q - np.eye(k), 1) < 100 * 2.2e-16
+
+
+def test_orthonormalize_dgks_refinement(sa, orc):
+    """A nearly dependent column triggers the DGKS second pass; the result is
+    orthogonal to working precision and lindep agrees with the oracle."""
+    n, k = 5000, 8
+    rng = np.random.default_rng(5)
+    X0 = rng.standard_normal((n, k))
+    X0[:, 5] = X0[:, 4] + 1e-9 * X0[:, 5]
+    X0[:, 7] = X0[:, 0] * 2.0                    # exactly dependent: lindep
+    V = sa.BV(n, k, sa.FP64)
+    V.set_matrix(X0)
+    Vo = X0.copy(order="F")
+    scale = np.linalg.norm(X0, axis=0)
+    for j in range(k):
+        nrm, lind = V.orthonormalize_column(j)
+        Vo, nrmo, lindo, _ = orc.bv_orthonormalize_column(Vo, j)
+        assert lind == bool(lindo), j
+        if j == 7:      # exact dependence: only rounding noise is left
+            assert nrm < 1e-13 * scale[j] and nrmo < 1e-13 * scale[j]
+        elif not lind:
+            assert abs(nrm - nrmo) <= 1e-6 * abs(nrmo)
+    Q = V.get_matrix(7)
+    assert np.linalg.norm(Q.T @ Q - np.eye(7), 1) < 100 * 2.2e-16
+
+
+def test_arnoldi_relation(sa, orc):
+    """BVMatArnoldi: A V_m - V_m H_m = beta v_{m+1} e_m^T (bvkrylov.c:36-37)."""
+    Ah = sa.convdiff3d(10, 9, 8)
+    A = sa.Mat(Ah)
+    m = 30
+    V = sa.BV(Ah.n, m + 1, sa.FP64)
+    V.set_column(0, orc.rand_uniform(0, Ah.n))
+    V.orthonormalize_column(0)
+    mm, beta, bd = V.mat_arnoldi(A, 0, m)
+    assert mm == m and not bd
+    Vm = V.get_matrix(m + 1)
+    H = V.coefficients(1, m + 1)            # column c holds H(:, c-1)
+    Ad = orc.convdiff3d(10, 9, 8).to_dense()
+    R = Ad @ Vm[:, :m] - Vm[:, :m + 1] @ np.vstack([H[:m + 1, :m]])
+    assert np.linalg.norm(R) < 1e-11 * np.linalg.norm(Ad)
+    assert abs(H[m, m - 1] - beta) < 1e-14 * beta
+    assert np.linalg.norm(Vm.T @ Vm - np.eye(m + 1)) < 1e-13
+
+
+@pytest.mark.parametrize("prec", [0, 1])
+def test_multinplace_matches_oracle(sa, orc, prec):
+    n, m = 5000, 41
+    rng = np.random.default_rng(2)
+    X0 = rng.standard_normal((n, m))
+    Q = rng.standard_normal((m, m))
+    V = sa.BV(n, m, prec)
+    V.set_matrix(X0)
+    V.set_active_columns(3, 37)
+    V.multinplace(Q, 5, 29)
+    out = V.get_matrix(m)
+    ref = orc.bv_multinplace(X0 if prec == 0 else X0.astype(np.float32).astype(np.float64), Q, 5, 29, l=3, k=37)
+    tol = 1e-12 if prec == 0 else 2e-5
+    assert np.max(np.abs(out - ref)) <= tol * np.abs(ref).max()
+
+
+# ------------------------------------------------------------------ full solves
+def _solve(sa, Ah, nev, ncv, tol, which, herm, v0=None, mixed=False, A32=None):
+    A = sa.Mat(Ah, sa.FP64)
+    E = sa.EPS()
+    E.setOperators(A, A32)
+    E.setProblemType(herm)
+    E.setWhichEigenpairs(which)
+    E.setDimensions(nev, ncv)
+    E.setTolerances(tol)
+    if v0 is not None:
+        E.setInitialSpace(v0)
+    if mixed:
+        E.setMixedPrecision(True, 1e-5)
+    E.solve()
+    return E, A
+
+
+def test_ex2_golden(sa, orc):
+    g = GOLD["ex2_1"]
+    Ah = sa.lap2d(72)
+    v0 = orc.rand_uniform(0, Ah.n)
+    E, _ = _solve(sa, Ah, 4, 20, 1e-8, "LM", True, v0)
+    vals = [E.getEigenpair(i, False)[0] for i in range(E.getConverged())]
+    r = orc.eps_solve(orc.lap2d(72), 4, 20, 1e-8, "LM", True, v0=v0)
+    assert E.getConverged() == r["nconv"]
+    assert E.getIterationNumber() == r["its"]
+    assert _rel(vals, r["eigr"]) < 1e-10
+    assert np.allclose(np.round(vals[:4], 5), g["values"], atol=1e-12)
+    for i in range(E.getConverged()):
+        assert E.computeError(i) < 1e-8
+
+
+def test_ex5_golden_nhep(sa, orc):
+    g = GOLD["ex5_1"]
+    Ah = sa.markov(15)
+    v0 = np.zeros(Ah.n)
+    v0[:3] = 1.0
+    E, _ = _solve(sa, Ah, 4, 0, 1e-8, "LR", False, v0)
+    r = orc.eps_solve(orc.markov(15), 4, 0, 1e-8, "LR", False, v0=v0)
+    vals = [E.getEigenpair(i, False)[0] for i in range(E.getConverged())]
+    assert E.getConverged() == r["nconv"]
+    assert _rel(vals, r["eigr"]) < 1e-10
+    assert np.allclose(np.round(vals[:4], 5), g["values"], atol=1e-12)
+    for i in range(E.getConverged()):
+        assert E.computeError(i) < 1e-8
+
+
+def test_ex4_rdb200_golden(sa, orc):
+    g = GOLD["ex4_1"]
+    d = np.load(os.path.join(os.path.dirname(__file__), "golden", "rdb200.npz"))
+    Ah = sa.CSR(int(d["n"]), d["rowptr"], d["col"], d["val"])
+    v0 = orc.rand_uniform(0, Ah.n)
+    E, _ = _solve(sa, Ah, 4, 0, 1e-8, "LM", False, v0)
+    assert E.getIterationNumber() == g["its"]
+    vals = [E.getEigenpair(i, False) for i in range(4)]
+    re = [v[0] for v in vals]
+    assert np.allclose(np.round(re, 5), g["values"], atol=1e-12)
+    for i in range(E.getConverged()):
+        assert E.computeError(i) < 1e-8
+
+
+def test_c1_matches_oracle(sa, orc):
+    """Workload 1: 100x100 Laplacian, nev 10, ncv 32, tol 1e-8, U[-1,1] seed 0.
+    The spectrum is degenerate, so the restart count depends on rounding (the
+    oracle itself changes with its thread count); parity is on the converged
+    values (exact closed forms), the residuals and the invariant subspace of
+    the simple eigenvalues."""
+    Ah = sa.lap2d(100)
+    v0 = orc.rand_uniform(0, Ah.n)
+    E, _ = _solve(sa, Ah, 10, 32, 1e-8, "LM", True, v0)
+    r = orc.eps_solve(orc.lap2d(100), 10, 32, 1e-8, "LM", True, v0=v0, nthreads=1)
+    nc = E.getConverged()
+    assert nc == r["nconv"]
+    assert abs(E.getIterationNumber() - r["its"]) <= 0.3 * r["its"]
+    vals, X = [], []
+    for i in range(nc):
+        re, im, xr, _ = E.getEigenpair(i)
+        vals.append(re)
+        X.append(xr)
+        assert E.computeError(i) <= 1e-8
+    n = 100
+    ev = np.array([4 - 2 * np.cos(i * np.pi / (n + 1)) - 2 * np.cos(j * np.pi / (n + 1))
+                   for i in range(1, n + 1) for j in range(1, n + 1)])
+    for v in vals:
+        assert np.min(np.abs(ev - v)) < 1e-10 * v
+    assert np.allclose(np.unique(np.round(vals, 8)), np.unique(np.round(r["eigr"], 8)), rtol=0, atol=1e-12)
+    # simple eigenvalues: eigenvectors agree up to sign (angle < 1e-6)
+    X = np.array(X).T
+    for i in range(nc):
+        if np.min(np.abs(np.delete(ev, np.argmin(np.abs(ev - vals[i]))) - vals[i])) > 1e-6:
+            j = int(np.argmin(np.abs(r["eigr"] - vals[i])))
+            c = abs(X[:, i] @ r["vecs"][:, j])
+            assert np.arccos(min(1.0, c)) < 1e-6
+
+
+def test_convdiff_nhep_library(sa, orc):
+    """Nonsymmetric 7-point convection-diffusion at 12^3, 20 largest real,
+    ncv 64, tol 1e-10 against dense LAPACK eigenvalues."""
+    Ah = sa.convdiff3d(12)
+    v0 = orc.rand_uniform(0, Ah.n)
+    E, _ = _solve(sa, Ah, 20, 64, 1e-10, "LR", False, v0)
+    nc = E.getConverged()
+    assert nc >= 20
+    vals = np.array([E.getEigenpair(i, False)[0] for i in range(20)])
+    ref = np.array(LIBRES["convdiff12"]["largest_real20"])
+    # degenerate copies may be skipped: every computed value is an eigenvalue
+    for v in vals:
+        assert np.min(np.abs(ref - v)) < 1e-9 * abs(v) or v < ref[-1]
+    for i in range(nc):
+        assert E.computeError(i) < 1e-9
+
+
+def test_mixed_precision_c1(sa, orc):
+    Ah = sa.lap2d(100)
+    v0 = orc.rand_uniform(0, Ah.n)
+    A32 = sa.Mat(Ah, sa.FP32)
+    E, _ = _solve(sa, Ah, 10, 32, 1e-10, "LM", True, v0, mixed=True, A32=A32)
+    r = orc.eps_solve(orc.lap2d(100), 10, 32, 1e-10, "LM", True, v0=v0)
+    nc = E.getConverged()
+    assert nc >= 10
+    st = E.getStats()
+    assert st["matvecs32"] > 0
+    n = 100
+    ev = np.array([4 - 2 * np.cos(i * np.pi / (n + 1)) - 2 * np.cos(j * np.pi / (n + 1))
+                   for i in range(1, n + 1) for j in range(1, n + 1)])
+    for i in range(nc):
+        v = E.getEigenpair(i, False)[0]
+        assert np.min(np.abs(ev - v)) < 1e-10 * v
+        assert E.computeError(i) <= 1e-10
