@@ -116,6 +116,18 @@ int sa_bv_get_coefficients(sa_bv V, int c0, int c1, double *buf_host);
    (BVMatArnoldi/BVMatLanczos bvkrylov.c:56-113,165-226): columns k+1..m are
    produced from V(:,k); *m is reduced on breakdown. */
 int sa_bv_mat_arnoldi(sa_bv V, sa_mat A, int k, int *m, double eta, double *beta, int *breakdown);
+/* Gram-Schmidt passes (CGS1 + DGKS refinements) run on V so far. */
+int sa_bv_get_pass_count(sa_bv V, int64_t *passes);
+
+/* ---- DS: projected dense problem (host FP64, LAPACK) ---------------------------
+ * DSSolve + DSSort of a compact DSHEP problem (src/sys/classes/ds/impls/hep/dshep.c:
+ * 221-347,383-426): diagonal d[n], off-diagonal/arrow e[n-1], l locked rows,
+ * arrowhead up to row k.  wr[n] receives the sorted eigenvalues and Q
+ * ((n+1) x (n+1), column-major) the eigenvectors. */
+int sa_ds_hep_compact(int n, int l, int k, const double *d, const double *e, int which,
+                      double *wr, double *Q);
+/* Path of the LAPACK library resolved at run time (0 when one was found). */
+int sa_lapack_info(char *buf, int len);
 
 /* ---- EPS: Krylov-Schur --------------------------------------------------------
  * Mirrors EPSCreate/EPSSetOperators/EPSSetProblemType/EPSSetWhichEigenpairs/
@@ -157,6 +169,54 @@ typedef struct {
   double  bytes_spmv, bytes_bv, bytes_restart;
 } sa_eps_stats;
 int sa_eps_get_stats(sa_eps eps, sa_eps_stats *st);
+
+/* ---- batched Krylov-Schur: many independent symmetric problems -----------------
+ * Each problem runs the default Krylov-Schur of krylovschur.c:227-348 (Lanczos
+ * + DSHEP, locking, keep 0.5) on its own; all problems share one block-diagonal
+ * operator and one basis in HBM, and one restart phase of every problem is one
+ * kernel launch (a workgroup per problem).  No counterpart type in the
+ * reference: it is the drop-in for a loop of EPSSolve calls over the problems.
+ * Input: P problems, n[p] rows each; rowptr holds the P local row-pointer
+ * arrays (n[p]+1 entries each) back to back, col/val the P local CSR arrays. */
+typedef struct sa_batch_s *sa_batch;
+int sa_batch_create(int P, const int32_t *n, const int32_t *rowptr, const int32_t *col,
+                    const double *val, sa_batch *batch);
+int sa_batch_destroy(sa_batch batch);
+/* mixed != 0: FP32 bulk to tol32, then FP64 polish seeded with the sum of the
+   FP32 Ritz vectors.  v0 optional (concatenated start vectors); NULL draws
+   U[-1,1) with seed sa_sub_seed(seed, p).  ncv < 64. */
+int sa_batch_solve(sa_batch batch, int which, int nev, int ncv, double tol, int max_it,
+                   int mixed, double tol32, const double *v0, uint64_t seed);
+int sa_batch_get_converged(sa_batch batch, int p, int *nconv, int *its, int *reason);
+int sa_batch_get_eigenpair(sa_batch batch, int p, int i, double *re, double *x_host);
+/* err[p*nev + i] = ||A x - lambda x|| / |lambda| in FP64 on the device. */
+int sa_batch_compute_errors(sa_batch batch, int nev, double *err);
+/* counters6: matvecs, passes, restart rounds (FP64), then the same for FP32;
+   seconds3: total, FP32 phase, host projected problems. */
+int sa_batch_get_stats(sa_batch batch, int64_t *counters6, double *seconds3);
+
+/* ---- learned warm start: MLP in BF16 on MFMA ---------------------------------
+ * theta (din <= 32) -> hidden -> hidden -> dout, ReLU, weights and activations
+ * BF16 on MFMA with FP32 accumulation, FP32 master weights, Adam.  No
+ * counterpart in the reference: it produces the vector a caller passes to
+ * EPSSetInitialSpace (epssetup.c:613-626).  W1 is stored hidden x 32 (input
+ * columns beyond din are zero), W2 hidden x hidden, W3 dout x hidden, row-major. */
+int sa_mlp_create(int din, int hidden, int dout, uint64_t seed, sa_mlp *net);
+int sa_mlp_destroy(sa_mlp net);
+/* Forward for a batch: X_host [batch x din] row-major -> Y_host [batch x dout] (FP32). */
+int sa_mlp_forward(sa_mlp net, int batch, const float *X_host, float *Y_host);
+/* One Adam step on a batch with mean-squared loss against T_host; returns the loss. */
+int sa_mlp_train_step(sa_mlp net, int batch, const float *X_host, const float *T_host,
+                      float lr, float *loss);
+/* Gradients of the mean-squared loss (no update) as FP32 host arrays. */
+int sa_mlp_gradients(sa_mlp net, int batch, const float *X_host, const float *T_host,
+                     float *gW1, float *gb1, float *gW2, float *gb2, float *gW3, float *gb3, float *loss);
+/* FP32 master parameters. */
+int sa_mlp_get_params(sa_mlp net, float *W1, float *b1, float *W2, float *b2, float *W3, float *b3);
+int sa_mlp_set_params(sa_mlp net, const float *W1, const float *b1, const float *W2,
+                      const float *b2, const float *W3, const float *b3);
+/* Bilinear prolongation of nmodes g x g modes (cell-centred) to nx x ny cell grids. */
+int sa_prolong_bilinear(int nmodes, int g, int nx, int ny, const float *modes_host, double *out_host);
 
 #ifdef __cplusplus
 }

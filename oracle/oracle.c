@@ -958,3 +958,28 @@ double orc_residual_rel(int n, const int32_t *rp, const int32_t *ci, const doubl
   free(u);
   return hypot(nr, ni) / hypot(kr, ki);
 }
+
+/* DSSolve + DSSort on a compact HEP problem (arrowhead rows l..k, tridiagonal
+   below), as src/sys/classes/ds/tests/test3.c drives it; returns wr[0..n) and
+   Q (ld x ld, ld = n+1). */
+int orc_ds_hep_compact(int n, int l, int k, const double *d, const double *e, int which, double *wr, double *Q)
+{
+  if (!lp_load()) return -10;
+  int ld = n + 1;
+  DSo ds = {0};
+  ds.ld = ld; ds.hermitian = 1; ds.which = which;
+  ds.Q = (double *)calloc((size_t)ld * ld, sizeof(double));
+  ds.T = (double *)calloc((size_t)3 * ld, sizeof(double));
+  ds.work = (double *)calloc((size_t)8 * ld * ld + 16 * ld, sizeof(double));
+  ds.perm = (int *)calloc(ld, sizeof(int));
+  memcpy(ds.T, d, sizeof(double) * n);
+  memcpy(ds.T + ld, e, sizeof(double) * (n - 1));
+  ds.n = n; ds.l = l; ds.k = k; ds.t = n;
+  ds.state = (l == 0 && k == 0) ? ST_INTERMEDIATE : ST_RAW;
+  double *wi = (double *)calloc(ld, sizeof(double));
+  int info = ds_solve_hep(&ds, wr, wi);
+  if (!info) ds_sort_hep(&ds, wr);
+  if (Q) memcpy(Q, ds.Q, sizeof(double) * (size_t)ld * ld);
+  free(ds.Q); free(ds.T); free(ds.work); free(ds.perm); free(wi);
+  return info;
+}

@@ -54,6 +54,8 @@ int  orc_bv_orthonormalize_column(int64_t n, double *V, int64_t ld, int j, doubl
 /* ---- dense projected problems (host LAPACK) ---------------------------- */
 int  orc_lapack_status(char *buf, int len);
 void orc_arrow_tridiag(int n, double *d, double *e, double *Q, int ld);
+int  orc_ds_hep_compact(int n, int l, int k, const double *d, const double *e, int which,
+                        double *wr, double *Q);
 
 /* ---- Krylov-Schur solve ------------------------------------------------ */
 enum { ORC_LM = 1, ORC_SM = 2, ORC_LR = 3, ORC_SR = 4 };

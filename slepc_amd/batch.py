@@ -1,0 +1,95 @@
+"""Batched Krylov-Schur over many independent symmetric problems (C ABI
+sa_batch_*): one block-diagonal operator, one basis, one kernel launch per
+restart phase with a workgroup per problem."""
+import ctypes as C
+
+import numpy as np
+
+from ._lib import check, lib
+
+
+def _sig():
+    L = lib()
+    vp = C.c_void_p
+    L.sa_batch_create.argtypes = [C.c_int, vp, vp, vp, vp, C.POINTER(vp)]
+    L.sa_batch_destroy.argtypes = [vp]
+    L.sa_batch_solve.argtypes = [vp, C.c_int, C.c_int, C.c_int, C.c_double, C.c_int, C.c_int, C.c_double, vp,
+                                 C.c_uint64]
+    L.sa_batch_get_converged.argtypes = [vp, C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_int), C.POINTER(C.c_int)]
+    L.sa_batch_get_eigenpair.argtypes = [vp, C.c_int, C.c_int, C.POINTER(C.c_double), vp]
+    L.sa_batch_compute_errors.argtypes = [vp, C.c_int, vp]
+    L.sa_batch_get_stats.argtypes = [vp, vp, vp]
+    return L
+
+
+def uneven_laplacians(P=512, seed=2, lo=32, hi=256, pot=10.0):
+    """The batched family: 2-D 5-point Laplacians on nx x ny rectangles with
+    nx, ny uniform integers in [lo, hi] and a diagonal potential pot*U[0,1);
+    problem p uses sub-seed sa_sub_seed(seed, p)."""
+    import slepc_amd as sa
+    L = lib()
+    out = []
+    span = hi - lo + 1
+    for p in range(P):
+        nx = lo + int(L.sa_u01(seed, 2 * p) * span)
+        ny = lo + int(L.sa_u01(seed, 2 * p + 1) * span)
+        out.append(sa.lap2d(nx, ny, pot, L.sa_sub_seed(seed, p)))
+    return out
+
+
+class Batch:
+    def __init__(self, csrs):
+        self.L = _sig()
+        self.csrs = csrs
+        self.P = len(csrs)
+        self.n = np.array([A.n for A in csrs], np.int32)
+        rp = np.concatenate([A.rowptr for A in csrs]).astype(np.int32)
+        ci = np.concatenate([A.col for A in csrs]).astype(np.int32)
+        va = np.concatenate([A.val for A in csrs]).astype(np.float64)
+        self.h = C.c_void_p()
+        check(self.L.sa_batch_create(self.P, self.n.ctypes.data, rp.ctypes.data, ci.ctypes.data, va.ctypes.data,
+                                     C.byref(self.h)))
+        self.nev = 0
+
+    def solve(self, nev, ncv, tol, which="LM", mixed=False, tol32=1e-5, v0=None, seed=0, max_it=0):
+        from . import WHICH
+        self.nev = nev
+        v = None
+        if v0 is not None:
+            v0 = np.ascontiguousarray(np.concatenate(v0) if isinstance(v0, (list, tuple)) else v0, np.float64)
+            v = v0.ctypes.data
+        check(self.L.sa_batch_solve(self.h, WHICH[which], nev, ncv, tol, max_it, 1 if mixed else 0, tol32, v, seed))
+
+    def converged(self, p):
+        a, b, c = C.c_int(), C.c_int(), C.c_int()
+        check(self.L.sa_batch_get_converged(self.h, p, C.byref(a), C.byref(b), C.byref(c)))
+        return a.value, b.value, c.value
+
+    def eigenpair(self, p, i, vector=True):
+        re = C.c_double()
+        x = np.empty(int(self.n[p])) if vector else None
+        check(self.L.sa_batch_get_eigenpair(self.h, p, i, C.byref(re), x.ctypes.data if vector else None))
+        return re.value, x
+
+    def errors(self):
+        e = np.zeros(self.P * self.nev)
+        check(self.L.sa_batch_compute_errors(self.h, self.nev, e.ctypes.data))
+        return e.reshape(self.P, self.nev)
+
+    def stats(self):
+        c = np.zeros(6, np.int64)
+        s = np.zeros(3)
+        check(self.L.sa_batch_get_stats(self.h, c.ctypes.data, s.ctypes.data))
+        return dict(matvecs=int(c[0]), passes=int(c[1]), rounds=int(c[2]), matvecs32=int(c[3]),
+                    passes32=int(c[4]), rounds32=int(c[5]), seconds=s[0], seconds32=s[1], seconds_ds=s[2])
+
+    def destroy(self):
+        if self.h:
+            self.L.sa_batch_destroy(self.h)
+            self.h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.destroy()
+        except Exception:
+            pass

@@ -1,0 +1,842 @@
+// Batched Krylov-Schur for many independent symmetric problems of uneven size.
+//
+// All problems live in one block-diagonal CSR operator and one basis V whose
+// rows are the concatenation of the problems' rows (each problem padded to a
+// multiple of 64 rows).  One restart phase is ONE launch of the sweep kernel:
+// workgroup b runs the complete Arnoldi/Lanczos sweep (SpMV, classical
+// Gram-Schmidt with DGKS refinement, normalisation) of problem order[b], so all
+// synchronisation is inside the workgroup and no grid-wide barrier exists.
+// Problems are ordered largest first so the hardware scheduler balances CUs.
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstring>
+#include <numeric>
+#include <thread>
+#include <vector>
+
+#include "sa_ds.h"
+#include "sa_internal.h"
+
+namespace sa {
+
+constexpr int BMAXC = 128;   // basis columns handled by the batched kernels
+
+template <typename T>
+struct BatchArgs {
+  T *V;
+  int64_t ldv;
+  const int32_t *rp, *ci;
+  const T *val;
+  const int *order;
+  const int64_t *off;
+  const int *n;
+  const int *k0, *nv;
+  int spmv;
+  double *dbuf;
+  int m;
+  double *beta;
+  int *stop;
+  int *passes;
+  double eta;
+};
+
+__device__ __forceinline__ double bsafe_sqrt(double a) { return a < 0.0 ? 0.0 : sqrt(a); }
+
+// Sums acc[0..ncol) over the workgroup into out[0..ncol) (FP64).
+template <typename T, int CM, int NT>
+__device__ __forceinline__ void block_sum(const T (&acc)[CM + 1], int ncol, double *red, double *out) {
+  const int t = threadIdx.x, lane = t & 63, w = t >> 6;
+  constexpr int NW = NT / 64;
+#pragma unroll
+  for (int i = 0; i <= CM; i++) {
+    if (i < ncol) {
+      double v = (double)acc[i];
+#pragma unroll
+      for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+      if (lane == 0) red[w * (CM + 1) + i] = v;
+    }
+  }
+  __syncthreads();
+  for (int i = t; i < ncol; i += NT) {
+    double s = 0.0;
+#pragma unroll
+    for (int q = 0; q < NW; q++) s += red[q * (CM + 1) + i];
+    out[i] = s;
+  }
+  __syncthreads();
+}
+
+template <typename T, int CM, int NT, int L>
+__global__ __launch_bounds__(NT) void batched_sweep_kernel(BatchArgs<T> a) {
+  __shared__ double s_red[(NT / 64) * (CM + 1)];
+  __shared__ double s_h[CM + 1];      // dots of the current pass (index c: v^T v)
+  __shared__ double s_hacc[CM + 1];   // accumulated coefficients of the column
+  __shared__ T s_coef[CM];
+  __shared__ int s_mode;              // 0 final (mult+scale), 1 continue (mult+dots), 2 pending (mult+dots, norm check)
+  __shared__ double s_scale, s_onrm, s_nrm;
+  __shared__ int s_pass, s_done;
+  const int t = threadIdx.x;
+  const int p = a.order[blockIdx.x];
+  const int n = a.n[p];
+  const int64_t off = a.off[p];
+  const int k0 = a.k0[p], nv = a.nv[p];
+  const int cfirst = a.spmv ? k0 + 1 : k0, clast = a.spmv ? nv : k0;
+  double *dbuf = a.dbuf + (size_t)p * a.m * a.m;
+  if (cfirst > clast || a.stop[p] >= 0) return;
+  for (int c = cfirst; c <= clast; c++) {
+    T *vc = a.V + (size_t)c * a.ldv + off;
+    // ---- SpMV: v_c = A v_{c-1} (global column indices of the block-diagonal operator)
+    if (a.spmv) {
+      const T *x = a.V + (size_t)(c - 1) * a.ldv;
+      constexpr int RPS = NT / L;
+      const int g = t / L, lg = t % L;
+      for (int r0 = 0; r0 < n; r0 += RPS) {
+        const int r = r0 + g;
+        T s = (T)0;
+        if (r < n) {
+          const int64_t row = off + r;
+          const int32_t pe = a.rp[row + 1];
+          for (int32_t q = a.rp[row] + lg; q < pe; q += L) s += a.val[q] * x[a.ci[q]];
+        }
+#pragma unroll
+        for (int o = L / 2; o > 0; o >>= 1) s += __shfl_xor(s, o, L);
+        if (lg == 0 && r < n) vc[r] = s;
+      }
+      __syncthreads();
+    }
+    // ---- pass-1 dot products V(:,0..c)^T v_c
+    {
+      T acc[CM + 1];
+#pragma unroll
+      for (int i = 0; i <= CM; i++) acc[i] = (T)0;
+      for (int r = t; r < n; r += NT) {
+        const T v = vc[r];
+#pragma unroll
+        for (int i = 0; i < CM; i++)
+          if (i < c) acc[i] += a.V[(size_t)i * a.ldv + off + r] * v;
+        acc[CM] += v * v;
+      }
+      if (c < CM) acc[c] = acc[CM];
+      block_sum<T, CM, NT>(acc, c + 1, s_red, s_h);
+    }
+    for (int i = t; i < c; i += NT) s_hacc[i] = 0.0;
+    if (t == 0) { s_pass = 0; s_done = 0; }
+    __syncthreads();
+    // ---- CGS1 passes with DGKS refinement (bvorthog.c:91-217)
+    while (true) {
+      if (t == 0) {
+        const double self = s_h[c];
+        double sum = 0.0;
+        for (int i = 0; i < c; i++) sum += s_h[i] * s_h[i];
+        s_pass++;
+        const double onrm = bsafe_sqrt(self);
+        const double est = onrm * onrm - sum;
+        s_onrm = onrm;
+        if (est <= 0.0) {
+          s_mode = 2;
+          s_scale = 1.0;
+        } else {
+          const double nrm = sqrt(est);
+          if (nrm != 0.0 && nrm < a.eta * onrm && s_pass < 3) {
+            s_mode = 1;
+            s_scale = 1.0;
+          } else {
+            s_mode = 0;
+            s_nrm = nrm;
+            s_scale = (nrm != 0.0 && nrm != 1.0) ? 1.0 / nrm : 1.0;
+          }
+        }
+      }
+      __syncthreads();
+      for (int i = t; i < c; i += NT) {
+        s_hacc[i] += s_h[i];
+        s_coef[i] = (T)s_h[i];
+      }
+      __syncthreads();
+      const int mode = s_mode;
+      const T scl = (T)s_scale;
+      T acc[CM + 1];
+#pragma unroll
+      for (int i = 0; i <= CM; i++) acc[i] = (T)0;
+      for (int r = t; r < n; r += NT) {
+        T s = (T)0;
+#pragma unroll
+        for (int i = 0; i < CM; i++)
+          if (i < c) s += a.V[(size_t)i * a.ldv + off + r] * s_coef[i];
+        const T vn = (vc[r] - s) * scl;
+        vc[r] = vn;
+        if (mode != 0) {
+#pragma unroll
+          for (int i = 0; i < CM; i++)
+            if (i < c) acc[i] += a.V[(size_t)i * a.ldv + off + r] * vn;
+          acc[CM] += vn * vn;
+        }
+      }
+      if (mode == 0) break;
+      if (c < CM) acc[c] = acc[CM];
+      __syncthreads();
+      block_sum<T, CM, NT>(acc, c + 1, s_red, s_h);
+      if (mode == 2) {   // the estimate failed: the new v^T v is the true norm
+        if (t == 0) {
+          const double nrm = bsafe_sqrt(s_h[c]);
+          if (!(nrm != 0.0 && nrm < a.eta * s_onrm && s_pass < 3)) {
+            s_done = 1;
+            s_nrm = nrm;
+            s_scale = (nrm != 0.0 && nrm != 1.0) ? 1.0 / nrm : 1.0;
+          }
+        }
+        __syncthreads();
+        if (s_done) {
+          const T sc = (T)s_scale;
+          for (int r = t; r < n; r += NT) vc[r] *= sc;
+          break;
+        }
+      }
+    }
+    __syncthreads();
+    // ---- store coefficients and the norm (bvorthog.c:209-215), breakdown test
+    int lind = 0;
+    {
+      const double nrm = s_nrm, onrm = s_onrm;
+      lind = !(nrm != 0.0 && fabs(nrm) >= a.eta * fabs(onrm));
+      for (int i = t; i < c; i += NT) dbuf[(size_t)c * a.m + i] = s_hacc[i];
+      if (t == 0) {
+        dbuf[(size_t)c * a.m + c] = lind ? 0.0 : nrm;
+        a.beta[p] = nrm;
+        a.passes[p] += s_pass;
+        if (lind) a.stop[p] = c;
+      }
+    }
+    __syncthreads();
+    if (lind) break;
+  }
+}
+
+// In-place restart GEMM per problem: V_p(:,s:e) = V_p(:,l:k) Q_p(l:k,s:e), one
+// workgroup per (problem, 64-row chunk); FP64/FP32 via plain FMA (the block is
+// at most 128 x 128 and the kernel is bound by the V traffic).
+template <typename T>
+__global__ __launch_bounds__(256) void batched_mip_kernel(T *V, int64_t ldv, const int *chunk_prob,
+                                                          const int *chunk_row, const int64_t *off, const int *n,
+                                                          const int *ml, const int *mk, const int *ms, const int *me,
+                                                          const double *Q, int m) {
+  __shared__ T As[64][65];   // K <= 64 columns (ncv < 64)
+  const int g = blockIdx.x, p = chunk_prob[g], r0 = chunk_row[g];
+  const int l = ml[p], k = mk[p], s = ms[p], e = me[p];
+  const int K = k - l, N = e - s;
+  if (K <= 0 || N <= 0) return;
+  const int rows = min(64, n[p] - r0);
+  T *Vp = V + off[p] + r0;
+  const int t = threadIdx.x, row = t & 63, cg = t >> 6;
+  for (int i = cg; i < K; i += 4) As[i][row] = (row < rows) ? Vp[(size_t)(l + i) * ldv + row] : (T)0;
+  __syncthreads();
+  const double *Qp = Q + (size_t)p * m * m;
+  for (int j = cg; j < N; j += 4) {
+    T acc = (T)0;
+    for (int i = 0; i < K; i++) acc += As[i][row] * (T)Qp[(size_t)(l + i) + (size_t)(s + j) * m];
+    if (row < rows) Vp[(size_t)(s + j) * ldv + row] = acc;
+  }
+}
+
+template <typename T>
+__global__ void batched_copy_kernel(T *V, int64_t ldv, const int *chunk_prob, const int *chunk_row, const int64_t *off,
+                                    const int *n, const int *src, const int *dst) {
+  const int g = blockIdx.x, p = chunk_prob[g], r = chunk_row[g] + (int)threadIdx.x;
+  if (src[p] < 0 || r >= n[p]) return;
+  V[(size_t)dst[p] * ldv + off[p] + r] = V[(size_t)src[p] * ldv + off[p] + r];
+}
+
+template <typename T>
+__global__ void batched_scale_kernel(T *V, int64_t ldv, const int *chunk_prob, const int *chunk_row,
+                                     const int64_t *off, const int *n, const int *col, const double *alpha) {
+  const int g = blockIdx.x, p = chunk_prob[g], r = chunk_row[g] + (int)threadIdx.x;
+  if (col[p] < 0 || r >= n[p]) return;
+  V[(size_t)col[p] * ldv + off[p] + r] *= (T)alpha[p];
+}
+
+// ---------------------------------------------------------------------------
+// host side
+// ---------------------------------------------------------------------------
+struct BatchDev {
+  int P = 0, m = 0, prec = SA_FP64;
+  int64_t ldv = 0, nrows = 0, nnz = 0;
+  void *V = nullptr;
+  int32_t *rp = nullptr, *ci = nullptr;
+  void *val = nullptr;
+  int lanes = 4;
+  int64_t *off = nullptr;
+  int *n = nullptr, *order = nullptr, *k0 = nullptr, *nv = nullptr, *stop = nullptr, *passes = nullptr;
+  double *dbuf = nullptr, *beta = nullptr, *Q = nullptr, *alpha = nullptr;
+  int *chunk_prob = nullptr, *chunk_row = nullptr, nchunks = 0;
+  int *ml = nullptr, *mk = nullptr, *ms = nullptr, *me = nullptr, *src = nullptr, *dst = nullptr;
+};
+
+template <typename T, int CM, int NT>
+static void launch_sweep_t(const BatchDev &B, int nact, int spmv, double eta, hipStream_t s) {
+  BatchArgs<T> a;
+  a.V = (T *)B.V; a.ldv = B.ldv; a.rp = B.rp; a.ci = B.ci; a.val = (const T *)B.val;
+  a.order = B.order; a.off = B.off; a.n = B.n; a.k0 = B.k0; a.nv = B.nv; a.spmv = spmv;
+  a.dbuf = B.dbuf; a.m = B.m; a.beta = B.beta; a.stop = B.stop; a.passes = B.passes; a.eta = eta;
+  switch (B.lanes) {
+    case 4: hipLaunchKernelGGL((batched_sweep_kernel<T, CM, NT, 4>), dim3(nact), dim3(NT), 0, s, a); break;
+    case 8: hipLaunchKernelGGL((batched_sweep_kernel<T, CM, NT, 8>), dim3(nact), dim3(NT), 0, s, a); break;
+    case 16: hipLaunchKernelGGL((batched_sweep_kernel<T, CM, NT, 16>), dim3(nact), dim3(NT), 0, s, a); break;
+    default: hipLaunchKernelGGL((batched_sweep_kernel<T, CM, NT, 32>), dim3(nact), dim3(NT), 0, s, a); break;
+  }
+}
+
+static int launch_sweep(const BatchDev &B, int nact, int spmv, double eta, hipStream_t s) {
+  if (nact == 0) return 0;
+  const int cmax = B.m - 1;   // largest column index
+  if (B.prec == SA_FP64) {
+    if (cmax < 32) launch_sweep_t<double, 32, 512>(B, nact, spmv, eta, s);
+    else if (cmax < 64) launch_sweep_t<double, 64, 256>(B, nact, spmv, eta, s);
+    else return set_error("batched basis wider than 64 columns in FP64"), 2;
+  } else {
+    if (cmax < 32) launch_sweep_t<float, 32, 512>(B, nact, spmv, eta, s);
+    else if (cmax < 64) launch_sweep_t<float, 64, 512>(B, nact, spmv, eta, s);
+    else return set_error("batched basis wider than 64 columns in FP32"), 2;
+  }
+  SA_HIP(hipGetLastError());
+  return 0;
+}
+
+}  // namespace sa
+
+using namespace sa;
+
+struct sa_batch_s {
+  int P = 0;
+  std::vector<int> n;
+  std::vector<int64_t> off;
+  int64_t nrows = 0, nnz = 0;
+  // host CSR of the block-diagonal operator (FP64), kept for both precisions
+  std::vector<int32_t> rp, ci;
+  std::vector<double> val;
+  BatchDev dev64, dev32;
+  // results
+  std::vector<std::vector<double>> eig, err;
+  std::vector<std::vector<int>> perm;
+  std::vector<int> nconv, its, reason;
+  int64_t matvecs = 0, matvecs32 = 0, passes = 0, passes32 = 0;
+  double seconds = 0, seconds32 = 0, seconds_ds = 0;
+  int its_max = 0, its_max32 = 0, solved = 0;
+};
+
+static void free_dev(BatchDev &B) {
+  void *ptrs[] = {B.V, B.rp, B.ci, B.val, B.off, B.n, B.order, B.k0, B.nv, B.stop, B.passes, B.dbuf, B.beta,
+                  B.Q, B.alpha, B.chunk_prob, B.chunk_row, B.ml, B.mk, B.ms, B.me, B.src, B.dst};
+  for (void *p : ptrs)
+    if (p) (void)hipFree(p);
+  B = BatchDev();
+}
+
+static int alloc_dev(sa_batch_s *b, BatchDev &B, int prec, int m) {
+  if (B.V && B.m == m && B.prec == prec) return 0;
+  const bool keep_op = B.rp && B.prec == prec;
+  if (!keep_op) free_dev(B);
+  else if (B.V) {
+    (void)hipFree(B.V);
+    B.V = nullptr;
+  }
+  hipStream_t s = stream();
+  B.P = b->P;
+  B.m = m;
+  B.prec = prec;
+  B.ldv = b->nrows;
+  B.nrows = b->nrows;
+  B.nnz = b->nnz;
+  const size_t es = prec_size(prec);
+  SA_HIP(hipMalloc(&B.V, es * (size_t)B.ldv * m));
+  SA_HIP(hipMemsetAsync(B.V, 0, es * (size_t)B.ldv * m, s));
+  if (keep_op) return 0;
+  const int P = b->P;
+  SA_HIP(hipMalloc(&B.rp, sizeof(int32_t) * (b->nrows + 1)));
+  SA_HIP(hipMalloc(&B.ci, sizeof(int32_t) * std::max<int64_t>(1, b->nnz)));
+  SA_HIP(hipMalloc(&B.val, es * std::max<int64_t>(1, b->nnz)));
+  SA_HIP(hipMemcpyAsync(B.rp, b->rp.data(), sizeof(int32_t) * (b->nrows + 1), hipMemcpyHostToDevice, s));
+  SA_HIP(hipMemcpyAsync(B.ci, b->ci.data(), sizeof(int32_t) * b->nnz, hipMemcpyHostToDevice, s));
+  if (prec == SA_FP64) {
+    SA_HIP(hipMemcpyAsync(B.val, b->val.data(), sizeof(double) * b->nnz, hipMemcpyHostToDevice, s));
+  } else {
+    std::vector<float> f(b->val.begin(), b->val.end());
+    SA_HIP(hipMemcpyAsync(B.val, f.data(), sizeof(float) * b->nnz, hipMemcpyHostToDevice, s));
+    SA_HIP(hipStreamSynchronize(s));
+  }
+  const double avg = b->nrows ? (double)b->nnz / b->nrows : 1.0;
+  B.lanes = avg <= 4.5 ? 4 : (avg <= 9 ? 8 : (avg <= 18 ? 16 : 32));
+  SA_HIP(hipMalloc(&B.off, sizeof(int64_t) * P));
+  SA_HIP(hipMalloc(&B.n, sizeof(int) * P));
+  int *ints[] = {nullptr};
+  (void)ints;
+  for (int **q : {&B.order, &B.k0, &B.nv, &B.stop, &B.passes, &B.ml, &B.mk, &B.ms, &B.me, &B.src, &B.dst})
+    SA_HIP(hipMalloc(q, sizeof(int) * P));
+  SA_HIP(hipMalloc(&B.dbuf, sizeof(double) * (size_t)P * 128 * 128));
+  SA_HIP(hipMalloc(&B.Q, sizeof(double) * (size_t)P * 128 * 128));
+  SA_HIP(hipMalloc(&B.beta, sizeof(double) * P));
+  SA_HIP(hipMalloc(&B.alpha, sizeof(double) * P));
+  SA_HIP(hipMemcpyAsync(B.off, b->off.data(), sizeof(int64_t) * P, hipMemcpyHostToDevice, s));
+  SA_HIP(hipMemcpyAsync(B.n, b->n.data(), sizeof(int) * P, hipMemcpyHostToDevice, s));
+  SA_HIP(hipMemsetAsync(B.passes, 0, sizeof(int) * P, s));
+  std::vector<int> cp, cr;
+  for (int p = 0; p < P; p++)
+    for (int r = 0; r < b->n[p]; r += 64) {
+      cp.push_back(p);
+      cr.push_back(r);
+    }
+  B.nchunks = (int)cp.size();
+  SA_HIP(hipMalloc(&B.chunk_prob, sizeof(int) * std::max(1, B.nchunks)));
+  SA_HIP(hipMalloc(&B.chunk_row, sizeof(int) * std::max(1, B.nchunks)));
+  SA_HIP(hipMemcpyAsync(B.chunk_prob, cp.data(), sizeof(int) * B.nchunks, hipMemcpyHostToDevice, s));
+  SA_HIP(hipMemcpyAsync(B.chunk_row, cr.data(), sizeof(int) * B.nchunks, hipMemcpyHostToDevice, s));
+  SA_HIP(hipStreamSynchronize(s));
+  return 0;
+}
+
+namespace {
+struct ProbState {
+  DenseProblem ds;
+  std::vector<double> eigr, eigi, errest;
+  int nconv = 0, l = 0, its = 0, reason = 0, k = 0, nv = 0, k0 = 0;
+  bool active = true, restart_random = false;
+  double beta = 0.0;
+  int stop = -1;
+};
+
+template <typename F>
+void parallel_for(int n, F f) {
+  const int nt = std::max(1, std::min(16, (int)std::thread::hardware_concurrency()));
+  if (n < 8 || nt == 1) {
+    for (int i = 0; i < n; i++) f(i);
+    return;
+  }
+  std::vector<std::thread> th;
+  for (int w = 0; w < nt; w++)
+    th.emplace_back([&, w]() {
+      for (int i = w; i < n; i += nt) f(i);
+    });
+  for (auto &x : th) x.join();
+}
+double nowb() { return std::chrono::duration<double>(std::chrono::steady_clock::now().time_since_epoch()).count(); }
+}  // namespace
+
+// One batched Krylov-Schur solve (symmetric problems, Lanczos + DSHEP on each)
+// in basis precision prec; starts from column 0 of V.  Mirrors
+// EPSSolve_KrylovSchur_Default (krylovschur.c:227-348) per problem.
+static int batch_ks(sa_batch_s *b, BatchDev &B, int which, int nev, int ncv, double tol, int max_it,
+                    std::vector<ProbState> &ps, int64_t &matvecs, int64_t &passes_out, int &its_max, double &tds) {
+  const int P = b->P, m = ncv + 1, ld = m, mpd = ncv;
+  const double eta = 0.7071, keep = 0.5;
+  hipStream_t s = stream();
+  SA_CHECK(lapack_ready(), "no LAPACK library found for the projected problem");
+  ps.assign(P, ProbState());
+  for (int p = 0; p < P; p++) {
+    ps[p].ds.allocate(ld, true, which);
+    ps[p].eigr.assign(ld, 0.0);
+    ps[p].eigi.assign(ld, 0.0);
+    ps[p].errest.assign(ld, 0.0);
+  }
+  std::vector<int> order(P), k0(P), nv(P), stop(P), passes0(P), passes1(P), zero(P, 0);
+  std::vector<double> beta(P), dbuf, Qh((size_t)P * m * m, 0.0);
+  std::vector<int> ml(P), mk(P), msv(P), mev(P), src(P), dst(P);
+  // problems by decreasing size: the scheduler hands the largest sweeps out first
+  std::iota(order.begin(), order.end(), 0);
+  std::stable_sort(order.begin(), order.end(), [&](int x, int y) { return b->n[x] > b->n[y]; });
+  SA_HIP(hipMemcpyAsync(B.passes, zero.data(), sizeof(int) * P, hipMemcpyHostToDevice, s));
+  // orthonormalise the start vectors (column 0): EPSGetStartVector
+  for (int p = 0; p < P; p++) { k0[p] = 0; nv[p] = 0; stop[p] = -1; }
+  SA_HIP(hipMemcpyAsync(B.order, order.data(), sizeof(int) * P, hipMemcpyHostToDevice, s));
+  SA_HIP(hipMemcpyAsync(B.k0, k0.data(), sizeof(int) * P, hipMemcpyHostToDevice, s));
+  SA_HIP(hipMemcpyAsync(B.nv, nv.data(), sizeof(int) * P, hipMemcpyHostToDevice, s));
+  SA_HIP(hipMemcpyAsync(B.stop, stop.data(), sizeof(int) * P, hipMemcpyHostToDevice, s));
+  SA_TRY(launch_sweep(B, P, 0, eta, s));
+  SA_HIP(hipMemcpyAsync(stop.data(), B.stop, sizeof(int) * P, hipMemcpyDeviceToHost, s));
+  SA_HIP(hipStreamSynchronize(s));
+  for (int p = 0; p < P; p++) SA_CHECK(stop[p] < 0, "a start vector is zero");
+  its_max = 0;
+  matvecs = 0;
+  while (true) {
+    // ---- set up the sweep of every active problem
+    std::vector<int> act;
+    for (int i = 0; i < P; i++) {
+      const int p = order[i];
+      ProbState &S = ps[p];
+      if (!S.active) continue;
+      S.its++;
+      S.nv = std::min(S.nconv + mpd, ncv);
+      S.k0 = S.nconv + S.l;
+      S.ds.set_dimensions(S.nv, S.nconv, S.k0);
+      act.push_back(p);
+    }
+    if (act.empty()) break;
+    for (int p = 0; p < P; p++) {
+      k0[p] = ps[p].k0;
+      nv[p] = ps[p].active ? ps[p].nv : ps[p].k0;
+      stop[p] = -1;
+    }
+    SA_HIP(hipMemcpyAsync(B.order, act.data(), sizeof(int) * act.size(), hipMemcpyHostToDevice, s));
+    SA_HIP(hipMemcpyAsync(B.k0, k0.data(), sizeof(int) * P, hipMemcpyHostToDevice, s));
+    SA_HIP(hipMemcpyAsync(B.nv, nv.data(), sizeof(int) * P, hipMemcpyHostToDevice, s));
+    SA_HIP(hipMemcpyAsync(B.stop, stop.data(), sizeof(int) * P, hipMemcpyHostToDevice, s));
+    SA_TRY(launch_sweep(B, (int)act.size(), 1, eta, s));
+    dbuf.resize((size_t)P * m * m);
+    SA_HIP(hipMemcpyAsync(dbuf.data(), B.dbuf, sizeof(double) * (size_t)P * m * m, hipMemcpyDeviceToHost, s));
+    SA_HIP(hipMemcpyAsync(beta.data(), B.beta, sizeof(double) * P, hipMemcpyDeviceToHost, s));
+    SA_HIP(hipMemcpyAsync(stop.data(), B.stop, sizeof(int) * P, hipMemcpyDeviceToHost, s));
+    SA_HIP(hipStreamSynchronize(s));
+    const double t0 = nowb();
+    its_max++;
+    // ---- host: projected problems of all active problems in parallel
+    std::vector<int> fail(act.size(), 0);
+    parallel_for((int)act.size(), [&](int ia) {
+      const int p = act[ia];
+      ProbState &S = ps[p];
+      int nvp = S.nv;
+      const int breakdown = stop[p] >= 0;
+      if (breakdown) nvp = stop[p];
+      const double bet = beta[p];
+      const double *Bp = dbuf.data() + (size_t)p * m * m;
+      double *d = S.ds.d(), *e = S.ds.e();
+      for (int j = S.k0; j < nvp; j++) {
+        d[j] = Bp[(size_t)(j + 1) * m + j];
+        e[j] = Bp[(size_t)(j + 1) * m + j + 1];
+      }
+      S.nv = nvp;
+      S.ds.set_dimensions(nvp, S.nconv, S.k0);
+      S.ds.set_state(S.l ? DS_RAW : DS_INTERMEDIATE);
+      if (S.ds.solve(S.eigr.data(), S.eigi.data())) { fail[ia] = 1; return; }
+      S.ds.sort(S.eigr.data(), S.eigi.data());
+      S.ds.update_extra_row();
+      int k, marker = -1;
+      for (k = S.nconv; k < nvp; k++) {
+        double rn = 0.0;
+        int cp = 0;
+        S.ds.ritz_residual(k, &rn, &cp);
+        rn *= bet;
+        const double w = std::fabs(S.eigr[k]);
+        S.errest[k] = w != 0.0 ? rn / w : 1.79e308;
+        if (marker == -1 && S.errest[k] >= tol) marker = k;
+        if (marker != -1) break;
+      }
+      if (marker != -1) k = marker;
+      if (k >= nev) S.reason = 1;
+      else if (S.its >= max_it) S.reason = -1;
+      int newl = 0;
+      if (!(S.reason != 0 || breakdown || k == nvp)) newl = std::max(1, (int)((nvp - k) * keep));
+      S.restart_random = false;
+      if (S.reason == 0) {
+        if (breakdown || k == nvp) {
+          if (k < nev) S.restart_random = true;
+        } else {
+          S.ds.truncate(k + newl);
+        }
+      }
+      // restart GEMM parameters: V(:,nconv:k+l) = V(:,nconv:nv) Q
+      std::memcpy(Qh.data() + (size_t)p * m * m, S.ds.Q(), sizeof(double) * m * m);
+      ml[p] = S.nconv;
+      mk[p] = nvp;
+      msv[p] = S.nconv;
+      mev[p] = k + newl;
+      src[p] = (S.reason == 0 && !breakdown) ? nvp : -1;
+      dst[p] = k + newl;
+      S.k = k;
+      S.l = newl;
+    });
+    for (size_t ia = 0; ia < act.size(); ia++) SA_CHECK(!fail[ia], "LAPACK failure in a batched projected problem");
+    for (int p = 0; p < P; p++)
+      if (!ps[p].active) { ml[p] = mk[p] = msv[p] = mev[p] = 0; src[p] = -1; dst[p] = 0; }
+    for (int p : act) matvecs += ps[p].nv - ps[p].k0;
+    tds += nowb() - t0;
+    SA_HIP(hipMemcpyAsync(B.Q, Qh.data(), sizeof(double) * (size_t)P * m * m, hipMemcpyHostToDevice, s));
+    for (auto pr : {std::make_pair(B.ml, &ml), std::make_pair(B.mk, &mk), std::make_pair(B.ms, &msv),
+                    std::make_pair(B.me, &mev), std::make_pair(B.src, &src), std::make_pair(B.dst, &dst)})
+      SA_HIP(hipMemcpyAsync(pr.first, pr.second->data(), sizeof(int) * P, hipMemcpyHostToDevice, s));
+    if (B.prec == SA_FP64) {
+      hipLaunchKernelGGL(batched_mip_kernel<double>, dim3(B.nchunks), dim3(256), 0, s, (double *)B.V, B.ldv,
+                         B.chunk_prob, B.chunk_row, B.off, B.n, B.ml, B.mk, B.ms, B.me, B.Q, m);
+      hipLaunchKernelGGL(batched_copy_kernel<double>, dim3(B.nchunks), dim3(64), 0, s, (double *)B.V, B.ldv,
+                         B.chunk_prob, B.chunk_row, B.off, B.n, B.src, B.dst);
+    } else {
+      hipLaunchKernelGGL(batched_mip_kernel<float>, dim3(B.nchunks), dim3(256), 0, s, (float *)B.V, B.ldv,
+                         B.chunk_prob, B.chunk_row, B.off, B.n, B.ml, B.mk, B.ms, B.me, B.Q, m);
+      hipLaunchKernelGGL(batched_copy_kernel<float>, dim3(B.nchunks), dim3(64), 0, s, (float *)B.V, B.ldv,
+                         B.chunk_prob, B.chunk_row, B.off, B.n, B.src, B.dst);
+    }
+    SA_HIP(hipGetLastError());
+    // breakdown restarts (EPSGetStartVector(k)): random column, orthonormalised
+    std::vector<int> rs;
+    for (int p : act)
+      if (ps[p].restart_random) rs.push_back(p);
+    if (!rs.empty()) {
+      SA_HIP(hipStreamSynchronize(s));
+      for (int p : rs) {
+        std::vector<double> x(b->n[p]);
+        sa_rand_uniform(0x5EEDULL + (uint64_t)ps[p].k, b->n[p], x.data());
+        const size_t es = prec_size(B.prec);
+        char *dstp = (char *)B.V + es * ((size_t)ps[p].k * B.ldv + b->off[p]);
+        if (B.prec == SA_FP64) {
+          SA_HIP(hipMemcpy(dstp, x.data(), sizeof(double) * x.size(), hipMemcpyHostToDevice));
+        } else {
+          std::vector<float> f(x.begin(), x.end());
+          SA_HIP(hipMemcpy(dstp, f.data(), sizeof(float) * f.size(), hipMemcpyHostToDevice));
+        }
+        k0[p] = ps[p].k;
+      }
+      std::vector<int> st2(P, 0);
+      for (int p = 0; p < P; p++) st2[p] = -1;
+      SA_HIP(hipMemcpyAsync(B.order, rs.data(), sizeof(int) * rs.size(), hipMemcpyHostToDevice, s));
+      SA_HIP(hipMemcpyAsync(B.k0, k0.data(), sizeof(int) * P, hipMemcpyHostToDevice, s));
+      SA_HIP(hipMemcpyAsync(B.stop, st2.data(), sizeof(int) * P, hipMemcpyHostToDevice, s));
+      SA_TRY(launch_sweep(B, (int)rs.size(), 0, eta, s));
+      SA_HIP(hipMemcpyAsync(st2.data(), B.stop, sizeof(int) * P, hipMemcpyDeviceToHost, s));
+      SA_HIP(hipStreamSynchronize(s));
+      for (int p : rs)
+        if (st2[p] >= 0) ps[p].reason = -2;
+    }
+    for (int p : act) {
+      ProbState &S = ps[p];
+      S.nconv = S.k;
+      if (S.reason != 0) S.active = false;
+    }
+  }
+  std::vector<int> pz(P);
+  SA_HIP(hipMemcpyAsync(pz.data(), B.passes, sizeof(int) * P, hipMemcpyDeviceToHost, s));
+  SA_HIP(hipStreamSynchronize(s));
+  passes_out = 0;
+  for (int p = 0; p < P; p++) passes_out += pz[p];
+  return 0;
+}
+
+static void sort_perm(int which, const std::vector<double> &eigr, int nconv, std::vector<int> &perm) {
+  perm.resize(nconv);
+  std::iota(perm.begin(), perm.end(), 0);
+  // SlepcSortEigenvalues for real eigenvalues (slepcsc.c:89-140)
+  for (int i = nconv - 1; i >= 0; i--) {
+    const double re = eigr[perm[i]];
+    int j = i + 1;
+    while (j < nconv) {
+      if (sc_compare(which, re, 0.0, eigr[perm[j]], 0.0) <= 0) break;
+      std::swap(perm[j - 1], perm[j]);
+      j++;
+    }
+  }
+}
+
+extern "C" {
+
+int sa_batch_create(int P, const int32_t *n, const int32_t *rowptr, const int32_t *col, const double *val,
+                    sa_batch *out) {
+  SA_TRY(ensure_init());
+  SA_CHECK(P > 0 && n && rowptr && col && val && out, "invalid batch arguments");
+  auto *b = new sa_batch_s();
+  b->P = P;
+  b->n.assign(n, n + P);
+  b->off.resize(P);
+  int64_t rows = 0, nnz = 0, rp_off = 0;
+  for (int p = 0; p < P; p++) {
+    SA_CHECK(n[p] > 0, "empty problem in batch");
+    b->off[p] = rows;
+    rows += (n[p] + 63) / 64 * 64;
+    nnz += rowptr[rp_off + n[p]];
+    rp_off += n[p] + 1;
+  }
+  SA_CHECK(nnz < INT32_MAX && rows < INT32_MAX, "batch too large for int32 CSR indices");
+  b->nrows = rows;
+  b->nnz = nnz;
+  b->rp.assign(rows + 1, 0);
+  b->ci.resize(nnz);
+  b->val.resize(nnz);
+  int64_t z = 0, vo = 0;
+  rp_off = 0;
+  for (int p = 0; p < P; p++) {
+    const int32_t *rpp = rowptr + rp_off;
+    const int64_t base = b->off[p];
+    for (int r = 0; r < n[p]; r++) {
+      b->rp[base + r] = (int32_t)z;
+      for (int32_t q = rpp[r]; q < rpp[r + 1]; q++) {
+        SA_CHECK(col[vo + q] >= 0 && col[vo + q] < n[p], "column index out of range in batch");
+        b->ci[z] = (int32_t)(base + col[vo + q]);
+        b->val[z] = val[vo + q];
+        z++;
+      }
+    }
+    const int64_t padded_end = (p + 1 < P) ? b->off[p + 1] : rows;
+    for (int64_t r = base + n[p]; r < padded_end; r++) b->rp[r] = (int32_t)z;
+    vo += rpp[n[p]];
+    rp_off += n[p] + 1;
+  }
+  b->rp[rows] = (int32_t)z;
+  *out = b;
+  return 0;
+}
+
+int sa_batch_destroy(sa_batch b) {
+  if (!b) return 0;
+  free_dev(b->dev64);
+  free_dev(b->dev32);
+  delete b;
+  return 0;
+}
+
+// Solves every problem of the batch: nev eigenpairs of `which`, ncv basis
+// vectors, tolerance tol.  mixed != 0: FP32 bulk to tol32, then FP64 polish from
+// the sum of the FP32 Ritz vectors.  v0 (optional): concatenated start vectors
+// (problem p at offset sum_{q<p} n_q); NULL draws U[-1,1) with seed
+// sa_sub_seed(seed, p).
+int sa_batch_solve(sa_batch b, int which, int nev, int ncv, double tol, int max_it, int mixed, double tol32,
+                   const double *v0, uint64_t seed) {
+  SA_TRY(ensure_init());
+  SA_CHECK(b && nev > 0 && ncv > nev && ncv < 64 && tol > 0, "invalid batch solve arguments (ncv < 64)");
+  const int P = b->P, m = ncv + 1;
+  hipStream_t s = stream();
+  if (max_it <= 0) max_it = 100000;
+  const double t0 = nowb();
+  // start vectors into column 0 of the working basis
+  std::vector<double> x0((size_t)b->nrows, 0.0);
+  {
+    int64_t vo = 0;
+    for (int p = 0; p < P; p++) {
+      if (v0) std::memcpy(x0.data() + b->off[p], v0 + vo, sizeof(double) * b->n[p]);
+      else sa_rand_uniform(sa_sub_seed(seed, p), b->n[p], x0.data() + b->off[p]);
+      vo += b->n[p];
+    }
+  }
+  SA_TRY(alloc_dev(b, b->dev64, SA_FP64, m));
+  std::vector<ProbState> ps;
+  b->matvecs32 = b->passes32 = 0;
+  b->its_max32 = 0;
+  b->seconds32 = 0;
+  double tds = 0.0;
+  if (mixed) {
+    SA_TRY(alloc_dev(b, b->dev32, SA_FP32, m));
+    std::vector<float> f(x0.begin(), x0.end());
+    SA_HIP(hipMemcpyAsync(b->dev32.V, f.data(), sizeof(float) * b->nrows, hipMemcpyHostToDevice, s));
+    SA_HIP(hipStreamSynchronize(s));
+    const double t1 = nowb();
+    SA_TRY(batch_ks(b, b->dev32, which, nev, ncv, tol32, max_it, ps, b->matvecs32, b->passes32, b->its_max32, tds));
+    // polish seeds: sum of the first nev Ritz vectors of each problem, in FP64
+    SA_TRY(launch_convert(SA_FP32, b->dev32.V, b->dev32.ldv, SA_FP64, b->dev64.V, b->dev64.ldv, b->nrows, m, s));
+    std::vector<double> Qh((size_t)P * m * m, 0.0);
+    std::vector<int> ml(P, 0), mk(P, 0), msv(P, 0), mev(P, 0);
+    for (int p = 0; p < P; p++) {
+      std::vector<int> perm;
+      sort_perm(which, ps[p].eigr, ps[p].nconv, perm);
+      const int nr = std::min(nev, ps[p].nconv);
+      for (int i = 0; i < nr; i++) Qh[(size_t)p * m * m + perm[i]] = 1.0;   // column 0 of Q
+      if (nr == 0) Qh[(size_t)p * m * m] = 1.0;                             // keep the start vector
+      ml[p] = 0; mk[p] = std::max(1, ps[p].nconv); msv[p] = 0; mev[p] = 1;
+    }
+    BatchDev &B = b->dev64;
+    SA_HIP(hipMemcpyAsync(B.Q, Qh.data(), sizeof(double) * (size_t)P * m * m, hipMemcpyHostToDevice, s));
+    SA_HIP(hipMemcpyAsync(B.ml, ml.data(), sizeof(int) * P, hipMemcpyHostToDevice, s));
+    SA_HIP(hipMemcpyAsync(B.mk, mk.data(), sizeof(int) * P, hipMemcpyHostToDevice, s));
+    SA_HIP(hipMemcpyAsync(B.ms, msv.data(), sizeof(int) * P, hipMemcpyHostToDevice, s));
+    SA_HIP(hipMemcpyAsync(B.me, mev.data(), sizeof(int) * P, hipMemcpyHostToDevice, s));
+    hipLaunchKernelGGL(batched_mip_kernel<double>, dim3(B.nchunks), dim3(256), 0, s, (double *)B.V, B.ldv,
+                       B.chunk_prob, B.chunk_row, B.off, B.n, B.ml, B.mk, B.ms, B.me, B.Q, m);
+    SA_HIP(hipGetLastError());
+    SA_HIP(hipStreamSynchronize(s));
+    b->seconds32 = nowb() - t1;
+  } else {
+    SA_HIP(hipMemcpyAsync(b->dev64.V, x0.data(), sizeof(double) * b->nrows, hipMemcpyHostToDevice, s));
+    SA_HIP(hipStreamSynchronize(s));
+  }
+  SA_TRY(batch_ks(b, b->dev64, which, nev, ncv, tol, max_it, ps, b->matvecs, b->passes, b->its_max, tds));
+  b->nconv.resize(P);
+  b->its.resize(P);
+  b->reason.resize(P);
+  b->eig.resize(P);
+  b->err.resize(P);
+  b->perm.resize(P);
+  for (int p = 0; p < P; p++) {
+    b->nconv[p] = ps[p].nconv;
+    b->its[p] = ps[p].its;
+    b->reason[p] = ps[p].reason;
+    b->eig[p] = ps[p].eigr;
+    b->err[p] = ps[p].errest;
+    sort_perm(which, ps[p].eigr, ps[p].nconv, b->perm[p]);
+  }
+  b->seconds = nowb() - t0;
+  b->seconds_ds = tds;
+  b->solved = 1;
+  return 0;
+}
+
+int sa_batch_get_converged(sa_batch b, int p, int *nconv, int *its, int *reason) {
+  SA_CHECK(b && b->solved && p >= 0 && p < b->P, "invalid problem index");
+  if (nconv) *nconv = b->nconv[p];
+  if (its) *its = b->its[p];
+  if (reason) *reason = b->reason[p];
+  return 0;
+}
+
+int sa_batch_get_eigenpair(sa_batch b, int p, int i, double *re, double *x_host) {
+  SA_CHECK(b && b->solved && p >= 0 && p < b->P && i >= 0 && i < b->nconv[p], "invalid eigenpair index");
+  const int k = b->perm[p][i];
+  if (re) *re = b->eig[p][k];
+  if (x_host) {
+    const BatchDev &B = b->dev64;
+    SA_HIP(hipMemcpy(x_host, (double *)B.V + (size_t)k * B.ldv + b->off[p], sizeof(double) * b->n[p],
+                     hipMemcpyDeviceToHost));
+  }
+  return 0;
+}
+
+// Relative residual ||A x - lambda x|| / |lambda| of every requested pair on
+// the device in FP64 (EPSComputeError, epssolve.c:742-815); err[p*nev + i].
+int sa_batch_compute_errors(sa_batch b, int nev, double *err) {
+  SA_TRY(ensure_init());
+  SA_CHECK(b && b->solved && err, "batch not solved");
+  const BatchDev &B = b->dev64;
+  hipStream_t s = stream();
+  double *y = nullptr, *part = nullptr, *out = nullptr;
+  SA_HIP(hipMalloc(&y, sizeof(double) * b->nrows));
+  SA_HIP(hipMalloc(&part, sizeof(double) * 1024));
+  SA_HIP(hipMalloc(&out, sizeof(double)));
+  Mat A;
+  A.n = (int)b->nrows;
+  A.nnz = b->nnz;
+  A.prec = SA_FP64;
+  A.rowptr = B.rp;
+  A.col = B.ci;
+  A.val = B.val;
+  A.lanes = B.lanes;
+  int rc = 0;
+  for (int p = 0; p < b->P && !rc; p++) {
+    for (int i = 0; i < nev; i++) {
+      if (i >= b->nconv[p]) { err[(size_t)p * nev + i] = 1e300; continue; }
+      const int k = b->perm[p][i];
+      const double lam = b->eig[p][k];
+      const double *x = (const double *)B.V + (size_t)k * B.ldv;
+      Mat sub = A;   // rows of problem p only (column indices are global)
+      sub.n = b->n[p];
+      sub.rowptr = B.rp + b->off[p];
+      if ((rc = launch_spmv(sub, x, y + b->off[p], s))) break;
+      if ((rc = launch_axpy_norm(SA_FP64, y + b->off[p], x + b->off[p], lam, b->n[p], part, out, s))) break;
+      double h = 0.0;
+      if (hipMemcpyAsync(&h, out, sizeof(double), hipMemcpyDeviceToHost, s) != hipSuccess ||
+          hipStreamSynchronize(s) != hipSuccess) { rc = 1; break; }
+      err[(size_t)p * nev + i] = std::sqrt(h) / std::fabs(lam);
+    }
+  }
+  (void)hipFree(y);
+  (void)hipFree(part);
+  (void)hipFree(out);
+  return rc;
+}
+
+// matvecs, Gram-Schmidt passes, restart rounds (FP64 then FP32 phase), seconds.
+int sa_batch_get_stats(sa_batch b, int64_t *counters6, double *seconds3) {
+  SA_CHECK(b && b->solved, "batch not solved");
+  if (counters6) {
+    counters6[0] = b->matvecs; counters6[1] = b->passes; counters6[2] = b->its_max;
+    counters6[3] = b->matvecs32; counters6[4] = b->passes32; counters6[5] = b->its_max32;
+  }
+  if (seconds3) {
+    seconds3[0] = b->seconds; seconds3[1] = b->seconds32; seconds3[2] = b->seconds_ds;
+  }
+  return 0;
+}
+
+}  // extern "C"

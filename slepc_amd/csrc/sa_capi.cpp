@@ -2,6 +2,7 @@
 #include <algorithm>
 #include <chrono>
 #include <cmath>
+#include <cstdio>
 #include <cstring>
 #include <mutex>
 #include <string>
@@ -667,6 +668,45 @@ int sa_timer_stop(double *ms) {
   float f = 0.f;
   SA_HIP(hipEventElapsedTime(&f, g_ev0, g_ev1));
   *ms = f;
+  return 0;
+}
+}  // extern "C"
+
+// ---------------------------------------------------------------- DS entry
+extern "C" {
+// DSSolve + DSSort of a compact DSHEP problem (dshep.c:221-347,383-426): diagonal
+// d[0..n), off-diagonal/arrow e[0..n-1), locked rows l, arrowhead up to row k.
+int sa_ds_hep_compact(int n, int l, int k, const double *d, const double *e, int which, double *wr,
+                      double *Q) {
+  SA_CHECK(n > 0 && l >= 0 && l <= n && k >= 0 && k <= n && d && e && wr, "invalid DS arguments");
+  SA_CHECK(lapack_ready(), "no LAPACK library found for the projected problem");
+  DenseProblem ds;
+  const int ld = n + 1;
+  ds.allocate(ld, true, which);
+  std::memcpy(ds.d(), d, sizeof(double) * n);
+  std::memcpy(ds.e(), e, sizeof(double) * (n - 1));
+  ds.set_dimensions(n, l, k);
+  ds.set_state((l == 0 && k == 0) ? DS_INTERMEDIATE : DS_RAW);
+  std::vector<double> wi(ld, 0.0);
+  SA_CHECK(ds.solve(wr, wi.data()) == 0, "steqr failed");
+  ds.sort(wr, wi.data());
+  if (Q) std::memcpy(Q, ds.Q(), sizeof(double) * (size_t)ld * ld);
+  return 0;
+}
+// Which LAPACK library the projected problem uses (path into buf).
+int sa_lapack_info(char *buf, int len) {
+  std::string p;
+  const bool ok = lapack_ready(&p);
+  if (buf && len > 0) std::snprintf(buf, (size_t)len, "%s", p.c_str());
+  return ok ? 0 : 1;
+}
+}  // extern "C"
+
+extern "C" {
+// Total Gram-Schmidt passes run on this basis so far (device counter).
+int sa_bv_get_pass_count(sa_bv B, int64_t *passes) {
+  SA_CHECK(B && passes, "null argument");
+  *passes = B->V.passes_done;
   return 0;
 }
 }  // extern "C"

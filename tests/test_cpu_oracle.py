@@ -1,0 +1,265 @@
+"""CPU tests: the oracle pinned against the reference's golden outputs, closed
+forms and independent library results; host-side generators of the product
+bit-exact against the oracle; the C ABI exports what include/slepc_amd.h declares."""
+import ctypes as C
+import json
+import os
+import re
+
+import numpy as np
+import pytest
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(HERE)
+GOLD = json.load(open(os.path.join(HERE, "golden", "reference_outputs.json")))
+LIBRES = json.load(open(os.path.join(HERE, "golden", "library_results.json")))
+
+
+def lap_exact(nx, ny):
+    return np.sort([4 - 2 * np.cos(i * np.pi / (ny + 1)) - 2 * np.cos(j * np.pi / (nx + 1))
+                    for i in range(1, ny + 1) for j in range(1, nx + 1)])[::-1]
+
+
+# ------------------------------------------------------------------ BV ops (test1.c golden)
+def test_oracle_bv_test1(orc):
+    g = GOLD["bv_test1"]
+    X0 = np.array(g["X0"])
+    Y = np.array(g["Y_after_mult"])
+    z = np.array([2.0, -1.0, 0.5, -0.25, 0.125])
+    y0 = orc.bv_multvec(X0, -1.0, z, 1.0, Y[:, 0])
+    assert np.array_equal(y0, np.array(g["Y_after_multvec"])[:, 0])
+    assert np.array_equal(orc.bv_dotvec(X0, y0), np.array(g["z_after_dotvec"]))
+    X = orc.bv_multinplace(X0, np.array(g["Q"]), 1, 3)
+    X *= 2.0
+    assert np.array_equal(X, np.array(g["X_after_multinplace_scale"]))
+    assert abs(np.linalg.norm(X[:, 0]) - g["norm2_X0"]) < 5e-5
+    assert abs(np.linalg.norm(X) - g["normF_X"]) < 5e-5
+
+
+def test_oracle_cgs_dgks(orc):
+    """CGS+DGKS: orthonormal to < 100 eps (test2.c criterion); h holds the
+    projection coefficients and the norm; an exactly dependent column is
+    reduced to rounding noise."""
+    rng = np.random.default_rng(3)
+    n, k = 300, 10
+    X = rng.standard_normal((n, k))
+    X[:, 6] = X[:, 2] + 1e-10 * X[:, 6]
+    V = X.copy(order="F")
+    for j in range(k):
+        v_before = V[:, j].copy()
+        V, nrm, lind, h = orc.bv_orthonormalize_column(V, j)
+        assert not lind
+        # v_before = sum_i h_i v_i + nrm * v_j
+        recon = V[:, :j] @ h[:j] + h[j] * V[:, j]
+        assert np.linalg.norm(recon - v_before) < 1e-12 * np.linalg.norm(v_before)
+    assert np.linalg.norm(V.T @ V - np.eye(k), 1) < 100 * 2.2e-16
+
+
+def test_oracle_arrow_tridiag(orc):
+    """DSArrowTridiag: the orthogonal Q reduces the arrowhead to tridiagonal
+    form with the same spectrum."""
+    rng = np.random.default_rng(1)
+    n = 9
+    d = rng.standard_normal(n)
+    e = rng.standard_normal(n - 1)
+    A = np.diag(d)
+    A[n - 1, :n - 1] = e
+    A[:n - 1, n - 1] = e
+    dd, ee, Q = orc.arrow_tridiag(d, e)
+    T = np.diag(dd) + np.diag(ee[:n - 1], 1) + np.diag(ee[:n - 1], -1)
+    assert np.linalg.norm(Q.T @ Q - np.eye(n)) < 1e-14
+    assert np.linalg.norm(Q.T @ A @ Q - T) < 1e-13 * np.linalg.norm(A)
+
+
+def test_oracle_ds_test3_golden(orc):
+    """src/sys/classes/ds/tests/test3.c: d_i = i+1, e_i = 1 for i >= l, n=9 l=2 k=5."""
+    g = GOLD["ds_test3"]
+    n, l, k = g["n"], g["l"], g["k"]
+    d = np.arange(1, n + 1, dtype=float)
+    e = np.array([0.0 if i < l else 1.0 for i in range(n - 1)])
+    wr = np.zeros(n + 1)
+    Q = np.zeros((n + 1) * (n + 1))
+    rc = orc.lib().orc_ds_hep_compact(n, l, k, d.ctypes.data_as(C.c_void_p), e.ctypes.data_as(C.c_void_p),
+                                      1, wr.ctypes.data_as(C.c_void_p), Q.ctypes.data_as(C.c_void_p))
+    assert rc == 0
+    assert np.allclose(np.round(wr[:n], 5), g["eigs"], atol=1e-12)
+    # the same numbers from an independent dense eigensolver (arrowhead + tridiagonal)
+    A = np.diag(d)
+    for i in range(l, k):
+        A[k, i] = A[i, k] = e[i]
+    for i in range(k, n - 1):
+        A[i, i + 1] = A[i + 1, i] = e[i]
+    w = np.linalg.eigvalsh(A[l:, l:])
+    assert np.allclose(np.sort(wr[l:n]), np.sort(w), atol=1e-12)
+
+
+# ------------------------------------------------------------------ whole solves vs goldens
+def test_oracle_ex2_golden(orc):
+    g = GOLD["ex2_1"]
+    A = orc.lap2d(g["n"])
+    r = orc.eps_solve(A, g["nev"], g["ncv"], 1e-8, "LM", True, v0=orc.rand_uniform(0, A.n))
+    assert np.allclose(np.round(r["eigr"][:4], 5), g["values"], atol=1e-12)
+    ex = lap_exact(g["n"], g["n"])
+    for v in r["eigr"]:
+        assert np.min(np.abs(ex - v)) < 1e-8 * v
+    for i in range(r["nconv"]):
+        assert orc.residual_rel(A, r["eigr"][i], 0.0, r["vecs"][:, i]) < 1e-8
+
+
+def test_oracle_ex5_golden_nhep(orc):
+    g = GOLD["ex5_1"]
+    A = orc.markov(g["m"])
+    v0 = np.zeros(A.n)
+    v0[:3] = 1
+    r = orc.eps_solve(A, g["nev"], 0, 1e-8, "LR", False, v0=v0)
+    assert np.allclose(np.round(r["eigr"][:4], 5), g["values"], atol=1e-12)
+    assert abs(r["eigr"][0] - 1.0) < 1e-10          # stochastic matrix: rightmost = 1
+    for i in range(r["nconv"]):
+        assert orc.residual_rel(A, r["eigr"][i], r["eigi"][i], r["vecs"][:, i]) < 1e-8
+
+
+def test_oracle_ex4_rdb200_golden(orc):
+    g = GOLD["ex4_1"]
+    d = np.load(os.path.join(HERE, "golden", "rdb200.npz"))
+    A = orc.CSR(int(d["n"]), d["rowptr"], d["col"], d["val"])
+    r = orc.eps_solve(A, g["nev"], 0, 1e-8, "LM", False, v0=orc.rand_uniform(0, A.n))
+    assert np.allclose(np.round(r["eigr"][:4], 5), g["values"], atol=1e-12)
+    assert r["its"] == g["its"]
+    w = np.linalg.eigvals(A.to_dense())
+    for v in r["eigr"]:
+        assert np.min(np.abs(w - v)) < 1e-8 * abs(v)
+
+
+def test_oracle_test27_closed_form(orc):
+    g = GOLD["test27_1"]
+    A = orc.lap2d(g["n"])
+    r = orc.eps_solve(A, g["nev"], 0, 1e-10, "SR", True, v0=orc.rand_uniform(0, A.n))
+    ex = lap_exact(10, 10)
+    for v in r["eigr"]:
+        assert np.min(np.abs(ex - v)) < 1e-9
+    # distinct values printed by the reference test appear among the computed ones
+    for v in sorted(set(g["values"]))[:3]:
+        assert np.min(np.abs(r["eigr"] - v)) < 1e-5
+
+
+def test_oracle_convdiff_closed_form(orc):
+    """Largest real eigenvalue of the 7-point convection-diffusion operator:
+    6 + 2 sum_d sqrt((1+0.1 b_d)(1-0.1 b_d)) cos(pi/(N+1))."""
+    N = 14
+    A = orc.convdiff3d(N)
+    r = orc.eps_solve(A, 20, 64, 1e-10, "LR", False, v0=orc.rand_uniform(0, A.n))
+    b = np.array([1.0, 0.5, 0.25])
+    top = 6 + 2 * np.sum(np.sqrt((1 + 0.1 * b) * (1 - 0.1 * b))) * np.cos(np.pi / (N + 1))
+    assert abs(r["eigr"][0] - top) < 1e-10 * top
+    assert np.max(np.abs(r["eigi"])) < 1e-8
+
+
+def test_oracle_library_results(orc):
+    """Independent dense LAPACK results (written to fixtures by make_fixtures.py)."""
+    for f in LIBRES["fv2d"]:
+        A = orc.fv2d(f["nx"], f["ny"], np.array(f["theta"]))
+        assert abs(A.sigma - f["sigma"]) <= 1e-13 * f["sigma"]
+        r = orc.eps_solve(A, 16, 64, 1e-10, "LR", True, v0=orc.rand_uniform(0, A.n))
+        ref = np.array(f["largest16"])
+        for v in r["eigr"][:16]:
+            assert np.min(np.abs(ref - v)) < 1e-9 * v
+    C12 = orc.convdiff3d(12)
+    r = orc.eps_solve(C12, 20, 64, 1e-10, "LR", False, v0=orc.rand_uniform(0, C12.n))
+    ref = np.array(LIBRES["convdiff12"]["largest_real20"])
+    for v in r["eigr"][:20]:
+        assert np.min(np.abs(ref - v)) < 1e-9 * v
+    T = orc.stencil27(10, seed=1)
+    r = orc.eps_solve(T, 32, 96, 1e-10, "LM", True, v0=orc.rand_uniform(0, T.n))
+    ref = np.array(LIBRES["stencil27_10"]["largest_mag32"])
+    for v in r["eigr"][:32]:
+        assert np.min(np.abs(ref - v)) < 1e-9 * abs(v)
+
+
+def test_oracle_planted_mistake_is_caught(orc):
+    """A wrong stencil constant and a dropped neighbour term (planted in the
+    oracle's input, on the CPU) move the computed spectrum off the closed
+    form: the closed-form checks above are able to fail.  (Flipping every
+    off-diagonal sign would not: that matrix is similar to the original.)"""
+    A = orc.lap2d(12)
+    A.val = np.where(A.val == 4.0, 3.999, A.val)     # planted mistake 1
+    A.val[A.rowptr[5]] = 0.0                          # planted mistake 2
+    r = orc.eps_solve(A, 4, 0, 1e-10, "SR", True, v0=orc.rand_uniform(0, A.n))
+    ex = lap_exact(12, 12)
+    assert min(np.min(np.abs(ex - v)) for v in r["eigr"]) > 1e-6 or \
+        max(np.min(np.abs(ex - v)) for v in r["eigr"]) > 1e-6
+
+
+# ------------------------------------------------------------------ product host code (no GPU)
+def _same(a, b):
+    return (a.n == b.n and np.array_equal(a.rowptr, b.rowptr) and np.array_equal(a.col, b.col)
+            and a.val.tobytes() == b.val.tobytes())
+
+
+def test_generators_bit_exact(orc):
+    import slepc_amd as sa
+    assert _same(sa.lap2d(17, 13), orc.lap2d(17, 13))
+    assert _same(sa.lap2d(31, 40, 10.0, 7), orc.lap2d_pot(31, 40, 7, 10.0))
+    assert _same(sa.convdiff3d(9, 7, 5), orc.convdiff3d(9, 7, 5))
+    assert _same(sa.stencil27(6, 5, 4, 1), orc.stencil27(6, 5, 4, 1))
+    assert _same(sa.markov(15), orc.markov(15))
+    for s in range(3):
+        th = orc.fv_theta(5, s)
+        assert np.array_equal(th, sa.fv_theta(5, s))
+        a, b = sa.fv2d(40, 33, th), orc.fv2d(40, 33, th)
+        assert _same(a, b) and a.sigma == b.sigma
+    assert np.array_equal(sa.rand_uniform(0, 1000), orc.rand_uniform(0, 1000))
+
+
+def test_generators_full_size_bit_exact(orc):
+    """Workload matrices at their full sizes, assembled by the product and the oracle."""
+    import slepc_amd as sa
+    a, b = sa.convdiff3d(128), orc.convdiff3d(128)
+    assert a.nnz == 14581760 and _same(a, b)
+    del a, b
+    a, b = sa.stencil27(128, seed=1), orc.stencil27(128, seed=1)
+    assert a.nnz == 55742968 and _same(a, b)
+    del a, b
+    th = orc.fv_theta(5, 0)
+    a, b = sa.fv2d(256, 256, th), orc.fv2d(256, 256, th)
+    assert a.nnz == 326656 and _same(a, b)
+    assert sa.lap2d(100).nnz == 49600
+
+
+def test_csr_assemble_insert_values():
+    """MatSetValues(INSERT_VALUES)+MatAssemblyEnd: sorted columns, last value wins."""
+    import slepc_amd as sa
+    rows = [2, 0, 0, 1, 0, 2, 0]
+    cols = [1, 2, 0, 1, 2, 0, 1]
+    vals = [5., 1., 2., 3., 4., 6., 7.]
+    A = sa.csr_assemble(3, rows, cols, vals)
+    assert A.rowptr.tolist() == [0, 3, 4, 6]
+    assert A.col.tolist() == [0, 1, 2, 1, 0, 1]
+    assert A.val.tolist() == [2., 7., 4., 3., 6., 5.]
+    with pytest.raises(sa.SlepcAmdError):
+        sa.csr_assemble(3, [3], [0], [1.0])
+
+
+def test_product_ds_hep_golden():
+    """The product's host DS (sa_ds_hep_compact) against ds/tests test3 golden."""
+    import slepc_amd as sa
+    g = GOLD["ds_test3"]
+    n, l, k = g["n"], g["l"], g["k"]
+    d = np.arange(1, n + 1, dtype=float)
+    e = np.array([0.0 if i < l else 1.0 for i in range(n - 1)])
+    wr = np.zeros(n + 1)
+    L = sa.lib()
+    L.sa_ds_hep_compact.argtypes = [C.c_int, C.c_int, C.c_int, C.c_void_p, C.c_void_p, C.c_int, C.c_void_p,
+                                    C.c_void_p]
+    assert L.sa_ds_hep_compact(n, l, k, d.ctypes.data, e.ctypes.data, 1, wr.ctypes.data, None) == 0
+    assert np.allclose(np.round(wr[:n], 5), g["eigs"], atol=1e-12)
+
+
+def test_c_abi_exports_every_declared_symbol():
+    import slepc_amd as sa
+    hdr = open(os.path.join(ROOT, "include", "slepc_amd.h")).read()
+    hdr = re.sub(r"/\*.*?\*/", "", hdr, flags=re.S)
+    names = sorted(set(re.findall(r"\b(sa_[a-z0-9_]+)\s*\(", hdr)))
+    assert len(names) > 40
+    L = sa.lib()
+    missing = [nm for nm in names if not hasattr(L, nm)]
+    assert not missing, missing
