@@ -28,18 +28,16 @@ int bv_alloc(BV &V, int64_t n, int m, int prec) {
   SA_HIP(hipHostMalloc((void **)&V.st_host, sizeof(GsState)));
   // Partition rows into workgroup ranges of whole 64-row chunks.
   const int64_t chunks = std::max<int64_t>(1, (n + 63) / 64);
+  // at most 1024 workgroups and at least 4 chunks each, so the one-workgroup
+  // reduction of the partial sums stays short for small problems
   const int64_t maxG = 1024;
-  int64_t G = std::min(maxG, chunks);
+  int64_t G = std::min(maxG, std::max<int64_t>(1, (chunks + 3) / 4));
   const int64_t cpb = (chunks + G - 1) / G;
   G = (chunks + cpb - 1) / cpb;
   V.G = (int)G;
   V.rows_per_block = (int)(cpb * 64);
   V.ws.maxG = (int)G;
-  const int ngroups = (int)((G + 31) / 32);
   SA_HIP(hipMalloc(&V.ws.part, sizeof(double) * (size_t)G * (GS_MAXC + 1)));
-  SA_HIP(hipMalloc(&V.ws.gpart, sizeof(double) * (size_t)ngroups * (GS_MAXC + 1)));
-  SA_HIP(hipMalloc(&V.ws.cnt, sizeof(unsigned) * (size_t)(1 + ngroups)));
-  SA_HIP(hipMemsetAsync(V.ws.cnt, 0, sizeof(unsigned) * (size_t)(1 + ngroups), stream()));
   SA_HIP(hipMalloc(&V.dq, sizeof(double) * (size_t)m * m + 64 * sizeof(double)));
   SA_HIP(hipHostMalloc((void **)&V.hq, sizeof(double) * (size_t)m * m + 64 * sizeof(double)));
   SA_HIP(hipStreamSynchronize(stream()));
@@ -52,8 +50,6 @@ void bv_free(BV &V) {
   if (V.st) (void)hipFree(V.st);
   if (V.st_host) (void)hipHostFree(V.st_host);
   if (V.ws.part) (void)hipFree(V.ws.part);
-  if (V.ws.gpart) (void)hipFree(V.ws.gpart);
-  if (V.ws.cnt) (void)hipFree(V.ws.cnt);
   if (V.dq) (void)hipFree(V.dq);
   if (V.hq) (void)hipHostFree(V.hq);
   V = BV();

@@ -4,6 +4,7 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <atomic>
 #include <mutex>
 #include <string>
 #include <vector>
@@ -16,8 +17,8 @@ namespace sa {
 
 namespace {
 thread_local std::string g_err;
-hipStream_t g_stream = nullptr;
-int g_device = -1;
+thread_local hipStream_t t_stream = nullptr;   // one stream per host thread
+std::atomic<int> g_device{-1};
 int g_cus = 0;
 std::mutex g_init_mu;
 double now() {
@@ -26,11 +27,20 @@ double now() {
 }  // namespace
 
 void set_error(const std::string &msg) { g_err = msg; }
-hipStream_t stream() { return g_stream; }
+// Each host thread works on its own non-blocking stream, so independent solves
+// issued from several threads overlap on the GPU.
+hipStream_t stream() {
+  if (!t_stream && g_device >= 0) {
+    if (hipSetDevice(g_device) != hipSuccess ||
+        hipStreamCreateWithFlags(&t_stream, hipStreamNonBlocking) != hipSuccess)
+      t_stream = nullptr;
+  }
+  return t_stream;
+}
 int num_cus() { return g_cus; }
 
 int ensure_init() {
-  if (g_stream) return 0;
+  if (g_device >= 0 && stream()) return 0;
   return sa_init(0);
 }
 
@@ -64,7 +74,7 @@ extern "C" {
 
 int sa_init(int device) {
   std::lock_guard<std::mutex> lk(g_init_mu);
-  if (g_stream && device == g_device) return 0;
+  if (g_device == device && t_stream) return 0;
   int ndev = 0;
   SA_HIP(hipGetDeviceCount(&ndev));
   SA_CHECK(ndev > 0, "no HIP device found");
@@ -75,17 +85,18 @@ int sa_init(int device) {
   SA_CHECK(std::string(prop.gcnArchName).rfind("gfx950", 0) == 0,
            std::string("libslepc_amd is built for gfx950, found ") + prop.gcnArchName);
   g_cus = prop.multiProcessorCount;
-  if (g_stream) (void)hipStreamDestroy(g_stream);
-  SA_HIP(hipStreamCreateWithFlags(&g_stream, hipStreamNonBlocking));
+  if (t_stream) (void)hipStreamDestroy(t_stream);
+  t_stream = nullptr;
   g_device = device;
+  SA_CHECK(stream() != nullptr, "could not create the HIP stream");
   return 0;
 }
 
 const char *sa_last_error(void) { return g_err.c_str(); }
-void *sa_stream(void) { return (void *)g_stream; }
+void *sa_stream(void) { return (void *)stream(); }
 int sa_synchronize(void) {
   API_INIT();
-  SA_HIP(hipStreamSynchronize(g_stream));
+  SA_HIP(hipStreamSynchronize(stream()));
   return 0;
 }
 int sa_malloc(void **p, int64_t bytes) {
@@ -99,14 +110,14 @@ int sa_free(void *p) {
 }
 int sa_memcpy_h2d(void *dst, const void *src, int64_t bytes) {
   API_INIT();
-  SA_HIP(hipMemcpyAsync(dst, src, (size_t)bytes, hipMemcpyHostToDevice, g_stream));
-  SA_HIP(hipStreamSynchronize(g_stream));
+  SA_HIP(hipMemcpyAsync(dst, src, (size_t)bytes, hipMemcpyHostToDevice, stream()));
+  SA_HIP(hipStreamSynchronize(stream()));
   return 0;
 }
 int sa_memcpy_d2h(void *dst, const void *src, int64_t bytes) {
   API_INIT();
-  SA_HIP(hipMemcpyAsync(dst, src, (size_t)bytes, hipMemcpyDeviceToHost, g_stream));
-  SA_HIP(hipStreamSynchronize(g_stream));
+  SA_HIP(hipMemcpyAsync(dst, src, (size_t)bytes, hipMemcpyDeviceToHost, stream()));
+  SA_HIP(hipStreamSynchronize(stream()));
   return 0;
 }
 
@@ -133,7 +144,7 @@ int sa_mat_create_csr(int n, int64_t nnz, const int32_t *rp, const int32_t *ci, 
   const double avg = n ? (double)nnz / n : 1.0;
   A.lanes = avg <= 4.5 ? 4 : (avg <= 9 ? 8 : (avg <= 18 ? 16 : 32));
   const size_t es = prec_size(prec);
-  hipStream_t s = g_stream;
+  hipStream_t s = stream();
   if (hipMalloc(&A.rowptr, sizeof(int32_t) * (n + 1)) != hipSuccess ||
       hipMalloc(&A.col, sizeof(int32_t) * std::max<int64_t>(1, nnz)) != hipSuccess ||
       hipMalloc(&A.val, es * std::max<int64_t>(1, nnz)) != hipSuccess) {
@@ -177,7 +188,7 @@ int sa_mat_get_info(sa_mat M, int *n, int64_t *nnz, int *prec) {
 int sa_mat_mult(sa_mat M, const void *x, void *y) {
   API_INIT();
   SA_CHECK(M && x && y, "null argument to sa_mat_mult");
-  return launch_spmv(M->A, x, y, g_stream);
+  return launch_spmv(M->A, x, y, stream());
 }
 
 // ---------------------------------------------------------------- BV
@@ -250,9 +261,9 @@ int sa_bv_dotvec(sa_bv B, const void *x, double *q) {
   BV &V = B->V;
   const int nc = V.k - V.l;
   double *qd = q ? V.dq : V.dbuf;   // q == NULL: column 0 of the coefficient buffer
-  SA_TRY(launch_dotvec(V, V.l, V.k, x, qd, g_stream));
-  if (q && nc > 0) SA_HIP(hipMemcpyAsync(q, V.dq, sizeof(double) * nc, hipMemcpyDeviceToHost, g_stream));
-  SA_HIP(hipStreamSynchronize(g_stream));
+  SA_TRY(launch_dotvec(V, V.l, V.k, x, qd, stream()));
+  if (q && nc > 0) SA_HIP(hipMemcpyAsync(q, V.dq, sizeof(double) * nc, hipMemcpyDeviceToHost, stream()));
+  SA_HIP(hipStreamSynchronize(stream()));
   return 0;
 }
 
@@ -265,12 +276,12 @@ int sa_bv_multvec(sa_bv B, double alpha, double beta, void *y, const double *q) 
   if (q) {
     if (nc > 0) {
       std::memcpy(V.hq, q, sizeof(double) * nc);
-      SA_HIP(hipMemcpyAsync(V.dq, V.hq, sizeof(double) * nc, hipMemcpyHostToDevice, g_stream));
+      SA_HIP(hipMemcpyAsync(V.dq, V.hq, sizeof(double) * nc, hipMemcpyHostToDevice, stream()));
     }
     qd = V.dq;
   }
-  SA_TRY(launch_multvec(V, V.l, V.k, alpha, qd, beta, y, g_stream));
-  SA_HIP(hipStreamSynchronize(g_stream));
+  SA_TRY(launch_multvec(V, V.l, V.k, alpha, qd, beta, y, stream()));
+  SA_HIP(hipStreamSynchronize(stream()));
   return 0;
 }
 
@@ -283,9 +294,9 @@ int sa_bv_multinplace(sa_bv B, const double *Q, int ldq, int s_, int e) {
   const int64_t bytes = sizeof(double) * (int64_t)ldq * e;   // Q holds at least e columns
   SA_CHECK(bytes <= (int64_t)sizeof(double) * V.m * V.m + 64 * (int64_t)sizeof(double), "Q too large");
   std::memcpy(V.hq, Q, bytes);
-  SA_HIP(hipMemcpyAsync(V.dq, V.hq, bytes, hipMemcpyHostToDevice, g_stream));
-  SA_TRY(launch_multinplace(V, V.l, V.k, s_, e, V.dq, ldq, g_stream));
-  SA_HIP(hipStreamSynchronize(g_stream));
+  SA_HIP(hipMemcpyAsync(V.dq, V.hq, bytes, hipMemcpyHostToDevice, stream()));
+  SA_TRY(launch_multinplace(V, V.l, V.k, s_, e, V.dq, ldq, stream()));
+  SA_HIP(hipStreamSynchronize(stream()));
   return 0;
 }
 
@@ -294,11 +305,11 @@ int sa_bv_scale_column(sa_bv B, int j, double alpha) {
   SA_CHECK(B && j < B->V.m, "invalid column");
   BV &V = B->V;
   if (j >= 0) {
-    SA_TRY(launch_scale(V.prec, col_ptr(V, j), V.n, alpha, g_stream));
+    SA_TRY(launch_scale(V.prec, col_ptr(V, j), V.n, alpha, stream()));
   } else {
-    for (int c = V.l; c < V.k; c++) SA_TRY(launch_scale(V.prec, col_ptr(V, c), V.n, alpha, g_stream));
+    for (int c = V.l; c < V.k; c++) SA_TRY(launch_scale(V.prec, col_ptr(V, c), V.n, alpha, stream()));
   }
-  SA_HIP(hipStreamSynchronize(g_stream));
+  SA_HIP(hipStreamSynchronize(stream()));
   return 0;
 }
 
@@ -307,10 +318,10 @@ int sa_bv_norm_column(sa_bv B, int j, double *nrm) {
   SA_CHECK(B && nrm && j >= 0 && j < B->V.m, "invalid column");
   BV &V = B->V;
   double *out = V.dq;
-  SA_TRY(launch_dot_self(V.prec, col_ptr(V, j), V.n, out, V.ws.part, g_stream));
+  SA_TRY(launch_dot_self(V.prec, col_ptr(V, j), V.n, out, V.ws.part, stream()));
   double h = 0.0;
-  SA_HIP(hipMemcpyAsync(&h, out, sizeof(double), hipMemcpyDeviceToHost, g_stream));
-  SA_HIP(hipStreamSynchronize(g_stream));
+  SA_HIP(hipMemcpyAsync(&h, out, sizeof(double), hipMemcpyDeviceToHost, stream()));
+  SA_HIP(hipStreamSynchronize(stream()));
   *nrm = std::sqrt(h);
   return 0;
 }
@@ -334,8 +345,8 @@ int sa_bv_copy_column(sa_bv B, int src, int dst) {
   SA_CHECK(B && src >= 0 && dst >= 0 && src < B->V.m && dst < B->V.m, "invalid column");
   BV &V = B->V;
   SA_HIP(hipMemcpyAsync(col_ptr(V, dst), col_ptr(V, src), prec_size(V.prec) * V.n, hipMemcpyDeviceToDevice,
-                        g_stream));
-  SA_HIP(hipStreamSynchronize(g_stream));
+                        stream()));
+  SA_HIP(hipStreamSynchronize(stream()));
   return 0;
 }
 
@@ -343,8 +354,8 @@ int sa_bv_mat_mult_column(sa_bv B, sa_mat M, int j) {
   API_INIT();
   SA_CHECK(B && M && j >= 0 && j + 1 < B->V.m, "invalid column");
   SA_CHECK(M->A.prec == B->V.prec && M->A.n == B->V.n, "matrix and basis differ in size or precision");
-  SA_TRY(launch_spmv(M->A, col_ptr(B->V, j), col_ptr(B->V, j + 1), g_stream));
-  SA_HIP(hipStreamSynchronize(g_stream));
+  SA_TRY(launch_spmv(M->A, col_ptr(B->V, j), col_ptr(B->V, j + 1), stream()));
+  SA_HIP(hipStreamSynchronize(stream()));
   return 0;
 }
 
@@ -360,8 +371,8 @@ int sa_bv_get_coefficients(sa_bv B, int c0, int c1, double *buf) {
   const int m = B->V.m;
   if (c1 > c0)
     SA_HIP(hipMemcpyAsync(buf, B->V.dbuf + (size_t)c0 * m, sizeof(double) * m * (c1 - c0), hipMemcpyDeviceToHost,
-                          g_stream));
-  SA_HIP(hipStreamSynchronize(g_stream));
+                          stream()));
+  SA_HIP(hipStreamSynchronize(stream()));
   return 0;
 }
 
@@ -591,7 +602,7 @@ int sa_eps_compute_error(sa_eps E, int i, int type, double *error) {
   const int k = r.perm[i];
   BV &V = E->V;
   const Mat &A = E->A->A;
-  hipStream_t s = g_stream;
+  hipStream_t s = stream();
   double *tmp = nullptr, *out = nullptr;
   SA_HIP(hipMalloc((void **)&tmp, sizeof(double) * std::max<int64_t>(1, V.n) * 2));
   SA_HIP(hipMalloc((void **)&out, sizeof(double) * 2));
@@ -647,7 +658,7 @@ int sa_eps_get_stats(sa_eps E, sa_eps_stats *st) {
 
 // ---------------------------------------------------------------- timing
 namespace {
-hipEvent_t g_ev0 = nullptr, g_ev1 = nullptr;
+thread_local hipEvent_t g_ev0 = nullptr, g_ev1 = nullptr;
 }
 extern "C" {
 // HIP events recorded on the library stream around a timed region.
@@ -657,13 +668,13 @@ int sa_timer_start(void) {
     SA_HIP(hipEventCreate(&g_ev0));
     SA_HIP(hipEventCreate(&g_ev1));
   }
-  SA_HIP(hipEventRecord(g_ev0, g_stream));
+  SA_HIP(hipEventRecord(g_ev0, stream()));
   return 0;
 }
 int sa_timer_stop(double *ms) {
   API_INIT();
   SA_CHECK(g_ev0 && ms, "timer not started");
-  SA_HIP(hipEventRecord(g_ev1, g_stream));
+  SA_HIP(hipEventRecord(g_ev1, stream()));
   SA_HIP(hipEventSynchronize(g_ev1));
   float f = 0.f;
   SA_HIP(hipEventElapsedTime(&f, g_ev0, g_ev1));

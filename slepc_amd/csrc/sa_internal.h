@@ -32,8 +32,8 @@ int num_cus();
 
 // ---- Gram-Schmidt step machine ---------------------------------------------
 // Column c is orthogonalised against columns 0..c-1 by up to five kernels
-// (slots 1..5).  Kernel slot i reads mode[i]; the last workgroup of a kernel
-// that reduces dot products writes the decision for slot i+1.
+// (slots 1..5).  Kernel slot i reads mode[i]; the reduction that follows a
+// kernel computing dot products writes the decision for slot i+1.
 constexpr int GS_MAXC = 128;        // largest column index + 1 handled
 constexpr int GS_SLOTS = 7;
 enum GsMode : int { GS_IDLE = 0, GS_MULT_SCALE = 1, GS_MULT_DOTS = 2, GS_SCALE_ONLY = 3 };
@@ -52,9 +52,7 @@ struct GsState {
 };
 
 struct Workspace {           // device scratch for partial sums of one BV
-  double *part = nullptr;    // [maxG][GS_MAXC+1]
-  double *gpart = nullptr;   // [maxGroups][GS_MAXC+1]
-  unsigned *cnt = nullptr;   // [1 + maxGroups]
+  double *part = nullptr;    // [GS_MAXC+1][maxG], column-major partial sums
   int maxG = 0;
 };
 

@@ -11,75 +11,126 @@ namespace sa {
 constexpr int BLK = 256;
 constexpr int CH = 64;          // rows per chunk (one row per lane of a wave)
 constexpr int LDP = CH + 1;     // padded LDS column stride
-constexpr int GROUP = 32;       // first level of the cross-workgroup reduction
-
-__device__ __forceinline__ double coherent_load(const double *p) {
-  return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-}
 __device__ __forceinline__ double safe_sqrt(double a) { return a < 0.0 ? 0.0 : sqrt(a); }
 
 // ---------------------------------------------------------------------------
-// Cross-workgroup reduction of per-workgroup partial sums part[b][0..ncol).
-// Two levels: the last workgroup of each group of GROUP sums its group, the
-// last group sums the group totals.  Summation order is fixed (deterministic).
-// Returns true in the single workgroup that holds the totals in out_lds.
+// Cross-workgroup reductions use no fences and no atomics (an agent-scope
+// release on MI355X writes back the XCD's L2, and atomics from 8 XCDs
+// serialise): every workgroup stores its partial sums column-major,
+// part[col * G + b]; the end of the kernel publishes them and a one-workgroup
+// kernel adds them in a fixed order (deterministic results).
 // ---------------------------------------------------------------------------
-__device__ bool reduce_partials(const double *acc_t, int ncol, double *part, double *gpart,
-                                unsigned *cnt, int G, double *out_lds, int *s_flag) {
-  const int t = threadIdx.x, b = blockIdx.x;
-  if (t < ncol) part[(size_t)b * ncol + t] = *acc_t;
-  __threadfence();
-  __syncthreads();
-  const int g = b / GROUP, ng = (G + GROUP - 1) / GROUP, gsz = min(GROUP, G - g * GROUP);
-  if (t == 0) *s_flag = (atomicAdd(&cnt[1 + g], 1u) == (unsigned)(gsz - 1));
-  __syncthreads();
-  if (!*s_flag) return false;
-  __threadfence();
-  if (t < ncol) {
-    double s = 0.0;
-    for (int bb = g * GROUP; bb < g * GROUP + gsz; bb++) s += coherent_load(&part[(size_t)bb * ncol + t]);
-    gpart[(size_t)g * ncol + t] = s;
+constexpr int RBLK = 1024;
+
+__device__ void sum_partials_block(const double *part, int G, int ncol, double *out) {
+  const int t = threadIdx.x, lane = t & 63, w = t >> 6, nw = blockDim.x >> 6;
+  for (int col = w; col < ncol; col += nw) {
+    const double *p = part + (size_t)col * G;
+    double s0 = 0.0, s1 = 0.0, s2 = 0.0, s3 = 0.0;   // four independent load chains
+    int b = lane;
+    for (; b + 192 < G; b += 256) {
+      s0 += p[b];
+      s1 += p[b + 64];
+      s2 += p[b + 128];
+      s3 += p[b + 192];
+    }
+    for (; b < G; b += 64) s0 += p[b];
+    double s = (s0 + s1) + (s2 + s3);
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) s += __shfl_xor(s, o, 64);
+    if (lane == 0) out[col] = s;
   }
-  if (t == 0) cnt[1 + g] = 0u;
-  __threadfence();
-  __syncthreads();
-  if (t == 0) *s_flag = (atomicAdd(&cnt[0], 1u) == (unsigned)(ng - 1));
-  __syncthreads();
-  if (!*s_flag) return false;
-  __threadfence();
-  if (t < ncol) {
-    double s = 0.0;
-    for (int gg = 0; gg < ng; gg++) s += coherent_load(&gpart[(size_t)gg * ncol + t]);
-    out_lds[t] = s;
-  }
-  if (t == 0) cnt[0] = 0u;
-  __syncthreads();
-  return true;
+}
+
+__global__ __launch_bounds__(RBLK) void colsum_kernel(const double *part, int G, int ncol, double *out) {
+  sum_partials_block(part, G, ncol, out);
 }
 
 // ---------------------------------------------------------------------------
-// Gram-Schmidt step kernel.
-//   first (slot 1): v = A V(:,c-1) (if rp) or v = V(:,c); dots V(:,0..c)^T v
-//   slot i > 1    : mode[i] decides: v = (v - V(:,0..c-1) coef) * scale
-//                   and/or dots of the updated v for the next pass.
-// The last workgroup evaluates the CGS1/DGKS rules of the reference
-// (bvorthog.c:91-217) and writes the decision for slot i+1.
+// Gram-Schmidt step for column c (classical Gram-Schmidt with DGKS refinement
+// "if needed", bvorthog.c:91-217, then the scaling of bvorthog.c:417-422).
+//   slot 1 (first): v = A V(:,c-1) (when rp) or v = V(:,c); dots V(:,0..c)^T v
+//   slot i > 1    : mode[i] decides: v = (v - V(:,0..c-1) coef) * scale and/or
+//                   the dots of the updated v for the next pass
+// A reduction kernel after each full-grid step evaluates the rules of the
+// reference and writes mode[i+1].  Slots 4-5 (third pass, failed norm
+// estimate) run as one workgroup that reduces in LDS.
 // ---------------------------------------------------------------------------
 template <typename T>
 struct GsArgs {
   int64_t n, ld;
   T *V;
-  int c, slot, first;
+  int c, slot, first, inblock;
   const int32_t *rp, *ci;
   const T *val;
   GsState *st;
   double *dbuf;
   int m;
-  double *part, *gpart;
-  unsigned *cnt;
+  double *part;
   int G, rpb;
   double eta;
 };
+
+// Decision after the dot products of one pass are known (dots[0..c], dots[c]
+// = v^T v).  Runs in one workgroup; thread 0 decides, all threads record.
+__device__ void gs_decide(GsState *st, const double *dots, int c, int slot, int first, double eta, double *dbuf,
+                          int m, int *s_record, int *s_out) {
+  const int t = threadIdx.x;
+  if (t == 0) {
+    const int out = slot + 1;
+    const double self = dots[c];
+    double sumsq = 0.0;
+    for (int i = 0; i < c; i++) sumsq += dots[i] * dots[i];
+    int mode_out = GS_IDLE, record = 0, fin = 0;
+    double scale_out = 1.0, nrm_f = 0.0, onrm_f = 0.0;
+    if (first) { st->pass = 0; st->pending = 0; st->lindep = 0; }
+    if (st->pending) {                      // true norm of the previous update
+      const double nrm = safe_sqrt(self);
+      st->pending = 0;
+      if (!(nrm != 0.0 && nrm < eta * st->onrm && st->pass < 3)) {
+        fin = 1; nrm_f = nrm; onrm_f = st->onrm; mode_out = GS_SCALE_ONLY;
+      }
+    }
+    if (!fin) {                              // one CGS1 pass (bvorthog.c:91-132)
+      st->pass++;
+      st->passes_total++;
+      record = 1;
+      const double onrm = safe_sqrt(self);
+      const double est = onrm * onrm - sumsq;
+      if (est <= 0.0) {
+        st->pending = 1; st->onrm = onrm; mode_out = GS_MULT_DOTS;
+      } else {
+        const double nrm = sqrt(est);
+        if (nrm != 0.0 && nrm < eta * onrm && st->pass < 3) {
+          st->onrm = onrm; mode_out = GS_MULT_DOTS;
+        } else {
+          fin = 1; nrm_f = nrm; onrm_f = onrm; mode_out = GS_MULT_SCALE;
+        }
+      }
+    }
+    if (fin) {                               // bvorthog.c:175-216 + 417-422
+      const int lind = !(nrm_f != 0.0 && fabs(nrm_f) >= eta * fabs(onrm_f));
+      st->lindep = lind;
+      st->nrm = nrm_f;
+      st->beta = nrm_f;
+      dbuf[(size_t)c * m + c] = lind ? 0.0 : nrm_f;
+      scale_out = (nrm_f != 0.0 && nrm_f != 1.0) ? 1.0 / nrm_f : 1.0;
+      if (lind) { st->stop = 1; st->stop_col = c; }
+    }
+    st->mode[out] = mode_out;
+    st->scale[out] = scale_out;
+    if (first)
+      for (int q = out + 1; q < GS_SLOTS; q++) st->mode[q] = GS_IDLE;
+    *s_record = record;
+    *s_out = out;
+  }
+  __syncthreads();
+  if (*s_record && t < c) {
+    double *h = dbuf + (size_t)c * m;
+    h[t] = (first ? 0.0 : h[t]) + dots[t];   // BV_CleanCoefficients + BV_AddCoefficients
+    st->coef[*s_out][t] = dots[t];
+  }
+}
 
 template <typename T, int L>
 __global__ __launch_bounds__(BLK) void gs_kernel(GsArgs<T> a) {
@@ -90,8 +141,8 @@ __global__ __launch_bounds__(BLK) void gs_kernel(GsArgs<T> a) {
   T *cf = vs + CH;                           // [c]
   T *rt = cf + c;                            // [BLK]
   __shared__ double s_dots[GS_MAXC + 1];
-  __shared__ double s_red[BLK];
-  __shared__ int s_mode, s_flag, s_record, s_out;
+  __shared__ double s_acc[BLK];
+  __shared__ int s_mode, s_record, s_out;
   __shared__ double s_scale;
   const int t = threadIdx.x;
 
@@ -113,6 +164,10 @@ __global__ __launch_bounds__(BLK) void gs_kernel(GsArgs<T> a) {
   const T scl = (T)s_scale;
   if (do_mult)
     for (int i = t; i < c; i += BLK) cf[i] = (T)a.st->coef[a.slot][i];
+  // dot-product mapping: ncol = c+1 columns (column c is v itself), nsl row slices
+  const int ncol = c + 1, nsl = BLK / ncol;
+  const int dcol = t % ncol, dsl = t / ncol;
+  const bool dact = do_dots && dsl < nsl;
 
   double acc = 0.0;
   const int64_t rb = (int64_t)blockIdx.x * a.rpb;
@@ -177,79 +232,45 @@ __global__ __launch_bounds__(BLK) void gs_kernel(GsArgs<T> a) {
       }
       __syncthreads();
     }
-    // 4. dot products with the (updated) vector; thread c takes v^T v
-    if (do_dots && t <= c) {
-      const T *colp = (t < c) ? (Vs + t * LDP) : vs;
+    // 4. dot products with the (updated) vector, nsl threads per column
+    if (dact) {
+      const T *colp = (dcol < c) ? (Vs + dcol * LDP) : vs;
       T s = (T)0;
-      for (int r = 0; r < rows; r++) s += colp[r] * vs[r];
+      for (int r = dsl; r < rows; r += nsl) s += colp[r] * vs[r];
       acc += (double)s;
     }
     __syncthreads();
   }
   if (!do_dots) return;
-
-  if (!reduce_partials(&acc, c + 1, a.part, a.gpart, a.cnt, a.G, s_dots, &s_flag)) return;
-
-  // ---- decision (single workgroup) ----
-  s_red[t] = (t < c) ? s_dots[t] * s_dots[t] : 0.0;
+  s_acc[t] = acc;
   __syncthreads();
-  for (int w = BLK / 2; w > 0; w >>= 1) {
-    if (t < w) s_red[t] += s_red[t + w];
-    __syncthreads();
+  if (t < ncol) {
+    double s = 0.0;
+    for (int q = 0; q < nsl; q++) s += s_acc[q * ncol + t];
+    if (a.inblock) s_dots[t] = s;
+    else a.part[(size_t)t * a.G + blockIdx.x] = s;
   }
-  if (t == 0) {
-    GsState *st = a.st;
-    const int out = a.slot + 1;
-    const double self = s_dots[c];
-    int mode_out = GS_IDLE, record = 0, fin = 0;
-    double scale_out = 1.0, nrm_f = 0.0, onrm_f = 0.0;
-    if (a.first) { st->pass = 0; st->pending = 0; st->lindep = 0; }
-    if (st->pending) {                      // true norm of the previous update
-      const double nrm = safe_sqrt(self);
-      st->pending = 0;
-      if (!(nrm != 0.0 && nrm < a.eta * st->onrm && st->pass < 3)) {
-        fin = 1; nrm_f = nrm; onrm_f = st->onrm; mode_out = GS_SCALE_ONLY;
-      }
-    }
-    if (!fin) {                              // one CGS1 pass (bvorthog.c:91-132)
-      st->pass++;
-      st->passes_total++;
-      record = 1;
-      const double onrm = safe_sqrt(self);
-      const double est = onrm * onrm - s_red[0];
-      if (est <= 0.0) {
-        st->pending = 1; st->onrm = onrm; mode_out = GS_MULT_DOTS;
-      } else {
-        const double nrm = sqrt(est);
-        if (nrm != 0.0 && nrm < a.eta * onrm && st->pass < 3) {
-          st->onrm = onrm; mode_out = GS_MULT_DOTS;
-        } else {
-          fin = 1; nrm_f = nrm; onrm_f = onrm; mode_out = GS_MULT_SCALE;
-        }
-      }
-    }
-    if (fin) {                               // bvorthog.c:175-216 + 417-422
-      const int lind = !(nrm_f != 0.0 && fabs(nrm_f) >= a.eta * fabs(onrm_f));
-      st->lindep = lind;
-      st->nrm = nrm_f;
-      st->beta = nrm_f;
-      a.dbuf[(size_t)c * a.m + c] = lind ? 0.0 : nrm_f;
-      scale_out = (nrm_f != 0.0 && nrm_f != 1.0) ? 1.0 / nrm_f : 1.0;
-      if (lind) { st->stop = 1; st->stop_col = c; }
-    }
-    st->mode[out] = mode_out;
-    st->scale[out] = scale_out;
-    if (a.first)
-      for (int s = out + 1; s < GS_SLOTS; s++) st->mode[s] = GS_IDLE;
-    s_record = record;
-    s_out = out;
+  if (!a.inblock) return;
+  __syncthreads();
+  gs_decide(a.st, s_dots, c, a.slot, a.first, a.eta, a.dbuf, a.m, &s_record, &s_out);
+}
+
+// Reduction + decision after the full-grid step of `slot`.
+__global__ __launch_bounds__(RBLK) void gs_reduce_kernel(GsState *st, const double *part, int G, int c, int slot,
+                                                         int first, double eta, double *dbuf, int m) {
+  __shared__ double s_dots[GS_MAXC + 1];
+  __shared__ int s_go, s_record, s_out;
+  if (threadIdx.x == 0) {
+    int go;
+    if (st->stop && (first || c > st->stop_col)) go = 0;
+    else go = first ? 1 : (st->mode[slot] == GS_MULT_DOTS);
+    s_go = go;
   }
   __syncthreads();
-  if (s_record && t < c) {
-    double *h = a.dbuf + (size_t)c * a.m;
-    h[t] = (a.first ? 0.0 : h[t]) + s_dots[t];   // BV_CleanCoefficients + BV_AddCoefficients
-    a.st->coef[s_out][t] = s_dots[t];
-  }
+  if (!s_go) return;
+  sum_partials_block(part, G, c + 1, s_dots);
+  __syncthreads();
+  gs_decide(st, s_dots, c, slot, first, eta, dbuf, m, &s_record, &s_out);
 }
 
 __global__ void gs_reset_kernel(GsState *st) {
@@ -270,21 +291,38 @@ static int gs_launch(BV &V, const Mat *A, int c, double eta, hipStream_t s) {
   a.rp = A ? A->rowptr : nullptr; a.ci = A ? A->col : nullptr;
   a.val = A ? static_cast<const T *>(A->val) : nullptr;
   a.st = V.st; a.dbuf = V.dbuf; a.m = V.m;
-  a.part = V.ws.part; a.gpart = V.ws.gpart; a.cnt = V.ws.cnt;
-  a.G = V.G; a.rpb = V.rows_per_block; a.eta = eta;
+  a.part = V.ws.part;
+  a.eta = eta;
   const size_t shm = ((size_t)c * LDP + CH + c + BLK) * sizeof(T);
-  SA_CHECK(shm + 4096 <= 65536, "basis too wide for the LDS tile of the Gram-Schmidt kernel");
+  SA_CHECK(shm + 4096 + 8 * BLK <= 65536, "basis too wide for the LDS tile of the Gram-Schmidt kernel");
   const int lanes = A ? A->lanes : 8;
+  // Slots 1-3 (one or two passes, the common case) run on the full grid, each
+  // followed by a one-workgroup reduction.  Slots 4-5 (a third pass or a failed
+  // norm estimate) run as one workgroup over all rows that reduces in LDS:
+  // slow but rare, and an idle one costs one tiny launch.
   for (int slot = 1; slot <= 5; slot++) {
     a.slot = slot;
     a.first = (slot == 1);
     if (slot > 1) { a.rp = nullptr; a.ci = nullptr; a.val = nullptr; }
-    switch (lanes) {
-      case 4: hipLaunchKernelGGL((gs_kernel<T, 4>), dim3(V.G), dim3(BLK), shm, s, a); break;
-      case 8: hipLaunchKernelGGL((gs_kernel<T, 8>), dim3(V.G), dim3(BLK), shm, s, a); break;
-      case 16: hipLaunchKernelGGL((gs_kernel<T, 16>), dim3(V.G), dim3(BLK), shm, s, a); break;
-      default: hipLaunchKernelGGL((gs_kernel<T, 32>), dim3(V.G), dim3(BLK), shm, s, a); break;
+    int grid = V.G;
+    a.G = V.G;
+    a.rpb = V.rows_per_block;
+    a.inblock = 0;
+    if (slot >= 4) {
+      grid = 1;
+      a.G = 1;
+      a.rpb = (int)((V.n + CH - 1) / CH * CH);
+      a.inblock = 1;
     }
+    switch (lanes) {
+      case 4: hipLaunchKernelGGL((gs_kernel<T, 4>), dim3(grid), dim3(BLK), shm, s, a); break;
+      case 8: hipLaunchKernelGGL((gs_kernel<T, 8>), dim3(grid), dim3(BLK), shm, s, a); break;
+      case 16: hipLaunchKernelGGL((gs_kernel<T, 16>), dim3(grid), dim3(BLK), shm, s, a); break;
+      default: hipLaunchKernelGGL((gs_kernel<T, 32>), dim3(grid), dim3(BLK), shm, s, a); break;
+    }
+    if (slot <= 3)
+      hipLaunchKernelGGL(gs_reduce_kernel, dim3(1), dim3(RBLK), 0, s, V.st, (const double *)V.ws.part, V.G, c, slot,
+                         a.first, eta, V.dbuf, V.m);
   }
   SA_HIP(hipGetLastError());
   return 0;
@@ -349,14 +387,13 @@ int launch_spmv(const Mat &A, const void *x, void *y, hipStream_t s) {
 // ---------------------------------------------------------------------------
 template <typename T>
 __global__ __launch_bounds__(BLK) void dotvec_kernel(int64_t n, int64_t ld, const T *__restrict__ V, int ncol,
-                                                     const T *__restrict__ x, int rpb, double *part,
-                                                     double *gpart, unsigned *cnt, int G, double *q_out) {
+                                                     const T *__restrict__ x, int rpb, double *part, int G) {
   extern __shared__ __align__(16) unsigned char smem_raw[];
   T *Vs = reinterpret_cast<T *>(smem_raw);
   T *xs = Vs + (size_t)ncol * LDP;
-  __shared__ double s_tot[GS_MAXC + 1];
-  __shared__ int s_flag;
+  __shared__ double s_acc[BLK];
   const int t = threadIdx.x;
+  const int nsl = BLK / ncol, dcol = t % ncol, dsl = t / ncol;
   double acc = 0.0;
   const int64_t rb = (int64_t)blockIdx.x * rpb, re = min(n, rb + rpb);
   for (int64_t r0 = rb; r0 < re; r0 += CH) {
@@ -365,15 +402,20 @@ __global__ __launch_bounds__(BLK) void dotvec_kernel(int64_t n, int64_t ld, cons
     if (t < CH) xs[t] = (t < rows) ? x[r0 + t] : (T)0;
     for (int i = cg; i < ncol; i += 4) Vs[i * LDP + row] = (row < rows) ? V[(size_t)i * ld + r0 + row] : (T)0;
     __syncthreads();
-    if (t < ncol) {
+    if (dsl < nsl) {
       T s = (T)0;
-      for (int r = 0; r < rows; r++) s += Vs[t * LDP + r] * xs[r];
+      for (int r = dsl; r < rows; r += nsl) s += Vs[dcol * LDP + r] * xs[r];
       acc += (double)s;
     }
     __syncthreads();
   }
-  if (!reduce_partials(&acc, ncol, part, gpart, cnt, G, s_tot, &s_flag)) return;
-  if (t < ncol) q_out[t] = s_tot[t];
+  s_acc[t] = acc;
+  __syncthreads();
+  if (t < ncol) {
+    double s = 0.0;
+    for (int q = 0; q < nsl; q++) s += s_acc[q * ncol + t];
+    part[(size_t)t * G + blockIdx.x] = s;
+  }
 }
 
 int launch_dotvec(BV &V, int l, int k, const void *x, double *q_dev, hipStream_t s) {
@@ -384,12 +426,11 @@ int launch_dotvec(BV &V, int l, int k, const void *x, double *q_dev, hipStream_t
   const size_t shm = ((size_t)ncol * LDP + CH) * es;
   if (V.prec == SA_FP64)
     hipLaunchKernelGGL(dotvec_kernel<double>, dim3(V.G), dim3(BLK), shm, s, V.n, V.ld,
-                       (const double *)col_ptr(V, l), ncol, (const double *)x, V.rows_per_block,
-                       V.ws.part, V.ws.gpart, V.ws.cnt, V.G, q_dev);
+                       (const double *)col_ptr(V, l), ncol, (const double *)x, V.rows_per_block, V.ws.part, V.G);
   else
     hipLaunchKernelGGL(dotvec_kernel<float>, dim3(V.G), dim3(BLK), shm, s, V.n, V.ld,
-                       (const float *)col_ptr(V, l), ncol, (const float *)x, V.rows_per_block,
-                       V.ws.part, V.ws.gpart, V.ws.cnt, V.G, q_dev);
+                       (const float *)col_ptr(V, l), ncol, (const float *)x, V.rows_per_block, V.ws.part, V.G);
+  hipLaunchKernelGGL(colsum_kernel, dim3(1), dim3(RBLK), 0, s, (const double *)V.ws.part, V.G, ncol, q_dev);
   SA_HIP(hipGetLastError());
   return 0;
 }

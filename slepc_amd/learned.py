@@ -1,10 +1,14 @@
 """Learned warm start for the parameterised finite-volume family.
 
 A small network maps the 8 Karhunen-Loeve coefficients theta of the
-conductivity field to the 16 lowest modes of -div(k_theta grad u) on a coarse
-64x64 grid.  Its training targets are eigenvectors computed by this package's
-own Krylov-Schur solver on the coarse operators; forward, backward and Adam run
-in the library's BF16 MFMA kernels.  At solve time the predicted modes are
+conductivity field to 16 vectors on a coarse 64x64 grid that span the
+eigenspace of the 16 lowest modes of -div(k_theta grad u).  The training
+targets are the orthogonal projections X X^T w_i of 16 fixed random vectors w_i
+onto that eigenspace, X being the eigenvectors computed by this package's own
+Krylov-Schur solver on the coarse operators: unlike the eigenvectors
+themselves, the projections do not depend on the sign or on the rotation of
+eigenvectors inside (near-)degenerate clusters, so the map theta -> target is
+smooth.  Forward, backward and Adam run in the library's BF16 MFMA kernels.  At solve time the predicted modes are
 prolonged bilinearly to the fine grid and summed into the start vector passed
 through EPSSetInitialSpace; Krylov-Schur's own convergence test still decides,
 so accuracy never depends on the network, and a degenerate prediction falls
@@ -122,32 +126,54 @@ def coarse_modes(sa, theta, nmodes=NMODES, g=COARSE, tol=1e-6):
     return X
 
 
-def sign_fix(X, ref):
-    """Flips each mode so that its inner product with the reference mode is >= 0."""
-    s = np.sign(np.sum(X * ref, axis=1))
-    s[s == 0] = 1.0
-    return X * s[:, None]
+def coarse_modes_batch(sa, thetas, nmodes=NMODES, g=COARSE, tol=1e-6, ncv=48):
+    """The same eigenvectors for many parameter draws at once with the batched
+    solver (one workgroup per coarse problem, one launch per restart phase)."""
+    from .batch import Batch
+    csrs = [sa.fv2d(g, g, th) for th in thetas]
+    B = Batch(csrs)
+    B.solve(nmodes, ncv, tol, "LR", seed=0, max_it=4000)
+    out = []
+    for p in range(len(csrs)):
+        nconv, _, _ = B.converged(p)
+        if nconv < nmodes:
+            out.append(None)
+            continue
+        out.append(np.stack([B.eigenpair(p, i)[1] for i in range(nmodes)]))
+    B.destroy()
+    return out
+
+
+def probe_vectors(n=COARSE * COARSE, k=NMODES, seed=77):
+    """Fixed random vectors w_i (columns) that are projected onto the eigenspace."""
+    rng = np.random.default_rng(seed)
+    return rng.standard_normal((n, k))
+
+
+def projection_targets(X, W):
+    """Rows i = X^T X w_i / ||.||: projections of the probes onto span(X rows).
+    X: [k, n] orthonormal rows (eigenvectors); W: [n, k]."""
+    P = (X.T @ (X @ W)).T
+    return P / np.linalg.norm(P, axis=1, keepdims=True)
 
 
 class WarmStart:
     """Trained network + the rules to turn its output into a start vector."""
 
-    def __init__(self, net, ref, scale):
-        self.net, self.ref, self.scale = net, ref, scale
+    def __init__(self, net, probes, scale):
+        self.net, self.probes, self.scale = net, probes, scale
 
     @classmethod
     def train(cls, sa, n_train=256, epochs=60, batch=64, lr=1e-3, seed=1, log=None):
         t0 = time.perf_counter()
-        ref = coarse_modes(sa, np.zeros(DIN))
+        W = probe_vectors()
         thetas, targets = [], []
-        for s in range(n_train):
-            th = sa.fv_theta(THETA_SEED_TRAIN, s)
-            try:
-                X = coarse_modes(sa, th)
-            except RuntimeError:
+        draws = [sa.fv_theta(THETA_SEED_TRAIN, s) for s in range(n_train)]
+        for th, X in zip(draws, coarse_modes_batch(sa, draws)):
+            if X is None:
                 continue
             thetas.append(th)
-            targets.append(sign_fix(X, ref).ravel())
+            targets.append(projection_targets(X, W).ravel())
         thetas = np.array(thetas, np.float32)
         scale = float(COARSE)               # unit-norm modes have entries ~ 1/64
         T = np.array(targets, np.float32) * scale
@@ -164,7 +190,7 @@ class WarmStart:
         if log:
             log("learned warm start: %d coarse solves in %.1f s, %d epochs in %.1f s, loss %.4f -> %.4f"
                 % (len(thetas), t1 - t0, epochs, t2 - t1, losses[0], losses[-1]))
-        ws = cls(net, ref, scale)
+        ws = cls(net, W, scale)
         ws.train_losses = losses
         ws.n_train = len(thetas)
         return ws
