@@ -4,8 +4,11 @@
  * Plain pointers and sizes only.  Every function returns 0 on success and a
  * nonzero code on failure; sa_last_error() returns the message (the analogue
  * of PetscErrorCode + PetscCheck).  "_dev" pointers are device (HBM) addresses,
- * "_host" pointers are host memory.  All work is ordered on the library's HIP
- * stream (sa_stream()).
+ * "_host" pointers are host memory.  Each host thread has its own HIP stream
+ * (sa_stream() returns the calling thread's); a call orders its work on that
+ * stream and returns once the results it reports are ready.  Objects may be
+ * read from several threads at once (a matrix shared by concurrent solves)
+ * but must not be modified from two threads at once.
  *
  * The functions map onto the SLEPc/PETSc interfaces the reference's Krylov-Schur
  * calls (paths relative to the SLEPc 3.22.2 source tree); the registration
@@ -73,6 +76,24 @@ int64_t sa_gen_fv2d(int nx, int ny, const double *theta8, double *sigma,
                     int32_t *rowptr, int32_t *col, double *val);
 int64_t sa_gen_markov(int m, int32_t *rowptr, int32_t *col, double *val);
 void    sa_fv2d_theta(uint64_t seed, int sample, double *theta8);
+
+/* ---- file formats ---------------------------------------------------------
+ * MatLoad of a PETSc binary AIJ file (as src/eps/tutorials/ex4.c loads the
+ * .petsc files of share/slepc/datafiles/matrices): big-endian int32 header
+ * {1211216, M, N, nz}, M row lengths, nz column indices, nz float64 values.
+ * Called with rowptr = col = val = NULL it returns the sizes only. */
+int sa_mat_load_petsc_binary(const char *path, int *nrows, int *ncols, int64_t *nnz,
+                             int32_t *rowptr, int32_t *col, double *val);
+/* VecView / VecLoad with a binary viewer: {1211214, n} then n float64 values;
+   append != 0 writes after the vectors already in the file (EPSVectorsView). */
+int sa_vec_write_petsc_binary(const char *path, int64_t n, const double *x, int append);
+int sa_vec_load_petsc_binary(const char *path, int64_t index, int64_t *n, double *x);
+/* Text of EPSValuesView (src/eps/interface/epsview.c:602-640): format 0 is the
+   default ASCII viewer ("%.5f" per value), 1 is PETSC_VIEWER_ASCII_MATLAB
+   ("Lambda_<name> = [" with "%18.16e").  Returns the text length and writes at
+   most len-1 characters plus a zero into buf when buf != NULL; -1 on error. */
+int64_t sa_format_eigenvalues(int n, const double *re, const double *im, int format, const char *name,
+                              char *buf, int64_t len);
 void    sa_rand_uniform(uint64_t seed, int64_t n, double *out_host); /* U[-1,1) */
 double  sa_u01(uint64_t seed, uint64_t i);
 uint64_t sa_sub_seed(uint64_t seed, uint64_t p);
@@ -132,7 +153,7 @@ int sa_lapack_info(char *buf, int len);
 /* ---- EPS: Krylov-Schur --------------------------------------------------------
  * Mirrors EPSCreate/EPSSetOperators/EPSSetProblemType/EPSSetWhichEigenpairs/
  * EPSSetDimensions/EPSSetTolerances/EPSSetInitialSpace/EPSSolve/EPSGetConverged/
- * EPSGetEigenpair/EPSComputeError (src/eps/interface/*.c) for the default
+ * EPSGetEigenpair/EPSComputeError (the files of src/eps/interface) for the default
  * Krylov-Schur variant (src/eps/impls/krylov/krylovschur/krylovschur.c:93-348). */
 int sa_eps_create(sa_eps *eps);
 int sa_eps_destroy(sa_eps eps);

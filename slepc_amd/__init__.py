@@ -101,6 +101,48 @@ def csr_assemble(n, rows, cols, vals):
     return CSR(n, rp, ci, v)
 
 
+# ------------------------------------------------------------------ file formats
+def load_petsc_binary(path):
+    """MatLoad of a PETSc binary AIJ file -> CSR (square matrices)."""
+    L = lib()
+    M, N, nz = C.c_int(), C.c_int(), C.c_int64()
+    bp = path.encode()
+    check(L.sa_mat_load_petsc_binary(bp, C.byref(M), C.byref(N), C.byref(nz), None, None, None))
+    if M.value != N.value:
+        raise SlepcAmdError("matrix in %s is %d x %d, not square" % (path, M.value, N.value))
+    rp = np.zeros(M.value + 1, np.int32)
+    ci = np.zeros(nz.value, np.int32)
+    v = np.zeros(nz.value, np.float64)
+    check(L.sa_mat_load_petsc_binary(bp, C.byref(M), C.byref(N), C.byref(nz), rp.ctypes.data, ci.ctypes.data,
+                                     v.ctypes.data))
+    return CSR(M.value, rp, ci, v)
+
+
+def load_petsc_vec(path, index=0):
+    """VecLoad of the index-th vector of a PETSc binary file."""
+    L = lib()
+    n = C.c_int64()
+    bp = path.encode()
+    check(L.sa_vec_load_petsc_binary(bp, index, C.byref(n), None))
+    x = np.zeros(n.value)
+    check(L.sa_vec_load_petsc_binary(bp, index, C.byref(n), x.ctypes.data))
+    return x
+
+
+def format_eigenvalues(re, im=None, fmt="ascii", name="eps"):
+    """Text of EPSValuesView for the given eigenvalues ("ascii" or "matlab")."""
+    re = np.ascontiguousarray(re, np.float64)
+    im = np.zeros_like(re) if im is None else np.ascontiguousarray(im, np.float64)
+    f = {"ascii": 0, "matlab": 1}[fmt]
+    L = lib()
+    need = L.sa_format_eigenvalues(len(re), re.ctypes.data, im.ctypes.data, f, name.encode(), None, 0)
+    if need < 0:
+        raise SlepcAmdError(L.sa_last_error().decode())
+    buf = C.create_string_buffer(int(need) + 1)
+    L.sa_format_eigenvalues(len(re), re.ctypes.data, im.ctypes.data, f, name.encode(), buf, need + 1)
+    return buf.value.decode()
+
+
 # ------------------------------------------------------------------ device objects
 class DeviceArray:
     """A device buffer owned by the library allocator."""
@@ -331,6 +373,40 @@ class EPS:
             re, im, _, _ = self.getEigenpair(i, vectors=False)
             out.append(complex(re, im) if im else re)
         return out
+
+    def valuesView(self, fmt="ascii", name="eps", path=None):
+        """EPSValuesView (epsview.c:530-640): "ascii" or "matlab" text, or
+        "binary" (the converged eigenvalues as big-endian complex doubles, as
+        PetscViewerBinaryWrite of PETSC_COMPLEX writes them).  Returns the
+        text (ascii/matlab) and writes it to `path` when given."""
+        n = self.getConverged()
+        re = np.zeros(n)
+        im = np.zeros(n)
+        for i in range(n):
+            re[i], im[i], _, _ = self.getEigenpair(i, vectors=False)
+        if fmt == "binary":
+            if path is None:
+                raise ValueError("binary view needs a path")
+            np.column_stack([re, im]).astype(">f8").tofile(path)
+            return None
+        text = format_eigenvalues(re, im, fmt, name)
+        if path is not None:
+            with open(path, "w") as f:
+                f.write(text)
+        return text
+
+    def vectorsView(self, path):
+        """EPSVectorsView with a binary viewer: every converged eigenvector
+        (real part; the imaginary part follows for a complex pair) appended
+        to `path` in PETSc binary Vec format."""
+        L = lib()
+        first = True
+        for i in range(self.getConverged()):
+            _, im, xr, xi = self.getEigenpair(i)
+            for x in ([xr, xi] if im else [xr]):
+                x = np.ascontiguousarray(x)
+                check(L.sa_vec_write_petsc_binary(path.encode(), len(x), x.ctypes.data, 0 if first else 1))
+                first = False
 
     def getErrorEstimate(self, i):
         v = C.c_double()

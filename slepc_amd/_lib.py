@@ -58,6 +58,11 @@ SIGNATURES = {
     "sa_gen_fv2d": (C.c_int64, [C.c_int, C.c_int, _F64P, C.POINTER(C.c_double), _VP, _VP, _VP]),
     "sa_gen_markov": (C.c_int64, [C.c_int, _VP, _VP, _VP]),
     "sa_fv2d_theta": (None, [C.c_uint64, C.c_int, _F64P]),
+    "sa_mat_load_petsc_binary": (C.c_int, [C.c_char_p, C.POINTER(C.c_int), C.POINTER(C.c_int),
+                                           C.POINTER(C.c_int64), _VP, _VP, _VP]),
+    "sa_vec_write_petsc_binary": (C.c_int, [C.c_char_p, C.c_int64, _VP, C.c_int]),
+    "sa_vec_load_petsc_binary": (C.c_int, [C.c_char_p, C.c_int64, C.POINTER(C.c_int64), _VP]),
+    "sa_format_eigenvalues": (C.c_int64, [C.c_int, _VP, _VP, C.c_int, C.c_char_p, _VP, C.c_int64]),
     "sa_rand_uniform": (None, [C.c_uint64, C.c_int64, _F64P]),
     "sa_u01": (C.c_double, [C.c_uint64, C.c_uint64]),
     "sa_sub_seed": (C.c_uint64, [C.c_uint64, C.c_uint64]),

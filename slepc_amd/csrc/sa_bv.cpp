@@ -1,6 +1,7 @@
 // BV (basis vectors) host side: allocation and the Arnoldi/Lanczos sweep that
 // chains the fused SpMV + Gram-Schmidt kernels without host round trips.
 #include <algorithm>
+#include <cstdlib>
 #include <cstring>
 
 #include "sa_internal.h"
@@ -23,21 +24,22 @@ int bv_alloc(BV &V, int64_t n, int m, int prec) {
   SA_HIP(hipMemsetAsync(V.data, 0, (size_t)V.ld * m * es, stream()));
   SA_HIP(hipMalloc(&V.dbuf, sizeof(double) * (size_t)m * m));
   SA_HIP(hipMemsetAsync(V.dbuf, 0, sizeof(double) * (size_t)m * m, stream()));
-  SA_HIP(hipMalloc(&V.st, sizeof(GsState)));
-  SA_HIP(hipMemsetAsync(V.st, 0, sizeof(GsState), stream()));
+  SA_HIP(hipMalloc(&V.st, sizeof(GsState) * GS_NSTATE));
+  SA_HIP(hipMemsetAsync(V.st, 0, sizeof(GsState) * GS_NSTATE, stream()));
   SA_HIP(hipHostMalloc((void **)&V.st_host, sizeof(GsState)));
   // Partition rows into workgroup ranges of whole 64-row chunks.
   const int64_t chunks = std::max<int64_t>(1, (n + 63) / 64);
-  // at most 1024 workgroups and at least 4 chunks each, so the one-workgroup
-  // reduction of the partial sums stays short for small problems
-  const int64_t maxG = 1024;
-  int64_t G = std::min(maxG, std::max<int64_t>(1, (chunks + 3) / 4));
+  // at most 1024 workgroups and at least gs_min_chunks() chunks each, so the
+  // reductions of the partial sums stay short for small problems
+  const int64_t maxG = 1024, minc = gs_min_chunks();
+  int64_t G = std::min(maxG, std::max<int64_t>(1, (chunks + minc - 1) / minc));
   const int64_t cpb = (chunks + G - 1) / G;
   G = (chunks + cpb - 1) / cpb;
   V.G = (int)G;
   V.rows_per_block = (int)(cpb * 64);
   V.ws.maxG = (int)G;
-  SA_HIP(hipMalloc(&V.ws.part, sizeof(double) * (size_t)G * (GS_MAXC + 1)));
+  SA_HIP(hipMalloc(&V.ws.part, sizeof(double) * 2 * (size_t)G * (GS_MAXC + 1)));
+  SA_HIP(hipMalloc(&V.ws.dots, sizeof(double) * 2 * (GS_MAXC + 1)));
   SA_HIP(hipMalloc(&V.dq, sizeof(double) * (size_t)m * m + 64 * sizeof(double)));
   SA_HIP(hipHostMalloc((void **)&V.hq, sizeof(double) * (size_t)m * m + 64 * sizeof(double)));
   SA_HIP(hipStreamSynchronize(stream()));
@@ -45,27 +47,80 @@ int bv_alloc(BV &V, int64_t n, int m, int prec) {
 }
 
 void bv_free(BV &V) {
+  for (auto &g : V.graphs)
+    if (g.exec) (void)hipGraphExecDestroy(g.exec);
   if (V.data) (void)hipFree(V.data);
   if (V.dbuf) (void)hipFree(V.dbuf);
   if (V.st) (void)hipFree(V.st);
   if (V.st_host) (void)hipHostFree(V.st_host);
   if (V.ws.part) (void)hipFree(V.ws.part);
+  if (V.ws.dots) (void)hipFree(V.ws.dots);
   if (V.dq) (void)hipFree(V.dq);
   if (V.hq) (void)hipHostFree(V.hq);
   V = BV();
 }
 
+static bool graphs_enabled() {
+  static const bool on = [] {
+    const char *e = std::getenv("SLEPC_AMD_GRAPHS");
+    return !(e && e[0] == '0');
+  }();
+  return on;
+}
+
+// The launches of one sweep, captured once per (operator, k, m, eta) into a
+// HIP graph and replayed on later restarts with the same range: one graph
+// launch replaces 5(m-k)+1 kernel launches.  The graph holds kernels only.
+static int sweep_graph(BV &V, const Mat &A, int k, int m, double eta, hipStream_t s, hipGraphExec_t *out) {
+  *out = nullptr;
+  for (auto &g : V.graphs)
+    if (g.rp == A.rowptr && g.ci == A.col && g.val == A.val && g.lanes == A.lanes && g.k == k && g.m == m &&
+        g.eta == eta) {
+      *out = g.exec;
+      return 0;
+    }
+  if (V.graphs.size() >= 256) return 0;      // cache full: launch directly
+  SA_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+  int rc = launch_gs_reset(V, s);
+  for (int j = k; j < m && !rc; j++) rc = launch_gs_column(V, &A, j + 1, eta, s);
+  hipGraph_t graph = nullptr;
+  const hipError_t ec = hipStreamEndCapture(s, &graph);
+  if (rc || ec != hipSuccess) {
+    if (graph) (void)hipGraphDestroy(graph);
+    if (rc) return rc;
+    SA_HIP(ec);
+  }
+  hipGraphExec_t exec = nullptr;
+  const hipError_t ei = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+  (void)hipGraphDestroy(graph);
+  SA_HIP(ei);
+  SweepGraph g;
+  g.rp = A.rowptr; g.ci = A.col; g.val = A.val; g.lanes = A.lanes;
+  g.k = k; g.m = m; g.eta = eta; g.exec = exec;
+  V.graphs.push_back(g);
+  *out = exec;
+  return 0;
+}
+
 // BVMatArnoldi/BVMatLanczos (bvkrylov.c:56-113,165-226): for j = k..m-1,
 // V(:,j+1) = A V(:,j) followed by BVOrthonormalizeColumn(V,j+1).  All steps
-// are enqueued back to back; the device state machine applies the DGKS rule
-// and stops the sweep at a breakdown.  One synchronisation at the end reads
-// the state (breakdown column and the last norm).
+// are enqueued back to back (as one graph launch when graphs are on); the
+// device state machine applies the DGKS rule and stops the sweep at a
+// breakdown.  One synchronisation at the end reads the state (breakdown
+// column and the last norm).
 int bv_sweep(BV &V, const Mat &A, int k, int *m, double eta, double *beta, int *breakdown) {
   SA_CHECK(k >= 0 && *m > k && *m < V.m, "invalid Arnoldi range");
+  SA_CHECK(A.prec == V.prec && A.n == V.n, "matrix and basis sizes or precisions differ");
   hipStream_t s = stream();
-  SA_TRY(launch_gs_reset(V, s));
-  for (int j = k; j < *m; j++) SA_TRY(launch_gs_column(V, &A, j + 1, eta, s));
-  SA_HIP(hipMemcpyAsync(V.st_host, V.st, sizeof(GsState), hipMemcpyDeviceToHost, s));
+  hipGraphExec_t exec = nullptr;
+  if (graphs_enabled()) SA_TRY(sweep_graph(V, A, k, *m, eta, s, &exec));
+  if (exec) {
+    SA_HIP(hipGraphLaunch(exec, s));
+  } else {
+    SA_TRY(launch_gs_reset(V, s));
+    for (int j = k; j < *m; j++) SA_TRY(launch_gs_column(V, &A, j + 1, eta, s));
+  }
+  SA_HIP(hipMemcpyAsync(V.st_host, V.st + 2, sizeof(GsState), hipMemcpyDeviceToHost, s));
   SA_HIP(hipStreamSynchronize(s));
   const GsState &st = *V.st_host;
   if (st.stop) {
@@ -84,7 +139,7 @@ int bv_orthonormalize(BV &V, int j, double eta, double *norm, int *lindep) {
   hipStream_t s = stream();
   SA_TRY(launch_gs_reset(V, s));
   SA_TRY(launch_gs_column(V, nullptr, j, eta, s));
-  SA_HIP(hipMemcpyAsync(V.st_host, V.st, sizeof(GsState), hipMemcpyDeviceToHost, s));
+  SA_HIP(hipMemcpyAsync(V.st_host, V.st + 2, sizeof(GsState), hipMemcpyDeviceToHost, s));
   SA_HIP(hipStreamSynchronize(s));
   if (norm) *norm = V.st_host->nrm;
   if (lindep) *lindep = V.st_host->lindep;

@@ -1,0 +1,55 @@
+// Device-side constants and helpers shared by the HIP sources (gfx950,
+// wave size 64, 256-thread workgroups).
+#pragma once
+#include <hip/hip_runtime.h>
+
+namespace sa {
+
+constexpr int BLK = 256;
+constexpr int CH = 64;          // rows per chunk (one row per lane of a wave)
+constexpr int LDP = CH + 1;     // padded LDS column stride
+constexpr int RBLK = 1024;      // threads of the one-workgroup reductions
+
+__device__ __forceinline__ double safe_sqrt(double a) { return a < 0.0 ? 0.0 : sqrt(a); }
+
+// out[col] = sum_b part[col * G + b] for col < ncol.  Each wave reduces four
+// columns at a time (columns w, w+nw, w+2nw, w+3nw) with four loads per column
+// in flight, then a shuffle reduction; the order of the additions is fixed, so
+// every caller reducing the same partials obtains the same bits.
+__device__ inline void sum_partials_block(const double *part, int G, int ncol, double *out) {
+  constexpr int NB = 4;
+  const int t = threadIdx.x, lane = t & 63, w = t >> 6, nw = blockDim.x >> 6;
+  for (int c0 = w; c0 < ncol; c0 += NB * nw) {
+    double s[NB];
+#pragma unroll
+    for (int k = 0; k < NB; k++) s[k] = 0.0;
+    int b = lane;
+    for (; b + 192 < G; b += 256) {
+#pragma unroll
+      for (int k = 0; k < NB; k++) {
+        const int col = c0 + k * nw;
+        if (col < ncol) {
+          const double *p = part + (size_t)col * G;
+          s[k] += (p[b] + p[b + 64]) + (p[b + 128] + p[b + 192]);
+        }
+      }
+    }
+    for (; b < G; b += 64) {
+#pragma unroll
+      for (int k = 0; k < NB; k++) {
+        const int col = c0 + k * nw;
+        if (col < ncol) s[k] += part[(size_t)col * G + b];
+      }
+    }
+#pragma unroll
+    for (int k = 0; k < NB; k++) {
+      double v = s[k];
+#pragma unroll
+      for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+      const int col = c0 + k * nw;
+      if (lane == 0 && col < ncol) out[col] = v;
+    }
+  }
+}
+
+}  // namespace sa

@@ -31,28 +31,37 @@ int ensure_init();
 int num_cus();
 
 // ---- Gram-Schmidt step machine ---------------------------------------------
-// Column c is orthogonalised against columns 0..c-1 by up to five kernels
-// (slots 1..5).  Kernel slot i reads mode[i]; the reduction that follows a
-// kernel computing dot products writes the decision for slot i+1.
+// Column c is orthogonalised against columns 0..c-1 by three kernels (slots
+// 1..3).  Slot s reads the state st[(s+1)%3] left by the previous kernel and
+// workgroup 0 writes st[s-1]; the state after a column is st[2].
 constexpr int GS_MAXC = 128;        // largest column index + 1 handled
-constexpr int GS_SLOTS = 7;
-enum GsMode : int { GS_IDLE = 0, GS_MULT_SCALE = 1, GS_MULT_DOTS = 2, GS_SCALE_ONLY = 3 };
+constexpr int GS_NSTATE = 3;
 
 struct GsState {
-  int mode[GS_SLOTS];
-  int pass;       // Gram-Schmidt passes done for the current column
-  int pending;    // estimate was <= 0: next reduction evaluates the true norm
-  int lindep;     // final linear-dependence flag of the current column
-  int stop;       // breakdown happened: later steps of the sweep are no-ops
+  int pass;         // Gram-Schmidt passes done for the current column
+  int pending;      // estimate was <= 0: next decision evaluates the true norm
+  int lindep;       // final linear-dependence flag of the current column
+  int stop;         // breakdown happened: later steps of the sweep are no-ops
   int stop_col;
-  int passes_total;
+  int done;         // current column finished (normalised)
+  int dots_ready;   // the kernel that wrote this state produced dot products
+  int slow_total;   // columns finished by the one-workgroup path (third pass)
+  int64_t passes_total;
   double onrm, nrm, beta;
-  double scale[GS_SLOTS];
-  double coef[GS_SLOTS][GS_MAXC];
+};
+
+// An instantiated HIP graph of one sweep (reset + columns k+1..m); the kernel
+// arguments baked into it are fixed by the basis and by this key.
+struct SweepGraph {
+  const void *rp = nullptr, *ci = nullptr, *val = nullptr;
+  int lanes = 0, k = 0, m = 0;
+  double eta = 0.0;
+  hipGraphExec_t exec = nullptr;
 };
 
 struct Workspace {           // device scratch for partial sums of one BV
-  double *part = nullptr;    // [GS_MAXC+1][maxG], column-major partial sums
+  double *part = nullptr;    // 2 x [GS_MAXC+1][maxG], column-major partial sums
+  double *dots = nullptr;    // 2 x [GS_MAXC+1] reduced dot products
   int maxG = 0;
 };
 
@@ -62,8 +71,9 @@ struct BV {
   int l = 0, k = 0;
   void *data = nullptr;        // T[ld * m]
   double *dbuf = nullptr;      // coefficient buffer m x m (FP64) on device
-  GsState *st = nullptr;       // device
-  GsState *st_host = nullptr;  // pinned host mirror
+  GsState *st = nullptr;       // device, GS_NSTATE states (ping-pong)
+  GsState *st_host = nullptr;  // pinned host mirror of st[2]
+  std::vector<SweepGraph> graphs;   // captured Arnoldi sweeps, reused across restarts
   Workspace ws;
   double *dq = nullptr;        // device staging for Q / q vectors (m*m doubles)
   double *hq = nullptr;        // pinned host staging
@@ -85,6 +95,7 @@ int launch_spmv(const Mat &A, const void *x, void *y, hipStream_t s);
 // One Arnoldi/orthonormalisation step for column c of V; spmv_from >= 0 first
 // computes V(:,c) = A V(:,spmv_from).  eta: DGKS threshold.
 int launch_gs_column(BV &V, const Mat *A, int c, double eta, hipStream_t s);
+int gs_min_chunks();        // fewest 64-row chunks per workgroup of the GS kernels
 int launch_dotvec(BV &V, int l, int k, const void *x, double *q_dev, hipStream_t s);
 int launch_multvec(BV &V, int l, int k, double alpha, const double *q_dev, double beta, void *y,
                    hipStream_t s);

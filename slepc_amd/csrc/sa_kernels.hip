@@ -1,17 +1,14 @@
-// HIP kernels for gfx950 (MI355X): CSR SpMV, the fused Gram-Schmidt step
-// (SpMV + classical Gram-Schmidt + DGKS refinement), BV block kernels and the
-// restart GEMM on MFMA.  Wave size 64; 256-thread workgroups.
-#include "sa_internal.h"
-
+// HIP kernels for gfx950 (MI355X): CSR SpMV, the BV block kernels (dot
+// products, linear combinations, norms, scaling, conversion) and the restart
+// GEMM on MFMA.  Wave size 64; 256-thread workgroups.  The fused Gram-Schmidt
+// step of the Arnoldi sweep lives in sa_gs.hip.
 #include <algorithm>
 #include <cmath>
 
-namespace sa {
+#include "sa_device.h"
+#include "sa_internal.h"
 
-constexpr int BLK = 256;
-constexpr int CH = 64;          // rows per chunk (one row per lane of a wave)
-constexpr int LDP = CH + 1;     // padded LDS column stride
-__device__ __forceinline__ double safe_sqrt(double a) { return a < 0.0 ? 0.0 : sqrt(a); }
+namespace sa {
 
 // ---------------------------------------------------------------------------
 // Cross-workgroup reductions use no fences and no atomics (an agent-scope
@@ -20,326 +17,8 @@ __device__ __forceinline__ double safe_sqrt(double a) { return a < 0.0 ? 0.0 : s
 // part[col * G + b]; the end of the kernel publishes them and a one-workgroup
 // kernel adds them in a fixed order (deterministic results).
 // ---------------------------------------------------------------------------
-constexpr int RBLK = 1024;
-
-__device__ void sum_partials_block(const double *part, int G, int ncol, double *out) {
-  const int t = threadIdx.x, lane = t & 63, w = t >> 6, nw = blockDim.x >> 6;
-  for (int col = w; col < ncol; col += nw) {
-    const double *p = part + (size_t)col * G;
-    double s0 = 0.0, s1 = 0.0, s2 = 0.0, s3 = 0.0;   // four independent load chains
-    int b = lane;
-    for (; b + 192 < G; b += 256) {
-      s0 += p[b];
-      s1 += p[b + 64];
-      s2 += p[b + 128];
-      s3 += p[b + 192];
-    }
-    for (; b < G; b += 64) s0 += p[b];
-    double s = (s0 + s1) + (s2 + s3);
-#pragma unroll
-    for (int o = 32; o > 0; o >>= 1) s += __shfl_xor(s, o, 64);
-    if (lane == 0) out[col] = s;
-  }
-}
-
 __global__ __launch_bounds__(RBLK) void colsum_kernel(const double *part, int G, int ncol, double *out) {
   sum_partials_block(part, G, ncol, out);
-}
-
-// ---------------------------------------------------------------------------
-// Gram-Schmidt step for column c (classical Gram-Schmidt with DGKS refinement
-// "if needed", bvorthog.c:91-217, then the scaling of bvorthog.c:417-422).
-//   slot 1 (first): v = A V(:,c-1) (when rp) or v = V(:,c); dots V(:,0..c)^T v
-//   slot i > 1    : mode[i] decides: v = (v - V(:,0..c-1) coef) * scale and/or
-//                   the dots of the updated v for the next pass
-// A reduction kernel after each full-grid step evaluates the rules of the
-// reference and writes mode[i+1].  Slots 4-5 (third pass, failed norm
-// estimate) run as one workgroup that reduces in LDS.
-// ---------------------------------------------------------------------------
-template <typename T>
-struct GsArgs {
-  int64_t n, ld;
-  T *V;
-  int c, slot, first, inblock;
-  const int32_t *rp, *ci;
-  const T *val;
-  GsState *st;
-  double *dbuf;
-  int m;
-  double *part;
-  int G, rpb;
-  double eta;
-};
-
-// Decision after the dot products of one pass are known (dots[0..c], dots[c]
-// = v^T v).  Runs in one workgroup; thread 0 decides, all threads record.
-__device__ void gs_decide(GsState *st, const double *dots, int c, int slot, int first, double eta, double *dbuf,
-                          int m, int *s_record, int *s_out) {
-  const int t = threadIdx.x;
-  if (t == 0) {
-    const int out = slot + 1;
-    const double self = dots[c];
-    double sumsq = 0.0;
-    for (int i = 0; i < c; i++) sumsq += dots[i] * dots[i];
-    int mode_out = GS_IDLE, record = 0, fin = 0;
-    double scale_out = 1.0, nrm_f = 0.0, onrm_f = 0.0;
-    if (first) { st->pass = 0; st->pending = 0; st->lindep = 0; }
-    if (st->pending) {                      // true norm of the previous update
-      const double nrm = safe_sqrt(self);
-      st->pending = 0;
-      if (!(nrm != 0.0 && nrm < eta * st->onrm && st->pass < 3)) {
-        fin = 1; nrm_f = nrm; onrm_f = st->onrm; mode_out = GS_SCALE_ONLY;
-      }
-    }
-    if (!fin) {                              // one CGS1 pass (bvorthog.c:91-132)
-      st->pass++;
-      st->passes_total++;
-      record = 1;
-      const double onrm = safe_sqrt(self);
-      const double est = onrm * onrm - sumsq;
-      if (est <= 0.0) {
-        st->pending = 1; st->onrm = onrm; mode_out = GS_MULT_DOTS;
-      } else {
-        const double nrm = sqrt(est);
-        if (nrm != 0.0 && nrm < eta * onrm && st->pass < 3) {
-          st->onrm = onrm; mode_out = GS_MULT_DOTS;
-        } else {
-          fin = 1; nrm_f = nrm; onrm_f = onrm; mode_out = GS_MULT_SCALE;
-        }
-      }
-    }
-    if (fin) {                               // bvorthog.c:175-216 + 417-422
-      const int lind = !(nrm_f != 0.0 && fabs(nrm_f) >= eta * fabs(onrm_f));
-      st->lindep = lind;
-      st->nrm = nrm_f;
-      st->beta = nrm_f;
-      dbuf[(size_t)c * m + c] = lind ? 0.0 : nrm_f;
-      scale_out = (nrm_f != 0.0 && nrm_f != 1.0) ? 1.0 / nrm_f : 1.0;
-      if (lind) { st->stop = 1; st->stop_col = c; }
-    }
-    st->mode[out] = mode_out;
-    st->scale[out] = scale_out;
-    if (first)
-      for (int q = out + 1; q < GS_SLOTS; q++) st->mode[q] = GS_IDLE;
-    *s_record = record;
-    *s_out = out;
-  }
-  __syncthreads();
-  if (*s_record && t < c) {
-    double *h = dbuf + (size_t)c * m;
-    h[t] = (first ? 0.0 : h[t]) + dots[t];   // BV_CleanCoefficients + BV_AddCoefficients
-    st->coef[*s_out][t] = dots[t];
-  }
-}
-
-template <typename T, int L>
-__global__ __launch_bounds__(BLK) void gs_kernel(GsArgs<T> a) {
-  extern __shared__ __align__(16) unsigned char smem_raw[];
-  const int c = a.c;
-  T *Vs = reinterpret_cast<T *>(smem_raw);   // [c][LDP]
-  T *vs = Vs + (size_t)c * LDP;              // [CH]
-  T *cf = vs + CH;                           // [c]
-  T *rt = cf + c;                            // [BLK]
-  __shared__ double s_dots[GS_MAXC + 1];
-  __shared__ double s_acc[BLK];
-  __shared__ int s_mode, s_record, s_out;
-  __shared__ double s_scale;
-  const int t = threadIdx.x;
-
-  if (t == 0) {
-    int md;
-    const GsState *st = a.st;
-    if (st->stop && (a.first || c > st->stop_col)) md = GS_IDLE;
-    else md = a.first ? 100 : st->mode[a.slot];
-    s_mode = md;
-    s_scale = (md == GS_MULT_SCALE || md == GS_SCALE_ONLY) ? st->scale[a.slot] : 1.0;
-  }
-  __syncthreads();
-  const int mode = s_mode;
-  if (mode == GS_IDLE) return;
-  const bool do_mult = (mode == GS_MULT_SCALE || mode == GS_MULT_DOTS);
-  const bool do_upd = do_mult || mode == GS_SCALE_ONLY;
-  const bool do_dots = (mode == 100 || mode == GS_MULT_DOTS);
-  const bool need_tile = (do_mult || do_dots) && c > 0;
-  const T scl = (T)s_scale;
-  if (do_mult)
-    for (int i = t; i < c; i += BLK) cf[i] = (T)a.st->coef[a.slot][i];
-  // dot-product mapping: ncol = c+1 columns (column c is v itself), nsl row slices
-  const int ncol = c + 1, nsl = BLK / ncol;
-  const int dcol = t % ncol, dsl = t / ncol;
-  const bool dact = do_dots && dsl < nsl;
-
-  double acc = 0.0;
-  const int64_t rb = (int64_t)blockIdx.x * a.rpb;
-  const int64_t re = min(a.n, rb + a.rpb);
-  T *vc = a.V + (size_t)c * a.ld;
-  for (int64_t r0 = rb; r0 < re; r0 += CH) {
-    const int rows = (int)min((int64_t)CH, re - r0);
-    // 1. the vector chunk
-    if (mode == 100 && a.rp) {
-      const T *x = a.V + (size_t)(c - 1) * a.ld;
-      constexpr int RPS = BLK / L;
-      const int g = t / L, lg = t % L;
-      for (int rr = g; rr < CH; rr += RPS) {
-        T s = (T)0;
-        if (rr < rows) {
-          const int64_t row = r0 + rr;
-          const int32_t pe = a.rp[row + 1];
-          for (int32_t p = a.rp[row] + lg; p < pe; p += L) s += a.val[p] * x[a.ci[p]];
-        }
-#pragma unroll
-        for (int off = L / 2; off > 0; off >>= 1) s += __shfl_xor(s, off, L);
-        if (lg == 0) {
-          vs[rr] = (rr < rows) ? s : (T)0;
-          if (rr < rows) vc[r0 + rr] = s;
-        }
-      }
-    } else if (t < CH) {
-      vs[t] = (t < rows) ? vc[r0 + t] : (T)0;
-    }
-    // 2. the tile V(r0:r0+CH, 0:c) into LDS, coalesced along rows
-    if (need_tile) {
-      const int row = t & (CH - 1), cg = t >> 6;
-      const bool in = row < rows;
-      const T *src = a.V + r0 + row;
-      int i = cg;
-      for (; i + 12 < c; i += 16) {
-        T x0 = in ? src[(size_t)i * a.ld] : (T)0;
-        T x1 = in ? src[(size_t)(i + 4) * a.ld] : (T)0;
-        T x2 = in ? src[(size_t)(i + 8) * a.ld] : (T)0;
-        T x3 = in ? src[(size_t)(i + 12) * a.ld] : (T)0;
-        Vs[i * LDP + row] = x0;
-        Vs[(i + 4) * LDP + row] = x1;
-        Vs[(i + 8) * LDP + row] = x2;
-        Vs[(i + 12) * LDP + row] = x3;
-      }
-      for (; i < c; i += 4) Vs[i * LDP + row] = in ? src[(size_t)i * a.ld] : (T)0;
-    }
-    __syncthreads();
-    // 3. update v = (v - V coef) * scale
-    if (do_upd) {
-      const int row = t & (CH - 1), cg = t >> 6;
-      T s = (T)0;
-      if (do_mult)
-        for (int i = cg; i < c; i += 4) s += Vs[i * LDP + row] * cf[i];
-      rt[t] = s;
-      __syncthreads();
-      if (t < CH) {
-        const T tot = (rt[t] + rt[t + 64]) + (rt[t + 128] + rt[t + 192]);
-        const T vn = (vs[t] - tot) * scl;
-        vs[t] = (t < rows) ? vn : (T)0;
-        if (t < rows) vc[r0 + t] = vn;
-      }
-      __syncthreads();
-    }
-    // 4. dot products with the (updated) vector, nsl threads per column
-    if (dact) {
-      const T *colp = (dcol < c) ? (Vs + dcol * LDP) : vs;
-      T s = (T)0;
-      for (int r = dsl; r < rows; r += nsl) s += colp[r] * vs[r];
-      acc += (double)s;
-    }
-    __syncthreads();
-  }
-  if (!do_dots) return;
-  s_acc[t] = acc;
-  __syncthreads();
-  if (t < ncol) {
-    double s = 0.0;
-    for (int q = 0; q < nsl; q++) s += s_acc[q * ncol + t];
-    if (a.inblock) s_dots[t] = s;
-    else a.part[(size_t)t * a.G + blockIdx.x] = s;
-  }
-  if (!a.inblock) return;
-  __syncthreads();
-  gs_decide(a.st, s_dots, c, a.slot, a.first, a.eta, a.dbuf, a.m, &s_record, &s_out);
-}
-
-// Reduction + decision after the full-grid step of `slot`.
-__global__ __launch_bounds__(RBLK) void gs_reduce_kernel(GsState *st, const double *part, int G, int c, int slot,
-                                                         int first, double eta, double *dbuf, int m) {
-  __shared__ double s_dots[GS_MAXC + 1];
-  __shared__ int s_go, s_record, s_out;
-  if (threadIdx.x == 0) {
-    int go;
-    if (st->stop && (first || c > st->stop_col)) go = 0;
-    else go = first ? 1 : (st->mode[slot] == GS_MULT_DOTS);
-    s_go = go;
-  }
-  __syncthreads();
-  if (!s_go) return;
-  sum_partials_block(part, G, c + 1, s_dots);
-  __syncthreads();
-  gs_decide(st, s_dots, c, slot, first, eta, dbuf, m, &s_record, &s_out);
-}
-
-__global__ void gs_reset_kernel(GsState *st) {
-  if (threadIdx.x == 0) {
-    st->stop = 0;
-    st->stop_col = 0;
-    st->pending = 0;
-    st->lindep = 0;
-    st->beta = 0.0;
-    for (int s = 0; s < GS_SLOTS; s++) st->mode[s] = GS_IDLE;
-  }
-}
-
-template <typename T>
-static int gs_launch(BV &V, const Mat *A, int c, double eta, hipStream_t s) {
-  GsArgs<T> a;
-  a.n = V.n; a.ld = V.ld; a.V = static_cast<T *>(V.data); a.c = c;
-  a.rp = A ? A->rowptr : nullptr; a.ci = A ? A->col : nullptr;
-  a.val = A ? static_cast<const T *>(A->val) : nullptr;
-  a.st = V.st; a.dbuf = V.dbuf; a.m = V.m;
-  a.part = V.ws.part;
-  a.eta = eta;
-  const size_t shm = ((size_t)c * LDP + CH + c + BLK) * sizeof(T);
-  SA_CHECK(shm + 4096 + 8 * BLK <= 65536, "basis too wide for the LDS tile of the Gram-Schmidt kernel");
-  const int lanes = A ? A->lanes : 8;
-  // Slots 1-3 (one or two passes, the common case) run on the full grid, each
-  // followed by a one-workgroup reduction.  Slots 4-5 (a third pass or a failed
-  // norm estimate) run as one workgroup over all rows that reduces in LDS:
-  // slow but rare, and an idle one costs one tiny launch.
-  for (int slot = 1; slot <= 5; slot++) {
-    a.slot = slot;
-    a.first = (slot == 1);
-    if (slot > 1) { a.rp = nullptr; a.ci = nullptr; a.val = nullptr; }
-    int grid = V.G;
-    a.G = V.G;
-    a.rpb = V.rows_per_block;
-    a.inblock = 0;
-    if (slot >= 4) {
-      grid = 1;
-      a.G = 1;
-      a.rpb = (int)((V.n + CH - 1) / CH * CH);
-      a.inblock = 1;
-    }
-    switch (lanes) {
-      case 4: hipLaunchKernelGGL((gs_kernel<T, 4>), dim3(grid), dim3(BLK), shm, s, a); break;
-      case 8: hipLaunchKernelGGL((gs_kernel<T, 8>), dim3(grid), dim3(BLK), shm, s, a); break;
-      case 16: hipLaunchKernelGGL((gs_kernel<T, 16>), dim3(grid), dim3(BLK), shm, s, a); break;
-      default: hipLaunchKernelGGL((gs_kernel<T, 32>), dim3(grid), dim3(BLK), shm, s, a); break;
-    }
-    if (slot <= 3)
-      hipLaunchKernelGGL(gs_reduce_kernel, dim3(1), dim3(RBLK), 0, s, V.st, (const double *)V.ws.part, V.G, c, slot,
-                         a.first, eta, V.dbuf, V.m);
-  }
-  SA_HIP(hipGetLastError());
-  return 0;
-}
-
-int launch_gs_column(BV &V, const Mat *A, int c, double eta, hipStream_t s) {
-  SA_CHECK(c >= 0 && c < GS_MAXC && c < V.m, "column index out of range for the Gram-Schmidt kernel");
-  SA_CHECK(!A || (A->prec == V.prec && A->n == V.n), "matrix and basis sizes or precisions differ");
-  SA_CHECK(!A || c > 0, "SpMV step needs a previous column");
-  if (V.prec == SA_FP64) return gs_launch<double>(V, A, c, eta, s);
-  return gs_launch<float>(V, A, c, eta, s);
-}
-
-int launch_gs_reset(BV &V, hipStream_t s) {
-  hipLaunchKernelGGL(gs_reset_kernel, dim3(1), dim3(64), 0, s, V.st);
-  SA_HIP(hipGetLastError());
-  return 0;
 }
 
 // ---------------------------------------------------------------------------
