@@ -10,7 +10,7 @@ import os
 import numpy as np
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(_HERE, "libslepc_amd.so")
+LIB_PATH = os.environ.get("SLEPC_AMD_LIBRARY") or os.path.join(_HERE, "libslepc_amd.so")
 _LIB = None
 
 FP64, FP32 = 0, 1

@@ -144,7 +144,19 @@ __device__ double gs_rows(const GsArgs<T> &a, int64_t rb, int64_t re, bool spmv,
   double acc = 0.0;
   for (int64_t r0 = rb; r0 < re; r0 += CH) {
     const int rows = (int)min((int64_t)CH, re - r0);
-    // 1. the vector chunk
+    const bool in = prow < rows;
+    const T *src = a.V + r0 + prow;
+    // 1. the first GS_UT tile loads, issued before the SpMV's dependent
+    //    gathers so that their latencies overlap
+    T x[GS_UT];
+    if (need_tile) {
+#pragma unroll
+      for (int u = 0; u < GS_UT; u++) {
+        const int i = pcg + 4 * u;
+        x[u] = (in && i < c) ? src[(size_t)i * a.ld] : (T)0;
+      }
+    }
+    // 2. the vector chunk
     if (spmv) {
       const T *x = a.V + (size_t)(c - 1) * a.ld;
       constexpr int RPS = BLK / L;
@@ -166,12 +178,15 @@ __device__ double gs_rows(const GsArgs<T> &a, int64_t rb, int64_t re, bool spmv,
     } else if (t < CH) {
       vs[t] = (t < rows) ? vc[r0 + t] : (T)0;
     }
-    // 2. the tile V(r0:r0+CH, 0:c) into LDS, coalesced along rows
+    // 3. the tile V(r0:r0+CH, 0:c) into LDS: the first batch from registers,
+    //    the rest loaded GS_UT at a time (coalesced along rows)
     if (need_tile) {
-      const bool in = prow < rows;
-      const T *src = a.V + r0 + prow;
-      for (int i0 = 0; i0 < c; i0 += 4 * GS_UT) {
-        T x[GS_UT];
+#pragma unroll
+      for (int u = 0; u < GS_UT; u++) {
+        const int i = pcg + 4 * u;
+        if (i < c) Vs[i * LDP + prow] = x[u];
+      }
+      for (int i0 = 4 * GS_UT; i0 < c; i0 += 4 * GS_UT) {
 #pragma unroll
         for (int u = 0; u < GS_UT; u++) {
           const int i = i0 + pcg + 4 * u;
@@ -185,7 +200,7 @@ __device__ double gs_rows(const GsArgs<T> &a, int64_t rb, int64_t re, bool spmv,
       }
     }
     __syncthreads();
-    // 3. update v = (v - V cf) * scl, four threads per row
+    // 4. update v = (v - V cf) * scl, four threads per row
     if (upd) {
       const int row = t & (CH - 1), cg = t >> 6;
       T s = (T)0;
@@ -201,7 +216,7 @@ __device__ double gs_rows(const GsArgs<T> &a, int64_t rb, int64_t re, bool spmv,
       }
       __syncthreads();
     }
-    // 4. dot products with the (updated) vector, nsl threads per column
+    // 5. dot products with the (updated) vector, nsl threads per column
     if (dact) {
       const T *colp = (dcol < c) ? (Vs + dcol * LDP) : vs;
       T s = (T)0;
@@ -311,11 +326,25 @@ __global__ __launch_bounds__(BLK) void gs_kernel(GsArgs<T> a) {
   if (dots) gs_combine(acc, ncol, s_acc, nullptr, a.part_out, a.G);
 }
 
-// One-workgroup reduction of a slot's partial sums, before the next slot.
-__global__ __launch_bounds__(RBLK) void gs_sum_kernel(const GsState *st, const double *part, int G, int ncol,
-                                                      double *dots) {
+// Reduction of a slot's partial sums before the next slot: one workgroup per
+// column, each thread adding at most G/BLK partials (one memory round trip),
+// then a fixed-order tree over the workgroup (deterministic).
+__global__ __launch_bounds__(BLK) void gs_sum_kernel(const GsState *st, const double *part, int G, double *dots) {
+  __shared__ double red[BLK / 64];
   if (!st->dots_ready) return;
-  sum_partials_block(part, G, ncol, dots);
+  const int col = blockIdx.x, t = threadIdx.x;
+  const double *p = part + (size_t)col * G;
+  double s = 0.0;
+  for (int b = t; b < G; b += BLK) s += p[b];
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) s += __shfl_xor(s, o, 64);
+  if ((t & 63) == 0) red[t >> 6] = s;
+  __syncthreads();
+  if (t == 0) {
+    double tot = 0.0;
+    for (int w = 0; w < BLK / 64; w++) tot += red[w];
+    dots[col] = tot;
+  }
 }
 
 __global__ void gs_reset_kernel(GsState *st) {
@@ -330,7 +359,7 @@ __global__ void gs_reset_kernel(GsState *st) {
 int gs_min_chunks() {
   static const int v = [] {
     const char *e = std::getenv("SLEPC_AMD_GS_MIN_CHUNKS");
-    const int k = e ? std::atoi(e) : 2;
+    const int k = e ? std::atoi(e) : 1;
     return k < 1 ? 1 : k;
   }();
   return v;
@@ -362,8 +391,8 @@ static int gs_launch(BV &V, const Mat *A, int c, double eta, hipStream_t s) {
       default: hipLaunchKernelGGL((gs_kernel<T, 32>), dim3(V.G), dim3(BLK), shm, s, a); break;
     }
     if (slot < 3)
-      hipLaunchKernelGGL(gs_sum_kernel, dim3(1), dim3(RBLK), 0, s, (const GsState *)(V.st + (slot - 1)),
-                         (const double *)a.part_out, V.G, c + 1, slot == 1 ? da : db);
+      hipLaunchKernelGGL(gs_sum_kernel, dim3(c + 1), dim3(BLK), 0, s, (const GsState *)(V.st + (slot - 1)),
+                         (const double *)a.part_out, V.G, slot == 1 ? da : db);
   }
   SA_HIP(hipGetLastError());
   return 0;
