@@ -12,10 +12,11 @@
 //   slot 3: dots of pass 2 -> decision -> v = (v - V h) * 1/norm; the rare
 //           third pass or failed norm estimate is finished by workgroup 0
 //           alone, over all rows, with in-workgroup reductions
-// A one-workgroup kernel adds the partial sums between the slots in a fixed
-// order (no fence or atomic: an agent-scope release on MI355X writes back the
-// XCD's L2).  Every workgroup of slots 2-3 reads the same sums and takes the
-// same decision; workgroup 0 records the coefficients, the norm and the state.
+// A reduction kernel (one workgroup per column) adds the partial sums between
+// the slots in a fixed order (no fence or atomic: an agent-scope release on
+// MI355X writes back the XCD's L2).  Every workgroup of slots 2-3 reads the
+// same sums and takes the same decision; workgroup 0 records the
+// coefficients, the norm and the state.
 // Slot s reads the state st[(s+1)%3] and writes st[s-1], so no workgroup reads
 // a state that is being written.
 #include <cstdlib>
@@ -158,7 +159,7 @@ __device__ double gs_rows(const GsArgs<T> &a, int64_t rb, int64_t re, bool spmv,
     }
     // 2. the vector chunk
     if (spmv) {
-      const T *x = a.V + (size_t)(c - 1) * a.ld;
+      const T *xin = a.V + (size_t)(c - 1) * a.ld;
       constexpr int RPS = BLK / L;
       const int g = t / L, lg = t % L;
       for (int rr = g; rr < CH; rr += RPS) {
@@ -166,7 +167,7 @@ __device__ double gs_rows(const GsArgs<T> &a, int64_t rb, int64_t re, bool spmv,
         if (rr < rows) {
           const int64_t row = r0 + rr;
           const int32_t pe = a.rp[row + 1];
-          for (int32_t p = a.rp[row] + lg; p < pe; p += L) s += a.val[p] * x[a.ci[p]];
+          for (int32_t p = a.rp[row] + lg; p < pe; p += L) s += a.val[p] * xin[a.ci[p]];
         }
 #pragma unroll
         for (int off = L / 2; off > 0; off >>= 1) s += __shfl_xor(s, off, L);
