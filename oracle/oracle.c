@@ -283,6 +283,13 @@ static int lp_try(const char *path)
   if (!(LP.lartg && LP.rot && LP.steqr && LP.gehrd && LP.orghr && LP.hseqr && LP.trexc && LP.trevc)) { dlclose(h); return 0; }
   void (*setlayer)(int) = (void (*)(int))dlsym(h, "MKL_Set_Threading_Layer");
   if (setlayer) setlayer(1); /* MKL_THREADING_SEQUENTIAL */
+  /* the projected problems are <= 97 x 97: one BLAS thread (a pool sized to
+     every core of a large host only adds wake-up latency) */
+  const char *nts[] = {"scipy_openblas_set_num_threads", "openblas_set_num_threads", NULL};
+  for (int i = 0; nts[i]; i++) {
+    void (*setnt)(int) = (void (*)(int))dlsym(h, nts[i]);
+    if (setnt) { setnt(1); break; }
+  }
   snprintf(LP.path, sizeof LP.path, "%s", path);
   return 1;
 }

@@ -47,8 +47,6 @@ int bv_alloc(BV &V, int64_t n, int m, int prec) {
 }
 
 void bv_free(BV &V) {
-  for (auto &g : V.graphs)
-    if (g.exec) (void)hipGraphExecDestroy(g.exec);
   if (V.data) (void)hipFree(V.data);
   if (V.dbuf) (void)hipFree(V.dbuf);
   if (V.st) (void)hipFree(V.st);
@@ -60,66 +58,19 @@ void bv_free(BV &V) {
   V = BV();
 }
 
-static bool graphs_enabled() {
-  static const bool on = [] {
-    const char *e = std::getenv("SLEPC_AMD_GRAPHS");
-    return !(e && e[0] == '0');
-  }();
-  return on;
-}
-
-// The launches of one sweep, captured once per (operator, k, m, eta) into a
-// HIP graph and replayed on later restarts with the same range: one graph
-// launch replaces 5(m-k)+1 kernel launches.  The graph holds kernels only.
-static int sweep_graph(BV &V, const Mat &A, int k, int m, double eta, hipStream_t s, hipGraphExec_t *out) {
-  *out = nullptr;
-  for (auto &g : V.graphs)
-    if (g.rp == A.rowptr && g.ci == A.col && g.val == A.val && g.lanes == A.lanes && g.k == k && g.m == m &&
-        g.eta == eta) {
-      *out = g.exec;
-      return 0;
-    }
-  if (V.graphs.size() >= 256) return 0;      // cache full: launch directly
-  SA_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
-  int rc = launch_gs_reset(V, s);
-  for (int j = k; j < m && !rc; j++) rc = launch_gs_column(V, &A, j + 1, eta, s);
-  hipGraph_t graph = nullptr;
-  const hipError_t ec = hipStreamEndCapture(s, &graph);
-  if (rc || ec != hipSuccess) {
-    if (graph) (void)hipGraphDestroy(graph);
-    if (rc) return rc;
-    SA_HIP(ec);
-  }
-  hipGraphExec_t exec = nullptr;
-  const hipError_t ei = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
-  (void)hipGraphDestroy(graph);
-  SA_HIP(ei);
-  SweepGraph g;
-  g.rp = A.rowptr; g.ci = A.col; g.val = A.val; g.lanes = A.lanes;
-  g.k = k; g.m = m; g.eta = eta; g.exec = exec;
-  V.graphs.push_back(g);
-  *out = exec;
-  return 0;
-}
-
 // BVMatArnoldi/BVMatLanczos (bvkrylov.c:56-113,165-226): for j = k..m-1,
 // V(:,j+1) = A V(:,j) followed by BVOrthonormalizeColumn(V,j+1).  All steps
-// are enqueued back to back (as one graph launch when graphs are on); the
-// device state machine applies the DGKS rule and stops the sweep at a
-// breakdown.  One synchronisation at the end reads the state (breakdown
-// column and the last norm).
+// are enqueued back to back; the device state machine applies the DGKS rule
+// and stops the sweep at a breakdown.  One synchronisation at the end reads
+// the state (breakdown column and the last norm).  (Replaying the sweep as a
+// captured HIP graph measured no faster: the kernels run back to back with
+// no gaps, so the time is inside them.)
 int bv_sweep(BV &V, const Mat &A, int k, int *m, double eta, double *beta, int *breakdown) {
   SA_CHECK(k >= 0 && *m > k && *m < V.m, "invalid Arnoldi range");
   SA_CHECK(A.prec == V.prec && A.n == V.n, "matrix and basis sizes or precisions differ");
   hipStream_t s = stream();
-  hipGraphExec_t exec = nullptr;
-  if (graphs_enabled()) SA_TRY(sweep_graph(V, A, k, *m, eta, s, &exec));
-  if (exec) {
-    SA_HIP(hipGraphLaunch(exec, s));
-  } else {
-    SA_TRY(launch_gs_reset(V, s));
-    for (int j = k; j < *m; j++) SA_TRY(launch_gs_column(V, &A, j + 1, eta, s));
-  }
+  SA_TRY(launch_gs_reset(V, s));
+  for (int j = k; j < *m; j++) SA_TRY(launch_gs_column(V, &A, j + 1, eta, s));
   SA_HIP(hipMemcpyAsync(V.st_host, V.st + 2, sizeof(GsState), hipMemcpyDeviceToHost, s));
   SA_HIP(hipStreamSynchronize(s));
   const GsState &st = *V.st_host;

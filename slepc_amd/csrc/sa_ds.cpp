@@ -67,6 +67,14 @@ bool try_lib(const char *path) {
     return false;
   }
   if (auto layer = (void (*)(int))dlsym(h, "MKL_Set_Threading_Layer")) layer(1);  // sequential
+  // Projected problems are <= 129 x 129: one BLAS thread per caller (an
+  // OpenBLAS pool sized to every core of the host only adds wake-up latency,
+  // and concurrent solves run their DS on their own host threads).
+  for (const char *nm : {"scipy_openblas_set_num_threads", "openblas_set_num_threads"})
+    if (auto setnt = (void (*)(int))dlsym(h, nm)) {
+      setnt(1);
+      break;
+    }
   L.ok = true;
   L.path = path;
   LA = L;

@@ -50,15 +50,6 @@ struct GsState {
   double onrm, nrm, beta;
 };
 
-// An instantiated HIP graph of one sweep (reset + columns k+1..m); the kernel
-// arguments baked into it are fixed by the basis and by this key.
-struct SweepGraph {
-  const void *rp = nullptr, *ci = nullptr, *val = nullptr;
-  int lanes = 0, k = 0, m = 0;
-  double eta = 0.0;
-  hipGraphExec_t exec = nullptr;
-};
-
 struct Workspace {           // device scratch for partial sums of one BV
   double *part = nullptr;    // 2 x [GS_MAXC+1][maxG], column-major partial sums
   double *dots = nullptr;    // 2 x [GS_MAXC+1] reduced dot products
@@ -73,7 +64,6 @@ struct BV {
   double *dbuf = nullptr;      // coefficient buffer m x m (FP64) on device
   GsState *st = nullptr;       // device, GS_NSTATE states (ping-pong)
   GsState *st_host = nullptr;  // pinned host mirror of st[2]
-  std::vector<SweepGraph> graphs;   // captured Arnoldi sweeps, reused across restarts
   Workspace ws;
   double *dq = nullptr;        // device staging for Q / q vectors (m*m doubles)
   double *hq = nullptr;        // pinned host staging
