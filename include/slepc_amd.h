@@ -76,6 +76,9 @@ int64_t sa_gen_fv2d(int nx, int ny, const double *theta8, double *sigma,
                     int32_t *rowptr, int32_t *col, double *val);
 int64_t sa_gen_markov(int m, int32_t *rowptr, int32_t *col, double *val);
 void    sa_fv2d_theta(uint64_t seed, int sample, double *theta8);
+void    sa_rand_uniform(uint64_t seed, int64_t n, double *out_host); /* U[-1,1) */
+double  sa_u01(uint64_t seed, uint64_t i);
+uint64_t sa_sub_seed(uint64_t seed, uint64_t p);
 
 /* ---- file formats ---------------------------------------------------------
  * MatLoad of a PETSc binary AIJ file (as src/eps/tutorials/ex4.c loads the
@@ -94,9 +97,6 @@ int sa_vec_load_petsc_binary(const char *path, int64_t index, int64_t *n, double
    most len-1 characters plus a zero into buf when buf != NULL; -1 on error. */
 int64_t sa_format_eigenvalues(int n, const double *re, const double *im, int format, const char *name,
                               char *buf, int64_t len);
-void    sa_rand_uniform(uint64_t seed, int64_t n, double *out_host); /* U[-1,1) */
-double  sa_u01(uint64_t seed, uint64_t i);
-uint64_t sa_sub_seed(uint64_t seed, uint64_t p);
 
 /* ---- BV: basis vectors ------------------------------------------------------
  * Replaces BVCreate_Mat storage (src/sys/classes/bv/impls/mat/bvmat.c:385-524)
@@ -205,7 +205,7 @@ int sa_batch_create(int P, const int32_t *n, const int32_t *rowptr, const int32_
 int sa_batch_destroy(sa_batch batch);
 /* mixed != 0: FP32 bulk to tol32, then FP64 polish seeded with the sum of the
    FP32 Ritz vectors.  v0 optional (concatenated start vectors); NULL draws
-   U[-1,1) with seed sa_sub_seed(seed, p).  ncv < 64. */
+   U[-1,1) with seed sa_sub_seed(seed, p).  ncv <= 64. */
 int sa_batch_solve(sa_batch batch, int which, int nev, int ncv, double tol, int max_it,
                    int mixed, double tol32, const double *v0, uint64_t seed);
 int sa_batch_get_converged(sa_batch batch, int p, int *nconv, int *its, int *reason);
@@ -215,6 +215,14 @@ int sa_batch_compute_errors(sa_batch batch, int nev, double *err);
 /* counters6: matvecs, passes, restart rounds (FP64), then the same for FP32;
    seconds3: total, FP32 phase, host projected problems. */
 int sa_batch_get_stats(sa_batch batch, int64_t *counters6, double *seconds3);
+/* Sweep kernel of the batch: 0 autotune (default; both are timed on the first
+   restart round of each solve phase and the faster is kept), 1 one workgroup
+   per problem runs the whole sweep (ncv < 64), 2 lock-step: each Gram-Schmidt
+   slot of every active problem is one launch over all their 64-row chunks.
+   The getter returns the mode used by the last FP64 phase and the autotuner's
+   timings of one sweep (milliseconds, -1 when not timed). */
+int sa_batch_set_sweep(sa_batch batch, int mode);
+int sa_batch_get_sweep(sa_batch batch, int *mode, double *ms_persistent, double *ms_wide);
 
 /* ---- learned warm start: MLP in BF16 on MFMA ---------------------------------
  * theta (din <= 32) -> hidden -> hidden -> dout, ReLU, weights and activations

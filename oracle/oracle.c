@@ -364,11 +364,20 @@ void orc_bv_dotvec(int64_t n, int k, const double *V, int64_t ld, const double *
 /* y = beta*y + alpha*V(:,0:k) q : BVMultVec_BLAS_Private (bvblas.c:56-67) */
 void orc_bv_multvec(int64_t n, int k, double alpha, const double *V, int64_t ld, const double *q, double beta, double *y)
 {
+  /* rows in blocks, columns streamed one after the other inside a block (the
+     same additions per row, in the same order, as a row-by-row loop) */
+  const int64_t RB = 1024;
 #pragma omp parallel for schedule(static)
-  for (int64_t r = 0; r < n; r++) {
-    double s = 0.0;
-    for (int c = 0; c < k; c++) s += V[r + (int64_t)c * ld] * q[c];
-    y[r] = beta * y[r] + alpha * s;
+  for (int64_t r0 = 0; r0 < n; r0 += RB) {
+    const int64_t r1 = r0 + RB < n ? r0 + RB : n;
+    double s[1024];
+    for (int64_t r = r0; r < r1; r++) s[r - r0] = 0.0;
+    for (int c = 0; c < k; c++) {
+      const double *vc = V + (int64_t)c * ld;
+      const double qc = q[c];
+      for (int64_t r = r0; r < r1; r++) s[r - r0] += vc[r] * qc;
+    }
+    for (int64_t r = r0; r < r1; r++) y[r] = beta * y[r] + alpha * s[r - r0];
   }
 }
 

@@ -1,6 +1,7 @@
 """Batched Krylov-Schur over many independent symmetric problems (C ABI
-sa_batch_*): one block-diagonal operator, one basis, one kernel launch per
-restart phase with a workgroup per problem."""
+sa_batch_*): one block-diagonal operator, one basis; each restart sweep is
+either one launch with a workgroup per problem, or lock-step launches per
+Gram-Schmidt slot over the row chunks of all problems (autotuned)."""
 import ctypes as C
 
 import numpy as np
@@ -19,7 +20,12 @@ def _sig():
     L.sa_batch_get_eigenpair.argtypes = [vp, C.c_int, C.c_int, C.POINTER(C.c_double), vp]
     L.sa_batch_compute_errors.argtypes = [vp, C.c_int, vp]
     L.sa_batch_get_stats.argtypes = [vp, vp, vp]
+    L.sa_batch_set_sweep.argtypes = [vp, C.c_int]
+    L.sa_batch_get_sweep.argtypes = [vp, C.POINTER(C.c_int), C.POINTER(C.c_double), C.POINTER(C.c_double)]
     return L
+
+
+SWEEP_MODES = {"auto": 0, "persistent": 1, "wide": 2}
 
 
 def uneven_laplacians(P=512, seed=2, lo=32, hi=256, pot=10.0):
@@ -82,6 +88,17 @@ class Batch:
         check(self.L.sa_batch_get_stats(self.h, c.ctypes.data, s.ctypes.data))
         return dict(matvecs=int(c[0]), passes=int(c[1]), rounds=int(c[2]), matvecs32=int(c[3]),
                     passes32=int(c[4]), rounds32=int(c[5]), seconds=s[0], seconds32=s[1], seconds_ds=s[2])
+
+    def set_sweep(self, mode):
+        """"auto" (time both sweep kernels on the first restart round), "persistent"
+        (a workgroup per problem, ncv < 64) or "wide" (lock-step over row chunks)."""
+        check(self.L.sa_batch_set_sweep(self.h, SWEEP_MODES[mode]))
+
+    def sweep_info(self):
+        m, a, b = C.c_int(), C.c_double(), C.c_double()
+        check(self.L.sa_batch_get_sweep(self.h, C.byref(m), C.byref(a), C.byref(b)))
+        names = {v: k for k, v in SWEEP_MODES.items()}
+        return dict(mode=names.get(m.value, "none"), ms_persistent=a.value, ms_wide=b.value)
 
     def destroy(self):
         if self.h:

@@ -7,20 +7,27 @@
 // Gram-Schmidt with DGKS refinement, normalisation) of problem order[b], so all
 // synchronisation is inside the workgroup and no grid-wide barrier exists.
 // Problems are ordered largest first so the hardware scheduler balances CUs.
+//
+// A second sweep ("wide", lock-step) runs each Gram-Schmidt slot of every
+// active problem as ONE launch over the 64-row chunks of all those problems
+// (the slot kernels of sa_gs_device.h with per-problem state), so problems
+// with many rows are spread over many workgroups.  An autotuner times both
+// sweeps on the first restart round of a solve and keeps the faster one.
 #include <algorithm>
 #include <chrono>
 #include <cmath>
+#include <cstdlib>
 #include <cstring>
 #include <numeric>
 #include <thread>
 #include <vector>
 
 #include "sa_ds.h"
+#include "sa_gs_device.h"
 #include "sa_internal.h"
 
 namespace sa {
 
-constexpr int BMAXC = 128;   // basis columns handled by the batched kernels
 
 template <typename T>
 struct BatchArgs {
@@ -256,6 +263,101 @@ __global__ void batched_scale_kernel(T *V, int64_t ldv, const int *chunk_prob, c
 }
 
 // ---------------------------------------------------------------------------
+// Lock-step ("wide") sweep.  Step j orthogonalises column c_p = k0[p]+1+j of
+// every listed problem p (c_p = k0[p] without SpMV); problems whose sweep is
+// shorter keep their state chain and skip the work.  One workgroup per active
+// 64-row chunk; per problem, partial sums are [c+1][G_p] at chunk offset
+// pc0[p] * m, reduced dots are [m] at p * m, states are GS_NSTATE at p.
+// ---------------------------------------------------------------------------
+template <typename T>
+struct BwArgs {
+  T *V;
+  int64_t ldv;
+  const int32_t *rp, *ci;
+  const T *val;
+  const int64_t *off;
+  const int *n, *k0, *nv;
+  const int *cprob, *crow;    // active chunks: problem, first local row
+  const int *pc0, *pnc;       // per problem: first active chunk, number of chunks
+  int j, spmv, slot, m;
+  GsState *st;
+  double *dbuf;
+  double *part_out;
+  const double *dots_in;
+  double eta;
+};
+
+__device__ __forceinline__ int bw_col(const int *k0, int p, int spmv, int j) { return k0[p] + (spmv ? 1 + j : 0); }
+
+template <typename T, int L>
+__global__ __launch_bounds__(BLK) void bw_slot_kernel(BwArgs<T> b) {
+  const int g = blockIdx.x, p = b.cprob[g];
+  const int c = bw_col(b.k0, p, b.spmv, b.j);
+  const int bidx = g - b.pc0[p];
+  GsState *st = b.st + (size_t)GS_NSTATE * p;
+  if (c > b.nv[p]) {           // this problem's sweep ended: carry its state forward
+    if (bidx == 0 && threadIdx.x == 0) {
+      GsState S = st[(b.slot + 1) % GS_NSTATE];
+      S.done = 1;
+      S.dots_ready = 0;
+      st[b.slot - 1] = S;
+    }
+    return;
+  }
+  GsArgs<T> a;
+  a.n = b.n[p];
+  a.ld = b.ldv;
+  a.V = b.V + b.off[p];
+  a.c = c;
+  a.slot = b.slot;
+  const bool sp = b.spmv && b.slot == 1;
+  a.rp = sp ? b.rp + b.off[p] : nullptr;      // row pointers of the problem's rows (global nnz index)
+  a.ci = sp ? b.ci : nullptr;                 // global column indices of the block-diagonal operator
+  a.val = sp ? b.val : nullptr;
+  a.xin = sp ? b.V + (size_t)(c - 1) * b.ldv : nullptr;
+  a.st = st;
+  a.dbuf = b.dbuf + (size_t)p * b.m * b.m;
+  a.m = b.m;
+  a.dots_in = b.dots_in + (size_t)p * b.m;
+  a.part_out = b.part_out + (size_t)b.pc0[p] * b.m;
+  a.G = b.pnc[p];
+  a.rpb = CH;
+  a.eta = b.eta;
+  const int64_t rb = b.crow[g];
+  gs_slot<T, L>(a, rb, min(a.n, rb + CH), bidx);
+}
+
+__global__ __launch_bounds__(BLK) void bw_sum_kernel(const int *list, const int *k0, const int *nv, int spmv, int j,
+                                                     int slot, const GsState *st, const double *part, const int *pc0,
+                                                     const int *pnc, int m, double *dots) {
+  const int p = list[blockIdx.y], col = blockIdx.x;
+  const int c = bw_col(k0, p, spmv, j);
+  if (c > nv[p] || col > c) return;
+  // (summed even when the slot produced no dots: the next slot then ignores them)
+  gs_sum_column(part + (size_t)pc0[p] * m, pnc[p], col, dots + (size_t)p * m);
+}
+
+__global__ void bw_reset_kernel(const int *list, int nl, GsState *st) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= nl) return;
+  GsState &S = st[(size_t)GS_NSTATE * list[i] + 2];
+  S.stop = 0; S.stop_col = 0; S.pending = 0; S.lindep = 0; S.pass = 0;
+  S.done = 0; S.dots_ready = 0; S.beta = 0.0;
+}
+
+// Results of a wide sweep in the persistent sweep's output arrays.
+__global__ void bw_export_kernel(const int *list, int nl, GsState *st, int *stop, double *beta, int *passes) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= nl) return;
+  const int p = list[i];
+  GsState &S = st[(size_t)GS_NSTATE * p + 2];
+  stop[p] = S.stop ? S.stop_col : -1;
+  beta[p] = S.beta;
+  passes[p] += (int)S.passes_total;
+  S.passes_total = 0;
+}
+
+// ---------------------------------------------------------------------------
 // host side
 // ---------------------------------------------------------------------------
 struct BatchDev {
@@ -270,6 +372,11 @@ struct BatchDev {
   double *dbuf = nullptr, *beta = nullptr, *Q = nullptr, *alpha = nullptr;
   int *chunk_prob = nullptr, *chunk_row = nullptr, nchunks = 0;
   int *ml = nullptr, *mk = nullptr, *ms = nullptr, *me = nullptr, *src = nullptr, *dst = nullptr;
+  // lock-step sweep
+  GsState *wst = nullptr;
+  double *wpart = nullptr, *wdots = nullptr;   // 2 x [nchunks][m], 2 x [P][m]
+  int *cprob = nullptr, *crow = nullptr, *pc0 = nullptr, *pnc = nullptr;
+  int wm = 0;                                  // basis width the wide buffers were sized for
 };
 
 template <typename T, int CM, int NT>
@@ -302,12 +409,123 @@ static int launch_sweep(const BatchDev &B, int nact, int spmv, double eta, hipSt
   return 0;
 }
 
+// The one-workgroup-per-problem sweep keeps acc[c] in registers up to 64 columns.
+static bool persistent_fits(const BatchDev &B) { return B.m - 1 < 64; }
+
+template <typename T>
+static void launch_bw_slot(const BwArgs<T> &a, int lanes, int grid, size_t shm, hipStream_t s) {
+  switch (lanes) {
+    case 4: hipLaunchKernelGGL((bw_slot_kernel<T, 4>), dim3(grid), dim3(BLK), shm, s, a); break;
+    case 8: hipLaunchKernelGGL((bw_slot_kernel<T, 8>), dim3(grid), dim3(BLK), shm, s, a); break;
+    case 16: hipLaunchKernelGGL((bw_slot_kernel<T, 16>), dim3(grid), dim3(BLK), shm, s, a); break;
+    default: hipLaunchKernelGGL((bw_slot_kernel<T, 32>), dim3(grid), dim3(BLK), shm, s, a); break;
+  }
+}
+
+template <typename T>
+static int wide_sweep_t(BatchDev &B, const std::vector<int> &list, const std::vector<int> &k0,
+                        const std::vector<int> &nv, int nch, int spmv, double eta, hipStream_t s) {
+  const int P = B.P, m = B.m, nl = (int)list.size();
+  int steps = 1;
+  if (spmv) {
+    steps = 0;
+    for (int p : list) steps = std::max(steps, nv[p] - k0[p]);
+  }
+  hipLaunchKernelGGL(bw_reset_kernel, dim3((nl + 255) / 256), dim3(256), 0, s, B.order, nl, B.wst);
+  double *pa = B.wpart, *pb = B.wpart + (size_t)B.nchunks * m;
+  double *da = B.wdots, *db = B.wdots + (size_t)P * m;
+  BwArgs<T> a;
+  a.V = (T *)B.V; a.ldv = B.ldv; a.rp = B.rp; a.ci = B.ci; a.val = (const T *)B.val;
+  a.off = B.off; a.n = B.n; a.k0 = B.k0; a.nv = B.nv;
+  a.cprob = B.cprob; a.crow = B.crow; a.pc0 = B.pc0; a.pnc = B.pnc;
+  a.spmv = spmv; a.m = m; a.st = B.wst; a.dbuf = B.dbuf; a.eta = eta;
+  for (int j = 0; j < steps; j++) {
+    int cmax = 0;
+    for (int p : list) {
+      const int c = k0[p] + (spmv ? 1 + j : 0);
+      if (c <= nv[p]) cmax = std::max(cmax, c);
+    }
+    const size_t shm = gs_lds_bytes(cmax, sizeof(T));
+    SA_CHECK(shm + 4096 + 8 * BLK <= 65536, "basis too wide for the LDS tile of the Gram-Schmidt kernel");
+    a.j = j;
+    for (int slot = 1; slot <= 3; slot++) {
+      a.slot = slot;
+      a.part_out = slot == 1 ? pa : pb;
+      a.dots_in = slot == 2 ? da : db;
+      launch_bw_slot<T>(a, B.lanes, nch, shm, s);
+      if (slot < 3)
+        hipLaunchKernelGGL(bw_sum_kernel, dim3(cmax + 1, nl), dim3(BLK), 0, s, (const int *)B.order,
+                           (const int *)B.k0, (const int *)B.nv, spmv, j, slot, (const GsState *)B.wst,
+                           (const double *)a.part_out, (const int *)B.pc0, (const int *)B.pnc, m,
+                           slot == 1 ? da : db);
+    }
+  }
+  hipLaunchKernelGGL(bw_export_kernel, dim3((nl + 255) / 256), dim3(256), 0, s, (const int *)B.order, nl, B.wst,
+                     B.stop, B.beta, B.passes);
+  SA_HIP(hipGetLastError());
+  return 0;
+}
+
+// Lock-step sweep of the listed problems; their k0/nv (host copies given) and
+// the list itself (B.order) are already on the device.  Buffers are allocated
+// on first use for the current basis width.
+static int wide_sweep(BatchDev &B, const std::vector<int> &nrows, const std::vector<int> &list,
+                      const std::vector<int> &k0, const std::vector<int> &nv, int spmv, double eta, hipStream_t s) {
+  if (list.empty()) return 0;
+  const int P = B.P, m = B.m;
+  if (B.wm != m) {
+    for (double **q : {&B.wpart, &B.wdots})
+      if (*q) { (void)hipFree(*q); *q = nullptr; }
+    if (!B.wst) {
+      SA_HIP(hipMalloc(&B.wst, sizeof(GsState) * GS_NSTATE * P));
+      SA_HIP(hipMemsetAsync(B.wst, 0, sizeof(GsState) * GS_NSTATE * P, s));
+      for (int **q : {&B.cprob, &B.crow}) SA_HIP(hipMalloc(q, sizeof(int) * std::max(1, B.nchunks)));
+      for (int **q : {&B.pc0, &B.pnc}) SA_HIP(hipMalloc(q, sizeof(int) * P));
+    }
+    SA_HIP(hipMalloc(&B.wpart, sizeof(double) * 2 * (size_t)std::max(1, B.nchunks) * m));
+    SA_HIP(hipMalloc(&B.wdots, sizeof(double) * 2 * (size_t)P * m));
+    SA_HIP(hipMemsetAsync(B.wpart, 0, sizeof(double) * 2 * (size_t)std::max(1, B.nchunks) * m, s));
+    SA_HIP(hipMemsetAsync(B.wdots, 0, sizeof(double) * 2 * (size_t)P * m, s));
+    B.wm = m;
+  }
+  std::vector<int> cp, cr, pc0(P, 0), pnc(P, 0);
+  for (int p : list) {
+    pc0[p] = (int)cp.size();
+    for (int r = 0; r < nrows[p]; r += CH) {
+      cp.push_back(p);
+      cr.push_back(r);
+    }
+    pnc[p] = (int)cp.size() - pc0[p];
+  }
+  const int nch = (int)cp.size();
+  SA_CHECK(nch <= B.nchunks, "chunk list larger than the batch");
+  SA_HIP(hipMemcpyAsync(B.cprob, cp.data(), sizeof(int) * nch, hipMemcpyHostToDevice, s));
+  SA_HIP(hipMemcpyAsync(B.crow, cr.data(), sizeof(int) * nch, hipMemcpyHostToDevice, s));
+  SA_HIP(hipMemcpyAsync(B.pc0, pc0.data(), sizeof(int) * P, hipMemcpyHostToDevice, s));
+  SA_HIP(hipMemcpyAsync(B.pnc, pnc.data(), sizeof(int) * P, hipMemcpyHostToDevice, s));
+  int rc = B.prec == SA_FP64 ? wide_sweep_t<double>(B, list, k0, nv, nch, spmv, eta, s)
+                             : wide_sweep_t<float>(B, list, k0, nv, nch, spmv, eta, s);
+  // the host vectors must outlive the asynchronous copies
+  SA_HIP(hipStreamSynchronize(s));
+  return rc;
+}
+
+// Sweep modes: 1 one workgroup per problem (persistent), 2 lock-step (wide).
+static int run_sweep(BatchDev &B, int mode, const std::vector<int> &nrows, const std::vector<int> &list,
+                     const std::vector<int> &k0, const std::vector<int> &nv, int spmv, double eta, hipStream_t s) {
+  if (mode == 1) return launch_sweep(B, (int)list.size(), spmv, eta, s);
+  return wide_sweep(B, nrows, list, k0, nv, spmv, eta, s);
+}
+
 }  // namespace sa
 
 using namespace sa;
 
 struct sa_batch_s {
   int P = 0;
+  int sweep_req = 0;                    // 0 autotune, 1 persistent, 2 lock-step
+  int sweep_used = 0;                   // mode of the last solve's FP64 phase
+  double ms_persistent = -1.0, ms_wide = -1.0;   // autotuner timings of one restart sweep
   std::vector<int> n;
   std::vector<int64_t> off;
   int64_t nrows = 0, nnz = 0;
@@ -325,8 +543,10 @@ struct sa_batch_s {
 };
 
 static void free_dev(BatchDev &B) {
-  void *ptrs[] = {B.V, B.rp, B.ci, B.val, B.off, B.n, B.order, B.k0, B.nv, B.stop, B.passes, B.dbuf, B.beta,
-                  B.Q, B.alpha, B.chunk_prob, B.chunk_row, B.ml, B.mk, B.ms, B.me, B.src, B.dst};
+  void *ptrs[] = {B.V,   B.rp,    B.ci,         B.val,        B.off,  B.n,    B.order, B.k0,  B.nv,
+                  B.stop, B.passes, B.dbuf,     B.beta,       B.Q,    B.alpha, B.chunk_prob, B.chunk_row,
+                  B.ml,  B.mk,    B.ms,         B.me,         B.src,  B.dst,  B.wst,   B.wpart, B.wdots,
+                  B.cprob, B.crow, B.pc0,       B.pnc};
   for (void *p : ptrs)
     if (p) (void)hipFree(p);
   B = BatchDev();
@@ -450,7 +670,11 @@ static int batch_ks(sa_batch_s *b, BatchDev &B, int which, int nev, int ncv, dou
   SA_HIP(hipMemcpyAsync(B.k0, k0.data(), sizeof(int) * P, hipMemcpyHostToDevice, s));
   SA_HIP(hipMemcpyAsync(B.nv, nv.data(), sizeof(int) * P, hipMemcpyHostToDevice, s));
   SA_HIP(hipMemcpyAsync(B.stop, stop.data(), sizeof(int) * P, hipMemcpyHostToDevice, s));
-  SA_TRY(launch_sweep(B, P, 0, eta, s));
+  // sweep mode: requested, or decided by timing both on the first restart round
+  int mode = b->sweep_req;
+  SA_CHECK(mode != 1 || persistent_fits(B), "the one-workgroup-per-problem sweep handles ncv < 64");
+  if (mode == 0 && !persistent_fits(B)) mode = 2;
+  SA_TRY(run_sweep(B, mode == 2 ? 2 : 1, b->n, order, k0, nv, 0, eta, s));
   SA_HIP(hipMemcpyAsync(stop.data(), B.stop, sizeof(int) * P, hipMemcpyDeviceToHost, s));
   SA_HIP(hipStreamSynchronize(s));
   for (int p = 0; p < P; p++) SA_CHECK(stop[p] < 0, "a start vector is zero");
@@ -479,7 +703,29 @@ static int batch_ks(sa_batch_s *b, BatchDev &B, int which, int nev, int ncv, dou
     SA_HIP(hipMemcpyAsync(B.k0, k0.data(), sizeof(int) * P, hipMemcpyHostToDevice, s));
     SA_HIP(hipMemcpyAsync(B.nv, nv.data(), sizeof(int) * P, hipMemcpyHostToDevice, s));
     SA_HIP(hipMemcpyAsync(B.stop, stop.data(), sizeof(int) * P, hipMemcpyHostToDevice, s));
-    SA_TRY(launch_sweep(B, (int)act.size(), 1, eta, s));
+    if (mode == 0) {
+      // autotune: the same sweep (it reads only columns <= k0) with both
+      // kernels, timed with events; the second run's results are kept
+      hipEvent_t ev[3];
+      for (auto &e : ev) SA_HIP(hipEventCreate(&e));
+      SA_HIP(hipEventRecord(ev[0], s));
+      SA_TRY(run_sweep(B, 1, b->n, act, k0, nv, 1, eta, s));
+      SA_HIP(hipEventRecord(ev[1], s));
+      SA_HIP(hipMemcpyAsync(B.stop, stop.data(), sizeof(int) * P, hipMemcpyHostToDevice, s));
+      SA_TRY(run_sweep(B, 2, b->n, act, k0, nv, 1, eta, s));
+      SA_HIP(hipEventRecord(ev[2], s));
+      SA_HIP(hipEventSynchronize(ev[2]));
+      float t1 = 0.f, t2 = 0.f;
+      SA_HIP(hipEventElapsedTime(&t1, ev[0], ev[1]));
+      SA_HIP(hipEventElapsedTime(&t2, ev[1], ev[2]));
+      for (auto &e : ev) (void)hipEventDestroy(e);
+      b->ms_persistent = t1;
+      b->ms_wide = t2;
+      mode = t2 < t1 ? 2 : 1;
+    } else {
+      SA_TRY(run_sweep(B, mode, b->n, act, k0, nv, 1, eta, s));
+    }
+    b->sweep_used = mode;
     dbuf.resize((size_t)P * m * m);
     SA_HIP(hipMemcpyAsync(dbuf.data(), B.dbuf, sizeof(double) * (size_t)P * m * m, hipMemcpyDeviceToHost, s));
     SA_HIP(hipMemcpyAsync(beta.data(), B.beta, sizeof(double) * P, hipMemcpyDeviceToHost, s));
@@ -588,7 +834,7 @@ static int batch_ks(sa_batch_s *b, BatchDev &B, int which, int nev, int ncv, dou
       SA_HIP(hipMemcpyAsync(B.order, rs.data(), sizeof(int) * rs.size(), hipMemcpyHostToDevice, s));
       SA_HIP(hipMemcpyAsync(B.k0, k0.data(), sizeof(int) * P, hipMemcpyHostToDevice, s));
       SA_HIP(hipMemcpyAsync(B.stop, st2.data(), sizeof(int) * P, hipMemcpyHostToDevice, s));
-      SA_TRY(launch_sweep(B, (int)rs.size(), 0, eta, s));
+      SA_TRY(run_sweep(B, mode, b->n, rs, k0, nv, 0, eta, s));
       SA_HIP(hipMemcpyAsync(st2.data(), B.stop, sizeof(int) * P, hipMemcpyDeviceToHost, s));
       SA_HIP(hipStreamSynchronize(s));
       for (int p : rs)
@@ -631,6 +877,8 @@ int sa_batch_create(int P, const int32_t *n, const int32_t *rowptr, const int32_
   SA_CHECK(P > 0 && n && rowptr && col && val && out, "invalid batch arguments");
   auto *b = new sa_batch_s();
   b->P = P;
+  if (const char *e = std::getenv("SLEPC_AMD_BATCH_SWEEP"))
+    b->sweep_req = !std::strcmp(e, "persistent") ? 1 : (!std::strcmp(e, "wide") ? 2 : 0);
   b->n.assign(n, n + P);
   b->off.resize(P);
   int64_t rows = 0, nnz = 0, rp_off = 0;
@@ -687,7 +935,7 @@ int sa_batch_destroy(sa_batch b) {
 int sa_batch_solve(sa_batch b, int which, int nev, int ncv, double tol, int max_it, int mixed, double tol32,
                    const double *v0, uint64_t seed) {
   SA_TRY(ensure_init());
-  SA_CHECK(b && nev > 0 && ncv > nev && ncv < 64 && tol > 0, "invalid batch solve arguments (ncv < 64)");
+  SA_CHECK(b && nev > 0 && ncv > nev && ncv <= 64 && tol > 0, "invalid batch solve arguments (ncv <= 64)");
   const int P = b->P, m = ncv + 1;
   hipStream_t s = stream();
   if (max_it <= 0) max_it = 100000;
@@ -836,6 +1084,22 @@ int sa_batch_get_stats(sa_batch b, int64_t *counters6, double *seconds3) {
   if (seconds3) {
     seconds3[0] = b->seconds; seconds3[1] = b->seconds32; seconds3[2] = b->seconds_ds;
   }
+  return 0;
+}
+
+// Sweep kernel choice: 0 autotune (time both on the first restart round of
+// every solve phase, keep the faster), 1 one workgroup per problem, 2 lock-step.
+int sa_batch_set_sweep(sa_batch b, int mode) {
+  SA_CHECK(b && mode >= 0 && mode <= 2, "invalid sweep mode");
+  b->sweep_req = mode;
+  return 0;
+}
+
+int sa_batch_get_sweep(sa_batch b, int *mode, double *ms_persistent, double *ms_wide) {
+  SA_CHECK(b, "null batch");
+  if (mode) *mode = b->sweep_used;
+  if (ms_persistent) *ms_persistent = b->ms_persistent;
+  if (ms_wide) *ms_wide = b->ms_wide;
   return 0;
 }
 

@@ -40,6 +40,9 @@ int bv_alloc(BV &V, int64_t n, int m, int prec) {
   V.ws.maxG = (int)G;
   SA_HIP(hipMalloc(&V.ws.part, sizeof(double) * 2 * (size_t)G * (GS_MAXC + 1)));
   SA_HIP(hipMalloc(&V.ws.dots, sizeof(double) * 2 * (GS_MAXC + 1)));
+  // the reductions read every partial row even after a slot that produced none
+  SA_HIP(hipMemsetAsync(V.ws.part, 0, sizeof(double) * 2 * (size_t)G * (GS_MAXC + 1), stream()));
+  SA_HIP(hipMemsetAsync(V.ws.dots, 0, sizeof(double) * 2 * (GS_MAXC + 1), stream()));
   SA_HIP(hipMalloc(&V.dq, sizeof(double) * (size_t)m * m + 64 * sizeof(double)));
   SA_HIP(hipHostMalloc((void **)&V.hq, sizeof(double) * (size_t)m * m + 64 * sizeof(double)));
   SA_HIP(hipStreamSynchronize(stream()));
