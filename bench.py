@@ -251,15 +251,23 @@ def main():
         vs = sa.rand_uniform(0, Sh.n)
         s3 = {"workload": "27-pt 128^3 + diag U[0,1] seed 1 (n=2,097,152, nnz 55,742,968) HEP, nev 32 largest "
                           "magnitude, ncv 96, tol 1e-10"}
+        vals = {}
         for name, mixed in (("plain_fp64", False), ("mixed", True)):
             E, dt, good, w = solve_one(sa, S64, S32, "LM", True, 32, 96, 1e-10, vs, mixed)
             st = E.getStats()
             s3[name] = {"eigenpairs_per_s": round(good / dt, 3), "seconds": round(dt, 3), "good_pairs": good,
                         "worst_rel_residual": w, "matvecs": st["matvecs"], "matvecs32": st["matvecs32"],
                         "seconds32": round(st["seconds32"], 3)}
+            vals[name] = np.sort([E.getEigenpair(i, vectors=False)[0] for i in range(min(32, E.getConverged()))])
             del E
         s3["speedup_mixed_vs_plain"] = round(s3["mixed"]["eigenpairs_per_s"]
                                              / max(1e-12, s3["plain_fp64"]["eigenpairs_per_s"]), 3)
+        # error of each mixed-precision eigenvalue against the FP64 solve of the
+        # same matrix and start vector (the FP64 path matches the CPU oracle in
+        # the tests at smaller sizes)
+        if len(vals["mixed"]) == len(vals["plain_fp64"]) and len(vals["mixed"]):
+            rel = np.abs(vals["mixed"] - vals["plain_fp64"]) / np.abs(vals["plain_fp64"])
+            s3["mixed_vs_fp64_eigenvalue_rel_err"] = {"max": float(rel.max()), "per_pair": [float(x) for x in rel]}
         scale["stencil27_128"] = s3
         log("scale stencil27: %s" % json.dumps(s3))
         del S64, S32, Sh
