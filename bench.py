@@ -153,6 +153,84 @@ def run_family(sa, probs, v0, net, method, streams, warmup):
     return r
 
 
+def run_scale(sa):
+    """The 128^3 workloads: convection-diffusion (plain FP64) and the 27-point
+    stencil (mixed vs plain FP64)."""
+    scale = {}
+    Ch = sa.convdiff3d(128)
+    C64 = sa.Mat(Ch, sa.FP64)
+    A64_nnz[0] = Ch.nnz
+    gbs2, tb2, ms2, p2 = sweep_roofline(sa, C64, Ch.n, 64)
+    # non-normal operator: the true residual exceeds the Ritz estimate (the
+    # oracle shows the same), so the estimate tolerance is 1e-11 and pairs
+    # are counted at a true relative residual <= 1e-10
+    E, dt, good, w = solve_one(sa, C64, None, "LR", False, 20, 64, 1e-11, sa.rand_uniform(0, Ch.n), False,
+                               tol_req=1e-10)
+    st = E.getStats()
+    scale["convdiff_128"] = {
+        "workload": "convection-diffusion 7-pt 128^3 (n=2,097,152, nnz 14,581,760) NHEP, nev 20 largest real, "
+                    "ncv 64, estimate tol 1e-11, counted at true rel. residual 1e-10, plain FP64",
+        "eigenpairs_per_s": round(good / dt, 3), "seconds": round(dt, 3), "good_pairs": good,
+        "worst_rel_residual": w, "restarts": st["its"], "matvecs": st["matvecs"],
+        "gs_sweep_GBs": round(gbs2, 1), "gs_sweep_frac": round(gbs2 / HBM_PEAK_GBS, 4),
+        "gs_sweep_traffic_bytes": profiled_traffic("convdiff_128"),
+        "gs_sweep_ms": round(ms2, 3), "gs_sweep_passes_per_step": p2,
+        "solve_GBs_algorithmic": round((st["bytes_spmv"] + st["bytes_bv"] + st["bytes_restart"]) / dt / 1e9, 1)}
+    log("scale convdiff: %s" % json.dumps(scale["convdiff_128"]))
+    del E, C64, Ch
+    Sh = sa.stencil27(128, seed=1)
+    S64, S32 = sa.Mat(Sh, sa.FP64), sa.Mat(Sh, sa.FP32)
+    vs = sa.rand_uniform(0, Sh.n)
+    s3 = {"workload": "27-pt 128^3 + diag U[0,1] seed 1 (n=2,097,152, nnz 55,742,968) HEP, nev 32 largest "
+                      "magnitude, ncv 96, tol 1e-10"}
+    vals = {}
+    for name, mixed in (("plain_fp64", False), ("mixed", True)):
+        E, dt, good, w = solve_one(sa, S64, S32, "LM", True, 32, 96, 1e-10, vs, mixed)
+        st = E.getStats()
+        s3[name] = {"eigenpairs_per_s": round(good / dt, 3), "seconds": round(dt, 3), "good_pairs": good,
+                    "worst_rel_residual": w, "matvecs": st["matvecs"], "matvecs32": st["matvecs32"],
+                    "seconds32": round(st["seconds32"], 3)}
+        vals[name] = np.sort([E.getEigenpair(i, vectors=False)[0] for i in range(min(32, E.getConverged()))])
+        del E
+    s3["speedup_mixed_vs_plain"] = round(s3["mixed"]["eigenpairs_per_s"]
+                                         / max(1e-12, s3["plain_fp64"]["eigenpairs_per_s"]), 3)
+    # error of each mixed-precision eigenvalue against the FP64 solve of the
+    # same matrix and start vector (the FP64 path matches the CPU oracle in
+    # the tests at smaller sizes)
+    if len(vals["mixed"]) == len(vals["plain_fp64"]) and len(vals["mixed"]):
+        rel = np.abs(vals["mixed"] - vals["plain_fp64"]) / np.abs(vals["plain_fp64"])
+        s3["mixed_vs_fp64_eigenvalue_rel_err"] = {"max": float(rel.max()), "per_pair": [float(x) for x in rel]}
+    scale["stencil27_128"] = s3
+    log("scale stencil27: %s" % json.dumps(s3))
+    return scale
+
+
+def run_batched(sa):
+    """The 512-problem uneven family, plain FP64 and mixed (autotuned sweep)."""
+    from slepc_amd.batch import Batch, uneven_laplacians
+    csrs = uneven_laplacians(P=512, seed=2)
+    B = Batch(csrs)
+    out = {"workload": "512 Laplacians nx,ny in U{32..256} + diag 10 U[0,1) seed 2, nev 8 LM, ncv 24, "
+                       "tol 1e-10 (%d rows, %d nnz)" % (sum(A.n for A in csrs), sum(A.nnz for A in csrs))}
+    for name, mixed in (("plain_fp64", False), ("mixed", True)):
+        t0 = time.perf_counter()
+        B.solve(8, 24, 1e-10, "LM", mixed=mixed, seed=2)
+        err = B.errors()
+        good = 0
+        for p in range(len(csrs)):
+            nc = min(8, B.converged(p)[0])
+            good += int(np.sum(err[p, :nc] <= 1e-10))
+        dt = time.perf_counter() - t0
+        st = B.stats()
+        out[name] = {"eigenpairs_per_s": round(good / dt, 1), "seconds": round(dt, 3), "good_pairs": good,
+                     "worst_rel_residual": float(err.max()), "restart_rounds": st["rounds"],
+                     "restart_rounds32": st["rounds32"], "host_ds_s": round(st["seconds_ds"], 3),
+                     "sweep": B.sweep_info()}
+    log("batched: %s" % json.dumps(out))
+    B.destroy()
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -168,6 +246,8 @@ def main():
                     help="only the Arnoldi-sweep roofline measurements (for profiling runs)")
     ap.add_argument("--sweep-case", default="all", choices=["all", "fv2d_256", "convdiff_128"],
                     help="operator of --sweep-only")
+    ap.add_argument("--only", choices=["scale", "batched"],
+                    help="run one section alone and print its JSON (for profiling runs)")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py measures one GPU: this build has no multi-GPU path "
@@ -187,6 +267,9 @@ def main():
             out[name] = {"GBs": round(gbs, 1), "frac": round(gbs / HBM_PEAK_GBS, 4), "ms": round(ms, 3),
                          "bytes": tb, "passes_per_step": passes}
         print(json.dumps(out))
+        return
+    if args.only:
+        print(json.dumps(run_scale(sa) if args.only == "scale" else run_batched(sa)))
         return
 
     net = None
@@ -211,6 +294,27 @@ def main():
         res["learned"] = run_family(sa, probs, v0, net, "learned", args.streams, args.warmup)
     res["mixed"] = run_family(sa, probs, v0, net, "mixed", args.streams, args.warmup)
 
+    # ---------------- the reference's CPU-runnable case: 2-D Laplacian 100x100
+    c1 = None
+    if not args.no_cpu:
+        from oracle import oracle as orc
+        L1 = sa.lap2d(100, 100)
+        v1 = sa.rand_uniform(0, L1.n)
+        M1 = sa.Mat(L1)
+        solve_one(sa, M1, None, "LM", True, 10, 32, 1e-8, v1, False)           # warm-up
+        E1, dt1, good1, w1 = solve_one(sa, M1, None, "LM", True, 10, 32, 1e-8, v1, False)
+        A1 = orc.CSR(L1.n, L1.rowptr, L1.col, L1.val)
+        nth = min(host_cores(), 16)
+        r1 = orc.eps_solve(A1, 10, 32, 1e-8, "LM", True, v0=v1, nthreads=nth, nvec=0)
+        r1s = orc.eps_solve(A1, 10, 32, 1e-8, "LM", True, v0=v1, nthreads=1, nvec=0)
+        c1 = {"workload": "2-D 5-pt Laplacian 100x100 (n=10,000), HEP, nev 10 LM, ncv 32, tol 1e-8, start U[-1,1] "
+                          "seed 0",
+              "gpu_eigenpairs_per_s": round(good1 / dt1, 2), "gpu_seconds": round(dt1, 4), "gpu_pairs": good1,
+              "gpu_restarts": E1.getIterationNumber(),
+              "cpu_oracle_seconds": round(r1["seconds"], 4), "cpu_oracle_threads": nth,
+              "cpu_oracle_seconds_1_thread": round(r1s["seconds"], 4), "cpu_oracle_restarts": r1["its"]}
+        log("c1: %s" % json.dumps(c1))
+
     # ---------------- roofline of the fused Gram-Schmidt kernels
     A64_nnz[0] = probs[0][1].nnz
     gbs, tot_bytes, ms, passes = sweep_roofline(sa, probs[0][2], n, NCV)
@@ -222,80 +326,8 @@ def main():
                     "is a few microseconds, so latency bounds it; traffic = counter-measured HBM bytes per sweep "
                     "from profiles/gs_sweep_fv2d_256.txt" % (passes, tot_bytes / 1e6, ms)}
 
-    # ---------------- at scale
-    scale = {}
-    if not args.no_scale:
-        Ch = sa.convdiff3d(128)
-        C64 = sa.Mat(Ch, sa.FP64)
-        A64_nnz[0] = Ch.nnz
-        gbs2, tb2, ms2, p2 = sweep_roofline(sa, C64, Ch.n, 64)
-        # non-normal operator: the true residual exceeds the Ritz estimate (the
-        # oracle shows the same), so the estimate tolerance is 1e-11 and pairs
-        # are counted at a true relative residual <= 1e-10
-        E, dt, good, w = solve_one(sa, C64, None, "LR", False, 20, 64, 1e-11, sa.rand_uniform(0, Ch.n), False,
-                                   tol_req=1e-10)
-        st = E.getStats()
-        scale["convdiff_128"] = {
-            "workload": "convection-diffusion 7-pt 128^3 (n=2,097,152, nnz 14,581,760) NHEP, nev 20 largest real, "
-                        "ncv 64, estimate tol 1e-11, counted at true rel. residual 1e-10, plain FP64",
-            "eigenpairs_per_s": round(good / dt, 3), "seconds": round(dt, 3), "good_pairs": good,
-            "worst_rel_residual": w, "restarts": st["its"], "matvecs": st["matvecs"],
-            "gs_sweep_GBs": round(gbs2, 1), "gs_sweep_frac": round(gbs2 / HBM_PEAK_GBS, 4),
-            "gs_sweep_traffic_bytes": profiled_traffic("convdiff_128"),
-            "gs_sweep_ms": round(ms2, 3), "gs_sweep_passes_per_step": p2,
-            "solve_GBs_algorithmic": round((st["bytes_spmv"] + st["bytes_bv"] + st["bytes_restart"]) / dt / 1e9, 1)}
-        log("scale convdiff: %s" % json.dumps(scale["convdiff_128"]))
-        del E, C64, Ch
-        Sh = sa.stencil27(128, seed=1)
-        S64, S32 = sa.Mat(Sh, sa.FP64), sa.Mat(Sh, sa.FP32)
-        vs = sa.rand_uniform(0, Sh.n)
-        s3 = {"workload": "27-pt 128^3 + diag U[0,1] seed 1 (n=2,097,152, nnz 55,742,968) HEP, nev 32 largest "
-                          "magnitude, ncv 96, tol 1e-10"}
-        vals = {}
-        for name, mixed in (("plain_fp64", False), ("mixed", True)):
-            E, dt, good, w = solve_one(sa, S64, S32, "LM", True, 32, 96, 1e-10, vs, mixed)
-            st = E.getStats()
-            s3[name] = {"eigenpairs_per_s": round(good / dt, 3), "seconds": round(dt, 3), "good_pairs": good,
-                        "worst_rel_residual": w, "matvecs": st["matvecs"], "matvecs32": st["matvecs32"],
-                        "seconds32": round(st["seconds32"], 3)}
-            vals[name] = np.sort([E.getEigenpair(i, vectors=False)[0] for i in range(min(32, E.getConverged()))])
-            del E
-        s3["speedup_mixed_vs_plain"] = round(s3["mixed"]["eigenpairs_per_s"]
-                                             / max(1e-12, s3["plain_fp64"]["eigenpairs_per_s"]), 3)
-        # error of each mixed-precision eigenvalue against the FP64 solve of the
-        # same matrix and start vector (the FP64 path matches the CPU oracle in
-        # the tests at smaller sizes)
-        if len(vals["mixed"]) == len(vals["plain_fp64"]) and len(vals["mixed"]):
-            rel = np.abs(vals["mixed"] - vals["plain_fp64"]) / np.abs(vals["plain_fp64"])
-            s3["mixed_vs_fp64_eigenvalue_rel_err"] = {"max": float(rel.max()), "per_pair": [float(x) for x in rel]}
-        scale["stencil27_128"] = s3
-        log("scale stencil27: %s" % json.dumps(s3))
-        del S64, S32, Sh
-
-    # ---------------- batched uneven family
-    batched = None
-    if not args.no_batched:
-        from slepc_amd.batch import Batch, uneven_laplacians
-        csrs = uneven_laplacians(P=512, seed=2)
-        B = Batch(csrs)
-        out = {"workload": "512 Laplacians nx,ny in U{32..256} + diag 10 U[0,1) seed 2, nev 8 LM, ncv 24, "
-                           "tol 1e-10 (%d rows, %d nnz)" % (sum(A.n for A in csrs), sum(A.nnz for A in csrs))}
-        for name, mixed in (("plain_fp64", False), ("mixed", True)):
-            t0 = time.perf_counter()
-            B.solve(8, 24, 1e-10, "LM", mixed=mixed, seed=2)
-            err = B.errors()
-            good = 0
-            for p in range(len(csrs)):
-                nc = min(8, B.converged(p)[0])
-                good += int(np.sum(err[p, :nc] <= 1e-10))
-            dt = time.perf_counter() - t0
-            st = B.stats()
-            out[name] = {"eigenpairs_per_s": round(good / dt, 1), "seconds": round(dt, 3), "good_pairs": good,
-                         "worst_rel_residual": float(err.max()), "restart_rounds": st["rounds"],
-                         "restart_rounds32": st["rounds32"], "host_ds_s": round(st["seconds_ds"], 3)}
-        batched = out
-        log("batched: %s" % json.dumps(out))
-        B.destroy()
+    scale = run_scale(sa) if not args.no_scale else {}
+    batched = run_batched(sa) if not args.no_batched else None
 
     # ---------------- CPU baseline (the oracle) on the headline problem
     cpu = None
@@ -340,6 +372,7 @@ def main():
             "worst_rel_residual": max(v["worst"] for v in res.values())},
         "roofline": roof,
         "cpu_baseline": cpu,
+        "reference_cpu_case": c1,
         "at_scale": scale or None,
         "batched": batched,
     }

@@ -300,3 +300,34 @@ def test_mixed_precision_c1(sa, orc):
         v = E.getEigenpair(i, False)[0]
         assert np.min(np.abs(ev - v)) < 1e-10 * v
         assert E.computeError(i) <= 1e-10
+
+
+def test_concurrent_solves_match_sequential(sa):
+    """Solves running at once on several host threads (one HIP stream each,
+    sharing the operators) give the same eigenvalues and iteration counts as
+    the same solves one after the other."""
+    import concurrent.futures as cf
+    probs = [sa.Mat(sa.fv2d(48, 48, sa.fv_theta(5, s))) for s in range(6)]
+    n = 48 * 48
+    v0 = sa.rand_uniform(0, n)
+
+    def solve(i):
+        E = sa.EPS()
+        E.setOperators(probs[i])
+        E.setProblemType(True)
+        E.setWhichEigenpairs("LR")
+        E.setDimensions(8, 32)
+        E.setTolerances(1e-10)
+        E.setInitialSpace(v0)
+        E.solve()
+        vals = [E.getEigenpair(j, vectors=False)[0] for j in range(E.getConverged())]
+        errs = [E.computeError(j) for j in range(min(8, E.getConverged()))]
+        return vals, E.getIterationNumber(), errs
+
+    seq = [solve(i) for i in range(len(probs))]
+    with cf.ThreadPoolExecutor(max_workers=4) as ex:
+        conc = list(ex.map(solve, range(len(probs))))
+    for (v1, it1, e1), (v2, it2, e2) in zip(seq, conc):
+        assert it1 == it2
+        assert np.array_equal(v1, v2)          # same kernels, same order: identical bits
+        assert max(e2) <= 1e-10
