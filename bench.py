@@ -102,6 +102,36 @@ def sweep_roofline(sa, A64, n, m):
     return total / (ms.value * 1e-3) / 1e9, total, ms.value, passes
 
 
+def spmv_roofline(sa, Ah, reps=10):
+    """Standalone SpMV (sa_mat_mult, FP64) on operator Ah: the average of
+    `reps` calls, each timed alone with HIP events after copies of 268 MB
+    evict the operator from the 256 MB last-level cache (so it streams from
+    HBM, as inside a solve).  Algorithmic bytes: values and column indices
+    (12 B per nonzero), row pointers, x and y."""
+    L = sa.lib()
+    A = sa.Mat(Ah, sa.FP64)
+    n = Ah.n
+    x = sa.DeviceArray.from_host(sa.rand_uniform(3, n))
+    y = sa.DeviceArray(8 * n)
+    F = sa.BV(1 << 24, 4, sa.FP64)        # 4 columns of 134 MB for the evicting copies
+    A.mult(x, y)                          # warm-up
+    tot = 0.0
+    for _ in range(reps):
+        F.copy_column(0, 1)
+        F.copy_column(2, 3)
+        ms = C.c_double()
+        sa.check(L.sa_timer_start())
+        A.mult(x, y)
+        sa.check(L.sa_timer_stop(C.byref(ms)))
+        tot += ms.value
+    F.destroy()
+    A.destroy()
+    us = tot / reps * 1e3
+    nbytes = Ah.nnz * 12.0 + 4.0 * (n + 1) + 16.0 * n
+    gbs = nbytes / (us * 1e-6) / 1e9
+    return {"us": round(us, 2), "GBs": round(gbs, 1), "frac": round(gbs / HBM_PEAK_GBS, 4), "bytes": nbytes}
+
+
 def profiled_traffic(case):
     """HBM bytes per sweep measured with the FETCH_SIZE / WRITE_SIZE counters
     (calibrated against a device copy), as recorded in the committed profile
@@ -176,6 +206,7 @@ def run_scale(sa):
         "gs_sweep_traffic_bytes": profiled_traffic("convdiff_128"),
         "gs_sweep_ms": round(ms2, 3), "gs_sweep_passes_per_step": p2,
         "solve_GBs_algorithmic": round((st["bytes_spmv"] + st["bytes_bv"] + st["bytes_restart"]) / dt / 1e9, 1)}
+    scale["convdiff_128"]["spmv"] = spmv_roofline(sa, Ch)
     log("scale convdiff: %s" % json.dumps(scale["convdiff_128"]))
     del E, C64, Ch
     Sh = sa.stencil27(128, seed=1)
@@ -183,6 +214,10 @@ def run_scale(sa):
     vs = sa.rand_uniform(0, Sh.n)
     s3 = {"workload": "27-pt 128^3 + diag U[0,1] seed 1 (n=2,097,152, nnz 55,742,968) HEP, nev 32 largest "
                       "magnitude, ncv 96, tol 1e-10"}
+    A64_nnz[0] = Sh.nnz
+    gbs3, tb3, ms3, p3 = sweep_roofline(sa, S64, Sh.n, 96)
+    s3.update({"gs_sweep_GBs": round(gbs3, 1), "gs_sweep_frac": round(gbs3 / HBM_PEAK_GBS, 4),
+               "gs_sweep_ms": round(ms3, 3), "gs_sweep_passes_per_step": p3, "spmv": spmv_roofline(sa, Sh)})
     vals = {}
     for name, mixed in (("plain_fp64", False), ("mixed", True)):
         E, dt, good, w = solve_one(sa, S64, S32, "LM", True, 32, 96, 1e-10, vs, mixed)
@@ -244,8 +279,9 @@ def main():
     ap.add_argument("--no-learned", action="store_true", help="skip training the warm-start network")
     ap.add_argument("--sweep-only", action="store_true",
                     help="only the Arnoldi-sweep roofline measurements (for profiling runs)")
-    ap.add_argument("--sweep-case", default="all", choices=["all", "fv2d_256", "convdiff_128"],
+    ap.add_argument("--sweep-case", default="all", choices=["all", "fv2d_256", "convdiff_128", "stencil27_128"],
                     help="operator of --sweep-only")
+    ap.add_argument("--spmv", action="store_true", help="with --sweep-only: also time the standalone SpMV")
     ap.add_argument("--only", choices=["scale", "batched"],
                     help="run one section alone and print its JSON (for profiling runs)")
     args = ap.parse_args()
@@ -258,14 +294,17 @@ def main():
     sa.init(0)
     if args.sweep_only:
         out = {}
-        for name in ("fv2d_256", "convdiff_128"):
+        for name in ("fv2d_256", "convdiff_128", "stencil27_128"):
             if args.sweep_case not in ("all", name):
                 continue
-            Ah = sa.fv2d(GRID, GRID, sa.fv_theta(THETA_SEED_TEST, 0)) if name == "fv2d_256" else sa.convdiff3d(128)
+            Ah = (sa.fv2d(GRID, GRID, sa.fv_theta(THETA_SEED_TEST, 0)) if name == "fv2d_256" else
+                  sa.convdiff3d(128) if name == "convdiff_128" else sa.stencil27(128, seed=1))
             A64_nnz[0] = Ah.nnz
-            gbs, tb, ms, passes = sweep_roofline(sa, sa.Mat(Ah), Ah.n, NCV)
+            gbs, tb, ms, passes = sweep_roofline(sa, sa.Mat(Ah), Ah.n, 96 if name == "stencil27_128" else NCV)
             out[name] = {"GBs": round(gbs, 1), "frac": round(gbs / HBM_PEAK_GBS, 4), "ms": round(ms, 3),
                          "bytes": tb, "passes_per_step": passes}
+            if args.spmv:
+                out[name]["spmv"] = spmv_roofline(sa, Ah)
         print(json.dumps(out))
         return
     if args.only:
@@ -325,6 +364,8 @@ def main():
                     "algorithmic in %.3f ms (HIP events); the 34 MB basis stays cache-resident and each kernel "
                     "is a few microseconds, so latency bounds it; traffic = counter-measured HBM bytes per sweep "
                     "from profiles/gs_sweep_fv2d_256.txt" % (passes, tot_bytes / 1e6, ms)}
+
+    roof["spmv_fv2d_256"] = spmv_roofline(sa, probs[0][1])
 
     scale = run_scale(sa) if not args.no_scale else {}
     batched = run_batched(sa) if not args.no_batched else None

@@ -588,8 +588,6 @@ static int alloc_dev(sa_batch_s *b, BatchDev &B, int prec, int m) {
   B.lanes = avg <= 4.5 ? 4 : (avg <= 9 ? 8 : (avg <= 18 ? 16 : 32));
   SA_HIP(hipMalloc(&B.off, sizeof(int64_t) * P));
   SA_HIP(hipMalloc(&B.n, sizeof(int) * P));
-  int *ints[] = {nullptr};
-  (void)ints;
   for (int **q : {&B.order, &B.k0, &B.nv, &B.stop, &B.passes, &B.ml, &B.mk, &B.ms, &B.me, &B.src, &B.dst})
     SA_HIP(hipMalloc(q, sizeof(int) * P));
   SA_HIP(hipMalloc(&B.dbuf, sizeof(double) * (size_t)P * 128 * 128));

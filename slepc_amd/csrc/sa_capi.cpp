@@ -44,6 +44,21 @@ int ensure_init() {
   return sa_init(0);
 }
 
+int spmv_pick_rows(const int32_t *rp, int64_t n) {
+  int best = 0;
+  for (int rb : {256, 128, 64, 32}) {
+    bool fits = true;
+    for (int64_t r = 0; r < n && fits; r += rb) fits = rp[std::min<int64_t>(n, r + rb)] - rp[r] <= SPMV_NZB;
+    if (!fits) continue;
+    best = rb;
+    if ((n + rb - 1) / rb >= SPMV_MIN_BLOCKS) break;   // enough workgroups to fill the GPU
+  }
+  return best;
+}
+
+// Value/index bytes above which an operator uses the stream SpMV.
+constexpr int64_t SPMV_STREAM_BYTES = 64ll << 20;
+
 }  // namespace sa
 
 using namespace sa;
@@ -144,6 +159,10 @@ int sa_mat_create_csr(int n, int64_t nnz, const int32_t *rp, const int32_t *ci, 
   const double avg = n ? (double)nnz / n : 1.0;
   A.lanes = avg <= 4.5 ? 4 : (avg <= 9 ? 8 : (avg <= 18 ? 16 : 32));
   const size_t es = prec_size(prec);
+  // operators larger than the caches: the stream SpMV (small ones keep the
+  // vector-CSR kernel, whose many short workgroups win when the operator is
+  // cache-resident)
+  A.spmv_rb = nnz * (int64_t)(es + 4) > SPMV_STREAM_BYTES ? spmv_pick_rows(rp, n) : 0;
   hipStream_t s = stream();
   if (hipMalloc(&A.rowptr, sizeof(int32_t) * (n + 1)) != hipSuccess ||
       hipMalloc(&A.col, sizeof(int32_t) * std::max<int64_t>(1, nnz)) != hipSuccess ||

@@ -50,10 +50,16 @@ static int gs_launch(BV &V, const Mat *A, int c, double eta, hipStream_t s) {
   SA_CHECK(shm + 4096 + 8 * BLK <= 65536, "basis too wide for the LDS tile of the Gram-Schmidt kernel");
   double *pa = V.ws.part, *pb = V.ws.part + (size_t)V.G * (GS_MAXC + 1);
   double *da = V.ws.dots, *db = V.ws.dots + (GS_MAXC + 1);
-  const int lanes = A ? A->lanes : 8;
+  // Operators larger than the caches run the stream SpMV as a kernel of its
+  // own, V(:,c) = A V(:,c-1), and slot 1 then reads v like slots 2-3: the
+  // fused SpMV gathers row by row inside the slot and costs 2-3x the stream
+  // kernel there.  Small operators stay fused (one launch fewer per step).
+  const bool fused = A && A->spmv_rb == 0;
+  if (A && !fused) SA_TRY(launch_spmv(*A, a.V + (size_t)(c - 1) * a.ld, a.V + (size_t)c * a.ld, s));
+  const int lanes = fused ? A->lanes : 8;
   for (int slot = 1; slot <= 3; slot++) {
     a.slot = slot;
-    const bool spmv = slot == 1 && A;
+    const bool spmv = slot == 1 && fused;
     a.rp = spmv ? A->rowptr : nullptr;
     a.ci = spmv ? A->col : nullptr;
     a.val = spmv ? static_cast<const T *>(A->val) : nullptr;

@@ -36,6 +36,8 @@ int num_cus();
 // workgroup 0 writes st[s-1]; the state after a column is st[2].
 constexpr int GS_MAXC = 128;        // largest column index + 1 handled
 constexpr int GS_NSTATE = 3;
+constexpr int SPMV_PT = 8;          // nonzeros per thread of the stream SpMV
+constexpr int SPMV_NZB = 256 * SPMV_PT;   // nonzeros per block it stages in LDS (256-thread workgroups)
 
 struct GsState {
   int pass;         // Gram-Schmidt passes done for the current column
@@ -78,7 +80,16 @@ struct Mat {
   int32_t *rowptr = nullptr, *col = nullptr;
   void *val = nullptr;
   int lanes = 8;               // lanes per row of the vector-CSR kernels
+  int spmv_rb = 0;             // rows per workgroup of the stream SpMV (0: vector-CSR kernel);
+                               // set for operators larger than the caches
 };
+
+// Rows per workgroup of the stream SpMV for the rows [0, n) of rp (rp[0] need
+// not be 0): the largest RB of {256, 128, 64, 32} whose every RB-row block
+// holds at most SPMV_NZB nonzeros and which gives at least SPMV_MIN_BLOCKS
+// workgroups (else the smallest RB that fits); 0 when none fits.
+constexpr int64_t SPMV_MIN_BLOCKS = 1024;
+int spmv_pick_rows(const int32_t *rp, int64_t n);
 
 // ---- kernel launchers (sa_kernels.hip) -------------------------------------
 int launch_spmv(const Mat &A, const void *x, void *y, hipStream_t s);

@@ -22,7 +22,8 @@ __global__ __launch_bounds__(RBLK) void colsum_kernel(const double *part, int G,
 }
 
 // ---------------------------------------------------------------------------
-// Standalone vector-CSR SpMV: L lanes per row, shuffle reduction.
+// Vector-CSR SpMV: L lanes per row, shuffle reduction (matrices whose row
+// blocks are too dense for the stream kernel below).
 // ---------------------------------------------------------------------------
 template <typename T, int L>
 __global__ __launch_bounds__(BLK) void spmv_kernel(int n, const int32_t *__restrict__ rp,
@@ -41,8 +42,74 @@ __global__ __launch_bounds__(BLK) void spmv_kernel(int n, const int32_t *__restr
   if (lg == 0 && row < n) y[row] = s;
 }
 
+// Stream CSR SpMV for operators larger than the caches: RB rows per
+// workgroup.  The workgroup's nonzeros (a contiguous range of val/col) are
+// loaded with every lane active, each thread issuing its SPMV_PT value/index
+// loads before the dependent gathers of x; the products meet in LDS and
+// BLK/RB threads per row add them (a strided partial sum per thread, then a
+// shuffle tree).  Values and indices are read once, so they are loaded
+// non-temporally and do not evict x from the caches.  A block whose nonzeros
+// do not fit the LDS buffer is computed straight from memory instead, so the
+// kernel is correct for any CSR; the host picks RB so that every block fits.
+template <typename T, int RB>
+__global__ __launch_bounds__(BLK) void spmv_stream_kernel(int n, const int32_t *__restrict__ rp,
+                                                          const int32_t *__restrict__ ci,
+                                                          const T *__restrict__ val, const T *__restrict__ x,
+                                                          T *__restrict__ y) {
+  constexpr int TPR = BLK / RB;
+  constexpr int NZB = BLK * SPMV_PT;
+  static_assert(NZB == SPMV_NZB, "block capacity differs from the host's");
+  __shared__ T prod[NZB];
+  __shared__ int32_t srp[RB + 1];
+  const int t = threadIdx.x;
+  const int64_t r0 = (int64_t)blockIdx.x * RB;
+  const int rows = (int)min((int64_t)RB, (int64_t)n - r0);
+  for (int i = t; i <= rows; i += BLK) srp[i] = rp[r0 + i];
+  __syncthreads();
+  const int32_t p0 = srp[0], nz = srp[rows] - p0;
+  const int r = t / TPR, q = t % TPR;
+  T s = (T)0;
+  if (nz <= NZB) {                       // uniform across the workgroup
+    T v[SPMV_PT];
+    int32_t c[SPMV_PT];
+#pragma unroll
+    for (int k = 0; k < SPMV_PT; k++) {
+      const int i = k * BLK + t;
+      if (i < nz) {
+        v[k] = __builtin_nontemporal_load(val + p0 + i);
+        c[k] = __builtin_nontemporal_load(ci + p0 + i);
+      }
+    }
+#pragma unroll
+    for (int k = 0; k < SPMV_PT; k++) {
+      const int i = k * BLK + t;
+      if (i < nz) prod[i] = v[k] * x[c[k]];
+    }
+    __syncthreads();
+    if (r < rows)
+      for (int p = srp[r] - p0 + q; p < srp[r + 1] - p0; p += TPR) s += prod[p];
+  } else if (r < rows) {
+    for (int32_t p = srp[r] + q; p < srp[r + 1]; p += TPR) s += val[p] * x[ci[p]];
+  }
+#pragma unroll
+  for (int off = TPR / 2; off > 0; off >>= 1) s += __shfl_xor(s, off, TPR);
+  if (q == 0 && r < rows) y[r0 + r] = s;
+}
+
 template <typename T>
 static int spmv_t(const Mat &A, const T *x, T *y, hipStream_t s) {
+  if (A.spmv_rb > 0) {
+    const int grid = (int)(((int64_t)A.n + A.spmv_rb - 1) / A.spmv_rb);
+    const T *val = (const T *)A.val;
+    switch (A.spmv_rb) {
+      case 256: hipLaunchKernelGGL((spmv_stream_kernel<T, 256>), dim3(grid), dim3(BLK), 0, s, A.n, A.rowptr, A.col, val, x, y); break;
+      case 128: hipLaunchKernelGGL((spmv_stream_kernel<T, 128>), dim3(grid), dim3(BLK), 0, s, A.n, A.rowptr, A.col, val, x, y); break;
+      case 64: hipLaunchKernelGGL((spmv_stream_kernel<T, 64>), dim3(grid), dim3(BLK), 0, s, A.n, A.rowptr, A.col, val, x, y); break;
+      default: hipLaunchKernelGGL((spmv_stream_kernel<T, 32>), dim3(grid), dim3(BLK), 0, s, A.n, A.rowptr, A.col, val, x, y); break;
+    }
+    SA_HIP(hipGetLastError());
+    return 0;
+  }
   const int64_t threads = (int64_t)A.n * A.lanes;
   const int grid = (int)((threads + BLK - 1) / BLK);
   switch (A.lanes) {

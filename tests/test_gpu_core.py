@@ -38,6 +38,41 @@ def test_spmv_fp64_matches_oracle(sa, orc, gen):
     assert np.all(np.abs(y - yo) <= bound + 1e-300)
 
 
+def _ragged(sa, n, lens, seed):
+    """CSR with the given row lengths, random ascending columns and small
+    integer values (so that every product and partial sum is exact)."""
+    rng = np.random.default_rng(seed)
+    rp = np.zeros(n + 1, np.int32)
+    rp[1:] = np.cumsum(lens)
+    col = np.concatenate([np.sort(rng.choice(n, size=int(k), replace=False)) for k in lens]).astype(np.int32)
+    val = rng.integers(-3, 4, size=int(rp[-1])).astype(np.float64)
+    return sa.CSR(n, rp, col, val)
+
+
+# Row-length patterns that select each block size of the stream SpMV (at most
+# 2,048 nonzeros per block of 256/128/64/32 rows) and the vector-CSR kernel
+# (no block size fits), with empty rows and sizes not divisible by the block.
+@pytest.mark.parametrize("case", ["rows256", "rows128", "rows64", "rows32", "vector", "one_long_row"])
+@pytest.mark.parametrize("prec", ["fp64", "fp32"])
+def test_spmv_row_blocks_exact(sa, case, prec):
+    rng = np.random.default_rng(11)
+    n = {"rows256": 1001, "rows128": 777, "rows64": 555, "rows32": 333, "vector": 301, "one_long_row": 1500}[case]
+    lo, hi = {"rows256": (0, 8), "rows128": (9, 15), "rows64": (20, 30), "rows32": (40, 60), "vector": (100, 120),
+              "one_long_row": (0, 5)}[case]
+    lens = rng.integers(lo, hi + 1, size=n)
+    if case == "one_long_row":
+        lens[700] = 1400
+    Ah = _ragged(sa, n, lens, 5)
+    x = (np.arange(n) % 7 - 3).astype(np.float64)
+    dense = np.zeros((n, n))
+    for r in range(n):
+        dense[r, Ah.col[Ah.rowptr[r]:Ah.rowptr[r + 1]]] = Ah.val[Ah.rowptr[r]:Ah.rowptr[r + 1]]
+    ref = dense @ x
+    M = sa.Mat(Ah, sa.FP64 if prec == "fp64" else sa.FP32)
+    y = M.mult_host(x if prec == "fp64" else x.astype(np.float32)).astype(np.float64)
+    assert np.array_equal(y, ref)
+
+
 def test_spmv_fp32(sa, orc):
     Ah = sa.stencil27(16, 15, 14, seed=1)
     Ao = orc.stencil27(16, 15, 14, seed=1)
