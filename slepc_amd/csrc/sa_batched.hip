@@ -289,7 +289,7 @@ struct BwArgs {
 
 __device__ __forceinline__ int bw_col(const int *k0, int p, int spmv, int j) { return k0[p] + (spmv ? 1 + j : 0); }
 
-template <typename T, int L>
+template <typename T, int L, bool NT>
 __global__ __launch_bounds__(BLK) void bw_slot_kernel(BwArgs<T> b) {
   const int g = blockIdx.x, p = b.cprob[g];
   const int c = bw_col(b.k0, p, b.spmv, b.j);
@@ -324,7 +324,7 @@ __global__ __launch_bounds__(BLK) void bw_slot_kernel(BwArgs<T> b) {
   a.rpb = CH;
   a.eta = b.eta;
   const int64_t rb = b.crow[g];
-  gs_slot<T, L>(a, rb, min(a.n, rb + CH), bidx);
+  gs_slot<T, L, NT>(a, rb, min(a.n, rb + CH), bidx);
 }
 
 __global__ __launch_bounds__(BLK) void bw_sum_kernel(const int *list, const int *k0, const int *nv, int spmv, int j,
@@ -412,20 +412,27 @@ static int launch_sweep(const BatchDev &B, int nact, int spmv, double eta, hipSt
 // The one-workgroup-per-problem sweep keeps acc[c] in registers up to 64 columns.
 static bool persistent_fits(const BatchDev &B) { return B.m - 1 < 64; }
 
-template <typename T>
-static void launch_bw_slot(const BwArgs<T> &a, int lanes, int grid, size_t shm, hipStream_t s) {
+template <typename T, bool NT>
+static void launch_bw_slot_nt(const BwArgs<T> &a, int lanes, int grid, size_t shm, hipStream_t s) {
   switch (lanes) {
-    case 4: hipLaunchKernelGGL((bw_slot_kernel<T, 4>), dim3(grid), dim3(BLK), shm, s, a); break;
-    case 8: hipLaunchKernelGGL((bw_slot_kernel<T, 8>), dim3(grid), dim3(BLK), shm, s, a); break;
-    case 16: hipLaunchKernelGGL((bw_slot_kernel<T, 16>), dim3(grid), dim3(BLK), shm, s, a); break;
-    default: hipLaunchKernelGGL((bw_slot_kernel<T, 32>), dim3(grid), dim3(BLK), shm, s, a); break;
+    case 4: hipLaunchKernelGGL((bw_slot_kernel<T, 4, NT>), dim3(grid), dim3(BLK), shm, s, a); break;
+    case 8: hipLaunchKernelGGL((bw_slot_kernel<T, 8, NT>), dim3(grid), dim3(BLK), shm, s, a); break;
+    case 16: hipLaunchKernelGGL((bw_slot_kernel<T, 16, NT>), dim3(grid), dim3(BLK), shm, s, a); break;
+    default: hipLaunchKernelGGL((bw_slot_kernel<T, 32, NT>), dim3(grid), dim3(BLK), shm, s, a); break;
   }
+}
+
+template <typename T>
+static void launch_bw_slot(const BwArgs<T> &a, int lanes, bool nt, int grid, size_t shm, hipStream_t s) {
+  if (nt) launch_bw_slot_nt<T, true>(a, lanes, grid, shm, s);
+  else launch_bw_slot_nt<T, false>(a, lanes, grid, shm, s);
 }
 
 template <typename T>
 static int wide_sweep_t(BatchDev &B, const std::vector<int> &list, const std::vector<int> &k0,
                         const std::vector<int> &nv, int nch, int spmv, double eta, hipStream_t s) {
   const int P = B.P, m = B.m, nl = (int)list.size();
+  const bool nt = B.ldv * m * (int64_t)sizeof(T) > GS_NT_BYTES;
   int steps = 1;
   if (spmv) {
     steps = 0;
@@ -452,7 +459,7 @@ static int wide_sweep_t(BatchDev &B, const std::vector<int> &list, const std::ve
       a.slot = slot;
       a.part_out = slot == 1 ? pa : pb;
       a.dots_in = slot == 2 ? da : db;
-      launch_bw_slot<T>(a, B.lanes, nch, shm, s);
+      launch_bw_slot<T>(a, B.lanes, nt, nch, shm, s);
       if (slot < 3)
         hipLaunchKernelGGL(bw_sum_kernel, dim3(cmax + 1, nl), dim3(BLK), 0, s, (const int *)B.order,
                            (const int *)B.k0, (const int *)B.nv, spmv, j, slot, (const GsState *)B.wst,

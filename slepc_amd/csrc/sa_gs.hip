@@ -9,10 +9,20 @@
 
 namespace sa {
 
-template <typename T, int L>
+template <typename T, int L, bool NT>
 __global__ __launch_bounds__(BLK) void gs_kernel(GsArgs<T> a) {
   const int64_t rb = (int64_t)blockIdx.x * a.rpb;
-  gs_slot<T, L>(a, rb, min(a.n, rb + a.rpb), (int)blockIdx.x);
+  gs_slot<T, L, NT>(a, rb, min(a.n, rb + a.rpb), (int)blockIdx.x);
+}
+
+template <typename T, bool NT>
+static void launch_gs_slot(const GsArgs<T> &a, int lanes, int G, size_t shm, hipStream_t s) {
+  switch (lanes) {
+    case 4: hipLaunchKernelGGL((gs_kernel<T, 4, NT>), dim3(G), dim3(BLK), shm, s, a); break;
+    case 8: hipLaunchKernelGGL((gs_kernel<T, 8, NT>), dim3(G), dim3(BLK), shm, s, a); break;
+    case 16: hipLaunchKernelGGL((gs_kernel<T, 16, NT>), dim3(G), dim3(BLK), shm, s, a); break;
+    default: hipLaunchKernelGGL((gs_kernel<T, 32, NT>), dim3(G), dim3(BLK), shm, s, a); break;
+  }
 }
 
 // Reduction of a slot's partial sums before the next slot: one workgroup per
@@ -57,6 +67,7 @@ static int gs_launch(BV &V, const Mat *A, int c, double eta, hipStream_t s) {
   const bool fused = A && A->spmv_rb == 0;
   if (A && !fused) SA_TRY(launch_spmv(*A, a.V + (size_t)(c - 1) * a.ld, a.V + (size_t)c * a.ld, s));
   const int lanes = fused ? A->lanes : 8;
+  const bool nt = (int64_t)V.ld * V.m * (int64_t)sizeof(T) > GS_NT_BYTES;
   for (int slot = 1; slot <= 3; slot++) {
     a.slot = slot;
     const bool spmv = slot == 1 && fused;
@@ -66,12 +77,8 @@ static int gs_launch(BV &V, const Mat *A, int c, double eta, hipStream_t s) {
     a.xin = spmv ? a.V + (size_t)(c - 1) * a.ld : nullptr;
     a.part_out = slot == 1 ? pa : pb;
     a.dots_in = slot == 2 ? da : db;
-    switch (lanes) {
-      case 4: hipLaunchKernelGGL((gs_kernel<T, 4>), dim3(V.G), dim3(BLK), shm, s, a); break;
-      case 8: hipLaunchKernelGGL((gs_kernel<T, 8>), dim3(V.G), dim3(BLK), shm, s, a); break;
-      case 16: hipLaunchKernelGGL((gs_kernel<T, 16>), dim3(V.G), dim3(BLK), shm, s, a); break;
-      default: hipLaunchKernelGGL((gs_kernel<T, 32>), dim3(V.G), dim3(BLK), shm, s, a); break;
-    }
+    if (nt) launch_gs_slot<T, true>(a, lanes, V.G, shm, s);
+    else launch_gs_slot<T, false>(a, lanes, V.G, shm, s);
     if (slot < 3)
       hipLaunchKernelGGL(gs_sum_kernel, dim3(c + 1), dim3(BLK), 0, s, (const double *)a.part_out, V.G,
                          slot == 1 ? da : db);

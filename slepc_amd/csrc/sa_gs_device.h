@@ -130,7 +130,16 @@ __device__ void gs_decide_block(const GsArgs<T> &a, const double *s_dots, GsStat
 // uniform arguments.
 constexpr int GS_UT = 8;
 
-template <typename T, int L>
+// A basis tile load.  NT: non-temporal, for bases larger than the last-level
+// cache, whose columns are not re-read before they are evicted anyway (about
+// 3% faster at 2.1M rows x 65-97 columns; 4% slower on a cache-resident basis).
+template <bool NT, typename T>
+__device__ __forceinline__ T tile_load(const T *p) {
+  if constexpr (NT) return __builtin_nontemporal_load(p);
+  else return *p;
+}
+
+template <typename T, int L, bool NT>
 __device__ double gs_rows(const GsArgs<T> &a, int64_t rb, int64_t re, bool spmv, bool mult, bool upd, bool dots,
                           T scl, T *Vs, T *vs, const T *cf, T *rt) {
   const int t = threadIdx.x, c = a.c;
@@ -152,7 +161,7 @@ __device__ double gs_rows(const GsArgs<T> &a, int64_t rb, int64_t re, bool spmv,
 #pragma unroll
       for (int u = 0; u < GS_UT; u++) {
         const int i = pcg + 4 * u;
-        x[u] = (in && i < c) ? src[(size_t)i * a.ld] : (T)0;
+        x[u] = (in && i < c) ? tile_load<NT>(src + (size_t)i * a.ld) : (T)0;
       }
     }
     // 2. the vector chunk
@@ -188,7 +197,7 @@ __device__ double gs_rows(const GsArgs<T> &a, int64_t rb, int64_t re, bool spmv,
 #pragma unroll
         for (int u = 0; u < GS_UT; u++) {
           const int i = i0 + pcg + 4 * u;
-          x[u] = (in && i < c) ? src[(size_t)i * a.ld] : (T)0;
+          x[u] = (in && i < c) ? tile_load<NT>(src + (size_t)i * a.ld) : (T)0;
         }
 #pragma unroll
         for (int u = 0; u < GS_UT; u++) {
@@ -247,7 +256,7 @@ inline size_t gs_lds_bytes(int c, size_t es) { return ((size_t)c * LDP + CH + c 
 
 // One slot of column a.c for the rows [rb, re) of workgroup bidx of this
 // basis (bidx 0 is the lead).  Every workgroup of the basis calls it.
-template <typename T, int L>
+template <typename T, int L, bool NT>
 __device__ void gs_slot(const GsArgs<T> &a, int64_t rb, int64_t re, int bidx) {
   extern __shared__ __align__(16) unsigned char smem_raw[];
   const int c = a.c;
@@ -283,7 +292,7 @@ __device__ void gs_slot(const GsArgs<T> &a, int64_t rb, int64_t re, int bidx) {
     }
     if (stopped) return;
     const double acc =
-        gs_rows<T, L>(a, rb, re, a.rp != nullptr, false, false, true, (T)1, Vs, vs, cf, rt);
+        gs_rows<T, L, NT>(a, rb, re, a.rp != nullptr, false, false, true, (T)1, Vs, vs, cf, rt);
     gs_combine(acc, ncol, s_acc, nullptr, a.part_out, a.G, bidx);
     return;
   }
@@ -308,7 +317,7 @@ __device__ void gs_slot(const GsArgs<T> &a, int64_t rb, int64_t re, int bidx) {
     while (act == ACT_MULT_DOTS) {             // ends: S.pass < 3 bounds the passes
       for (int i = t; i < c; i += BLK) cf[i] = (T)s_dots[i];
       __syncthreads();
-      const double acc = gs_rows<T, L>(a, 0, a.n, false, true, true, true, (T)1, Vs, vs, cf, rt);
+      const double acc = gs_rows<T, L, NT>(a, 0, a.n, false, true, true, true, (T)1, Vs, vs, cf, rt);
       gs_combine(acc, ncol, s_acc, s_dots, nullptr, 0, 0);
       gs_decide_block(a, s_dots, &s_st, &s_act, &s_rec, &s_scale, true);
       act = s_act;
@@ -317,7 +326,7 @@ __device__ void gs_slot(const GsArgs<T> &a, int64_t rb, int64_t re, int bidx) {
     if (mult)
       for (int i = t; i < c; i += BLK) cf[i] = (T)s_dots[i];
     __syncthreads();
-    gs_rows<T, L>(a, 0, a.n, false, mult, true, false, (T)s_scale, Vs, vs, cf, rt);
+    gs_rows<T, L, NT>(a, 0, a.n, false, mult, true, false, (T)s_scale, Vs, vs, cf, rt);
     return;
   }
 
@@ -326,7 +335,7 @@ __device__ void gs_slot(const GsArgs<T> &a, int64_t rb, int64_t re, int bidx) {
   if (mult)
     for (int i = t; i < c; i += BLK) cf[i] = (T)s_dots[i];
   __syncthreads();
-  const double acc = gs_rows<T, L>(a, rb, re, false, mult, true, dots, (T)s_scale, Vs, vs, cf, rt);
+  const double acc = gs_rows<T, L, NT>(a, rb, re, false, mult, true, dots, (T)s_scale, Vs, vs, cf, rt);
   if (dots) gs_combine(acc, ncol, s_acc, nullptr, a.part_out, a.G, bidx);
 }
 

@@ -36,6 +36,7 @@ int num_cus();
 // workgroup 0 writes st[s-1]; the state after a column is st[2].
 constexpr int GS_MAXC = 128;        // largest column index + 1 handled
 constexpr int GS_NSTATE = 3;
+constexpr int64_t GS_NT_BYTES = 256ll << 20;   // bases above this size use non-temporal tile loads
 constexpr int SPMV_PT = 8;          // nonzeros per thread of the stream SpMV
 constexpr int SPMV_NZB = 256 * SPMV_PT;   // nonzeros per block it stages in LDS (256-thread workgroups)
 
