@@ -17,8 +17,10 @@ reported beside it.  A pair counts only if its FP64 relative residual
 
 Further sections: the 27-point stencil at 128^3 (mixed vs plain), the 7-point
 convection-diffusion operator at 128^3 (plain FP64), the 512-problem batched
-family, the roofline of the fused Gram-Schmidt kernels (HIP events over an
-Arnoldi sweep, algorithmic bytes / time) and the CPU oracle baseline.
+family, the roofline of the Arnoldi sweep kernels (HIP events over a sweep,
+algorithmic bytes / time) on each operator, the standalone SpMV on each
+operator (operator evicted from the caches before every call) and the CPU
+oracle baseline.
 """
 import argparse
 import concurrent.futures as cf
