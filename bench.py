@@ -134,6 +134,30 @@ def spmv_roofline(sa, Ah, reps=10):
     return {"us": round(us, 2), "GBs": round(gbs, 1), "frac": round(gbs / HBM_PEAK_GBS, 4), "bytes": nbytes}
 
 
+def restart_gemm_roofline(sa, n, K, N, reps=5):
+    """BVMultInPlace V(:,0:N) = V(:,0:K) Q(0:K,0:N) (the Krylov-Schur restart
+    shape) on an n-row FP64 basis: average of `reps` calls timed with HIP
+    events.  Algorithmic bytes: K columns read, N columns written."""
+    L = sa.lib()
+    V = sa.BV(n, K + 1, sa.FP64)
+    Q = np.random.default_rng(4).standard_normal((K + 1, K + 1))
+    V.set_active_columns(0, K)
+    V.multinplace(Q, 0, N)                # warm-up
+    tot = 0.0
+    for _ in range(reps):
+        ms = C.c_double()
+        sa.check(L.sa_timer_start())
+        V.multinplace(Q, 0, N)
+        sa.check(L.sa_timer_stop(C.byref(ms)))
+        tot += ms.value
+    V.destroy()
+    us = tot / reps * 1e3
+    nbytes = (K + N) * n * 8.0
+    gbs = nbytes / (us * 1e-6) / 1e9
+    return {"n": n, "K": K, "N": N, "us": round(us, 1), "GBs": round(gbs, 1), "frac": round(gbs / HBM_PEAK_GBS, 4),
+            "gflops": round(2.0 * n * K * N / (us * 1e-6) / 1e9, 1)}
+
+
 def profiled_traffic(case):
     """HBM bytes per sweep measured with the FETCH_SIZE / WRITE_SIZE counters
     (calibrated against a device copy), as recorded in the committed profile
@@ -237,6 +261,7 @@ def run_scale(sa):
     if len(vals["mixed"]) == len(vals["plain_fp64"]) and len(vals["mixed"]):
         rel = np.abs(vals["mixed"] - vals["plain_fp64"]) / np.abs(vals["plain_fp64"])
         s3["mixed_vs_fp64_eigenvalue_rel_err"] = {"max": float(rel.max()), "per_pair": [float(x) for x in rel]}
+    s3["restart_gemm"] = restart_gemm_roofline(sa, Sh.n, 96, 48)
     scale["stencil27_128"] = s3
     log("scale stencil27: %s" % json.dumps(s3))
     return scale
@@ -307,6 +332,8 @@ def main():
                          "bytes": tb, "passes_per_step": passes}
             if args.spmv:
                 out[name]["spmv"] = spmv_roofline(sa, Ah)
+                if name == "stencil27_128":
+                    out[name]["restart_gemm"] = restart_gemm_roofline(sa, Ah.n, 96, 48)
         print(json.dumps(out))
         return
     if args.only:

@@ -190,18 +190,23 @@ def test_arnoldi_relation(sa, orc):
     assert np.linalg.norm(Vm.T @ Vm - np.eye(m + 1)) < 1e-13
 
 
+# Shapes: ragged n/K/N in place (Q staged in LDS); more 64-row blocks than
+# workgroups (grid-stride loop); 64x64 (4 output tiles); a Q slice too large
+# for LDS (the tile-in-LDS kernel); K = 3, N = 1.
+@pytest.mark.parametrize("shape", [(5000, 41, 3, 37, 5, 29), (70001, 20, 0, 20, 0, 10), (1000, 70, 0, 64, 0, 64),
+                                   (300, 127, 2, 122, 10, 110), (777, 10, 0, 3, 0, 1)])
 @pytest.mark.parametrize("prec", [0, 1])
-def test_multinplace_matches_oracle(sa, orc, prec):
-    n, m = 5000, 41
+def test_multinplace_matches_oracle(sa, orc, prec, shape):
+    n, m, l, k, s0, e0 = shape
     rng = np.random.default_rng(2)
     X0 = rng.standard_normal((n, m))
     Q = rng.standard_normal((m, m))
     V = sa.BV(n, m, prec)
     V.set_matrix(X0)
-    V.set_active_columns(3, 37)
-    V.multinplace(Q, 5, 29)
+    V.set_active_columns(l, k)
+    V.multinplace(Q, s0, e0)
     out = V.get_matrix(m)
-    ref = orc.bv_multinplace(X0 if prec == 0 else X0.astype(np.float32).astype(np.float64), Q, 5, 29, l=3, k=37)
+    ref = orc.bv_multinplace(X0 if prec == 0 else X0.astype(np.float32).astype(np.float64), Q, s0, e0, l=l, k=k)
     tol = 1e-12 if prec == 0 else 2e-5
     assert np.max(np.abs(out - ref)) <= tol * np.abs(ref).max()
 
