@@ -29,9 +29,12 @@ int bv_alloc(BV &V, int64_t n, int m, int prec) {
   SA_HIP(hipHostMalloc((void **)&V.st_host, sizeof(GsState)));
   // Partition rows into workgroup ranges of whole 64-row chunks.
   const int64_t chunks = std::max<int64_t>(1, (n + 63) / 64);
-  // at most 1024 workgroups and at least gs_min_chunks() chunks each, so the
-  // reductions of the partial sums stay short for small problems
-  const int64_t maxG = 1024, minc = gs_min_chunks();
+  // at most gs_max_blocks() workgroups (1024, or 4096 for a basis larger than
+  // the last-level cache, whose slot kernels then fit more workgroups per CU)
+  // and at least gs_min_chunks() chunks each, so the reductions of the
+  // partial sums stay short
+  const bool big = V.ld * m * (int64_t)es > GS_NT_BYTES;   // the slot kernels' NT flag
+  const int64_t maxG = gs_max_blocks(big), minc = gs_min_chunks();
   int64_t G = std::min(maxG, std::max<int64_t>(1, (chunks + minc - 1) / minc));
   const int64_t cpb = (chunks + G - 1) / G;
   G = (chunks + cpb - 1) / cpb;

@@ -50,6 +50,15 @@ int gs_min_chunks() {
   return v;
 }
 
+int gs_max_blocks(bool big) {
+  static const int v = [] {
+    const char *e = std::getenv("SLEPC_AMD_GS_MAX_BLOCKS");
+    const int k = e ? std::atoi(e) : 0;
+    return k < 0 ? 0 : k;
+  }();
+  return v > 0 ? v : (big ? 4096 : 1024);
+}
+
 template <typename T>
 static int gs_launch(BV &V, const Mat *A, int c, double eta, hipStream_t s) {
   GsArgs<T> a{};

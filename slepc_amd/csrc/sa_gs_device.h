@@ -133,6 +133,7 @@ constexpr int GS_UT = 8;
 // A basis tile load.  NT: non-temporal, for bases larger than the last-level
 // cache, whose columns are not re-read before they are evicted anyway (about
 // 3% faster at 2.1M rows x 65-97 columns; 4% slower on a cache-resident basis).
+// The same flag selects wave-uniform column addressing in gs_rows.
 template <bool NT, typename T>
 __device__ __forceinline__ T tile_load(const T *p) {
   if constexpr (NT) return __builtin_nontemporal_load(p);
@@ -147,7 +148,12 @@ __device__ double gs_rows(const GsArgs<T> &a, int64_t rb, int64_t re, bool spmv,
   const int dcol = t % ncol, dsl = t / ncol;
   const bool dact = dots && dsl < nsl;
   const bool need_tile = (mult || dots) && c > 0;
-  const int prow = t & (CH - 1), pcg = t >> 6;
+  // NT (a basis larger than the last-level cache): the wave's column group is
+  // marked wave-uniform, so the column bases of the tile loads are scalar and
+  // a lane keeps only its row offset (61 instead of 111 VGPRs: room for more
+  // workgroups per CU); on a cache-resident basis the scalar address
+  // arithmetic measured slower, so it keeps the per-lane form there
+  const int prow = t & (CH - 1), pcg = NT ? __builtin_amdgcn_readfirstlane(t >> 6) : (t >> 6);
   T *vc = a.V + (size_t)c * a.ld;
   double acc = 0.0;
   for (int64_t r0 = rb; r0 < re; r0 += CH) {
