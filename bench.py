@@ -158,6 +158,49 @@ def restart_gemm_roofline(sa, n, K, N, reps=5):
             "gflops": round(2.0 * n * K * N / (us * 1e-6) / 1e9, 1)}
 
 
+def block_ops_roofline(sa, Ah, nc=32, reps=5):
+    """The block BV operations on an n-row FP64 basis: BVDot of two nc-column
+    blocks (algorithmic bytes 2 nc n 8), BVMatMult of 8 columns through the
+    operator (values and indices once, row pointers, 8 x and 8 y columns) and
+    BVOrthogonalize (chol) of nc columns; HIP events, averages of `reps`."""
+    L = sa.lib()
+    n = Ah.n
+
+    def timed(fn):
+        fn()                              # warm-up
+        tot = 0.0
+        for _ in range(reps):
+            ms = C.c_double()
+            sa.check(L.sa_timer_start())
+            fn()
+            sa.check(L.sa_timer_stop(C.byref(ms)))
+            tot += ms.value
+        return tot / reps * 1e3
+
+    X, Y = sa.BV(n, nc, sa.FP64), sa.BV(n, nc, sa.FP64)
+    for j in range(nc):
+        X.set_column(j, sa.rand_uniform(10 + j, n))
+        Y.set_column(j, sa.rand_uniform(100 + j, n))
+    us_dot = timed(lambda: X.dot(Y))
+    A = sa.Mat(Ah)
+    X.set_active_columns(0, 8)
+    Y.set_active_columns(0, 8)
+    us_mm = timed(lambda: X.matmult(A, Y))
+    X.set_active_columns(0, nc)
+    # after the first call the block is orthonormal; later calls do the same work
+    us_orth = timed(lambda: X.orthogonalize("chol", with_R=False))
+    X.destroy()
+    Y.destroy()
+    b_dot = 2.0 * nc * n * 8
+    b_mm = Ah.nnz * 12.0 + 4.0 * (n + 1) + 2 * 8 * n * 8.0
+    out = {"bv_dot": {"cols": nc, "us": round(us_dot, 1), "GBs": round(b_dot / us_dot / 1e3, 1),
+                      "frac": round(b_dot / us_dot / 1e3 / HBM_PEAK_GBS, 4)},
+           "bv_matmult": {"cols": 8, "us": round(us_mm, 1), "GBs": round(b_mm / us_mm / 1e3, 1),
+                          "frac": round(b_mm / us_mm / 1e3 / HBM_PEAK_GBS, 4)},
+           "bv_orthogonalize_chol": {"cols": nc, "us": round(us_orth, 1)}}
+    return out
+
+
 def profiled_traffic(case):
     """HBM bytes per sweep measured with the FETCH_SIZE / WRITE_SIZE counters
     (calibrated against a device copy), as recorded in the committed profile
@@ -233,6 +276,7 @@ def run_scale(sa):
         "gs_sweep_ms": round(ms2, 3), "gs_sweep_passes_per_step": p2,
         "solve_GBs_algorithmic": round((st["bytes_spmv"] + st["bytes_bv"] + st["bytes_restart"]) / dt / 1e9, 1)}
     scale["convdiff_128"]["spmv"] = spmv_roofline(sa, Ch)
+    scale["convdiff_128"]["block_ops"] = block_ops_roofline(sa, Ch)
     log("scale convdiff: %s" % json.dumps(scale["convdiff_128"]))
     del E, C64, Ch
     Sh = sa.stencil27(128, seed=1)
@@ -334,6 +378,8 @@ def main():
                 out[name]["spmv"] = spmv_roofline(sa, Ah)
                 if name == "stencil27_128":
                     out[name]["restart_gemm"] = restart_gemm_roofline(sa, Ah.n, 96, 48)
+                if name == "convdiff_128":
+                    out[name]["block_ops"] = block_ops_roofline(sa, Ah)
         print(json.dumps(out))
         return
     if args.only:

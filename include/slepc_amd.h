@@ -137,6 +137,26 @@ int sa_bv_get_coefficients(sa_bv V, int c0, int c1, double *buf_host);
    (BVMatArnoldi/BVMatLanczos bvkrylov.c:56-113,165-226): columns k+1..m are
    produced from V(:,k); *m is reduced on breakdown. */
 int sa_bv_mat_arnoldi(sa_bv V, sa_mat A, int k, int *m, double eta, double *beta, int *breakdown);
+/* Block operations (BVDot bvglobal.c:86-120, BVMatMult bvops.c:692-722,
+   BVOrthogonalize bvorthog.c:741-767).
+   sa_bv_dot: M(Y.l+i, X.l+j) = Y(:,Y.l+i)^T X(:,X.l+j) over the active columns,
+   M host column-major with ldm >= Y.k; other entries untouched.
+   sa_bv_matmult: Y(:, Y.l+j) = A X(:, X.l+j); X and Y need equally many active
+   columns.
+   sa_bv_orthogonalize: QR of the active columns V(:, l:k), the leading ones
+   assumed orthonormal: method SA_ORTHOG_BLOCK_GS (column by column, CGS+DGKS),
+   _CHOL (block GS against the leading columns, R = chol(V2^T V2), V2 = V2 R^-1)
+   or _SVQB (V2 = V2 D U Lambda^-1/2 from the eigendecomposition of the scaled
+   Gram matrix; R is then not triangular).  R (host, ldr >= k, or NULL)
+   receives columns l..k-1 of the factor: rows 0..l-1 the block-GS
+   coefficients, rows l..k-1 the factor of V2 (zeros below the diagonal for GS
+   and CHOL).  A dependent column stops GS with an error, as in the reference. */
+#define SA_ORTHOG_BLOCK_GS 0
+#define SA_ORTHOG_BLOCK_CHOL 1
+#define SA_ORTHOG_BLOCK_SVQB 2
+int sa_bv_dot(sa_bv X, sa_bv Y, double *M_host, int ldm);
+int sa_bv_matmult(sa_bv X, sa_mat A, sa_bv Y);
+int sa_bv_orthogonalize(sa_bv V, int method, double *R_host, int ldr);
 /* Gram-Schmidt passes (CGS1 + DGKS refinements) run on V so far. */
 int sa_bv_get_pass_count(sa_bv V, int64_t *passes);
 

@@ -14,6 +14,8 @@
  *   EPSConvergedRelative           src/eps/interface/epsdefault.c:224-232
  *   EPSStoppingBasic               src/eps/interface/epsdefault.c:290-302
  *   SlepcSortEigenvalues           src/sys/slepcsc.c:89-140
+ *   BVDot / BVMatMult              src/sys/classes/bv/interface/bvglobal.c:86-120, bvops.c:692-722
+ *   BVOrthogonalize (gs/chol/svqb) src/sys/classes/bv/interface/bvorthog.c:489-767, bvlapack.c:136-330
  * The dense projected problems call host LAPACK exactly where the reference does
  * (steqr, gehrd, orghr, hseqr, trexc, trevc, lartg, rot); the LAPACK library is
  * resolved at run time (scipy's bundled OpenBLAS or MKL).
@@ -258,10 +260,15 @@ typedef void (*f_hseqr)(const char *, const char *, const int *, const int *, co
 typedef void (*f_trexc)(const char *, const int *, double *, const int *, double *, const int *, int *, int *, double *, int *, size_t);
 typedef void (*f_trevc)(const char *, const char *, int *, const int *, const double *, const int *, double *, const int *, double *,
                         const int *, const int *, int *, double *, int *, size_t, size_t);
+typedef void (*f_potrf)(const char *, const int *, double *, const int *, int *, size_t);
+typedef void (*f_trtri)(const char *, const char *, const int *, double *, const int *, int *, size_t, size_t);
+typedef void (*f_syev)(const char *, const char *, const int *, double *, const int *, double *, double *, const int *, int *,
+                       size_t, size_t);
 static struct {
   int tried, ok;
   char path[512];
   f_lartg lartg; f_rot rot; f_steqr steqr; f_gehrd gehrd; f_orghr orghr; f_hseqr hseqr; f_trexc trexc; f_trevc trevc;
+  f_potrf potrf; f_trtri trtri; f_syev syev;   /* block orthogonalisation only */
 } LP;
 
 static void *lp_sym(void *h, const char *name)
@@ -280,6 +287,8 @@ static int lp_try(const char *path)
   LP.steqr = (f_steqr)lp_sym(h, "dsteqr"); LP.gehrd = (f_gehrd)lp_sym(h, "dgehrd");
   LP.orghr = (f_orghr)lp_sym(h, "dorghr"); LP.hseqr = (f_hseqr)lp_sym(h, "dhseqr");
   LP.trexc = (f_trexc)lp_sym(h, "dtrexc"); LP.trevc = (f_trevc)lp_sym(h, "dtrevc");
+  LP.potrf = (f_potrf)lp_sym(h, "dpotrf"); LP.trtri = (f_trtri)lp_sym(h, "dtrtri");
+  LP.syev = (f_syev)lp_sym(h, "dsyev");
   if (!(LP.lartg && LP.rot && LP.steqr && LP.gehrd && LP.orghr && LP.hseqr && LP.trexc && LP.trevc)) { dlclose(h); return 0; }
   void (*setlayer)(int) = (void (*)(int))dlsym(h, "MKL_Set_Threading_Layer");
   if (setlayer) setlayer(1); /* MKL_THREADING_SEQUENTIAL */
@@ -475,6 +484,117 @@ int orc_bv_orthonormalize_column(int64_t n, double *V, int64_t ld, int j, double
   if (norm) *norm = nrm;
   if (lindep) *lindep = ld_;
   return (int)passes;
+}
+
+/* ======================================================================== */
+/* Block BV operations                                                        */
+/* ======================================================================== */
+/* BVDot (bvglobal.c:86-120 -> BVDot_BLAS_Private bvblas.c:199-233, gemm "C","N"):
+   M(i, j) = Y(:,i) . X(:,j), i < my, j < mx, M leading dimension ldm. */
+void orc_bv_dot(int64_t n, const double *X, int64_t ldx, int mx, const double *Y, int64_t ldy, int my, double *M, int ldm)
+{
+  for (int j = 0; j < mx; j++)
+    for (int i = 0; i < my; i++) {
+      double s = 0.0;
+      for (int64_t r = 0; r < n; r++) s += Y[r + (int64_t)i * ldy] * X[r + (int64_t)j * ldx];
+      M[i + (int64_t)j * ldm] = s;
+    }
+}
+
+/* BVMatMult (bvops.c:692-722, column by column as BV_MATMULT_VECS):
+   Y(:, j) = A X(:, j), j < nc. */
+void orc_bv_matmult(int n, const int32_t *rp, const int32_t *ci, const double *val, const double *X, int64_t ldx,
+                    double *Y, int64_t ldy, int nc)
+{
+  for (int j = 0; j < nc; j++) orc_spmv(n, rp, ci, val, X + (int64_t)j * ldx, Y + (int64_t)j * ldy);
+}
+
+/* BVOrthogonalize (bvorthog.c:741-767) of columns l..k-1 of V (n x m, ld),
+   the leading ones orthonormal.  method 0: BVOrthogonalize_GS (:507-548),
+   column by column; 1: BVOrthogonalize_Chol (:588-603) with
+   BVMatCholInv_LAPACK_Private (bvlapack.c:136-203); 2: BVOrthogonalize_SVQB
+   (:671-686) with BVMatSVQB_LAPACK_Private (bvlapack.c:261-330).  Methods 1-2
+   first run BVOrthogonalize_BlockGS (:489-502) against the leading columns.
+   R (ldr >= k) receives columns l..k-1 of the factor (zeros below the
+   diagonal of the triangular ones).  Returns 0, or -1 on breakdown or a
+   LAPACK failure. */
+int orc_bv_orthogonalize(int64_t n, double *V, int64_t ld, int m, int l, int k, int method, double *R, int ldr)
+{
+  const int nk = k - l;
+  if (nk <= 0) return 0;
+  if (method == 0) {
+    double *buf = (double *)calloc((size_t)m * m, sizeof(double));
+    for (int j = l; j < k; j++) {
+      double nrm;
+      int lind;
+      orc_bv_orthonormalize_column(n, V, ld, j, buf, m, 0.7071, &nrm, &lind);
+      if (nrm == 0.0) { free(buf); return -1; }
+      if (R)
+        for (int i = 0; i < k; i++) R[i + (int64_t)j * ldr] = i < j ? buf[i + (int64_t)j * m] : (i == j ? nrm : 0.0);
+    }
+    free(buf);
+    return 0;
+  }
+  if (!lp_load() || !LP.potrf || !LP.trtri || !LP.syev) return -1;
+  double *R12 = (double *)calloc((size_t)(l > 0 ? l : 1) * nk, sizeof(double));
+  double *G = (double *)malloc(sizeof(double) * nk * nk), *Rf = (double *)calloc((size_t)nk * nk, sizeof(double));
+  double *S = (double *)calloc((size_t)nk * nk, sizeof(double)), *Q = (double *)calloc((size_t)k * k, sizeof(double));
+  int rc = 0, info = 0;
+  if (l) {                                /* V2 -= V1 (V1^T V2) */
+    orc_bv_dot(n, V + (int64_t)l * ld, ld, nk, V, ld, l, R12, l);
+    for (int j = 0; j < nk; j++) {
+      for (int i = 0; i < l; i++) Q[i + (int64_t)(l + j) * k] = -R12[i + (int64_t)j * l];
+      Q[(l + j) + (int64_t)(l + j) * k] = 1.0;
+    }
+    orc_bv_multinplace(n, V, ld, 0, k, l, k, Q, k);
+  }
+  orc_bv_dot(n, V + (int64_t)l * ld, ld, nk, V + (int64_t)l * ld, ld, nk, G, nk);
+  if (method == 1) {
+    memcpy(Rf, G, sizeof(double) * nk * nk);
+    LP.potrf("U", &nk, Rf, &nk, &info, 1);
+    if (info) {                           /* retry on the diagonally perturbed matrix */
+      memcpy(Rf, G, sizeof(double) * nk * nk);
+      for (int i = 0; i < nk; i++) Rf[i + (int64_t)i * nk] += 50.0 * 2.220446049250313e-16;
+      LP.potrf("U", &nk, Rf, &nk, &info, 1);
+    }
+    if (info) rc = -1;
+    for (int j = 0; j < nk; j++)
+      for (int i = j + 1; i < nk; i++) Rf[i + (int64_t)j * nk] = 0.0;
+    memcpy(S, Rf, sizeof(double) * nk * nk);
+    if (!rc) {
+      LP.trtri("U", "N", &nk, S, &nk, &info, 1, 1);
+      if (info) rc = -1;
+    }
+  } else {
+    double *D = (double *)malloc(sizeof(double) * nk), *w = (double *)malloc(sizeof(double) * nk);
+    double *U = (double *)malloc(sizeof(double) * nk * nk), q;
+    int lwork = -1;
+    for (int i = 0; i < nk; i++) D[i] = 1.0 / sqrt(G[i + (int64_t)i * nk]);
+    for (int j = 0; j < nk; j++)
+      for (int i = 0; i < nk; i++) U[i + (int64_t)j * nk] = G[i + (int64_t)j * nk] * D[i] * D[j];
+    LP.syev("V", "L", &nk, U, &nk, w, &q, &lwork, &info, 1, 1);
+    lwork = (int)q > 1 ? (int)q : 1;
+    double *work = (double *)malloc(sizeof(double) * lwork);
+    LP.syev("V", "L", &nk, U, &nk, w, work, &lwork, &info, 1, 1);
+    if (info) rc = -1;
+    for (int j = 0; j < nk; j++)
+      for (int i = 0; i < nk; i++) {
+        S[i + (int64_t)j * nk] = U[i + (int64_t)j * nk] * D[i] / sqrt(w[j]);
+        Rf[i + (int64_t)j * nk] = U[j + (int64_t)i * nk] * sqrt(w[i]) / D[j];
+      }
+    free(D); free(w); free(U); free(work);
+  }
+  if (!rc) {                              /* V2 = V2 S */
+    memset(Q, 0, sizeof(double) * k * k);
+    for (int j = 0; j < nk; j++)
+      for (int i = 0; i < nk; i++) Q[(l + i) + (int64_t)(l + j) * k] = S[i + (int64_t)j * nk];
+    orc_bv_multinplace(n, V, ld, l, k, l, k, Q, k);
+    if (R)
+      for (int j = 0; j < nk; j++)
+        for (int i = 0; i < k; i++) R[i + (int64_t)(l + j) * ldr] = i < l ? R12[i + (int64_t)j * l] : Rf[(i - l) + (int64_t)j * nk];
+  }
+  free(R12); free(G); free(Rf); free(S); free(Q);
+  return rc;
 }
 
 /* ======================================================================== */

@@ -51,6 +51,13 @@ double orc_bv_norm2(int64_t n, const double *x);
 int  orc_bv_orthonormalize_column(int64_t n, double *V, int64_t ld, int j, double *buffer,
                                   int m, double eta, double *norm, int *lindep);
 
+/* ---- block BV operations (BVDot, BVMatMult, BVOrthogonalize) ----------- */
+void orc_bv_dot(int64_t n, const double *X, int64_t ldx, int mx, const double *Y, int64_t ldy, int my,
+                double *M, int ldm);
+void orc_bv_matmult(int n, const int32_t *rp, const int32_t *ci, const double *val, const double *X,
+                    int64_t ldx, double *Y, int64_t ldy, int nc);
+int  orc_bv_orthogonalize(int64_t n, double *V, int64_t ld, int m, int l, int k, int method, double *R, int ldr);
+
 /* ---- dense projected problems (host LAPACK) ---------------------------- */
 int  orc_lapack_status(char *buf, int len);
 void orc_arrow_tridiag(int n, double *d, double *e, double *Q, int ld);

@@ -69,6 +69,10 @@ def lib():
                                                    C.c_int, C.c_double,
                                                    C.POINTER(C.c_double), C.POINTER(C.c_int)]
         L.orc_arrow_tridiag.argtypes = [C.c_int, f64p, f64p, f64p, C.c_int]
+        L.orc_bv_dot.argtypes = [C.c_int64, f64p, C.c_int64, C.c_int, f64p, C.c_int64, C.c_int, f64p, C.c_int]
+        L.orc_bv_matmult.argtypes = [C.c_int, i32p, i32p, f64p, f64p, C.c_int64, f64p, C.c_int64, C.c_int]
+        L.orc_bv_orthogonalize.argtypes = [C.c_int64, f64p, C.c_int64, C.c_int, C.c_int, C.c_int, C.c_int,
+                                           C.c_void_p, C.c_int]
         L.orc_eps_solve.argtypes = [C.c_int, i32p, i32p, f64p, C.POINTER(Opts),
                                     C.c_void_p, C.POINTER(Result)]
         L.orc_residual_rel.restype = C.c_double
@@ -210,6 +214,37 @@ def bv_orthonormalize_column(V, j, eta=0.7071):
     lind = C.c_int()
     lib().orc_bv_orthonormalize_column(n, flat, n, j, buf, m, eta, C.byref(nrm), C.byref(lind))
     return flat.reshape((n, m), order="F"), nrm.value, lind.value, buf[j * m:j * m + j + 1].copy()
+
+
+def bv_dot(X, Y):
+    """M = Y^T X (BVDot, all columns)."""
+    Xf, Yf = np.asfortranarray(X, np.float64), np.asfortranarray(Y, np.float64)
+    n, mx = Xf.shape
+    my = Yf.shape[1]
+    M = np.zeros((my, mx), order="F")
+    lib().orc_bv_dot(n, Xf.ravel(order="F"), n, mx, Yf.ravel(order="F"), n, my, M.ravel(order="K"), my)
+    return M
+
+
+def bv_matmult(A, X):
+    """Y = A X column by column (BVMatMult)."""
+    Xf = np.asfortranarray(X, np.float64)
+    n, nc = Xf.shape
+    Y = np.zeros((n, nc), order="F")
+    lib().orc_bv_matmult(A.n, A.rowptr, A.col, A.val, Xf.ravel(order="F"), n, Y.ravel(order="K"), n, nc)
+    return Y
+
+
+def bv_orthogonalize(V, l, k, method):
+    """BVOrthogonalize of columns l..k-1 (method 0 gs, 1 chol, 2 svqb).
+    Returns (V', R) with R k x k (columns l..k-1 filled), or raises on breakdown."""
+    n, m = V.shape
+    flat = np.array(V, order="F").ravel(order="F")
+    R = np.zeros((k, k), order="F")
+    rc = lib().orc_bv_orthogonalize(n, flat, n, m, l, k, method, R.ctypes.data, k)
+    if rc:
+        raise RuntimeError("BVOrthogonalize breakdown or LAPACK failure")
+    return flat.reshape((n, m), order="F"), R
 
 
 def arrow_tridiag(d, e):

@@ -221,6 +221,11 @@ class BV:
     def set_active_columns(self, l, k):
         check(lib().sa_bv_set_active_columns(self.h, l, k))
 
+    def get_active_columns(self):
+        l, k = C.c_int(), C.c_int()
+        check(lib().sa_bv_get_active_columns(self.h, C.byref(l), C.byref(k)))
+        return l.value, k.value
+
     def set_column(self, j, x):
         check(lib().sa_bv_set_column(self.h, j, np.ascontiguousarray(x, np.float64)))
 
@@ -257,6 +262,28 @@ class BV:
     def multinplace(self, Q, s, e):
         Qf = np.asfortranarray(Q, np.float64)
         check(lib().sa_bv_multinplace(self.h, Qf.ravel(order="F"), Qf.shape[0], s, e))
+
+    def dot(self, Y):
+        """BVDot(self, Y, M): M = Y^T self over the active columns, returned as
+        a Y.k x self.k array (rows below Y.l and columns below self.l zero)."""
+        l, k = self.get_active_columns()
+        ly, ky = Y.get_active_columns()
+        M = np.zeros((ky, k), order="F")
+        check(lib().sa_bv_dot(self.h, Y.h, M.ctypes.data, ky))
+        return M
+
+    def matmult(self, A, Y):
+        """BVMatMult(self, A, Y): Y(:, Y.l+j) = A self(:, self.l+j)."""
+        check(lib().sa_bv_matmult(self.h, A.h, Y.h))
+
+    def orthogonalize(self, method="chol", with_R=True):
+        """BVOrthogonalize of the active columns with block method gs, chol or
+        svqb; returns the k x k factor R (columns l..k-1 filled) or None."""
+        m = {"gs": 0, "chol": 1, "svqb": 2}[method]
+        l, k = self.get_active_columns()
+        R = np.zeros((k, k), order="F") if with_R else None
+        check(lib().sa_bv_orthogonalize(self.h, m, R.ctypes.data if with_R else None, k))
+        return R
 
     def scale_column(self, j, alpha):
         check(lib().sa_bv_scale_column(self.h, j, alpha))

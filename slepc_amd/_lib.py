@@ -86,6 +86,9 @@ SIGNATURES = {
     "sa_bv_orthonormalize_column": (C.c_int, [_VP, C.c_int, C.c_double, C.POINTER(C.c_double),
                                               C.POINTER(C.c_int)]),
     "sa_bv_get_coefficients": (C.c_int, [_VP, C.c_int, C.c_int, _F64P]),
+    "sa_bv_dot": (C.c_int, [_VP, _VP, _VP, C.c_int]),
+    "sa_bv_matmult": (C.c_int, [_VP, _VP, _VP]),
+    "sa_bv_orthogonalize": (C.c_int, [_VP, C.c_int, _VP, C.c_int]),
     "sa_bv_mat_arnoldi": (C.c_int, [_VP, _VP, C.c_int, C.POINTER(C.c_int), C.c_double,
                                     C.POINTER(C.c_double), C.POINTER(C.c_int)]),
     "sa_eps_create": (C.c_int, [C.POINTER(_VP)]),

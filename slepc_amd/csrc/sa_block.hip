@@ -1,0 +1,223 @@
+// Block BV kernels for gfx950 (MI355X): BVDot (M = Y^T X on MFMA, per-
+// workgroup partial tiles reduced in a fixed order) and BVMatMult (Y = A X,
+// a vector-CSR SpMM that reads each nonzero once for up to SPMM_NC columns).
+// Reference: bvglobal.c:86-120 (BVDot), bvops.c:692-722 (BVMatMult).
+#include <algorithm>
+
+#include "sa_gs_device.h"
+
+namespace sa {
+
+// ---------------------------------------------------------------------------
+// BVDot: M(i, j) = Y(:, i) . X(:, j), i < my <= 64, j < mx <= 64.  A
+// workgroup walks its rows in 32-row chunks, stages the chunk of both
+// blocks in LDS (stride 33: the 16 lanes of a tile row read 16 columns of
+// one row without bank conflicts) and accumulates 16x16 tiles of M with
+// MFMA 16x16x4 (operand A = Y^T, operand B = X); wave w owns tiles w, w+4,
+// w+8, w+12.  The tile values of every workgroup go to part[(i + j*my) * G
+// + b], summed afterwards in a fixed order (deterministic results).
+// ---------------------------------------------------------------------------
+constexpr int BD_ROWS = 32, BD_LDP = BD_ROWS + 1, BD_MAXC = 64, BD_MAXG = 1024;
+
+template <typename T, bool NT>
+__global__ __launch_bounds__(BLK) void bvdot_kernel(int64_t n, int64_t rpb, const T *__restrict__ X, int64_t ldx,
+                                                    int mx, const T *__restrict__ Y, int64_t ldy, int my,
+                                                    double *__restrict__ part, int G) {
+  typedef typename Vec4<T>::type V4;
+  __shared__ T Ys[BD_MAXC * BD_LDP];
+  __shared__ T Xs[BD_MAXC * BD_LDP];
+  const int t = threadIdx.x, lane = t & 63, w = t >> 6, g = lane >> 4, c16 = lane & 15;
+  const int nty = (my + 15) / 16, ntx = (mx + 15) / 16, ntile = nty * ntx;
+  V4 acc[4];
+#pragma unroll
+  for (int q = 0; q < 4; q++) acc[q] = (V4){0, 0, 0, 0};
+  const int64_t rb = (int64_t)blockIdx.x * rpb, re = min(n, rb + rpb);
+  const int row = t & (BD_ROWS - 1), cg = t / BD_ROWS;      // 8 column groups
+  for (int64_t r0 = rb; r0 < re; r0 += BD_ROWS) {
+    const int rows = (int)min((int64_t)BD_ROWS, re - r0);
+    const bool in = row < rows;
+    for (int i = cg; i < my; i += BLK / BD_ROWS)
+      Ys[i * BD_LDP + row] = in ? tile_load<NT>(Y + (size_t)i * ldy + r0 + row) : (T)0;
+    for (int i = cg; i < mx; i += BLK / BD_ROWS)
+      Xs[i * BD_LDP + row] = in ? tile_load<NT>(X + (size_t)i * ldx + r0 + row) : (T)0;
+    __syncthreads();
+#pragma unroll
+    for (int q = 0; q < 4; q++) {
+      const int tile = w + 4 * q;
+      if (tile < ntile) {
+        const int yi = (tile / ntx) * 16 + c16, xj = (tile % ntx) * 16 + c16;
+#pragma unroll
+        for (int st = 0; st < BD_ROWS / 4; st++) {
+          const int kk = 4 * st + g;
+          const T a = yi < my ? Ys[yi * BD_LDP + kk] : (T)0;
+          const T b = xj < mx ? Xs[xj * BD_LDP + kk] : (T)0;
+          acc[q] = mfma16x16x4(a, b, acc[q]);
+        }
+      }
+    }
+    __syncthreads();
+  }
+#pragma unroll
+  for (int q = 0; q < 4; q++) {
+    const int tile = w + 4 * q;
+    if (tile < ntile) {
+      const int j = (tile % ntx) * 16 + c16;
+#pragma unroll
+      for (int v = 0; v < 4; v++) {
+        const int i = (tile / ntx) * 16 + mfma_row<T>(g, v);
+        if (i < my && j < mx) part[((size_t)i + (size_t)j * my) * G + blockIdx.x] = (double)acc[q][v];
+      }
+    }
+  }
+}
+
+// out[col] = sum over the G partials of entry col, one workgroup per entry.
+__global__ __launch_bounds__(BLK) void bvdot_sum_kernel(const double *part, int G, double *out) {
+  gs_sum_column(part, G, (int)blockIdx.x, out);
+}
+
+template <typename T>
+static int bvdot_block(const BV &X, int lx, int mx, const BV &Y, int ly, int my, double *part, int G, int64_t rpb,
+                       double *out, hipStream_t s) {
+  const T *xp = static_cast<const T *>(X.data) + (size_t)lx * X.ld;
+  const T *yp = static_cast<const T *>(Y.data) + (size_t)ly * Y.ld;
+  // blocks read once from HBM: non-temporal loads (as the Gram-Schmidt tiles)
+  const bool nt = X.n * (int64_t)(mx + my) * (int64_t)sizeof(T) > GS_NT_BYTES;
+  if (nt)
+    hipLaunchKernelGGL((bvdot_kernel<T, true>), dim3(G), dim3(BLK), 0, s, X.n, rpb, xp, X.ld, mx, yp, Y.ld, my, part, G);
+  else
+    hipLaunchKernelGGL((bvdot_kernel<T, false>), dim3(G), dim3(BLK), 0, s, X.n, rpb, xp, X.ld, mx, yp, Y.ld, my, part, G);
+  hipLaunchKernelGGL(bvdot_sum_kernel, dim3(my * mx), dim3(BLK), 0, s, (const double *)part, G, out);
+  SA_HIP(hipGetLastError());
+  return 0;
+}
+
+// Partial-sum workspace of BVDot, one per host thread (each thread has its
+// own stream): kept between calls, grown when a call needs more, released
+// with the process (at most 32 MB; freeing it from a thread-exit destructor
+// could run after the HIP runtime has shut down).
+struct BdWork {
+  double *part = nullptr, *blk = nullptr;
+  int G = 0;
+};
+
+int launch_bvdot(const BV &X, int lx, int kx, const BV &Y, int ly, int ky, double *M_dev, hipStream_t s) {
+  const int mx = kx - lx, my = ky - ly;
+  if (mx <= 0 || my <= 0) return 0;
+  SA_CHECK(X.prec == Y.prec && X.n == Y.n, "BVDot: bases differ in length or precision");
+  const int64_t chunks = std::max<int64_t>(1, (X.n + BD_ROWS - 1) / BD_ROWS);
+  const int G = (int)std::min<int64_t>(BD_MAXG, chunks);
+  const int64_t rpb = (chunks + G - 1) / G * BD_ROWS;
+  thread_local BdWork W;
+  if (W.G < G) {
+    if (W.part) (void)hipFree(W.part);
+    W.part = nullptr;
+    W.G = 0;
+    SA_HIP(hipMalloc(&W.part, sizeof(double) * (size_t)BD_MAXC * BD_MAXC * G));
+    W.G = G;
+  }
+  if (!W.blk) SA_HIP(hipMalloc(&W.blk, sizeof(double) * BD_MAXC * BD_MAXC));
+  double *part = W.part, *blk = W.blk;
+  int rc = 0;
+  // blocks of at most 64 x 64 entries; block (bi, bj) lands in M(bi:, bj:)
+  for (int bj = 0; bj < mx && !rc; bj += BD_MAXC)
+    for (int bi = 0; bi < my && !rc; bi += BD_MAXC) {
+      const int nx = std::min(BD_MAXC, mx - bj), ny = std::min(BD_MAXC, my - bi);
+      rc = X.prec == SA_FP64 ? bvdot_block<double>(X, lx + bj, nx, Y, ly + bi, ny, part, G, rpb, blk, s)
+                             : bvdot_block<float>(X, lx + bj, nx, Y, ly + bi, ny, part, G, rpb, blk, s);
+      if (!rc && hipMemcpy2DAsync(M_dev + bi + (size_t)bj * my, sizeof(double) * my, blk, sizeof(double) * ny,
+                                  sizeof(double) * ny, nx, hipMemcpyDeviceToDevice, s) != hipSuccess)
+        rc = 1;
+    }
+  if (hipStreamSynchronize(s) != hipSuccess && !rc) rc = 1;
+  if (rc == 1) set_error("BVDot: HIP error");
+  return rc;
+}
+
+// ---------------------------------------------------------------------------
+// BVMatMult: Y(:, j) = A X(:, j) for j < nc <= SPMM_NC.  L lanes per row;
+// each nonzero's value and column index are loaded once and applied to the
+// nc columns, so the operator is read once per SPMM_NC columns.  The sum of
+// each column runs in the order of the vector-CSR SpMV.
+// ---------------------------------------------------------------------------
+constexpr int SPMM_NC = 8;
+
+template <typename T, int L, bool NT>
+__global__ __launch_bounds__(BLK) void spmm_kernel(int n, const int32_t *__restrict__ rp, const int32_t *__restrict__ ci,
+                                                   const T *__restrict__ val, const T *__restrict__ X, int64_t ldx,
+                                                   T *__restrict__ Y, int64_t ldy, int nc) {
+  const int lg = threadIdx.x % L;
+  const int64_t row = ((int64_t)blockIdx.x * BLK + threadIdx.x) / L;
+  T s[SPMM_NC];
+#pragma unroll
+  for (int j = 0; j < SPMM_NC; j++) s[j] = (T)0;
+  if (row < n) {
+    const int32_t pe = rp[row + 1];
+    for (int32_t p = rp[row] + lg; p < pe; p += L) {
+      const T v = NT ? __builtin_nontemporal_load(val + p) : val[p];
+      const int32_t c = NT ? __builtin_nontemporal_load(ci + p) : ci[p];
+#pragma unroll
+      for (int j = 0; j < SPMM_NC; j++)
+        if (j < nc) s[j] += v * X[(size_t)j * ldx + c];
+    }
+  }
+  // every lane of the group ends with each column's sum; lane q keeps the
+  // columns q, q+L, ... so the stores need SPMM_NC/L instructions, not SPMM_NC
+  constexpr int NS = (SPMM_NC + L - 1) / L;
+  T mine[NS];
+#pragma unroll
+  for (int u = 0; u < NS; u++) mine[u] = (T)0;
+#pragma unroll
+  for (int j = 0; j < SPMM_NC; j++) {
+    T sj = s[j];
+#pragma unroll
+    for (int off = L / 2; off > 0; off >>= 1) sj += __shfl_xor(sj, off, L);
+    if (j % L == lg) mine[j / L] = sj;
+  }
+  if (row < n) {
+#pragma unroll
+    for (int u = 0; u < NS; u++) {
+      const int j = u * L + lg;
+      if (j < nc) Y[(size_t)j * ldy + row] = mine[u];
+    }
+  }
+}
+
+template <typename T, bool NT>
+static void spmm_t(const Mat &A, const T *X, int64_t ldx, T *Y, int64_t ldy, int nc, hipStream_t s) {
+  const int grid = (int)(((int64_t)A.n * A.lanes + BLK - 1) / BLK);
+  const T *val = static_cast<const T *>(A.val);
+  switch (A.lanes) {
+    case 4: hipLaunchKernelGGL((spmm_kernel<T, 4, NT>), dim3(grid), dim3(BLK), 0, s, A.n, A.rowptr, A.col, val, X, ldx, Y, ldy, nc); break;
+    case 8: hipLaunchKernelGGL((spmm_kernel<T, 8, NT>), dim3(grid), dim3(BLK), 0, s, A.n, A.rowptr, A.col, val, X, ldx, Y, ldy, nc); break;
+    case 16: hipLaunchKernelGGL((spmm_kernel<T, 16, NT>), dim3(grid), dim3(BLK), 0, s, A.n, A.rowptr, A.col, val, X, ldx, Y, ldy, nc); break;
+    default: hipLaunchKernelGGL((spmm_kernel<T, 32, NT>), dim3(grid), dim3(BLK), 0, s, A.n, A.rowptr, A.col, val, X, ldx, Y, ldy, nc); break;
+  }
+}
+
+int launch_spmm(const Mat &A, const BV &X, int lx, const BV &Y, int ly, int ncols, hipStream_t s) {
+  SA_CHECK(A.prec == X.prec && X.prec == Y.prec && A.n == X.n && A.n == Y.n,
+           "BVMatMult: operator and bases differ in size or precision");
+  if (A.n == 0) return 0;
+  for (int j0 = 0; j0 < ncols; j0 += SPMM_NC) {
+    const int nc = std::min(SPMM_NC, ncols - j0);
+    // an operator larger than the caches (spmv_rb set) is read once per pass:
+    // non-temporal value/index loads keep the x columns cached
+    const bool nt = A.spmv_rb > 0;
+    if (X.prec == SA_FP64) {
+      const double *xp = static_cast<const double *>(X.data) + (size_t)(lx + j0) * X.ld;
+      double *yp = static_cast<double *>(Y.data) + (size_t)(ly + j0) * Y.ld;
+      if (nt) spmm_t<double, true>(A, xp, X.ld, yp, Y.ld, nc, s);
+      else spmm_t<double, false>(A, xp, X.ld, yp, Y.ld, nc, s);
+    } else {
+      const float *xp = static_cast<const float *>(X.data) + (size_t)(lx + j0) * X.ld;
+      float *yp = static_cast<float *>(Y.data) + (size_t)(ly + j0) * Y.ld;
+      if (nt) spmm_t<float, true>(A, xp, X.ld, yp, Y.ld, nc, s);
+      else spmm_t<float, false>(A, xp, X.ld, yp, Y.ld, nc, s);
+    }
+  }
+  SA_HIP(hipGetLastError());
+  return 0;
+}
+
+}  // namespace sa

@@ -1,9 +1,13 @@
 // BV (basis vectors) host side: allocation and the Arnoldi/Lanczos sweep that
 // chains the fused SpMV + Gram-Schmidt kernels without host round trips.
 #include <algorithm>
+#include <cfloat>
+#include <cmath>
 #include <cstdlib>
 #include <cstring>
+#include <vector>
 
+#include "sa_ds.h"
 #include "sa_internal.h"
 
 namespace sa {
@@ -101,6 +105,104 @@ int bv_orthonormalize(BV &V, int j, double eta, double *norm, int *lindep) {
   if (norm) *norm = V.st_host->nrm;
   if (lindep) *lindep = V.st_host->lindep;
   V.passes_done = V.st_host->passes_total;
+  return 0;
+}
+
+// ---- BVOrthogonalize (bvorthog.c:489-767, bvlapack.c:136-330) --------------
+namespace {
+
+// V(:, s:e) = V(:, lq:kq) Q(lq:kq, s:e) with Q in V.hq (leading dimension
+// ldq): the restart GEMM, in place.
+int apply_q(BV &V, int lq, int kq, int s_, int e, int ldq, hipStream_t s) {
+  SA_HIP(hipMemcpyAsync(V.dq, V.hq, sizeof(double) * (size_t)ldq * e, hipMemcpyHostToDevice, s));
+  SA_TRY(launch_multinplace(V, lq, kq, s_, e, V.dq, ldq, s));
+  SA_HIP(hipStreamSynchronize(s));
+  return 0;
+}
+
+// BVOrthogonalize_BlockGS (bvorthog.c:489-502): R12 = V(:,0:l)^T V(:,l:k),
+// V(:,l:k) -= V(:,0:l) R12, the update as the restart GEMM with Q = [-R12; I].
+int block_gs(BV &V, int l, int k, std::vector<double> &R12, double *M_dev, hipStream_t s) {
+  const int nk = k - l;
+  SA_TRY(launch_bvdot(V, l, k, V, 0, l, M_dev, s));
+  R12.assign((size_t)l * nk, 0.0);
+  SA_HIP(hipMemcpy(R12.data(), M_dev, sizeof(double) * (size_t)l * nk, hipMemcpyDeviceToHost));
+  std::fill(V.hq, V.hq + (size_t)k * k, 0.0);
+  for (int j = 0; j < nk; j++) {
+    for (int i = 0; i < l; i++) V.hq[i + (size_t)(l + j) * k] = -R12[i + (size_t)j * l];
+    V.hq[(l + j) + (size_t)(l + j) * k] = 1.0;
+  }
+  return apply_q(V, 0, k, l, k, k, s);
+}
+
+}  // namespace
+
+int bv_orthogonalize(BV &V, int method, double *R, int ldr) {
+  SA_CHECK(method >= 0 && method <= 2, "unknown block orthogonalization method (0 gs, 1 chol, 2 svqb)");
+  const int l = V.l, k = V.k, nk = k - l;
+  SA_CHECK(!R || ldr >= k, "R is smaller than the number of active columns");
+  if (nk <= 0) return 0;
+  hipStream_t s = stream();
+  if (method == 0) {   // BVOrthogonalize_GS (bvorthog.c:507-548): column by column
+    std::vector<double> h(V.m);
+    for (int j = l; j < k; j++) {
+      double nrm = 0.0;
+      int lind = 0;
+      SA_TRY(bv_orthonormalize(V, j, 0.7071, &nrm, &lind));
+      SA_CHECK(nrm != 0.0, "Breakdown in BVOrthogonalize due to a linearly dependent column");
+      if (R) {
+        SA_HIP(hipMemcpy(h.data(), V.dbuf + (size_t)j * V.m, sizeof(double) * V.m, hipMemcpyDeviceToHost));
+        for (int i = 0; i < k; i++) R[i + (size_t)j * ldr] = i < j ? h[i] : (i == j ? nrm : 0.0);
+      }
+    }
+    return 0;
+  }
+  SA_CHECK(lapack_ready(), "no LAPACK library found for the block orthogonalization");
+  double *M_dev = nullptr;
+  SA_HIP(hipMalloc(&M_dev, sizeof(double) * (size_t)k * k));
+  std::vector<double> R12, G((size_t)nk * nk);
+  int rc = l ? block_gs(V, l, k, R12, M_dev, s) : 0;
+  if (!rc) rc = launch_bvdot(V, l, k, V, l, k, M_dev, s);            // G = V2^T V2
+  if (!rc && hipMemcpy(G.data(), M_dev, sizeof(double) * G.size(), hipMemcpyDeviceToHost) != hipSuccess) rc = 1;
+  (void)hipFree(M_dev);
+  if (rc == 1) set_error("HIP error in the block orthogonalization");
+  if (rc) return rc;
+  std::vector<double> Rf(G.size(), 0.0), S(G.size(), 0.0);
+  if (method == 1) {   // BVOrthogonalize_Chol: R = chol(G), V2 = V2 inv(R) (bvlapack.c:136-203)
+    Rf = G;
+    int info = la_potrf_upper(nk, Rf.data(), nk);
+    if (info) {        // retry on the diagonally perturbed matrix
+      Rf = G;
+      for (int i = 0; i < nk; i++) Rf[i + (size_t)i * nk] += 50.0 * DBL_EPSILON;
+      info = la_potrf_upper(nk, Rf.data(), nk);
+    }
+    SA_CHECK(info == 0, "BVOrthogonalize: Cholesky factorization (potrf) failed");
+    for (int j = 0; j < nk; j++)
+      for (int i = j + 1; i < nk; i++) Rf[i + (size_t)j * nk] = 0.0;
+    S = Rf;
+    info = la_trtri_upper(nk, S.data(), nk);
+    SA_CHECK(info == 0, "BVOrthogonalize: triangular inverse (trtri) failed");
+  } else {             // BVOrthogonalize_SVQB (bvlapack.c:261-330)
+    std::vector<double> D(nk), w(nk), U(G.size());
+    for (int i = 0; i < nk; i++) D[i] = 1.0 / std::sqrt(G[i + (size_t)i * nk]);
+    for (int j = 0; j < nk; j++)
+      for (int i = 0; i < nk; i++) U[i + (size_t)j * nk] = G[i + (size_t)j * nk] * D[i] * D[j];
+    const int info = la_syev_lower(nk, U.data(), nk, w.data());
+    SA_CHECK(info == 0, "BVOrthogonalize: symmetric eigensolver (syev) failed");
+    for (int j = 0; j < nk; j++)
+      for (int i = 0; i < nk; i++) {
+        S[i + (size_t)j * nk] = U[i + (size_t)j * nk] * D[i] / std::sqrt(w[j]);      // D U Lambda^{-1/2}
+        Rf[i + (size_t)j * nk] = U[j + (size_t)i * nk] * std::sqrt(w[i]) / D[j];    // its inverse
+      }
+  }
+  std::fill(V.hq, V.hq + (size_t)k * k, 0.0);
+  for (int j = 0; j < nk; j++)
+    for (int i = 0; i < nk; i++) V.hq[(l + i) + (size_t)(l + j) * k] = S[i + (size_t)j * nk];
+  SA_TRY(apply_q(V, l, k, l, k, k, s));
+  if (R)
+    for (int j = 0; j < nk; j++)
+      for (int i = 0; i < k; i++)
+        R[i + (size_t)(l + j) * ldr] = i < l ? R12[i + (size_t)j * l] : Rf[(i - l) + (size_t)j * nk];
   return 0;
 }
 

@@ -395,6 +395,50 @@ int sa_bv_get_coefficients(sa_bv B, int c0, int c1, double *buf) {
   return 0;
 }
 
+int sa_bv_dot(sa_bv X, sa_bv Y, double *M, int ldm) {
+  API_INIT();
+  SA_CHECK(X && Y && M, "null argument");
+  const BV &x = X->V, &y = Y->V;
+  SA_CHECK(ldm >= y.k, "M has fewer rows than the active columns of Y");
+  SA_CHECK(x.n == y.n && x.prec == y.prec, "X and Y differ in length or precision");
+  const int mx = x.k - x.l, my = y.k - y.l;
+  if (mx <= 0 || my <= 0) return 0;
+  // the result goes through X's coefficient staging (m x m doubles >= mx x my
+  // when Y has no more columns than X; else a temporary)
+  const bool stage = (int64_t)mx * my <= (int64_t)x.m * x.m;
+  double *dev = stage ? x.dq : nullptr;
+  if (!stage) SA_HIP(hipMalloc(&dev, sizeof(double) * (size_t)mx * my));
+  std::vector<double> h((size_t)mx * my);
+  int rc = launch_bvdot(x, x.l, x.k, y, y.l, y.k, dev, stream());
+  if (!rc && hipMemcpy(h.data(), dev, sizeof(double) * h.size(), hipMemcpyDeviceToHost) != hipSuccess) {
+    set_error("HIP error in BVDot");
+    rc = 1;
+  }
+  if (!stage) (void)hipFree(dev);
+  if (rc) return rc;
+  for (int j = 0; j < mx; j++)
+    for (int i = 0; i < my; i++) M[(size_t)(y.l + i) + (size_t)(x.l + j) * ldm] = h[i + (size_t)j * my];
+  return 0;
+}
+
+int sa_bv_matmult(sa_bv X, sa_mat A, sa_bv Y) {
+  API_INIT();
+  SA_CHECK(X && A && Y, "null argument");
+  const BV &x = X->V, &y = Y->V;
+  SA_CHECK(x.k - x.l == y.k - y.l, "Y and X have different numbers of active columns");
+  SA_CHECK(A->A.n == x.n && A->A.n == y.n && A->A.prec == x.prec && x.prec == y.prec,
+           "operator and bases differ in size or precision");
+  SA_TRY(launch_spmm(A->A, x, x.l, y, y.l, x.k - x.l, stream()));
+  SA_HIP(hipStreamSynchronize(stream()));
+  return 0;
+}
+
+int sa_bv_orthogonalize(sa_bv V, int method, double *R, int ldr) {
+  API_INIT();
+  SA_CHECK(V, "null argument");
+  return bv_orthogonalize(V->V, method, R, ldr);
+}
+
 int sa_bv_mat_arnoldi(sa_bv B, sa_mat M, int k, int *m, double eta, double *beta, int *breakdown) {
   API_INIT();
   SA_CHECK(B && M && m && beta && breakdown, "null argument");

@@ -52,4 +52,26 @@ __device__ inline void sum_partials_block(const double *part, int G, int ncol, d
   }
 }
 
+// MFMA 16x16x4 in FP64 and FP32: C(16x16) += A(16x4) B(4x16).  Lane l
+// supplies A(l%16, l/16) and B(l/16, l%16); element v of its result is
+// C(g+4v, l%16) in FP64 and C(4g+v, l%16) in FP32, g = l/16.
+typedef double dbl4 __attribute__((ext_vector_type(4)));
+typedef float flt4 __attribute__((ext_vector_type(4)));
+
+__device__ __forceinline__ dbl4 mfma16x16x4(double a, double b, dbl4 c) {
+  return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
+}
+__device__ __forceinline__ flt4 mfma16x16x4(float a, float b, flt4 c) {
+  return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
+}
+template <typename T> struct Vec4;
+template <> struct Vec4<double> { typedef dbl4 type; };
+template <> struct Vec4<float> { typedef flt4 type; };
+
+// Row of the MFMA 16x16x4 result held in element v by lane group g.
+template <typename T>
+__device__ __forceinline__ int mfma_row(int g, int v) {
+  return sizeof(T) == 8 ? g + 4 * v : 4 * g + v;
+}
+
 }  // namespace sa

@@ -27,6 +27,10 @@ typedef void (*f_trexc)(const char *, const int *, double *, const int *, double
                         size_t);
 typedef void (*f_trevc)(const char *, const char *, int *, const int *, const double *, const int *, double *, const int *,
                         double *, const int *, const int *, int *, double *, int *, size_t, size_t);
+typedef void (*f_potrf)(const char *, const int *, double *, const int *, int *, size_t);
+typedef void (*f_trtri)(const char *, const char *, const int *, double *, const int *, int *, size_t, size_t);
+typedef void (*f_syev)(const char *, const char *, const int *, double *, const int *, double *, double *, const int *,
+                       int *, size_t, size_t);
 
 struct Lapack {
   bool ok = false;
@@ -39,6 +43,9 @@ struct Lapack {
   f_hseqr hseqr = nullptr;
   f_trexc trexc = nullptr;
   f_trevc trevc = nullptr;
+  f_potrf potrf = nullptr;   // block orthogonalisation (BVOrthogonalize)
+  f_trtri trtri = nullptr;
+  f_syev syev = nullptr;
 };
 Lapack LA;
 std::once_flag la_once;
@@ -62,7 +69,11 @@ bool try_lib(const char *path) {
   L.hseqr = (f_hseqr)sym(h, "dhseqr");
   L.trexc = (f_trexc)sym(h, "dtrexc");
   L.trevc = (f_trevc)sym(h, "dtrevc");
-  if (!(L.lartg && L.rot && L.steqr && L.gehrd && L.orghr && L.hseqr && L.trexc && L.trevc)) {
+  L.potrf = (f_potrf)sym(h, "dpotrf");
+  L.trtri = (f_trtri)sym(h, "dtrtri");
+  L.syev = (f_syev)sym(h, "dsyev");
+  if (!(L.lartg && L.rot && L.steqr && L.gehrd && L.orghr && L.hseqr && L.trexc && L.trevc && L.potrf && L.trtri &&
+        L.syev)) {
     dlclose(h);
     return false;
   }
@@ -101,6 +112,32 @@ bool lapack_ready(std::string *path) {
   std::call_once(la_once, load);
   if (path) *path = LA.ok ? LA.path : std::string("not found");
   return LA.ok;
+}
+
+int la_potrf_upper(int n, double *a, int lda) {
+  if (!lapack_ready()) return -1000;
+  int info = 0;
+  LA.potrf("U", &n, a, &lda, &info, 1);
+  return info;
+}
+
+int la_trtri_upper(int n, double *a, int lda) {
+  if (!lapack_ready()) return -1000;
+  int info = 0;
+  LA.trtri("U", "N", &n, a, &lda, &info, 1, 1);
+  return info;
+}
+
+int la_syev_lower(int n, double *a, int lda, double *w) {
+  if (!lapack_ready()) return -1000;
+  int info = 0, lwork = -1;
+  double q = 0.0;
+  LA.syev("V", "L", &n, a, &lda, w, &q, &lwork, &info, 1, 1);
+  if (info) return info;
+  lwork = std::max(1, (int)q);
+  std::vector<double> work(lwork);
+  LA.syev("V", "L", &n, a, &lda, w, work.data(), &lwork, &info, 1, 1);
+  return info;
 }
 
 int sc_compare(int which, double ar, double ai, double br, double bi) {

@@ -9,6 +9,12 @@ namespace sa {
 
 bool lapack_ready(std::string *path = nullptr);
 
+// LAPACK routines of the block orthogonalisation (bvlapack.c:136-330); each
+// returns LAPACK's info (-1000 when no LAPACK was found).
+int la_potrf_upper(int n, double *a, int lda);          // dpotrf("U")
+int la_trtri_upper(int n, double *a, int lda);          // dtrtri("U", "N")
+int la_syev_lower(int n, double *a, int lda, double *w);  // dsyev("V", "L"), ascending w
+
 enum DsState { DS_RAW = 0, DS_INTERMEDIATE = 1, DS_CONDENSED = 2, DS_TRUNCATED = 3 };
 
 // SlepcCompare* (src/sys/slepcsc.c:152-200): >0 when (ar,ai) must follow (br,bi)

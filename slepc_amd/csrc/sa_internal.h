@@ -111,6 +111,10 @@ int launch_convert(int src_prec, const void *src, int64_t src_ld, int dst_prec, 
 int launch_axpy_norm(int prec, const void *Ax, const void *x, double lambda, int64_t n,
                      double *part, double *out_dev, hipStream_t s);
 int launch_fill_random(int prec, void *x, int64_t n, uint64_t seed, hipStream_t s);
+// BVDot: M_dev(i + j*(ky-ly)) = Y(:,ly+i) . X(:,lx+j) (sa_block.hip); synchronises s
+int launch_bvdot(const BV &X, int lx, int kx, const BV &Y, int ly, int ky, double *M_dev, hipStream_t s);
+// BVMatMult: Y(:, ly+j) = A X(:, lx+j), j < ncols (sa_block.hip)
+int launch_spmm(const Mat &A, const BV &X, int lx, const BV &Y, int ly, int ncols, hipStream_t s);
 
 // ---- generators (sa_problems.cpp) ------------------------------------------
 uint64_t splitmix64(uint64_t z);
@@ -121,6 +125,9 @@ int bv_alloc(BV &V, int64_t n, int m, int prec);
 void bv_free(BV &V);
 int bv_sweep(BV &V, const Mat &A, int k, int *m, double eta, double *beta, int *breakdown);
 int bv_orthonormalize(BV &V, int j, double eta, double *norm, int *lindep);
+// BVOrthogonalize of the active columns (method 0 gs, 1 chol, 2 svqb); R: host
+// column-major, leading dimension ldr >= V.k, or null
+int bv_orthogonalize(BV &V, int method, double *R, int ldr);
 inline size_t prec_size(int prec) { return prec == SA_FP64 ? 8 : 4; }
 inline char *col_ptr(const BV &V, int j) {
   return static_cast<char *>(V.data) + (size_t)j * V.ld * prec_size(V.prec);

@@ -220,18 +220,6 @@ int launch_multvec(BV &V, int l, int k, double alpha, const double *q_dev, doubl
 // Q-in-LDS kernel below is the common path (2.6x faster at 2.1M rows).
 // ---------------------------------------------------------------------------
 constexpr int MIP_NT = 8;   // up to 8 output tiles of 16 columns (128 columns)
-typedef double dbl4 __attribute__((ext_vector_type(4)));
-typedef float flt4 __attribute__((ext_vector_type(4)));
-
-__device__ __forceinline__ dbl4 mfma16x16x4(double a, double b, dbl4 c) {
-  return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
-}
-__device__ __forceinline__ flt4 mfma16x16x4(float a, float b, flt4 c) {
-  return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
-}
-template <typename T> struct Vec4;
-template <> struct Vec4<double> { typedef dbl4 type; };
-template <> struct Vec4<float> { typedef flt4 type; };
 
 template <typename T>
 __global__ __launch_bounds__(BLK) void multinplace_kernel(int64_t n, int64_t ld, T *V, int l, int K,

@@ -263,3 +263,76 @@ def test_c_abi_exports_every_declared_symbol():
     L = sa.lib()
     missing = [nm for nm in names if not hasattr(L, nm)]
     assert not missing, missing
+
+
+# ------------------------------------------------------------------ block BV operations (oracle)
+def _test11_matrix(n=20, k=8):
+    """The input of the reference's bv/tests/test11.c (lines 76-89):
+    X(i+j, j) = (3i + j - 2) / (2(i + j + 1)) for i <= n/2, i + j < n."""
+    X = np.zeros((n, k))
+    for j in range(k):
+        for i in range(n // 2 + 1):
+            if i + j < n:
+                X[i + j, j] = (3.0 * i + j - 2) / (2 * (i + j + 1))
+    return X
+
+
+@pytest.mark.parametrize("method", [0, 1, 2])
+def test_oracle_bv_orthogonalize_test11(orc, method):
+    """bv/tests/test11.c (n=20, l=2, k=8; output test11_1.out / test11_6.out):
+    orthogonalizing the leading columns, then the active ones, leaves the level
+    of orthogonality of Q1, Q2 and Q and the residual ||X - Q R|| below
+    100 eps, for gs, chol and svqb.  For gs and chol R is also the textbook QR
+    factor (numpy's, up to the signs of its rows)."""
+    X = _test11_matrix()
+    l, k = 2, 8
+    V1, R1 = orc.bv_orthogonalize(X.copy(), 0, l, method)
+    V, R2 = orc.bv_orthogonalize(V1, l, k, method)
+    R = np.zeros((k, k))
+    R[:l, :l] = R1[:l, :l]
+    R[:, l:] = R2[:, l:]
+    eps = 2.220446049250313e-16
+    assert np.linalg.norm(V[:, :l].T @ V[:, :l] - np.eye(l)) < 100 * eps
+    assert np.linalg.norm(V[:, l:].T @ V[:, l:] - np.eye(k - l)) < 100 * eps
+    assert np.linalg.norm(V.T @ V - np.eye(k)) < 100 * eps
+    assert np.linalg.norm(X - V @ R) < 100 * eps
+    if method in (0, 1):
+        Rn = np.linalg.qr(X)[1]
+        assert np.allclose(np.abs(R), np.abs(Rn), atol=1e-13)
+        assert np.allclose(np.tril(R, -1), 0.0)
+
+
+def test_oracle_bv_orthogonalize_breakdown(orc):
+    """A zero column stops gs with an error (BVOrthogonalize_GS, bvorthog.c:537)."""
+    X = _test11_matrix()
+    X[:, 3] = 0.0
+    with pytest.raises(RuntimeError):
+        orc.bv_orthogonalize(X, 0, 8, 0)
+
+
+def test_oracle_bv_matmult_test7(orc):
+    """bv/tests/test7.c (m=n=10, k=5; output test7_1.out "Norm of error: 0."):
+    X(:, j) = ones in rows 0..j, B the 1-D Laplacian (2, -1): BVMatMult equals
+    the dense product exactly."""
+    n, k = 10, 5
+    X = np.zeros((n, k))
+    for j in range(k):
+        X[:j + 1, j] = 1.0
+    rp = [0]
+    col, val = [], []
+    for i in range(n):
+        for c, v in ((i - 1, -1.0), (i, 2.0), (i + 1, -1.0)):
+            if 0 <= c < n:
+                col.append(c)
+                val.append(v)
+        rp.append(len(col))
+    A = orc.CSR(n, np.array(rp, np.int32), np.array(col, np.int32), np.array(val))
+    Y = orc.bv_matmult(A, X)
+    assert np.array_equal(Y, A.to_dense() @ X)
+
+
+def test_oracle_bv_dot_matches_numpy(orc):
+    rng = np.random.default_rng(3)
+    X, Y = rng.standard_normal((300, 7)), rng.standard_normal((300, 5))
+    M = orc.bv_dot(X, Y)
+    assert np.max(np.abs(M - Y.T @ X)) < 1e-13 * np.abs(Y.T @ X).max()

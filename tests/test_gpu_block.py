@@ -1,0 +1,149 @@
+"""GPU parity tests of the block BV operations (BVDot, BVMatMult,
+BVOrthogonalize) against the oracle and the reference's bv tests 7 and 11.
+All calls go through the C ABI."""
+import numpy as np
+import pytest
+
+from test_cpu_oracle import _test11_matrix
+
+pytestmark = pytest.mark.gpu
+EPS = 2.220446049250313e-16
+
+
+def _align_signs(Q, R, Qref):
+    """SVQB takes its columns from eigenvectors (syev), whose signs follow the
+    rounding of the Gram matrix: flip columns of Q (and rows of R, so that
+    Q R is unchanged) to match Qref before comparing."""
+    sgn = np.where(np.sum(Q * Qref, axis=0) < 0, -1.0, 1.0)
+    return Q * sgn, R * sgn[:R.shape[0], None]
+
+
+def _bv(sa, X, prec=0):
+    V = sa.BV(X.shape[0], X.shape[1], prec)
+    V.set_matrix(X)
+    return V
+
+
+@pytest.mark.parametrize("method", ["gs", "chol", "svqb"])
+def test_orthogonalize_test11(sa, orc, method):
+    """bv/tests/test11.c (n=20, l=2, k=8): level of orthogonality of Q1, Q2
+    and Q and residual ||X - Q R|| below 100 eps; Q and R equal the oracle's."""
+    X = _test11_matrix()
+    l, k = 2, 8
+    V = _bv(sa, X)
+    V.set_active_columns(0, l)
+    R1 = V.orthogonalize(method)
+    V.set_active_columns(l, k)
+    R2 = V.orthogonalize(method)
+    Q = V.get_matrix(k)
+    R = np.zeros((k, k))
+    R[:l, :l] = R1[:l, :l]
+    R[:, l:] = R2[:, l:]
+    assert np.linalg.norm(Q[:, :l].T @ Q[:, :l] - np.eye(l)) < 100 * EPS
+    assert np.linalg.norm(Q[:, l:].T @ Q[:, l:] - np.eye(k - l)) < 100 * EPS
+    assert np.linalg.norm(Q.T @ Q - np.eye(k)) < 100 * EPS
+    assert np.linalg.norm(X - Q @ R) < 100 * EPS
+    m = {"gs": 0, "chol": 1, "svqb": 2}[method]
+    Vo, Ro1 = orc.bv_orthogonalize(X.copy(), 0, l, m)
+    Vo, Ro2 = orc.bv_orthogonalize(Vo, l, k, m)
+    Q, R2 = _align_signs(Q, R2, Vo) if method == "svqb" else (Q, R2)
+    assert np.max(np.abs(Q - Vo)) < 1e-13
+    assert np.max(np.abs(R2[:, l:] - Ro2[:, l:])) < 1e-13
+
+
+@pytest.mark.parametrize("method", ["gs", "chol", "svqb"])
+def test_orthogonalize_large_matches_oracle(sa, orc, method):
+    """100,001 rows, 40 columns, 8 leading ones already orthonormal (more
+    64-row blocks than workgroups, BVDot split over two 32-row chunks per
+    step): Q and R agree with the oracle, Q is orthonormal and X = Q R."""
+    n, k, l = 100001, 40, 8
+    rng = np.random.default_rng(9)
+    X = rng.standard_normal((n, k))
+    X[:, :l] = np.linalg.qr(X[:, :l])[0]
+    V = _bv(sa, X)
+    V.set_active_columns(l, k)
+    R = V.orthogonalize(method)
+    Q = V.get_matrix(k)
+    m = {"gs": 0, "chol": 1, "svqb": 2}[method]
+    Vo, Ro = orc.bv_orthogonalize(X.copy(), l, k, m)
+    Q, R = _align_signs(Q, R, Vo) if method == "svqb" else (Q, R)
+    assert np.linalg.norm(Q.T @ Q - np.eye(k)) < 1e-13
+    Rf = R.copy()
+    Rf[:l, :l] = np.eye(l)
+    assert np.linalg.norm(X - Q @ Rf) < 1e-12 * np.linalg.norm(X)
+    assert np.max(np.abs(Q - Vo)) < 1e-11
+    assert np.max(np.abs(R[:, l:] - Ro[:, l:])) < 1e-11 * np.abs(Ro).max()
+
+
+def test_orthogonalize_breakdown_raises(sa):
+    """A zero column stops gs with an error, as in bvorthog.c:537."""
+    X = _test11_matrix()
+    X[:, 3] = 0.0
+    V = _bv(sa, X)
+    with pytest.raises(RuntimeError):
+        V.orthogonalize("gs")
+
+
+@pytest.mark.parametrize("prec", [0, 1])
+def test_bv_dot_matches_oracle(sa, orc, prec):
+    """BVDot with active ranges (X columns 3..73, Y columns 2..52: blocks of
+    64 x 64 and remainders) on 70,001 rows: entries in rows Y.l.., columns
+    X.l.. equal Y^T X; the others are untouched."""
+    n = 70001
+    rng = np.random.default_rng(4)
+    Xh, Yh = rng.standard_normal((n, 75)), rng.standard_normal((n, 55))
+    if prec == 1:
+        Xh, Yh = Xh.astype(np.float32).astype(np.float64), Yh.astype(np.float32).astype(np.float64)
+    X, Y = _bv(sa, Xh, prec), _bv(sa, Yh, prec)
+    X.set_active_columns(3, 73)
+    Y.set_active_columns(2, 52)
+    M = X.dot(Y)
+    ref = orc.bv_dot(Xh[:, 3:73], Yh[:, 2:52])
+    tol = 1e-12 if prec == 0 else 2e-5
+    assert np.max(np.abs(M[2:52, 3:73] - ref)) <= tol * np.abs(ref).max()
+    assert not M[:2, :].any() and not M[:, :3].any()
+
+
+def test_bv_matmult_test7_exact(sa, orc):
+    """bv/tests/test7.c (m=n=10, k=5, Y with active columns 2..7): BVMatMult
+    equals column-by-column MatMult exactly (test7_1.out: "Norm of error: 0.")."""
+    n, k = 10, 5
+    Xh = np.zeros((n, k))
+    for j in range(k):
+        Xh[:j + 1, j] = 1.0
+    rp, col, val = [0], [], []
+    for i in range(n):
+        for c, v in ((i - 1, -1.0), (i, 2.0), (i + 1, -1.0)):
+            if 0 <= c < n:
+                col.append(c)
+                val.append(v)
+        rp.append(len(col))
+    Ah = sa.CSR(n, np.array(rp, np.int32), np.array(col, np.int32), np.array(val))
+    A = sa.Mat(Ah)
+    X = _bv(sa, Xh)
+    Y = sa.BV(n, k + 4, 0)
+    Y.set_active_columns(2, k + 2)
+    X.matmult(A, Y)
+    out = Y.get_matrix(k + 4)[:, 2:k + 2]
+    for j in range(k):
+        assert np.array_equal(out[:, j], A.mult_host(Xh[:, j]))
+
+
+@pytest.mark.parametrize("prec", [0, 1])
+def test_bv_matmult_stencil27(sa, orc, prec):
+    """BVMatMult of 11 columns (a pass of 8 and a pass of 3) through the
+    27-point operator with its diagonal rounded to integers and integer-valued
+    columns, so that every sum is exact in either precision: equal to the
+    oracle bit for bit."""
+    Ah = sa.stencil27(17, 16, 15, seed=1)
+    Ao = orc.stencil27(17, 16, 15, seed=1)
+    n = Ah.n
+    Xh = (np.arange(n * 11).reshape(n, 11, order="F") % 5 - 2).astype(np.float64)
+    Ah.val = np.round(Ah.val)
+    Ao.val = np.round(Ao.val)
+    A = sa.Mat(Ah, prec)
+    X = _bv(sa, Xh, prec)
+    Y = sa.BV(n, 11, prec)
+    X.matmult(A, Y)
+    ref = orc.bv_matmult(Ao, Xh)
+    assert np.array_equal(Y.get_matrix(11), ref)
