@@ -905,6 +905,13 @@ int orc_eps_solve(int n, const int32_t *rp, const int32_t *ci, const double *val
   int max_it = o->max_it > 0 ? o->max_it : (100 > 2 * n / ncv ? 100 : 2 * n / ncv);
   double tol = o->tol > 0 ? o->tol : 1e-8, keep = o->keep > 0 ? o->keep : 0.5;
   int herm = o->hermitian, which = o->which ? o->which : ORC_LM, lock = o->lock;
+  /* EPSARNOLDI (arnoldi.c:87-172): explicit restart, always DSNHEP (:188);
+     ncv <= nev + mpd (:181) */
+  const int arn = o->solver == 1;
+  if (arn) {
+    if (ncv > nev + mpd) return -11;
+    herm = 0;
+  }
   KSo ks = {0};
   ks.n = n; ks.rp = rp; ks.ci = ci; ks.val = val; ks.m = ncv + 1; ks.ld = n;
   ks.eta = o->eta > 0 ? o->eta : 0.7071;
@@ -943,7 +950,7 @@ int orc_eps_solve(int n, const int32_t *rp, const int32_t *ci, const double *val
   while (reason == 0) {
     its++;
     int nv = nconv + mpd < ncv ? nconv + mpd : ncv;
-    ds.n = nv; ds.t = nv; ds.l = nconv; ds.k = nconv + l;          /* DSSetDimensions */
+    ds.n = nv; ds.t = nv; ds.l = nconv; ds.k = arn ? 0 : nconv + l;   /* DSSetDimensions */
     double beta = 0.0; int breakdown = 0, k0 = nconv + l;
     ks_expand(&ks, k0, &nv, &beta, &breakdown);
     int bm = ks.m;
@@ -955,7 +962,7 @@ int orc_eps_solve(int n, const int32_t *rp, const int32_t *ci, const double *val
       memcpy(H + (int64_t)(nv - 1) * ld, ks.buf + (int64_t)nv * bm, sizeof(double) * nv);
       if (ld > nv) H[nv + (nv - 1) * ld] = ks.buf[nv + (int64_t)nv * bm];
     }
-    ds.n = nv; ds.t = nv; ds.l = nconv; ds.k = nconv + l;
+    ds.n = nv; ds.t = nv; ds.l = nconv; ds.k = arn ? 0 : nconv + l;
     ds.state = l ? ST_RAW : ST_INTERMEDIATE;
     int info = herm ? ds_solve_hep(&ds, eigr, eigi) : ds_solve_nhep(&ds, eigr, eigi);
     if (info) { rc = -20 - info; goto done; }
@@ -979,6 +986,21 @@ int orc_eps_solve(int n, const int32_t *rp, const int32_t *ci, const double *val
       if (marker != -1) break;
     }
     if (marker != -1) k = marker;
+    if (arn) {
+      /* BVMultInPlace(V,Q,nconv,min(k+1,nv)) (arnoldi.c:150-153); the
+         (k+1)-th Schur vector is the next start vector */
+      int e = k + 1 < nv ? k + 1 : nv;
+      orc_bv_multinplace(n, ks.V, ks.ld, nconv, nv, nconv, e, ds.Q, ld);
+      if (k >= nev) reason = 1;
+      else if (its >= max_it) reason = -1;
+      if (reason == 0 && breakdown) {                          /* arnoldi.c:156-162 */
+        int bd = 0;
+        ks_start_vector(&ks, k, nini, &bd);
+        if (bd) reason = -2;
+      }
+      nconv = k;
+      continue;
+    }
     /* EPSStoppingBasic (epsdefault.c:290-302) */
     if (k >= nev) reason = 1;
     else if (its >= max_it) reason = -1;

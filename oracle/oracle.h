@@ -64,7 +64,7 @@ void orc_arrow_tridiag(int n, double *d, double *e, double *Q, int ld);
 int  orc_ds_hep_compact(int n, int l, int k, const double *d, const double *e, int which,
                         double *wr, double *Q);
 
-/* ---- Krylov-Schur solve ------------------------------------------------ */
+/* ---- Krylov-Schur and Arnoldi solves ----------------------------------- */
 enum { ORC_LM = 1, ORC_SM = 2, ORC_LR = 3, ORC_SR = 4 };
 typedef struct {
   int     hermitian;   /* 1: Lanczos + DSHEP, 0: Arnoldi + DSNHEP */
@@ -75,6 +75,7 @@ typedef struct {
   double  eta;         /* DGKS threshold, default 0.7071 */
   int     lock;        /* locking variant (default 1) */
   int     nthreads;    /* OpenMP threads, 0 = default */
+  int     solver;      /* 0 Krylov-Schur, 1 Arnoldi with explicit restart (arnoldi.c) */
 } orc_opts;
 
 typedef struct {

@@ -21,7 +21,7 @@ class Opts(C.Structure):
     _fields_ = [("hermitian", C.c_int), ("which", C.c_int), ("nev", C.c_int),
                 ("ncv", C.c_int), ("mpd", C.c_int), ("max_it", C.c_int),
                 ("tol", C.c_double), ("keep", C.c_double), ("eta", C.c_double),
-                ("lock", C.c_int), ("nthreads", C.c_int)]
+                ("lock", C.c_int), ("nthreads", C.c_int), ("solver", C.c_int)]
 
 
 class Result(C.Structure):
@@ -258,9 +258,10 @@ def arrow_tridiag(d, e):
 
 # ---------------------------------------------------------------- solve
 def eps_solve(A, nev, ncv=0, tol=1e-8, which="LM", hermitian=True, v0=None, max_it=0,
-              mpd=0, nvec=None, nthreads=0, keep=0.5, lock=1):
+              mpd=0, nvec=None, nthreads=0, keep=0.5, lock=1, solver="krylovschur"):
     o = Opts(hermitian=1 if hermitian else 0, which=WHICH[which], nev=nev, ncv=ncv, mpd=mpd,
-             max_it=max_it, tol=tol, keep=keep, eta=0.7071, lock=lock, nthreads=nthreads)
+             max_it=max_it, tol=tol, keep=keep, eta=0.7071, lock=lock, nthreads=nthreads,
+             solver={"krylovschur": 0, "arnoldi": 1}[solver])
     ncv_eff = ncv if ncv else min(A.n, max(2 * nev, nev + 15))
     eigr = np.zeros(ncv_eff + 1)
     eigi = np.zeros(ncv_eff + 1)

@@ -349,6 +349,10 @@ class EPS:
     def setProblemType(self, hermitian):
         check(lib().sa_eps_set_problem_type(self.h, 1 if hermitian else 0))
 
+    def setType(self, eps_type):
+        """EPSSetType: "krylovschur" (default) or "arnoldi"."""
+        check(lib().sa_eps_set_type(self.h, {"krylovschur": 0, "arnoldi": 1}[eps_type]))
+
     def setWhichEigenpairs(self, which):
         check(lib().sa_eps_set_which(self.h, WHICH[which] if isinstance(which, str) else which))
 

@@ -96,6 +96,7 @@ SIGNATURES = {
     "sa_eps_set_operators": (C.c_int, [_VP, _VP]),
     "sa_eps_set_operators_fp32": (C.c_int, [_VP, _VP]),
     "sa_eps_set_problem_type": (C.c_int, [_VP, C.c_int]),
+    "sa_eps_set_type": (C.c_int, [_VP, C.c_int]),
     "sa_eps_set_which": (C.c_int, [_VP, C.c_int]),
     "sa_eps_set_dimensions": (C.c_int, [_VP, C.c_int, C.c_int, C.c_int]),
     "sa_eps_set_tolerances": (C.c_int, [_VP, C.c_double, C.c_int]),

@@ -480,6 +480,12 @@ int sa_eps_set_problem_type(sa_eps E, int hermitian) {
   return 0;
 }
 
+int sa_eps_set_type(sa_eps E, int type) {
+  SA_CHECK(E && (type == SA_EPS_KRYLOVSCHUR || type == SA_EPS_ARNOLDI), "invalid EPS type");
+  E->p.solver = type;
+  return 0;
+}
+
 int sa_eps_set_which(sa_eps E, int which) {
   SA_CHECK(E && which >= 1 && which <= 4, "invalid which");
   E->p.which = which;

@@ -73,9 +73,14 @@ int ks_solve(const KsParams &p, const Mat &A, BV &V, bool have_initial, KsResult
   const int n = A.n, ncv = p.ncv, nev = p.nev, mpd = p.mpd, ld = ncv + 1;
   SA_CHECK(V.m == ncv + 1 && V.n == n && V.prec == A.prec, "basis does not match the operator/ncv");
   SA_CHECK(ncv + 1 < GS_MAXC, "ncv too large for the fused Gram-Schmidt kernels");
+  // EPSARNOLDI (arnoldi.c:87-172, setup :176-194): explicit restart and
+  // always a DSNHEP projected problem; ncv <= nev + mpd
+  const bool arn = p.solver == 1;
+  const bool dsherm = p.hermitian && !arn;
+  SA_CHECK(!arn || ncv <= nev + mpd, "The value of ncv must not be larger than nev+mpd");
   hipStream_t s = stream();
   DenseProblem ds;
-  ds.allocate(ld, p.hermitian, p.which);
+  ds.allocate(ld, dsherm, p.which);
   std::vector<double> eigr(ld, 0.0), eigi(ld, 0.0), errest(ld, 0.0), buf, scratch;
   const double t0 = now();
   double tds = 0.0;
@@ -98,7 +103,7 @@ int ks_solve(const KsParams &p, const Mat &A, BV &V, bool have_initial, KsResult
     its++;
     int nv = std::min(nconv + mpd, ncv);
     const int k0 = nconv + l;
-    ds.set_dimensions(nv, nconv, k0);
+    ds.set_dimensions(nv, nconv, arn ? 0 : k0);
     double beta = 0.0;
     int breakdown = 0;
     const int64_t pbefore = V.passes_done;
@@ -120,7 +125,7 @@ int ks_solve(const KsParams &p, const Mat &A, BV &V, bool have_initial, KsResult
     SA_HIP(hipStreamSynchronize(s));
     const double td0 = now();
     auto B = [&](int row, int colc) { return buf[(size_t)(colc - k0 - 1) * m + row]; };
-    if (p.hermitian) {
+    if (dsherm) {
       double *d = ds.d(), *e = ds.e();
       for (int j = k0; j < nv; j++) {
         d[j] = B(j, j + 1);
@@ -133,7 +138,7 @@ int ks_solve(const KsParams &p, const Mat &A, BV &V, bool have_initial, KsResult
       for (int i = 0; i < nv; i++) H[i + (size_t)(nv - 1) * ld] = B(i, nv);
       if (ld > nv) H[nv + (size_t)(nv - 1) * ld] = B(nv, nv);
     }
-    ds.set_dimensions(nv, nconv, k0);
+    ds.set_dimensions(nv, nconv, arn ? 0 : k0);
     ds.set_state(l ? DS_RAW : DS_INTERMEDIATE);
     int info = ds.solve(eigr.data(), eigi.data());
     SA_CHECK(info == 0, "LAPACK failure in the projected problem (solve)");
@@ -158,6 +163,29 @@ int ks_solve(const KsParams &p, const Mat &A, BV &V, bool have_initial, KsResult
       if (marker != -1) break;
     }
     if (marker != -1) k = marker;
+    if (arn) {
+      // BVMultInPlace(V,Q,nconv,min(k+1,nv)) (arnoldi.c:150-153): the
+      // (k+1)-th Schur vector starts the next sweep
+      tds += now() - td0;
+      const int e = std::min(k + 1, nv);
+      if (e > nconv) {
+        std::memcpy(V.hq, ds.Q(), sizeof(double) * (size_t)ld * ld);
+        SA_HIP(hipMemcpyAsync(V.dq, V.hq, sizeof(double) * (size_t)ld * ld, hipMemcpyHostToDevice, s));
+        SA_TRY(launch_multinplace(V, nconv, nv, nconv, e, V.dq, ld, s));
+        SA_HIP(hipStreamSynchronize(s));   // the staging buffer is rewritten next iteration
+        r.bytes_restart += ((double)(nv - nconv) + (double)(e - nconv)) * V.n * es;
+      }
+      if (k >= nev) reason = 1;
+      else if (its >= p.max_it) reason = -1;
+      if (reason == 0 && breakdown) {    // arnoldi.c:156-162
+        int bd = 0;
+        SA_TRY(start_vector(V, k, false, p.eta, &bd, scratch));
+        if (bd) reason = -2;
+      }
+      nconv = k;
+      if (p.monitor) p.monitor(p.monitor_ctx, its, nconv, eigr.data(), errest.data(), nv);
+      continue;
+    }
     // EPSStoppingBasic (epsdefault.c:290-302)
     if (k >= nev) reason = 1;
     else if (its >= p.max_it) reason = -1;
@@ -198,7 +226,7 @@ int ks_solve(const KsParams &p, const Mat &A, BV &V, bool have_initial, KsResult
   int info = ds.trim_and_vectors(nconv);
   SA_CHECK(info == 0, "LAPACK failure computing eigenvectors of the projected problem");
   tds += now() - td1;
-  if (!p.hermitian && nconv > 0) {
+  if (!dsherm && nconv > 0) {
     // the last restart may still be reading the pinned staging buffer
     SA_HIP(hipStreamSynchronize(s));
     std::memcpy(V.hq, ds.X(), sizeof(double) * (size_t)ld * ld);

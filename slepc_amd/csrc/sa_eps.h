@@ -11,6 +11,7 @@ struct KsParams {
   int which = SA_LARGEST_MAGNITUDE;
   int nev = 1, ncv = 16, mpd = 16, max_it = 100;
   double tol = 1e-8, keep = 0.5, eta = 0.7071;
+  int solver = 0;              // 0 Krylov-Schur, 1 Arnoldi with explicit restart (EPSARNOLDI)
   void (*monitor)(void *, int, int, const double *, const double *, int) = nullptr;
   void *monitor_ctx = nullptr;
 };
@@ -27,8 +28,9 @@ struct KsResult {
 double spmv_bytes(const Mat &A);
 int bv_upload_column(BV &V, int j, const double *x);
 int bv_download_column(BV &V, int j, double *x);
-// Runs Krylov-Schur on A with basis V (V.m == ncv+1).  If have_initial, column
-// 0 of V holds the initial vector (EPSSetInitialSpace).
+// Runs Krylov-Schur (or, with p.solver == 1, explicitly restarted Arnoldi) on
+// A with basis V (V.m == ncv+1).  If have_initial, column 0 of V holds the
+// initial vector (EPSSetInitialSpace).
 int ks_solve(const KsParams &p, const Mat &A, BV &V, bool have_initial, KsResult &r);
 
 }  // namespace sa

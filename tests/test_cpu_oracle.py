@@ -336,3 +336,56 @@ def test_oracle_bv_dot_matches_numpy(orc):
     X, Y = rng.standard_normal((300, 7)), rng.standard_normal((300, 5))
     M = orc.bv_dot(X, Y)
     assert np.max(np.abs(M - Y.T @ X)) < 1e-13 * np.abs(Y.T @ X).max()
+
+
+# ------------------------------------------------------------------ EPSARNOLDI (oracle)
+def _lap1d(orc, n):
+    rp, col, val = [0], [], []
+    for i in range(n):
+        for c, v in ((i - 1, -1.0), (i, 2.0), (i + 1, -1.0)):
+            if 0 <= c < n:
+                col.append(c)
+                val.append(v)
+        rp.append(len(col))
+    return orc.CSR(n, np.array(rp, np.int32), np.array(col, np.int32), np.array(val))
+
+
+# eps/tests/output/test2_1.out (run with -eps_type arnoldi, among others)
+TEST2_LARGEST = [3.98974, 3.95906, 3.90828, 3.83792]
+TEST2_SMALLEST = [0.01026, 0.04094, 0.09172, 0.16208]
+
+
+@pytest.mark.parametrize("which", ["LR", "SR"])
+def test_oracle_arnoldi_test2(orc, which):
+    """eps/tests/test2.c with -eps_type arnoldi: 1-D Laplacian n = 30, nev 4,
+    tol max(1000 eps, 1e-9): the values printed by the reference
+    (test2_1.out) and the closed form 2 - 2 cos(k pi / 31)."""
+    A = _lap1d(orc, 30)
+    tol = max(1000 * 2.220446049250313e-16, 1e-9)
+    r = orc.eps_solve(A, 4, 0, tol, which, True, v0=orc.rand_uniform(0, A.n), solver="arnoldi")
+    assert r["nconv"] >= 4
+    gold = TEST2_LARGEST if which == "LR" else TEST2_SMALLEST
+    assert list(np.round(r["eigr"][:4], 5)) == gold
+    ex = np.sort(2 - 2 * np.cos(np.arange(1, 31) * np.pi / 31))
+    ex = ex[::-1][:4] if which == "LR" else ex[:4]
+    assert np.max(np.abs(r["eigr"][:4] - ex)) < 1e-9
+
+
+def test_oracle_arnoldi_nhep(orc):
+    """Arnoldi on non-symmetric operators.  Convection-diffusion at 12^3, 20
+    largest real, ncv 40: the values are dense LAPACK's (explicit restart
+    from one Schur vector needs ~600 restarts here; with nev 6, ncv 30 it
+    stalls at 4, so max_it is raised).  The Markov model of ex5/test9 (m = 15,
+    ncv 8): 1 and 0.97137 as printed in test9_1.out, and the dense values."""
+    A = orc.convdiff3d(12)
+    r = orc.eps_solve(A, 20, 40, 1e-10, "LR", False, v0=orc.rand_uniform(0, A.n), solver="arnoldi", max_it=3000)
+    assert r["nconv"] >= 20
+    ref = np.array(LIBRES["convdiff12"]["largest_real20"])
+    for v in r["eigr"][:20]:
+        assert np.min(np.abs(ref - v)) < 1e-9 * v
+    M = orc.markov(15)
+    r = orc.eps_solve(M, 4, 8, 1e-8, "LR", False, v0=orc.rand_uniform(0, M.n), solver="arnoldi", max_it=300)
+    assert r["nconv"] >= 4
+    assert list(np.round(r["eigr"][:2], 5)) == [1.0, 0.97137]
+    w = np.sort(np.linalg.eigvals(M.to_dense()).real)[::-1][:4]
+    assert np.max(np.abs(r["eigr"][:4] - w)) < 1e-7
