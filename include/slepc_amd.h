@@ -242,7 +242,8 @@ int sa_batch_compute_errors(sa_batch batch, int nev, double *err);
    seconds3: total, FP32 phase, host projected problems. */
 int sa_batch_get_stats(sa_batch batch, int64_t *counters6, double *seconds3);
 /* Sweep kernel of the batch: 0 autotune (default; both are timed on the first
-   restart round of each solve phase and the faster is kept), 1 one workgroup
+   restart round of the first solve phase of each precision and basis width,
+   and the faster is kept for later solves of the batch), 1 one workgroup
    per problem runs the whole sweep (ncv < 64), 2 lock-step: each Gram-Schmidt
    slot of every active problem is one launch over all their 64-row chunks.
    The getter returns the mode used by the last FP64 phase and the autotuner's

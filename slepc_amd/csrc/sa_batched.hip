@@ -533,6 +533,7 @@ struct sa_batch_s {
   int sweep_req = 0;                    // 0 autotune, 1 persistent, 2 lock-step
   int sweep_used = 0;                   // mode of the last solve's FP64 phase
   double ms_persistent = -1.0, ms_wide = -1.0;   // autotuner timings of one restart sweep
+  int tuned_m[2] = {0, 0}, tuned_mode[2] = {0, 0};   // autotuner's choice per precision (FP64, FP32) and basis width
   std::vector<int> n;
   std::vector<int64_t> off;
   int64_t nrows = 0, nnz = 0;
@@ -679,6 +680,8 @@ static int batch_ks(sa_batch_s *b, BatchDev &B, int which, int nev, int ncv, dou
   int mode = b->sweep_req;
   SA_CHECK(mode != 1 || persistent_fits(B), "the one-workgroup-per-problem sweep handles ncv < 64");
   if (mode == 0 && !persistent_fits(B)) mode = 2;
+  const int pi = B.prec == SA_FP64 ? 0 : 1;
+  if (mode == 0 && b->tuned_m[pi] == B.m) mode = b->tuned_mode[pi];   // tuned by an earlier solve
   SA_TRY(run_sweep(B, mode == 2 ? 2 : 1, b->n, order, k0, nv, 0, eta, s));
   SA_HIP(hipMemcpyAsync(stop.data(), B.stop, sizeof(int) * P, hipMemcpyDeviceToHost, s));
   SA_HIP(hipStreamSynchronize(s));
@@ -727,6 +730,8 @@ static int batch_ks(sa_batch_s *b, BatchDev &B, int which, int nev, int ncv, dou
       b->ms_persistent = t1;
       b->ms_wide = t2;
       mode = t2 < t1 ? 2 : 1;
+      b->tuned_m[pi] = B.m;
+      b->tuned_mode[pi] = mode;
     } else {
       SA_TRY(run_sweep(B, mode, b->n, act, k0, nv, 1, eta, s));
     }
@@ -1093,10 +1098,13 @@ int sa_batch_get_stats(sa_batch b, int64_t *counters6, double *seconds3) {
 }
 
 // Sweep kernel choice: 0 autotune (time both on the first restart round of
-// every solve phase, keep the faster), 1 one workgroup per problem, 2 lock-step.
+// the first solve phase of each precision and basis width, keep the faster
+// for later solves of the batch), 1 one workgroup per problem, 2 lock-step.
+// Setting a mode forgets earlier autotuner choices.
 int sa_batch_set_sweep(sa_batch b, int mode) {
   SA_CHECK(b && mode >= 0 && mode <= 2, "invalid sweep mode");
   b->sweep_req = mode;
+  b->tuned_m[0] = b->tuned_m[1] = 0;
   return 0;
 }
 

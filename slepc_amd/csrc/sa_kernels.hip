@@ -57,6 +57,7 @@ __global__ __launch_bounds__(BLK) void spmv_stream_kernel(int n, const int32_t *
                                                           const T *__restrict__ val, const T *__restrict__ x,
                                                           T *__restrict__ y) {
   constexpr int TPR = BLK / RB;
+  static_assert(RB <= BLK && BLK % RB == 0, "rows per block must divide the workgroup");
   constexpr int NZB = BLK * SPMV_PT;
   static_assert(NZB == SPMV_NZB, "block capacity differs from the host's");
   __shared__ T prod[NZB];
