@@ -181,11 +181,14 @@ int sa_eps_set_operators(sa_eps eps, sa_mat A);
 /* Optional second operator in FP32 used by the mixed-precision bulk phase. */
 int sa_eps_set_operators_fp32(sa_eps eps, sa_mat A32);
 int sa_eps_set_problem_type(sa_eps eps, int hermitian);
-/* EPSSetType: SA_EPS_KRYLOVSCHUR (default, krylovschur.c) or SA_EPS_ARNOLDI
-   (explicitly restarted Arnoldi, arnoldi.c:87-172: always a non-symmetric
-   projected problem, ncv <= nev + mpd). */
+/* EPSSetType: SA_EPS_KRYLOVSCHUR (default, krylovschur.c), SA_EPS_ARNOLDI
+   (explicitly restarted Arnoldi, arnoldi.c:52-131: always a non-symmetric
+   projected problem, ncv <= nev + mpd) or SA_EPS_LANCZOS (explicitly
+   restarted Lanczos with full reorthogonalization, lanczos.c:518-697 with
+   -eps_lanczos_reorthog full; symmetric problems, ncv <= nev + mpd). */
 #define SA_EPS_KRYLOVSCHUR 0
 #define SA_EPS_ARNOLDI 1
+#define SA_EPS_LANCZOS 2
 int sa_eps_set_type(sa_eps eps, int type);
 int sa_eps_set_which(sa_eps eps, int which);
 int sa_eps_set_dimensions(sa_eps eps, int nev, int ncv, int mpd);

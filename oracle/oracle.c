@@ -14,6 +14,7 @@
  *   EPSConvergedRelative           src/eps/interface/epsdefault.c:224-232
  *   EPSStoppingBasic               src/eps/interface/epsdefault.c:290-302
  *   SlepcSortEigenvalues           src/sys/slepcsc.c:89-140
+ *   EPSArnoldi / EPSLanczos(full)  src/eps/impls/krylov/arnoldi/arnoldi.c:31-131, lanczos/lanczos.c:518-697
  *   BVDot / BVMatMult              src/sys/classes/bv/interface/bvglobal.c:86-120, bvops.c:692-722
  *   BVOrthogonalize (gs/chol/svqb) src/sys/classes/bv/interface/bvorthog.c:489-767, bvlapack.c:136-330
  * The dense projected problems call host LAPACK exactly where the reference does
@@ -905,13 +906,21 @@ int orc_eps_solve(int n, const int32_t *rp, const int32_t *ci, const double *val
   int max_it = o->max_it > 0 ? o->max_it : (100 > 2 * n / ncv ? 100 : 2 * n / ncv);
   double tol = o->tol > 0 ? o->tol : 1e-8, keep = o->keep > 0 ? o->keep : 0.5;
   int herm = o->hermitian, which = o->which ? o->which : ORC_LM, lock = o->lock;
-  /* EPSARNOLDI (arnoldi.c:87-172): explicit restart, always DSNHEP (:188);
+  /* EPSARNOLDI (arnoldi.c:52-131): explicit restart, always DSNHEP (:45);
      ncv <= nev + mpd (:181) */
   const int arn = o->solver == 1;
   if (arn) {
     if (ncv > nev + mpd) return -11;
     herm = 0;
   }
+  /* EPSLANCZOS, full reorthogonalization (lanczos.c:518-697): explicit
+     restart from the most wanted unconverged Ritz vector, DSHEP */
+  const int lanc = o->solver == 2;
+  if (lanc && (ncv > nev + mpd || !herm)) return -11;
+  double anorm = -1.0;
+  double *bnd = (double *)calloc(ncv + 1, sizeof(double)), *yr = (double *)calloc(ncv + 1, sizeof(double));
+  char *cv = (char *)calloc(ncv + 1, 1);
+  int *lp = (int *)calloc(ncv + 1, sizeof(int));
   KSo ks = {0};
   ks.n = n; ks.rp = rp; ks.ci = ci; ks.val = val; ks.m = ncv + 1; ks.ld = n;
   ks.eta = o->eta > 0 ? o->eta : 0.7071;
@@ -950,7 +959,7 @@ int orc_eps_solve(int n, const int32_t *rp, const int32_t *ci, const double *val
   while (reason == 0) {
     its++;
     int nv = nconv + mpd < ncv ? nconv + mpd : ncv;
-    ds.n = nv; ds.t = nv; ds.l = nconv; ds.k = arn ? 0 : nconv + l;   /* DSSetDimensions */
+    ds.n = nv; ds.t = nv; ds.l = nconv; ds.k = (arn || lanc) ? 0 : nconv + l;   /* DSSetDimensions */
     double beta = 0.0; int breakdown = 0, k0 = nconv + l;
     ks_expand(&ks, k0, &nv, &beta, &breakdown);
     int bm = ks.m;
@@ -962,10 +971,67 @@ int orc_eps_solve(int n, const int32_t *rp, const int32_t *ci, const double *val
       memcpy(H + (int64_t)(nv - 1) * ld, ks.buf + (int64_t)nv * bm, sizeof(double) * nv);
       if (ld > nv) H[nv + (nv - 1) * ld] = ks.buf[nv + (int64_t)nv * bm];
     }
-    ds.n = nv; ds.t = nv; ds.l = nconv; ds.k = arn ? 0 : nconv + l;
+    ds.n = nv; ds.t = nv; ds.l = nconv; ds.k = (arn || lanc) ? 0 : nconv + l;
     ds.state = l ? ST_RAW : ST_INTERMEDIATE;
     int info = herm ? ds_solve_hep(&ds, eigr, eigi) : ds_solve_nhep(&ds, eigr, eigi);
     if (info) { rc = -20 - info; goto done; }
+    if (lanc) {
+      ds_sort_hep(&ds, eigr);
+      double *Y = ds.Q;
+      for (int i = nconv; i < nv; i++) if (fabs(eigr[i]) > anorm) anorm = fabs(eigr[i]);
+      for (int i = nconv; i < nv; i++) {                     /* beta |y_m| + eps ||A||, relative */
+        double res = beta * fabs(Y[nv - 1 + (int64_t)i * ld]) + 2.220446049250313e-16 * anorm;
+        bnd[i] = eigr[i] != 0.0 ? res / fabs(eigr[i]) : 1.79e308;
+        cv[i] = bnd[i] < tol ? 'C' : 'N';
+      }
+      if (!breakdown) {                                      /* restart vector (lanczos.c:577-593) */
+        int rs = nconv;
+        while (rs < nv && cv[rs] != 'N') rs++;
+        if (rs >= nv) breakdown = 1;
+        else {
+          for (int i = rs + 1; i < nv; i++)
+            if (cv[i] == 'N' && sc_compare(which, eigr[rs], 0.0, eigr[i], 0.0) > 0) rs = i;
+          for (int i = nconv; i < nv; i++) yr[i - nconv] = Y[i + (int64_t)rs * ld];
+          orc_bv_multvec(n, nv - nconv, 1.0, ks.V + (int64_t)nconv * ks.ld, ks.ld, yr, 0.0, ks.V + (int64_t)nv * ks.ld);
+        }
+      }
+      int k;                                                 /* converged first (:780-808) */
+      for (int i = nconv; i < nv; i++) lp[i] = i;
+      for (k = nconv; k < nv; k++) {
+        if (cv[lp[k]] != 'C') {
+          int j = k + 1;
+          while (j < nv && cv[lp[j]] != 'C') j++;
+          if (j >= nv) break;
+          int t = lp[k]; lp[k] = lp[j]; lp[j] = t;
+        }
+      }
+      for (int i = nconv; i < k; i++) {
+        int x = lp[i];
+        if (x != i) {
+          int j = i + 1;
+          while (lp[j] != i) j++;
+          double td = eigr[x]; eigr[x] = eigr[i]; eigr[i] = td;
+          td = bnd[x]; bnd[x] = bnd[i]; bnd[i] = td;
+          char tc = cv[x]; cv[x] = cv[i]; cv[i] = tc;
+          lp[j] = x; lp[i] = i;
+          for (int q = 0; q < nv; q++) { td = Y[q + (int64_t)x * ld]; Y[q + (int64_t)x * ld] = Y[q + (int64_t)i * ld]; Y[q + (int64_t)i * ld] = td; }
+        }
+      }
+      if (k > nconv) orc_bv_multinplace(n, ks.V, ks.ld, nconv, nv, nconv, k, Y, ld);
+      for (int i = nconv; i < nv; i++) errest[i] = bnd[i];
+      nconv = k;
+      if (k >= nev) reason = 1;
+      else if (its >= max_it) reason = -1;
+      if (reason == 0) {                                     /* BVCopyColumn(V,nv,nconv) (:826) */
+        memcpy(ks.V + (int64_t)nconv * ks.ld, ks.V + (int64_t)nv * ks.ld, sizeof(double) * (size_t)n);
+        if (breakdown) {
+          int bd = 0;
+          ks_start_vector(&ks, nconv, nini, &bd);
+          if (bd) reason = -2;
+        }
+      }
+      continue;
+    }
     if (herm) { ds_sort_hep(&ds, eigr); ds_update_extra_row_hep(&ds); }
     else {
       if ((info = ds_sort_nhep(&ds, eigr, eigi))) { rc = -40; goto done; }
@@ -987,13 +1053,13 @@ int orc_eps_solve(int n, const int32_t *rp, const int32_t *ci, const double *val
     }
     if (marker != -1) k = marker;
     if (arn) {
-      /* BVMultInPlace(V,Q,nconv,min(k+1,nv)) (arnoldi.c:150-153); the
+      /* BVMultInPlace(V,Q,nconv,min(k+1,nv)) (arnoldi.c:109-115); the
          (k+1)-th Schur vector is the next start vector */
       int e = k + 1 < nv ? k + 1 : nv;
       orc_bv_multinplace(n, ks.V, ks.ld, nconv, nv, nconv, e, ds.Q, ld);
       if (k >= nev) reason = 1;
       else if (its >= max_it) reason = -1;
-      if (reason == 0 && breakdown) {                          /* arnoldi.c:156-162 */
+      if (reason == 0 && breakdown) {                          /* arnoldi.c:118-125 */
         int bd = 0;
         ks_start_vector(&ks, k, nini, &bd);
         if (bd) reason = -2;
@@ -1090,6 +1156,7 @@ done:
 #endif
   free(ks.V); free(ks.buf); free(ds.A); free(ds.Q); free(ds.X); free(ds.T); free(ds.work); free(ds.perm); free(ds.iwork);
   free(eigr); free(eigi); free(errest); free(perm);
+  free(bnd); free(yr); free(cv); free(lp);
   return rc;
 }
 

@@ -75,7 +75,8 @@ typedef struct {
   double  eta;         /* DGKS threshold, default 0.7071 */
   int     lock;        /* locking variant (default 1) */
   int     nthreads;    /* OpenMP threads, 0 = default */
-  int     solver;      /* 0 Krylov-Schur, 1 Arnoldi with explicit restart (arnoldi.c) */
+  int     solver;      /* 0 Krylov-Schur, 1 Arnoldi with explicit restart (arnoldi.c),
+                          2 Lanczos with full reorthogonalization (lanczos.c) */
 } orc_opts;
 
 typedef struct {

@@ -261,7 +261,7 @@ def eps_solve(A, nev, ncv=0, tol=1e-8, which="LM", hermitian=True, v0=None, max_
               mpd=0, nvec=None, nthreads=0, keep=0.5, lock=1, solver="krylovschur"):
     o = Opts(hermitian=1 if hermitian else 0, which=WHICH[which], nev=nev, ncv=ncv, mpd=mpd,
              max_it=max_it, tol=tol, keep=keep, eta=0.7071, lock=lock, nthreads=nthreads,
-             solver={"krylovschur": 0, "arnoldi": 1}[solver])
+             solver={"krylovschur": 0, "arnoldi": 1, "lanczos": 2}[solver])
     ncv_eff = ncv if ncv else min(A.n, max(2 * nev, nev + 15))
     eigr = np.zeros(ncv_eff + 1)
     eigi = np.zeros(ncv_eff + 1)

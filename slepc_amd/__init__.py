@@ -350,8 +350,9 @@ class EPS:
         check(lib().sa_eps_set_problem_type(self.h, 1 if hermitian else 0))
 
     def setType(self, eps_type):
-        """EPSSetType: "krylovschur" (default) or "arnoldi"."""
-        check(lib().sa_eps_set_type(self.h, {"krylovschur": 0, "arnoldi": 1}[eps_type]))
+        """EPSSetType: "krylovschur" (default), "arnoldi" or "lanczos" (full
+        reorthogonalization)."""
+        check(lib().sa_eps_set_type(self.h, {"krylovschur": 0, "arnoldi": 1, "lanczos": 2}[eps_type]))
 
     def setWhichEigenpairs(self, which):
         check(lib().sa_eps_set_which(self.h, WHICH[which] if isinstance(which, str) else which))

@@ -481,7 +481,7 @@ int sa_eps_set_problem_type(sa_eps E, int hermitian) {
 }
 
 int sa_eps_set_type(sa_eps E, int type) {
-  SA_CHECK(E && (type == SA_EPS_KRYLOVSCHUR || type == SA_EPS_ARNOLDI), "invalid EPS type");
+  SA_CHECK(E && (type == SA_EPS_KRYLOVSCHUR || type == SA_EPS_ARNOLDI || type == SA_EPS_LANCZOS), "invalid EPS type");
   E->p.solver = type;
   return 0;
 }

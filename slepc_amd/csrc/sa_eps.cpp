@@ -1,9 +1,12 @@
-// Krylov-Schur eigensolver driver: the restart loop of
-// EPSSolve_KrylovSchur_Default (src/eps/impls/krylov/krylovschur/krylovschur.c:227-348)
-// over device-resident BV/Mat objects, with the projected problem on the host.
+// Eigensolver driver: the restart loop of EPSSolve_KrylovSchur_Default
+// (src/eps/impls/krylov/krylovschur/krylovschur.c:227-348), and the explicit
+// restarts of EPSARNOLDI (arnoldi.c:52-131) and EPSLANCZOS with full
+// reorthogonalization (lanczos.c:518-697), over device-resident BV/Mat
+// objects, with the projected problem on the host.
 #include "sa_eps.h"
 
 #include <algorithm>
+#include <cfloat>
 #include <chrono>
 #include <cmath>
 #include <cstring>
@@ -68,16 +71,110 @@ static int start_vector(BV &V, int i, bool have_initial, double eta, int *breakd
   return 0;
 }
 
+// The restart of EPSSolve_Lanczos with full reorthogonalization
+// (lanczos.c:555-690): residual bounds beta |y_m| + eps ||A||, the most
+// wanted unconverged Ritz vector as the next start vector (built in column
+// nv, then copied to the new column nconv), converged pairs moved to the
+// front, V(:,nconv:k) = V(:,nconv:nv) Y(nconv:nv, nconv:k), nconv = k.
+static int lanczos_restart(const KsParams &p, BV &V, const DenseProblem &ds, int &nconv, int nv, double beta,
+                           bool breakdown, std::vector<double> &eigr, std::vector<double> &errest, double &anorm,
+                           int its, int &reason, std::vector<double> &scratch, std::vector<double> &bnd,
+                           std::vector<char> &conv, std::vector<int> &perm, std::vector<double> &Y,
+                           std::vector<double> &yrst, KsResult &r) {
+  hipStream_t s = stream();
+  const int ld = ds.ld();
+  const double es = (double)prec_size(V.prec);
+  bnd.assign(ld, 0.0);
+  conv.assign(ld, 'N');
+  Y.assign(ds.Q(), ds.Q() + (size_t)ld * ld);
+  for (int i = nconv; i < nv; i++) anorm = std::max(anorm, std::fabs(eigr[i]));
+  for (int i = nconv; i < nv; i++) {
+    const double res = beta * std::fabs(Y[nv - 1 + (size_t)i * ld]) + DBL_EPSILON * anorm;
+    bnd[i] = eigr[i] != 0.0 ? res / std::fabs(eigr[i]) : 1.79e308;   // EPSConvergedRelative
+    conv[i] = bnd[i] < p.tol ? 'C' : 'N';
+  }
+  if (!breakdown) {
+    int rs = nconv;
+    while (rs < nv && conv[rs] != 'N') rs++;
+    if (rs >= nv) {
+      breakdown = true;
+    } else {
+      for (int i = rs + 1; i < nv; i++)
+        if (conv[i] == 'N' && sc_compare(p.which, eigr[rs], 0.0, eigr[i], 0.0) > 0) rs = i;
+      // BVMultColumn(V,1,0,nv,y): V(:,nv) = V(:,nconv:nv) Y(nconv:nv, rs)
+      yrst.assign(Y.begin() + (size_t)rs * ld + nconv, Y.begin() + (size_t)rs * ld + nv);
+      SA_HIP(hipMemcpy(V.ws.dots, yrst.data(), sizeof(double) * yrst.size(), hipMemcpyHostToDevice));
+      SA_TRY(launch_multvec(V, nconv, nv, 1.0, V.ws.dots, 0.0, col_ptr(V, nv), s));
+      r.bytes_bv += ((double)(nv - nconv) + 1.0) * V.n * es;
+    }
+  }
+  // count converged pairs and put them first
+  perm.resize(nv);
+  for (int i = nconv; i < nv; i++) perm[i] = i;
+  int k;
+  for (k = nconv; k < nv; k++) {
+    if (conv[perm[k]] != 'C') {
+      int j = k + 1;
+      while (j < nv && conv[perm[j]] != 'C') j++;
+      if (j >= nv) break;
+      std::swap(perm[k], perm[j]);
+    }
+  }
+  for (int i = nconv; i < k; i++) {
+    const int x = perm[i];
+    if (x != i) {
+      int j = i + 1;
+      while (perm[j] != i) j++;
+      std::swap(eigr[x], eigr[i]);
+      std::swap(bnd[x], bnd[i]);
+      std::swap(conv[x], conv[i]);
+      perm[j] = x;
+      perm[i] = i;
+      for (int q = 0; q < nv; q++) std::swap(Y[q + (size_t)x * ld], Y[q + (size_t)i * ld]);
+    }
+  }
+  if (k > nconv) {   // the converged Ritz vectors
+    std::memcpy(V.hq, Y.data(), sizeof(double) * (size_t)ld * ld);
+    SA_HIP(hipMemcpyAsync(V.dq, V.hq, sizeof(double) * (size_t)ld * ld, hipMemcpyHostToDevice, s));
+    SA_TRY(launch_multinplace(V, nconv, nv, nconv, k, V.dq, ld, s));
+    SA_HIP(hipStreamSynchronize(s));
+    r.bytes_restart += ((double)(nv - nconv) + (double)(k - nconv)) * V.n * es;
+  }
+  for (int i = nconv; i < nv; i++) errest[i] = bnd[i];
+  nconv = k;
+  if (k >= p.nev) reason = 1;
+  else if (its >= p.max_it) reason = -1;
+  if (reason == 0) {   // BVCopyColumn(V,nv,nconv); a random vector after a breakdown
+    SA_HIP(hipMemcpyAsync(col_ptr(V, nconv), col_ptr(V, nv), es * V.n, hipMemcpyDeviceToDevice, s));
+    r.bytes_bv += 2.0 * V.n * es;
+    if (breakdown) {
+      int bd = 0;
+      SA_TRY(start_vector(V, nconv, false, p.eta, &bd, scratch));
+      if (bd) reason = -2;
+    }
+  }
+  return 0;
+}
+
 int ks_solve(const KsParams &p, const Mat &A, BV &V, bool have_initial, KsResult &r) {
   SA_CHECK(lapack_ready(), "no LAPACK library found for the projected problem");
   const int n = A.n, ncv = p.ncv, nev = p.nev, mpd = p.mpd, ld = ncv + 1;
   SA_CHECK(V.m == ncv + 1 && V.n == n && V.prec == A.prec, "basis does not match the operator/ncv");
   SA_CHECK(ncv + 1 < GS_MAXC, "ncv too large for the fused Gram-Schmidt kernels");
-  // EPSARNOLDI (arnoldi.c:87-172, setup :176-194): explicit restart and
+  // EPSARNOLDI (arnoldi.c:52-131, setup :31-50): explicit restart and
   // always a DSNHEP projected problem; ncv <= nev + mpd
   const bool arn = p.solver == 1;
+  // EPSLANCZOS with full reorthogonalization (lanczos.c:518-697, setup
+  // :36-78): explicit restart from the most wanted unconverged Ritz
+  // vector, DSHEP, converged pairs moved to the front and locked
+  const bool lanc = p.solver == 2;
   const bool dsherm = p.hermitian && !arn;
-  SA_CHECK(!arn || ncv <= nev + mpd, "The value of ncv must not be larger than nev+mpd");
+  SA_CHECK(!((arn || lanc) && ncv > nev + mpd), "The value of ncv must not be larger than nev+mpd");
+  SA_CHECK(!lanc || p.hermitian, "EPSLANCZOS needs a symmetric problem");
+  double anorm = -1.0;              // running estimate of ||A|| (Lanczos)
+  std::vector<double> bnd, Yl, yrst;
+  std::vector<char> conv;
+  std::vector<int> lperm;
   hipStream_t s = stream();
   DenseProblem ds;
   ds.allocate(ld, dsherm, p.which);
@@ -103,7 +200,7 @@ int ks_solve(const KsParams &p, const Mat &A, BV &V, bool have_initial, KsResult
     its++;
     int nv = std::min(nconv + mpd, ncv);
     const int k0 = nconv + l;
-    ds.set_dimensions(nv, nconv, arn ? 0 : k0);
+    ds.set_dimensions(nv, nconv, (arn || lanc) ? 0 : k0);
     double beta = 0.0;
     int breakdown = 0;
     const int64_t pbefore = V.passes_done;
@@ -138,12 +235,19 @@ int ks_solve(const KsParams &p, const Mat &A, BV &V, bool have_initial, KsResult
       for (int i = 0; i < nv; i++) H[i + (size_t)(nv - 1) * ld] = B(i, nv);
       if (ld > nv) H[nv + (size_t)(nv - 1) * ld] = B(nv, nv);
     }
-    ds.set_dimensions(nv, nconv, arn ? 0 : k0);
+    ds.set_dimensions(nv, nconv, (arn || lanc) ? 0 : k0);
     ds.set_state(l ? DS_RAW : DS_INTERMEDIATE);
     int info = ds.solve(eigr.data(), eigi.data());
     SA_CHECK(info == 0, "LAPACK failure in the projected problem (solve)");
     info = ds.sort(eigr.data(), eigi.data());
     SA_CHECK(info == 0, "LAPACK failure in the projected problem (sort)");
+    if (lanc) {
+      SA_TRY(lanczos_restart(p, V, ds, nconv, nv, beta, breakdown != 0, eigr, errest, anorm, its, reason, scratch,
+                             bnd, conv, lperm, Yl, yrst, r));
+      tds += now() - td0;
+      if (p.monitor) p.monitor(p.monitor_ctx, its, nconv, eigr.data(), errest.data(), nv);
+      continue;
+    }
     ds.update_extra_row();
     // EPSKrylovConvergence (epskrylov.c:207-295) with EPSConvergedRelative (epsdefault.c:224-232)
     int k = nconv, marker = -1;
@@ -164,7 +268,7 @@ int ks_solve(const KsParams &p, const Mat &A, BV &V, bool have_initial, KsResult
     }
     if (marker != -1) k = marker;
     if (arn) {
-      // BVMultInPlace(V,Q,nconv,min(k+1,nv)) (arnoldi.c:150-153): the
+      // BVMultInPlace(V,Q,nconv,min(k+1,nv)) (arnoldi.c:109-115): the
       // (k+1)-th Schur vector starts the next sweep
       tds += now() - td0;
       const int e = std::min(k + 1, nv);
@@ -177,7 +281,7 @@ int ks_solve(const KsParams &p, const Mat &A, BV &V, bool have_initial, KsResult
       }
       if (k >= nev) reason = 1;
       else if (its >= p.max_it) reason = -1;
-      if (reason == 0 && breakdown) {    // arnoldi.c:156-162
+      if (reason == 0 && breakdown) {    // arnoldi.c:118-125
         int bd = 0;
         SA_TRY(start_vector(V, k, false, p.eta, &bd, scratch));
         if (bd) reason = -2;
@@ -221,10 +325,13 @@ int ks_solve(const KsParams &p, const Mat &A, BV &V, bool have_initial, KsResult
     nconv = k;
     if (p.monitor) p.monitor(p.monitor_ctx, its, nconv, eigr.data(), errest.data(), nv);
   }
-  // DSTruncate(nconv,TRUE); EPSComputeVectors (Schur: V <- V X)
+  // DSTruncate(nconv,TRUE); EPSComputeVectors (Schur: V <- V X).  Lanczos
+  // leaves Ritz vectors in the leading columns already.
   const double td1 = now();
-  int info = ds.trim_and_vectors(nconv);
-  SA_CHECK(info == 0, "LAPACK failure computing eigenvectors of the projected problem");
+  if (!lanc) {
+    int info = ds.trim_and_vectors(nconv);
+    SA_CHECK(info == 0, "LAPACK failure computing eigenvectors of the projected problem");
+  }
   tds += now() - td1;
   if (!dsherm && nconv > 0) {
     // the last restart may still be reading the pinned staging buffer

@@ -355,14 +355,16 @@ TEST2_LARGEST = [3.98974, 3.95906, 3.90828, 3.83792]
 TEST2_SMALLEST = [0.01026, 0.04094, 0.09172, 0.16208]
 
 
+@pytest.mark.parametrize("solver", ["arnoldi", "lanczos"])
 @pytest.mark.parametrize("which", ["LR", "SR"])
-def test_oracle_arnoldi_test2(orc, which):
-    """eps/tests/test2.c with -eps_type arnoldi: 1-D Laplacian n = 30, nev 4,
-    tol max(1000 eps, 1e-9): the values printed by the reference
-    (test2_1.out) and the closed form 2 - 2 cos(k pi / 31)."""
+def test_oracle_explicit_restart_test2(orc, which, solver):
+    """eps/tests/test2.c with -eps_type arnoldi and with -eps_type lanczos
+    -eps_lanczos_reorthog full: 1-D Laplacian n = 30, nev 4, tol
+    max(1000 eps, 1e-9): the values printed by the reference (test2_1.out)
+    and the closed form 2 - 2 cos(k pi / 31)."""
     A = _lap1d(orc, 30)
     tol = max(1000 * 2.220446049250313e-16, 1e-9)
-    r = orc.eps_solve(A, 4, 0, tol, which, True, v0=orc.rand_uniform(0, A.n), solver="arnoldi")
+    r = orc.eps_solve(A, 4, 0, tol, which, True, v0=orc.rand_uniform(0, A.n), solver=solver)
     assert r["nconv"] >= 4
     gold = TEST2_LARGEST if which == "LR" else TEST2_SMALLEST
     assert list(np.round(r["eigr"][:4], 5)) == gold
@@ -389,3 +391,14 @@ def test_oracle_arnoldi_nhep(orc):
     assert list(np.round(r["eigr"][:2], 5)) == [1.0, 0.97137]
     w = np.sort(np.linalg.eigvals(M.to_dense()).real)[::-1][:4]
     assert np.max(np.abs(r["eigr"][:4] - w)) < 1e-7
+
+
+def test_oracle_lanczos_stencil27(orc):
+    """Lanczos with full reorthogonalization on the 27-point 10^3 operator,
+    8 largest magnitude, ncv 24: the values are dense LAPACK's."""
+    T = orc.stencil27(10, seed=1)
+    r = orc.eps_solve(T, 8, 24, 1e-10, "LM", True, v0=orc.rand_uniform(0, T.n), solver="lanczos", max_it=2000)
+    assert r["nconv"] >= 8
+    ref = np.array(LIBRES["stencil27_10"]["largest_mag32"])
+    for v in r["eigr"][:8]:
+        assert np.min(np.abs(ref - v)) < 1e-12 * abs(v)

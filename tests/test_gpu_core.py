@@ -374,25 +374,31 @@ def test_concurrent_solves_match_sequential(sa):
 
 
 # ------------------------------------------------------------------ EPSARNOLDI
-@pytest.mark.parametrize("case", ["test2_LR", "test2_SR", "markov15", "convdiff12"])
-def test_arnoldi_matches_oracle(sa, orc, case):
-    """Explicitly restarted Arnoldi (EPSARNOLDI) through the C ABI against the
-    oracle's restatement from the same start vector: eigenvalues, restart
-    count and residuals.  test2: 1-D Laplacian n = 30 (values of
-    test2_1.out); markov15: NHEP, ncv 8; convdiff12: NHEP, 20 largest real,
-    ncv 40 (a long explicit-restart trajectory: values and residuals only)."""
+@pytest.mark.parametrize("solver,case", [("arnoldi", "test2_LR"), ("arnoldi", "test2_SR"), ("arnoldi", "markov15"),
+                                         ("arnoldi", "convdiff12"), ("lanczos", "test2_LR"), ("lanczos", "test2_SR"),
+                                         ("lanczos", "stencil27_10")])
+def test_explicit_restart_matches_oracle(sa, orc, solver, case):
+    """Explicitly restarted Arnoldi (EPSARNOLDI) and Lanczos with full
+    reorthogonalization (EPSLANCZOS) through the C ABI against the oracle's
+    restatements from the same start vector: eigenvalues, restart count and
+    residuals.  test2: 1-D Laplacian n = 30 (values of test2_1.out);
+    markov15: NHEP, ncv 8; convdiff12 (NHEP, 20 largest real, ncv 40) and
+    stencil27_10 (8 largest magnitude, ncv 24) are long explicit-restart
+    trajectories: values and residuals only."""
     from test_cpu_oracle import _lap1d, TEST2_LARGEST, TEST2_SMALLEST
     if case.startswith("test2"):
         Ao, which, herm, nev, ncv, tol, mi = _lap1d(orc, 30), case[-2:], True, 4, 0, 1e-9, 0
     elif case == "markov15":
         Ao, which, herm, nev, ncv, tol, mi = orc.markov(15), "LR", False, 4, 8, 1e-8, 300
+    elif case == "stencil27_10":
+        Ao, which, herm, nev, ncv, tol, mi = orc.stencil27(10, seed=1), "LM", True, 8, 24, 1e-10, 2000
     else:
         Ao, which, herm, nev, ncv, tol, mi = orc.convdiff3d(12), "LR", False, 20, 40, 1e-10, 3000
     Ah = sa.CSR(Ao.n, Ao.rowptr, Ao.col, Ao.val)
     v0 = orc.rand_uniform(0, Ao.n)
     A = sa.Mat(Ah)
     E = sa.EPS()
-    E.setType("arnoldi")
+    E.setType(solver)
     E.setOperators(A)
     E.setProblemType(herm)
     E.setWhichEigenpairs(which)
@@ -400,7 +406,7 @@ def test_arnoldi_matches_oracle(sa, orc, case):
     E.setTolerances(tol, mi)
     E.setInitialSpace(v0)
     E.solve()
-    r = orc.eps_solve(Ao, nev, ncv, tol, which, herm, v0=v0, solver="arnoldi", max_it=mi)
+    r = orc.eps_solve(Ao, nev, ncv, tol, which, herm, v0=v0, solver=solver, max_it=mi)
     nc = E.getConverged()
     assert nc >= nev and r["nconv"] >= nev
     vals = np.array([E.getEigenpair(i, False)[0] for i in range(nev)])
@@ -412,7 +418,7 @@ def test_arnoldi_matches_oracle(sa, orc, case):
     # the short trajectories restart exactly as the oracle's; the ~600-restart
     # convdiff12 run crawls through near-stagnation, where rounding decides
     # when each pair converges, so only its values and residuals are compared
-    if case != "convdiff12":
+    if case not in ("convdiff12", "stencil27_10"):
         assert abs(E.getIterationNumber() - r["its"]) <= 1
     for i in range(nev):
         assert E.computeError(i) < 10 * tol
