@@ -162,7 +162,7 @@ def block_ops_roofline(sa, Ah, nc=32, reps=5):
     """The block BV operations on an n-row FP64 basis: BVDot of two nc-column
     blocks (algorithmic bytes 2 nc n 8), BVMatMult of 8 columns through the
     operator (values and indices once, row pointers, 8 x and 8 y columns) and
-    BVOrthogonalize (chol) of nc columns; HIP events, averages of `reps`."""
+    BVOrthogonalize (chol, tsqr) of nc columns; HIP events, averages of `reps`."""
     L = sa.lib()
     n = Ah.n
 
@@ -189,6 +189,7 @@ def block_ops_roofline(sa, Ah, nc=32, reps=5):
     X.set_active_columns(0, nc)
     # after the first call the block is orthonormal; later calls do the same work
     us_orth = timed(lambda: X.orthogonalize("chol", with_R=False))
+    us_tsqr = timed(lambda: X.orthogonalize("tsqr", with_R=False))
     X.destroy()
     Y.destroy()
     b_dot = 2.0 * nc * n * 8
@@ -197,7 +198,8 @@ def block_ops_roofline(sa, Ah, nc=32, reps=5):
                       "frac": round(b_dot / us_dot / 1e3 / HBM_PEAK_GBS, 4)},
            "bv_matmult": {"cols": 8, "us": round(us_mm, 1), "GBs": round(b_mm / us_mm / 1e3, 1),
                           "frac": round(b_mm / us_mm / 1e3 / HBM_PEAK_GBS, 4)},
-           "bv_orthogonalize_chol": {"cols": nc, "us": round(us_orth, 1)}}
+           "bv_orthogonalize_chol": {"cols": nc, "us": round(us_orth, 1)},
+           "bv_orthogonalize_tsqr": {"cols": nc, "us": round(us_tsqr, 1)}}
     return out
 
 

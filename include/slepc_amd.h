@@ -150,10 +150,16 @@ int sa_bv_mat_arnoldi(sa_bv V, sa_mat A, int k, int *m, double eta, double *beta
    Gram matrix; R is then not triangular).  R (host, ldr >= k, or NULL)
    receives columns l..k-1 of the factor: rows 0..l-1 the block-GS
    coefficients, rows l..k-1 the factor of V2 (zeros below the diagonal for GS
-   and CHOL).  A dependent column stops GS with an error, as in the reference. */
+   and CHOL).  A dependent column stops GS with an error, as in the reference.
+   _TSQR (bvorthog.c:615-631, bvlapack.c:347-480): Householder QR over a tree
+   of 256-row tiles, Q formed explicitly; _TSQRCHOL (bvorthog.c:638-659): the
+   same tree for R only, then V2 = V2 R^-1.  Both return the Q and R that
+   geqrf/orgqr give on one process (same signs); at most 64 active columns. */
 #define SA_ORTHOG_BLOCK_GS 0
 #define SA_ORTHOG_BLOCK_CHOL 1
 #define SA_ORTHOG_BLOCK_SVQB 2
+#define SA_ORTHOG_BLOCK_TSQR 3
+#define SA_ORTHOG_BLOCK_TSQRCHOL 4
 int sa_bv_dot(sa_bv X, sa_bv Y, double *M_host, int ldm);
 int sa_bv_matmult(sa_bv X, sa_mat A, sa_bv Y);
 int sa_bv_orthogonalize(sa_bv V, int method, double *R_host, int ldr);

@@ -265,11 +265,14 @@ typedef void (*f_potrf)(const char *, const int *, double *, const int *, int *,
 typedef void (*f_trtri)(const char *, const char *, const int *, double *, const int *, int *, size_t, size_t);
 typedef void (*f_syev)(const char *, const char *, const int *, double *, const int *, double *, double *, const int *, int *,
                        size_t, size_t);
+typedef void (*f_geqrf)(const int *, const int *, double *, const int *, double *, double *, const int *, int *);
+typedef void (*f_orgqr)(const int *, const int *, const int *, double *, const int *, const double *, double *, const int *,
+                        int *);
 static struct {
   int tried, ok;
   char path[512];
   f_lartg lartg; f_rot rot; f_steqr steqr; f_gehrd gehrd; f_orghr orghr; f_hseqr hseqr; f_trexc trexc; f_trevc trevc;
-  f_potrf potrf; f_trtri trtri; f_syev syev;   /* block orthogonalisation only */
+  f_potrf potrf; f_trtri trtri; f_syev syev; f_geqrf geqrf; f_orgqr orgqr;   /* block orthogonalisation only */
 } LP;
 
 static void *lp_sym(void *h, const char *name)
@@ -290,6 +293,7 @@ static int lp_try(const char *path)
   LP.trexc = (f_trexc)lp_sym(h, "dtrexc"); LP.trevc = (f_trevc)lp_sym(h, "dtrevc");
   LP.potrf = (f_potrf)lp_sym(h, "dpotrf"); LP.trtri = (f_trtri)lp_sym(h, "dtrtri");
   LP.syev = (f_syev)lp_sym(h, "dsyev");
+  LP.geqrf = (f_geqrf)lp_sym(h, "dgeqrf"); LP.orgqr = (f_orgqr)lp_sym(h, "dorgqr");
   if (!(LP.lartg && LP.rot && LP.steqr && LP.gehrd && LP.orghr && LP.hseqr && LP.trexc && LP.trevc)) { dlclose(h); return 0; }
   void (*setlayer)(int) = (void (*)(int))dlsym(h, "MKL_Set_Threading_Layer");
   if (setlayer) setlayer(1); /* MKL_THREADING_SEQUENTIAL */
@@ -514,7 +518,9 @@ void orc_bv_matmult(int n, const int32_t *rp, const int32_t *ci, const double *v
    the leading ones orthonormal.  method 0: BVOrthogonalize_GS (:507-548),
    column by column; 1: BVOrthogonalize_Chol (:588-603) with
    BVMatCholInv_LAPACK_Private (bvlapack.c:136-203); 2: BVOrthogonalize_SVQB
-   (:671-686) with BVMatSVQB_LAPACK_Private (bvlapack.c:261-330).  Methods 1-2
+   (:671-686) with BVMatSVQB_LAPACK_Private (bvlapack.c:261-330); 3:
+   BVOrthogonalize_TSQR (:615-631), 4: BVOrthogonalize_TSQRCHOL (:638-659),
+   both as on one process (geqrf/orgqr of the whole block).  Methods 1-4
    first run BVOrthogonalize_BlockGS (:489-502) against the leading columns.
    R (ldr >= k) receives columns l..k-1 of the factor (zeros below the
    diagonal of the triangular ones).  Returns 0, or -1 on breakdown or a
@@ -536,7 +542,7 @@ int orc_bv_orthogonalize(int64_t n, double *V, int64_t ld, int m, int l, int k, 
     free(buf);
     return 0;
   }
-  if (!lp_load() || !LP.potrf || !LP.trtri || !LP.syev) return -1;
+  if (!lp_load() || !LP.potrf || !LP.trtri || !LP.syev || !LP.geqrf || !LP.orgqr) return -1;
   double *R12 = (double *)calloc((size_t)(l > 0 ? l : 1) * nk, sizeof(double));
   double *G = (double *)malloc(sizeof(double) * nk * nk), *Rf = (double *)calloc((size_t)nk * nk, sizeof(double));
   double *S = (double *)calloc((size_t)nk * nk, sizeof(double)), *Q = (double *)calloc((size_t)k * k, sizeof(double));
@@ -548,6 +554,41 @@ int orc_bv_orthogonalize(int64_t n, double *V, int64_t ld, int m, int l, int k, 
       Q[(l + j) + (int64_t)(l + j) * k] = 1.0;
     }
     orc_bv_multinplace(n, V, ld, 0, k, l, k, Q, k);
+  }
+  if (method >= 3) {
+    /* BVOrthogonalize_TSQR / _TSQRCHOL on one process (bvorthog.c:615-659,
+       bvlapack.c:347-398,510-540 with nlevels = 0 and size = 1): geqrf of V2;
+       R is its upper triangle; Q from orgqr (tsqr) or V2 inv(R) (tsqrchol). */
+    const int nn = (int)n;
+    double *A = (double *)malloc(sizeof(double) * (size_t)n * nk), *tau = (double *)malloc(sizeof(double) * nk), q;
+    for (int j = 0; j < nk; j++) memcpy(A + (size_t)j * n, V + (int64_t)(l + j) * ld, sizeof(double) * n);
+    int lwork = -1;
+    LP.geqrf(&nn, &nk, A, &nn, tau, &q, &lwork, &info);
+    lwork = (int)q > nk ? (int)q : nk;
+    double *work = (double *)malloc(sizeof(double) * lwork);
+    LP.geqrf(&nn, &nk, A, &nn, tau, work, &lwork, &info);
+    if (info) rc = -1;
+    for (int j = 0; j < nk; j++)
+      for (int i = 0; i < nk; i++) Rf[i + (int64_t)j * nk] = i <= j ? A[i + (size_t)j * n] : 0.0;
+    if (!rc && method == 3) {
+      LP.orgqr(&nn, &nk, &nk, A, &nn, tau, work, &lwork, &info);
+      if (info) rc = -1;
+      for (int j = 0; j < nk && !rc; j++) memcpy(V + (int64_t)(l + j) * ld, A + (size_t)j * n, sizeof(double) * n);
+    } else if (!rc) {
+      memcpy(S, Rf, sizeof(double) * nk * nk);
+      LP.trtri("U", "N", &nk, S, &nk, &info, 1, 1);
+      if (info) rc = -1;
+      memset(Q, 0, sizeof(double) * k * k);
+      for (int j = 0; j < nk; j++)
+        for (int i = 0; i < nk; i++) Q[(l + i) + (int64_t)(l + j) * k] = S[i + (int64_t)j * nk];
+      if (!rc) orc_bv_multinplace(n, V, ld, l, k, l, k, Q, k);
+    }
+    free(A); free(tau); free(work);
+    if (!rc && R)
+      for (int j = 0; j < nk; j++)
+        for (int i = 0; i < k; i++) R[i + (int64_t)(l + j) * ldr] = i < l ? R12[i + (int64_t)j * l] : Rf[(i - l) + (int64_t)j * nk];
+    free(R12); free(G); free(Rf); free(S); free(Q);
+    return rc;
   }
   orc_bv_dot(n, V + (int64_t)l * ld, ld, nk, V + (int64_t)l * ld, ld, nk, G, nk);
   if (method == 1) {

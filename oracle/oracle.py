@@ -236,7 +236,8 @@ def bv_matmult(A, X):
 
 
 def bv_orthogonalize(V, l, k, method):
-    """BVOrthogonalize of columns l..k-1 (method 0 gs, 1 chol, 2 svqb).
+    """BVOrthogonalize of columns l..k-1 (method 0 gs, 1 chol, 2 svqb,
+    3 tsqr, 4 tsqrchol).
     Returns (V', R) with R k x k (columns l..k-1 filled), or raises on breakdown."""
     n, m = V.shape
     flat = np.array(V, order="F").ravel(order="F")

@@ -277,9 +277,10 @@ class BV:
         check(lib().sa_bv_matmult(self.h, A.h, Y.h))
 
     def orthogonalize(self, method="chol", with_R=True):
-        """BVOrthogonalize of the active columns with block method gs, chol or
-        svqb; returns the k x k factor R (columns l..k-1 filled) or None."""
-        m = {"gs": 0, "chol": 1, "svqb": 2}[method]
+        """BVOrthogonalize of the active columns with block method gs, chol,
+        svqb, tsqr or tsqrchol; returns the k x k factor R (columns l..k-1
+        filled) or None."""
+        m = {"gs": 0, "chol": 1, "svqb": 2, "tsqr": 3, "tsqrchol": 4}[method]
         l, k = self.get_active_columns()
         R = np.zeros((k, k), order="F") if with_R else None
         check(lib().sa_bv_orthogonalize(self.h, m, R.ctypes.data if with_R else None, k))

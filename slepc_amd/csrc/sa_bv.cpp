@@ -138,7 +138,8 @@ int block_gs(BV &V, int l, int k, std::vector<double> &R12, double *M_dev, hipSt
 }  // namespace
 
 int bv_orthogonalize(BV &V, int method, double *R, int ldr) {
-  SA_CHECK(method >= 0 && method <= 2, "unknown block orthogonalization method (0 gs, 1 chol, 2 svqb)");
+  SA_CHECK(method >= 0 && method <= 4,
+           "unknown block orthogonalization method (0 gs, 1 chol, 2 svqb, 3 tsqr, 4 tsqrchol)");
   const int l = V.l, k = V.k, nk = k - l;
   SA_CHECK(!R || ldr >= k, "R is smaller than the number of active columns");
   if (nk <= 0) return 0;
@@ -162,6 +163,30 @@ int bv_orthogonalize(BV &V, int method, double *R, int ldr) {
   SA_HIP(hipMalloc(&M_dev, sizeof(double) * (size_t)k * k));
   std::vector<double> R12, G((size_t)nk * nk);
   int rc = l ? block_gs(V, l, k, R12, M_dev, s) : 0;
+  if (method >= 3) {   // BVOrthogonalize_TSQR / _TSQRCHOL (bvorthog.c:615-659)
+    (void)hipFree(M_dev);
+    if (rc) return rc;
+    std::vector<double> Rt(G.size()), f(nk);
+    SA_TRY(launch_tsqr(V, l, k, method == 3, Rt.data(), f.data(), s));
+    if (method == 4) {   // R only (bvlapack.c:510-560), then V2 = V2 inv(R) (bvorthog.c:654-655)
+      std::vector<double> Si(Rt), Vt(G.size()), Qt(G.size(), 0.0);
+      SA_CHECK(la_trtri_upper(nk, Si.data(), nk) == 0, "BVOrthogonalize: triangular inverse (trtri) failed");
+      SA_TRY(bv_top_block(V, l, k, Vt.data()));
+      for (int j = 0; j < nk; j++)           // first rows of V2 inv(R), for the signs
+        for (int a = 0; a <= j; a++)
+          for (int i = 0; i < nk; i++) Qt[i + (size_t)j * nk] += Vt[i + (size_t)a * nk] * Si[a + (size_t)j * nk];
+      tsqr_signs(Qt.data(), Rt.data(), V.n, nk, f.data());
+      std::fill(V.hq, V.hq + (size_t)k * k, 0.0);
+      for (int j = 0; j < nk; j++)
+        for (int i = 0; i < nk; i++) V.hq[(l + i) + (size_t)(l + j) * k] = Si[i + (size_t)j * nk] * f[j];
+      SA_TRY(apply_q(V, l, k, l, k, k, s));
+    }
+    if (R)
+      for (int j = 0; j < nk; j++)
+        for (int i = 0; i < k; i++)
+          R[i + (size_t)(l + j) * ldr] = i < l ? R12[i + (size_t)j * l] : f[i - l] * Rt[(i - l) + (size_t)j * nk];
+    return 0;
+  }
   if (!rc) rc = launch_bvdot(V, l, k, V, l, k, M_dev, s);            // G = V2^T V2
   if (!rc && hipMemcpy(G.data(), M_dev, sizeof(double) * G.size(), hipMemcpyDeviceToHost) != hipSuccess) rc = 1;
   (void)hipFree(M_dev);

@@ -115,6 +115,15 @@ int launch_fill_random(int prec, void *x, int64_t n, uint64_t seed, hipStream_t 
 int launch_bvdot(const BV &X, int lx, int kx, const BV &Y, int ly, int ky, double *M_dev, hipStream_t s);
 // BVMatMult: Y(:, ly+j) = A X(:, lx+j), j < ncols (sa_block.hip)
 int launch_spmm(const Mat &A, const BV &X, int lx, const BV &Y, int ly, int ncols, hipStream_t s);
+// TSQR of V(:, l:k) (k - l <= 64, V.n >= k - l): R_host ((k-l) x (k-l)) gets
+// the R of the tile tree; with formq, V(:, l:k) is overwritten by Q times the
+// signs f (k - l entries) that make Q and diag(f) R equal geqrf/orgqr's.
+int launch_tsqr(BV &V, int l, int k, bool formq, double *R_host, double *f, hipStream_t s);
+// The signs f that turn a QR factorization (Q, R), k x k block Qtop = the
+// first k rows of Q, into the one flat Householder QR (geqrf) computes.
+void tsqr_signs(const double *Qtop, const double *R, int64_t n, int k, double *f);
+// Rows 0..k-l-1 of V(:, l:k) into out ((k-l) x (k-l), column-major, FP64).
+int bv_top_block(const BV &V, int l, int k, double *out);
 
 // ---- generators (sa_problems.cpp) ------------------------------------------
 uint64_t splitmix64(uint64_t z);

@@ -277,13 +277,13 @@ def _test11_matrix(n=20, k=8):
     return X
 
 
-@pytest.mark.parametrize("method", [0, 1, 2])
+@pytest.mark.parametrize("method", [0, 1, 2, 3, 4])
 def test_oracle_bv_orthogonalize_test11(orc, method):
     """bv/tests/test11.c (n=20, l=2, k=8; output test11_1.out / test11_6.out):
     orthogonalizing the leading columns, then the active ones, leaves the level
     of orthogonality of Q1, Q2 and Q and the residual ||X - Q R|| below
-    100 eps, for gs, chol and svqb.  For gs and chol R is also the textbook QR
-    factor (numpy's, up to the signs of its rows)."""
+    100 eps, for gs, chol, svqb, tsqr and tsqrchol.  For all but svqb R is
+    also the textbook QR factor (numpy's, up to the signs of its rows)."""
     X = _test11_matrix()
     l, k = 2, 8
     V1, R1 = orc.bv_orthogonalize(X.copy(), 0, l, method)
@@ -296,10 +296,27 @@ def test_oracle_bv_orthogonalize_test11(orc, method):
     assert np.linalg.norm(V[:, l:].T @ V[:, l:] - np.eye(k - l)) < 100 * eps
     assert np.linalg.norm(V.T @ V - np.eye(k)) < 100 * eps
     assert np.linalg.norm(X - V @ R) < 100 * eps
-    if method in (0, 1):
+    if method != 2:
         Rn = np.linalg.qr(X)[1]
         assert np.allclose(np.abs(R), np.abs(Rn), atol=1e-13)
         assert np.allclose(np.tril(R, -1), 0.0)
+
+
+@pytest.mark.parametrize("method", [3, 4])
+def test_oracle_tsqr_test11_12(orc, method):
+    """bv/tests/test11.c -n 180 -l 0 -k 7 -bv_orthog_block tsqr
+    (test11_12.out): orthogonality and ||X - Q R|| below 100 eps; on one
+    process Q and R are LAPACK's Householder QR, the same factors, signs
+    included, as numpy's QR (a separate LAPACK build: equal to rounding)."""
+    rng = np.random.default_rng(12)
+    X = rng.standard_normal((180, 7))
+    V, R = orc.bv_orthogonalize(X.copy(), 0, 7, method)
+    eps = 2.220446049250313e-16
+    assert np.linalg.norm(V.T @ V - np.eye(7)) < 100 * eps
+    assert np.linalg.norm(X - V @ R) < 100 * eps * np.linalg.norm(X)
+    Qn, Rn = np.linalg.qr(X)
+    assert np.max(np.abs(R - Rn)) < 1e-13 * np.abs(Rn).max()
+    assert np.max(np.abs(V - Qn)) < 1e-13
 
 
 def test_oracle_bv_orthogonalize_breakdown(orc):

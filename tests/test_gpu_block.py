@@ -24,7 +24,10 @@ def _bv(sa, X, prec=0):
     return V
 
 
-@pytest.mark.parametrize("method", ["gs", "chol", "svqb"])
+METHODS = {"gs": 0, "chol": 1, "svqb": 2, "tsqr": 3, "tsqrchol": 4}
+
+
+@pytest.mark.parametrize("method", list(METHODS))
 def test_orthogonalize_test11(sa, orc, method):
     """bv/tests/test11.c (n=20, l=2, k=8): level of orthogonality of Q1, Q2
     and Q and residual ||X - Q R|| below 100 eps; Q and R equal the oracle's."""
@@ -43,7 +46,7 @@ def test_orthogonalize_test11(sa, orc, method):
     assert np.linalg.norm(Q[:, l:].T @ Q[:, l:] - np.eye(k - l)) < 100 * EPS
     assert np.linalg.norm(Q.T @ Q - np.eye(k)) < 100 * EPS
     assert np.linalg.norm(X - Q @ R) < 100 * EPS
-    m = {"gs": 0, "chol": 1, "svqb": 2}[method]
+    m = METHODS[method]
     Vo, Ro1 = orc.bv_orthogonalize(X.copy(), 0, l, m)
     Vo, Ro2 = orc.bv_orthogonalize(Vo, l, k, m)
     Q, R2 = _align_signs(Q, R2, Vo) if method == "svqb" else (Q, R2)
@@ -51,7 +54,7 @@ def test_orthogonalize_test11(sa, orc, method):
     assert np.max(np.abs(R2[:, l:] - Ro2[:, l:])) < 1e-13
 
 
-@pytest.mark.parametrize("method", ["gs", "chol", "svqb"])
+@pytest.mark.parametrize("method", list(METHODS))
 def test_orthogonalize_large_matches_oracle(sa, orc, method):
     """100,001 rows, 40 columns, 8 leading ones already orthonormal (more
     64-row blocks than workgroups, BVDot split over two 32-row chunks per
@@ -64,7 +67,7 @@ def test_orthogonalize_large_matches_oracle(sa, orc, method):
     V.set_active_columns(l, k)
     R = V.orthogonalize(method)
     Q = V.get_matrix(k)
-    m = {"gs": 0, "chol": 1, "svqb": 2}[method]
+    m = METHODS[method]
     Vo, Ro = orc.bv_orthogonalize(X.copy(), l, k, m)
     Q, R = _align_signs(Q, R, Vo) if method == "svqb" else (Q, R)
     assert np.linalg.norm(Q.T @ Q - np.eye(k)) < 1e-13
@@ -73,6 +76,81 @@ def test_orthogonalize_large_matches_oracle(sa, orc, method):
     assert np.linalg.norm(X - Q @ Rf) < 1e-12 * np.linalg.norm(X)
     assert np.max(np.abs(Q - Vo)) < 1e-11
     assert np.max(np.abs(R[:, l:] - Ro[:, l:])) < 1e-11 * np.abs(Ro).max()
+
+
+@pytest.mark.parametrize("method", ["tsqr", "tsqrchol"])
+@pytest.mark.parametrize("n,k", [(180, 7), (20, 20), (256 * 40 + 3, 16), (300001, 64), (70000, 33), (257, 64)])
+def test_tsqr_shapes_match_oracle(sa, orc, method, n, k):
+    """TSQR over tile trees of every depth: bv/tests/test11.c with -n 180 -l 0
+    -k 7 (test11_12.out: orthogonality and ||X - QR|| below 100 eps), a square
+    block, a last tile with fewer rows than columns, 64 columns over 300,001
+    rows (five tree levels), 33 columns (the 64-wide kernel with padding) and
+    a two-tile tree.  Q and R equal geqrf/orgqr's (the oracle), signs
+    included."""
+    rng = np.random.default_rng(n + k)
+    X = rng.standard_normal((n, k))
+    V = _bv(sa, X)
+    V.set_active_columns(0, k)
+    R = V.orthogonalize(method)
+    Q = V.get_matrix(k)
+    Vo, Ro = orc.bv_orthogonalize(X.copy(), 0, k, METHODS[method])
+    scale = np.sqrt(n) if n > 1000 else 1.0
+    assert np.linalg.norm(Q.T @ Q - np.eye(k)) < 100 * EPS * scale
+    assert np.linalg.norm(X - Q @ R) < 100 * EPS * scale * np.linalg.norm(X)
+    assert np.array_equal(np.sign(np.diag(R)), np.sign(np.diag(Ro)))
+    assert np.max(np.abs(Q - Vo)) < 1e-11
+    assert np.max(np.abs(R - Ro)) < 1e-11 * np.abs(Ro).max()
+
+
+def test_tsqr_ill_conditioned(sa, orc):
+    """Columns with condition number 1e13: TSQR keeps Q orthonormal to
+    rounding (Householder), where a Cholesky-based QR loses about 26 digits
+    of orthogonality; Q agrees with the oracle's Householder QR."""
+    n, k = 50000, 24
+    rng = np.random.default_rng(5)
+    U = np.linalg.qr(rng.standard_normal((n, k)))[0]
+    W = np.linalg.qr(rng.standard_normal((k, k)))[0]
+    X = (U * np.logspace(0, -13, k)) @ W.T
+    V = _bv(sa, X)
+    V.set_active_columns(0, k)
+    R = V.orthogonalize("tsqr")
+    Q = V.get_matrix(k)
+    assert np.linalg.norm(Q.T @ Q - np.eye(k)) < 1e-13
+    assert np.linalg.norm(X - Q @ R) < 1e-14 * np.linalg.norm(X)
+    Vo, Ro = orc.bv_orthogonalize(X.copy(), 0, k, 3)
+    assert np.array_equal(np.sign(np.diag(R)), np.sign(np.diag(Ro)))
+
+
+def test_tsqr_fp32(sa, orc):
+    """FP32 basis: Q orthonormal to FP32 rounding and close to the oracle's
+    FP64 Householder QR of the same (FP32-representable) columns."""
+    n, k = 100001, 20
+    rng = np.random.default_rng(6)
+    X = rng.standard_normal((n, k)).astype(np.float32).astype(np.float64)
+    V = _bv(sa, X, 1)
+    V.set_active_columns(0, k)
+    R = V.orthogonalize("tsqr")
+    Q = V.get_matrix(k)
+    Vo, Ro = orc.bv_orthogonalize(X.copy(), 0, k, 3)
+    assert np.linalg.norm(Q.T @ Q - np.eye(k)) < 1e-4
+    assert np.max(np.abs(Q - Vo)) < 1e-4
+    assert np.max(np.abs(R - Ro)) < 1e-4 * np.abs(Ro).max()
+
+
+def test_tsqrchol_dependent_column_raises(sa):
+    """A zero column leaves R singular: TSQRCHOL's triangular inverse fails
+    with an error (the trtri check of BVMatTriInv_LAPACK_Private, bvlapack.c:208-245), while
+    TSQR still returns an orthonormal Q."""
+    X = _test11_matrix()
+    X[:, 3] = 0.0
+    V = _bv(sa, X)
+    with pytest.raises(RuntimeError):
+        V.orthogonalize("tsqrchol")
+    V = _bv(sa, X)
+    R = V.orthogonalize("tsqr")
+    Q = V.get_matrix(X.shape[1])
+    assert np.linalg.norm(Q.T @ Q - np.eye(X.shape[1])) < 100 * EPS
+    assert np.linalg.norm(X - Q @ R) < 100 * EPS
 
 
 def test_orthogonalize_breakdown_raises(sa):
