@@ -29,7 +29,8 @@ typedef struct sa_eps_s *sa_eps;   /* Krylov-Schur eigensolver context        */
 typedef struct sa_mlp_s *sa_mlp;   /* learned warm-start network (BF16 MFMA)  */
 
 enum { SA_FP64 = 0, SA_FP32 = 1 };
-enum { SA_LARGEST_MAGNITUDE = 1, SA_SMALLEST_MAGNITUDE = 2, SA_LARGEST_REAL = 3, SA_SMALLEST_REAL = 4 };
+enum { SA_LARGEST_MAGNITUDE = 1, SA_SMALLEST_MAGNITUDE = 2, SA_LARGEST_REAL = 3, SA_SMALLEST_REAL = 4,
+       SA_LARGEST_IMAGINARY = 5, SA_SMALLEST_IMAGINARY = 6, SA_TARGET_MAGNITUDE = 7, SA_TARGET_REAL = 8 };
 enum { SA_ERROR_ABSOLUTE = 0, SA_ERROR_RELATIVE = 1 };
 
 /* ---- library ----------------------------------------------------------- */
@@ -196,7 +197,17 @@ int sa_eps_set_problem_type(sa_eps eps, int hermitian);
 #define SA_EPS_ARNOLDI 1
 #define SA_EPS_LANCZOS 2
 int sa_eps_set_type(sa_eps eps, int type);
+/* EPSSetWhichEigenpairs (the EPSWhich values of slepceps.h; the target ones
+   compare distances to the target, slepcsc.c:240-267) and EPSSetTarget. */
 int sa_eps_set_which(sa_eps eps, int which);
+int sa_eps_set_target(sa_eps eps, double target);
+/* EPSSetExtraction: SA_EPS_RITZ (default) or SA_EPS_HARMONIC (harmonic Ritz
+   values and vectors with respect to the target: the projected matrix is
+   translated by DSTranslateHarmonic before each solve, krylovschur.c:270-271,
+   311-320, arnoldi.c:96-97; Krylov-Schur and Arnoldi). */
+#define SA_EPS_RITZ 0
+#define SA_EPS_HARMONIC 1
+int sa_eps_set_extraction(sa_eps eps, int extraction);
 int sa_eps_set_dimensions(sa_eps eps, int nev, int ncv, int mpd);
 int sa_eps_set_tolerances(sa_eps eps, double tol, int max_it);
 /* EPSSetInitialSpace (epssetup.c:613-626): nvec host vectors of length n,

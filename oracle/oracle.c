@@ -265,6 +265,9 @@ typedef void (*f_potrf)(const char *, const int *, double *, const int *, int *,
 typedef void (*f_trtri)(const char *, const char *, const int *, double *, const int *, int *, size_t, size_t);
 typedef void (*f_syev)(const char *, const char *, const int *, double *, const int *, double *, double *, const int *, int *,
                        size_t, size_t);
+typedef void (*f_getrf)(const int *, const int *, double *, const int *, int *, int *);
+typedef void (*f_getrs)(const char *, const int *, const int *, const double *, const int *, const int *, double *,
+                        const int *, int *, size_t);
 typedef void (*f_geqrf)(const int *, const int *, double *, const int *, double *, double *, const int *, int *);
 typedef void (*f_orgqr)(const int *, const int *, const int *, double *, const int *, const double *, double *, const int *,
                         int *);
@@ -273,6 +276,7 @@ static struct {
   char path[512];
   f_lartg lartg; f_rot rot; f_steqr steqr; f_gehrd gehrd; f_orghr orghr; f_hseqr hseqr; f_trexc trexc; f_trevc trevc;
   f_potrf potrf; f_trtri trtri; f_syev syev; f_geqrf geqrf; f_orgqr orgqr;   /* block orthogonalisation only */
+  f_getrf getrf; f_getrs getrs;                                               /* harmonic extraction only */
 } LP;
 
 static void *lp_sym(void *h, const char *name)
@@ -294,6 +298,7 @@ static int lp_try(const char *path)
   LP.potrf = (f_potrf)lp_sym(h, "dpotrf"); LP.trtri = (f_trtri)lp_sym(h, "dtrtri");
   LP.syev = (f_syev)lp_sym(h, "dsyev");
   LP.geqrf = (f_geqrf)lp_sym(h, "dgeqrf"); LP.orgqr = (f_orgqr)lp_sym(h, "dorgqr");
+  LP.getrf = (f_getrf)lp_sym(h, "dgetrf"); LP.getrs = (f_getrs)lp_sym(h, "dgetrs");
   if (!(LP.lartg && LP.rot && LP.steqr && LP.gehrd && LP.orghr && LP.hseqr && LP.trexc && LP.trevc)) { dlclose(h); return 0; }
   void (*setlayer)(int) = (void (*)(int))dlsym(h, "MKL_Set_Threading_Layer");
   if (setlayer) setlayer(1); /* MKL_THREADING_SEQUENTIAL */
@@ -649,8 +654,9 @@ typedef struct {
   int *perm, *iwork;
 } DSo;
 
-/* SlepcCompare* (slepcsc.c:152-200): result>0 means (ar,ai) goes after (br,bi). */
-static int sc_compare(int which, double ar, double ai, double br, double bi)
+/* SlepcCompare* (slepcsc.c:152-267): result>0 means (ar,ai) goes after (br,bi);
+   the target ones compare distances to the target. */
+static int sc_compare_t(int which, double ar, double ai, double br, double bi, double target)
 {
   double a, b;
   switch (which) {
@@ -658,7 +664,58 @@ static int sc_compare(int which, double ar, double ai, double br, double bi)
     case ORC_SM: a = hypot(ar, ai); b = hypot(br, bi); return a > b ? 1 : (a < b ? -1 : 0);
     case ORC_LR: a = ar; b = br; return a < b ? 1 : (a > b ? -1 : 0);
     case ORC_SR: a = ar; b = br; return a > b ? 1 : (a < b ? -1 : 0);
+    case ORC_LI: a = fabs(ai); b = fabs(bi); return a < b ? 1 : (a > b ? -1 : 0);
+    case ORC_SI: a = fabs(ai); b = fabs(bi); return a > b ? 1 : (a < b ? -1 : 0);
+    case ORC_TM: a = hypot(ar - target, ai); b = hypot(br - target, bi); return a > b ? 1 : (a < b ? -1 : 0);
+    case ORC_TR: a = fabs(ar - target); b = fabs(br - target); return a > b ? 1 : (a < b ? -1 : 0);
   }
+  return 0;
+}
+#define sc_compare(w, ar, ai, br, bi) sc_compare_t((w), (ar), (ai), (br), (bi), cmp_target)
+static _Thread_local double cmp_target;   /* EPSSetTarget of the solve in progress on this thread */
+
+/* DSTranslateHarmonic_NHEP (dsnhep.c:466-537).  Forward: g = (A - tau I)^-T
+   beta e_n, A(:,n-1) += beta g.  Recover (ncol = l + k): gh = -Q(:,0:ncol)^T g,
+   A(i,j) += beta gh_i Q(n-1,j) for i < ncol, l <= j < ncol; g += Q(:,0:ncol) gh;
+   extra row of columns l..ncol-1 times gamma.  gamma = sqrt(1 + ||g||^2). */
+static int ds_translate_harmonic(DSo *ds, double tau, double beta, int recover, double *g, double *gamma)
+{
+  const int ld = ds->ld, n = ds->n;
+  double *A = ds->A;
+  if (!recover) {
+    double *W = (double *)malloc(sizeof(double) * (size_t)ld * n);
+    for (int j = 0; j < n; j++) memcpy(W + (int64_t)j * ld, A + (int64_t)j * ld, sizeof(double) * n);
+    for (int i = 0; i < n; i++) { g[i] = 0.0; W[i + (int64_t)i * ld] -= tau; }
+    g[n - 1] = beta;
+    int nn = n, info = 0, one = 1;
+    LP.getrf(&nn, &nn, W, &ld, ds->iwork, &info);
+    if (!info) LP.getrs("C", &nn, &one, W, &ld, ds->iwork, g, &ld, &info, 1);
+    free(W);
+    if (info) return info;
+    for (int i = 0; i < n; i++) A[i + (int64_t)(n - 1) * ld] += g[i] * beta;
+  } else {
+    const int ncol = ds->l + ds->k;
+    const double *Q = ds->Q;
+    double *gh = (double *)malloc(sizeof(double) * (ncol > 0 ? ncol : 1));
+    for (int j = 0; j < ncol; j++) {
+      double s = 0.0;
+      for (int i = 0; i < n; i++) s += Q[i + (int64_t)j * ld] * g[i];
+      gh[j] = -s;
+    }
+    for (int i = 0; i < ncol; i++)
+      for (int j = ds->l; j < ncol; j++) A[i + (int64_t)j * ld] += gh[i] * Q[n - 1 + (int64_t)j * ld] * beta;
+    for (int i = 0; i < n; i++) {
+      double s = g[i];
+      for (int j = 0; j < ncol; j++) s += Q[i + (int64_t)j * ld] * gh[j];
+      g[i] = s;
+    }
+    free(gh);
+  }
+  double ss = 0.0;
+  for (int i = 0; i < n; i++) ss += g[i] * g[i];
+  *gamma = sqrt(1.0 + ss);
+  if (recover)
+    for (int j = ds->l; j < ds->l + ds->k; j++) A[n + (int64_t)j * ld] *= *gamma;
   return 0;
 }
 
@@ -947,6 +1004,11 @@ int orc_eps_solve(int n, const int32_t *rp, const int32_t *ci, const double *val
   int max_it = o->max_it > 0 ? o->max_it : (100 > 2 * n / ncv ? 100 : 2 * n / ncv);
   double tol = o->tol > 0 ? o->tol : 1e-8, keep = o->keep > 0 ? o->keep : 0.5;
   int herm = o->hermitian, which = o->which ? o->which : ORC_LM, lock = o->lock;
+  /* harmonic extraction (krylovschur.c:238-240; arnoldi.c:65): NHEP projected problem */
+  const int harm = o->extraction == 1;
+  if (harm) herm = 0;
+  cmp_target = o->target;
+  double *gh = (double *)calloc(ncv + 2, sizeof(double)), gamma = 1.0;
   /* EPSARNOLDI (arnoldi.c:52-131): explicit restart, always DSNHEP (:45);
      ncv <= nev + mpd (:181) */
   const int arn = o->solver == 1;
@@ -957,7 +1019,7 @@ int orc_eps_solve(int n, const int32_t *rp, const int32_t *ci, const double *val
   /* EPSLANCZOS, full reorthogonalization (lanczos.c:518-697): explicit
      restart from the most wanted unconverged Ritz vector, DSHEP */
   const int lanc = o->solver == 2;
-  if (lanc && (ncv > nev + mpd || !herm)) return -11;
+  if (lanc && (ncv > nev + mpd || !herm || harm)) return -11;
   double anorm = -1.0;
   double *bnd = (double *)calloc(ncv + 1, sizeof(double)), *yr = (double *)calloc(ncv + 1, sizeof(double));
   char *cv = (char *)calloc(ncv + 1, 1);
@@ -1014,6 +1076,7 @@ int orc_eps_solve(int n, const int32_t *rp, const int32_t *ci, const double *val
     }
     ds.n = nv; ds.t = nv; ds.l = nconv; ds.k = (arn || lanc) ? 0 : nconv + l;
     ds.state = l ? ST_RAW : ST_INTERMEDIATE;
+    if (harm && ds_translate_harmonic(&ds, o->target, beta, 0, gh, &gamma)) { rc = -70; goto done; }
     int info = herm ? ds_solve_hep(&ds, eigr, eigi) : ds_solve_nhep(&ds, eigr, eigi);
     if (info) { rc = -20 - info; goto done; }
     if (lanc) {
@@ -1085,7 +1148,7 @@ int orc_eps_solve(int n, const int32_t *rp, const int32_t *ci, const double *val
       int cplx = 0;
       if (herm) resnorm = fabs(ds.Q[nv - 1 + k * ld]);
       else if ((info = ds_vector_nhep_some(&ds, k, &resnorm, &cplx))) { rc = -50; goto done; }
-      resnorm *= beta;
+      resnorm *= beta * gamma;
       double w = hypot(re, im);
       errest[k] = (w != 0.0) ? resnorm / w : 1.79e308;
       if (marker == -1 && errest[k] >= tol) marker = k;
@@ -1129,6 +1192,17 @@ int orc_eps_solve(int n, const int32_t *rp, const int32_t *ci, const double *val
           if (bd) reason = -2;
         }
       } else {                                                /* DSTruncate(k+l,FALSE) */
+        if (harm) {              /* undo the translation, gamma u^ = u - U g~ (krylovschur.c:311-320) */
+          ds.n = nv; ds.t = nv; ds.l = k; ds.k = l;
+          ds_translate_harmonic(&ds, 0.0, beta, 1, gh, &gamma);
+          double *u = ks.V + (int64_t)nv * ks.ld;
+          for (int64_t r = 0; r < n; r++) {
+            double acc = 0.0;
+            for (int j = 0; j < nv; j++) acc += ks.V[r + (int64_t)j * ks.ld] * gh[j];
+            u[r] = (u[r] - acc) / gamma;
+          }
+          ds.n = nv; ds.t = nv; ds.l = k; ds.k = nv;
+        }
         int nn = k + l;
         if (!herm && ds.k == ds.n) {
           for (int i = ds.l; i < nn; i++) ds.A[nn + i * ld] = ds.A[ds.n + i * ld];
@@ -1195,6 +1269,7 @@ done:
 #ifdef _OPENMP
   omp_set_num_threads(saved_threads);
 #endif
+  free(gh);
   free(ks.V); free(ks.buf); free(ds.A); free(ds.Q); free(ds.X); free(ds.T); free(ds.work); free(ds.perm); free(ds.iwork);
   free(eigr); free(eigi); free(errest); free(perm);
   free(bnd); free(yr); free(cv); free(lp);

@@ -65,7 +65,7 @@ int  orc_ds_hep_compact(int n, int l, int k, const double *d, const double *e, i
                         double *wr, double *Q);
 
 /* ---- Krylov-Schur and Arnoldi solves ----------------------------------- */
-enum { ORC_LM = 1, ORC_SM = 2, ORC_LR = 3, ORC_SR = 4 };
+enum { ORC_LM = 1, ORC_SM = 2, ORC_LR = 3, ORC_SR = 4, ORC_LI = 5, ORC_SI = 6, ORC_TM = 7, ORC_TR = 8 };
 typedef struct {
   int     hermitian;   /* 1: Lanczos + DSHEP, 0: Arnoldi + DSNHEP */
   int     which;       /* ORC_LM ... */
@@ -77,6 +77,8 @@ typedef struct {
   int     nthreads;    /* OpenMP threads, 0 = default */
   int     solver;      /* 0 Krylov-Schur, 1 Arnoldi with explicit restart (arnoldi.c),
                           2 Lanczos with full reorthogonalization (lanczos.c) */
+  int     extraction;  /* 0 Ritz, 1 harmonic Ritz (EPS_HARMONIC; Krylov-Schur and Arnoldi) */
+  double  target;      /* EPSSetTarget: which 7/8 and the harmonic shift */
 } orc_opts;
 
 typedef struct {

@@ -14,14 +14,15 @@ _HERE = os.path.dirname(os.path.abspath(__file__))
 _LIB = None
 
 LM, SM, LR, SR = 1, 2, 3, 4
-WHICH = {"LM": LM, "SM": SM, "LR": LR, "SR": SR}
+WHICH = {"LM": LM, "SM": SM, "LR": LR, "SR": SR, "LI": 5, "SI": 6, "TM": 7, "TR": 8}
 
 
 class Opts(C.Structure):
     _fields_ = [("hermitian", C.c_int), ("which", C.c_int), ("nev", C.c_int),
                 ("ncv", C.c_int), ("mpd", C.c_int), ("max_it", C.c_int),
                 ("tol", C.c_double), ("keep", C.c_double), ("eta", C.c_double),
-                ("lock", C.c_int), ("nthreads", C.c_int), ("solver", C.c_int)]
+                ("lock", C.c_int), ("nthreads", C.c_int), ("solver", C.c_int),
+                ("extraction", C.c_int), ("target", C.c_double)]
 
 
 class Result(C.Structure):
@@ -259,8 +260,10 @@ def arrow_tridiag(d, e):
 
 # ---------------------------------------------------------------- solve
 def eps_solve(A, nev, ncv=0, tol=1e-8, which="LM", hermitian=True, v0=None, max_it=0,
-              mpd=0, nvec=None, nthreads=0, keep=0.5, lock=1, solver="krylovschur"):
+              mpd=0, nvec=None, nthreads=0, keep=0.5, lock=1, solver="krylovschur",
+              extraction="ritz", target=0.0):
     o = Opts(hermitian=1 if hermitian else 0, which=WHICH[which], nev=nev, ncv=ncv, mpd=mpd,
+             extraction={"ritz": 0, "harmonic": 1}[extraction], target=float(target),
              max_it=max_it, tol=tol, keep=keep, eta=0.7071, lock=lock, nthreads=nthreads,
              solver={"krylovschur": 0, "arnoldi": 1, "lanczos": 2}[solver])
     ncv_eff = ncv if ncv else min(A.n, max(2 * nev, nev + 15))

@@ -17,7 +17,8 @@ from ._lib import (FP32, FP64, LARGEST_MAGNITUDE, LARGEST_REAL, SMALLEST_MAGNITU
 __all__ = ["Mat", "BV", "EPS", "CSR", "init", "lap2d", "convdiff3d", "stencil27", "fv2d", "markov",
            "fv_theta", "rand_uniform", "FP32", "FP64", "SlepcAmdError"]
 
-WHICH = {"LM": LARGEST_MAGNITUDE, "SM": SMALLEST_MAGNITUDE, "LR": LARGEST_REAL, "SR": SMALLEST_REAL}
+WHICH = {"LM": LARGEST_MAGNITUDE, "SM": SMALLEST_MAGNITUDE, "LR": LARGEST_REAL, "SR": SMALLEST_REAL,
+         "LI": 5, "SI": 6, "TM": 7, "TR": 8}
 
 
 def init(device=0):
@@ -356,7 +357,16 @@ class EPS:
         check(lib().sa_eps_set_type(self.h, {"krylovschur": 0, "arnoldi": 1, "lanczos": 2}[eps_type]))
 
     def setWhichEigenpairs(self, which):
+        """EPSSetWhichEigenpairs: LM, SM, LR, SR, LI, SI, TM (closest to the
+        target) or TR (closest real part)."""
         check(lib().sa_eps_set_which(self.h, WHICH[which] if isinstance(which, str) else which))
+
+    def setTarget(self, target):
+        check(lib().sa_eps_set_target(self.h, float(target)))
+
+    def setExtraction(self, extraction):
+        """EPSSetExtraction: "ritz" (default) or "harmonic"."""
+        check(lib().sa_eps_set_extraction(self.h, {"ritz": 0, "harmonic": 1}[extraction]))
 
     def setDimensions(self, nev, ncv=0, mpd=0):
         check(lib().sa_eps_set_dimensions(self.h, nev, ncv, mpd))

@@ -487,8 +487,20 @@ int sa_eps_set_type(sa_eps E, int type) {
 }
 
 int sa_eps_set_which(sa_eps E, int which) {
-  SA_CHECK(E && which >= 1 && which <= 4, "invalid which");
+  SA_CHECK(E && which >= SA_LARGEST_MAGNITUDE && which <= SA_TARGET_REAL, "invalid which");
   E->p.which = which;
+  return 0;
+}
+
+int sa_eps_set_target(sa_eps E, double target) {
+  SA_CHECK(E, "null EPS");
+  E->p.target = target;
+  return 0;
+}
+
+int sa_eps_set_extraction(sa_eps E, int extraction) {
+  SA_CHECK(E && (extraction == SA_EPS_RITZ || extraction == SA_EPS_HARMONIC), "unsupported extraction type");
+  E->p.extraction = extraction;
   return 0;
 }
 

@@ -31,6 +31,9 @@ typedef void (*f_potrf)(const char *, const int *, double *, const int *, int *,
 typedef void (*f_trtri)(const char *, const char *, const int *, double *, const int *, int *, size_t, size_t);
 typedef void (*f_syev)(const char *, const char *, const int *, double *, const int *, double *, double *, const int *,
                        int *, size_t, size_t);
+typedef void (*f_getrf)(const int *, const int *, double *, const int *, int *, int *);
+typedef void (*f_getrs)(const char *, const int *, const int *, const double *, const int *, const int *, double *,
+                        const int *, int *, size_t);
 
 struct Lapack {
   bool ok = false;
@@ -46,6 +49,8 @@ struct Lapack {
   f_potrf potrf = nullptr;   // block orthogonalisation (BVOrthogonalize)
   f_trtri trtri = nullptr;
   f_syev syev = nullptr;
+  f_getrf getrf = nullptr;   // harmonic extraction (DSTranslateHarmonic)
+  f_getrs getrs = nullptr;
 };
 Lapack LA;
 std::once_flag la_once;
@@ -72,8 +77,10 @@ bool try_lib(const char *path) {
   L.potrf = (f_potrf)sym(h, "dpotrf");
   L.trtri = (f_trtri)sym(h, "dtrtri");
   L.syev = (f_syev)sym(h, "dsyev");
+  L.getrf = (f_getrf)sym(h, "dgetrf");
+  L.getrs = (f_getrs)sym(h, "dgetrs");
   if (!(L.lartg && L.rot && L.steqr && L.gehrd && L.orghr && L.hseqr && L.trexc && L.trevc && L.potrf && L.trtri &&
-        L.syev)) {
+        L.syev && L.getrf && L.getrs)) {
     dlclose(h);
     return false;
   }
@@ -140,14 +147,65 @@ int la_syev_lower(int n, double *a, int lda, double *w) {
   return info;
 }
 
-int sc_compare(int which, double ar, double ai, double br, double bi) {
+int sc_compare(int which, double ar, double ai, double br, double bi, double target) {
   double a, b;
   switch (which) {
     case 1: a = std::hypot(ar, ai); b = std::hypot(br, bi); return a < b ? 1 : (a > b ? -1 : 0);
     case 2: a = std::hypot(ar, ai); b = std::hypot(br, bi); return a > b ? 1 : (a < b ? -1 : 0);
     case 3: return ar < br ? 1 : (ar > br ? -1 : 0);
     case 4: return ar > br ? 1 : (ar < br ? -1 : 0);
+    case 5: a = std::fabs(ai); b = std::fabs(bi); return a < b ? 1 : (a > b ? -1 : 0);   // slepcsc.c:204-220
+    case 6: a = std::fabs(ai); b = std::fabs(bi); return a > b ? 1 : (a < b ? -1 : 0);   // :222-238
+    case 7: a = std::hypot(ar - target, ai); b = std::hypot(br - target, bi);            // :240-253
+      return a > b ? 1 : (a < b ? -1 : 0);
+    case 8: a = std::fabs(ar - target); b = std::fabs(br - target); return a > b ? 1 : (a < b ? -1 : 0);   // :255-267
   }
+  return 0;
+}
+
+// DSTranslateHarmonic_NHEP (dsnhep.c:466-537).  Forward (recover = false):
+// g = (A - tau I)^-T beta e_n and A(:, n-1) += beta g.  Recover: with
+// ncol = l + k, gh = -Q(:, 0:ncol)^T g, A(i, j) += beta gh_i Q(n-1, j) for
+// i < ncol, l <= j < ncol, g += Q(:, 0:ncol) gh, and the extra row of columns
+// l..ncol-1 scaled by gamma.  gamma = |(1, ||g||)| in both directions.
+int DenseProblem::translate_harmonic(double tau, double beta, bool recover, double *g, double *gamma) {
+  const int ld = ld_, n = n_;
+  double *A = A_.data();
+  if (!recover) {
+    std::vector<double> W((size_t)ld * n);
+    for (int j = 0; j < n; j++)
+      for (int i = 0; i < n; i++) W[i + (size_t)j * ld] = A[i + (size_t)j * ld];
+    for (int i = 0; i < n; i++) g[i] = 0.0;
+    g[n - 1] = beta;
+    for (int i = 0; i < n; i++) W[i + (size_t)i * ld] -= tau;
+    int nn = n, info = 0, one = 1;
+    LA.getrf(&nn, &nn, W.data(), &ld, iwork_.data(), &info);
+    if (info) return info;
+    LA.getrs("C", &nn, &one, W.data(), &ld, iwork_.data(), g, &ld, &info, 1);
+    if (info) return info;
+    for (int i = 0; i < n; i++) A[i + (size_t)(n - 1) * ld] += g[i] * beta;
+  } else {
+    const int ncol = l_ + k_;
+    const double *Q = Q_.data();
+    std::vector<double> gh(ncol);
+    for (int j = 0; j < ncol; j++) {
+      double s = 0.0;
+      for (int i = 0; i < n; i++) s += Q[i + (size_t)j * ld] * g[i];
+      gh[j] = -s;
+    }
+    for (int i = 0; i < ncol; i++)
+      for (int j = l_; j < ncol; j++) A[i + (size_t)j * ld] += gh[i] * Q[n - 1 + (size_t)j * ld] * beta;
+    for (int i = 0; i < n; i++) {
+      double s = g[i];
+      for (int j = 0; j < ncol; j++) s += Q[i + (size_t)j * ld] * gh[j];
+      g[i] = s;
+    }
+  }
+  double nrm = 0.0;
+  for (int i = 0; i < n; i++) nrm = std::hypot(nrm, g[i]);
+  *gamma = std::hypot(1.0, nrm);
+  if (recover)   // the NHEP problem of Krylov-Schur always has the extra row
+    for (int j = l_; j < l_ + k_; j++) A[n + (size_t)j * ld] *= *gamma;
   return 0;
 }
 
@@ -264,11 +322,11 @@ int DenseProblem::sort(double *wr, double *wi) {
     for (int i = l + 1; i < t_; i++) {
       const double re = d[perm[i]];
       int j = i - 1;
-      int res = sc_compare(which_, re, 0.0, d[perm[j]], 0.0);
+      int res = sc_compare(which_, re, 0.0, d[perm[j]], 0.0, target_);
       while (res < 0 && j >= l) {
         std::swap(perm[j], perm[j + 1]);
         j--;
-        if (j >= l) res = sc_compare(which_, re, 0.0, d[perm[j]], 0.0);
+        if (j >= l) res = sc_compare(which_, re, 0.0, d[perm[j]], 0.0, target_);
       }
     }
     for (int i = l; i < n; i++) wr[i] = d[perm[i]];
@@ -292,7 +350,7 @@ int DenseProblem::sort(double *wr, double *wi) {
     double re = wr[i], im = wi[i];
     int pos = 0, j = (im != 0) ? i + 2 : i + 1;
     for (; j < n; j++) {
-      if (sc_compare(which_, re, im, wr[j], wi[j]) > 0) {
+      if (sc_compare(which_, re, im, wr[j], wi[j], target_) > 0) {
         re = wr[j];
         im = wi[j];
         pos = j;

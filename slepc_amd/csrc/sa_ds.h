@@ -17,8 +17,10 @@ int la_syev_lower(int n, double *a, int lda, double *w);  // dsyev("V", "L"), as
 
 enum DsState { DS_RAW = 0, DS_INTERMEDIATE = 1, DS_CONDENSED = 2, DS_TRUNCATED = 3 };
 
-// SlepcCompare* (src/sys/slepcsc.c:152-200): >0 when (ar,ai) must follow (br,bi)
-int sc_compare(int which, double ar, double ai, double br, double bi);
+// SlepcCompare* (src/sys/slepcsc.c:152-267): >0 when (ar,ai) must follow (br,bi).
+// which as EPSWhich: 1 LM, 2 SM, 3 LR, 4 SR, 5 LI, 6 SI, 7 target magnitude,
+// 8 target real (distance to target).
+int sc_compare(int which, double ar, double ai, double br, double bi, double target = 0.0);
 
 class DenseProblem {
  public:
@@ -26,6 +28,9 @@ class DenseProblem {
   // DSSetDimensions (dsops.c:130-165)
   void set_dimensions(int n, int l, int k) { n_ = n; l_ = l; k_ = k; t_ = n; }
   void set_state(int s) { state_ = s; }
+  void set_target(double t) { target_ = t; }      // comparison context of which 7/8
+  // DSTranslateHarmonic (NHEP): g has n entries; returns LAPACK's info
+  int translate_harmonic(double tau, double beta, bool recover, double *g, double *gamma);
   int n() const { return n_; }
   int ld() const { return ld_; }
   double *d() { return T_.data(); }          // HEP diagonal
@@ -44,6 +49,7 @@ class DenseProblem {
   int trim_and_vectors(int nconv);                 // DSTruncate(n,TRUE) + DSVectors(X) (NHEP)
  private:
   int ld_ = 0, n_ = 0, l_ = 0, k_ = 0, t_ = 0, state_ = DS_RAW, which_ = 1;
+  double target_ = 0.0;
   bool herm_ = true;
   std::vector<double> A_, Q_, X_, T_, work_;
   std::vector<int> perm_, iwork_;

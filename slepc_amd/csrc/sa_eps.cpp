@@ -100,7 +100,7 @@ static int lanczos_restart(const KsParams &p, BV &V, const DenseProblem &ds, int
       breakdown = true;
     } else {
       for (int i = rs + 1; i < nv; i++)
-        if (conv[i] == 'N' && sc_compare(p.which, eigr[rs], 0.0, eigr[i], 0.0) > 0) rs = i;
+        if (conv[i] == 'N' && sc_compare(p.which, eigr[rs], 0.0, eigr[i], 0.0, p.target) > 0) rs = i;
       // BVMultColumn(V,1,0,nv,y): V(:,nv) = V(:,nconv:nv) Y(nconv:nv, rs)
       yrst.assign(Y.begin() + (size_t)rs * ld + nconv, Y.begin() + (size_t)rs * ld + nv);
       SA_HIP(hipMemcpy(V.ws.dots, yrst.data(), sizeof(double) * yrst.size(), hipMemcpyHostToDevice));
@@ -168,9 +168,17 @@ int ks_solve(const KsParams &p, const Mat &A, BV &V, bool have_initial, KsResult
   // :36-78): explicit restart from the most wanted unconverged Ritz
   // vector, DSHEP, converged pairs moved to the front and locked
   const bool lanc = p.solver == 2;
-  const bool dsherm = p.hermitian && !arn;
+  // Harmonic extraction (krylovschur.c:238-240,270-271,311-320; arnoldi.c:65,
+  // 96-97): the projected matrix is translated by g b^T (DSTranslateHarmonic)
+  // before each solve, always as a non-symmetric problem, and residual bounds
+  // are scaled by gamma; Krylov-Schur undoes the translation at the restart.
+  const bool harm = p.extraction == 1;
+  const bool dsherm = p.hermitian && !arn && !harm;
   SA_CHECK(!((arn || lanc) && ncv > nev + mpd), "The value of ncv must not be larger than nev+mpd");
   SA_CHECK(!lanc || p.hermitian, "EPSLANCZOS needs a symmetric problem");
+  SA_CHECK(!(lanc && harm), "EPSLANCZOS does not support harmonic extraction");
+  std::vector<double> g(ld, 0.0);
+  double gamma = 1.0;
   double anorm = -1.0;              // running estimate of ||A|| (Lanczos)
   std::vector<double> bnd, Yl, yrst;
   std::vector<char> conv;
@@ -178,6 +186,7 @@ int ks_solve(const KsParams &p, const Mat &A, BV &V, bool have_initial, KsResult
   hipStream_t s = stream();
   DenseProblem ds;
   ds.allocate(ld, dsherm, p.which);
+  ds.set_target(p.target);
   std::vector<double> eigr(ld, 0.0), eigi(ld, 0.0), errest(ld, 0.0), buf, scratch;
   const double t0 = now();
   double tds = 0.0;
@@ -237,6 +246,10 @@ int ks_solve(const KsParams &p, const Mat &A, BV &V, bool have_initial, KsResult
     }
     ds.set_dimensions(nv, nconv, (arn || lanc) ? 0 : k0);
     ds.set_state(l ? DS_RAW : DS_INTERMEDIATE);
+    if (harm) {
+      const int hinfo = ds.translate_harmonic(p.target, beta, false, g.data(), &gamma);
+      SA_CHECK(hinfo == 0, "LAPACK failure in the harmonic translation (getrf/getrs)");
+    }
     int info = ds.solve(eigr.data(), eigi.data());
     SA_CHECK(info == 0, "LAPACK failure in the projected problem (solve)");
     info = ds.sort(eigr.data(), eigi.data());
@@ -256,7 +269,7 @@ int ks_solve(const KsParams &p, const Mat &A, BV &V, bool have_initial, KsResult
       int cplx = 0;
       info = ds.ritz_residual(k, &resnorm, &cplx);
       SA_CHECK(info == 0, "LAPACK failure in the projected problem (vectors)");
-      resnorm *= beta;
+      resnorm *= beta * gamma;
       const double w = std::hypot(eigr[k], eigi[k]);
       errest[k] = (w != 0.0) ? resnorm / w : 1.79e308;
       if (marker == -1 && errest[k] >= p.tol) marker = k;
@@ -308,6 +321,18 @@ int ks_solve(const KsParams &p, const Mat &A, BV &V, bool have_initial, KsResult
           if (bd) reason = -2;
         }
       } else {
+        if (harm) {   // undo the translation: gamma u^ = u - U g~ (krylovschur.c:311-320)
+          ds.set_dimensions(nv, k, l);
+          const int hinfo = ds.translate_harmonic(0.0, beta, true, g.data(), &gamma);
+          SA_CHECK(hinfo == 0, "harmonic translation failed");
+          std::memcpy(V.hq, g.data(), sizeof(double) * nv);
+          SA_HIP(hipMemcpyAsync(V.dq, V.hq, sizeof(double) * nv, hipMemcpyHostToDevice, s));
+          SA_TRY(launch_multvec(V, 0, nv, -1.0, V.dq, 1.0, col_ptr(V, nv), s));
+          SA_TRY(launch_scale(V.prec, col_ptr(V, nv), V.n, 1.0 / gamma, s));
+          SA_HIP(hipStreamSynchronize(s));   // the staging buffer takes Q next
+          r.bytes_bv += ((double)nv + 4.0) * V.n * es;
+          ds.set_dimensions(nv, k, nv);
+        }
         ds.truncate(k + l);
       }
     }
@@ -363,7 +388,7 @@ int ks_solve(const KsParams &p, const Mat &A, BV &V, bool have_initial, KsResult
       im = eigi[perm[i]];
     }
     while (j < nconv) {
-      if (sc_compare(p.which, re, im, eigr[perm[j]], eigi[perm[j]]) <= 0) break;
+      if (sc_compare(p.which, re, im, eigr[perm[j]], eigi[perm[j]], p.target) <= 0) break;
       if (!im) {
         if (eigi[perm[j]] == 0.0) {
           std::swap(perm[j - 1], perm[j]);

@@ -11,7 +11,9 @@ struct KsParams {
   int which = SA_LARGEST_MAGNITUDE;
   int nev = 1, ncv = 16, mpd = 16, max_it = 100;
   double tol = 1e-8, keep = 0.5, eta = 0.7071;
-  int solver = 0;              // 0 Krylov-Schur, 1 Arnoldi with explicit restart (EPSARNOLDI)
+  int solver = 0;              // 0 Krylov-Schur, 1 Arnoldi with explicit restart (EPSARNOLDI), 2 Lanczos
+  int extraction = 0;          // 0 Ritz, 1 harmonic Ritz (EPS_HARMONIC)
+  double target = 0.0;         // EPSSetTarget: comparison point of which 7/8 and the harmonic shift
   void (*monitor)(void *, int, int, const double *, const double *, int) = nullptr;
   void *monitor_ctx = nullptr;
 };

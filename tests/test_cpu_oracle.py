@@ -419,3 +419,41 @@ def test_oracle_lanczos_stencil27(orc):
     ref = np.array(LIBRES["stencil27_10"]["largest_mag32"])
     for v in r["eigr"][:8]:
         assert np.min(np.abs(ref - v)) < 1e-12 * abs(v)
+
+
+TEST2_INTERIOR = {30: [2.10130, 1.89870, 2.30286, 2.50131], 32: [2.09516, 2.28463, 1.90484, 2.47152]}
+
+
+@pytest.mark.parametrize("solver", ["krylovschur", "arnoldi"])
+@pytest.mark.parametrize("n", [30, 32])
+def test_oracle_harmonic_test2(orc, n, solver):
+    """eps/tests/test2.c, interior eigenvalues: target 2.1, which = target
+    magnitude, harmonic extraction for krylovschur and arnoldi (test2_1.out
+    for n = 30, test2_3.out for n = 32): the printed values in the printed
+    order, and the closed form 2 - 2 cos(k pi / (n+1)) of the four
+    eigenvalues closest to 2.1."""
+    A = _lap1d(orc, n)
+    tol = max(1000 * 2.220446049250313e-16, 1e-9)
+    r = orc.eps_solve(A, 4, 0, tol, "TM", True, v0=orc.rand_uniform(0, n), solver=solver,
+                      extraction="harmonic", target=2.1)
+    assert r["nconv"] >= 4
+    assert np.allclose(r["eigr"][:4], TEST2_INTERIOR[n], atol=5e-6)
+    exact = 2 - 2 * np.cos(np.arange(1, n + 1) * np.pi / (n + 1))
+    near = exact[np.argsort(np.abs(exact - 2.1))[:4]]
+    assert np.allclose(r["eigr"][:4], near, rtol=1e-9)
+
+
+def test_oracle_harmonic_nonsymmetric(orc):
+    """Harmonic Krylov-Schur on the non-symmetric 7-point convection-diffusion
+    operator (8^3), 6 eigenvalues closest to 3.0: dense LAPACK's eigenvalues
+    closest to the target, in that order."""
+    A = orc.convdiff3d(8)
+    r = orc.eps_solve(A, 6, 24, 1e-10, "TM", False, v0=orc.rand_uniform(0, A.n), extraction="harmonic",
+                      target=3.0, max_it=500)
+    assert r["nconv"] >= 6
+    ev = np.linalg.eigvals(A.to_dense())
+    ev = ev[np.argsort(np.abs(ev - 3.0), kind="stable")]
+    got = r["eigr"][:6] + 1j * r["eigi"][:6]
+    for z in got:
+        assert np.min(np.abs(ev[:8] - z)) < 1e-8
+    assert np.all(np.diff(np.abs(got - 3.0)) >= -1e-9)
