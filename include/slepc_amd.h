@@ -208,6 +208,13 @@ int sa_eps_set_target(sa_eps eps, double target);
 #define SA_EPS_RITZ 0
 #define SA_EPS_HARMONIC 1
 int sa_eps_set_extraction(sa_eps eps, int extraction);
+/* EPSSetTwoSided: two-sided Krylov-Schur (ks-twosided.c:126-241) that also
+   computes left eigenvectors (y^T A = lambda y^T), with a left basis swept by
+   A^T (built from the operator's CSR at the first solve).  Non-symmetric
+   problems, FP64, Ritz extraction. */
+int sa_eps_set_two_sided(sa_eps eps, int twosided);
+/* EPSGetLeftEigenvector: left eigenvector i in the order of sa_eps_get_eigenpair. */
+int sa_eps_get_left_eigenvector(sa_eps eps, int i, double *yr_host, double *yi_host);
 int sa_eps_set_dimensions(sa_eps eps, int nev, int ncv, int mpd);
 int sa_eps_set_tolerances(sa_eps eps, double tol, int max_it);
 /* EPSSetInitialSpace (epssetup.c:613-626): nvec host vectors of length n,

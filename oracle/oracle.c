@@ -958,6 +958,7 @@ typedef struct {
   double *buf;                   /* m x m coefficient buffer */
   double eta;
   int64_t matvecs, passes;
+  uint64_t seed;                 /* random start vectors: seed + i (0: 0x5EED) */
 } KSo;
 
 /* EPSGetStartVector (epssolve.c:841-873) with a caller-provided vector for i==0
@@ -965,7 +966,7 @@ typedef struct {
 static int ks_start_vector(KSo *ks, int i, int nini, int *breakdown)
 {
   double *vi = ks->V + (int64_t)i * ks->ld;
-  if (i > 0 || nini == 0) orc_rand_uniform(0x5EEDULL + (uint64_t)i, ks->n, vi);
+  if (i > 0 || nini == 0) orc_rand_uniform((ks->seed ? ks->seed : 0x5EEDULL) + (uint64_t)i, ks->n, vi);
   double norm; int lindep;
   orthogonalize_gs(ks->n, ks->V, ks->ld, i, ks->buf, ks->m, ks->eta, &norm, &lindep, &ks->passes);
   if (breakdown) *breakdown = lindep;
@@ -1273,6 +1274,354 @@ done:
   free(ks.V); free(ks.buf); free(ds.A); free(ds.Q); free(ds.X); free(ds.T); free(ds.work); free(ds.perm); free(ds.iwork);
   free(eigr); free(eigi); free(errest); free(perm);
   free(bnd); free(yr); free(cv); free(lp);
+  return rc;
+}
+
+/* ======================================================================== */
+/* Two-sided Krylov-Schur (ks-twosided.c:126-241) with DSNHEPTS (dsnhepts.c)   */
+/* ======================================================================== */
+static void ds_swap_sides(DSo *ds, double **B, double **Z, double **Y)
+{
+  double *t = ds->A; ds->A = *B; *B = t;
+  t = ds->Q; ds->Q = *Z; *Z = t;
+  t = ds->X; ds->X = *Y; *Y = t;
+}
+
+/* DSSortWithPermutation_NHEP_Private (dsutil.c:177-226) */
+static int ds_sort_with_permutation(DSo *ds, int *perm, double *wr, double *wi)
+{
+  int n = ds->n, ld = ds->ld, info = 0;
+  double *T = ds->A;
+  for (int i = ds->l; i < n - 1; i++) {
+    int pos = perm[i];
+    int inc = (pos < n - 1 && T[pos + 1 + pos * ld] != 0.0) ? 2 : 1;
+    if (pos != i) {
+      if (!((T[pos + (pos - 1) * ld] == 0.0 || perm[i + 1] == pos - 1) && (T[pos + 1 + pos * ld] == 0.0 || perm[i + 1] == pos + 1)))
+        return -200;
+      int ifst = pos + 1, ilst = i + 1;
+      LP.trexc("V", &n, T, &ld, ds->Q, &ld, &ifst, &ilst, ds->work, &info, 1);
+      if (info) return info;
+      for (int j = i + 1; j < n; j++) if (perm[j] >= i && perm[j] < pos) perm[j] += inc;
+      perm[i] = i;
+      if (inc == 2) perm[i + 1] = i + 1;
+    }
+    if (inc == 2) i++;
+  }
+  for (int j = ds->l; j < n; j++) {
+    wr[j] = T[j + j * ld];
+    if (j < n - 1 && T[j + 1 + j * ld] != 0.0) {
+      wi[j] = sqrt(fabs(T[j + 1 + j * ld])) * sqrt(fabs(T[j + (j + 1) * ld]));
+      wr[j + 1] = wr[j]; wi[j + 1] = -wi[j]; j++;
+    } else wi[j] = 0.0;
+  }
+  return 0;
+}
+
+/* Hessenberg columns k0..nv-1 (and the extra row) from the coefficient buffer */
+static void ts_fill(const KSo *ks, int k0, int nv, double *H, int ld)
+{
+  int bm = ks->m;
+  for (int j = k0; j < nv - 1; j++) memcpy(H + (int64_t)j * ld, ks->buf + (int64_t)(j + 1) * bm, sizeof(double) * (j + 2));
+  memcpy(H + (int64_t)(nv - 1) * ld, ks->buf + (int64_t)nv * bm, sizeof(double) * nv);
+  if (ld > nv) H[nv + (nv - 1) * ld] = ks->buf[nv + (int64_t)nv * bm];
+}
+
+/* EPSComputeVectors_Schur, two-sided part (epsdefault.c:118-167): V = V X,
+   W = W Y, W normalised (a pair by its joint norm), second column of each
+   pair with positive imaginary part negated */
+static void ts_vectors(int n, int nconv, double *V, double *W, const double *X, const double *Y, int ld, const double *eigi)
+{
+  if (nconv <= 0) return;
+  orc_bv_multinplace(n, V, n, 0, nconv, 0, nconv, X, ld);
+  orc_bv_multinplace(n, W, n, 0, nconv, 0, nconv, Y, ld);
+  for (int q = 0; q < nconv; q++) {
+    int pair = eigi[q] != 0.0 && q + 1 < nconv;
+    double nr = orc_bv_norm2(n, W + (int64_t)q * n);
+    if (pair) nr = hypot(nr, orc_bv_norm2(n, W + (int64_t)(q + 1) * n));
+    orc_bv_scale(n, W + (int64_t)q * n, 1.0 / nr);
+    if (pair) { orc_bv_scale(n, W + (int64_t)(q + 1) * n, (eigi[q] > 0.0 ? -1.0 : 1.0) / nr); q++; }
+  }
+}
+
+int orc_eps_solve_twosided(int n, const int32_t *rp, const int32_t *ci, const double *val, const orc_opts *o,
+                           const double *v0, orc_result *res, double *lvecs)
+{
+  if (!lp_load() || !LP.getrf || !LP.getrs) return -10;
+  int nev = o->nev > 0 ? o->nev : 1;
+  int ncv = o->ncv > 0 ? o->ncv : (nev + 15 > 2 * nev ? nev + 15 : 2 * nev);
+  if (ncv > n) ncv = n;
+  int mpd = o->mpd > 0 ? o->mpd : ncv;
+  int max_it = o->max_it > 0 ? o->max_it : (100 > 2 * n / ncv ? 100 : 2 * n / ncv);
+  double tol = o->tol > 0 ? o->tol : 1e-8, keep = o->keep > 0 ? o->keep : 0.5;
+  if (o->hermitian || o->solver != 0 || o->extraction != 0) return -11;
+  const int which = o->which ? o->which : ORC_LM;
+  cmp_target = o->target;
+  const int64_t nnz = rp[n];
+  /* A^T in CSR, columns ascending */
+  int32_t *trp = (int32_t *)calloc(n + 1, sizeof(int32_t)), *tci = (int32_t *)malloc(sizeof(int32_t) * (nnz ? nnz : 1));
+  int32_t *nxt = (int32_t *)malloc(sizeof(int32_t) * (n + 1));
+  double *tva = (double *)malloc(sizeof(double) * (nnz ? nnz : 1));
+  for (int64_t q = 0; q < nnz; q++) trp[ci[q] + 1]++;
+  for (int r = 0; r < n; r++) trp[r + 1] += trp[r];
+  for (int r = 0; r <= n; r++) nxt[r] = trp[r];
+  for (int r = 0; r < n; r++)
+    for (int32_t q = rp[r]; q < rp[r + 1]; q++) { int32_t d = nxt[ci[q]]++; tci[d] = r; tva[d] = val[q]; }
+  free(nxt);
+  const int ld = ncv + 1;
+  KSo ks = {0}, kw = {0};
+  ks.n = n; ks.rp = rp; ks.ci = ci; ks.val = val; ks.m = ncv + 1; ks.ld = n; ks.eta = o->eta > 0 ? o->eta : 0.7071;
+  kw = ks;
+  kw.rp = trp; kw.ci = tci; kw.val = tva; kw.seed = 0x1EF7ULL;
+  ks.V = (double *)calloc((size_t)n * ks.m, sizeof(double)); ks.buf = (double *)calloc((size_t)ks.m * ks.m, sizeof(double));
+  kw.V = (double *)calloc((size_t)n * kw.m, sizeof(double)); kw.buf = (double *)calloc((size_t)kw.m * kw.m, sizeof(double));
+  DSo ds = {0};
+  ds.ld = ld; ds.hermitian = 0; ds.which = which;
+  ds.A = (double *)calloc((size_t)ld * ld, sizeof(double)); ds.Q = (double *)calloc((size_t)ld * ld, sizeof(double));
+  ds.X = (double *)calloc((size_t)ld * ld, sizeof(double)); ds.T = (double *)calloc((size_t)3 * ld, sizeof(double));
+  ds.work = (double *)calloc((size_t)8 * ld * ld + 16 * ld, sizeof(double));
+  ds.perm = (int *)calloc(ld, sizeof(int)); ds.iwork = (int *)calloc(ld, sizeof(int));
+  double *B = (double *)calloc((size_t)ld * ld, sizeof(double)), *Z = (double *)calloc((size_t)ld * ld, sizeof(double));
+  double *Y = (double *)calloc((size_t)ld * ld, sizeof(double));
+  double *wr2 = (double *)calloc(ld, sizeof(double)), *wi2 = (double *)calloc(ld, sizeof(double));
+  double *eigr = (double *)calloc(ld, sizeof(double)), *eigi = (double *)calloc(ld, sizeof(double));
+  double *errest = (double *)calloc(ld, sizeof(double));
+  double *M = (double *)calloc((size_t)ncv * ncv, sizeof(double)), *Mlu = (double *)malloc(sizeof(double) * (size_t)ncv * ncv);
+  double *w = (double *)calloc(ld, sizeof(double)), *T1 = (double *)calloc((size_t)ld * ld, sizeof(double));
+  double *blk = (double *)calloc((size_t)ld * ld, sizeof(double));
+  int *ipiv = (int *)calloc(ld, sizeof(int)), *perm = (int *)calloc(ld, sizeof(int));
+  int *idx = (int *)calloc(ld, sizeof(int)), *idx2 = (int *)calloc(ld, sizeof(int)), *pp = (int *)calloc(ld, sizeof(int));
+  int rc = 0, nini = 0, info = 0;
+  double t0 = now_s();
+  if (v0) {
+    memcpy(ks.V, v0, sizeof(double) * (size_t)n);
+    double norm; int lindep;
+    orthogonalize_gs(n, ks.V, ks.ld, 0, ks.buf, ks.m, ks.eta, &norm, &lindep, &ks.passes);
+    if (norm == 0.0 || lindep) { rc = -3; goto done; }
+    orc_bv_scale(n, ks.V, 1.0 / norm);
+    nini = 1;
+  }
+  if (ks_start_vector(&ks, 0, nini, NULL) || ks_start_vector(&kw, 0, 0, NULL)) { rc = -3; goto done; }
+  int l = 0, nconv = 0, its = 0, reason = 0;
+  while (reason == 0) {
+    its++;
+    int nv = nconv + mpd < ncv ? nconv + mpd : ncv, k0 = nconv + l;
+    ds.n = nv; ds.t = nv; ds.l = nconv; ds.k = k0;
+    double beta = 0.0, betat = 0.0; int bd = 0, bdt = 0;
+    ks_expand(&ks, k0, &nv, &beta, &bd);
+    int nvt = nv;
+    ks_expand(&kw, k0, &nvt, &betat, &bdt);
+    if (nvt < nv) nv = nvt;
+    ts_fill(&ks, k0, nv, ds.A, ld);
+    ts_fill(&kw, k0, nv, B, ld);
+    ds.n = nv; ds.t = nv; ds.l = nconv; ds.k = k0;
+    ds.state = l == 0 ? ST_INTERMEDIATE : ST_RAW;
+    int breakdown = bd || bdt;
+    /* BVMatProject(V,NULL,W,M) (bvglobal.c:1013-1049) */
+    if (k0 > 0 && k0 < nv) {
+      orc_bv_dot(n, ks.V + (int64_t)k0 * n, n, nv - k0, kw.V, n, k0, blk, k0);
+      for (int j = k0; j < nv; j++) for (int i = 0; i < k0; i++) M[i + j * ncv] = blk[i + (j - k0) * k0];
+    }
+    if (k0 < nv) {
+      orc_bv_dot(n, ks.V, n, nv, kw.V + (int64_t)k0 * n, n, nv - k0, blk, nv - k0);
+      for (int j = 0; j < nv; j++) for (int i = 0; i < nv - k0; i++) M[(k0 + i) + j * ncv] = blk[i + j * (nv - k0)];
+    }
+    /* EPSTwoSidedRQUpdate1 (ks-twosided.c:27-73) */
+    memcpy(Mlu, M, sizeof(double) * (size_t)ncv * ncv);
+    int one = 1;
+    LP.getrf(&nv, &nv, Mlu, &ncv, ipiv, &info);
+    if (info) { rc = -71; goto done; }
+    orc_bv_dotvec(n, nv, kw.V, n, ks.V + (int64_t)nv * n, w);
+    LP.getrs("N", &nv, &one, Mlu, &ncv, ipiv, w, &ncv, &info, 1);
+    orc_bv_multvec(n, nv, -1.0, ks.V, n, w, 1.0, ks.V + (int64_t)nv * n);
+    for (int i = 0; i < nv; i++) ds.A[(nv - 1) * ld + i] += beta * w[i];
+    orc_bv_dotvec(n, nv, ks.V, n, kw.V + (int64_t)nv * n, w);
+    LP.getrs("C", &nv, &one, Mlu, &ncv, ipiv, w, &ncv, &info, 1);
+    orc_bv_multvec(n, nv, -1.0, kw.V, n, w, 1.0, kw.V + (int64_t)nv * n);
+    for (int i = 0; i < nv; i++) B[(nv - 1) * ld + i] += betat * w[i];
+    /* DSSolve_NHEPTS, DSSort_NHEPTS, DSUpdateExtraRow_NHEPTS */
+    int st = ds.state;
+    if ((info = ds_solve_nhep(&ds, eigr, eigi))) { rc = -20; goto done; }
+    ds_swap_sides(&ds, &B, &Z, &Y);
+    ds.state = st;
+    info = ds_solve_nhep(&ds, wr2, wi2);
+    ds_swap_sides(&ds, &B, &Z, &Y);
+    if (info) { rc = -21; goto done; }
+    if ((info = ds_sort_nhep(&ds, eigr, eigi))) { rc = -40; goto done; }
+    ds_swap_sides(&ds, &B, &Z, &Y);
+    info = ds_sort_nhep(&ds, wr2, wi2);
+    if (!info) {
+      int cont = 0;
+      for (int i = 0; i < ds.n; i++) {
+        if (hypot(wr2[i] - eigr[i], wi2[i] - eigi[i]) > 1.4901161193847656e-08) { idx2[cont] = i; idx[cont++] = i; }
+        pp[i] = -1;
+      }
+      if (cont) {
+        int id = 0;
+        for (int i = 0; i < cont; i++) {
+          double t = 1.79e308, sd;
+          for (int j = 0; j < cont; j++)
+            if (idx2[j] != -1 && (sd = hypot(wr2[idx[j]] - eigr[idx[i]], wi2[idx[j]] - eigi[idx[i]])) < t) { id = j; t = sd; }
+          pp[idx[i]] = idx[id];
+          idx2[id] = -1;
+        }
+        for (int i = 0; i < ds.n; i++) if (pp[i] == -1) pp[i] = i;
+        info = ds_sort_with_permutation(&ds, pp, wr2, wi2);
+      }
+    }
+    ds_swap_sides(&ds, &B, &Z, &Y);
+    if (info) { rc = -41; goto done; }
+    ds_update_extra_row_nhep(&ds);
+    ds_swap_sides(&ds, &B, &Z, &Y);
+    ds_update_extra_row_nhep(&ds);
+    ds_swap_sides(&ds, &B, &Z, &Y);
+    ds.k = ds.n;
+    /* EPSKrylovConvergence with right and left bounds */
+    double norm = orc_bv_norm2(n, ks.V + (int64_t)nv * n), norm2 = orc_bv_norm2(n, kw.V + (int64_t)nv * n);
+    int k, marker = -1;
+    for (k = nconv; k < nv; k++) {
+      double rn, rl; int cplx = 0, cplx2 = 0;
+      if (ds_vector_nhep_some(&ds, k, &rn, &cplx)) { rc = -50; goto done; }
+      double lam = hypot(eigr[k], eigi[k]);
+      errest[k] = lam != 0.0 ? rn * beta * norm / lam : 1.79e308;
+      if (marker == -1 && errest[k] >= tol) marker = k;
+      ds_swap_sides(&ds, &B, &Z, &Y);
+      info = ds_vector_nhep_some(&ds, k, &rl, &cplx2);
+      ds_swap_sides(&ds, &B, &Z, &Y);
+      if (info) { rc = -51; goto done; }
+      double le = lam != 0.0 ? rl * betat * norm2 / lam : 1.79e308;
+      if (le > errest[k]) errest[k] = le;
+      if (marker == -1 && le >= tol) marker = k;
+      if (cplx) { errest[k + 1] = errest[k]; k++; }
+      if (marker != -1) break;
+    }
+    if (marker != -1) k = marker;
+    if (k >= nev) reason = 1;
+    else if (its >= max_it) reason = -1;
+    int newl = 0;
+    if (!(reason != 0 || breakdown || k == nv)) {
+      newl = (int)((nv - k) * keep); if (newl < 1) newl = 1;
+      if (ds.A[(k + newl) + (k + newl - 1) * ld] != 0.0 || B[(k + newl) + (k + newl - 1) * ld] != 0.0) {
+        if (k + newl < nv - 1) newl++; else newl--;
+      }
+    }
+    l = newl;
+    orc_bv_multinplace(n, ks.V, n, nconv, nv, nconv, k + l, ds.Q, ld);
+    orc_bv_multinplace(n, kw.V, n, nconv, nv, nconv, k + l, Z, ld);
+    if (reason == 0 && !breakdown) {
+      memcpy(ks.V + (int64_t)(k + l) * n, ks.V + (int64_t)nv * n, sizeof(double) * (size_t)n);
+      memcpy(kw.V + (int64_t)(k + l) * n, kw.V + (int64_t)nv * n, sizeof(double) * (size_t)n);
+    }
+    if (reason == 0) {
+      if (breakdown || k == nv) {
+        if (k < nev) {
+          int b1 = 0, b2 = 0;
+          ks_start_vector(&ks, k, nini, &b1);
+          ks_start_vector(&kw, k, 0, &b2);
+          if (b1 || b2) reason = -2;
+        }
+      } else {                                       /* DSSetDimensions(n,k,dsk); DSTruncate(k+l,FALSE) */
+        ds.l = k;
+        int nn = k + l;
+        if (ds.k == ds.n) {
+          for (int i = ds.l; i < nn; i++) { ds.A[nn + i * ld] = ds.A[ds.n + i * ld]; B[nn + i * ld] = B[ds.n + i * ld]; }
+          for (int i = ds.l; i < ds.n; i++) { ds.A[ds.n + i * ld] = 0.0; B[ds.n + i * ld] = 0.0; }
+        }
+        ds.k = nn; ds.t = ds.n; ds.n = nn; ds.state = ST_TRUNCATED;
+      }
+      /* EPSTwoSidedRQUpdate2 (ks-twosided.c:75-124) */
+      int kk = k + l;
+      KSo *side[2] = {&ks, &kw};
+      double *H[2] = {ds.A, B};
+      for (int sd = 0; sd < 2; sd++) {
+        double nrm; int lind;
+        orthogonalize_gs(n, side[sd]->V, n, kk, side[sd]->buf, side[sd]->m, side[sd]->eta, &nrm, &lind, &side[sd]->passes);
+        orc_bv_scale(n, side[sd]->V + (int64_t)kk * n, 1.0 / nrm);
+        const double *c = side[sd]->buf + (int64_t)kk * side[sd]->m;
+        for (int j = nconv; j < kk; j++) {
+          for (int i = 0; i < kk; i++) H[sd][i + j * ld] += c[i] * H[sd][kk + j * ld];
+          H[sd][kk + j * ld] *= nrm;
+        }
+      }
+      for (int j = 0; j < nv; j++)
+        for (int i = 0; i < nv; i++) {
+          double acc = 0.0;
+          for (int q = 0; q < nv; q++) acc += M[i + q * ncv] * ds.Q[q + j * ld];
+          T1[i + j * ld] = acc;
+        }
+      for (int j = 0; j < nv; j++)
+        for (int i = 0; i < nv; i++) {
+          double acc = 0.0;
+          for (int q = 0; q < nv; q++) acc += Z[q + i * ld] * T1[q + j * ld];
+          M[i + j * ncv] = acc;
+        }
+    }
+    nconv = k;
+  }
+  /* DSTruncate(nconv,TRUE); right and left eigenvectors (epsdefault.c:105-169) */
+  for (int i = ds.l; i < ds.n; i++) { ds.A[ds.n + i * ld] = 0.0; B[ds.n + i * ld] = 0.0; }
+  ds.n = nconv; ds.l = 0; ds.k = 0; ds.t = nconv; ds.state = ST_RAW;
+  if (nconv > 0) {
+    if (ds_vectors_nhep_all(&ds, 0)) { rc = -60; goto done; }
+    ds_swap_sides(&ds, &B, &Z, &Y);
+    info = ds_vectors_nhep_all(&ds, 0);
+    ds_swap_sides(&ds, &B, &Z, &Y);
+    if (info) { rc = -61; goto done; }
+  }
+  /* EPSSolve computes the vectors at the first conjugate pair it reorders
+     (epssolve.c:159-176), so that pair already counts as positive */
+  int computed = 0;
+  for (int i = 0; i < nconv - 1; i++) {
+    if (eigi[i] != 0) {
+      if (eigi[i] < 0) {
+        eigi[i] = -eigi[i]; eigi[i + 1] = -eigi[i + 1];
+        if (!computed) { ts_vectors(n, nconv, ks.V, kw.V, ds.X, Y, ld, eigi); computed = 1; }
+        orc_bv_scale(n, ks.V + (int64_t)(i + 1) * n, -1.0);
+      }
+      i++;
+    }
+  }
+  if (!computed) ts_vectors(n, nconv, ks.V, kw.V, ds.X, Y, ld, eigi);
+  /* SlepcSortEigenvalues */
+  for (int i = 0; i < nconv; i++) perm[i] = i;
+  for (int i = nconv - 1; i >= 0; i--) {
+    double re = eigr[perm[i]], im = eigi[perm[i]];
+    int j = i + 1, tmp;
+    if (im != 0) { i--; im = eigi[perm[i]]; }
+    while (j < nconv) {
+      if (sc_compare(which, re, im, eigr[perm[j]], eigi[perm[j]]) <= 0) break;
+      if (!im) {
+        if (eigi[perm[j]] == 0.0) { tmp = perm[j - 1]; perm[j - 1] = perm[j]; perm[j] = tmp; j++; }
+        else { tmp = perm[j - 1]; perm[j - 1] = perm[j]; perm[j] = perm[j + 1]; perm[j + 1] = tmp; j += 2; }
+      } else {
+        if (eigi[perm[j]] == 0.0) { tmp = perm[j - 2]; perm[j - 2] = perm[j]; perm[j] = perm[j - 1]; perm[j - 1] = tmp; j++; }
+        else {
+          tmp = perm[j - 2]; perm[j - 2] = perm[j]; perm[j] = tmp;
+          tmp = perm[j - 1]; perm[j - 1] = perm[j + 1]; perm[j + 1] = tmp; j += 2;
+        }
+      }
+    }
+  }
+  res->seconds = now_s() - t0;
+  res->nconv = nconv; res->its = its; res->reason = reason; res->ncv_used = ncv;
+  res->matvecs = ks.matvecs + kw.matvecs; res->orth_passes = ks.passes + kw.passes;
+  for (int i = 0; i < nconv; i++) {
+    if (res->eigr) res->eigr[i] = eigr[perm[i]];
+    if (res->eigi) res->eigi[i] = eigi[perm[i]];
+    if (res->errest) res->errest[i] = errest[perm[i]];
+  }
+  {
+    int nvv = res->nvec < nconv ? res->nvec : nconv;
+    for (int i = 0; i < nvv; i++) {
+      if (res->vecs) memcpy(res->vecs + (int64_t)i * n, ks.V + (int64_t)perm[i] * n, sizeof(double) * (size_t)n);
+      if (lvecs) memcpy(lvecs + (int64_t)i * n, kw.V + (int64_t)perm[i] * n, sizeof(double) * (size_t)n);
+    }
+  }
+done:
+  free(trp); free(tci); free(tva);
+  free(ks.V); free(ks.buf); free(kw.V); free(kw.buf);
+  free(ds.A); free(ds.Q); free(ds.X); free(ds.T); free(ds.work); free(ds.perm); free(ds.iwork);
+  free(B); free(Z); free(Y); free(wr2); free(wi2); free(eigr); free(eigi); free(errest); free(M); free(Mlu);
+  free(w); free(T1); free(blk); free(ipiv); free(perm); free(idx); free(idx2); free(pp);
   return rc;
 }
 

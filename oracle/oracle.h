@@ -94,6 +94,10 @@ typedef struct {
 
 int orc_eps_solve(int n, const int32_t *rowptr, const int32_t *col, const double *val,
                   const orc_opts *o, const double *v0, orc_result *r);
+/* Two-sided Krylov-Schur (ks-twosided.c:126-241): as orc_eps_solve, plus the
+   left eigenvectors (real parts) in lvecs [n * nvec], may be NULL. */
+int orc_eps_solve_twosided(int n, const int32_t *rowptr, const int32_t *col, const double *val,
+                           const orc_opts *o, const double *v0, orc_result *r, double *lvecs);
 double orc_residual_rel(int n, const int32_t *rowptr, const int32_t *col, const double *val,
                         double lr, double li, const double *xr, const double *xi);
 

@@ -76,6 +76,8 @@ def lib():
                                            C.c_void_p, C.c_int]
         L.orc_eps_solve.argtypes = [C.c_int, i32p, i32p, f64p, C.POINTER(Opts),
                                     C.c_void_p, C.POINTER(Result)]
+        L.orc_eps_solve_twosided.argtypes = [C.c_int, i32p, i32p, f64p, C.POINTER(Opts),
+                                             C.c_void_p, C.POINTER(Result), C.c_void_p]
         L.orc_residual_rel.restype = C.c_double
         L.orc_residual_rel.argtypes = [C.c_int, i32p, i32p, f64p, C.c_double, C.c_double,
                                        f64p, C.c_void_p]
@@ -261,7 +263,7 @@ def arrow_tridiag(d, e):
 # ---------------------------------------------------------------- solve
 def eps_solve(A, nev, ncv=0, tol=1e-8, which="LM", hermitian=True, v0=None, max_it=0,
               mpd=0, nvec=None, nthreads=0, keep=0.5, lock=1, solver="krylovschur",
-              extraction="ritz", target=0.0):
+              extraction="ritz", target=0.0, twosided=False):
     o = Opts(hermitian=1 if hermitian else 0, which=WHICH[which], nev=nev, ncv=ncv, mpd=mpd,
              extraction={"ritz": 0, "harmonic": 1}[extraction], target=float(target),
              max_it=max_it, tol=tol, keep=keep, eta=0.7071, lock=lock, nthreads=nthreads,
@@ -282,14 +284,22 @@ def eps_solve(A, nev, ncv=0, tol=1e-8, which="LM", hermitian=True, v0=None, max_
     if v0 is not None:
         v0 = np.ascontiguousarray(v0, np.float64)
         v0p = v0.ctypes.data_as(C.c_void_p)
-    rc = lib().orc_eps_solve(A.n, A.rowptr, A.col, A.val, C.byref(o), v0p, C.byref(r))
+    lvecs = np.zeros(A.n * max(nvec, 1)) if twosided else None
+    if twosided:
+        rc = lib().orc_eps_solve_twosided(A.n, A.rowptr, A.col, A.val, C.byref(o), v0p, C.byref(r),
+                                          lvecs.ctypes.data_as(C.c_void_p))
+    else:
+        rc = lib().orc_eps_solve(A.n, A.rowptr, A.col, A.val, C.byref(o), v0p, C.byref(r))
     if rc != 0:
         raise RuntimeError("oracle eps_solve failed rc=%d" % rc)
     nc = r.nconv
     X = vecs[:A.n * min(nc, nvec)].reshape((min(nc, nvec), A.n)).T.copy() if nvec else None
-    return dict(nconv=nc, its=r.its, reason=r.reason, eigr=eigr[:nc].copy(), eigi=eigi[:nc].copy(),
-                errest=err[:nc].copy(), vecs=X, matvecs=r.matvecs, passes=r.orth_passes,
-                seconds=r.seconds)
+    out = dict(nconv=nc, its=r.its, reason=r.reason, eigr=eigr[:nc].copy(), eigi=eigi[:nc].copy(),
+               errest=err[:nc].copy(), vecs=X, matvecs=r.matvecs, passes=r.orth_passes,
+               seconds=r.seconds)
+    if twosided:
+        out["lvecs"] = lvecs[:A.n * min(nc, nvec)].reshape((min(nc, nvec), A.n)).T.copy() if nvec else None
+    return out
 
 
 def residual_rel(A, lr, li, xr, xi=None):

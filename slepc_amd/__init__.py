@@ -364,6 +364,16 @@ class EPS:
     def setTarget(self, target):
         check(lib().sa_eps_set_target(self.h, float(target)))
 
+    def setTwoSided(self, twosided=True):
+        """EPSSetTwoSided: also compute left eigenvectors (two-sided Krylov-Schur)."""
+        check(lib().sa_eps_set_two_sided(self.h, 1 if twosided else 0))
+
+    def getLeftEigenvector(self, i):
+        """EPSGetLeftEigenvector: (yr, yi) of eigenpair i."""
+        yr, yi = np.empty(self.A.n), np.empty(self.A.n)
+        check(lib().sa_eps_get_left_eigenvector(self.h, i, yr.ctypes.data, yi.ctypes.data))
+        return yr, yi
+
     def setExtraction(self, extraction):
         """EPSSetExtraction: "ritz" (default) or "harmonic"."""
         check(lib().sa_eps_set_extraction(self.h, {"ritz": 0, "harmonic": 1}[extraction]))

@@ -78,6 +78,10 @@ struct sa_eps_s {
   bool mixed = false;
   double tol32 = 1e-5;
   BV V;                        // FP64 basis holding the solution
+  bool twosided = false;       // EPSSetTwoSided
+  sa_mat AT = nullptr;         // A^T, built at the first two-sided solve
+  const void *AT_of = nullptr; // the operator AT was built from
+  BV W;                        // left basis (two-sided)
   KsResult res, res32;
   sa_eps_stats st{};
   bool solved = false;
@@ -456,6 +460,8 @@ int sa_eps_create(sa_eps *out) {
 int sa_eps_destroy(sa_eps E) {
   if (!E) return 0;
   bv_free(E->V);
+  bv_free(E->W);
+  if (E->AT) sa_mat_destroy(E->AT);
   delete E;
   return 0;
 }
@@ -489,6 +495,13 @@ int sa_eps_set_type(sa_eps E, int type) {
 int sa_eps_set_which(sa_eps E, int which) {
   SA_CHECK(E && which >= SA_LARGEST_MAGNITUDE && which <= SA_TARGET_REAL, "invalid which");
   E->p.which = which;
+  return 0;
+}
+
+int sa_eps_set_two_sided(sa_eps E, int twosided) {
+  SA_CHECK(E, "null EPS");
+  E->twosided = twosided != 0;
+  E->solved = false;
   return 0;
 }
 
@@ -605,7 +618,40 @@ int sa_eps_solve(sa_eps E) {
   } else if (have_init) {
     SA_TRY(bv_upload_column(E->V, 0, E->init.data()));
   }
-  SA_TRY(ks_solve(p, A, E->V, have_init, E->res));
+  if (E->twosided) {
+    SA_CHECK(!E->mixed, "the two-sided solver runs in FP64 only");
+    if (!E->AT || E->AT_of != (const void *)E->A) {   // A^T as CSR (host transpose of the device arrays)
+      if (E->AT) sa_mat_destroy(E->AT);
+      E->AT = nullptr;
+      const int n = A.n;
+      const int64_t nnz = A.nnz;
+      std::vector<int32_t> rp(n + 1), ci(nnz), trp(n + 1, 0), tci(nnz);
+      std::vector<double> va(nnz), tva(nnz);
+      SA_HIP(hipMemcpy(rp.data(), A.rowptr, sizeof(int32_t) * (n + 1), hipMemcpyDeviceToHost));
+      if (nnz) {
+        SA_HIP(hipMemcpy(ci.data(), A.col, sizeof(int32_t) * nnz, hipMemcpyDeviceToHost));
+        SA_HIP(hipMemcpy(va.data(), A.val, sizeof(double) * nnz, hipMemcpyDeviceToHost));
+      }
+      for (int64_t q = 0; q < nnz; q++) trp[ci[q] + 1]++;
+      for (int r = 0; r < n; r++) trp[r + 1] += trp[r];
+      std::vector<int32_t> next(trp.begin(), trp.end() - 1);
+      for (int r = 0; r < n; r++)          // rows in order: columns of A^T ascend
+        for (int32_t q = rp[r]; q < rp[r + 1]; q++) {
+          const int32_t dst = next[ci[q]]++;
+          tci[dst] = r;
+          tva[dst] = va[q];
+        }
+      SA_TRY(sa_mat_create_csr(n, nnz, trp.data(), tci.data(), tva.data(), SA_FP64, &E->AT));
+      E->AT_of = (const void *)E->A;
+    }
+    if (E->W.m != p.ncv + 1 || E->W.n != A.n) {
+      bv_free(E->W);
+      SA_TRY(bv_alloc(E->W, A.n, p.ncv + 1, SA_FP64));
+    }
+    SA_TRY(ks_twosided(p, A, E->AT->A, E->V, E->W, have_init, E->res));
+  } else {
+    SA_TRY(ks_solve(p, A, E->V, have_init, E->res));
+  }
   E->solved = true;
   sa_eps_stats &s = E->st;
   s.matvecs = E->res.matvecs;
@@ -643,6 +689,30 @@ int sa_eps_get_converged_reason(sa_eps E, int *reason) {
   return 0;
 }
 
+}  // extern "C"
+
+// EPS_GetEigenvector (epssolve.c): real part in column k, imaginary part in
+// column k+1 for a pair (k of the sorted index i)
+static int basis_eigvec(BV &B, const KsResult &r, int i, double *xr, double *xi) {
+  const int k = r.perm[i];
+  if (r.eigi[k] == 0.0) {
+    if (xr) SA_TRY(bv_download_column(B, k, xr));
+    if (xi) std::memset(xi, 0, sizeof(double) * B.n);
+  } else if (r.eigi[k] > 0) {
+    if (xr) SA_TRY(bv_download_column(B, k, xr));
+    if (xi) SA_TRY(bv_download_column(B, k + 1, xi));
+  } else {
+    if (xr) SA_TRY(bv_download_column(B, k - 1, xr));
+    if (xi) {
+      SA_TRY(bv_download_column(B, k, xi));
+      for (int64_t q = 0; q < B.n; q++) xi[q] = -xi[q];
+    }
+  }
+  return 0;
+}
+
+extern "C" {
+
 int sa_eps_get_eigenpair(sa_eps E, int i, double *re, double *im, double *xr, double *xi) {
   API_INIT();
   SA_CHECK(E && E->solved && i >= 0 && i < E->res.nconv, "eigenpair index out of range");
@@ -650,23 +720,15 @@ int sa_eps_get_eigenpair(sa_eps E, int i, double *re, double *im, double *xr, do
   const int k = r.perm[i];
   if (re) *re = r.eigr[k];
   if (im) *im = r.eigi[k];
-  // EPSGetEigenvector: real part in column k, imaginary part in column k+1 for a pair
-  if (xr || xi) {
-    if (r.eigi[k] == 0.0) {
-      if (xr) SA_TRY(bv_download_column(E->V, k, xr));
-      if (xi) std::memset(xi, 0, sizeof(double) * E->V.n);
-    } else if (r.eigi[k] > 0) {
-      if (xr) SA_TRY(bv_download_column(E->V, k, xr));
-      if (xi) SA_TRY(bv_download_column(E->V, k + 1, xi));
-    } else {
-      if (xr) SA_TRY(bv_download_column(E->V, k - 1, xr));
-      if (xi) {
-        SA_TRY(bv_download_column(E->V, k, xi));
-        for (int64_t q = 0; q < E->V.n; q++) xi[q] = -xi[q];
-      }
-    }
-  }
+  if (xr || xi) SA_TRY(basis_eigvec(E->V, r, i, xr, xi));
   return 0;
+}
+
+int sa_eps_get_left_eigenvector(sa_eps E, int i, double *yr, double *yi) {
+  API_INIT();
+  SA_CHECK(E && E->solved && i >= 0 && i < E->res.nconv, "eigenpair index out of range");
+  SA_CHECK(E->twosided, "Must request left vectors with sa_eps_set_two_sided");
+  return basis_eigvec(E->W, E->res, i, yr, yi);
 }
 
 int sa_eps_get_error_estimate(sa_eps E, int i, double *errest) {
@@ -720,6 +782,34 @@ int sa_eps_compute_error(sa_eps E, int i, int type, double *error) {
     }
     if (dy) (void)hipFree(dy);
     res = std::hypot(std::sqrt(nr), std::sqrt(ni));
+  }
+  // two-sided: the left residual ||A^T y - conj(lambda) y|| too, and the
+  // larger of the two (epssolve.c:777-782, EPSComputeResidualNorm_Private
+  // with trans = TRUE)
+  if (!rc && E->twosided) {
+    std::vector<double> yr(V.n), yi(V.n), u(V.n);
+    rc = basis_eigvec(E->W, r, i, yr.data(), ki != 0.0 ? yi.data() : nullptr);
+    double *dy = nullptr;
+    if (!rc && hipMalloc((void **)&dy, sizeof(double) * V.n) != hipSuccess) rc = 1;
+    const double eps = 2.220446049250313e-16;
+    double nr = 0.0, ni = 0.0;
+    if (!rc) rc = sa_memcpy_h2d(tmp, yr.data(), sizeof(double) * V.n);
+    if (!rc) rc = launch_spmv(E->AT->A, tmp, dy, s);
+    if (!rc) rc = sa_memcpy_d2h(u.data(), dy, sizeof(double) * V.n);
+    if (!rc) {
+      if (ki == 0.0) {
+        const double a = std::fabs(kr) > eps ? kr : 0.0;
+        for (int64_t q = 0; q < V.n; q++) { const double t = u[q] - a * yr[q]; nr += t * t; }
+      } else {
+        for (int64_t q = 0; q < V.n; q++) { const double t = u[q] - kr * yr[q] - ki * yi[q]; nr += t * t; }
+        rc = sa_memcpy_h2d(tmp, yi.data(), sizeof(double) * V.n);
+        if (!rc) rc = launch_spmv(E->AT->A, tmp, dy, s);
+        if (!rc) rc = sa_memcpy_d2h(u.data(), dy, sizeof(double) * V.n);
+        for (int64_t q = 0; q < V.n; q++) { const double t = u[q] - kr * yi[q] + ki * yr[q]; ni += t * t; }
+      }
+    }
+    if (dy) (void)hipFree(dy);
+    res = std::max(res, std::hypot(std::sqrt(nr), std::sqrt(ni)));
   }
   (void)hipFree(tmp);
   (void)hipFree(out);

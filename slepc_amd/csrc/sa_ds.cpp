@@ -135,6 +135,21 @@ int la_trtri_upper(int n, double *a, int lda) {
   return info;
 }
 
+int la_getrf(int n, double *a, int lda, int *ipiv) {
+  if (!lapack_ready()) return -1000;
+  int info = 0;
+  LA.getrf(&n, &n, a, &lda, ipiv, &info);
+  return info;
+}
+
+int la_getrs(char trans, int n, const double *a, int lda, const int *ipiv, double *b) {
+  if (!lapack_ready()) return -1000;
+  int info = 0, one = 1;
+  const char t[2] = {trans, 0};
+  LA.getrs(t, &n, &one, a, &lda, ipiv, b, &lda, &info, 1);
+  return info;
+}
+
 int la_syev_lower(int n, double *a, int lda, double *w) {
   if (!lapack_ready()) return -1000;
   int info = 0, lwork = -1;
@@ -488,6 +503,161 @@ int DenseProblem::trim_and_vectors(int nconv) {
     if (cp) i++;
   }
   return 0;
+}
+
+
+// ---- DSNHEPTS (dsnhepts.c) -------------------------------------------------
+void DenseProblem::enable_two_sided() {
+  B_.assign((size_t)ld_ * ld_, 0.0);
+  Z_.assign((size_t)ld_ * ld_, 0.0);
+  Y_.assign((size_t)ld_ * ld_, 0.0);
+  wr2_.assign(ld_, 0.0);
+  wi2_.assign(ld_, 0.0);
+}
+
+void DenseProblem::swap_sides() {
+  std::swap(A_, B_);
+  std::swap(Q_, Z_);
+  std::swap(X_, Y_);
+}
+
+// DSSolve_NHEPTS (dsnhepts.c:275-287): both matrices from the same state
+int DenseProblem::solve_ts(double *wr, double *wi) {
+  const int st = state_;
+  int info = solve(wr, wi);
+  if (info) return info;
+  swap_sides();
+  state_ = st;
+  info = solve(wr2_.data(), wi2_.data());
+  swap_sides();
+  return info;
+}
+
+// DSSortWithPermutation_NHEP_Private (dsutil.c:177-226) on (A, Q)
+int DenseProblem::sort_with_permutation(int *perm, double *wr, double *wi) {
+  const int ld = ld_, n = n_;
+  double *T = A_.data();
+  int nn = n, info = 0;
+  for (int i = l_; i < n - 1; i++) {
+    const int pos = perm[i];
+    const int inc = (pos < n - 1 && T[pos + 1 + (size_t)pos * ld] != 0.0) ? 2 : 1;
+    if (pos != i) {
+      if (!((T[pos + (size_t)(pos - 1) * ld] == 0.0 || perm[i + 1] == pos - 1) &&
+            (T[pos + 1 + (size_t)pos * ld] == 0.0 || perm[i + 1] == pos + 1)))
+        return -200;   // a 2x2 block would be split
+      int ifst = pos + 1, ilst = i + 1;
+      LA.trexc("V", &nn, T, &ld, Q_.data(), &ld, &ifst, &ilst, work_.data(), &info, 1);
+      if (info) return info;
+      for (int j = i + 1; j < n; j++)
+        if (perm[j] >= i && perm[j] < pos) perm[j] += inc;
+      perm[i] = i;
+      if (inc == 2) perm[i + 1] = i + 1;
+    }
+    if (inc == 2) i++;
+  }
+  for (int j = l_; j < n; j++) {
+    wr[j] = T[j + (size_t)j * ld];
+    if (j < n - 1 && T[j + 1 + (size_t)j * ld] != 0.0) {
+      wi[j] = std::sqrt(std::fabs(T[j + 1 + (size_t)j * ld])) * std::sqrt(std::fabs(T[j + (size_t)(j + 1) * ld]));
+      wr[j + 1] = wr[j];
+      wi[j + 1] = -wi[j];
+      j++;
+    } else {
+      wi[j] = 0.0;
+    }
+  }
+  return 0;
+}
+
+// DSSort_NHEPTS (dsnhepts.c:193-242): both Schur forms sorted by the
+// criterion, then B's eigenvalues matched to A's where they differ by more
+// than sqrt(eps) and B reordered to follow A.
+int DenseProblem::sort_ts(double *wr, double *wi) {
+  int info = sort(wr, wi);
+  if (info) return info;
+  swap_sides();
+  info = sort(wr2_.data(), wi2_.data());
+  if (info) { swap_sides(); return info; }
+  const int n = n_;
+  std::vector<int> idx(n), idx2(n), p(n);
+  int cont = 0;
+  const double seps = std::sqrt(2.220446049250313e-16);
+  for (int i = 0; i < n; i++) {
+    if (std::hypot(wr2_[i] - wr[i], wi2_[i] - wi[i]) > seps) {
+      idx2[cont] = i;
+      idx[cont++] = i;
+    }
+    p[i] = -1;
+  }
+  if (cont) {
+    int id = 0;
+    for (int i = 0; i < cont; i++) {
+      double t = 1.79e308, sd;
+      for (int j = 0; j < cont; j++)
+        if (idx2[j] != -1 && (sd = std::hypot(wr2_[idx[j]] - wr[idx[i]], wi2_[idx[j]] - wi[idx[i]])) < t) {
+          id = j;
+          t = sd;
+        }
+      p[idx[i]] = idx[id];
+      idx2[id] = -1;
+    }
+    for (int i = 0; i < n; i++)
+      if (p[i] == -1) p[i] = i;
+    info = sort_with_permutation(p.data(), wr2_.data(), wi2_.data());
+  }
+  swap_sides();
+  return info;
+}
+
+void DenseProblem::update_extra_row_ts() {   // dsnhepts.c:244-273
+  update_extra_row();
+  swap_sides();
+  update_extra_row();
+  swap_sides();
+}
+
+int DenseProblem::left_residual(int i, double *rnorm, int *cplx) {   // dsnhepts.c:54-119, left
+  swap_sides();
+  const int info = ritz_residual(i, rnorm, cplx);
+  swap_sides();
+  return info;
+}
+
+int DenseProblem::truncate_size_ts(int k, int nv, int l) const {   // dsnhepts.c:353-371
+  const int ld = ld_;
+  if (A_[k + l + (size_t)(k + l - 1) * ld] != 0.0 || B_[k + l + (size_t)(k + l - 1) * ld] != 0.0) {
+    if (k + l < nv - 1) l++;
+    else l--;
+  }
+  return l;
+}
+
+void DenseProblem::truncate_ts(int n) {   // dsnhepts.c:373-406, trim = FALSE
+  if (k_ == n_) {
+    for (int i = l_; i < n; i++) {
+      A_[n + (size_t)i * ld_] = A_[n_ + (size_t)i * ld_];
+      B_[n + (size_t)i * ld_] = B_[n_ + (size_t)i * ld_];
+    }
+    for (int i = l_; i < n_; i++) {
+      A_[n_ + (size_t)i * ld_] = 0.0;
+      B_[n_ + (size_t)i * ld_] = 0.0;
+    }
+  }
+  k_ = n;
+  t_ = n_;
+  n_ = n;
+  state_ = DS_TRUNCATED;
+}
+
+int DenseProblem::trim_and_vectors_ts(int nconv) {
+  const int n = n_, l = l_, k = k_, t = t_;
+  int info = trim_and_vectors(nconv);
+  if (info) return info;
+  n_ = n; l_ = l; k_ = k; t_ = t;
+  swap_sides();
+  info = trim_and_vectors(nconv);
+  swap_sides();
+  return info;
 }
 
 }  // namespace sa

@@ -14,6 +14,8 @@ bool lapack_ready(std::string *path = nullptr);
 int la_potrf_upper(int n, double *a, int lda);          // dpotrf("U")
 int la_trtri_upper(int n, double *a, int lda);          // dtrtri("U", "N")
 int la_syev_lower(int n, double *a, int lda, double *w);  // dsyev("V", "L"), ascending w
+int la_getrf(int n, double *a, int lda, int *ipiv);       // dgetrf
+int la_getrs(char trans, int n, const double *a, int lda, const int *ipiv, double *b);   // dgetrs, one rhs
 
 enum DsState { DS_RAW = 0, DS_INTERMEDIATE = 1, DS_CONDENSED = 2, DS_TRUNCATED = 3 };
 
@@ -32,6 +34,7 @@ class DenseProblem {
   // DSTranslateHarmonic (NHEP): g has n entries; returns LAPACK's info
   int translate_harmonic(double tau, double beta, bool recover, double *g, double *gamma);
   int n() const { return n_; }
+  int k() const { return k_; }
   int ld() const { return ld_; }
   double *d() { return T_.data(); }          // HEP diagonal
   double *e() { return T_.data() + ld_; }    // HEP off-diagonal / arrow
@@ -47,11 +50,28 @@ class DenseProblem {
   int truncate_size(int k, int nv, int l) const;   // DSGetTruncateSize_Default
   void truncate(int n);                            // DSTruncate(n,FALSE)
   int trim_and_vectors(int nconv);                 // DSTruncate(n,TRUE) + DSVectors(X) (NHEP)
+
+  // Two-sided problem (DSNHEPTS, dsnhepts.c): a second NHEP matrix B (the
+  // projection for the left space) with Schur vectors Z and eigenvectors Y.
+  void enable_two_sided();
+  double *B() { return B_.data(); }
+  const double *Z() const { return Z_.data(); }
+  const double *Y() const { return Y_.data(); }
+  int solve_ts(double *wr, double *wi);            // DSSolve_NHEPTS
+  int sort_ts(double *wr, double *wi);             // DSSort_NHEPTS: B follows A's order
+  void update_extra_row_ts();                      // DSUpdateExtraRow_NHEPTS
+  int left_residual(int i, double *rnorm, int *cplx);   // DSVectors(Y, i)
+  int truncate_size_ts(int k, int nv, int l) const;    // DSGetTruncateSize_NHEPTS
+  void truncate_ts(int n);                             // DSTruncate_NHEPTS(n,FALSE)
+  int trim_and_vectors_ts(int nconv);                  // DSTruncate(n,TRUE) + X and Y
  private:
+  void swap_sides();   // exchange (A, Q, X) with (B, Z, Y)
+  int sort_with_permutation(int *perm, double *wr, double *wi);   // on (A, Q)
   int ld_ = 0, n_ = 0, l_ = 0, k_ = 0, t_ = 0, state_ = DS_RAW, which_ = 1;
   double target_ = 0.0;
   bool herm_ = true;
   std::vector<double> A_, Q_, X_, T_, work_;
+  std::vector<double> B_, Z_, Y_, wr2_, wi2_;     // two-sided only
   std::vector<int> perm_, iwork_;
 };
 

@@ -57,10 +57,11 @@ int bv_download_column(BV &V, int j, double *x) {
 
 // EPSGetStartVector (epssolve.c:841-873): column i is a deterministic random
 // vector unless an initial vector is already in place (i == 0 && nini > 0).
-static int start_vector(BV &V, int i, bool have_initial, double eta, int *breakdown, std::vector<double> &scratch) {
+int eps_start_vector(BV &V, int i, bool have_initial, double eta, int *breakdown, std::vector<double> &scratch,
+                     uint64_t seed) {
   if (i > 0 || !have_initial) {
     scratch.resize(V.n);
-    sa_rand_uniform(0x5EEDULL + (uint64_t)i, V.n, scratch.data());
+    sa_rand_uniform(seed + (uint64_t)i, V.n, scratch.data());
     SA_TRY(bv_upload_column(V, i, scratch.data()));
   }
   double norm = 0.0;
@@ -149,7 +150,7 @@ static int lanczos_restart(const KsParams &p, BV &V, const DenseProblem &ds, int
     r.bytes_bv += 2.0 * V.n * es;
     if (breakdown) {
       int bd = 0;
-      SA_TRY(start_vector(V, nconv, false, p.eta, &bd, scratch));
+      SA_TRY(eps_start_vector(V, nconv, false, p.eta, &bd, scratch));
       if (bd) reason = -2;
     }
   }
@@ -201,7 +202,7 @@ int ks_solve(const KsParams &p, const Mat &A, BV &V, bool have_initial, KsResult
     SA_CHECK(!(norm == 0.0 || lindep), "initial vector is linearly dependent");
     r.bytes_bv += 3.0 * V.n * es;
   }
-  SA_TRY(start_vector(V, 0, have_initial, p.eta, nullptr, scratch));
+  SA_TRY(eps_start_vector(V, 0, have_initial, p.eta, nullptr, scratch));
   r.bytes_bv += 3.0 * V.n * es;
 
   int l = 0, nconv = 0, its = 0, reason = 0;
@@ -296,7 +297,7 @@ int ks_solve(const KsParams &p, const Mat &A, BV &V, bool have_initial, KsResult
       else if (its >= p.max_it) reason = -1;
       if (reason == 0 && breakdown) {    // arnoldi.c:118-125
         int bd = 0;
-        SA_TRY(start_vector(V, k, false, p.eta, &bd, scratch));
+        SA_TRY(eps_start_vector(V, k, false, p.eta, &bd, scratch));
         if (bd) reason = -2;
       }
       nconv = k;
@@ -317,7 +318,7 @@ int ks_solve(const KsParams &p, const Mat &A, BV &V, bool have_initial, KsResult
       if (breakdown || k == nv) {
         if (k < nev) {
           int bd = 0;
-          SA_TRY(start_vector(V, k, false, p.eta, &bd, scratch));
+          SA_TRY(eps_start_vector(V, k, false, p.eta, &bd, scratch));
           if (bd) reason = -2;
         }
       } else {

@@ -35,4 +35,16 @@ int bv_download_column(BV &V, int j, double *x);
 // initial vector (EPSSetInitialSpace).
 int ks_solve(const KsParams &p, const Mat &A, BV &V, bool have_initial, KsResult &r);
 
+// EPSGetStartVector (epssolve.c:841-873): column i of V from the initial
+// vector (i == 0 and have_initial) or a uniform random vector with seed
+// seed + i, orthonormalised against columns 0..i-1.
+constexpr uint64_t EPS_SEED_RIGHT = 0x5EEDULL, EPS_SEED_LEFT = 0x1EF7ULL;
+int eps_start_vector(BV &V, int i, bool have_initial, double eta, int *breakdown, std::vector<double> &scratch,
+                     uint64_t seed = EPS_SEED_RIGHT);
+
+// Two-sided Krylov-Schur (ks-twosided.c:126-241): right basis V of A and left
+// basis W of A^T (AT), DSNHEPTS projected problem; on return the first nconv
+// columns of V and W hold right and left eigenvectors.
+int ks_twosided(const KsParams &p, const Mat &A, const Mat &AT, BV &V, BV &W, bool have_initial, KsResult &r);
+
 }  // namespace sa

@@ -457,3 +457,32 @@ def test_oracle_harmonic_nonsymmetric(orc):
     for z in got:
         assert np.min(np.abs(ev[:8] - z)) < 1e-8
     assert np.all(np.diff(np.abs(got - 3.0)) >= -1e-9)
+
+
+@pytest.mark.parametrize("case", ["markov15", "convdiff8"])
+def test_oracle_twosided(orc, case):
+    """Two-sided Krylov-Schur (ks-twosided.c): eigenvalues equal the one-sided
+    solve (ex5_1.out holds the same values for -eps_two_sided 0 and 1 on the
+    Markov model); left eigenvectors satisfy y^T A = lambda y^T to the
+    tolerance, have unit norm and are biorthogonal to the right ones (Y^T X
+    diagonal)."""
+    if case == "markov15":
+        A, nev, ncv, tol, which = orc.markov(15), 4, 0, 1e-8, "LR"
+    else:
+        A, nev, ncv, tol, which = orc.convdiff3d(8), 6, 24, 1e-10, "LR"
+    v0 = orc.rand_uniform(0, A.n)
+    r = orc.eps_solve(A, nev, ncv, tol, which, False, v0=v0, twosided=True, nvec=nev)
+    r1 = orc.eps_solve(A, nev, ncv, tol, which, False, v0=v0)
+    assert r["nconv"] >= nev
+    assert np.allclose(r["eigr"][:nev], r1["eigr"][:nev], rtol=1e-9)
+    if case == "markov15":
+        assert list(np.round(r["eigr"][:nev], 5)) == [1.0, 0.97137, 0.90423, 0.85714]
+    D = A.to_dense()
+    X, Y = r["vecs"][:, :nev], r["lvecs"][:, :nev]
+    for i in range(nev):
+        lam = r["eigr"][i]
+        assert abs(np.linalg.norm(Y[:, i]) - 1.0) < 1e-12
+        assert np.linalg.norm(D.T @ Y[:, i] - lam * Y[:, i]) < 10 * tol * abs(lam)
+    G = Y.T @ X
+    assert np.max(np.abs(G - np.diag(np.diag(G)))) < 1e-6
+    assert np.min(np.abs(np.diag(G))) > 1e-3

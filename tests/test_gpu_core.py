@@ -469,3 +469,43 @@ def test_harmonic_matches_oracle(sa, orc, solver, case):
     # different host thread counts), so only values and residuals are compared
     for i in range(nev):
         assert E.computeError(i) < 10 * tol
+
+
+@pytest.mark.parametrize("case", ["markov15", "convdiff8", "convdiff12"])
+def test_twosided_matches_oracle(sa, orc, case):
+    """Two-sided Krylov-Schur through the C ABI against the oracle from the
+    same start vectors: eigenvalues (ex5_1.out on the Markov model), restart
+    count, right and left eigenvectors (up to sign), and EPSComputeError,
+    which takes the larger of the right and left residuals."""
+    if case == "markov15":
+        Ao, nev, ncv, tol = orc.markov(15), 4, 0, 1e-8
+    elif case == "convdiff8":
+        Ao, nev, ncv, tol = orc.convdiff3d(8), 6, 24, 1e-10
+    else:
+        Ao, nev, ncv, tol = orc.convdiff3d(12), 8, 32, 1e-10
+    Ah = sa.CSR(Ao.n, Ao.rowptr, Ao.col, Ao.val)
+    v0 = orc.rand_uniform(0, Ao.n)
+    E = sa.EPS()
+    E.setOperators(sa.Mat(Ah))
+    E.setProblemType(False)
+    E.setWhichEigenpairs("LR")
+    E.setDimensions(nev, ncv)
+    E.setTolerances(tol, 0)
+    E.setInitialSpace(v0)
+    E.setTwoSided(True)
+    E.solve()
+    r = orc.eps_solve(Ao, nev, ncv, tol, "LR", False, v0=v0, twosided=True, nvec=nev)
+    assert E.getConverged() >= nev and r["nconv"] >= nev
+    vals = np.array([E.getEigenpair(i, False)[0] for i in range(nev)])
+    assert np.max(np.abs(vals - r["eigr"][:nev]) / np.abs(r["eigr"][:nev])) < 1e-9
+    if case == "markov15":
+        assert list(np.round(vals, 5)) == [1.0, 0.97137, 0.90423, 0.85714]
+    if case != "convdiff12":
+        assert abs(E.getIterationNumber() - r["its"]) <= 1
+    for i in range(nev):
+        x = E.getEigenpair(i)[2]
+        y = E.getLeftEigenvector(i)[0]
+        xo, yo = r["vecs"][:, i], r["lvecs"][:, i]
+        assert min(np.abs(x - xo).max(), np.abs(x + xo).max()) < 1e-6
+        assert min(np.abs(y - yo).max(), np.abs(y + yo).max()) < 1e-6
+        assert E.computeError(i) < 10 * tol
