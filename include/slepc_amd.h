@@ -208,6 +208,21 @@ int sa_eps_set_target(sa_eps eps, double target);
 #define SA_EPS_RITZ 0
 #define SA_EPS_HARMONIC 1
 int sa_eps_set_extraction(sa_eps eps, int extraction);
+/* STSetType + STSetShift: SA_ST_SHIFT (sigma = 0, the default) or
+   SA_ST_SINVERT (shift-and-invert, src/sys/classes/st/impls/sinvert: the
+   operator is (A - sigma I)^-1, eigenvalues mapped back by sigma + 1/theta;
+   needs which SA_TARGET_MAGNITUDE or SA_TARGET_REAL).  Without shift_given
+   sigma is the target (STSetDefaultShift).  The solves run on the device by
+   BiCGStab (sa_eps_set_st_ksp; the reference's default is a sparse direct
+   LU, PREONLY + LU). */
+#define SA_ST_SHIFT 0
+#define SA_ST_SINVERT 1
+int sa_eps_set_st(sa_eps eps, int st_type, int shift_given, double shift);
+/* KSP of the spectral transformation: BiCGStab with SA_PC_JACOBI (default)
+   or SA_PC_NONE; rtol relative to ||b|| (default 1e-12), max_it (10000). */
+#define SA_PC_NONE 0
+#define SA_PC_JACOBI 1
+int sa_eps_set_st_ksp(sa_eps eps, int pc_type, double rtol, int max_it);
 /* EPSSetTwoSided: two-sided Krylov-Schur (ks-twosided.c:126-241) that also
    computes left eigenvectors (y^T A = lambda y^T), with a left basis swept by
    A^T (built from the operator's CSR at the first solve).  Non-symmetric

@@ -673,6 +673,22 @@ static int sc_compare_t(int which, double ar, double ai, double br, double bi, d
 }
 #define sc_compare(w, ar, ai, br, bi) sc_compare_t((w), (ar), (ai), (br), (bi), cmp_target)
 static _Thread_local double cmp_target;   /* EPSSetTarget of the solve in progress on this thread */
+static _Thread_local int cmp_sinv;        /* shift-and-invert: the DS compares back-transformed values */
+static _Thread_local double cmp_sigma;
+
+/* STBackTransform_Sinvert (sinvert.c:16-38) */
+static void sinv_map(double *re, double *im)
+{
+  if (*im == 0.0) *re = 1.0 / *re + cmp_sigma;
+  else { double t = *re * *re + *im * *im; *re = *re / t + cmp_sigma; *im = -*im / t; }
+}
+
+/* SlepcSCCompare of the DS (slepcsc.c:40-63): the ST map, then the criterion */
+static int ds_compare(int w, double ar, double ai, double br, double bi)
+{
+  if (cmp_sinv) { sinv_map(&ar, &ai); sinv_map(&br, &bi); }
+  return sc_compare_t(w, ar, ai, br, bi, cmp_target);
+}
 
 /* DSTranslateHarmonic_NHEP (dsnhep.c:466-537).  Forward: g = (A - tau I)^-T
    beta e_n, A(:,n-1) += beta g.  Recover (ncol = l + k): gh = -Q(:,0:ncol)^T g,
@@ -786,10 +802,10 @@ static void ds_sort_hep(DSo *ds, double *wr)
   for (int i = l + 1; i < ds->t; i++) {
     double re = d[perm[i]];
     int j = i - 1;
-    int res = sc_compare(ds->which, re, 0.0, d[perm[j]], 0.0);
+    int res = ds_compare(ds->which, re, 0.0, d[perm[j]], 0.0);
     while (res < 0 && j >= l) {
       int tmp = perm[j]; perm[j] = perm[j + 1]; perm[j + 1] = tmp; j--;
-      if (j >= l) res = sc_compare(ds->which, re, 0.0, d[perm[j]], 0.0);
+      if (j >= l) res = ds_compare(ds->which, re, 0.0, d[perm[j]], 0.0);
     }
   }
   for (int i = l; i < n; i++) wr[i] = d[perm[i]];
@@ -854,7 +870,7 @@ static int ds_sort_nhep(DSo *ds, double *wr, double *wi)
     int pos = 0, j = i + 1;
     if (im != 0) j = i + 2;
     for (; j < n; j++) {
-      if (sc_compare(ds->which, re, im, wr[j], wi[j]) > 0) { re = wr[j]; im = wi[j]; pos = j; }
+      if (ds_compare(ds->which, re, im, wr[j], wi[j]) > 0) { re = wr[j]; im = wi[j]; pos = j; }
       if (wi[j] != 0) j++;
     }
     if (pos) {
@@ -959,6 +975,8 @@ typedef struct {
   double eta;
   int64_t matvecs, passes;
   uint64_t seed;                 /* random start vectors: seed + i (0: 0x5EED) */
+  const double *lu;              /* shift-and-invert: LU of A - sigma I (n x n), or NULL */
+  const int *piv;
 } KSo;
 
 /* EPSGetStartVector (epssolve.c:841-873) with a caller-provided vector for i==0
@@ -980,7 +998,14 @@ static void ks_expand(KSo *ks, int k, int *m, double *beta, int *breakdown)
 {
   int lindep = 0;
   for (int j = k; j < *m; j++) {
-    orc_spmv(ks->n, ks->rp, ks->ci, ks->val, ks->V + (int64_t)j * ks->ld, ks->V + (int64_t)(j + 1) * ks->ld);
+    if (ks->lu) {                /* STApply of STSINVERT with PREONLY + LU: (A - sigma I)^-1 v */
+      double *y = ks->V + (int64_t)(j + 1) * ks->ld;
+      int nn = ks->n, one = 1, info = 0;
+      memcpy(y, ks->V + (int64_t)j * ks->ld, sizeof(double) * (size_t)nn);
+      LP.getrs("N", &nn, &one, ks->lu, &nn, ks->piv, y, &nn, &info, 1);
+    } else {
+      orc_spmv(ks->n, ks->rp, ks->ci, ks->val, ks->V + (int64_t)j * ks->ld, ks->V + (int64_t)(j + 1) * ks->ld);
+    }
     ks->matvecs++;
     double nrm;
     orthogonalize_gs(ks->n, ks->V, ks->ld, j + 1, ks->buf, ks->m, ks->eta, &nrm, &lindep, &ks->passes);
@@ -1010,6 +1035,23 @@ int orc_eps_solve(int n, const int32_t *rp, const int32_t *ci, const double *val
   if (harm) herm = 0;
   cmp_target = o->target;
   double *gh = (double *)calloc(ncv + 2, sizeof(double)), gamma = 1.0;
+  /* shift-and-invert (sinvert.c, epssetup.c:119): dense LU of A - sigma I */
+  const int sinv = o->st == 1;
+  if (sinv && (which != ORC_TM && which != ORC_TR)) return -12;
+  if (sinv && harm) return -13;
+  cmp_sinv = sinv; cmp_sigma = o->sigma;
+  double *lu = NULL; int *piv = NULL;
+  if (sinv) {
+    if (!lp_load() || !LP.getrf || !LP.getrs) return -10;
+    lu = (double *)calloc((size_t)n * n, sizeof(double)); piv = (int *)calloc(n, sizeof(int));
+    for (int r = 0; r < n; r++) {
+      for (int32_t q = rp[r]; q < rp[r + 1]; q++) lu[r + (int64_t)ci[q] * n] = val[q];
+      lu[r + (int64_t)r * n] -= o->sigma;
+    }
+    int nn = n, info = 0;
+    LP.getrf(&nn, &nn, lu, &nn, piv, &info);
+    if (info) { free(lu); free(piv); return -14; }
+  }
   /* EPSARNOLDI (arnoldi.c:52-131): explicit restart, always DSNHEP (:45);
      ncv <= nev + mpd (:181) */
   const int arn = o->solver == 1;
@@ -1028,6 +1070,7 @@ int orc_eps_solve(int n, const int32_t *rp, const int32_t *ci, const double *val
   KSo ks = {0};
   ks.n = n; ks.rp = rp; ks.ci = ci; ks.val = val; ks.m = ncv + 1; ks.ld = n;
   ks.eta = o->eta > 0 ? o->eta : 0.7071;
+  ks.lu = lu; ks.piv = piv;
   ks.V = (double *)calloc((size_t)n * ks.m, sizeof(double));
   ks.buf = (double *)calloc((size_t)ks.m * ks.m, sizeof(double));
   int ld = ncv + 1;
@@ -1224,6 +1267,9 @@ int orc_eps_solve(int n, const int32_t *rp, const int32_t *ci, const double *val
     if (ds_vectors_nhep_all(&ds, 0)) { rc = -60; goto done; }
     orc_bv_multinplace(n, ks.V, ks.ld, 0, nconv, 0, nconv, ds.X, ld);
   }
+  /* EPSComputeValues: STBackTransform, then the EPS sorting criterion (no map) */
+  if (sinv) for (int i = 0; i < nconv; i++) sinv_map(&eigr[i], &eigi[i]);
+  cmp_sinv = 0;
   /* reorder conjugate pairs, positive imaginary part first (epssolve.c:159-176) */
   for (int i = 0; i < nconv - 1; i++) {
     if (eigi[i] != 0) {
@@ -1270,7 +1316,8 @@ done:
 #ifdef _OPENMP
   omp_set_num_threads(saved_threads);
 #endif
-  free(gh);
+  free(gh); free(lu); free(piv);
+  cmp_sinv = 0;
   free(ks.V); free(ks.buf); free(ds.A); free(ds.Q); free(ds.X); free(ds.T); free(ds.work); free(ds.perm); free(ds.iwork);
   free(eigr); free(eigi); free(errest); free(perm);
   free(bnd); free(yr); free(cv); free(lp);

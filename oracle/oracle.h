@@ -79,6 +79,8 @@ typedef struct {
                           2 Lanczos with full reorthogonalization (lanczos.c) */
   int     extraction;  /* 0 Ritz, 1 harmonic Ritz (EPS_HARMONIC; Krylov-Schur and Arnoldi) */
   double  target;      /* EPSSetTarget: which 7/8 and the harmonic shift */
+  int     st;          /* 0 shift (sigma = 0), 1 shift-and-invert (STSINVERT) */
+  double  sigma;       /* shift of the shift-and-invert operator */
 } orc_opts;
 
 typedef struct {

@@ -364,6 +364,16 @@ class EPS:
     def setTarget(self, target):
         check(lib().sa_eps_set_target(self.h, float(target)))
 
+    def setST(self, st_type, shift=None):
+        """STSetType ("shift" or "sinvert") and STSetShift (default: the target)."""
+        check(lib().sa_eps_set_st(self.h, {"shift": 0, "sinvert": 1}[st_type], 0 if shift is None else 1,
+                                  0.0 if shift is None else float(shift)))
+
+    def setSTKSP(self, pc="jacobi", rtol=1e-12, max_it=10000):
+        """Linear solver of the spectral transformation: BiCGStab with a
+        Jacobi ("jacobi") or no ("none") preconditioner."""
+        check(lib().sa_eps_set_st_ksp(self.h, {"none": 0, "jacobi": 1}[pc], float(rtol), int(max_it)))
+
     def setTwoSided(self, twosided=True):
         """EPSSetTwoSided: also compute left eigenvectors (two-sided Krylov-Schur)."""
         check(lib().sa_eps_set_two_sided(self.h, 1 if twosided else 0))

@@ -95,6 +95,37 @@ int bv_sweep(BV &V, const Mat &A, int k, int *m, double eta, double *beta, int *
   return 0;
 }
 
+// BVMatArnoldi with the shift-and-invert operator (STApply of STSINVERT:
+// one linear solve per column, stsolve.c:16-60): V(:,j+1) = (A - sigma I)^-1
+// V(:,j) by the device BiCGStab, then the Gram-Schmidt slots without SpMV.
+int bv_sweep_sinvert(BV &V, const Mat &A, KspWork &w, double rtol, int max_it, int k, int *m, double eta,
+                     double *beta, int *breakdown, int64_t *ksp_its) {
+  SA_CHECK(k >= 0 && *m > k && *m < V.m, "invalid Arnoldi range");
+  SA_CHECK(A.prec == SA_FP64 && V.prec == SA_FP64 && A.n == V.n, "shift-and-invert needs an FP64 operator and basis");
+  hipStream_t s = stream();
+  SA_TRY(launch_gs_reset(V, s));
+  int stop = 0;
+  for (int j = k; j < *m && !stop; j++) {
+    int its = 0;
+    SA_TRY(ksp_solve(A, w, (const double *)col_ptr(V, j), (double *)col_ptr(V, j + 1), rtol, max_it, &its));
+    if (ksp_its) *ksp_its += its;
+    SA_TRY(launch_gs_column(V, nullptr, j + 1, eta, s));
+    SA_HIP(hipMemcpyAsync(V.st_host, V.st + 2, sizeof(GsState), hipMemcpyDeviceToHost, s));
+    SA_HIP(hipStreamSynchronize(s));
+    stop = V.st_host->stop;
+  }
+  const GsState &st = *V.st_host;
+  if (st.stop) {
+    *m = st.stop_col;
+    *breakdown = 1;
+  } else {
+    *breakdown = 0;
+  }
+  *beta = st.beta;
+  V.passes_done = st.passes_total;
+  return 0;
+}
+
 // BVOrthonormalizeColumn(V,j,PETSC_FALSE,&norm,&lindep) (bvorthog.c:380-427)
 int bv_orthonormalize(BV &V, int j, double eta, double *norm, int *lindep) {
   hipStream_t s = stream();

@@ -79,6 +79,7 @@ struct sa_eps_s {
   double tol32 = 1e-5;
   BV V;                        // FP64 basis holding the solution
   bool twosided = false;       // EPSSetTwoSided
+  bool shift_set = false;      // STSetShift given (else sigma = target, STSetDefaultShift)
   sa_mat AT = nullptr;         // A^T, built at the first two-sided solve
   const void *AT_of = nullptr; // the operator AT was built from
   BV W;                        // left basis (two-sided)
@@ -498,6 +499,25 @@ int sa_eps_set_which(sa_eps E, int which) {
   return 0;
 }
 
+int sa_eps_set_st(sa_eps E, int st_type, int shift_given, double shift) {
+  SA_CHECK(E && (st_type == SA_ST_SHIFT || st_type == SA_ST_SINVERT), "unsupported spectral transformation");
+  SA_CHECK(st_type == SA_ST_SINVERT || !shift_given || shift == 0.0, "the shift transformation runs with sigma = 0");
+  E->p.st_type = st_type;
+  E->shift_set = shift_given != 0;
+  E->p.st_shift = shift_given ? shift : 0.0;
+  E->solved = false;
+  return 0;
+}
+
+int sa_eps_set_st_ksp(sa_eps E, int pc_type, double rtol, int max_it) {
+  SA_CHECK(E && (pc_type == SA_PC_NONE || pc_type == SA_PC_JACOBI), "unsupported preconditioner");
+  SA_CHECK(rtol > 0.0 && rtol < 1.0 && max_it > 0, "invalid linear solver tolerances");
+  E->p.ksp_jacobi = pc_type == SA_PC_JACOBI;
+  E->p.ksp_rtol = rtol;
+  E->p.ksp_max_it = max_it;
+  return 0;
+}
+
 int sa_eps_set_two_sided(sa_eps E, int twosided) {
   SA_CHECK(E, "null EPS");
   E->twosided = twosided != 0;
@@ -561,6 +581,7 @@ static int eps_setup_dims(sa_eps E, KsParams &p) {
   SA_CHECK(p.ncv >= p.nev + 1 || (p.ncv == p.nev && p.ncv == n), "The value of ncv must be at least nev+1");
   SA_CHECK(p.ncv <= p.nev + p.mpd, "The value of ncv must not be larger than nev+mpd");
   if (!E->maxit_set) p.max_it = std::max(100, 2 * n / p.ncv);
+  if (p.st_type == SA_ST_SINVERT && !E->shift_set) p.st_shift = p.target;   // STSetDefaultShift
   return 0;
 }
 
@@ -618,6 +639,8 @@ int sa_eps_solve(sa_eps E) {
   } else if (have_init) {
     SA_TRY(bv_upload_column(E->V, 0, E->init.data()));
   }
+  SA_CHECK(!(p.st_type == SA_ST_SINVERT && (E->mixed || E->twosided)),
+           "shift-and-invert runs in FP64 with the one-sided solvers");
   if (E->twosided) {
     SA_CHECK(!E->mixed, "the two-sided solver runs in FP64 only");
     if (!E->AT || E->AT_of != (const void *)E->A) {   // A^T as CSR (host transpose of the device arrays)

@@ -178,6 +178,25 @@ int sc_compare(int which, double ar, double ai, double br, double bi, double tar
   return 0;
 }
 
+// SlepcSCCompare (slepcsc.c:40-63) with STBackTransform_Sinvert
+// (sinvert.c:16-38) as the map when shift-and-invert is on.
+int DenseProblem::cmp(double ar, double ai, double br, double bi) const {
+  if (sinv_) {
+    auto map = [&](double &re, double &im) {
+      if (im == 0.0) {
+        re = 1.0 / re + sigma_;
+      } else {
+        const double t = re * re + im * im;
+        re = re / t + sigma_;
+        im = -im / t;
+      }
+    };
+    map(ar, ai);
+    map(br, bi);
+  }
+  return sc_compare(which_, ar, ai, br, bi, target_);
+}
+
 // DSTranslateHarmonic_NHEP (dsnhep.c:466-537).  Forward (recover = false):
 // g = (A - tau I)^-T beta e_n and A(:, n-1) += beta g.  Recover: with
 // ncol = l + k, gh = -Q(:, 0:ncol)^T g, A(i, j) += beta gh_i Q(n-1, j) for
@@ -337,11 +356,11 @@ int DenseProblem::sort(double *wr, double *wi) {
     for (int i = l + 1; i < t_; i++) {
       const double re = d[perm[i]];
       int j = i - 1;
-      int res = sc_compare(which_, re, 0.0, d[perm[j]], 0.0, target_);
+      int res = cmp(re, 0.0, d[perm[j]], 0.0);
       while (res < 0 && j >= l) {
         std::swap(perm[j], perm[j + 1]);
         j--;
-        if (j >= l) res = sc_compare(which_, re, 0.0, d[perm[j]], 0.0, target_);
+        if (j >= l) res = cmp(re, 0.0, d[perm[j]], 0.0);
       }
     }
     for (int i = l; i < n; i++) wr[i] = d[perm[i]];
@@ -365,7 +384,7 @@ int DenseProblem::sort(double *wr, double *wi) {
     double re = wr[i], im = wi[i];
     int pos = 0, j = (im != 0) ? i + 2 : i + 1;
     for (; j < n; j++) {
-      if (sc_compare(which_, re, im, wr[j], wi[j], target_) > 0) {
+      if (cmp(re, im, wr[j], wi[j]) > 0) {
         re = wr[j];
         im = wi[j];
         pos = j;

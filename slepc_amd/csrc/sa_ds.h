@@ -31,6 +31,9 @@ class DenseProblem {
   void set_dimensions(int n, int l, int k) { n_ = n; l_ = l; k_ = k; t_ = n; }
   void set_state(int s) { state_ = s; }
   void set_target(double t) { target_ = t; }      // comparison context of which 7/8
+  // The DS sorting criterion compares back-transformed values (SlepcMap_ST,
+  // epssetup.c:237): with shift-and-invert, theta -> sigma + 1/theta.
+  void set_sinvert(bool on, double sigma) { sinv_ = on; sigma_ = sigma; }
   // DSTranslateHarmonic (NHEP): g has n entries; returns LAPACK's info
   int translate_harmonic(double tau, double beta, bool recover, double *g, double *gamma);
   int n() const { return n_; }
@@ -69,6 +72,9 @@ class DenseProblem {
   int sort_with_permutation(int *perm, double *wr, double *wi);   // on (A, Q)
   int ld_ = 0, n_ = 0, l_ = 0, k_ = 0, t_ = 0, state_ = DS_RAW, which_ = 1;
   double target_ = 0.0;
+  bool sinv_ = false;
+  double sigma_ = 0.0;
+  int cmp(double ar, double ai, double br, double bi) const;   // SlepcSCCompare with the map
   bool herm_ = true;
   std::vector<double> A_, Q_, X_, T_, work_;
   std::vector<double> B_, Z_, Y_, wr2_, wi2_;     // two-sided only

@@ -180,6 +180,23 @@ int ks_solve(const KsParams &p, const Mat &A, BV &V, bool have_initial, KsResult
   SA_CHECK(!(lanc && harm), "EPSLANCZOS does not support harmonic extraction");
   std::vector<double> g(ld, 0.0);
   double gamma = 1.0;
+  // Shift-and-invert (sinvert.c; epssetup.c:119): each operator application
+  // is a solve with A - sigma I; the DS compares back-transformed values and
+  // the eigenvalues are mapped back (EPSComputeValues) before the final sort.
+  const bool sinv = p.st_type == 1;
+  SA_CHECK(!sinv || (p.which == SA_TARGET_MAGNITUDE || p.which == SA_TARGET_REAL),
+           "Shift-and-invert requires a target 'which' (target magnitude or target real)");
+  SA_CHECK(!(sinv && harm), "Cannot use a spectral transformation combined with harmonic extraction");
+  KspWork kw;
+  struct KspFree {
+    KspWork &w;
+    ~KspFree() { ksp_free(w); }
+  } kw_free{kw};
+  if (sinv) SA_TRY(ksp_alloc(kw, A, p.st_shift, p.ksp_jacobi));
+  auto sweep = [&](int k0, int *nv, double *beta, int *bd) -> int {
+    if (!sinv) return bv_sweep(V, A, k0, nv, p.eta, beta, bd);
+    return bv_sweep_sinvert(V, A, kw, p.ksp_rtol, p.ksp_max_it, k0, nv, p.eta, beta, bd, &r.ksp_its);
+  };
   double anorm = -1.0;              // running estimate of ||A|| (Lanczos)
   std::vector<double> bnd, Yl, yrst;
   std::vector<char> conv;
@@ -188,6 +205,7 @@ int ks_solve(const KsParams &p, const Mat &A, BV &V, bool have_initial, KsResult
   DenseProblem ds;
   ds.allocate(ld, dsherm, p.which);
   ds.set_target(p.target);
+  ds.set_sinvert(sinv, p.st_shift);
   std::vector<double> eigr(ld, 0.0), eigi(ld, 0.0), errest(ld, 0.0), buf, scratch;
   const double t0 = now();
   double tds = 0.0;
@@ -214,7 +232,7 @@ int ks_solve(const KsParams &p, const Mat &A, BV &V, bool have_initial, KsResult
     double beta = 0.0;
     int breakdown = 0;
     const int64_t pbefore = V.passes_done;
-    SA_TRY(bv_sweep(V, A, k0, &nv, p.eta, &beta, &breakdown));
+    SA_TRY(sweep(k0, &nv, &beta, &breakdown));
     r.matvecs += nv - k0;
     // Algorithmic bytes of the sweep: SpMV, then per Gram-Schmidt pass the dot
     // products read columns 0..c-1 and the update reads them again plus v and
@@ -366,6 +384,17 @@ int ks_solve(const KsParams &p, const Mat &A, BV &V, bool have_initial, KsResult
     SA_HIP(hipMemcpyAsync(V.dq, V.hq, sizeof(double) * (size_t)ld * ld, hipMemcpyHostToDevice, s));
     SA_TRY(launch_multinplace(V, 0, nconv, 0, nconv, V.dq, ld, s));
   }
+  // EPSComputeValues: STBackTransform_Sinvert (sinvert.c:16-38)
+  if (sinv)
+    for (int i = 0; i < nconv; i++) {
+      if (eigi[i] == 0.0) {
+        eigr[i] = 1.0 / eigr[i] + p.st_shift;
+      } else {
+        const double t = eigr[i] * eigr[i] + eigi[i] * eigi[i];
+        eigr[i] = eigr[i] / t + p.st_shift;
+        eigi[i] = -eigi[i] / t;
+      }
+    }
   // reorder conjugate pairs, positive imaginary part first (epssolve.c:159-176)
   for (int i = 0; i < nconv - 1; i++) {
     if (eigi[i] != 0) {

@@ -14,6 +14,11 @@ struct KsParams {
   int solver = 0;              // 0 Krylov-Schur, 1 Arnoldi with explicit restart (EPSARNOLDI), 2 Lanczos
   int extraction = 0;          // 0 Ritz, 1 harmonic Ritz (EPS_HARMONIC)
   double target = 0.0;         // EPSSetTarget: comparison point of which 7/8 and the harmonic shift
+  int st_type = 0;             // 0 shift (sigma = 0), 1 shift-and-invert (STSINVERT)
+  double st_shift = 0.0;       // sigma of the shift-and-invert operator
+  bool ksp_jacobi = true;      // PCJACOBI for the BiCGStab solves (else none)
+  double ksp_rtol = 1e-12;     // relative residual of each solve
+  int ksp_max_it = 10000;
   void (*monitor)(void *, int, int, const double *, const double *, int) = nullptr;
   void *monitor_ctx = nullptr;
 };
@@ -23,6 +28,7 @@ struct KsResult {
   int64_t matvecs = 0, gs_passes = 0;
   double seconds = 0.0, seconds_ds = 0.0;
   double bytes_spmv = 0.0, bytes_bv = 0.0, bytes_restart = 0.0;
+  int64_t ksp_its = 0;             // BiCGStab iterations (shift-and-invert)
   std::vector<int> perm;           // sorted position -> column of V
   std::vector<double> eigr, eigi, errest;
 };

@@ -115,6 +115,23 @@ int launch_fill_random(int prec, void *x, int64_t n, uint64_t seed, hipStream_t 
 int launch_bvdot(const BV &X, int lx, int kx, const BV &Y, int ly, int ky, double *M_dev, hipStream_t s);
 // BVMatMult: Y(:, ly+j) = A X(:, lx+j), j < ncols (sa_block.hip)
 int launch_spmm(const Mat &A, const BV &X, int lx, const BV &Y, int ly, int ncols, hipStream_t s);
+// Shift-and-invert solves x = (A - sigma I)^-1 b (BiCGStab, FP64; optional
+// Jacobi preconditioning), workspace of 9 vectors + partials in one buffer.
+struct KspWork {
+  double *buf = nullptr;
+  int64_t n = 0;
+  int G = 1;
+  double sigma = 0.0;
+  bool jacobi = true;
+};
+int ksp_alloc(KspWork &w, const Mat &A, double sigma, bool jacobi);
+void ksp_free(KspWork &w);
+// rtol relative to ||b||; error when max_it is reached or on breakdown
+int ksp_solve(const Mat &A, KspWork &w, const double *b_dev, double *x_dev, double rtol, int max_it, int *its);
+
+int bv_sweep_sinvert(BV &V, const Mat &A, KspWork &w, double rtol, int max_it, int k, int *m, double eta,
+                     double *beta, int *breakdown, int64_t *ksp_its);
+
 // TSQR of V(:, l:k) (k - l <= 64, V.n >= k - l): R_host ((k-l) x (k-l)) gets
 // the R of the tile tree; with formq, V(:, l:k) is overwritten by Q times the
 // signs f (k - l entries) that make Q and diag(f) R equal geqrf/orgqr's.

@@ -1,0 +1,263 @@
+// Linear solves for the shift-and-invert spectral transformation
+// (STSINVERT: src/sys/classes/st/impls/sinvert/sinvert.c, STMatSolve in
+// st/interface/stsles.c): x = (A - sigma I)^-1 b by BiCGStab with right
+// Jacobi (or no) preconditioning, in FP64, every scalar kept in device
+// memory so that one iteration is a chain of stream-ordered kernels with no
+// host round trip; the host reads the residual norm every KSP_CHECK
+// iterations.  Dot products are per-workgroup partials summed by one
+// workgroup in a fixed order (deterministic).
+#include <algorithm>
+#include <cmath>
+#include <vector>
+
+#include "sa_device.h"
+#include "sa_internal.h"
+
+namespace sa {
+
+namespace {
+
+constexpr int KSP_MAXG = 1024;   // partial-sum workgroups
+constexpr int KSP_CHECK = 8;     // iterations between host residual checks
+
+// scalar slots in device memory
+enum { S_RHO = 0, S_ALPHA, S_OMEGA, S_BETA, S_RR, S_FLAG, S_NSCAL };
+
+__device__ __forceinline__ double block_sum(double v, double *sh) {
+  for (int m = 32; m >= 1; m >>= 1) v += __shfl_xor(v, m, 64);
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  __syncthreads();
+  if (lane == 0) sh[w] = v;
+  __syncthreads();
+  double s = 0.0;
+  for (int q = 0; q < (int)(blockDim.x >> 6); q++) s += sh[q];
+  return s;
+}
+
+__global__ __launch_bounds__(BLK) void ksp_init_kernel(int64_t n, const double *__restrict__ b, double *r,
+                                                       double *rhat, double *p, double *v, double *x, double *scal) {
+  const int64_t i = (int64_t)blockIdx.x * BLK + threadIdx.x;
+  if (i < n) {
+    r[i] = b[i];
+    rhat[i] = b[i];
+    p[i] = 0.0;
+    v[i] = 0.0;
+    x[i] = 0.0;
+  }
+  if (i == 0) {
+    scal[S_RHO] = 1.0;
+    scal[S_ALPHA] = 1.0;
+    scal[S_OMEGA] = 1.0;
+    scal[S_BETA] = 0.0;
+    scal[S_RR] = 0.0;
+    scal[S_FLAG] = 0.0;
+  }
+}
+
+// part[g] = partial a.b, part[G + g] = partial c.d (c may be null)
+__global__ __launch_bounds__(BLK) void ksp_dot2_kernel(int64_t n, const double *__restrict__ a,
+                                                       const double *__restrict__ b, const double *__restrict__ c,
+                                                       const double *__restrict__ d, double *part, int G) {
+  __shared__ double sh[BLK / 64];
+  double s1 = 0.0, s2 = 0.0;
+  for (int64_t i = (int64_t)blockIdx.x * BLK + threadIdx.x; i < n; i += (int64_t)G * BLK) {
+    s1 += a[i] * b[i];
+    if (c) s2 += c[i] * d[i];
+  }
+  s1 = block_sum(s1, sh);
+  s2 = block_sum(s2, sh);
+  if (threadIdx.x == 0) {
+    part[blockIdx.x] = s1;
+    part[G + blockIdx.x] = s2;
+  }
+}
+
+// y -= sigma x in place, then the partials of w.y (and of y.z when z given)
+__global__ __launch_bounds__(BLK) void ksp_shift_dot_kernel(int64_t n, double *y, const double *__restrict__ x,
+                                                            double sigma, const double *__restrict__ w,
+                                                            const double *__restrict__ z, double *part, int G) {
+  __shared__ double sh[BLK / 64];
+  double s1 = 0.0, s2 = 0.0;
+  for (int64_t i = (int64_t)blockIdx.x * BLK + threadIdx.x; i < n; i += (int64_t)G * BLK) {
+    const double yi = y[i] - sigma * x[i];
+    y[i] = yi;
+    s1 += w[i] * yi;
+    if (z) s2 += yi * z[i];
+  }
+  s1 = block_sum(s1, sh);
+  s2 = block_sum(s2, sh);
+  if (threadIdx.x == 0) {
+    part[blockIdx.x] = s1;
+    part[G + blockIdx.x] = s2;
+  }
+}
+
+// one workgroup: d0, d1 = sums of the partials (fixed order), then
+// op 0: rho' = d0, rr = d1, beta = (rho'/rho)(alpha/omega), rho = rho'
+// op 1: alpha = rho / d0
+// op 2: omega = d0 / d1 (t.s / t.t)
+// op 3: scal[0] = d0, scal[1] = d1 (plain sums, for the host checks)
+__global__ __launch_bounds__(BLK) void ksp_scalar_kernel(const double *__restrict__ part, int G, int op,
+                                                         double *scal) {
+  __shared__ double sh[BLK / 64];
+  double a = 0.0, b = 0.0;
+  for (int g = threadIdx.x; g < G; g += BLK) {
+    a += part[g];
+    b += part[G + g];
+  }
+  const double d0 = block_sum(a, sh), d1 = block_sum(b, sh);
+  if (threadIdx.x == 0) {
+    if (op == 0) {
+      const double rho = scal[S_RHO], alpha = scal[S_ALPHA], omega = scal[S_OMEGA];
+      const bool bad = d0 == 0.0 || rho == 0.0 || omega == 0.0;
+      scal[S_BETA] = bad ? 0.0 : (d0 / rho) * (alpha / omega);
+      scal[S_RHO] = d0;
+      scal[S_RR] = d1;
+      if (bad && d1 != 0.0) scal[S_FLAG] = 1.0;   // breakdown with a nonzero residual
+    } else if (op == 1) {
+      scal[S_ALPHA] = d0 != 0.0 ? scal[S_RHO] / d0 : 0.0;
+      if (d0 == 0.0) scal[S_FLAG] = 1.0;
+    } else if (op == 2) {
+      scal[S_OMEGA] = d1 != 0.0 ? d0 / d1 : 0.0;
+    } else {
+      scal[0] = d0;
+      scal[1] = d1;
+    }
+  }
+}
+
+// p = r + beta (p - omega v); phat = dinv .* p
+__global__ __launch_bounds__(BLK) void ksp_p_kernel(int64_t n, const double *__restrict__ r, double *p,
+                                                    const double *__restrict__ v, const double *__restrict__ dinv,
+                                                    double *phat, const double *__restrict__ scal) {
+  const int64_t i = (int64_t)blockIdx.x * BLK + threadIdx.x;
+  if (i >= n) return;
+  const double beta = scal[S_BETA], omega = scal[S_OMEGA];
+  const double pi = r[i] + beta * (p[i] - omega * v[i]);
+  p[i] = pi;
+  phat[i] = dinv ? dinv[i] * pi : pi;
+}
+
+// s = r - alpha v; shat = dinv .* s
+__global__ __launch_bounds__(BLK) void ksp_s_kernel(int64_t n, const double *__restrict__ r,
+                                                    const double *__restrict__ v, const double *__restrict__ dinv,
+                                                    double *s, double *shat, const double *__restrict__ scal) {
+  const int64_t i = (int64_t)blockIdx.x * BLK + threadIdx.x;
+  if (i >= n) return;
+  const double si = r[i] - scal[S_ALPHA] * v[i];
+  s[i] = si;
+  shat[i] = dinv ? dinv[i] * si : si;
+}
+
+// x += alpha phat + omega shat; r = s - omega t
+__global__ __launch_bounds__(BLK) void ksp_x_kernel(int64_t n, double *x, const double *__restrict__ phat,
+                                                    const double *__restrict__ shat, double *r,
+                                                    const double *__restrict__ s, const double *__restrict__ t,
+                                                    const double *__restrict__ scal) {
+  const int64_t i = (int64_t)blockIdx.x * BLK + threadIdx.x;
+  if (i >= n) return;
+  const double alpha = scal[S_ALPHA], omega = scal[S_OMEGA];
+  x[i] += alpha * phat[i] + omega * shat[i];
+  r[i] = s[i] - omega * t[i];
+}
+
+// dinv = 1 / (a_ii - sigma) (PCJACOBI on A - sigma I; 1 for a zero diagonal)
+__global__ __launch_bounds__(BLK) void ksp_jacobi_kernel(int n, const int32_t *__restrict__ rp,
+                                                         const int32_t *__restrict__ ci,
+                                                         const double *__restrict__ val, double sigma, double *dinv) {
+  const int i = blockIdx.x * BLK + threadIdx.x;
+  if (i >= n) return;
+  double d = -sigma;
+  for (int32_t q = rp[i]; q < rp[i + 1]; q++)
+    if (ci[q] == i) d += val[q];
+  dinv[i] = d != 0.0 ? 1.0 / d : 1.0;
+}
+
+}  // namespace
+
+int ksp_alloc(KspWork &w, const Mat &A, double sigma, bool jacobi) {
+  SA_CHECK(A.prec == SA_FP64, "the shift-and-invert solves run in FP64");
+  const int64_t n = A.n;
+  if (w.n != n || !w.buf) {
+    ksp_free(w);
+    SA_HIP(hipMalloc(&w.buf, sizeof(double) * ((size_t)9 * std::max<int64_t>(n, 1) + 2 * KSP_MAXG + 8)));
+    w.n = n;
+  }
+  w.G = (int)std::min<int64_t>(KSP_MAXG, std::max<int64_t>(1, (n + 4 * BLK - 1) / (4 * BLK)));
+  w.sigma = sigma;
+  w.jacobi = jacobi;
+  if (jacobi && n > 0) {
+    hipLaunchKernelGGL(ksp_jacobi_kernel, dim3((unsigned)((n + BLK - 1) / BLK)), dim3(BLK), 0, stream(), A.n,
+                       A.rowptr, A.col, (const double *)A.val, sigma, w.buf + 8 * n);
+    SA_HIP(hipGetLastError());
+  }
+  return 0;
+}
+
+void ksp_free(KspWork &w) {
+  if (w.buf) (void)hipFree(w.buf);
+  w.buf = nullptr;
+  w.n = 0;
+}
+
+int ksp_solve(const Mat &A, KspWork &w, const double *b, double *x, double rtol, int max_it, int *its_out) {
+  const int64_t n = w.n;
+  SA_CHECK(n == A.n && w.buf, "solver workspace does not match the operator");
+  hipStream_t s = stream();
+  double *r = w.buf, *rhat = r + n, *p = rhat + n, *v = p + n, *sv = v + n, *t = sv + n, *phat = t + n,
+         *shat = phat + n, *dinv = w.jacobi ? shat + n : nullptr;
+  double *part = w.buf + 9 * n, *scal = part + 2 * KSP_MAXG;
+  const int G = w.G;
+  const unsigned nb = (unsigned)std::max<int64_t>(1, (n + BLK - 1) / BLK);
+  const double sigma = w.sigma;
+  hipLaunchKernelGGL(ksp_init_kernel, dim3(nb), dim3(BLK), 0, s, n, b, r, rhat, p, v, x, scal);
+  // ||b|| for the relative test
+  double *spare = scal + S_NSCAL;   // two slots for the host checks
+  hipLaunchKernelGGL(ksp_dot2_kernel, dim3(G), dim3(BLK), 0, s, n, b, b, nullptr, nullptr, part, G);
+  hipLaunchKernelGGL(ksp_scalar_kernel, dim3(1), dim3(BLK), 0, s, part, G, 3, spare);
+  SA_HIP(hipGetLastError());
+  double hs[S_NSCAL + 2];
+  SA_HIP(hipMemcpyAsync(hs, spare, sizeof(double), hipMemcpyDeviceToHost, s));
+  SA_HIP(hipStreamSynchronize(s));
+  const double bnorm = std::sqrt(std::max(0.0, hs[0]));
+  int its = 0;
+  if (bnorm == 0.0) {   // x = 0 is the solution (set by the init kernel)
+    if (its_out) *its_out = 0;
+    return 0;
+  }
+  bool conv = false;
+  while (!conv) {
+    for (int q = 0; q < KSP_CHECK && its < max_it; q++, its++) {
+      hipLaunchKernelGGL(ksp_dot2_kernel, dim3(G), dim3(BLK), 0, s, n, rhat, r, r, r, part, G);
+      hipLaunchKernelGGL(ksp_scalar_kernel, dim3(1), dim3(BLK), 0, s, part, G, 0, scal);
+      hipLaunchKernelGGL(ksp_p_kernel, dim3(nb), dim3(BLK), 0, s, n, r, p, v, dinv, phat, scal);
+      SA_TRY(launch_spmv(A, phat, v, s));
+      hipLaunchKernelGGL(ksp_shift_dot_kernel, dim3(G), dim3(BLK), 0, s, n, v, phat, sigma, rhat, nullptr, part, G);
+      hipLaunchKernelGGL(ksp_scalar_kernel, dim3(1), dim3(BLK), 0, s, part, G, 1, scal);
+      hipLaunchKernelGGL(ksp_s_kernel, dim3(nb), dim3(BLK), 0, s, n, r, v, dinv, sv, shat, scal);
+      SA_TRY(launch_spmv(A, shat, t, s));
+      hipLaunchKernelGGL(ksp_shift_dot_kernel, dim3(G), dim3(BLK), 0, s, n, t, shat, sigma, sv, t, part, G);
+      hipLaunchKernelGGL(ksp_scalar_kernel, dim3(1), dim3(BLK), 0, s, part, G, 2, scal);
+      hipLaunchKernelGGL(ksp_x_kernel, dim3(nb), dim3(BLK), 0, s, n, x, phat, shat, r, sv, t, scal);
+    }
+    SA_HIP(hipGetLastError());
+    // the residual norm of the current x: one more r.r
+    hipLaunchKernelGGL(ksp_dot2_kernel, dim3(G), dim3(BLK), 0, s, n, r, r, nullptr, nullptr, part, G);
+    hipLaunchKernelGGL(ksp_scalar_kernel, dim3(1), dim3(BLK), 0, s, part, G, 3, spare);
+    SA_HIP(hipMemcpyAsync(hs, scal, sizeof(double) * (S_NSCAL + 2), hipMemcpyDeviceToHost, s));
+    SA_HIP(hipStreamSynchronize(s));
+    const double rnorm = std::sqrt(std::max(0.0, hs[S_NSCAL]));
+    if (rnorm <= rtol * bnorm) conv = true;
+    else if (hs[S_FLAG] != 0.0 || !std::isfinite(rnorm)) {
+      set_error("shift-and-invert: BiCGStab breakdown in (A - sigma I) x = b");
+      return 1;
+    } else if (its >= max_it) {
+      set_error("shift-and-invert: the linear solver did not converge within max_it iterations");
+      return 1;
+    }
+  }
+  if (its_out) *its_out = its;
+  return 0;
+}
+
+}  // namespace sa

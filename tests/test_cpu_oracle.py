@@ -486,3 +486,32 @@ def test_oracle_twosided(orc, case):
     G = Y.T @ X
     assert np.max(np.abs(G - np.diag(np.diag(G)))) < 1e-6
     assert np.min(np.abs(np.diag(G))) > 1e-3
+
+
+@pytest.mark.parametrize("solver", ["krylovschur", "arnoldi"])
+def test_oracle_sinvert_test9_5(orc, solver):
+    """eps/tests/test9.c -eps_nev 3 -st_type sinvert -eps_target .5 on the
+    Markov model m = 15 (test9_5.out): shift-and-invert with an exact solve
+    (the reference's default PREONLY + LU) gives the printed values in the
+    printed order, and dense LAPACK's eigenvalues closest to 0.5."""
+    A = orc.markov(15)
+    r = orc.eps_solve(A, 3, 0, 1e-8, "TM", False, v0=orc.rand_uniform(0, A.n), target=0.5, st="sinvert",
+                      solver=solver)
+    assert r["nconv"] >= 3
+    assert list(np.round(r["eigr"][:3], 5)) == [0.49842, 0.51928, 0.47897]
+    ev = np.linalg.eigvals(A.to_dense())
+    near = ev[np.argsort(np.abs(ev - 0.5))[:3]]
+    assert np.allclose(r["eigr"][:3], near.real, rtol=1e-9)
+
+
+@pytest.mark.parametrize("solver", ["lanczos", "krylovschur"])
+def test_oracle_sinvert_test2_interior(orc, solver):
+    """eps/tests/test2.c interior eigenvalues with Lanczos and shift-and-invert
+    at target 2.1 (A - 2.1 I indefinite; test2_1.out) and the closed form."""
+    A = _lap1d(orc, 30)
+    r = orc.eps_solve(A, 4, 0, 1e-9, "TM", True, v0=orc.rand_uniform(0, 30), target=2.1, st="sinvert",
+                      solver=solver)
+    assert r["nconv"] >= 4
+    assert np.allclose(r["eigr"][:4], TEST2_INTERIOR[30], atol=5e-6)
+    exact = 2 - 2 * np.cos(np.arange(1, 31) * np.pi / 31)
+    assert np.allclose(r["eigr"][:4], exact[np.argsort(np.abs(exact - 2.1))[:4]], rtol=1e-9)

@@ -509,3 +509,81 @@ def test_twosided_matches_oracle(sa, orc, case):
         assert min(np.abs(x - xo).max(), np.abs(x + xo).max()) < 1e-6
         assert min(np.abs(y - yo).max(), np.abs(y + yo).max()) < 1e-6
         assert E.computeError(i) < 10 * tol
+
+
+@pytest.mark.parametrize("solver,case", [("krylovschur", "markov15"), ("arnoldi", "markov15"),
+                                         ("lanczos", "test2"), ("krylovschur", "test2"),
+                                         ("krylovschur", "lap2d40")])
+def test_sinvert_matches_oracle(sa, orc, solver, case):
+    """Shift-and-invert through the C ABI (device BiCGStab with Jacobi, rtol
+    1e-13) against the oracle's exact solves (dense LU, the reference's
+    PREONLY + LU): eigenvalues in the same order to 1e-9, residuals.
+    markov15: test9_5.out (target 0.5); test2: test2_1.out interior values
+    (target 2.1, indefinite); lap2d40: smallest 6 of the 2-D Laplacian on a
+    40 x 37 grid (target 0, SPD; a non-square grid, so that no eigenvalue
+    is double: a single-vector Krylov method finds the second copy of a
+    double eigenvalue only through rounding, which differs between exact
+    and iterative solves)."""
+    from test_cpu_oracle import _lap1d, TEST2_INTERIOR
+    if case == "markov15":
+        Ao, herm, nev, tol, tgt = orc.markov(15), False, 3, 1e-8, 0.5
+    elif case == "test2":
+        Ao, herm, nev, tol, tgt = _lap1d(orc, 30), True, 4, 1e-9, 2.1
+    else:
+        Ao, herm, nev, tol, tgt = orc.lap2d(40, 37), True, 6, 1e-10, 0.0
+    Ah = sa.CSR(Ao.n, Ao.rowptr, Ao.col, Ao.val)
+    v0 = orc.rand_uniform(0, Ao.n)
+    E = sa.EPS()
+    E.setType(solver)
+    E.setOperators(sa.Mat(Ah))
+    E.setProblemType(herm)
+    E.setWhichEigenpairs("TM")
+    E.setTarget(tgt)
+    E.setST("sinvert")
+    E.setSTKSP("jacobi", 1e-13, 20000)
+    E.setDimensions(nev, 0)
+    E.setTolerances(tol, 0)
+    E.setInitialSpace(v0)
+    E.solve()
+    r = orc.eps_solve(Ao, nev, 0, tol, "TM", herm, v0=v0, target=tgt, st="sinvert", solver=solver)
+    assert E.getConverged() >= nev and r["nconv"] >= nev
+    pairs = [E.getEigenpair(i, False) for i in range(nev)]
+    vals = np.array([p[0] + 1j * p[1] for p in pairs])
+    ref = r["eigr"][:nev] + 1j * r["eigi"][:nev]
+    assert np.max(np.abs(vals - ref) / np.abs(ref)) < 1e-9
+    if case == "markov15":
+        assert list(np.round(vals.real, 5)) == [0.49842, 0.51928, 0.47897]
+    if case == "test2":
+        assert np.allclose(vals.real, TEST2_INTERIOR[30], atol=5e-6)
+    # the convergence test bounds the residual of the transformed problem:
+    # ||A x - lambda x|| / |lambda| <= ||A - sigma I|| tol / |lambda|
+    nrm = max(np.abs(Ao.val[Ao.rowptr[i]:Ao.rowptr[i + 1]]).sum() for i in range(Ao.n)) + abs(tgt)
+    for i in range(nev):
+        assert E.computeError(i) < 10 * tol * max(1.0, nrm / abs(vals[i]))
+
+
+def test_sinvert_laplacian_smallest(sa, orc):
+    """The 2-D Laplacian on a 100 x 97 grid (C1's stencil and size, with
+    distinct eigenvalues) with shift-and-invert at target 0: the 10 smallest
+    eigenvalues from the closed form 4 - 2 cos(i pi/101) - 2 cos(j pi/98),
+    every residual within ||A|| tol / |lambda| (the convergence test bounds
+    the residual of the transformed problem)."""
+    Ah = sa.lap2d(100, 97)
+    E = sa.EPS()
+    E.setOperators(sa.Mat(Ah))
+    E.setProblemType(True)
+    E.setWhichEigenpairs("TM")
+    E.setTarget(0.0)
+    E.setST("sinvert")
+    E.setSTKSP("jacobi", 1e-12, 20000)
+    E.setDimensions(10, 32)
+    E.setTolerances(1e-8, 0)
+    E.setInitialSpace(orc.rand_uniform(0, Ah.n))
+    E.solve()
+    assert E.getConverged() >= 10
+    kx, ky = np.arange(1, 101) * np.pi / 101, np.arange(1, 98) * np.pi / 98
+    exact = np.sort((4 - 2 * np.cos(kx)[:, None] - 2 * np.cos(ky)[None, :]).ravel())[:10]
+    vals = np.array([E.getEigenpair(i, False)[0] for i in range(10)])
+    assert np.allclose(np.sort(vals), exact, rtol=1e-8)
+    for i in range(10):
+        assert E.computeError(i) < 10 * 1e-8 * 8.0 / vals[i]
