@@ -38,7 +38,7 @@ sys.path.insert(0, ROOT)
 HBM_PEAK_GBS = 8000.0
 NEV, NCV, TOL, GRID = 16, 64, 1e-10, 256
 THETA_SEED_TEST = 5
-STREAMS = 4
+STREAMS = 8
 
 
 def log(*a):

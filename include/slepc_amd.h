@@ -155,7 +155,9 @@ int sa_bv_mat_arnoldi(sa_bv V, sa_mat A, int k, int *m, double eta, double *beta
    _TSQR (bvorthog.c:615-631, bvlapack.c:347-480): Householder QR over a tree
    of 256-row tiles, Q formed explicitly; _TSQRCHOL (bvorthog.c:638-659): the
    same tree for R only, then V2 = V2 R^-1.  Both return the Q and R that
-   geqrf/orgqr give on one process (same signs); at most 64 active columns. */
+   geqrf/orgqr give on one process (same signs).  Above 64 active columns
+   the block is factored in panels of 64 (each orthogonalised twice against
+   the previous panels, then by the tree), for both methods. */
 #define SA_ORTHOG_BLOCK_GS 0
 #define SA_ORTHOG_BLOCK_CHOL 1
 #define SA_ORTHOG_BLOCK_SVQB 2

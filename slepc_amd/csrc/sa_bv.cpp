@@ -166,6 +166,49 @@ int block_gs(BV &V, int l, int k, std::vector<double> &R12, double *M_dev, hipSt
   return apply_q(V, 0, k, l, k, k, s);
 }
 
+// TSQR of more than TS_MAXK columns: panels of 64 columns, each first
+// orthogonalised twice against the previous panels (block classical
+// Gram-Schmidt, BVDot + the restart GEMM), then factored by the tile tree;
+// at the end one sign fix from the first rows of the whole Q makes Q and R
+// those of geqrf/orgqr on the whole block.  R_host: (k-l) x (k-l).
+int tsqr_panels(BV &V, int l, int k, double *Rf, double *M_dev, hipStream_t s) {
+  constexpr int P = 64;
+  const int nk = k - l;
+  std::fill(Rf, Rf + (size_t)nk * nk, 0.0);
+  std::vector<double> R12, Rp, fp;
+  for (int c0 = l; c0 < k; c0 += P) {
+    const int w = std::min(P, k - c0), off = c0 - l;
+    for (int pass = 0; pass < 2 && off > 0; pass++) {
+      SA_TRY(launch_bvdot(V, c0, c0 + w, V, l, c0, M_dev, s));   // V(:,l:c0)^T V(:,c0:c0+w)
+      R12.resize((size_t)off * w);
+      SA_HIP(hipMemcpy(R12.data(), M_dev, sizeof(double) * R12.size(), hipMemcpyDeviceToHost));
+      std::fill(V.hq, V.hq + (size_t)k * k, 0.0);
+      for (int j = 0; j < w; j++) {
+        for (int i = 0; i < off; i++) {
+          V.hq[(l + i) + (size_t)(c0 + j) * k] = -R12[i + (size_t)j * off];
+          Rf[i + (size_t)(off + j) * nk] += R12[i + (size_t)j * off];
+        }
+        V.hq[(c0 + j) + (size_t)(c0 + j) * k] = 1.0;
+      }
+      SA_TRY(apply_q(V, l, c0 + w, c0, c0 + w, k, s));
+    }
+    Rp.assign((size_t)w * w, 0.0);
+    fp.assign(w, 1.0);
+    SA_TRY(launch_tsqr(V, c0, c0 + w, true, Rp.data(), fp.data(), s));
+    for (int j = 0; j < w; j++)
+      for (int i = 0; i <= j; i++) Rf[(off + i) + (size_t)(off + j) * nk] = fp[i] * Rp[i + (size_t)j * w];
+  }
+  std::vector<double> Qt((size_t)nk * nk), f(nk);
+  SA_TRY(bv_top_block(V, l, k, Qt.data()));
+  tsqr_signs(Qt.data(), Rf, V.n, nk, f.data());
+  for (int j = 0; j < nk; j++) {
+    if (f[j] < 0.0) SA_TRY(launch_scale(V.prec, col_ptr(V, l + j), V.n, -1.0, s));
+    for (int c = 0; c < nk; c++) Rf[j + (size_t)c * nk] *= f[j];
+  }
+  SA_HIP(hipStreamSynchronize(s));
+  return 0;
+}
+
 }  // namespace
 
 int bv_orthogonalize(BV &V, int method, double *R, int ldr) {
@@ -195,9 +238,18 @@ int bv_orthogonalize(BV &V, int method, double *R, int ldr) {
   std::vector<double> R12, G((size_t)nk * nk);
   int rc = l ? block_gs(V, l, k, R12, M_dev, s) : 0;
   if (method >= 3) {   // BVOrthogonalize_TSQR / _TSQRCHOL (bvorthog.c:615-659)
+    std::vector<double> Rt(G.size()), f(nk, 1.0);
+    if (!rc && nk > 64) {   // panels; TSQRCHOL's Q = V2 inv(R) is the same Q, formed here directly
+      rc = tsqr_panels(V, l, k, Rt.data(), M_dev, s);
+      (void)hipFree(M_dev);
+      if (rc) return rc;
+      if (R)
+        for (int j = 0; j < nk; j++)
+          for (int i = 0; i < k; i++) R[i + (size_t)(l + j) * ldr] = i < l ? R12[i + (size_t)j * l] : Rt[(i - l) + (size_t)j * nk];
+      return 0;
+    }
     (void)hipFree(M_dev);
     if (rc) return rc;
-    std::vector<double> Rt(G.size()), f(nk);
     SA_TRY(launch_tsqr(V, l, k, method == 3, Rt.data(), f.data(), s));
     if (method == 4) {   // R only (bvlapack.c:510-560), then V2 = V2 inv(R) (bvorthog.c:654-655)
       std::vector<double> Si(Rt), Vt(G.size()), Qt(G.size(), 0.0);

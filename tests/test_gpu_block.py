@@ -102,6 +102,30 @@ def test_tsqr_shapes_match_oracle(sa, orc, method, n, k):
     assert np.max(np.abs(R - Ro)) < 1e-11 * np.abs(Ro).max()
 
 
+@pytest.mark.parametrize("method", ["tsqr", "tsqrchol"])
+def test_tsqr_panels_match_oracle(sa, orc, method):
+    """More than 64 active columns (114 columns after 6 leading orthonormal
+    ones over 40,003 rows: panels of 64 and 50; a BV holds up to 127
+    columns): Q and R equal geqrf/orgqr's (the oracle), signs included; Q
+    orthonormal, X = Q R."""
+    n, l, k = 40003, 6, 120
+    rng = np.random.default_rng(21)
+    X = rng.standard_normal((n, k))
+    X[:, :l] = np.linalg.qr(X[:, :l])[0]
+    V = _bv(sa, X)
+    V.set_active_columns(l, k)
+    R = V.orthogonalize(method)
+    Q = V.get_matrix(k)
+    Vo, Ro = orc.bv_orthogonalize(X.copy(), l, k, METHODS[method])
+    assert np.linalg.norm(Q.T @ Q - np.eye(k)) < 1e-12
+    Rf = R.copy()
+    Rf[:l, :l] = np.eye(l)
+    assert np.linalg.norm(X - Q @ Rf) < 1e-12 * np.linalg.norm(X)
+    assert np.array_equal(np.sign(np.diag(R)[l:]), np.sign(np.diag(Ro)[l:]))
+    assert np.max(np.abs(Q - Vo)) < 1e-10
+    assert np.max(np.abs(R[:, l:] - Ro[:, l:])) < 1e-10 * np.abs(Ro).max()
+
+
 def test_tsqr_ill_conditioned(sa, orc):
     """Columns with condition number 1e13: TSQR keeps Q orthonormal to
     rounding (Householder), where a Cholesky-based QR loses about 26 digits
