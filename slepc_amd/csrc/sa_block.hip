@@ -19,13 +19,13 @@ namespace sa {
 // ---------------------------------------------------------------------------
 constexpr int BD_ROWS = 32, BD_LDP = BD_ROWS + 1, BD_MAXC = 64, BD_MAXG = 1024;
 
-template <typename T, bool NT>
+template <typename T, bool NT, int PC>
 __global__ __launch_bounds__(BLK) void bvdot_kernel(int64_t n, int64_t rpb, const T *__restrict__ X, int64_t ldx,
                                                     int mx, const T *__restrict__ Y, int64_t ldy, int my,
                                                     double *__restrict__ part, int G) {
   typedef typename Vec4<T>::type V4;
-  __shared__ T Ys[BD_MAXC * BD_LDP];
-  __shared__ T Xs[BD_MAXC * BD_LDP];
+  __shared__ T Ys[PC * (BLK / BD_ROWS) * BD_LDP];   // tiles sized for the columns handled
+  __shared__ T Xs[PC * (BLK / BD_ROWS) * BD_LDP];
   const int t = threadIdx.x, lane = t & 63, w = t >> 6, g = lane >> 4, c16 = lane & 15;
   const int nty = (my + 15) / 16, ntx = (mx + 15) / 16, ntile = nty * ntx;
   V4 acc[4];
@@ -33,14 +33,30 @@ __global__ __launch_bounds__(BLK) void bvdot_kernel(int64_t n, int64_t rpb, cons
   for (int q = 0; q < 4; q++) acc[q] = (V4){0, 0, 0, 0};
   const int64_t rb = (int64_t)blockIdx.x * rpb, re = min(n, rb + rpb);
   const int row = t & (BD_ROWS - 1), cg = t / BD_ROWS;      // 8 column groups
+  static_assert(PC * (BLK / BD_ROWS) <= BD_MAXC, "columns per thread and block");
+  // register double buffer: the next chunk's loads are in flight while the
+  // MFMA steps of the current one run (two chunks ahead measured slower: the
+  // extra registers halve the occupancy)
+  T py[PC], px[PC];
+  auto fetch = [&](int64_t r0) {
+    const bool in = r0 + row < re;
+#pragma unroll
+    for (int q = 0; q < PC; q++) {
+      const int i = cg + q * (BLK / BD_ROWS);
+      py[q] = (in && i < my) ? tile_load<NT>(Y + (size_t)i * ldy + r0 + row) : (T)0;
+      px[q] = (in && i < mx) ? tile_load<NT>(X + (size_t)i * ldx + r0 + row) : (T)0;
+    }
+  };
+  if (rb < re) fetch(rb);
   for (int64_t r0 = rb; r0 < re; r0 += BD_ROWS) {
-    const int rows = (int)min((int64_t)BD_ROWS, re - r0);
-    const bool in = row < rows;
-    for (int i = cg; i < my; i += BLK / BD_ROWS)
-      Ys[i * BD_LDP + row] = in ? tile_load<NT>(Y + (size_t)i * ldy + r0 + row) : (T)0;
-    for (int i = cg; i < mx; i += BLK / BD_ROWS)
-      Xs[i * BD_LDP + row] = in ? tile_load<NT>(X + (size_t)i * ldx + r0 + row) : (T)0;
+#pragma unroll
+    for (int q = 0; q < PC; q++) {
+      const int i = cg + q * (BLK / BD_ROWS);
+      if (i < my) Ys[i * BD_LDP + row] = py[q];
+      if (i < mx) Xs[i * BD_LDP + row] = px[q];
+    }
     __syncthreads();
+    if (r0 + BD_ROWS < re) fetch(r0 + BD_ROWS);
 #pragma unroll
     for (int q = 0; q < 4; q++) {
       const int tile = w + 4 * q;
@@ -83,10 +99,17 @@ static int bvdot_block(const BV &X, int lx, int mx, const BV &Y, int ly, int my,
   const T *yp = static_cast<const T *>(Y.data) + (size_t)ly * Y.ld;
   // blocks read once from HBM: non-temporal loads (as the Gram-Schmidt tiles)
   const bool nt = X.n * (int64_t)(mx + my) * (int64_t)sizeof(T) > GS_NT_BYTES;
-  if (nt)
-    hipLaunchKernelGGL((bvdot_kernel<T, true>), dim3(G), dim3(BLK), 0, s, X.n, rpb, xp, X.ld, mx, yp, Y.ld, my, part, G);
+  // registers for 4 columns per thread and block when both blocks have at
+  // most 32 columns (the occupancy the loads in flight need), else 8
+  const bool narrow = mx <= 4 * (BLK / BD_ROWS) && my <= 4 * (BLK / BD_ROWS);
+  if (nt && narrow)
+    hipLaunchKernelGGL((bvdot_kernel<T, true, 4>), dim3(G), dim3(BLK), 0, s, X.n, rpb, xp, X.ld, mx, yp, Y.ld, my, part, G);
+  else if (nt)
+    hipLaunchKernelGGL((bvdot_kernel<T, true, 8>), dim3(G), dim3(BLK), 0, s, X.n, rpb, xp, X.ld, mx, yp, Y.ld, my, part, G);
+  else if (narrow)
+    hipLaunchKernelGGL((bvdot_kernel<T, false, 4>), dim3(G), dim3(BLK), 0, s, X.n, rpb, xp, X.ld, mx, yp, Y.ld, my, part, G);
   else
-    hipLaunchKernelGGL((bvdot_kernel<T, false>), dim3(G), dim3(BLK), 0, s, X.n, rpb, xp, X.ld, mx, yp, Y.ld, my, part, G);
+    hipLaunchKernelGGL((bvdot_kernel<T, false, 8>), dim3(G), dim3(BLK), 0, s, X.n, rpb, xp, X.ld, mx, yp, Y.ld, my, part, G);
   hipLaunchKernelGGL(bvdot_sum_kernel, dim3(my * mx), dim3(BLK), 0, s, (const double *)part, G, out);
   SA_HIP(hipGetLastError());
   return 0;
