@@ -221,7 +221,9 @@ int sa_eps_set_extraction(sa_eps eps, int extraction);
 #define SA_ST_SINVERT 1
 int sa_eps_set_st(sa_eps eps, int st_type, int shift_given, double shift);
 /* KSP of the spectral transformation: BiCGStab with SA_PC_JACOBI (default)
-   or SA_PC_NONE; rtol relative to ||b|| (default 1e-12), max_it (10000). */
+   or SA_PC_NONE; rtol relative to ||b|| (default 1e-12), max_it (10000).
+   Jacobi needs the diagonal of A - sigma I bounded away from zero; a solve
+   that breaks down or does not converge ends the solve with an error. */
 #define SA_PC_NONE 0
 #define SA_PC_JACOBI 1
 int sa_eps_set_st_ksp(sa_eps eps, int pc_type, double rtol, int max_it);

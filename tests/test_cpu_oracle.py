@@ -515,3 +515,57 @@ def test_oracle_sinvert_test2_interior(orc, solver):
     assert np.allclose(r["eigr"][:4], TEST2_INTERIOR[30], atol=5e-6)
     exact = 2 - 2 * np.cos(np.arange(1, 31) * np.pi / 31)
     assert np.allclose(r["eigr"][:4], exact[np.argsort(np.abs(exact - 2.1))[:4]], rtol=1e-9)
+
+
+def _complex_csr(orc, n=300, seed=3):
+    """Sparse non-symmetric matrix with complex conjugate eigenvalue pairs at
+    the wanted end of the spectrum (7 nonzeros per row, diagonal shifted by 4)."""
+    rng = np.random.default_rng(seed)
+    rp, ci, va = [0], [], []
+    for r in range(n):
+        cols = sorted(set([r, (r + 1) % n, (r - 3) % n] + list(rng.choice(n, 3, replace=False))))
+        for c in cols:
+            ci.append(c)
+            va.append(rng.standard_normal() + (4.0 if c == r else 0.0))
+        rp.append(len(ci))
+    return orc.CSR(n, np.array(rp, np.int32), np.array(ci, np.int32), np.array(va))
+
+
+@pytest.mark.parametrize("variant", ["krylovschur", "twosided", "arnoldi", "sinvert"])
+def test_oracle_complex_pairs(orc, variant):
+    """Conjugate pairs through every non-symmetric path: the values equal dense
+    LAPACK's in the criterion's order (largest magnitude; closest to 4 with
+    shift-and-invert), each pair's true residual ||A x - lambda x|| / |lambda|
+    is below 1e-8 (x = re + i im from the two columns), and with two sides
+    the left pair satisfies A^T y = conj(lambda) y."""
+    A = _complex_csr(orc)
+    D = A.to_dense()
+    ev = np.linalg.eigvals(D)
+    kw = dict(twosided=True) if variant == "twosided" else dict(solver="arnoldi") if variant == "arnoldi" else {}
+    if variant == "sinvert":
+        r = orc.eps_solve(A, 4, 0, 1e-10, "TM", False, v0=orc.rand_uniform(0, A.n), target=4.0, st="sinvert", nvec=5)
+        ref = ev[np.argsort(np.abs(ev - 4.0), kind="stable")]
+        nev = 4
+    else:
+        r = orc.eps_solve(A, 6, 24, 1e-10, "LM", False, v0=orc.rand_uniform(0, A.n), max_it=500, nvec=7, **kw)
+        ref = ev[np.argsort(-np.abs(ev), kind="stable")]
+        nev = 6
+    got = r["eigr"][:nev] + 1j * r["eigi"][:nev]
+    assert np.any(r["eigi"][:nev] != 0.0)
+    for z in got:
+        assert np.min(np.abs(ref[:nev + 2] - z)) < 1e-9 * abs(z)
+    i = 0
+    while i < nev:
+        lr, li = r["eigr"][i], r["eigi"][i]
+        if li != 0.0:   # the pair's second column (one past nev when the pair straddles it)
+            xr, xi = r["vecs"][:, i], r["vecs"][:, i + 1]
+            assert orc.residual_rel(A, lr, li, xr, xi) < 1e-8
+            if variant == "twosided":
+                yr, yi = r["lvecs"][:, i], r["lvecs"][:, i + 1]
+                u1 = D.T @ yr - lr * yr - li * yi
+                u2 = D.T @ yi - lr * yi + li * yr
+                assert np.hypot(np.linalg.norm(u1), np.linalg.norm(u2)) < 1e-8 * abs(lr + 1j * li)
+            i += 2
+        else:
+            assert orc.residual_rel(A, lr, 0.0, r["vecs"][:, i]) < 1e-8
+            i += 1
