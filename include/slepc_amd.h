@@ -227,6 +227,19 @@ int sa_eps_set_st(sa_eps eps, int st_type, int shift_given, double shift);
 #define SA_PC_NONE 0
 #define SA_PC_JACOBI 1
 int sa_eps_set_st_ksp(sa_eps eps, int pc_type, double rtol, int max_it);
+/* EPSLanczosSetReorthog / EPSLanczosGetReorthog (lanczos.c:716-775; enum
+   EPSLanczosReorthogType, include/slepceps.h:445): reorthogonalization of
+   SA_EPS_LANCZOS.  FULL (the default here; the reference requires an explicit
+   choice, lanczos.c:54): classical Gram-Schmidt against every column.  LOCAL
+   (lanczos.c:89-118, with the restart steps of lanczos.c:570-575,633-660,
+   678-681): modified Gram-Schmidt against the locked columns and the two
+   previous Lanczos vectors only, purging repeated and spurious Ritz values at
+   each restart.  The selective, periodic, partial and delayed variants are
+   rejected with an error. */
+#define SA_LANCZOS_REORTHOG_LOCAL 0
+#define SA_LANCZOS_REORTHOG_FULL 1
+int sa_eps_lanczos_set_reorthog(sa_eps eps, int reorthog);
+int sa_eps_lanczos_get_reorthog(sa_eps eps, int *reorthog);
 /* EPSSetTwoSided: two-sided Krylov-Schur (ks-twosided.c:126-241) that also
    computes left eigenvectors (y^T A = lambda y^T), with a left basis swept by
    A^T (built from the operator's CSR at the first solve).  Non-symmetric

@@ -14,7 +14,8 @@
  *   EPSConvergedRelative           src/eps/interface/epsdefault.c:224-232
  *   EPSStoppingBasic               src/eps/interface/epsdefault.c:290-302
  *   SlepcSortEigenvalues           src/sys/slepcsc.c:89-140
- *   EPSArnoldi / EPSLanczos(full)  src/eps/impls/krylov/arnoldi/arnoldi.c:31-131, lanczos/lanczos.c:518-697
+ *   EPSArnoldi / EPSLanczos        src/eps/impls/krylov/arnoldi/arnoldi.c:31-131, lanczos/lanczos.c:518-697
+ *   EPSLanczos local (MGS)         lanczos/lanczos.c:58-62,89-118; bvorthog.c:52-85,145-215
  *   BVDot / BVMatMult              src/sys/classes/bv/interface/bvglobal.c:86-120, bvops.c:692-722
  *   BVOrthogonalize (gs/chol/svqb) src/sys/classes/bv/interface/bvorthog.c:489-767, bvlapack.c:136-330
  * The dense projected problems call host LAPACK exactly where the reference does
@@ -481,6 +482,46 @@ static void orthogonalize_gs(int64_t n, double *V, int64_t ld, int j, double *bu
   *lindep = !(nrm != 0.0 && fabs(nrm) >= eta * fabs(onrm));
   *norm = nrm;
   h[j] = *lindep ? 0.0 : nrm;                       /* BV_SetValue(k,k,h,nrm) */
+}
+
+/* One pass of modified Gram-Schmidt (BVOrthogonalizeMGS1, bvorthog.c:52-85) of
+   column j against the columns i < j with which[i] set (all when which is
+   NULL), in increasing order; coefficients added into h. */
+static void mgs1(int64_t n, double *V, int64_t ld, int j, const char *which, double *h, double *onrm, double *nrm)
+{
+  double *w = V + (int64_t)j * ld;
+  if (onrm) *onrm = orc_bv_norm2(n, w);
+  for (int i = 0; i < j; i++) {
+    if (which && !which[i]) continue;
+    const double *vi = V + (int64_t)i * ld;
+    double dot = 0.0;
+    for (int64_t r = 0; r < n; r++) dot += w[r] * vi[r];
+    for (int64_t r = 0; r < n; r++) w[r] -= dot * vi[r];
+    h[i] += dot;
+  }
+  if (nrm) *nrm = orc_bv_norm2(n, w);
+}
+
+/* BVOrthogonalizeGS (bvorthog.c:145-215) with BV_ORTHOG_MGS and refinement
+   IFNEEDED: repeat while ||v|| < eta ||v_before||, at most three passes. */
+static void orthogonalize_mgs(int64_t n, double *V, int64_t ld, int j, const char *which, double *buf, int m,
+                              double eta, double *norm, int *lindep, int64_t *passes)
+{
+  double onrm = 0.0, nrm = 0.0;
+  double *h = buf + (int64_t)j * m;
+  for (int i = 0; i < j; i++) h[i] = 0.0;
+  mgs1(n, V, ld, j, which, h, &onrm, &nrm);
+  (*passes)++;
+  int l = 1;
+  while (l < 3 && nrm != 0.0 && fabs(nrm) < eta * fabs(onrm)) {
+    l++;
+    onrm = nrm;
+    mgs1(n, V, ld, j, which, h, NULL, &nrm);
+    (*passes)++;
+  }
+  *lindep = !(nrm != 0.0 && fabs(nrm) >= eta * fabs(onrm));
+  *norm = nrm;
+  h[j] = *lindep ? 0.0 : nrm;
 }
 
 int orc_bv_orthonormalize_column(int64_t n, double *V, int64_t ld, int j, double *buffer, int m, double eta,
@@ -977,7 +1018,18 @@ typedef struct {
   uint64_t seed;                 /* random start vectors: seed + i (0: 0x5EED) */
   const double *lu;              /* shift-and-invert: LU of A - sigma I (n x n), or NULL */
   const int *piv;
+  int mgs;                       /* BV_ORTHOG_MGS (EPSLanczos other than full, lanczos.c:58-62) */
+  char *which;                   /* column mask of the local Lanczos sweep, m entries */
 } KSo;
+
+/* Orthonormalise column i against the columns before it (BVOrthonormalizeColumn,
+   bvorthog.c:380-426, without replacement): lindep, norm; scaled unless the
+   norm is 0 or 1. */
+static void ks_orthonormalize(KSo *ks, int i, double *norm, int *lindep)
+{
+  if (ks->mgs) orthogonalize_mgs(ks->n, ks->V, ks->ld, i, NULL, ks->buf, ks->m, ks->eta, norm, lindep, &ks->passes);
+  else orthogonalize_gs(ks->n, ks->V, ks->ld, i, ks->buf, ks->m, ks->eta, norm, lindep, &ks->passes);
+}
 
 /* EPSGetStartVector (epssolve.c:841-873) with a caller-provided vector for i==0
    or a deterministic random vector (seed 0x5EED + i) otherwise. */
@@ -986,7 +1038,7 @@ static int ks_start_vector(KSo *ks, int i, int nini, int *breakdown)
   double *vi = ks->V + (int64_t)i * ks->ld;
   if (i > 0 || nini == 0) orc_rand_uniform((ks->seed ? ks->seed : 0x5EEDULL) + (uint64_t)i, ks->n, vi);
   double norm; int lindep;
-  orthogonalize_gs(ks->n, ks->V, ks->ld, i, ks->buf, ks->m, ks->eta, &norm, &lindep, &ks->passes);
+  ks_orthonormalize(ks, i, &norm, &lindep);
   if (breakdown) *breakdown = lindep;
   else if (lindep || norm == 0.0) return -1;
   orc_bv_scale(ks->n, vi, 1.0 / norm);
@@ -994,19 +1046,47 @@ static int ks_start_vector(KSo *ks, int i, int nini, int *breakdown)
 }
 
 /* BVMatArnoldi / BVMatLanczos (bvkrylov.c:56-113,165-226) */
+/* y = Op x: A x, or (A - sigma I)^-1 x for STSINVERT with PREONLY + LU */
+static void ks_op(KSo *ks, const double *x, double *y)
+{
+  if (ks->lu) {
+    int nn = ks->n, one = 1, info = 0;
+    memcpy(y, x, sizeof(double) * (size_t)nn);
+    LP.getrs("N", &nn, &one, ks->lu, &nn, ks->piv, y, &nn, &info, 1);
+  } else {
+    orc_spmv(ks->n, ks->rp, ks->ci, ks->val, x, y);
+  }
+  ks->matvecs++;
+}
+
+/* EPSLocalLanczos (lanczos.c:89-118): column j+1 = Op v_j orthogonalised by
+   MGS against the columns before k and v_{j-1}, v_j only; the coefficient
+   buffer gets alpha_j (row j) and beta_j = the norm (row j+1, also on
+   breakdown) of column j+1. */
+static void ks_expand_local(KSo *ks, int k, int *m, double *beta, int *breakdown)
+{
+  int lindep = 0;
+  char *which = ks->which;
+  for (int i = 0; i < k; i++) which[i] = 1;
+  for (int j = k; j < *m; j++) {
+    ks_op(ks, ks->V + (int64_t)j * ks->ld, ks->V + (int64_t)(j + 1) * ks->ld);
+    which[j] = 1;
+    if (j - 2 >= k) which[j - 2] = 0;
+    double nrm;
+    orthogonalize_mgs(ks->n, ks->V, ks->ld, j + 1, which, ks->buf, ks->m, ks->eta, &nrm, &lindep, &ks->passes);
+    ks->buf[j + 1 + (int64_t)(j + 1) * ks->m] = nrm;
+    *beta = nrm;
+    if (lindep) { *m = j + 1; break; }
+    orc_bv_scale(ks->n, ks->V + (int64_t)(j + 1) * ks->ld, 1.0 / nrm);
+  }
+  *breakdown = lindep;
+}
+
 static void ks_expand(KSo *ks, int k, int *m, double *beta, int *breakdown)
 {
   int lindep = 0;
   for (int j = k; j < *m; j++) {
-    if (ks->lu) {                /* STApply of STSINVERT with PREONLY + LU: (A - sigma I)^-1 v */
-      double *y = ks->V + (int64_t)(j + 1) * ks->ld;
-      int nn = ks->n, one = 1, info = 0;
-      memcpy(y, ks->V + (int64_t)j * ks->ld, sizeof(double) * (size_t)nn);
-      LP.getrs("N", &nn, &one, ks->lu, &nn, ks->piv, y, &nn, &info, 1);
-    } else {
-      orc_spmv(ks->n, ks->rp, ks->ci, ks->val, ks->V + (int64_t)j * ks->ld, ks->V + (int64_t)(j + 1) * ks->ld);
-    }
-    ks->matvecs++;
+    ks_op(ks, ks->V + (int64_t)j * ks->ld, ks->V + (int64_t)(j + 1) * ks->ld);
     double nrm;
     orthogonalize_gs(ks->n, ks->V, ks->ld, j + 1, ks->buf, ks->m, ks->eta, &nrm, &lindep, &ks->passes);
     if (nrm != 1.0 && nrm != 0.0) orc_bv_scale(ks->n, ks->V + (int64_t)(j + 1) * ks->ld, 1.0 / nrm);
@@ -1071,6 +1151,8 @@ int orc_eps_solve(int n, const int32_t *rp, const int32_t *ci, const double *val
   ks.n = n; ks.rp = rp; ks.ci = ci; ks.val = val; ks.m = ncv + 1; ks.ld = n;
   ks.eta = o->eta > 0 ? o->eta : 0.7071;
   ks.lu = lu; ks.piv = piv;
+  ks.mgs = lanc && o->lanczos_local;
+  ks.which = (char *)calloc(ks.m + 1, 1);
   ks.V = (double *)calloc((size_t)n * ks.m, sizeof(double));
   ks.buf = (double *)calloc((size_t)ks.m * ks.m, sizeof(double));
   int ld = ncv + 1;
@@ -1084,7 +1166,7 @@ int orc_eps_solve(int n, const int32_t *rp, const int32_t *ci, const double *val
   ds.perm = (int *)calloc(ld, sizeof(int));
   ds.iwork = (int *)calloc(ld, sizeof(int));
   double *eigr = (double *)calloc(ld, sizeof(double)), *eigi = (double *)calloc(ld, sizeof(double));
-  double *errest = (double *)calloc(ld, sizeof(double));
+  double *errest = (double *)calloc(ld, sizeof(double)), *lockd = (double *)calloc(ld, sizeof(double));
   int *perm = (int *)calloc(ld, sizeof(int));
   for (int i = 0; i < ncv; i++) perm[i] = i;
   int rc = 0, nini = 0;
@@ -1094,7 +1176,7 @@ int orc_eps_solve(int n, const int32_t *rp, const int32_t *ci, const double *val
   if (v0) {
     memcpy(ks.V, v0, sizeof(double) * (size_t)n);
     double norm; int lindep;
-    orthogonalize_gs(n, ks.V, ks.ld, 0, ks.buf, ks.m, ks.eta, &norm, &lindep, &ks.passes);
+    ks_orthonormalize(&ks, 0, &norm, &lindep);
     if (norm == 0.0 || lindep) { rc = -3; goto done; }
     orc_bv_scale(n, ks.V, 1.0 / norm);
     nini = 1;
@@ -1108,7 +1190,8 @@ int orc_eps_solve(int n, const int32_t *rp, const int32_t *ci, const double *val
     int nv = nconv + mpd < ncv ? nconv + mpd : ncv;
     ds.n = nv; ds.t = nv; ds.l = nconv; ds.k = (arn || lanc) ? 0 : nconv + l;   /* DSSetDimensions */
     double beta = 0.0; int breakdown = 0, k0 = nconv + l;
-    ks_expand(&ks, k0, &nv, &beta, &breakdown);
+    if (ks.mgs) ks_expand_local(&ks, k0, &nv, &beta, &breakdown);
+    else ks_expand(&ks, k0, &nv, &beta, &breakdown);
     int bm = ks.m;
     if (herm) {
       for (int j = k0; j < nv; j++) { d[j] = ks.buf[j + (int64_t)(j + 1) * bm]; e[j] = ks.buf[j + 1 + (int64_t)(j + 1) * bm]; }
@@ -1121,6 +1204,11 @@ int orc_eps_solve(int n, const int32_t *rp, const int32_t *ci, const double *val
     ds.n = nv; ds.t = nv; ds.l = nconv; ds.k = (arn || lanc) ? 0 : nconv + l;
     ds.state = l ? ST_RAW : ST_INTERMEDIATE;
     if (harm && ds_translate_harmonic(&ds, o->target, beta, 0, gh, &gamma)) { rc = -70; goto done; }
+    /* EPSLanczos keeps the projected eigenvalues (ritz) apart from eps->eigr
+       and stores only the entries from nconv on (lanczos.c:663-666): the
+       leading diagonal of the projected problem is not that of the locked
+       values once the converged values are moved first */
+    if (lanc) memcpy(lockd, eigr, sizeof(double) * (size_t)nconv);
     int info = herm ? ds_solve_hep(&ds, eigr, eigi) : ds_solve_nhep(&ds, eigr, eigi);
     if (info) { rc = -20 - info; goto done; }
     if (lanc) {
@@ -1132,6 +1220,9 @@ int orc_eps_solve(int n, const int32_t *rp, const int32_t *ci, const double *val
         bnd[i] = eigr[i] != 0.0 ? res / fabs(eigr[i]) : 1.79e308;
         cv[i] = bnd[i] < tol ? 'C' : 'N';
       }
+      if (o->lanczos_local)                                  /* purge repeated Ritz values (:570-575) */
+        for (int i = nconv + 1; i < nv; i++)
+          if (cv[i] == 'C' && fabs((eigr[i] - eigr[i - 1]) / eigr[i]) < tol) cv[i] = 'R';
       if (!breakdown) {                                      /* restart vector (lanczos.c:577-593) */
         int rs = nconv;
         while (rs < nv && cv[rs] != 'N') rs++;
@@ -1166,12 +1257,46 @@ int orc_eps_solve(int n, const int32_t *rp, const int32_t *ci, const double *val
         }
       }
       if (k > nconv) orc_bv_multinplace(n, ks.V, ks.ld, nconv, nv, nconv, k, Y, ld);
+      if (o->lanczos_local) {                                /* purge spurious Ritz values (:633-660) */
+        double *w = (double *)malloc(sizeof(double) * (size_t)n);
+        for (int i = nconv; i < k; i++) {
+          double *vi = ks.V + (int64_t)i * ks.ld;
+          orc_bv_scale(n, vi, 1.0 / orc_bv_norm2(n, vi));
+          ks_op(&ks, vi, w);                                 /* STApply: the operator of the ST */
+          for (int64_t q = 0; q < n; q++) w[q] -= eigr[i] * vi[q];
+          double nr = orc_bv_norm2(n, w);
+          bnd[i] = eigr[i] != 0.0 ? nr / fabs(eigr[i]) : 1.79e308;
+          if (bnd[i] >= tol) cv[i] = 'S';
+        }
+        int i;
+        for (i = nconv; i < k; i++) {
+          if (cv[i] != 'C') {
+            int j = i + 1;
+            while (j < k && cv[j] != 'C') j++;
+            if (j >= k) break;
+            double td = eigr[j]; eigr[j] = eigr[i]; eigr[i] = td;
+            td = bnd[j]; bnd[j] = bnd[i]; bnd[i] = td;
+            char tc = cv[j]; cv[j] = cv[i]; cv[i] = tc;
+            double *a = ks.V + (int64_t)i * ks.ld, *b = ks.V + (int64_t)j * ks.ld;
+            for (int64_t q = 0; q < n; q++) { td = a[q]; a[q] = b[q]; b[q] = td; }
+          }
+        }
+        k = i;
+        free(w);
+      }
       for (int i = nconv; i < nv; i++) errest[i] = bnd[i];
+      memcpy(eigr, lockd, sizeof(double) * (size_t)nconv);
       nconv = k;
       if (k >= nev) reason = 1;
       else if (its >= max_it) reason = -1;
       if (reason == 0) {                                     /* BVCopyColumn(V,nv,nconv) (:826) */
         memcpy(ks.V + (int64_t)nconv * ks.ld, ks.V + (int64_t)nv * ks.ld, sizeof(double) * (size_t)n);
+        if (o->lanczos_local && !breakdown) {                /* reorthonormalise the restart vector (:678-681) */
+          double nrm; int lind;
+          ks_orthonormalize(&ks, nconv, &nrm, &lind);
+          if (nrm != 1.0 && nrm != 0.0) orc_bv_scale(n, ks.V + (int64_t)nconv * ks.ld, 1.0 / nrm);
+          if (lind) breakdown = 1;
+        }
         if (breakdown) {
           int bd = 0;
           ks_start_vector(&ks, nconv, nini, &bd);
@@ -1318,7 +1443,7 @@ done:
 #endif
   free(gh); free(lu); free(piv);
   cmp_sinv = 0;
-  free(ks.V); free(ks.buf); free(ds.A); free(ds.Q); free(ds.X); free(ds.T); free(ds.work); free(ds.perm); free(ds.iwork);
+  free(lockd); free(ks.V); free(ks.buf); free(ks.which); free(ds.A); free(ds.Q); free(ds.X); free(ds.T); free(ds.work); free(ds.perm); free(ds.iwork);
   free(eigr); free(eigi); free(errest); free(perm);
   free(bnd); free(yr); free(cv); free(lp);
   return rc;

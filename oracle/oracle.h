@@ -81,6 +81,7 @@ typedef struct {
   double  target;      /* EPSSetTarget: which 7/8 and the harmonic shift */
   int     st;          /* 0 shift (sigma = 0), 1 shift-and-invert (STSINVERT) */
   double  sigma;       /* shift of the shift-and-invert operator */
+  int     lanczos_local; /* Lanczos: 1 -eps_lanczos_reorthog local, 0 full */
 } orc_opts;
 
 typedef struct {

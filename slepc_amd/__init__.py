@@ -374,6 +374,16 @@ class EPS:
         Jacobi ("jacobi") or no ("none") preconditioner."""
         check(lib().sa_eps_set_st_ksp(self.h, {"none": 0, "jacobi": 1}[pc], float(rtol), int(max_it)))
 
+    def setLanczosReorthog(self, reorthog):
+        """EPSLanczosSetReorthog: "local" or "full" (the default here)."""
+        check(lib().sa_eps_lanczos_set_reorthog(self.h, {"local": 0, "full": 1}[reorthog]))
+
+    def getLanczosReorthog(self):
+        """EPSLanczosGetReorthog: "local" or "full"."""
+        v = C.c_int()
+        check(lib().sa_eps_lanczos_get_reorthog(self.h, C.byref(v)))
+        return ["local", "full"][v.value]
+
     def setTwoSided(self, twosided=True):
         """EPSSetTwoSided: also compute left eigenvectors (two-sided Krylov-Schur)."""
         check(lib().sa_eps_set_two_sided(self.h, 1 if twosided else 0))

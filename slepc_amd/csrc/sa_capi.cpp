@@ -518,6 +518,21 @@ int sa_eps_set_st_ksp(sa_eps E, int pc_type, double rtol, int max_it) {
   return 0;
 }
 
+int sa_eps_lanczos_set_reorthog(sa_eps E, int reorthog) {
+  SA_CHECK(E, "null EPS");
+  SA_CHECK(reorthog == SA_LANCZOS_REORTHOG_LOCAL || reorthog == SA_LANCZOS_REORTHOG_FULL,
+           "unsupported Lanczos reorthogonalization type (local and full are available)");
+  E->p.lanczos_local = reorthog == SA_LANCZOS_REORTHOG_LOCAL;
+  E->solved = false;
+  return 0;
+}
+
+int sa_eps_lanczos_get_reorthog(sa_eps E, int *reorthog) {
+  SA_CHECK(E && reorthog, "null argument");
+  *reorthog = E->p.lanczos_local ? SA_LANCZOS_REORTHOG_LOCAL : SA_LANCZOS_REORTHOG_FULL;
+  return 0;
+}
+
 int sa_eps_set_two_sided(sa_eps E, int twosided) {
   SA_CHECK(E, "null EPS");
   E->twosided = twosided != 0;

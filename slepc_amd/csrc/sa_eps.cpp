@@ -10,6 +10,7 @@
 #include <chrono>
 #include <cmath>
 #include <cstring>
+#include <functional>
 
 #include "sa_ds.h"
 
@@ -77,11 +78,19 @@ int eps_start_vector(BV &V, int i, bool have_initial, double eta, int *breakdown
 // wanted unconverged Ritz vector as the next start vector (built in column
 // nv, then copied to the new column nconv), converged pairs moved to the
 // front, V(:,nconv:k) = V(:,nconv:nv) Y(nconv:nv, nconv:k), nconv = k.
+// Local reorthogonalization (p.lanczos_local, -eps_lanczos_reorthog local):
+// with the default classical Gram-Schmidt the column mask of EPSLocalLanczos
+// (lanczos.c:89-118) is not applied (BVOrthogonalizeCGS1, bvorthog.c:91-132,
+// ignores it), so the sweep is the full one; the variant adds the restart
+// steps of lanczos.c:570-575 (repeated Ritz values), 633-660 (spurious Ritz
+// values: true residual of each converged vector through the operator op)
+// and 678-681 (reorthonormalised restart vector).  wtmp: one device vector.
 static int lanczos_restart(const KsParams &p, BV &V, const DenseProblem &ds, int &nconv, int nv, double beta,
                            bool breakdown, std::vector<double> &eigr, std::vector<double> &errest, double &anorm,
                            int its, int &reason, std::vector<double> &scratch, std::vector<double> &bnd,
                            std::vector<char> &conv, std::vector<int> &perm, std::vector<double> &Y,
-                           std::vector<double> &yrst, KsResult &r) {
+                           std::vector<double> &yrst, KsResult &r,
+                           const std::function<int(const void *, void *)> &op, void *wtmp) {
   hipStream_t s = stream();
   const int ld = ds.ld();
   const double es = (double)prec_size(V.prec);
@@ -94,6 +103,9 @@ static int lanczos_restart(const KsParams &p, BV &V, const DenseProblem &ds, int
     bnd[i] = eigr[i] != 0.0 ? res / std::fabs(eigr[i]) : 1.79e308;   // EPSConvergedRelative
     conv[i] = bnd[i] < p.tol ? 'C' : 'N';
   }
+  if (p.lanczos_local)   // purge repeated Ritz values
+    for (int i = nconv + 1; i < nv; i++)
+      if (conv[i] == 'C' && std::fabs((eigr[i] - eigr[i - 1]) / eigr[i]) < p.tol) conv[i] = 'R';
   if (!breakdown) {
     int rs = nconv;
     while (rs < nv && conv[rs] != 'N') rs++;
@@ -141,6 +153,38 @@ static int lanczos_restart(const KsParams &p, BV &V, const DenseProblem &ds, int
     SA_HIP(hipStreamSynchronize(s));
     r.bytes_restart += ((double)(nv - nconv) + (double)(k - nconv)) * V.n * es;
   }
+  if (p.lanczos_local && k > nconv) {   // purge spurious Ritz values
+    double h = 0.0;
+    for (int i = nconv; i < k; i++) {
+      void *vi = col_ptr(V, i);
+      SA_TRY(launch_dot_self(V.prec, vi, V.n, V.ws.dots, V.ws.part, s));
+      SA_HIP(hipMemcpyAsync(&h, V.ws.dots, sizeof(double), hipMemcpyDeviceToHost, s));
+      SA_HIP(hipStreamSynchronize(s));
+      SA_TRY(launch_scale(V.prec, vi, V.n, 1.0 / std::sqrt(h), s));
+      SA_TRY(op(vi, wtmp));
+      SA_TRY(launch_axpy_norm(V.prec, wtmp, vi, eigr[i], V.n, V.ws.part, V.ws.dots, s));
+      SA_HIP(hipMemcpyAsync(&h, V.ws.dots, sizeof(double), hipMemcpyDeviceToHost, s));
+      SA_HIP(hipStreamSynchronize(s));
+      const double nr = std::sqrt(h);
+      bnd[i] = eigr[i] != 0.0 ? nr / std::fabs(eigr[i]) : 1.79e308;
+      if (bnd[i] >= p.tol) conv[i] = 'S';
+    }
+    int i;
+    for (i = nconv; i < k; i++) {
+      if (conv[i] != 'C') {
+        int j = i + 1;
+        while (j < k && conv[j] != 'C') j++;
+        if (j >= k) break;
+        std::swap(eigr[j], eigr[i]);
+        std::swap(bnd[j], bnd[i]);
+        std::swap(conv[j], conv[i]);
+        SA_HIP(hipMemcpyAsync(wtmp, col_ptr(V, i), es * V.n, hipMemcpyDeviceToDevice, s));
+        SA_HIP(hipMemcpyAsync(col_ptr(V, i), col_ptr(V, j), es * V.n, hipMemcpyDeviceToDevice, s));
+        SA_HIP(hipMemcpyAsync(col_ptr(V, j), wtmp, es * V.n, hipMemcpyDeviceToDevice, s));
+      }
+    }
+    k = i;
+  }
   for (int i = nconv; i < nv; i++) errest[i] = bnd[i];
   nconv = k;
   if (k >= p.nev) reason = 1;
@@ -148,6 +192,15 @@ static int lanczos_restart(const KsParams &p, BV &V, const DenseProblem &ds, int
   if (reason == 0) {   // BVCopyColumn(V,nv,nconv); a random vector after a breakdown
     SA_HIP(hipMemcpyAsync(col_ptr(V, nconv), col_ptr(V, nv), es * V.n, hipMemcpyDeviceToDevice, s));
     r.bytes_bv += 2.0 * V.n * es;
+    if (p.lanczos_local && !breakdown) {   // reorthonormalise the restart vector (MGS, all columns)
+      std::vector<int> cols(nconv);
+      for (int i = 0; i < nconv; i++) cols[i] = i;
+      double nrm = 0.0;
+      int lind = 0;
+      SA_TRY(bv_mgs_column(V, nconv, cols.data(), nconv, p.eta, &nrm, &lind));
+      if (nrm != 1.0 && nrm != 0.0) SA_TRY(launch_scale(V.prec, col_ptr(V, nconv), V.n, 1.0 / nrm, s));
+      if (lind) breakdown = true;
+    }
     if (breakdown) {
       int bd = 0;
       SA_TRY(eps_start_vector(V, nconv, false, p.eta, &bd, scratch));
@@ -193,7 +246,48 @@ int ks_solve(const KsParams &p, const Mat &A, BV &V, bool have_initial, KsResult
     ~KspFree() { ksp_free(w); }
   } kw_free{kw};
   if (sinv) SA_TRY(ksp_alloc(kw, A, p.st_shift, p.ksp_jacobi));
+  // y = Op x: the operator of the spectral transformation (local Lanczos)
+  std::function<int(const void *, void *)> op_apply = [&](const void *x, void *y) -> int {
+    if (!sinv) return launch_spmv(A, x, y, stream());
+    int its = 0;
+    return ksp_solve(A, kw, (const double *)x, (double *)y, p.ksp_rtol, p.ksp_max_it, &its);
+  };
+  struct DevBuf {
+    void *p = nullptr;
+    ~DevBuf() { if (p) (void)hipFree(p); }
+  } wtmp;
+  if (lanc && p.lanczos_local) SA_HIP(hipMalloc(&wtmp.p, prec_size(V.prec) * (size_t)std::max(1, n)));
+  // EPSLocalLanczos (lanczos.c:89-118): V(:,j+1) = Op V(:,j) orthogonalised
+  // by MGS against the columns before k and V(:,j-1), V(:,j) only; the
+  // coefficient buffer gets alpha_j (row j) and beta_j = the norm (row j+1,
+  // also on breakdown), as the full sweep leaves them.
+  auto sweep_local = [&](int k, int *m, double *beta, int *bd) -> int {
+    std::vector<int> cols;
+    int lindep = 0;
+    for (int j = k; j < *m; j++) {
+      SA_TRY(op_apply(col_ptr(V, j), col_ptr(V, j + 1)));
+      cols.clear();
+      for (int i = 0; i < k; i++) cols.push_back(i);
+      if (j - 1 >= k) cols.push_back(j - 1);
+      cols.push_back(j);
+      double nrm = 0.0;
+      SA_TRY(bv_mgs_column(V, j + 1, cols.data(), (int)cols.size(), p.eta, &nrm, &lindep));
+      SA_HIP(hipMemcpyAsync(V.dbuf + (size_t)(j + 1) * V.m + j + 1, &nrm, sizeof(double), hipMemcpyHostToDevice,
+                            stream()));
+      SA_HIP(hipStreamSynchronize(stream()));
+      r.bytes_bv += (3.0 * cols.size() + 3.0) * V.n * prec_size(V.prec);
+      *beta = nrm;
+      if (lindep) {
+        *m = j + 1;
+        break;
+      }
+      SA_TRY(launch_scale(V.prec, col_ptr(V, j + 1), V.n, 1.0 / nrm, stream()));
+    }
+    *bd = lindep;
+    return 0;
+  };
   auto sweep = [&](int k0, int *nv, double *beta, int *bd) -> int {
+    if (lanc && p.lanczos_local) return sweep_local(k0, nv, beta, bd);
     if (!sinv) return bv_sweep(V, A, k0, nv, p.eta, beta, bd);
     return bv_sweep_sinvert(V, A, kw, p.ksp_rtol, p.ksp_max_it, k0, nv, p.eta, beta, bd, &r.ksp_its);
   };
@@ -206,7 +300,7 @@ int ks_solve(const KsParams &p, const Mat &A, BV &V, bool have_initial, KsResult
   ds.allocate(ld, dsherm, p.which);
   ds.set_target(p.target);
   ds.set_sinvert(sinv, p.st_shift);
-  std::vector<double> eigr(ld, 0.0), eigi(ld, 0.0), errest(ld, 0.0), buf, scratch;
+  std::vector<double> eigr(ld, 0.0), eigi(ld, 0.0), errest(ld, 0.0), buf, scratch, locked;
   const double t0 = now();
   double tds = 0.0;
   r = KsResult();
@@ -237,7 +331,7 @@ int ks_solve(const KsParams &p, const Mat &A, BV &V, bool have_initial, KsResult
     // Algorithmic bytes of the sweep: SpMV, then per Gram-Schmidt pass the dot
     // products read columns 0..c-1 and the update reads them again plus v and
     // writes v, i.e. (2c+2) n s per pass; passes averaged over the sweep.
-    if (nv > k0) {
+    if (nv > k0 && !(lanc && p.lanczos_local)) {   // (the local sweep counts its own bytes)
       const double avgp = (double)(V.passes_done - pbefore) / (double)(nv - k0);
       for (int c = k0 + 1; c <= nv; c++) r.bytes_bv += avgp * (2.0 * c + 2.0) * V.n * es;
     }
@@ -269,13 +363,19 @@ int ks_solve(const KsParams &p, const Mat &A, BV &V, bool have_initial, KsResult
       const int hinfo = ds.translate_harmonic(p.target, beta, false, g.data(), &gamma);
       SA_CHECK(hinfo == 0, "LAPACK failure in the harmonic translation (getrf/getrs)");
     }
+    // EPSLanczos keeps the projected eigenvalues apart from eps->eigr and
+    // stores only the entries from nconv on (lanczos.c:663-666): the leading
+    // diagonal of the projected problem is not that of the locked values once
+    // the converged values have been moved first.
+    if (lanc) locked.assign(eigr.begin(), eigr.begin() + nconv);
     int info = ds.solve(eigr.data(), eigi.data());
     SA_CHECK(info == 0, "LAPACK failure in the projected problem (solve)");
     info = ds.sort(eigr.data(), eigi.data());
     SA_CHECK(info == 0, "LAPACK failure in the projected problem (sort)");
     if (lanc) {
       SA_TRY(lanczos_restart(p, V, ds, nconv, nv, beta, breakdown != 0, eigr, errest, anorm, its, reason, scratch,
-                             bnd, conv, lperm, Yl, yrst, r));
+                             bnd, conv, lperm, Yl, yrst, r, op_apply, wtmp.p));
+      std::copy(locked.begin(), locked.end(), eigr.begin());
       tds += now() - td0;
       if (p.monitor) p.monitor(p.monitor_ctx, its, nconv, eigr.data(), errest.data(), nv);
       continue;

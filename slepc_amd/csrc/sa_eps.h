@@ -19,6 +19,7 @@ struct KsParams {
   bool ksp_jacobi = true;      // PCJACOBI for the BiCGStab solves (else none)
   double ksp_rtol = 1e-12;     // relative residual of each solve
   int ksp_max_it = 10000;
+  bool lanczos_local = false;  // EPSLanczosSetReorthog: local (else full)
   void (*monitor)(void *, int, int, const double *, const double *, int) = nullptr;
   void *monitor_ctx = nullptr;
 };

@@ -151,6 +151,10 @@ int bv_alloc(BV &V, int64_t n, int m, int prec);
 void bv_free(BV &V);
 int bv_sweep(BV &V, const Mat &A, int k, int *m, double eta, double *beta, int *breakdown);
 int bv_orthonormalize(BV &V, int j, double eta, double *norm, int *lindep);
+// Modified Gram-Schmidt of column j against the columns cols[0..nc) (each < j,
+// in that order), refinement IFNEEDED (sa_mgs.hip): coefficients in the
+// coefficient buffer column j, row j = the norm (0 on lindep); not scaled.
+int bv_mgs_column(BV &V, int j, const int *cols, int nc, double eta, double *norm, int *lindep);
 // BVOrthogonalize of the active columns (method 0 gs, 1 chol, 2 svqb); R: host
 // column-major, leading dimension ldr >= V.k, or null
 int bv_orthogonalize(BV &V, int method, double *R, int ldr);

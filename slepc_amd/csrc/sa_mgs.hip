@@ -1,0 +1,146 @@
+// Modified Gram-Schmidt of one basis column against a chosen set of columns
+// (BV_ORTHOG_MGS: BVOrthogonalizeMGS1, src/sys/classes/bv/interface/
+// bvorthog.c:52-85, with refinement IFNEEDED, bvorthog.c:145-215), as used by
+// EPSLanczos with local reorthogonalization (lanczos.c:58-62,89-118): each
+// step needs the coefficient of the previous column before it can start, so
+// one column costs two stream-ordered launches, a fused "subtract the previous
+// column, partial dot with the next" pass over n rows and a one-workgroup
+// finish that leaves the coefficient in device memory for the next pass.  The
+// host reads the two norms once per pass.  Partial sums are reduced in a
+// fixed order, so the result does not depend on timing.
+#include <algorithm>
+#include <cmath>
+
+#include "sa_device.h"
+#include "sa_internal.h"
+
+namespace sa {
+
+namespace {
+
+__device__ __forceinline__ double block_total(double v, double *red) {
+  red[threadIdx.x] = v;
+  __syncthreads();
+  for (int w = BLK / 2; w > 0; w >>= 1) {
+    if ((int)threadIdx.x < w) red[threadIdx.x] += red[threadIdx.x + w];
+    __syncthreads();
+  }
+  return red[0];
+}
+
+// w -= (*coef) vp (when vp is set), then part[b] = partial sum of w . vn, or
+// of w . w when vn is null.  The dot uses the stored (rounded) w.
+template <typename T>
+__global__ __launch_bounds__(BLK) void mgs_step_kernel(T *w, const T *vp, const double *coef, const T *vn,
+                                                       int64_t n, double *part) {
+  __shared__ double red[BLK];
+  const double c = vp ? *coef : 0.0;
+  double acc = 0.0;
+  for (int64_t i = (int64_t)blockIdx.x * BLK + threadIdx.x; i < n; i += (int64_t)gridDim.x * BLK) {
+    T x = w[i];
+    if (vp) {
+      x = (T)((double)x - c * (double)vp[i]);
+      w[i] = x;
+    }
+    const double xd = (double)x;
+    acc += vn ? xd * (double)vn[i] : xd * xd;
+  }
+  const double s = block_total(acc, red);
+  if (threadIdx.x == 0) part[blockIdx.x] = s;
+}
+
+// s = sum of part[0..np); dot mode (h set): h += s and *slot = s (the
+// coefficient of the next subtraction); norm mode: *slot = sqrt(s), and
+// hn = sqrt(s) when hn is set.
+__global__ __launch_bounds__(BLK) void mgs_finish_kernel(const double *part, int np, double *h, double *slot,
+                                                         double *hn) {
+  __shared__ double red[BLK];
+  double a = 0.0;
+  for (int i = threadIdx.x; i < np; i += BLK) a += part[i];
+  const double s = block_total(a, red);
+  if (threadIdx.x == 0) {
+    if (h) {
+      *h += s;
+      *slot = s;
+    } else {
+      const double r = safe_sqrt(s);
+      *slot = r;
+      if (hn) *hn = r;
+    }
+  }
+}
+
+}  // namespace
+
+// One MGS pass: column j of V against the columns cols[0..nc) in order, each
+// coefficient added into h[cols[q]]; onrm (device) gets ||v|| before, nrm
+// (device) ||v|| after; hn (device, or null) also gets ||v|| after.
+static int mgs_pass(BV &V, int j, const int *cols, int nc, double *h, double *slot, double *onrm, double *nrm,
+                    double *hn, hipStream_t s) {
+  const int64_t n = V.n;
+  const int64_t cap = 2 * (int64_t)V.ws.maxG * (GS_MAXC + 1);
+  const int grid = (int)std::max<int64_t>(1, std::min<int64_t>({1024, (n + BLK - 1) / BLK, cap}));
+  double *part = V.ws.part;
+  auto step = [&](int prev, int next) {
+    if (V.prec == SA_FP64) {
+      const double *vp = prev >= 0 ? (const double *)col_ptr(V, prev) : nullptr;
+      const double *vn = next >= 0 ? (const double *)col_ptr(V, next) : nullptr;
+      hipLaunchKernelGGL(mgs_step_kernel<double>, dim3(grid), dim3(BLK), 0, s, (double *)col_ptr(V, j), vp, slot,
+                         vn, n, part);
+    } else {
+      const float *vp = prev >= 0 ? (const float *)col_ptr(V, prev) : nullptr;
+      const float *vn = next >= 0 ? (const float *)col_ptr(V, next) : nullptr;
+      hipLaunchKernelGGL(mgs_step_kernel<float>, dim3(grid), dim3(BLK), 0, s, (float *)col_ptr(V, j), vp, slot,
+                         vn, n, part);
+    }
+  };
+  if (onrm) {
+    step(-1, -1);
+    hipLaunchKernelGGL(mgs_finish_kernel, dim3(1), dim3(BLK), 0, s, part, grid, nullptr, onrm, nullptr);
+  }
+  int prev = -1;
+  for (int q = 0; q < nc; q++) {
+    step(prev, cols[q]);
+    hipLaunchKernelGGL(mgs_finish_kernel, dim3(1), dim3(BLK), 0, s, part, grid, h + cols[q], slot, nullptr);
+    prev = cols[q];
+  }
+  step(prev, -1);
+  hipLaunchKernelGGL(mgs_finish_kernel, dim3(1), dim3(BLK), 0, s, part, grid, nullptr, nrm, hn);
+  SA_HIP(hipGetLastError());
+  V.passes_done++;
+  return 0;
+}
+
+int bv_mgs_column(BV &V, int j, const int *cols, int nc, double eta, double *norm, int *lindep) {
+  SA_CHECK(j >= 0 && j < V.m && nc >= 0, "column index out of range");
+  for (int q = 0; q < nc; q++) SA_CHECK(cols[q] >= 0 && cols[q] < j, "orthogonalization column out of range");
+  hipStream_t s = stream();
+  const int m = V.m;
+  double *h = V.dbuf + (size_t)j * m;                 // coefficients of column j
+  double *sc = V.dq + (size_t)m * m;                  // scalar slots past the staging area
+  double *slot = sc, *nrm_d = sc + 1, *onrm_d = sc + 2;
+  SA_HIP(hipMemsetAsync(h, 0, sizeof(double) * (size_t)(j + 1), s));   // BV_CleanCoefficients
+  double hs[2];
+  SA_TRY(mgs_pass(V, j, cols, nc, h, slot, onrm_d, nrm_d, nullptr, s));
+  SA_HIP(hipMemcpyAsync(hs, nrm_d, 2 * sizeof(double), hipMemcpyDeviceToHost, s));
+  SA_HIP(hipStreamSynchronize(s));
+  double nrm = hs[0], onrm = hs[1];
+  int l = 1;
+  while (l < 3 && nrm != 0.0 && std::fabs(nrm) < eta * std::fabs(onrm)) {
+    l++;
+    onrm = nrm;
+    SA_TRY(mgs_pass(V, j, cols, nc, h, slot, nullptr, nrm_d, nullptr, s));
+    SA_HIP(hipMemcpyAsync(hs, nrm_d, sizeof(double), hipMemcpyDeviceToHost, s));
+    SA_HIP(hipStreamSynchronize(s));
+    nrm = hs[0];
+  }
+  const int ld = !(nrm != 0.0 && std::fabs(nrm) >= eta * std::fabs(onrm));
+  const double hj = ld ? 0.0 : nrm;
+  SA_HIP(hipMemcpyAsync(h + j, &hj, sizeof(double), hipMemcpyHostToDevice, s));
+  SA_HIP(hipStreamSynchronize(s));
+  if (norm) *norm = nrm;
+  if (lindep) *lindep = ld;
+  return 0;
+}
+
+}  // namespace sa

@@ -372,16 +372,23 @@ TEST2_LARGEST = [3.98974, 3.95906, 3.90828, 3.83792]
 TEST2_SMALLEST = [0.01026, 0.04094, 0.09172, 0.16208]
 
 
-@pytest.mark.parametrize("solver", ["arnoldi", "lanczos"])
+def _solver(name):
+    """"lanczos-local" -> EPSLANCZOS with -eps_lanczos_reorthog local."""
+    if name == "lanczos-local":
+        return dict(solver="lanczos", reorthog="local")
+    return dict(solver=name)
+
+
+@pytest.mark.parametrize("solver", ["arnoldi", "lanczos", "lanczos-local"])
 @pytest.mark.parametrize("which", ["LR", "SR"])
 def test_oracle_explicit_restart_test2(orc, which, solver):
     """eps/tests/test2.c with -eps_type arnoldi and with -eps_type lanczos
-    -eps_lanczos_reorthog full: 1-D Laplacian n = 30, nev 4, tol
-    max(1000 eps, 1e-9): the values printed by the reference (test2_1.out)
+    -eps_lanczos_reorthog full / local (test2_2): 1-D Laplacian n = 30, nev 4,
+    tol max(1000 eps, 1e-9): the values printed by the reference (test2_1.out)
     and the closed form 2 - 2 cos(k pi / 31)."""
     A = _lap1d(orc, 30)
     tol = max(1000 * 2.220446049250313e-16, 1e-9)
-    r = orc.eps_solve(A, 4, 0, tol, which, True, v0=orc.rand_uniform(0, A.n), solver=solver)
+    r = orc.eps_solve(A, 4, 0, tol, which, True, v0=orc.rand_uniform(0, A.n), **_solver(solver))
     assert r["nconv"] >= 4
     gold = TEST2_LARGEST if which == "LR" else TEST2_SMALLEST
     assert list(np.round(r["eigr"][:4], 5)) == gold
@@ -419,6 +426,34 @@ def test_oracle_lanczos_stencil27(orc):
     ref = np.array(LIBRES["stencil27_10"]["largest_mag32"])
     for v in r["eigr"][:8]:
         assert np.min(np.abs(ref - v)) < 1e-12 * abs(v)
+
+
+# eps/tests/test8.c (2-D Laplacian of ex2, matrix-free; the same values for
+# -eps_type lanczos -eps_lanczos_reorthog local): test8_1.out, test8_2.out,
+# test8_3.out -- grid, nev, ncv, mpd, which, max_it, printed values
+TEST8 = {
+    "1": (20, 4, 11, 0, "LM", 40000, [7.95532, 7.88881, 7.88881, 7.82229]),
+    "2": (10, 4, 0, 0, "SR", 600, [0.16203, 0.39851, 0.39851, 0.63499]),
+    "3": (10, 12, 21, 9, "SR", 1000, [0.16203, 0.39851, 0.39851, 0.63499, 0.77129, 0.77129, 1.00777, 1.00777,
+                                      1.25018, 1.25018, 1.38056, 1.48666]),
+}
+
+
+@pytest.mark.parametrize("case", sorted(TEST8))
+def test_oracle_lanczos_local_test8(orc, case):
+    """EPSLanczos with local reorthogonalization on the cases of test8.c
+    (tol 1e-8; case 3 sets mpd 9, so ncv = nev + mpd = 21 as
+    EPSSetDimensions_Default gives): the printed values, double eigenvalues
+    included, and the closed form 4 - 2 cos(i pi/(m+1)) - 2 cos(j pi/(m+1))."""
+    m, nev, ncv, mpd, which, max_it, gold = TEST8[case]
+    A = orc.lap2d(m)
+    r = orc.eps_solve(A, nev, ncv, 1e-8, which, True, max_it=max_it, mpd=mpd, solver="lanczos", reorthog="local")
+    assert r["nconv"] >= nev
+    assert list(np.round(r["eigr"][:nev], 5)) == gold
+    c = 2 * np.cos(np.arange(1, m + 1) * np.pi / (m + 1))
+    ex = np.sort((4 - c[:, None] - c[None, :]).ravel())
+    ex = ex[::-1][:nev] if which == "LM" else ex[:nev]
+    assert np.max(np.abs(r["eigr"][:nev] - ex)) < 1e-7 * np.max(np.abs(ex))
 
 
 TEST2_INTERIOR = {30: [2.10130, 1.89870, 2.30286, 2.50131], 32: [2.09516, 2.28463, 1.90484, 2.47152]}
@@ -504,13 +539,16 @@ def test_oracle_sinvert_test9_5(orc, solver):
     assert np.allclose(r["eigr"][:3], near.real, rtol=1e-9)
 
 
-@pytest.mark.parametrize("solver", ["lanczos", "krylovschur"])
+@pytest.mark.parametrize("solver", ["lanczos", "lanczos-local", "krylovschur"])
 def test_oracle_sinvert_test2_interior(orc, solver):
     """eps/tests/test2.c interior eigenvalues with Lanczos and shift-and-invert
-    at target 2.1 (A - 2.1 I indefinite; test2_1.out) and the closed form."""
+    at target 2.1 (A - 2.1 I indefinite; test2_1.out) and the closed form.
+    With local reorthogonalization the operator's dominant eigenvalue
+    (1/(2.1013 - 2.1)) comes back as ghost copies within one sweep; they are
+    purged as repeated or spurious, so each eigenvalue appears once."""
     A = _lap1d(orc, 30)
     r = orc.eps_solve(A, 4, 0, 1e-9, "TM", True, v0=orc.rand_uniform(0, 30), target=2.1, st="sinvert",
-                      solver=solver)
+                      **_solver(solver))
     assert r["nconv"] >= 4
     assert np.allclose(r["eigr"][:4], TEST2_INTERIOR[30], atol=5e-6)
     exact = 2 - 2 * np.cos(np.arange(1, 31) * np.pi / 31)

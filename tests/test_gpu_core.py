@@ -424,6 +424,65 @@ def test_explicit_restart_matches_oracle(sa, orc, solver, case):
         assert E.computeError(i) < 10 * tol
 
 
+@pytest.mark.parametrize("case", ["test2_LR", "test2_SR", "test2_interior", "test8_1", "test8_2", "test8_3",
+                                  "stencil27_10"])
+def test_lanczos_local_matches_oracle(sa, orc, case):
+    """EPSLanczos with local reorthogonalization (MGS against the locked
+    columns and the two previous Lanczos vectors, purging of repeated and
+    spurious Ritz values) through the C ABI against the oracle from the same
+    start vector: eigenvalues to 1e-9 and the values printed by the
+    reference.  test2: 1-D Laplacian n = 30 (test2_1.out, the interior case
+    with shift-and-invert at 2.1, where ghost copies of the dominant value
+    are purged); test8_*: 2-D Laplacian (test8_1.out .. test8_3.out, double
+    eigenvalues included); stencil27_10: 8 largest magnitude, ncv 24.
+    Without orthogonality the restart counts depend on rounding, so they are
+    not compared."""
+    from test_cpu_oracle import _lap1d, TEST2_LARGEST, TEST2_SMALLEST, TEST2_INTERIOR, TEST8
+    sinv, gold, mpd, v0 = False, None, 0, None
+    if case.startswith("test2"):
+        Ao, nev, ncv, tol, mi = _lap1d(orc, 30), 4, 0, 1e-9, 0
+        which = {"test2_LR": "LR", "test2_SR": "SR", "test2_interior": "TM"}[case]
+        gold = {"LR": TEST2_LARGEST, "SR": TEST2_SMALLEST, "TM": TEST2_INTERIOR[30]}[which]
+        sinv = which == "TM"
+        v0 = orc.rand_uniform(0, Ao.n)
+    elif case.startswith("test8"):
+        m, nev, ncv, mpd, which, mi, gold = TEST8[case[-1]]
+        Ao, tol = orc.lap2d(m), 1e-8
+    else:
+        Ao, which, nev, ncv, tol, mi = orc.stencil27(10, seed=1), "LM", 8, 24, 1e-10, 2000
+        v0 = orc.rand_uniform(0, Ao.n)
+    if v0 is None:
+        v0 = orc.rand_uniform(0, Ao.n)
+    E = sa.EPS()
+    E.setType("lanczos")
+    with pytest.raises(Exception):   # selective: not available
+        sa._lib.check(sa._lib.lib().sa_eps_lanczos_set_reorthog(E.h, 2))
+    E.setLanczosReorthog("local")
+    assert E.getLanczosReorthog() == "local"
+    E.setOperators(sa.Mat(sa.CSR(Ao.n, Ao.rowptr, Ao.col, Ao.val)))
+    E.setProblemType(True)
+    E.setWhichEigenpairs(which)
+    if sinv:
+        E.setTarget(2.1)
+        E.setST("sinvert")
+        E.setSTKSP("jacobi", 1e-13, 20000)
+    E.setDimensions(nev, ncv, mpd)
+    E.setTolerances(tol, mi)
+    E.setInitialSpace(v0)
+    E.solve()
+    kw = dict(target=2.1, st="sinvert") if sinv else {}
+    r = orc.eps_solve(Ao, nev, ncv, tol, which, True, v0=v0, solver="lanczos", reorthog="local", max_it=mi,
+                      mpd=mpd, **kw)
+    assert E.getConverged() >= nev and r["nconv"] >= nev
+    vals = np.array([E.getEigenpair(i, False)[0] for i in range(nev)])
+    assert np.max(np.abs(vals - r["eigr"][:nev]) / np.abs(r["eigr"][:nev])) < 1e-9
+    if gold is not None:
+        assert np.allclose(vals, gold, atol=5e-6)
+    bound = 10 * tol * (max(1.0, 4.2 / np.min(np.abs(vals))) if sinv else 1.0)
+    for i in range(nev):
+        assert E.computeError(i) < bound
+
+
 @pytest.mark.parametrize("solver,case", [("krylovschur", "test2_30"), ("krylovschur", "test2_32"),
                                          ("arnoldi", "test2_30"), ("arnoldi", "test2_32"),
                                          ("krylovschur", "convdiff8")])
