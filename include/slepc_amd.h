@@ -234,10 +234,16 @@ int sa_eps_set_st_ksp(sa_eps eps, int pc_type, double rtol, int max_it);
    (lanczos.c:89-118, with the restart steps of lanczos.c:570-575,633-660,
    678-681): modified Gram-Schmidt against the locked columns and the two
    previous Lanczos vectors only, purging repeated and spurious Ritz values at
-   each restart.  The selective, periodic, partial and delayed variants are
-   rejected with an error. */
+   each restart.  PERIODIC and PARTIAL (EPSPartialLanczos, lanczos.c:338-441):
+   local steps plus a second MGS against the columns of the current sweep
+   when the omega recurrence estimates a loss of orthogonality above
+   sqrt(eps/ncv) (periodic: all of them, twice in a row; partial: the
+   intervals around the largest estimates).  The selective and delayed
+   variants are rejected with an error. */
 #define SA_LANCZOS_REORTHOG_LOCAL 0
 #define SA_LANCZOS_REORTHOG_FULL 1
+#define SA_LANCZOS_REORTHOG_PERIODIC 3
+#define SA_LANCZOS_REORTHOG_PARTIAL 4
 int sa_eps_lanczos_set_reorthog(sa_eps eps, int reorthog);
 int sa_eps_lanczos_get_reorthog(sa_eps eps, int *reorthog);
 /* EPSSetTwoSided: two-sided Krylov-Schur (ks-twosided.c:126-241) that also

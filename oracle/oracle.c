@@ -1082,6 +1082,123 @@ static void ks_expand_local(KSo *ks, int k, int *m, double *beta, int *breakdown
   *breakdown = lindep;
 }
 
+/* update_omega (lanczos.c:260-293): the omega recurrence of partial
+   reorthogonalization.  B(x) is the reference's beta argument, which the
+   caller passes shifted by one (lanczos.c:398), so B(x) = beta[x - 1]; alpha
+   and omega are indexed from 0 whatever the number of locked columns.  The
+   final exchange of the reference copies omega_old into omega and leaves
+   omega_old as it is (its second assignment reads the value just written). */
+static void update_omega(double *omega, double *omega_old, int j, const double *alpha, const double *beta,
+                         double eps1, double anorm)
+{
+#define B(x) beta[(x) - 1]
+  const double T = eps1 * anorm, binv = 1.0 / B(j + 1);
+  omega_old[0] = B(1) * omega[1] + (alpha[0] - alpha[j]) * omega[0] - B(j) * omega_old[0];
+  omega_old[0] = omega_old[0] > 0 ? binv * (omega_old[0] + T) : binv * (omega_old[0] - T);
+  for (int q = 1; q < j - 1; q++) {
+    omega_old[q] = B(q + 1) * omega[q + 1] + (alpha[q] - alpha[j]) * omega[q] + B(q) * omega[q - 1] - B(j) * omega_old[q];
+    omega_old[q] = omega_old[q] > 0 ? binv * (omega_old[q] + T) : binv * (omega_old[q] - T);
+  }
+  omega_old[j - 1] = binv * T;
+  for (int q = 0; q < j; q++) omega[q] = omega_old[q];
+  omega[j] = eps1;
+#undef B
+}
+
+/* compute_int (lanczos.c:295-333): which[i] for |mu_i| >= delta (else the
+   largest), widened left and right while |mu| >= eta. */
+static void compute_int(char *which, const double *mu, int j, double delta, double eta)
+{
+  int found = 0, maxpos = 0;
+  double mx = 0.0;
+  for (int i = 0; i < j; i++) {
+    if (fabs(mu[i]) >= delta) { which[i] = 1; found = 1; }
+    else which[i] = 0;
+    if (fabs(mu[i]) > mx) { maxpos = i; mx = fabs(mu[i]); }
+  }
+  if (!found) which[maxpos] = 1;
+  for (int i = 0; i < j; i++) {
+    if (which[i]) {
+      for (int q = i; q >= 0; q--) {
+        if (fabs(mu[q]) < eta || which[q]) break;
+        which[q] = 1;
+      }
+      for (int q = i + 1; q < j; q++) {
+        if (fabs(mu[q]) < eta || which[q]) break;
+        which[q] = 1;
+      }
+    }
+  }
+}
+
+/* EPSPartialLanczos (lanczos.c:338-441), periodic or partial: a local step
+   (MGS against the locked columns and v_{j-1}, v_j), the omega estimate of the
+   loss of orthogonality, and when it exceeds delta a second MGS against the
+   columns k..j (periodic: all of them, and the next step too; partial: the
+   intervals of compute_int); once norm * delta < anorm * eps1 the rest of
+   the sweep orthogonalises against every column.  alpha, beta: the projected
+   problem's diagonal and off-diagonal (locked entries included, as the
+   recurrence reads them).  anorm < 0: estimated from the recurrence. */
+static void ks_expand_partial(KSo *ks, int k, int *M, double *betam, int *breakdown, double anorm, int periodic,
+                              double *alpha, double *beta)
+{
+  const int m = *M, n = ks->n, ncv = ks->m - 1, sz = (m > 100 ? m : 100) + 1;
+  const double meps = 2.220446049250313e-16;
+  const double eps1 = sqrt((double)n) * meps / 2, delta = sqrt(meps) / sqrt((double)ncv);
+  const double eta = pow(meps, 0.75) / sqrt((double)ncv);
+  double *omega = (double *)calloc(sz, sizeof(double)), *omega_old = (double *)calloc(sz, sizeof(double));
+  char *which = (char *)calloc(sz, 1), *which2 = (char *)calloc(sz, 1);
+  int estimate = 0, force = 0, fro = 0, lindep = 0;
+  if (anorm < 0.0) { anorm = 1.0; estimate = 1; }
+  for (int i = 0; i < k; i++) which[i] = 1;
+  double norm = 0.0;
+  for (int j = k; j < m; j++) {
+    ks_op(ks, ks->V + (int64_t)j * ks->ld, ks->V + (int64_t)(j + 1) * ks->ld);
+    const double *h = ks->buf + (int64_t)(j + 1) * ks->m;
+    if (fro) {
+      orthogonalize_mgs(n, ks->V, ks->ld, j + 1, NULL, ks->buf, ks->m, ks->eta, &norm, &lindep, &ks->passes);
+      alpha[j] = h[j];
+    } else {
+      which[j] = 1;
+      if (j - 2 >= k) which[j - 2] = 0;
+      orthogonalize_mgs(n, ks->V, ks->ld, j + 1, which, ks->buf, ks->m, ks->eta, &norm, &lindep, &ks->passes);
+      alpha[j] = h[j];
+      beta[j] = norm;
+      if (estimate) {
+        if (j > k) anorm = fmax(anorm, fabs(alpha[j]) + norm + beta[j - 1]);
+        else anorm = fmax(anorm, fabs(alpha[j]) + norm);
+      }
+      int reorth = 0;
+      if (j > k) {
+        update_omega(omega, omega_old, j, alpha, beta, eps1, anorm);
+        for (int i = 0; i < j - k; i++) if (fabs(omega[i]) > delta) reorth = 1;
+      }
+      if (reorth || force) {
+        for (int i = 0; i < k; i++) which2[i] = 0;
+        for (int i = k; i <= j; i++) which2[i] = 1;
+        if (periodic) {
+          force = !force;
+          for (int i = 0; i < j - k; i++) omega[i] = eps1;
+        } else if (force) {
+          force = 0;
+        } else {
+          force = 1;
+          compute_int(which2 + k, omega, j - k, delta, eta);
+          for (int i = 0; i < j - k; i++) if (which2[i + k]) omega[i] = eps1;
+        }
+        orthogonalize_mgs(n, ks->V, ks->ld, j + 1, which2, ks->buf, ks->m, ks->eta, &norm, &lindep, &ks->passes);
+      }
+    }
+    if (lindep || norm < n * anorm * meps) { *M = j + 1; break; }
+    if (!fro && norm * delta < anorm * eps1) fro = 1;
+    beta[j] = norm;
+    orc_bv_scale(n, ks->V + (int64_t)(j + 1) * ks->ld, 1.0 / norm);
+  }
+  *betam = beta[*M - 1];
+  *breakdown = lindep;
+  free(omega); free(omega_old); free(which); free(which2);
+}
+
 static void ks_expand(KSo *ks, int k, int *m, double *beta, int *breakdown)
 {
   int lindep = 0;
@@ -1151,7 +1268,7 @@ int orc_eps_solve(int n, const int32_t *rp, const int32_t *ci, const double *val
   ks.n = n; ks.rp = rp; ks.ci = ci; ks.val = val; ks.m = ncv + 1; ks.ld = n;
   ks.eta = o->eta > 0 ? o->eta : 0.7071;
   ks.lu = lu; ks.piv = piv;
-  ks.mgs = lanc && o->lanczos_local;
+  ks.mgs = lanc && (o->lanczos_local || o->lanczos_partial);
   ks.which = (char *)calloc(ks.m + 1, 1);
   ks.V = (double *)calloc((size_t)n * ks.m, sizeof(double));
   ks.buf = (double *)calloc((size_t)ks.m * ks.m, sizeof(double));
@@ -1190,10 +1307,14 @@ int orc_eps_solve(int n, const int32_t *rp, const int32_t *ci, const double *val
     int nv = nconv + mpd < ncv ? nconv + mpd : ncv;
     ds.n = nv; ds.t = nv; ds.l = nconv; ds.k = (arn || lanc) ? 0 : nconv + l;   /* DSSetDimensions */
     double beta = 0.0; int breakdown = 0, k0 = nconv + l;
-    if (ks.mgs) ks_expand_local(&ks, k0, &nv, &beta, &breakdown);
+    const int partial = lanc && o->lanczos_partial;
+    if (partial) ks_expand_partial(&ks, k0, &nv, &beta, &breakdown, anorm, o->lanczos_partial == 1, d, e);
+    else if (ks.mgs) ks_expand_local(&ks, k0, &nv, &beta, &breakdown);
     else ks_expand(&ks, k0, &nv, &beta, &breakdown);
     int bm = ks.m;
-    if (herm) {
+    if (partial) {
+      /* the sweep wrote the projected problem itself */
+    } else if (herm) {
       for (int j = k0; j < nv; j++) { d[j] = ks.buf[j + (int64_t)(j + 1) * bm]; e[j] = ks.buf[j + 1 + (int64_t)(j + 1) * bm]; }
     } else {
       double *H = ds.A;

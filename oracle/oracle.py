@@ -23,7 +23,8 @@ class Opts(C.Structure):
                 ("tol", C.c_double), ("keep", C.c_double), ("eta", C.c_double),
                 ("lock", C.c_int), ("nthreads", C.c_int), ("solver", C.c_int),
                 ("extraction", C.c_int), ("target", C.c_double),
-                ("st", C.c_int), ("sigma", C.c_double), ("lanczos_local", C.c_int)]
+                ("st", C.c_int), ("sigma", C.c_double), ("lanczos_local", C.c_int),
+                ("lanczos_partial", C.c_int)]
 
 
 class Result(C.Structure):
@@ -269,7 +270,8 @@ def eps_solve(A, nev, ncv=0, tol=1e-8, which="LM", hermitian=True, v0=None, max_
     o = Opts(hermitian=1 if hermitian else 0, which=WHICH[which], nev=nev, ncv=ncv, mpd=mpd,
              extraction={"ritz": 0, "harmonic": 1}[extraction], target=float(target),
              st={"shift": 0, "sinvert": 1}[st], sigma=float(target if sigma is None else sigma),
-             lanczos_local={"full": 0, "local": 1}[reorthog],
+             lanczos_local={"full": 0, "local": 1, "periodic": 0, "partial": 0}[reorthog],
+             lanczos_partial={"full": 0, "local": 0, "periodic": 1, "partial": 2}[reorthog],
              max_it=max_it, tol=tol, keep=keep, eta=0.7071, lock=lock, nthreads=nthreads,
              solver={"krylovschur": 0, "arnoldi": 1, "lanczos": 2}[solver])
     ncv_eff = ncv if ncv else min(A.n, max(2 * nev, nev + 15))

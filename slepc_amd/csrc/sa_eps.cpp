@@ -78,6 +78,143 @@ int eps_start_vector(BV &V, int i, bool have_initial, double eta, int *breakdown
 // wanted unconverged Ritz vector as the next start vector (built in column
 // nv, then copied to the new column nconv), converged pairs moved to the
 // front, V(:,nconv:k) = V(:,nconv:nv) Y(nconv:nv, nconv:k), nconv = k.
+// ---- EPSPartialLanczos (lanczos.c:338-441): periodic and partial ----------
+// The omega recurrence (update_omega, lanczos.c:260-293) estimates the loss of
+// orthogonality of the next Lanczos vector against each earlier one; its
+// beta argument is the caller's shifted by one, so B(x) = beta[x - 1], and
+// alpha / omega are indexed from 0 whatever the number of locked columns.  The
+// reference's closing exchange copies omega_old into omega and leaves
+// omega_old unchanged; kept as it is.
+static void lz_update_omega(double *omega, double *omega_old, int j, const double *alpha, const double *beta,
+                            double eps1, double anorm) {
+  auto B = [&](int x) { return beta[x - 1]; };
+  const double T = eps1 * anorm, binv = 1.0 / B(j + 1);
+  omega_old[0] = B(1) * omega[1] + (alpha[0] - alpha[j]) * omega[0] - B(j) * omega_old[0];
+  omega_old[0] = omega_old[0] > 0 ? binv * (omega_old[0] + T) : binv * (omega_old[0] - T);
+  for (int q = 1; q < j - 1; q++) {
+    omega_old[q] = B(q + 1) * omega[q + 1] + (alpha[q] - alpha[j]) * omega[q] + B(q) * omega[q - 1] -
+                   B(j) * omega_old[q];
+    omega_old[q] = omega_old[q] > 0 ? binv * (omega_old[q] + T) : binv * (omega_old[q] - T);
+  }
+  omega_old[j - 1] = binv * T;
+  for (int q = 0; q < j; q++) omega[q] = omega_old[q];
+  omega[j] = eps1;
+}
+
+// compute_int (lanczos.c:295-333): columns with |mu| >= delta (else the
+// largest), widened on both sides while |mu| >= eta.
+static void lz_compute_int(char *which, const double *mu, int j, double delta, double eta) {
+  bool found = false;
+  int maxpos = 0;
+  double mx = 0.0;
+  for (int i = 0; i < j; i++) {
+    which[i] = std::fabs(mu[i]) >= delta;
+    found = found || which[i];
+    if (std::fabs(mu[i]) > mx) {
+      maxpos = i;
+      mx = std::fabs(mu[i]);
+    }
+  }
+  if (!found) which[maxpos] = 1;
+  for (int i = 0; i < j; i++) {
+    if (!which[i]) continue;
+    for (int q = i; q >= 0; q--) {
+      if (std::fabs(mu[q]) < eta || which[q]) break;
+      which[q] = 1;
+    }
+    for (int q = i + 1; q < j; q++) {
+      if (std::fabs(mu[q]) < eta || which[q]) break;
+      which[q] = 1;
+    }
+  }
+}
+
+// The sweep: a local step (MGS against the locked columns and v_{j-1}, v_j),
+// the omega update, and when some |omega| exceeds delta a second MGS against
+// columns k..j (periodic: all, and again at the next step; partial: the
+// intervals of compute_int, then all at the next step); after
+// norm * delta < anorm * eps1 every later step orthogonalises against all
+// columns.  alpha / beta: the projected problem's diagonal and off-diagonal,
+// written in place (the recurrence also reads the locked entries).
+static int lz_sweep_partial(BV &V, const std::function<int(const void *, void *)> &op, int k, int *M,
+                            double *betam, int *breakdown, double anorm, bool periodic, int ncv, double eta_bv,
+                            double *alpha, double *beta, KsResult &r) {
+  hipStream_t s = stream();
+  const int m = *M;
+  const double n = (double)V.n, meps = DBL_EPSILON;
+  const double eps1 = std::sqrt(n) * meps / 2, delta = std::sqrt(meps) / std::sqrt((double)ncv);
+  const double eta = std::pow(meps, 0.75) / std::sqrt((double)ncv);
+  const size_t sz = (size_t)std::max(m, 100) + 1;
+  std::vector<double> omega(sz, 0.0), omega_old(sz, 0.0);
+  std::vector<char> which(sz, 0), which2(sz, 0);
+  const bool estimate = anorm < 0.0;
+  if (estimate) anorm = 1.0;
+  for (int i = 0; i < k; i++) which[i] = 1;
+  bool force = false, fro = false;
+  int lindep = 0;
+  double norm = 0.0;
+  std::vector<int> cols;
+  auto mgs = [&](int c, const char *mask) -> int {   // column c against the masked columns < c
+    cols.clear();
+    for (int i = 0; i < c; i++)
+      if (!mask || mask[i]) cols.push_back(i);
+    SA_TRY(bv_mgs_column(V, c, cols.data(), (int)cols.size(), eta_bv, &norm, &lindep));
+    r.bytes_bv += (4.0 * cols.size() + 4.0) * V.n * prec_size(V.prec);
+    return 0;
+  };
+  auto coef = [&](int c, int row, double *out) -> int {   // coefficient buffer entry (row, column c)
+    SA_HIP(hipMemcpyAsync(out, V.dbuf + (size_t)c * V.m + row, sizeof(double), hipMemcpyDeviceToHost, s));
+    SA_HIP(hipStreamSynchronize(s));
+    return 0;
+  };
+  for (int j = k; j < m; j++) {
+    SA_TRY(op(col_ptr(V, j), col_ptr(V, j + 1)));
+    if (fro) {
+      SA_TRY(mgs(j + 1, nullptr));
+      SA_TRY(coef(j + 1, j, &alpha[j]));
+    } else {
+      which[j] = 1;
+      if (j - 2 >= k) which[j - 2] = 0;
+      SA_TRY(mgs(j + 1, which.data()));
+      SA_TRY(coef(j + 1, j, &alpha[j]));
+      beta[j] = norm;
+      if (estimate) anorm = std::max(anorm, std::fabs(alpha[j]) + norm + (j > k ? beta[j - 1] : 0.0));
+      bool reorth = false;
+      if (j > k) {
+        lz_update_omega(omega.data(), omega_old.data(), j, alpha, beta, eps1, anorm);
+        for (int i = 0; i < j - k; i++)
+          if (std::fabs(omega[i]) > delta) reorth = true;
+      }
+      if (reorth || force) {
+        for (int i = 0; i < k; i++) which2[i] = 0;
+        for (int i = k; i <= j; i++) which2[i] = 1;
+        if (periodic) {
+          force = !force;
+          for (int i = 0; i < j - k; i++) omega[i] = eps1;
+        } else if (force) {
+          force = false;
+        } else {
+          force = true;
+          lz_compute_int(which2.data() + k, omega.data(), j - k, delta, eta);
+          for (int i = 0; i < j - k; i++)
+            if (which2[i + k]) omega[i] = eps1;
+        }
+        SA_TRY(mgs(j + 1, which2.data()));
+      }
+    }
+    if (lindep || norm < n * anorm * meps) {
+      *M = j + 1;
+      break;
+    }
+    if (!fro && norm * delta < anorm * eps1) fro = true;
+    beta[j] = norm;
+    SA_TRY(launch_scale(V.prec, col_ptr(V, j + 1), V.n, 1.0 / norm, s));
+  }
+  *betam = beta[*M - 1];
+  *breakdown = lindep;
+  return 0;
+}
+
 // Local reorthogonalization (p.lanczos_local, -eps_lanczos_reorthog local):
 // with the default classical Gram-Schmidt the column mask of EPSLocalLanczos
 // (lanczos.c:89-118) is not applied (BVOrthogonalizeCGS1, bvorthog.c:91-132,
@@ -326,12 +463,17 @@ int ks_solve(const KsParams &p, const Mat &A, BV &V, bool have_initial, KsResult
     double beta = 0.0;
     int breakdown = 0;
     const int64_t pbefore = V.passes_done;
-    SA_TRY(sweep(k0, &nv, &beta, &breakdown));
+    const bool partial = lanc && p.lanczos_partial != 0;
+    if (partial)
+      SA_TRY(lz_sweep_partial(V, op_apply, k0, &nv, &beta, &breakdown, anorm, p.lanczos_partial == 1, ncv, p.eta,
+                              ds.d(), ds.e(), r));
+    else
+      SA_TRY(sweep(k0, &nv, &beta, &breakdown));
     r.matvecs += nv - k0;
     // Algorithmic bytes of the sweep: SpMV, then per Gram-Schmidt pass the dot
     // products read columns 0..c-1 and the update reads them again plus v and
     // writes v, i.e. (2c+2) n s per pass; passes averaged over the sweep.
-    if (nv > k0 && !(lanc && p.lanczos_local)) {   // (the local sweep counts its own bytes)
+    if (nv > k0 && !(lanc && (p.lanczos_local || partial))) {   // (the MGS sweeps count their own bytes)
       const double avgp = (double)(V.passes_done - pbefore) / (double)(nv - k0);
       for (int c = k0 + 1; c <= nv; c++) r.bytes_bv += avgp * (2.0 * c + 2.0) * V.n * es;
     }
@@ -344,7 +486,9 @@ int ks_solve(const KsParams &p, const Mat &A, BV &V, bool have_initial, KsResult
     SA_HIP(hipStreamSynchronize(s));
     const double td0 = now();
     auto B = [&](int row, int colc) { return buf[(size_t)(colc - k0 - 1) * m + row]; };
-    if (dsherm) {
+    if (partial) {
+      // the sweep wrote the projected problem itself
+    } else if (dsherm) {
       double *d = ds.d(), *e = ds.e();
       for (int j = k0; j < nv; j++) {
         d[j] = B(j, j + 1);

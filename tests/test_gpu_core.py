@@ -424,12 +424,15 @@ def test_explicit_restart_matches_oracle(sa, orc, solver, case):
         assert E.computeError(i) < 10 * tol
 
 
+@pytest.mark.parametrize("reorthog", ["local", "periodic", "partial"])
 @pytest.mark.parametrize("case", ["test2_LR", "test2_SR", "test2_interior", "test8_1", "test8_2", "test8_3",
                                   "stencil27_10"])
-def test_lanczos_local_matches_oracle(sa, orc, case):
+def test_lanczos_local_matches_oracle(sa, orc, case, reorthog):
     """EPSLanczos with local reorthogonalization (MGS against the locked
     columns and the two previous Lanczos vectors, purging of repeated and
-    spurious Ritz values) through the C ABI against the oracle from the same
+    spurious Ritz values) and with periodic / partial reorthogonalization
+    (local steps plus MGS against the sweep's columns when the omega
+    estimate calls for it) through the C ABI against the oracle from the same
     start vector: eigenvalues to 1e-9 and the values printed by the
     reference.  test2: 1-D Laplacian n = 30 (test2_1.out, the interior case
     with shift-and-invert at 2.1, where ghost copies of the dominant value
@@ -457,8 +460,8 @@ def test_lanczos_local_matches_oracle(sa, orc, case):
     E.setType("lanczos")
     with pytest.raises(Exception):   # selective: not available
         sa._lib.check(sa._lib.lib().sa_eps_lanczos_set_reorthog(E.h, 2))
-    E.setLanczosReorthog("local")
-    assert E.getLanczosReorthog() == "local"
+    E.setLanczosReorthog(reorthog)
+    assert E.getLanczosReorthog() == reorthog
     E.setOperators(sa.Mat(sa.CSR(Ao.n, Ao.rowptr, Ao.col, Ao.val)))
     E.setProblemType(True)
     E.setWhichEigenpairs(which)
@@ -471,7 +474,7 @@ def test_lanczos_local_matches_oracle(sa, orc, case):
     E.setInitialSpace(v0)
     E.solve()
     kw = dict(target=2.1, st="sinvert") if sinv else {}
-    r = orc.eps_solve(Ao, nev, ncv, tol, which, True, v0=v0, solver="lanczos", reorthog="local", max_it=mi,
+    r = orc.eps_solve(Ao, nev, ncv, tol, which, True, v0=v0, solver="lanczos", reorthog=reorthog, max_it=mi,
                       mpd=mpd, **kw)
     assert E.getConverged() >= nev and r["nconv"] >= nev
     vals = np.array([E.getEigenpair(i, False)[0] for i in range(nev)])
