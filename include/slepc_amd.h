@@ -238,10 +238,13 @@ int sa_eps_set_st_ksp(sa_eps eps, int pc_type, double rtol, int max_it);
    local steps plus a second MGS against the columns of the current sweep
    when the omega recurrence estimates a loss of orthogonality above
    sqrt(eps/ncv) (periodic: all of them, twice in a row; partial: the
-   intervals around the largest estimates).  The selective and delayed
-   variants are rejected with an error. */
+   intervals around the largest estimates).  SELECTIVE (EPSSelectiveLanczos,
+   lanczos.c:181-258): local steps plus orthogonalization against the Ritz
+   vectors of the current sweep whose bound is below sqrt(eps) ||A||.  The
+   delayed variant is rejected with an error. */
 #define SA_LANCZOS_REORTHOG_LOCAL 0
 #define SA_LANCZOS_REORTHOG_FULL 1
+#define SA_LANCZOS_REORTHOG_SELECTIVE 2
 #define SA_LANCZOS_REORTHOG_PERIODIC 3
 #define SA_LANCZOS_REORTHOG_PARTIAL 4
 int sa_eps_lanczos_set_reorthog(sa_eps eps, int reorthog);

@@ -255,6 +255,9 @@ int64_t orc_gen_markov(int m, int32_t *rp, int32_t *ci, double *v)
 typedef void (*f_lartg)(const double *, const double *, double *, double *, double *);
 typedef void (*f_rot)(const int *, double *, const int *, double *, const int *, const double *, const double *);
 typedef void (*f_steqr)(const char *, const int *, double *, double *, double *, const int *, double *, int *, size_t);
+typedef void (*f_stevr)(const char *, const char *, const int *, double *, double *, const double *, const double *,
+                        const int *, const int *, const double *, int *, double *, double *, const int *, int *,
+                        double *, const int *, int *, const int *, int *, size_t, size_t);
 typedef void (*f_gehrd)(const int *, const int *, const int *, double *, const int *, double *, double *, const int *, int *);
 typedef void (*f_orghr)(const int *, const int *, const int *, double *, const int *, const double *, double *, const int *, int *);
 typedef void (*f_hseqr)(const char *, const char *, const int *, const int *, const int *, double *, const int *, double *, double *,
@@ -275,6 +278,7 @@ typedef void (*f_orgqr)(const int *, const int *, const int *, double *, const i
 static struct {
   int tried, ok;
   char path[512];
+  f_stevr stevr;   /* optional: selective Lanczos */
   f_lartg lartg; f_rot rot; f_steqr steqr; f_gehrd gehrd; f_orghr orghr; f_hseqr hseqr; f_trexc trexc; f_trevc trevc;
   f_potrf potrf; f_trtri trtri; f_syev syev; f_geqrf geqrf; f_orgqr orgqr;   /* block orthogonalisation only */
   f_getrf getrf; f_getrs getrs;                                               /* harmonic extraction only */
@@ -293,7 +297,7 @@ static int lp_try(const char *path)
   void *h = dlopen(path, RTLD_NOW | RTLD_LOCAL);
   if (!h) return 0;
   LP.lartg = (f_lartg)lp_sym(h, "dlartg"); LP.rot = (f_rot)lp_sym(h, "drot");
-  LP.steqr = (f_steqr)lp_sym(h, "dsteqr"); LP.gehrd = (f_gehrd)lp_sym(h, "dgehrd");
+  LP.steqr = (f_steqr)lp_sym(h, "dsteqr"); LP.stevr = (f_stevr)lp_sym(h, "dstevr"); LP.gehrd = (f_gehrd)lp_sym(h, "dgehrd");
   LP.orghr = (f_orghr)lp_sym(h, "dorghr"); LP.hseqr = (f_hseqr)lp_sym(h, "dhseqr");
   LP.trexc = (f_trexc)lp_sym(h, "dtrexc"); LP.trevc = (f_trevc)lp_sym(h, "dtrevc");
   LP.potrf = (f_potrf)lp_sym(h, "dpotrf"); LP.trtri = (f_trtri)lp_sym(h, "dtrtri");
@@ -1144,7 +1148,8 @@ static void ks_expand_partial(KSo *ks, int k, int *M, double *betam, int *breakd
 {
   const int m = *M, n = ks->n, ncv = ks->m - 1, sz = (m > 100 ? m : 100) + 1;
   const double meps = 2.220446049250313e-16;
-  const double eps1 = sqrt((double)n) * meps / 2, delta = sqrt(meps) / sqrt((double)ncv);
+  const double sqeps = 1.490116119384766e-08;   /* PETSc's PETSC_SQRT_MACHINE_EPSILON (double) */
+  const double eps1 = sqrt((double)n) * meps / 2, delta = sqeps / sqrt((double)ncv);
   const double eta = pow(meps, 0.75) / sqrt((double)ncv);
   double *omega = (double *)calloc(sz, sizeof(double)), *omega_old = (double *)calloc(sz, sizeof(double));
   char *which = (char *)calloc(sz, 1), *which2 = (char *)calloc(sz, 1);
@@ -1197,6 +1202,88 @@ static void ks_expand_partial(KSo *ks, int k, int *M, double *betam, int *breakd
   *betam = beta[*M - 1];
   *breakdown = lindep;
   free(omega); free(omega_old); free(which); free(which2);
+}
+
+/* BVOrthogonalizeVec (bvorthog.c:145-215 with a vector) of w against the
+   columns 0..nc-1 of W, MGS with refinement IFNEEDED. */
+static void mgs_vec(int64_t n, const double *W, int64_t ld, int nc, double *w, double eta, double *norm,
+                    int *lindep)
+{
+  double onrm = orc_bv_norm2(n, w), nrm;
+  for (int pass = 0; pass < 3; pass++) {
+    if (pass > 0) {
+      if (!(nrm != 0.0 && fabs(nrm) < eta * fabs(onrm))) break;
+      onrm = nrm;
+    }
+    for (int i = 0; i < nc; i++) {
+      const double *vi = W + (int64_t)i * ld;
+      double dot = 0.0;
+      for (int64_t r = 0; r < n; r++) dot += w[r] * vi[r];
+      for (int64_t r = 0; r < n; r++) w[r] -= dot * vi[r];
+    }
+    nrm = orc_bv_norm2(n, w);
+  }
+  *lindep = !(nrm != 0.0 && fabs(nrm) >= eta * fabs(onrm));
+  *norm = nrm;
+}
+
+/* EPSSelectiveLanczos (lanczos.c:181-258): a local step, then the
+   eigen-decomposition (stevr, DenseTridiagonal lanczos.c:139-179) of the
+   tridiagonal block k..j; Ritz vectors whose bound norm |y_last| is below
+   sqrt(eps) ||A|| are formed in AV (again only when their number grew), and
+   the new vector is orthogonalised against them.  beta_j keeps the norm of
+   the local step; the vector is scaled by the final norm. */
+static int ks_expand_selective(KSo *ks, int k, int *M, double *betam, int *breakdown, double anorm, double *alpha,
+                               double *beta, double *AV)
+{
+  const int m = *M, n = ks->n;
+  const double sqeps = 1.490116119384766e-08;   /* PETSc's PETSC_SQRT_MACHINE_EPSILON (double) */
+  double *d = (double *)calloc(m + 1, sizeof(double)), *e = (double *)calloc(m + 1, sizeof(double));
+  double *ritz = (double *)calloc(m + 1, sizeof(double)), *Y = (double *)calloc((size_t)m * m + 1, sizeof(double));
+  double *work = (double *)calloc(20 * (size_t)m + 1, sizeof(double));
+  int *iwork = (int *)calloc(10 * (size_t)m + 1, sizeof(int)), *isuppz = (int *)calloc(2 * (size_t)m + 2, sizeof(int));
+  char *which = (char *)calloc(m + 1, 1);
+  int nritz = 0, lindep = 0, rc = 0;
+  double norm = 0.0;
+  for (int i = 0; i < k; i++) which[i] = 1;
+  for (int j = k; j < m; j++) {
+    double *vj1 = ks->V + (int64_t)(j + 1) * ks->ld;
+    ks_op(ks, ks->V + (int64_t)j * ks->ld, vj1);
+    which[j] = 1;
+    if (j - 2 >= k) which[j - 2] = 0;
+    orthogonalize_mgs(n, ks->V, ks->ld, j + 1, which, ks->buf, ks->m, ks->eta, &norm, &lindep, &ks->passes);
+    alpha[j] = ks->buf[j + (int64_t)(j + 1) * ks->m];
+    beta[j] = norm;
+    if (lindep) { *M = j + 1; break; }
+    int nn = j - k + 1, mf = 0, info = 0, lw = 20 * nn, liw = 10 * nn, il = 0, iu = 0;
+    const double vl = 0.0, vu = 0.0, abstol = 0.0;
+    for (int i = 0; i < nn; i++) { d[i] = alpha[i + k]; e[i] = beta[i + k]; }
+    LP.stevr("V", "A", &nn, d, e, &vl, &vu, &il, &iu, &abstol, &mf, ritz, Y, &nn, isuppz, work, &lw, iwork, &liw,
+             &info, 1, 1);
+    if (info) { rc = -1; break; }
+    for (int i = 0; i < nn; i++) if (fabs(ritz[i]) > anorm) anorm = fabs(ritz[i]);
+    int nritzo = 0;
+    for (int i = 0; i < nn; i++) if (norm * fabs(Y[(int64_t)i * nn + nn - 1]) < sqeps * anorm) nritzo++;
+    if (nritzo > nritz) {
+      nritz = 0;
+      for (int i = 0; i < nn; i++)
+        if (norm * fabs(Y[(int64_t)i * nn + nn - 1]) < sqeps * anorm) {
+          orc_bv_multvec(n, nn, 1.0, ks->V + (int64_t)k * ks->ld, ks->ld, Y + (int64_t)i * nn, 0.0,
+                         AV + (int64_t)nritz * ks->ld);
+          nritz++;
+        }
+    }
+    if (nritz > 0) {
+      mgs_vec(n, AV, ks->ld, nritz, vj1, ks->eta, &norm, &lindep);
+      ks->passes++;
+      if (lindep) { *M = j + 1; break; }
+    }
+    orc_bv_scale(n, vj1, 1.0 / norm);
+  }
+  *betam = beta[*M - 1];
+  *breakdown = lindep;
+  free(d); free(e); free(ritz); free(Y); free(work); free(iwork); free(isuppz); free(which);
+  return rc;
 }
 
 static void ks_expand(KSo *ks, int k, int *m, double *beta, int *breakdown)
@@ -1260,6 +1347,7 @@ int orc_eps_solve(int n, const int32_t *rp, const int32_t *ci, const double *val
      restart from the most wanted unconverged Ritz vector, DSHEP */
   const int lanc = o->solver == 2;
   if (lanc && (ncv > nev + mpd || !herm || harm)) return -11;
+  if (lanc && o->lanczos_partial == 3 && !LP.stevr) return -10;
   double anorm = -1.0;
   double *bnd = (double *)calloc(ncv + 1, sizeof(double)), *yr = (double *)calloc(ncv + 1, sizeof(double));
   char *cv = (char *)calloc(ncv + 1, 1);
@@ -1269,6 +1357,9 @@ int orc_eps_solve(int n, const int32_t *rp, const int32_t *ci, const double *val
   ks.eta = o->eta > 0 ? o->eta : 0.7071;
   ks.lu = lu; ks.piv = piv;
   ks.mgs = lanc && (o->lanczos_local || o->lanczos_partial);
+  /* selective: the Ritz vectors' basis AV (BVDuplicate of V, lanczos.c:63-71) */
+  double *AVs = NULL;
+  if (lanc && o->lanczos_partial == 3) AVs = (double *)calloc((size_t)n * (ncv + 1), sizeof(double));
   ks.which = (char *)calloc(ks.m + 1, 1);
   ks.V = (double *)calloc((size_t)n * ks.m, sizeof(double));
   ks.buf = (double *)calloc((size_t)ks.m * ks.m, sizeof(double));
@@ -1307,8 +1398,10 @@ int orc_eps_solve(int n, const int32_t *rp, const int32_t *ci, const double *val
     int nv = nconv + mpd < ncv ? nconv + mpd : ncv;
     ds.n = nv; ds.t = nv; ds.l = nconv; ds.k = (arn || lanc) ? 0 : nconv + l;   /* DSSetDimensions */
     double beta = 0.0; int breakdown = 0, k0 = nconv + l;
-    const int partial = lanc && o->lanczos_partial;
-    if (partial) ks_expand_partial(&ks, k0, &nv, &beta, &breakdown, anorm, o->lanczos_partial == 1, d, e);
+    const int partial = lanc ? o->lanczos_partial : 0;
+    if (partial == 3) {
+      if (ks_expand_selective(&ks, k0, &nv, &beta, &breakdown, anorm, d, e, AVs)) { rc = -20; goto done; }
+    } else if (partial) ks_expand_partial(&ks, k0, &nv, &beta, &breakdown, anorm, o->lanczos_partial == 1, d, e);
     else if (ks.mgs) ks_expand_local(&ks, k0, &nv, &beta, &breakdown);
     else ks_expand(&ks, k0, &nv, &beta, &breakdown);
     int bm = ks.m;
@@ -1564,7 +1657,7 @@ done:
 #endif
   free(gh); free(lu); free(piv);
   cmp_sinv = 0;
-  free(lockd); free(ks.V); free(ks.buf); free(ks.which); free(ds.A); free(ds.Q); free(ds.X); free(ds.T); free(ds.work); free(ds.perm); free(ds.iwork);
+  free(AVs); free(lockd); free(ks.V); free(ks.buf); free(ks.which); free(ds.A); free(ds.Q); free(ds.X); free(ds.T); free(ds.work); free(ds.perm); free(ds.iwork);
   free(eigr); free(eigi); free(errest); free(perm);
   free(bnd); free(yr); free(cv); free(lp);
   return rc;

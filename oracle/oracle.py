@@ -270,8 +270,8 @@ def eps_solve(A, nev, ncv=0, tol=1e-8, which="LM", hermitian=True, v0=None, max_
     o = Opts(hermitian=1 if hermitian else 0, which=WHICH[which], nev=nev, ncv=ncv, mpd=mpd,
              extraction={"ritz": 0, "harmonic": 1}[extraction], target=float(target),
              st={"shift": 0, "sinvert": 1}[st], sigma=float(target if sigma is None else sigma),
-             lanczos_local={"full": 0, "local": 1, "periodic": 0, "partial": 0}[reorthog],
-             lanczos_partial={"full": 0, "local": 0, "periodic": 1, "partial": 2}[reorthog],
+             lanczos_local=1 if reorthog == "local" else 0,
+             lanczos_partial={"full": 0, "local": 0, "periodic": 1, "partial": 2, "selective": 3}[reorthog],
              max_it=max_it, tol=tol, keep=keep, eta=0.7071, lock=lock, nthreads=nthreads,
              solver={"krylovschur": 0, "arnoldi": 1, "lanczos": 2}[solver])
     ncv_eff = ncv if ncv else min(A.n, max(2 * nev, nev + 15))

@@ -375,16 +375,16 @@ class EPS:
         check(lib().sa_eps_set_st_ksp(self.h, {"none": 0, "jacobi": 1}[pc], float(rtol), int(max_it)))
 
     def setLanczosReorthog(self, reorthog):
-        """EPSLanczosSetReorthog: "local", "full" (the default here), "periodic"
-        or "partial"."""
-        check(lib().sa_eps_lanczos_set_reorthog(self.h, {"local": 0, "full": 1, "periodic": 3,
-                                                         "partial": 4}[reorthog]))
+        """EPSLanczosSetReorthog: "local", "full" (the default here), "selective",
+        "periodic" or "partial"."""
+        check(lib().sa_eps_lanczos_set_reorthog(self.h, {"local": 0, "full": 1, "selective": 2,
+                                                         "periodic": 3, "partial": 4}[reorthog]))
 
     def getLanczosReorthog(self):
-        """EPSLanczosGetReorthog: "local", "full", "periodic" or "partial"."""
+        """EPSLanczosGetReorthog: "local", "full", "selective", "periodic" or "partial"."""
         v = C.c_int()
         check(lib().sa_eps_lanczos_get_reorthog(self.h, C.byref(v)))
-        return {0: "local", 1: "full", 3: "periodic", 4: "partial"}[v.value]
+        return {0: "local", 1: "full", 2: "selective", 3: "periodic", 4: "partial"}[v.value]
 
     def setTwoSided(self, twosided=True):
         """EPSSetTwoSided: also compute left eigenvectors (two-sided Krylov-Schur)."""

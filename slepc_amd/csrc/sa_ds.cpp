@@ -28,6 +28,9 @@ typedef void (*f_trexc)(const char *, const int *, double *, const int *, double
 typedef void (*f_trevc)(const char *, const char *, int *, const int *, const double *, const int *, double *, const int *,
                         double *, const int *, const int *, int *, double *, int *, size_t, size_t);
 typedef void (*f_potrf)(const char *, const int *, double *, const int *, int *, size_t);
+typedef void (*f_stevr)(const char *, const char *, const int *, double *, double *, const double *,
+                        const double *, const int *, const int *, const double *, int *, double *, double *,
+                        const int *, int *, double *, const int *, int *, const int *, int *, size_t, size_t);
 typedef void (*f_trtri)(const char *, const char *, const int *, double *, const int *, int *, size_t, size_t);
 typedef void (*f_syev)(const char *, const char *, const int *, double *, const int *, double *, double *, const int *,
                        int *, size_t, size_t);
@@ -47,6 +50,7 @@ struct Lapack {
   f_trexc trexc = nullptr;
   f_trevc trevc = nullptr;
   f_potrf potrf = nullptr;   // block orthogonalisation (BVOrthogonalize)
+  f_stevr stevr = nullptr;   // selective Lanczos (optional)
   f_trtri trtri = nullptr;
   f_syev syev = nullptr;
   f_getrf getrf = nullptr;   // harmonic extraction (DSTranslateHarmonic)
@@ -79,6 +83,7 @@ bool try_lib(const char *path) {
   L.syev = (f_syev)sym(h, "dsyev");
   L.getrf = (f_getrf)sym(h, "dgetrf");
   L.getrs = (f_getrs)sym(h, "dgetrs");
+  L.stevr = (f_stevr)sym(h, "dstevr");
   if (!(L.lartg && L.rot && L.steqr && L.gehrd && L.orghr && L.hseqr && L.trexc && L.trevc && L.potrf && L.trtri &&
         L.syev && L.getrf && L.getrs)) {
     dlclose(h);
@@ -132,6 +137,20 @@ int la_trtri_upper(int n, double *a, int lda) {
   if (!lapack_ready()) return -1000;
   int info = 0;
   LA.trtri("U", "N", &n, a, &lda, &info, 1, 1);
+  return info;
+}
+
+int la_stevr(int n, double *d, double *e, double *w, double *z) {
+  if (!lapack_ready() || !LA.stevr) return -100;
+  if (n <= 0) return 0;
+  const int lwork = 20 * n, liwork = 10 * n;
+  std::vector<double> work(lwork);
+  std::vector<int> iwork(liwork), isuppz(2 * n);
+  const double vl = 0.0, vu = 0.0, abstol = 0.0;
+  const int il = 0, iu = 0;
+  int m = 0, info = 0;
+  LA.stevr("V", "A", &n, d, e, &vl, &vu, &il, &iu, &abstol, &m, w, z, &n, isuppz.data(), work.data(), &lwork,
+           iwork.data(), &liwork, &info, 1, 1);
   return info;
 }
 

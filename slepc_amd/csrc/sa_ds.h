@@ -15,6 +15,9 @@ int la_potrf_upper(int n, double *a, int lda);          // dpotrf("U")
 int la_trtri_upper(int n, double *a, int lda);          // dtrtri("U", "N")
 int la_syev_lower(int n, double *a, int lda, double *w);  // dsyev("V", "L"), ascending w
 int la_getrf(int n, double *a, int lda, int *ipiv);       // dgetrf
+// dstevr("V", "A"): eigenvalues w (ascending) and eigenvectors z (n x n,
+// leading dimension n) of the tridiagonal (d, e); d, e are overwritten.
+int la_stevr(int n, double *d, double *e, double *w, double *z);
 int la_getrs(char trans, int n, const double *a, int lda, const int *ipiv, double *b);   // dgetrs, one rhs
 
 enum DsState { DS_RAW = 0, DS_INTERMEDIATE = 1, DS_CONDENSED = 2, DS_TRUNCATED = 3 };

@@ -78,6 +78,10 @@ int eps_start_vector(BV &V, int i, bool have_initial, double eta, int *breakdown
 // wanted unconverged Ritz vector as the next start vector (built in column
 // nv, then copied to the new column nconv), converged pairs moved to the
 // front, V(:,nconv:k) = V(:,nconv:nv) Y(nconv:nv, nconv:k), nconv = k.
+// PETSc's PETSC_SQRT_MACHINE_EPSILON for double: a decimal literal one ulp
+// above 2^-26, which the thresholds below compare against.
+constexpr double LZ_SQRT_EPS = 1.490116119384766e-08;
+
 // ---- EPSPartialLanczos (lanczos.c:338-441): periodic and partial ----------
 // The omega recurrence (update_omega, lanczos.c:260-293) estimates the loss of
 // orthogonality of the next Lanczos vector against each earlier one; its
@@ -142,7 +146,7 @@ static int lz_sweep_partial(BV &V, const std::function<int(const void *, void *)
   hipStream_t s = stream();
   const int m = *M;
   const double n = (double)V.n, meps = DBL_EPSILON;
-  const double eps1 = std::sqrt(n) * meps / 2, delta = std::sqrt(meps) / std::sqrt((double)ncv);
+  const double eps1 = std::sqrt(n) * meps / 2, delta = LZ_SQRT_EPS / std::sqrt((double)ncv);
   const double eta = std::pow(meps, 0.75) / std::sqrt((double)ncv);
   const size_t sz = (size_t)std::max(m, 100) + 1;
   std::vector<double> omega(sz, 0.0), omega_old(sz, 0.0);
@@ -209,6 +213,80 @@ static int lz_sweep_partial(BV &V, const std::function<int(const void *, void *)
     if (!fro && norm * delta < anorm * eps1) fro = true;
     beta[j] = norm;
     SA_TRY(launch_scale(V.prec, col_ptr(V, j + 1), V.n, 1.0 / norm, s));
+  }
+  *betam = beta[*M - 1];
+  *breakdown = lindep;
+  return 0;
+}
+
+// ---- EPSSelectiveLanczos (lanczos.c:181-258) --------------------------------
+// A local step, then the eigen-decomposition (stevr, DenseTridiagonal
+// lanczos.c:139-179) of the tridiagonal block k..j; the Ritz vectors whose
+// bound norm |y_last| is below sqrt(eps) ||A|| are formed in AV (again only
+// when their number grew) and the new vector is orthogonalised against them
+// (BVOrthogonalizeVec, MGS).  beta_j keeps the norm of the local step while
+// the vector is scaled by the final norm, as in the reference.
+static int lz_sweep_selective(BV &V, BV &AV, const std::function<int(const void *, void *)> &op, int k, int *M,
+                              double *betam, int *breakdown, double anorm, double eta_bv, double *alpha,
+                              double *beta, KsResult &r) {
+  hipStream_t s = stream();
+  const int m = *M;
+  SA_CHECK(AV.n == V.n && AV.prec == V.prec && AV.m >= m, "selective Lanczos workspace does not match the basis");
+  std::vector<double> d(m + 1), e(m + 1), ritz(m + 1), Y((size_t)m * m + 1);
+  std::vector<int> cols;
+  std::vector<char> which(m + 1, 0);
+  for (int i = 0; i < k; i++) which[i] = 1;
+  int nritz = 0, lindep = 0;
+  double norm = 0.0;
+  const double es = (double)prec_size(V.prec);
+  for (int j = k; j < m; j++) {
+    void *vj1 = col_ptr(V, j + 1);
+    SA_TRY(op(col_ptr(V, j), vj1));
+    which[j] = 1;
+    if (j - 2 >= k) which[j - 2] = 0;
+    cols.clear();
+    for (int i = 0; i <= j; i++)
+      if (which[i]) cols.push_back(i);
+    SA_TRY(bv_mgs_column(V, j + 1, cols.data(), (int)cols.size(), eta_bv, &norm, &lindep));
+    r.bytes_bv += (4.0 * cols.size() + 4.0) * V.n * es;
+    SA_HIP(hipMemcpyAsync(&alpha[j], V.dbuf + (size_t)(j + 1) * V.m + j, sizeof(double), hipMemcpyDeviceToHost, s));
+    SA_HIP(hipStreamSynchronize(s));
+    beta[j] = norm;
+    if (lindep) {
+      *M = j + 1;
+      break;
+    }
+    const int nn = j - k + 1;
+    for (int i = 0; i < nn; i++) {
+      d[i] = alpha[i + k];
+      e[i] = beta[i + k];
+    }
+    const int info = la_stevr(nn, d.data(), e.data(), ritz.data(), Y.data());
+    SA_CHECK(info == 0, "LAPACK failure in the selective Lanczos step (stevr)");
+    for (int i = 0; i < nn; i++) anorm = std::max(anorm, std::fabs(ritz[i]));
+    auto near = [&](int i) { return norm * std::fabs(Y[(size_t)i * nn + nn - 1]) < LZ_SQRT_EPS * anorm; };
+    int nritzo = 0;
+    for (int i = 0; i < nn; i++) nritzo += near(i);
+    if (nritzo > nritz) {
+      nritz = 0;
+      for (int i = 0; i < nn; i++) {
+        if (!near(i)) continue;
+        SA_HIP(hipMemcpyAsync(AV.dq, &Y[(size_t)i * nn], sizeof(double) * nn, hipMemcpyHostToDevice, s));
+        SA_TRY(launch_multvec(V, k, k + nn, 1.0, AV.dq, 0.0, col_ptr(AV, nritz), s));
+        SA_HIP(hipStreamSynchronize(s));
+        r.bytes_bv += (nn + 1.0) * V.n * es;
+        nritz++;
+      }
+    }
+    if (nritz > 0) {
+      SA_TRY(bv_mgs_vec(AV, nritz, vj1, eta_bv, &norm, &lindep));
+      r.bytes_bv += (4.0 * nritz + 4.0) * V.n * es;
+      if (lindep) {
+        *M = j + 1;
+        break;
+      }
+    }
+    SA_TRY(launch_scale(V.prec, vj1, V.n, 1.0 / norm, s));
   }
   *betam = beta[*M - 1];
   *breakdown = lindep;
@@ -394,6 +472,13 @@ int ks_solve(const KsParams &p, const Mat &A, BV &V, bool have_initial, KsResult
     ~DevBuf() { if (p) (void)hipFree(p); }
   } wtmp;
   if (lanc && p.lanczos_local) SA_HIP(hipMalloc(&wtmp.p, prec_size(V.prec) * (size_t)std::max(1, n)));
+  // selective: the basis of nearly converged Ritz vectors (BVDuplicate of V, lanczos.c:63-71)
+  struct BVHold {
+    BV b;
+    ~BVHold() { bv_free(b); }
+  } avh;
+  BV &AV = avh.b;
+  if (lanc && p.lanczos_partial == 3) SA_TRY(bv_alloc(AV, V.n, V.m, V.prec));
   // EPSLocalLanczos (lanczos.c:89-118): V(:,j+1) = Op V(:,j) orthogonalised
   // by MGS against the columns before k and V(:,j-1), V(:,j) only; the
   // coefficient buffer gets alpha_j (row j) and beta_j = the norm (row j+1,
@@ -464,7 +549,9 @@ int ks_solve(const KsParams &p, const Mat &A, BV &V, bool have_initial, KsResult
     int breakdown = 0;
     const int64_t pbefore = V.passes_done;
     const bool partial = lanc && p.lanczos_partial != 0;
-    if (partial)
+    if (partial && p.lanczos_partial == 3)
+      SA_TRY(lz_sweep_selective(V, AV, op_apply, k0, &nv, &beta, &breakdown, anorm, p.eta, ds.d(), ds.e(), r));
+    else if (partial)
       SA_TRY(lz_sweep_partial(V, op_apply, k0, &nv, &beta, &breakdown, anorm, p.lanczos_partial == 1, ncv, p.eta,
                               ds.d(), ds.e(), r));
     else

@@ -155,6 +155,9 @@ int bv_orthonormalize(BV &V, int j, double eta, double *norm, int *lindep);
 // in that order), refinement IFNEEDED (sa_mgs.hip): coefficients in the
 // coefficient buffer column j, row j = the norm (0 on lindep); not scaled.
 int bv_mgs_column(BV &V, int j, const int *cols, int nc, double eta, double *norm, int *lindep);
+// BVOrthogonalizeVec with MGS: the device vector w (W.n rows, W's precision)
+// against the columns 0..nc-1 of W, refinement IFNEEDED; not scaled.
+int bv_mgs_vec(BV &W, int nc, void *w, double eta, double *norm, int *lindep);
 // BVOrthogonalize of the active columns (method 0 gs, 1 chol, 2 svqb); R: host
 // column-major, leading dimension ldr >= V.k, or null
 int bv_orthogonalize(BV &V, int method, double *R, int ldr);

@@ -10,6 +10,7 @@
 // fixed order, so the result does not depend on timing.
 #include <algorithm>
 #include <cmath>
+#include <vector>
 
 #include "sa_device.h"
 #include "sa_internal.h"
@@ -72,26 +73,25 @@ __global__ __launch_bounds__(BLK) void mgs_finish_kernel(const double *part, int
 
 }  // namespace
 
-// One MGS pass: column j of V against the columns cols[0..nc) in order, each
-// coefficient added into h[cols[q]]; onrm (device) gets ||v|| before, nrm
-// (device) ||v|| after; hn (device, or null) also gets ||v|| after.
-static int mgs_pass(BV &V, int j, const int *cols, int nc, double *h, double *slot, double *onrm, double *nrm,
-                    double *hn, hipStream_t s) {
-  const int64_t n = V.n;
-  const int64_t cap = 2 * (int64_t)V.ws.maxG * (GS_MAXC + 1);
+// One MGS pass: the vector w (W.n rows, W's precision) against the columns
+// cols[0..nc) of W in order, each coefficient added into h[cols[q]]; onrm
+// (device) gets ||w|| before, nrm (device) ||w|| after; hn (device, or null)
+// also gets ||w|| after.  Partial sums in W's workspace.
+static int mgs_pass(const BV &W, void *w, const int *cols, int nc, double *h, double *slot, double *onrm,
+                    double *nrm, double *hn, hipStream_t s) {
+  const int64_t n = W.n;
+  const int64_t cap = 2 * (int64_t)W.ws.maxG * (GS_MAXC + 1);
   const int grid = (int)std::max<int64_t>(1, std::min<int64_t>({1024, (n + BLK - 1) / BLK, cap}));
-  double *part = V.ws.part;
+  double *part = W.ws.part;
   auto step = [&](int prev, int next) {
-    if (V.prec == SA_FP64) {
-      const double *vp = prev >= 0 ? (const double *)col_ptr(V, prev) : nullptr;
-      const double *vn = next >= 0 ? (const double *)col_ptr(V, next) : nullptr;
-      hipLaunchKernelGGL(mgs_step_kernel<double>, dim3(grid), dim3(BLK), 0, s, (double *)col_ptr(V, j), vp, slot,
-                         vn, n, part);
+    if (W.prec == SA_FP64) {
+      const double *vp = prev >= 0 ? (const double *)col_ptr(W, prev) : nullptr;
+      const double *vn = next >= 0 ? (const double *)col_ptr(W, next) : nullptr;
+      hipLaunchKernelGGL(mgs_step_kernel<double>, dim3(grid), dim3(BLK), 0, s, (double *)w, vp, slot, vn, n, part);
     } else {
-      const float *vp = prev >= 0 ? (const float *)col_ptr(V, prev) : nullptr;
-      const float *vn = next >= 0 ? (const float *)col_ptr(V, next) : nullptr;
-      hipLaunchKernelGGL(mgs_step_kernel<float>, dim3(grid), dim3(BLK), 0, s, (float *)col_ptr(V, j), vp, slot,
-                         vn, n, part);
+      const float *vp = prev >= 0 ? (const float *)col_ptr(W, prev) : nullptr;
+      const float *vn = next >= 0 ? (const float *)col_ptr(W, next) : nullptr;
+      hipLaunchKernelGGL(mgs_step_kernel<float>, dim3(grid), dim3(BLK), 0, s, (float *)w, vp, slot, vn, n, part);
     }
   };
   if (onrm) {
@@ -107,34 +107,55 @@ static int mgs_pass(BV &V, int j, const int *cols, int nc, double *h, double *sl
   step(prev, -1);
   hipLaunchKernelGGL(mgs_finish_kernel, dim3(1), dim3(BLK), 0, s, part, grid, nullptr, nrm, hn);
   SA_HIP(hipGetLastError());
-  V.passes_done++;
   return 0;
+}
+
+// BVOrthogonalizeGS with MGS and refinement IFNEEDED (bvorthog.c:175-185):
+// passes while ||w|| < eta ||w_before||, at most three; returns the final
+// norm and the lindep test of the last pass.
+static int mgs_refined(const BV &W, void *w, const int *cols, int nc, double *h, double eta, double *norm,
+                       int *lindep, int64_t *passes) {
+  hipStream_t s = stream();
+  double *sc = W.dq + (size_t)W.m * W.m;              // scalar slots past the staging area
+  double *slot = sc, *nrm_d = sc + 1, *onrm_d = sc + 2;
+  double hs[2];
+  SA_TRY(mgs_pass(W, w, cols, nc, h, slot, onrm_d, nrm_d, nullptr, s));
+  SA_HIP(hipMemcpyAsync(hs, nrm_d, 2 * sizeof(double), hipMemcpyDeviceToHost, s));
+  SA_HIP(hipStreamSynchronize(s));
+  (*passes)++;
+  double nrm = hs[0], onrm = hs[1];
+  int l = 1;
+  while (l < 3 && nrm != 0.0 && std::fabs(nrm) < eta * std::fabs(onrm)) {
+    l++;
+    onrm = nrm;
+    SA_TRY(mgs_pass(W, w, cols, nc, h, slot, nullptr, nrm_d, nullptr, s));
+    SA_HIP(hipMemcpyAsync(hs, nrm_d, sizeof(double), hipMemcpyDeviceToHost, s));
+    SA_HIP(hipStreamSynchronize(s));
+    (*passes)++;
+    nrm = hs[0];
+  }
+  *norm = nrm;
+  *lindep = !(nrm != 0.0 && std::fabs(nrm) >= eta * std::fabs(onrm));
+  return 0;
+}
+
+int bv_mgs_vec(BV &W, int nc, void *w, double eta, double *norm, int *lindep) {
+  SA_CHECK(nc >= 0 && nc <= W.m, "column count out of range");
+  std::vector<int> cols(nc);
+  for (int q = 0; q < nc; q++) cols[q] = q;
+  SA_HIP(hipMemsetAsync(W.dbuf, 0, sizeof(double) * (size_t)std::max(nc, 1), stream()));
+  return mgs_refined(W, w, cols.data(), nc, W.dbuf, eta, norm, lindep, &W.passes_done);
 }
 
 int bv_mgs_column(BV &V, int j, const int *cols, int nc, double eta, double *norm, int *lindep) {
   SA_CHECK(j >= 0 && j < V.m && nc >= 0, "column index out of range");
   for (int q = 0; q < nc; q++) SA_CHECK(cols[q] >= 0 && cols[q] < j, "orthogonalization column out of range");
   hipStream_t s = stream();
-  const int m = V.m;
-  double *h = V.dbuf + (size_t)j * m;                 // coefficients of column j
-  double *sc = V.dq + (size_t)m * m;                  // scalar slots past the staging area
-  double *slot = sc, *nrm_d = sc + 1, *onrm_d = sc + 2;
+  double *h = V.dbuf + (size_t)j * V.m;               // coefficients of column j
   SA_HIP(hipMemsetAsync(h, 0, sizeof(double) * (size_t)(j + 1), s));   // BV_CleanCoefficients
-  double hs[2];
-  SA_TRY(mgs_pass(V, j, cols, nc, h, slot, onrm_d, nrm_d, nullptr, s));
-  SA_HIP(hipMemcpyAsync(hs, nrm_d, 2 * sizeof(double), hipMemcpyDeviceToHost, s));
-  SA_HIP(hipStreamSynchronize(s));
-  double nrm = hs[0], onrm = hs[1];
-  int l = 1;
-  while (l < 3 && nrm != 0.0 && std::fabs(nrm) < eta * std::fabs(onrm)) {
-    l++;
-    onrm = nrm;
-    SA_TRY(mgs_pass(V, j, cols, nc, h, slot, nullptr, nrm_d, nullptr, s));
-    SA_HIP(hipMemcpyAsync(hs, nrm_d, sizeof(double), hipMemcpyDeviceToHost, s));
-    SA_HIP(hipStreamSynchronize(s));
-    nrm = hs[0];
-  }
-  const int ld = !(nrm != 0.0 && std::fabs(nrm) >= eta * std::fabs(onrm));
+  double nrm = 0.0;
+  int ld = 0;
+  SA_TRY(mgs_refined(V, col_ptr(V, j), cols, nc, h, eta, &nrm, &ld, &V.passes_done));
   const double hj = ld ? 0.0 : nrm;
   SA_HIP(hipMemcpyAsync(h + j, &hj, sizeof(double), hipMemcpyHostToDevice, s));
   SA_HIP(hipStreamSynchronize(s));

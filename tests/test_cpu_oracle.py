@@ -379,12 +379,13 @@ def _solver(name):
     return dict(solver=name)
 
 
-@pytest.mark.parametrize("solver", ["arnoldi", "lanczos", "lanczos-local", "lanczos-periodic", "lanczos-partial"])
+@pytest.mark.parametrize("solver", ["arnoldi", "lanczos", "lanczos-local", "lanczos-periodic", "lanczos-partial",
+                                    "lanczos-selective"])
 @pytest.mark.parametrize("which", ["LR", "SR"])
 def test_oracle_explicit_restart_test2(orc, which, solver):
     """eps/tests/test2.c with -eps_type arnoldi and with -eps_type lanczos
-    -eps_lanczos_reorthog full / local / periodic / partial (test2_2): 1-D
-    Laplacian n = 30, nev 4,
+    -eps_lanczos_reorthog full / local / periodic / partial (test2_2) /
+    selective (test2_2_selective): 1-D Laplacian n = 30, nev 4,
     tol max(1000 eps, 1e-9): the values printed by the reference (test2_1.out)
     and the closed form 2 - 2 cos(k pi / 31)."""
     A = _lap1d(orc, 30)
@@ -440,14 +441,14 @@ TEST8 = {
 }
 
 
-@pytest.mark.parametrize("reorthog", ["local", "periodic", "partial"])
+@pytest.mark.parametrize("reorthog", ["local", "periodic", "partial", "selective"])
 @pytest.mark.parametrize("case", sorted(TEST8))
 def test_oracle_lanczos_local_test8(orc, case, reorthog):
     """EPSLanczos with local reorthogonalization on the cases of test8.c
     (tol 1e-8; case 3 sets mpd 9, so ncv = nev + mpd = 21 as
     EPSSetDimensions_Default gives): the printed values, double eigenvalues
     included, and the closed form 4 - 2 cos(i pi/(m+1)) - 2 cos(j pi/(m+1)).
-    The periodic and partial variants must find the same values."""
+    The periodic, partial and selective variants must find the same values."""
     m, nev, ncv, mpd, which, max_it, gold = TEST8[case]
     A = orc.lap2d(m)
     r = orc.eps_solve(A, nev, ncv, 1e-8, which, True, max_it=max_it, mpd=mpd, solver="lanczos", reorthog=reorthog)
@@ -462,19 +463,19 @@ def test_oracle_lanczos_local_test8(orc, case, reorthog):
 def test_oracle_lanczos_partial_reorthogonalizes(orc):
     """Periodic and partial reorthogonalization run extra Gram-Schmidt passes
     when the omega estimate of the loss of orthogonality exceeds
-    sqrt(eps/ncv): on the 27-point 10^3 operator (8 largest magnitude, ncv 24)
+    sqrt(eps/ncv), selective when Ritz vectors of the sweep converge: on the 27-point 10^3 operator (8 largest magnitude, ncv 24)
     they take more passes per operator application than the local variant
     (at most three per application with refinement), and fewer than twice
     as many (the reorthogonalization is occasional, not every step)."""
     T = orc.stencil27(10, seed=1)
     per = {}
-    for ro in ["local", "periodic", "partial"]:
+    for ro in ["local", "periodic", "partial", "selective"]:
         r = orc.eps_solve(T, 8, 24, 1e-10, "LM", True, v0=orc.rand_uniform(0, T.n), solver="lanczos",
                           max_it=2000, reorthog=ro)
         assert r["nconv"] >= 8
         per[ro] = r["passes"] / r["matvecs"]
-    assert per["periodic"] > per["local"] and per["partial"] > per["local"]
-    assert per["periodic"] < 2 * per["local"] and per["partial"] < 2 * per["local"]
+    for ro in ["periodic", "partial", "selective"]:
+        assert per["local"] < per[ro] < 2 * per["local"]
 
 
 TEST2_INTERIOR = {30: [2.10130, 1.89870, 2.30286, 2.50131], 32: [2.09516, 2.28463, 1.90484, 2.47152]}
@@ -560,7 +561,8 @@ def test_oracle_sinvert_test9_5(orc, solver):
     assert np.allclose(r["eigr"][:3], near.real, rtol=1e-9)
 
 
-@pytest.mark.parametrize("solver", ["lanczos", "lanczos-local", "lanczos-periodic", "lanczos-partial", "krylovschur"])
+@pytest.mark.parametrize("solver", ["lanczos", "lanczos-local", "lanczos-periodic", "lanczos-partial",
+                                    "lanczos-selective", "krylovschur"])
 def test_oracle_sinvert_test2_interior(orc, solver):
     """eps/tests/test2.c interior eigenvalues with Lanczos and shift-and-invert
     at target 2.1 (A - 2.1 I indefinite; test2_1.out) and the closed form.

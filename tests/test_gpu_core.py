@@ -424,15 +424,16 @@ def test_explicit_restart_matches_oracle(sa, orc, solver, case):
         assert E.computeError(i) < 10 * tol
 
 
-@pytest.mark.parametrize("reorthog", ["local", "periodic", "partial"])
+@pytest.mark.parametrize("reorthog", ["local", "periodic", "partial", "selective"])
 @pytest.mark.parametrize("case", ["test2_LR", "test2_SR", "test2_interior", "test8_1", "test8_2", "test8_3",
                                   "stencil27_10"])
 def test_lanczos_local_matches_oracle(sa, orc, case, reorthog):
     """EPSLanczos with local reorthogonalization (MGS against the locked
     columns and the two previous Lanczos vectors, purging of repeated and
-    spurious Ritz values) and with periodic / partial reorthogonalization
+    spurious Ritz values), with periodic / partial reorthogonalization
     (local steps plus MGS against the sweep's columns when the omega
-    estimate calls for it) through the C ABI against the oracle from the same
+    estimate calls for it) and with selective reorthogonalization (against
+    the sweep's nearly converged Ritz vectors) through the C ABI against the oracle from the same
     start vector: eigenvalues to 1e-9 and the values printed by the
     reference.  test2: 1-D Laplacian n = 30 (test2_1.out, the interior case
     with shift-and-invert at 2.1, where ghost copies of the dominant value
@@ -458,8 +459,8 @@ def test_lanczos_local_matches_oracle(sa, orc, case, reorthog):
         v0 = orc.rand_uniform(0, Ao.n)
     E = sa.EPS()
     E.setType("lanczos")
-    with pytest.raises(Exception):   # selective: not available
-        sa._lib.check(sa._lib.lib().sa_eps_lanczos_set_reorthog(E.h, 2))
+    with pytest.raises(Exception):   # delayed: not available
+        sa._lib.check(sa._lib.lib().sa_eps_lanczos_set_reorthog(E.h, 5))
     E.setLanczosReorthog(reorthog)
     assert E.getLanczosReorthog() == reorthog
     E.setOperators(sa.Mat(sa.CSR(Ao.n, Ao.rowptr, Ao.col, Ao.val)))
