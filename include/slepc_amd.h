@@ -240,13 +240,19 @@ int sa_eps_set_st_ksp(sa_eps eps, int pc_type, double rtol, int max_it);
    sqrt(eps/ncv) (periodic: all of them, twice in a row; partial: the
    intervals around the largest estimates).  SELECTIVE (EPSSelectiveLanczos,
    lanczos.c:181-258): local steps plus orthogonalization against the Ritz
-   vectors of the current sweep whose bound is below sqrt(eps) ||A||.  The
-   delayed variant is rejected with an error. */
+   vectors of the current sweep whose bound is below sqrt(eps) ||A||.
+   DELAYED (EPSDelayedArnoldi, epskrylov.c:24-114): one classical
+   Gram-Schmidt pass per step with the normalisation delayed by one step and
+   a second pass by two.  As in the reference it keeps orthogonality only
+   while the restart vectors are far from converged: restarted from nearly
+   converged Ritz vectors it loses it (the reference tests it at tol 1e-8 on
+   a 30 x 30 tridiagonal).  Other values are rejected with an error. */
 #define SA_LANCZOS_REORTHOG_LOCAL 0
 #define SA_LANCZOS_REORTHOG_FULL 1
 #define SA_LANCZOS_REORTHOG_SELECTIVE 2
 #define SA_LANCZOS_REORTHOG_PERIODIC 3
 #define SA_LANCZOS_REORTHOG_PARTIAL 4
+#define SA_LANCZOS_REORTHOG_DELAYED 5
 int sa_eps_lanczos_set_reorthog(sa_eps eps, int reorthog);
 int sa_eps_lanczos_get_reorthog(sa_eps eps, int *reorthog);
 /* EPSSetTwoSided: two-sided Krylov-Schur (ks-twosided.c:126-241) that also

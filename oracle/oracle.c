@@ -1286,6 +1286,61 @@ static int ks_expand_selective(KSo *ks, int k, int *M, double *betam, int *break
   return rc;
 }
 
+/* EPSDelayedArnoldi (epskrylov.c:24-114; lanczos.c:495-512 with refinement
+   other than NEVER): one CGS pass per step whose normalisation is delayed by
+   one step and whose reorthogonalisation by two.  At step j the dot products
+   of v_{j+1} and v_j with the earlier columns are taken before v_j is
+   normalised, the coefficients rescaled; the unnormalised v_j (t) is
+   reorthogonalised against v_0..v_{j-1} and, one step later, normalised into
+   column j.  H: m x m, leading dimension m (the caller's T). */
+static void ks_expand_delayed(KSo *ks, int k, int m, double *H, double *betam)
+{
+  const int64_t N = ks->n, ld = ks->ld;
+  double *V = ks->V;
+  double *u = (double *)calloc((size_t)N, sizeof(double)), *t = (double *)calloc((size_t)N, sizeof(double));
+  double *lhh = (double *)calloc((size_t)m + 1, sizeof(double));
+  double norm1 = 0.0, norm2 = 1.0;
+#define COL(c) (V + (int64_t)(c) * ld)
+  for (int j = k; j < m; j++) {
+    ks_op(ks, COL(j), COL(j + 1));
+    orc_bv_dotvec(N, j + 1, V, ld, COL(j + 1), H + (int64_t)m * j);   /* BVDotColumn(V, j+1) */
+    if (j > k) {
+      orc_bv_dotvec(N, j, V, ld, COL(j), lhh);                          /* BVDotColumn(V, j) */
+      double dot = 0.0;
+      for (int64_t r = 0; r < N; r++) dot += COL(j)[r] * COL(j)[r];
+      if (j > k + 1) norm2 = orc_bv_norm2(N, u);
+      norm1 = sqrt(dot);
+      for (int i = 0; i < j; i++) H[(int64_t)m * j + i] /= norm1;
+      H[(int64_t)m * j + j] /= dot;
+      memcpy(t, COL(j), sizeof(double) * (size_t)N);
+      orc_bv_scale(N, COL(j), 1.0 / norm1);
+      orc_bv_scale(N, COL(j + 1), 1.0 / norm1);
+    }
+    orc_bv_multvec(N, j + 1, -1.0, V, ld, H + (int64_t)m * j, 1.0, COL(j + 1));   /* BVMultColumn(V, j+1) */
+    if (j > k) {
+      orc_bv_multvec(N, j, -1.0, V, ld, lhh, 1.0, t);
+      for (int i = 0; i < j; i++) H[(int64_t)m * (j - 1) + i] += lhh[i];
+    }
+    if (j > k + 1) {
+      memcpy(COL(j - 1), u, sizeof(double) * (size_t)N);
+      orc_bv_scale(N, COL(j - 1), 1.0 / norm2);
+      H[(int64_t)m * (j - 2) + j - 1] = norm2;
+    }
+    if (j < m - 1) memcpy(u, t, sizeof(double) * (size_t)N);
+  }
+  norm2 = orc_bv_norm2(N, t);
+  orc_bv_scale(N, t, 1.0 / norm2);
+  memcpy(COL(m - 1), t, sizeof(double) * (size_t)N);
+  H[(int64_t)m * (m - 2) + m - 1] = norm2;
+  orc_bv_dotvec(N, m, V, ld, COL(m), lhh);
+  orc_bv_multvec(N, m, -1.0, V, ld, lhh, 1.0, COL(m));
+  for (int i = 0; i < m; i++) H[(int64_t)m * (m - 1) + i] += lhh[i];
+  *betam = orc_bv_norm2(N, COL(m));
+  orc_bv_scale(N, COL(m), 1.0 / *betam);
+#undef COL
+  free(u); free(t); free(lhh);
+}
+
 static void ks_expand(KSo *ks, int k, int *m, double *beta, int *breakdown)
 {
   int lindep = 0;
@@ -1401,6 +1456,15 @@ int orc_eps_solve(int n, const int32_t *rp, const int32_t *ci, const double *val
     const int partial = lanc ? o->lanczos_partial : 0;
     if (partial == 3) {
       if (ks_expand_selective(&ks, k0, &nv, &beta, &breakdown, anorm, d, e, AVs)) { rc = -20; goto done; }
+    } else if (partial == 4) {                          /* lanczos.c:495-512 */
+      if (nv - k0 < 2) { rc = -14; goto done; }
+      double *Td = (double *)calloc((size_t)nv * nv, sizeof(double));
+      ks_expand_delayed(&ks, k0, nv, Td, &beta);
+      breakdown = 0;
+      for (int i = k0; i < nv - 1; i++) { d[i] = Td[(int64_t)nv * i + i]; e[i] = Td[(int64_t)nv * i + i + 1]; }
+      d[nv - 1] = Td[(int64_t)nv * (nv - 1) + nv - 1];
+      e[nv - 1] = beta;
+      free(Td);
     } else if (partial) ks_expand_partial(&ks, k0, &nv, &beta, &breakdown, anorm, o->lanczos_partial == 1, d, e);
     else if (ks.mgs) ks_expand_local(&ks, k0, &nv, &beta, &breakdown);
     else ks_expand(&ks, k0, &nv, &beta, &breakdown);

@@ -271,7 +271,8 @@ def eps_solve(A, nev, ncv=0, tol=1e-8, which="LM", hermitian=True, v0=None, max_
              extraction={"ritz": 0, "harmonic": 1}[extraction], target=float(target),
              st={"shift": 0, "sinvert": 1}[st], sigma=float(target if sigma is None else sigma),
              lanczos_local=1 if reorthog == "local" else 0,
-             lanczos_partial={"full": 0, "local": 0, "periodic": 1, "partial": 2, "selective": 3}[reorthog],
+             lanczos_partial={"full": 0, "local": 0, "periodic": 1, "partial": 2, "selective": 3,
+                               "delayed": 4}[reorthog],
              max_it=max_it, tol=tol, keep=keep, eta=0.7071, lock=lock, nthreads=nthreads,
              solver={"krylovschur": 0, "arnoldi": 1, "lanczos": 2}[solver])
     ncv_eff = ncv if ncv else min(A.n, max(2 * nev, nev + 15))

@@ -376,15 +376,16 @@ class EPS:
 
     def setLanczosReorthog(self, reorthog):
         """EPSLanczosSetReorthog: "local", "full" (the default here), "selective",
-        "periodic" or "partial"."""
+        "periodic", "partial" or "delayed"."""
         check(lib().sa_eps_lanczos_set_reorthog(self.h, {"local": 0, "full": 1, "selective": 2,
-                                                         "periodic": 3, "partial": 4}[reorthog]))
+                                                         "periodic": 3, "partial": 4, "delayed": 5}[reorthog]))
 
     def getLanczosReorthog(self):
-        """EPSLanczosGetReorthog: "local", "full", "selective", "periodic" or "partial"."""
+        """EPSLanczosGetReorthog: "local", "full", "selective", "periodic", "partial"
+        or "delayed"."""
         v = C.c_int()
         check(lib().sa_eps_lanczos_get_reorthog(self.h, C.byref(v)))
-        return {0: "local", 1: "full", 2: "selective", 3: "periodic", 4: "partial"}[v.value]
+        return {0: "local", 1: "full", 2: "selective", 3: "periodic", 4: "partial", 5: "delayed"}[v.value]
 
     def setTwoSided(self, twosided=True):
         """EPSSetTwoSided: also compute left eigenvectors (two-sided Krylov-Schur)."""

@@ -520,13 +520,13 @@ int sa_eps_set_st_ksp(sa_eps E, int pc_type, double rtol, int max_it) {
 
 int sa_eps_lanczos_set_reorthog(sa_eps E, int reorthog) {
   SA_CHECK(E, "null EPS");
-  SA_CHECK(reorthog >= SA_LANCZOS_REORTHOG_LOCAL && reorthog <= SA_LANCZOS_REORTHOG_PARTIAL,
-           "unsupported Lanczos reorthogonalization type (local, full, selective, periodic and partial are "
-           "available)");
+  SA_CHECK(reorthog >= SA_LANCZOS_REORTHOG_LOCAL && reorthog <= SA_LANCZOS_REORTHOG_DELAYED,
+           "unsupported Lanczos reorthogonalization type");
   E->p.lanczos_local = reorthog == SA_LANCZOS_REORTHOG_LOCAL;
   E->p.lanczos_partial = reorthog == SA_LANCZOS_REORTHOG_PERIODIC    ? 1
                          : reorthog == SA_LANCZOS_REORTHOG_PARTIAL   ? 2
                          : reorthog == SA_LANCZOS_REORTHOG_SELECTIVE ? 3
+                         : reorthog == SA_LANCZOS_REORTHOG_DELAYED   ? 4
                                                                      : 0;
   E->solved = false;
   return 0;
@@ -538,6 +538,7 @@ int sa_eps_lanczos_get_reorthog(sa_eps E, int *reorthog) {
               : E->p.lanczos_partial == 1 ? SA_LANCZOS_REORTHOG_PERIODIC
               : E->p.lanczos_partial == 2 ? SA_LANCZOS_REORTHOG_PARTIAL
               : E->p.lanczos_partial == 3 ? SA_LANCZOS_REORTHOG_SELECTIVE
+              : E->p.lanczos_partial == 4 ? SA_LANCZOS_REORTHOG_DELAYED
                                           : SA_LANCZOS_REORTHOG_FULL;
   return 0;
 }

@@ -293,6 +293,88 @@ static int lz_sweep_selective(BV &V, BV &AV, const std::function<int(const void 
   return 0;
 }
 
+// ---- EPSDelayedArnoldi (epskrylov.c:24-114; lanczos.c:495-512) ------------
+// One classical Gram-Schmidt pass per step whose normalisation is delayed by
+// one step and whose reorthogonalisation by two: at step j the dot products of
+// v_{j+1} and v_j with the earlier columns are taken before v_j is normalised
+// and the coefficients rescaled; the unnormalised v_j (t) is reorthogonalised
+// against v_0..v_{j-1} and, one step later, normalised into column j.  H: host
+// m x m, leading dimension m.  u, t: device vectors of V.n entries.
+static int lz_sweep_delayed(BV &V, const std::function<int(const void *, void *)> &op, int k, int m, double *H,
+                            double *betam, void *u, void *t, KsResult &r) {
+  hipStream_t s = stream();
+  SA_CHECK(m - k >= 2, "the delayed Lanczos sweep needs at least two steps (ncv - nconv >= 2)");
+  const size_t vb = prec_size(V.prec) * (size_t)V.n;
+  const double es = (double)prec_size(V.prec);
+  std::vector<double> lhh(m + 1, 0.0);
+  double norm1 = 0.0, norm2 = 1.0, h = 0.0;
+  double *qd = V.dq;   // device staging of one coefficient vector
+  auto dotcol = [&](int c, int nc, double *out) -> int {   // out = V(:, 0:nc)^T V(:, c)
+    SA_TRY(launch_dotvec(V, 0, nc, col_ptr(V, c), qd, s));
+    SA_HIP(hipMemcpyAsync(out, qd, sizeof(double) * nc, hipMemcpyDeviceToHost, s));
+    SA_HIP(hipStreamSynchronize(s));
+    r.bytes_bv += (nc + 1.0) * V.n * es;
+    return 0;
+  };
+  auto multcol = [&](int nc, const double *q, void *y) -> int {   // y -= V(:, 0:nc) q
+    SA_HIP(hipMemcpyAsync(qd, q, sizeof(double) * nc, hipMemcpyHostToDevice, s));
+    SA_TRY(launch_multvec(V, 0, nc, -1.0, qd, 1.0, y, s));
+    SA_HIP(hipStreamSynchronize(s));
+    r.bytes_bv += (nc + 2.0) * V.n * es;
+    return 0;
+  };
+  auto sumsq = [&](const void *x, double *out) -> int {
+    SA_TRY(launch_dot_self(V.prec, x, V.n, V.ws.dots, V.ws.part, s));
+    SA_HIP(hipMemcpyAsync(out, V.ws.dots, sizeof(double), hipMemcpyDeviceToHost, s));
+    SA_HIP(hipStreamSynchronize(s));
+    return 0;
+  };
+  for (int j = k; j < m; j++) {
+    SA_TRY(op(col_ptr(V, j), col_ptr(V, j + 1)));
+    double *Hj = H + (size_t)m * j;
+    SA_TRY(dotcol(j + 1, j + 1, Hj));
+    if (j > k) {
+      SA_TRY(dotcol(j, j, lhh.data()));
+      double dot = 0.0;
+      SA_TRY(sumsq(col_ptr(V, j), &dot));
+      if (j > k + 1) {
+        SA_TRY(sumsq(u, &h));
+        norm2 = std::sqrt(h);
+      }
+      norm1 = std::sqrt(dot);
+      for (int i = 0; i < j; i++) Hj[i] /= norm1;
+      Hj[j] /= dot;
+      SA_HIP(hipMemcpyAsync(t, col_ptr(V, j), vb, hipMemcpyDeviceToDevice, s));
+      SA_TRY(launch_scale(V.prec, col_ptr(V, j), V.n, 1.0 / norm1, s));
+      SA_TRY(launch_scale(V.prec, col_ptr(V, j + 1), V.n, 1.0 / norm1, s));
+    }
+    SA_TRY(multcol(j + 1, Hj, col_ptr(V, j + 1)));
+    if (j > k) {
+      SA_TRY(multcol(j, lhh.data(), t));
+      for (int i = 0; i < j; i++) H[(size_t)m * (j - 1) + i] += lhh[i];
+    }
+    if (j > k + 1) {
+      SA_HIP(hipMemcpyAsync(col_ptr(V, j - 1), u, vb, hipMemcpyDeviceToDevice, s));
+      SA_TRY(launch_scale(V.prec, col_ptr(V, j - 1), V.n, 1.0 / norm2, s));
+      H[(size_t)m * (j - 2) + j - 1] = norm2;
+    }
+    if (j < m - 1) SA_HIP(hipMemcpyAsync(u, t, vb, hipMemcpyDeviceToDevice, s));
+  }
+  SA_TRY(sumsq(t, &h));
+  norm2 = std::sqrt(h);
+  SA_TRY(launch_scale(V.prec, t, V.n, 1.0 / norm2, s));
+  SA_HIP(hipMemcpyAsync(col_ptr(V, m - 1), t, vb, hipMemcpyDeviceToDevice, s));
+  H[(size_t)m * (m - 2) + m - 1] = norm2;
+  SA_TRY(dotcol(m, m, lhh.data()));
+  SA_TRY(multcol(m, lhh.data(), col_ptr(V, m)));
+  for (int i = 0; i < m; i++) H[(size_t)m * (m - 1) + i] += lhh[i];
+  SA_TRY(sumsq(col_ptr(V, m), &h));
+  *betam = std::sqrt(h);
+  SA_TRY(launch_scale(V.prec, col_ptr(V, m), V.n, 1.0 / *betam, s));
+  SA_HIP(hipStreamSynchronize(s));
+  return 0;
+}
+
 // Local reorthogonalization (p.lanczos_local, -eps_lanczos_reorthog local):
 // with the default classical Gram-Schmidt the column mask of EPSLocalLanczos
 // (lanczos.c:89-118) is not applied (BVOrthogonalizeCGS1, bvorthog.c:91-132,
@@ -471,7 +553,8 @@ int ks_solve(const KsParams &p, const Mat &A, BV &V, bool have_initial, KsResult
     void *p = nullptr;
     ~DevBuf() { if (p) (void)hipFree(p); }
   } wtmp;
-  if (lanc && p.lanczos_local) SA_HIP(hipMalloc(&wtmp.p, prec_size(V.prec) * (size_t)std::max(1, n)));
+  if (lanc && (p.lanczos_local || p.lanczos_partial == 4))   // local: one vector; delayed: u and t
+    SA_HIP(hipMalloc(&wtmp.p, prec_size(V.prec) * (size_t)std::max(1, n) * (p.lanczos_partial == 4 ? 2 : 1)));
   // selective: the basis of nearly converged Ritz vectors (BVDuplicate of V, lanczos.c:63-71)
   struct BVHold {
     BV b;
@@ -549,9 +632,21 @@ int ks_solve(const KsParams &p, const Mat &A, BV &V, bool have_initial, KsResult
     int breakdown = 0;
     const int64_t pbefore = V.passes_done;
     const bool partial = lanc && p.lanczos_partial != 0;
-    if (partial && p.lanczos_partial == 3)
+    if (partial && p.lanczos_partial == 3) {
       SA_TRY(lz_sweep_selective(V, AV, op_apply, k0, &nv, &beta, &breakdown, anorm, p.eta, ds.d(), ds.e(), r));
-    else if (partial)
+    } else if (partial && p.lanczos_partial == 4) {   // lanczos.c:495-512
+      std::vector<double> Td((size_t)nv * nv, 0.0);
+      const size_t vb = prec_size(V.prec) * (size_t)n;
+      SA_TRY(lz_sweep_delayed(V, op_apply, k0, nv, Td.data(), &beta, wtmp.p, static_cast<char *>(wtmp.p) + vb, r));
+      breakdown = 0;
+      double *d = ds.d(), *e = ds.e();
+      for (int i = k0; i < nv - 1; i++) {
+        d[i] = Td[(size_t)nv * i + i];
+        e[i] = Td[(size_t)nv * i + i + 1];
+      }
+      d[nv - 1] = Td[(size_t)nv * (nv - 1) + nv - 1];
+      e[nv - 1] = beta;
+    } else if (partial)
       SA_TRY(lz_sweep_partial(V, op_apply, k0, &nv, &beta, &breakdown, anorm, p.lanczos_partial == 1, ncv, p.eta,
                               ds.d(), ds.e(), r));
     else

@@ -460,6 +460,50 @@ def test_oracle_lanczos_local_test8(orc, case, reorthog):
     assert np.max(np.abs(r["eigr"][:nev] - ex)) < 1e-7 * np.max(np.abs(ex))
 
 
+def _tridiag_random(orc, n=30, seed=5):
+    """eps/tests/test3.c's first matrix: tridiag(-1, d, -1) with a random
+    diagonal d (here numpy's, seed 5: PetscRandom's stream is not available)."""
+    d = np.random.default_rng(seed).random(n)
+    rp, ci, va = [0], [], []
+    for i in range(n):
+        for c, v in ((i - 1, -1.0), (i, d[i]), (i + 1, -1.0)):
+            if 0 <= c < n:
+                ci.append(c)
+                va.append(v)
+        rp.append(len(ci))
+    return orc.CSR(n, np.array(rp, np.int32), np.array(ci, np.int32), np.array(va))
+
+
+@pytest.mark.parametrize("case", ["test2_LR", "test2_interior", "test8_2", "test3_tridiag"])
+def test_oracle_lanczos_delayed(orc, case):
+    """EPSLanczos with delayed reorthogonalization (EPSDelayedArnoldi) where
+    it converges: test2 largest and interior (sinvert at 2.1) values of
+    test2_1.out, test8_2.out, and the smallest four of a test3-like
+    tridiagonal at test3_2_lanczos_delayed's tol 1e-8 against dense LAPACK,
+    restarting exactly as full reorthogonalization does there."""
+    kw = {}
+    if case.startswith("test2"):
+        A, nev, ncv, mpd, tol, mi = _lap1d(orc, 30), 4, 0, 0, 1e-9, 0
+        which = "LR" if case == "test2_LR" else "TM"
+        if which == "TM":
+            kw = dict(target=2.1, st="sinvert")
+        gold = TEST2_LARGEST if which == "LR" else TEST2_INTERIOR[30]
+    elif case == "test8_2":
+        m, nev, ncv, mpd, which, mi, gold = TEST8["2"]
+        A, tol = orc.lap2d(m), 1e-8
+    else:
+        A, nev, ncv, mpd, tol, mi, which = _tridiag_random(orc), 4, 0, 0, 1e-8, 500, "SR"
+        gold = list(np.round(np.sort(np.linalg.eigvalsh(A.to_dense()))[:4], 5))
+    v0 = orc.rand_uniform(0, A.n)
+    r = orc.eps_solve(A, nev, ncv, tol, which, True, v0=v0, max_it=mi, mpd=mpd, solver="lanczos",
+                      reorthog="delayed", **kw)
+    assert r["nconv"] >= nev
+    assert np.allclose(r["eigr"][:nev], gold, atol=5e-6)
+    if case == "test3_tridiag":
+        f = orc.eps_solve(A, nev, ncv, tol, which, True, v0=v0, max_it=mi, solver="lanczos")
+        assert f["its"] == r["its"]
+
+
 def test_oracle_lanczos_partial_reorthogonalizes(orc):
     """Periodic and partial reorthogonalization run extra Gram-Schmidt passes
     when the omega estimate of the loss of orthogonality exceeds

@@ -459,8 +459,8 @@ def test_lanczos_local_matches_oracle(sa, orc, case, reorthog):
         v0 = orc.rand_uniform(0, Ao.n)
     E = sa.EPS()
     E.setType("lanczos")
-    with pytest.raises(Exception):   # delayed: not available
-        sa._lib.check(sa._lib.lib().sa_eps_lanczos_set_reorthog(E.h, 5))
+    with pytest.raises(Exception):   # not an EPSLanczosReorthogType
+        sa._lib.check(sa._lib.lib().sa_eps_lanczos_set_reorthog(E.h, 6))
     E.setLanczosReorthog(reorthog)
     assert E.getLanczosReorthog() == reorthog
     E.setOperators(sa.Mat(sa.CSR(Ao.n, Ao.rowptr, Ao.col, Ao.val)))
@@ -485,6 +485,50 @@ def test_lanczos_local_matches_oracle(sa, orc, case, reorthog):
     bound = 10 * tol * (max(1.0, 4.2 / np.min(np.abs(vals))) if sinv else 1.0)
     for i in range(nev):
         assert E.computeError(i) < bound
+
+
+@pytest.mark.parametrize("case", ["test2_LR", "test2_interior", "test8_2", "test3_tridiag"])
+def test_lanczos_delayed_matches_oracle(sa, orc, case):
+    """EPSLanczos with delayed reorthogonalization through the C ABI against
+    the oracle from the same start vector, on the cases where the delayed
+    scheme converges (see test_oracle_lanczos_delayed): eigenvalues to 1e-9
+    and the printed / dense values."""
+    from test_cpu_oracle import _lap1d, _tridiag_random, TEST2_LARGEST, TEST2_INTERIOR, TEST8
+    sinv, mpd = False, 0
+    if case.startswith("test2"):
+        Ao, nev, ncv, tol, mi = _lap1d(orc, 30), 4, 0, 1e-9, 0
+        which = "LR" if case == "test2_LR" else "TM"
+        sinv = which == "TM"
+        gold = TEST2_LARGEST if which == "LR" else TEST2_INTERIOR[30]
+    elif case == "test8_2":
+        m, nev, ncv, mpd, which, mi, gold = TEST8["2"]
+        Ao, tol = orc.lap2d(m), 1e-8
+    else:
+        Ao, nev, ncv, tol, mi, which = _tridiag_random(orc), 4, 0, 1e-8, 500, "SR"
+        gold = np.sort(np.linalg.eigvalsh(Ao.to_dense()))[:4]
+    v0 = orc.rand_uniform(0, Ao.n)
+    E = sa.EPS()
+    E.setType("lanczos")
+    E.setLanczosReorthog("delayed")
+    assert E.getLanczosReorthog() == "delayed"
+    E.setOperators(sa.Mat(sa.CSR(Ao.n, Ao.rowptr, Ao.col, Ao.val)))
+    E.setProblemType(True)
+    E.setWhichEigenpairs(which)
+    if sinv:
+        E.setTarget(2.1)
+        E.setST("sinvert")
+        E.setSTKSP("jacobi", 1e-13, 20000)
+    E.setDimensions(nev, ncv, mpd)
+    E.setTolerances(tol, mi)
+    E.setInitialSpace(v0)
+    E.solve()
+    kw = dict(target=2.1, st="sinvert") if sinv else {}
+    r = orc.eps_solve(Ao, nev, ncv, tol, which, True, v0=v0, solver="lanczos", reorthog="delayed", max_it=mi,
+                      mpd=mpd, **kw)
+    assert E.getConverged() >= nev and r["nconv"] >= nev
+    vals = np.array([E.getEigenpair(i, False)[0] for i in range(nev)])
+    assert np.max(np.abs(vals - r["eigr"][:nev]) / np.abs(r["eigr"][:nev])) < 1e-9
+    assert np.allclose(vals, gold, atol=5e-6)
 
 
 @pytest.mark.parametrize("solver,case", [("krylovschur", "test2_30"), ("krylovschur", "test2_32"),
