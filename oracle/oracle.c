@@ -16,6 +16,9 @@
  *   SlepcSortEigenvalues           src/sys/slepcsc.c:89-140
  *   EPSArnoldi / EPSLanczos        src/eps/impls/krylov/arnoldi/arnoldi.c:31-131, lanczos/lanczos.c:518-697
  *   EPSLanczos local (MGS)         lanczos/lanczos.c:58-62,89-118; bvorthog.c:52-85,145-215
+ *   EPSLanczos periodic/partial    lanczos/lanczos.c:260-441
+ *   EPSLanczos selective           lanczos/lanczos.c:139-258 (LAPACK stevr)
+ *   EPSLanczos delayed             epskrylov.c:24-114, lanczos/lanczos.c:495-512
  *   BVDot / BVMatMult              src/sys/classes/bv/interface/bvglobal.c:86-120, bvops.c:692-722
  *   BVOrthogonalize (gs/chol/svqb) src/sys/classes/bv/interface/bvorthog.c:489-767, bvlapack.c:136-330
  * The dense projected problems call host LAPACK exactly where the reference does
