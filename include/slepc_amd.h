@@ -227,6 +227,12 @@ int sa_eps_set_st(sa_eps eps, int st_type, int shift_given, double shift);
 #define SA_PC_NONE 0
 #define SA_PC_JACOBI 1
 int sa_eps_set_st_ksp(sa_eps eps, int pc_type, double rtol, int max_it);
+/* EPSArnoldiSetDelayed / EPSArnoldiGetDelayed (arnoldi.c:148-220): delayed
+   reorthogonalization in SA_EPS_ARNOLDI (EPSDelayedArnoldi,
+   epskrylov.c:24-114, writing the projected Hessenberg matrix; the reference
+   itself warns that it may be unstable, test9.c). */
+int sa_eps_arnoldi_set_delayed(sa_eps eps, int delayed);
+int sa_eps_arnoldi_get_delayed(sa_eps eps, int *delayed);
 /* EPSLanczosSetReorthog / EPSLanczosGetReorthog (lanczos.c:716-775; enum
    EPSLanczosReorthogType, include/slepceps.h:445): reorthogonalization of
    SA_EPS_LANCZOS.  FULL (the default here; the reference requires an explicit

@@ -1295,8 +1295,9 @@ static int ks_expand_selective(KSo *ks, int k, int *M, double *betam, int *break
    of v_{j+1} and v_j with the earlier columns are taken before v_j is
    normalised, the coefficients rescaled; the unnormalised v_j (t) is
    reorthogonalised against v_0..v_{j-1} and, one step later, normalised into
-   column j.  H: m x m, leading dimension m (the caller's T). */
-static void ks_expand_delayed(KSo *ks, int k, int m, double *H, double *betam)
+   column j.  H: leading dimension ldh >= m (Lanczos: a scratch T; Arnoldi:
+   the projected problem's A). */
+static void ks_expand_delayed(KSo *ks, int k, int m, double *H, int ldh, double *betam)
 {
   const int64_t N = ks->n, ld = ks->ld;
   double *V = ks->V;
@@ -1306,38 +1307,38 @@ static void ks_expand_delayed(KSo *ks, int k, int m, double *H, double *betam)
 #define COL(c) (V + (int64_t)(c) * ld)
   for (int j = k; j < m; j++) {
     ks_op(ks, COL(j), COL(j + 1));
-    orc_bv_dotvec(N, j + 1, V, ld, COL(j + 1), H + (int64_t)m * j);   /* BVDotColumn(V, j+1) */
+    orc_bv_dotvec(N, j + 1, V, ld, COL(j + 1), H + (int64_t)ldh * j);   /* BVDotColumn(V, j+1) */
     if (j > k) {
       orc_bv_dotvec(N, j, V, ld, COL(j), lhh);                          /* BVDotColumn(V, j) */
       double dot = 0.0;
       for (int64_t r = 0; r < N; r++) dot += COL(j)[r] * COL(j)[r];
       if (j > k + 1) norm2 = orc_bv_norm2(N, u);
       norm1 = sqrt(dot);
-      for (int i = 0; i < j; i++) H[(int64_t)m * j + i] /= norm1;
-      H[(int64_t)m * j + j] /= dot;
+      for (int i = 0; i < j; i++) H[(int64_t)ldh * j + i] /= norm1;
+      H[(int64_t)ldh * j + j] /= dot;
       memcpy(t, COL(j), sizeof(double) * (size_t)N);
       orc_bv_scale(N, COL(j), 1.0 / norm1);
       orc_bv_scale(N, COL(j + 1), 1.0 / norm1);
     }
-    orc_bv_multvec(N, j + 1, -1.0, V, ld, H + (int64_t)m * j, 1.0, COL(j + 1));   /* BVMultColumn(V, j+1) */
+    orc_bv_multvec(N, j + 1, -1.0, V, ld, H + (int64_t)ldh * j, 1.0, COL(j + 1));   /* BVMultColumn(V, j+1) */
     if (j > k) {
       orc_bv_multvec(N, j, -1.0, V, ld, lhh, 1.0, t);
-      for (int i = 0; i < j; i++) H[(int64_t)m * (j - 1) + i] += lhh[i];
+      for (int i = 0; i < j; i++) H[(int64_t)ldh * (j - 1) + i] += lhh[i];
     }
     if (j > k + 1) {
       memcpy(COL(j - 1), u, sizeof(double) * (size_t)N);
       orc_bv_scale(N, COL(j - 1), 1.0 / norm2);
-      H[(int64_t)m * (j - 2) + j - 1] = norm2;
+      H[(int64_t)ldh * (j - 2) + j - 1] = norm2;
     }
     if (j < m - 1) memcpy(u, t, sizeof(double) * (size_t)N);
   }
   norm2 = orc_bv_norm2(N, t);
   orc_bv_scale(N, t, 1.0 / norm2);
   memcpy(COL(m - 1), t, sizeof(double) * (size_t)N);
-  H[(int64_t)m * (m - 2) + m - 1] = norm2;
+  H[(int64_t)ldh * (m - 2) + m - 1] = norm2;
   orc_bv_dotvec(N, m, V, ld, COL(m), lhh);
   orc_bv_multvec(N, m, -1.0, V, ld, lhh, 1.0, COL(m));
-  for (int i = 0; i < m; i++) H[(int64_t)m * (m - 1) + i] += lhh[i];
+  for (int i = 0; i < m; i++) H[(int64_t)ldh * (m - 1) + i] += lhh[i];
   *betam = orc_bv_norm2(N, COL(m));
   orc_bv_scale(N, COL(m), 1.0 / *betam);
 #undef COL
@@ -1462,17 +1463,22 @@ int orc_eps_solve(int n, const int32_t *rp, const int32_t *ci, const double *val
     } else if (partial == 4) {                          /* lanczos.c:495-512 */
       if (nv - k0 < 2) { rc = -14; goto done; }
       double *Td = (double *)calloc((size_t)nv * nv, sizeof(double));
-      ks_expand_delayed(&ks, k0, nv, Td, &beta);
+      ks_expand_delayed(&ks, k0, nv, Td, nv, &beta);
       breakdown = 0;
       for (int i = k0; i < nv - 1; i++) { d[i] = Td[(int64_t)nv * i + i]; e[i] = Td[(int64_t)nv * i + i + 1]; }
       d[nv - 1] = Td[(int64_t)nv * (nv - 1) + nv - 1];
       e[nv - 1] = beta;
       free(Td);
     } else if (partial) ks_expand_partial(&ks, k0, &nv, &beta, &breakdown, anorm, o->lanczos_partial == 1, d, e);
+    else if (arn && o->arnoldi_delayed) {                /* arnoldi.c:88-91 */
+      if (nv - k0 < 2) { rc = -14; goto done; }
+      ks_expand_delayed(&ks, k0, nv, ds.A, ld, &beta);
+      breakdown = 0;
+    }
     else if (ks.mgs) ks_expand_local(&ks, k0, &nv, &beta, &breakdown);
     else ks_expand(&ks, k0, &nv, &beta, &breakdown);
     int bm = ks.m;
-    if (partial) {
+    if (partial || (arn && o->arnoldi_delayed)) {
       /* the sweep wrote the projected problem itself */
     } else if (herm) {
       for (int j = k0; j < nv; j++) { d[j] = ks.buf[j + (int64_t)(j + 1) * bm]; e[j] = ks.buf[j + 1 + (int64_t)(j + 1) * bm]; }

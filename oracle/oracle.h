@@ -82,6 +82,7 @@ typedef struct {
   int     st;          /* 0 shift (sigma = 0), 1 shift-and-invert (STSINVERT) */
   double  sigma;       /* shift of the shift-and-invert operator */
   int     lanczos_local; /* Lanczos: 1 -eps_lanczos_reorthog local, 0 full */
+  int     arnoldi_delayed; /* EPSArnoldiSetDelayed */
   int     lanczos_partial; /* Lanczos: 1 periodic, 2 partial, 3 selective, 4 delayed (lanczos_local 0) */
 } orc_opts;
 

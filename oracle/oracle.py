@@ -24,7 +24,7 @@ class Opts(C.Structure):
                 ("lock", C.c_int), ("nthreads", C.c_int), ("solver", C.c_int),
                 ("extraction", C.c_int), ("target", C.c_double),
                 ("st", C.c_int), ("sigma", C.c_double), ("lanczos_local", C.c_int),
-                ("lanczos_partial", C.c_int)]
+                ("arnoldi_delayed", C.c_int), ("lanczos_partial", C.c_int)]
 
 
 class Result(C.Structure):
@@ -271,6 +271,7 @@ def eps_solve(A, nev, ncv=0, tol=1e-8, which="LM", hermitian=True, v0=None, max_
              extraction={"ritz": 0, "harmonic": 1}[extraction], target=float(target),
              st={"shift": 0, "sinvert": 1}[st], sigma=float(target if sigma is None else sigma),
              lanczos_local=1 if reorthog == "local" else 0,
+             arnoldi_delayed=1 if (solver == "arnoldi" and reorthog == "delayed") else 0,
              lanczos_partial={"full": 0, "local": 0, "periodic": 1, "partial": 2, "selective": 3,
                                "delayed": 4}[reorthog],
              max_it=max_it, tol=tol, keep=keep, eta=0.7071, lock=lock, nthreads=nthreads,

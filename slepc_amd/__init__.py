@@ -374,6 +374,16 @@ class EPS:
         Jacobi ("jacobi") or no ("none") preconditioner."""
         check(lib().sa_eps_set_st_ksp(self.h, {"none": 0, "jacobi": 1}[pc], float(rtol), int(max_it)))
 
+    def setArnoldiDelayed(self, delayed=True):
+        """EPSArnoldiSetDelayed: delayed reorthogonalization in EPSARNOLDI."""
+        check(lib().sa_eps_arnoldi_set_delayed(self.h, 1 if delayed else 0))
+
+    def getArnoldiDelayed(self):
+        """EPSArnoldiGetDelayed."""
+        v = C.c_int()
+        check(lib().sa_eps_arnoldi_get_delayed(self.h, C.byref(v)))
+        return bool(v.value)
+
     def setLanczosReorthog(self, reorthog):
         """EPSLanczosSetReorthog: "local", "full" (the default here), "selective",
         "periodic", "partial" or "delayed"."""

@@ -101,6 +101,8 @@ SIGNATURES = {
     "sa_eps_set_target": (C.c_int, [_VP, C.c_double]),
     "sa_eps_set_extraction": (C.c_int, [_VP, C.c_int]),
     "sa_eps_set_two_sided": (C.c_int, [_VP, C.c_int]),
+    "sa_eps_arnoldi_set_delayed": (C.c_int, [_VP, C.c_int]),
+    "sa_eps_arnoldi_get_delayed": (C.c_int, [_VP, C.POINTER(C.c_int)]),
     "sa_eps_lanczos_set_reorthog": (C.c_int, [_VP, C.c_int]),
     "sa_eps_lanczos_get_reorthog": (C.c_int, [_VP, C.POINTER(C.c_int)]),
     "sa_eps_set_st": (C.c_int, [_VP, C.c_int, C.c_int, C.c_double]),

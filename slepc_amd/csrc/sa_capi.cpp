@@ -518,6 +518,19 @@ int sa_eps_set_st_ksp(sa_eps E, int pc_type, double rtol, int max_it) {
   return 0;
 }
 
+int sa_eps_arnoldi_set_delayed(sa_eps E, int delayed) {
+  SA_CHECK(E, "null EPS");
+  E->p.arnoldi_delayed = delayed != 0;
+  E->solved = false;
+  return 0;
+}
+
+int sa_eps_arnoldi_get_delayed(sa_eps E, int *delayed) {
+  SA_CHECK(E && delayed, "null argument");
+  *delayed = E->p.arnoldi_delayed ? 1 : 0;
+  return 0;
+}
+
 int sa_eps_lanczos_set_reorthog(sa_eps E, int reorthog) {
   SA_CHECK(E, "null EPS");
   SA_CHECK(reorthog >= SA_LANCZOS_REORTHOG_LOCAL && reorthog <= SA_LANCZOS_REORTHOG_DELAYED,

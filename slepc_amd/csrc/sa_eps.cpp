@@ -298,10 +298,11 @@ static int lz_sweep_selective(BV &V, BV &AV, const std::function<int(const void 
 // one step and whose reorthogonalisation by two: at step j the dot products of
 // v_{j+1} and v_j with the earlier columns are taken before v_j is normalised
 // and the coefficients rescaled; the unnormalised v_j (t) is reorthogonalised
-// against v_0..v_{j-1} and, one step later, normalised into column j.  H: host
-// m x m, leading dimension m.  u, t: device vectors of V.n entries.
+// against v_0..v_{j-1} and, one step later, normalised into column j.  H: host,
+// leading dimension ldh >= m (Lanczos: a scratch T; Arnoldi, arnoldi.c:88-91:
+// the projected problem's A).  u, t: device vectors of V.n entries.
 static int lz_sweep_delayed(BV &V, const std::function<int(const void *, void *)> &op, int k, int m, double *H,
-                            double *betam, void *u, void *t, KsResult &r) {
+                            int ldh, double *betam, void *u, void *t, KsResult &r) {
   hipStream_t s = stream();
   SA_CHECK(m - k >= 2, "the delayed Lanczos sweep needs at least two steps (ncv - nconv >= 2)");
   const size_t vb = prec_size(V.prec) * (size_t)V.n;
@@ -331,7 +332,7 @@ static int lz_sweep_delayed(BV &V, const std::function<int(const void *, void *)
   };
   for (int j = k; j < m; j++) {
     SA_TRY(op(col_ptr(V, j), col_ptr(V, j + 1)));
-    double *Hj = H + (size_t)m * j;
+    double *Hj = H + (size_t)ldh * j;
     SA_TRY(dotcol(j + 1, j + 1, Hj));
     if (j > k) {
       SA_TRY(dotcol(j, j, lhh.data()));
@@ -351,12 +352,12 @@ static int lz_sweep_delayed(BV &V, const std::function<int(const void *, void *)
     SA_TRY(multcol(j + 1, Hj, col_ptr(V, j + 1)));
     if (j > k) {
       SA_TRY(multcol(j, lhh.data(), t));
-      for (int i = 0; i < j; i++) H[(size_t)m * (j - 1) + i] += lhh[i];
+      for (int i = 0; i < j; i++) H[(size_t)ldh * (j - 1) + i] += lhh[i];
     }
     if (j > k + 1) {
       SA_HIP(hipMemcpyAsync(col_ptr(V, j - 1), u, vb, hipMemcpyDeviceToDevice, s));
       SA_TRY(launch_scale(V.prec, col_ptr(V, j - 1), V.n, 1.0 / norm2, s));
-      H[(size_t)m * (j - 2) + j - 1] = norm2;
+      H[(size_t)ldh * (j - 2) + j - 1] = norm2;
     }
     if (j < m - 1) SA_HIP(hipMemcpyAsync(u, t, vb, hipMemcpyDeviceToDevice, s));
   }
@@ -364,10 +365,10 @@ static int lz_sweep_delayed(BV &V, const std::function<int(const void *, void *)
   norm2 = std::sqrt(h);
   SA_TRY(launch_scale(V.prec, t, V.n, 1.0 / norm2, s));
   SA_HIP(hipMemcpyAsync(col_ptr(V, m - 1), t, vb, hipMemcpyDeviceToDevice, s));
-  H[(size_t)m * (m - 2) + m - 1] = norm2;
+  H[(size_t)ldh * (m - 2) + m - 1] = norm2;
   SA_TRY(dotcol(m, m, lhh.data()));
   SA_TRY(multcol(m, lhh.data(), col_ptr(V, m)));
-  for (int i = 0; i < m; i++) H[(size_t)m * (m - 1) + i] += lhh[i];
+  for (int i = 0; i < m; i++) H[(size_t)ldh * (m - 1) + i] += lhh[i];
   SA_TRY(sumsq(col_ptr(V, m), &h));
   *betam = std::sqrt(h);
   SA_TRY(launch_scale(V.prec, col_ptr(V, m), V.n, 1.0 / *betam, s));
@@ -553,8 +554,9 @@ int ks_solve(const KsParams &p, const Mat &A, BV &V, bool have_initial, KsResult
     void *p = nullptr;
     ~DevBuf() { if (p) (void)hipFree(p); }
   } wtmp;
-  if (lanc && (p.lanczos_local || p.lanczos_partial == 4))   // local: one vector; delayed: u and t
-    SA_HIP(hipMalloc(&wtmp.p, prec_size(V.prec) * (size_t)std::max(1, n) * (p.lanczos_partial == 4 ? 2 : 1)));
+  const bool delayed = (lanc && p.lanczos_partial == 4) || (arn && p.arnoldi_delayed);
+  if ((lanc && p.lanczos_local) || delayed)   // local: one vector; delayed: u and t
+    SA_HIP(hipMalloc(&wtmp.p, prec_size(V.prec) * (size_t)std::max(1, n) * (delayed ? 2 : 1)));
   // selective: the basis of nearly converged Ritz vectors (BVDuplicate of V, lanczos.c:63-71)
   struct BVHold {
     BV b;
@@ -637,7 +639,8 @@ int ks_solve(const KsParams &p, const Mat &A, BV &V, bool have_initial, KsResult
     } else if (partial && p.lanczos_partial == 4) {   // lanczos.c:495-512
       std::vector<double> Td((size_t)nv * nv, 0.0);
       const size_t vb = prec_size(V.prec) * (size_t)n;
-      SA_TRY(lz_sweep_delayed(V, op_apply, k0, nv, Td.data(), &beta, wtmp.p, static_cast<char *>(wtmp.p) + vb, r));
+      SA_TRY(lz_sweep_delayed(V, op_apply, k0, nv, Td.data(), nv, &beta, wtmp.p, static_cast<char *>(wtmp.p) + vb,
+                              r));
       breakdown = 0;
       double *d = ds.d(), *e = ds.e();
       for (int i = k0; i < nv - 1; i++) {
@@ -646,6 +649,10 @@ int ks_solve(const KsParams &p, const Mat &A, BV &V, bool have_initial, KsResult
       }
       d[nv - 1] = Td[(size_t)nv * (nv - 1) + nv - 1];
       e[nv - 1] = beta;
+    } else if (arn && p.arnoldi_delayed) {   // arnoldi.c:88-91
+      SA_TRY(lz_sweep_delayed(V, op_apply, k0, nv, ds.A(), ld, &beta, wtmp.p,
+                              static_cast<char *>(wtmp.p) + prec_size(V.prec) * (size_t)n, r));
+      breakdown = 0;
     } else if (partial)
       SA_TRY(lz_sweep_partial(V, op_apply, k0, &nv, &beta, &breakdown, anorm, p.lanczos_partial == 1, ncv, p.eta,
                               ds.d(), ds.e(), r));
@@ -655,7 +662,8 @@ int ks_solve(const KsParams &p, const Mat &A, BV &V, bool have_initial, KsResult
     // Algorithmic bytes of the sweep: SpMV, then per Gram-Schmidt pass the dot
     // products read columns 0..c-1 and the update reads them again plus v and
     // writes v, i.e. (2c+2) n s per pass; passes averaged over the sweep.
-    if (nv > k0 && !(lanc && (p.lanczos_local || partial))) {   // (the MGS sweeps count their own bytes)
+    if (nv > k0 && !(lanc && (p.lanczos_local || partial)) && !(arn && p.arnoldi_delayed)) {
+      // (the other sweeps count their own bytes)
       const double avgp = (double)(V.passes_done - pbefore) / (double)(nv - k0);
       for (int c = k0 + 1; c <= nv; c++) r.bytes_bv += avgp * (2.0 * c + 2.0) * V.n * es;
     }
@@ -668,7 +676,7 @@ int ks_solve(const KsParams &p, const Mat &A, BV &V, bool have_initial, KsResult
     SA_HIP(hipStreamSynchronize(s));
     const double td0 = now();
     auto B = [&](int row, int colc) { return buf[(size_t)(colc - k0 - 1) * m + row]; };
-    if (partial) {
+    if (partial || (arn && p.arnoldi_delayed)) {
       // the sweep wrote the projected problem itself
     } else if (dsherm) {
       double *d = ds.d(), *e = ds.e();

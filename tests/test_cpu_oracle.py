@@ -504,6 +504,19 @@ def test_oracle_lanczos_delayed(orc, case):
         assert f["its"] == r["its"]
 
 
+def test_oracle_arnoldi_delayed_test9_3(orc):
+    """eps/tests/test9.c suffix 3: -eps_type arnoldi -eps_arnoldi_delayed
+    -eps_largest_real -eps_nev 3 -eps_tol 1e-7 on the Markov model m = 15:
+    the values of test9_3.out, and the same restarts as without delay."""
+    A = orc.markov(15)
+    v0 = orc.rand_uniform(0, A.n)
+    r = orc.eps_solve(A, 3, 0, 1e-7, "LR", False, v0=v0, solver="arnoldi", reorthog="delayed")
+    f = orc.eps_solve(A, 3, 0, 1e-7, "LR", False, v0=v0, solver="arnoldi")
+    assert r["nconv"] >= 3
+    assert list(np.round(r["eigr"][:3], 5)) == [1.0, 0.97137, 0.90423]
+    assert r["its"] == f["its"]
+
+
 def test_oracle_lanczos_partial_reorthogonalizes(orc):
     """Periodic and partial reorthogonalization run extra Gram-Schmidt passes
     when the omega estimate of the loss of orthogonality exceeds

@@ -531,6 +531,31 @@ def test_lanczos_delayed_matches_oracle(sa, orc, case):
     assert np.allclose(vals, gold, atol=5e-6)
 
 
+def test_arnoldi_delayed_matches_oracle(sa, orc):
+    """EPSArnoldi with delayed reorthogonalization (test9.c suffix 3: Markov
+    m = 15, 3 largest real, tol 1e-7) through the C ABI against the oracle:
+    the values of test9_3.out in order and the restart count."""
+    Ao = orc.markov(15)
+    v0 = orc.rand_uniform(0, Ao.n)
+    E = sa.EPS()
+    E.setType("arnoldi")
+    E.setArnoldiDelayed(True)
+    assert E.getArnoldiDelayed()
+    E.setOperators(sa.Mat(sa.CSR(Ao.n, Ao.rowptr, Ao.col, Ao.val)))
+    E.setProblemType(False)
+    E.setWhichEigenpairs("LR")
+    E.setDimensions(3, 0)
+    E.setTolerances(1e-7, 0)
+    E.setInitialSpace(v0)
+    E.solve()
+    r = orc.eps_solve(Ao, 3, 0, 1e-7, "LR", False, v0=v0, solver="arnoldi", reorthog="delayed")
+    assert E.getConverged() >= 3
+    vals = np.array([E.getEigenpair(i, False)[0] for i in range(3)])
+    assert list(np.round(vals, 5)) == [1.0, 0.97137, 0.90423]
+    assert np.max(np.abs(vals - r["eigr"][:3])) < 1e-9
+    assert abs(E.getIterationNumber() - r["its"]) <= 1
+
+
 @pytest.mark.parametrize("solver,case", [("krylovschur", "test2_30"), ("krylovschur", "test2_32"),
                                          ("arnoldi", "test2_30"), ("arnoldi", "test2_32"),
                                          ("krylovschur", "convdiff8")])
