@@ -227,6 +227,15 @@ int sa_eps_set_st(sa_eps eps, int st_type, int shift_given, double shift);
 #define SA_PC_NONE 0
 #define SA_PC_JACOBI 1
 int sa_eps_set_st_ksp(sa_eps eps, int pc_type, double rtol, int max_it);
+/* EPSKrylovSchurSetRestart / GetRestart (krylovschur.c:350-410): the
+   fraction keep in [0.1, 0.9] of the non-converged basis kept at restart
+   (default 0.5).  EPSKrylovSchurSetLocking / GetLocking (krylovschur.c:
+   427-490): with lock 0 converged pairs stay in the active basis (the
+   restart keeps them among the l vectors, krylovschur.c:295); default 1. */
+int sa_eps_krylovschur_set_restart(sa_eps eps, double keep);
+int sa_eps_krylovschur_get_restart(sa_eps eps, double *keep);
+int sa_eps_krylovschur_set_locking(sa_eps eps, int lock);
+int sa_eps_krylovschur_get_locking(sa_eps eps, int *lock);
 /* EPSArnoldiSetDelayed / EPSArnoldiGetDelayed (arnoldi.c:148-220): delayed
    reorthogonalization in SA_EPS_ARNOLDI (EPSDelayedArnoldi,
    epskrylov.c:24-114, writing the projected Hessenberg matrix; the reference

@@ -374,6 +374,24 @@ class EPS:
         Jacobi ("jacobi") or no ("none") preconditioner."""
         check(lib().sa_eps_set_st_ksp(self.h, {"none": 0, "jacobi": 1}[pc], float(rtol), int(max_it)))
 
+    def setKrylovSchurRestart(self, keep):
+        """EPSKrylovSchurSetRestart: fraction of the basis kept at restart."""
+        check(lib().sa_eps_krylovschur_set_restart(self.h, float(keep)))
+
+    def getKrylovSchurRestart(self):
+        v = C.c_double()
+        check(lib().sa_eps_krylovschur_get_restart(self.h, C.byref(v)))
+        return v.value
+
+    def setKrylovSchurLocking(self, lock):
+        """EPSKrylovSchurSetLocking: locking (True, default) or non-locking."""
+        check(lib().sa_eps_krylovschur_set_locking(self.h, 1 if lock else 0))
+
+    def getKrylovSchurLocking(self):
+        v = C.c_int()
+        check(lib().sa_eps_krylovschur_get_locking(self.h, C.byref(v)))
+        return bool(v.value)
+
     def setArnoldiDelayed(self, delayed=True):
         """EPSArnoldiSetDelayed: delayed reorthogonalization in EPSARNOLDI."""
         check(lib().sa_eps_arnoldi_set_delayed(self.h, 1 if delayed else 0))

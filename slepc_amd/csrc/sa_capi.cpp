@@ -518,6 +518,33 @@ int sa_eps_set_st_ksp(sa_eps E, int pc_type, double rtol, int max_it) {
   return 0;
 }
 
+int sa_eps_krylovschur_set_restart(sa_eps E, double keep) {
+  SA_CHECK(E, "null EPS");
+  SA_CHECK(keep >= 0.1 && keep <= 0.9, "The keep argument must be in the range [0.1,0.9]");
+  E->p.keep = keep;
+  E->solved = false;
+  return 0;
+}
+
+int sa_eps_krylovschur_get_restart(sa_eps E, double *keep) {
+  SA_CHECK(E && keep, "null argument");
+  *keep = E->p.keep;
+  return 0;
+}
+
+int sa_eps_krylovschur_set_locking(sa_eps E, int lock) {
+  SA_CHECK(E, "null EPS");
+  E->p.lock = lock != 0;
+  E->solved = false;
+  return 0;
+}
+
+int sa_eps_krylovschur_get_locking(sa_eps E, int *lock) {
+  SA_CHECK(E && lock, "null argument");
+  *lock = E->p.lock ? 1 : 0;
+  return 0;
+}
+
 int sa_eps_arnoldi_set_delayed(sa_eps E, int delayed) {
   SA_CHECK(E, "null EPS");
   E->p.arnoldi_delayed = delayed != 0;

@@ -765,6 +765,11 @@ int ks_solve(const KsParams &p, const Mat &A, BV &V, bool have_initial, KsResult
       if (!p.hermitian) newl = ds.truncate_size(k, nv, newl);
     }
     l = newl;
+    const int kconv = k;
+    if (!p.lock && l > 0) {   // non-locking variant (krylovschur.c:295): reset the converged count
+      l += k;
+      k = 0;
+    }
     tds += now() - td0;
     if (reason == 0) {
       if (breakdown || k == nv) {
@@ -801,7 +806,7 @@ int ks_solve(const KsParams &p, const Mat &A, BV &V, bool have_initial, KsResult
       r.bytes_bv += 2.0 * V.n * es;
     }
     nconv = k;
-    if (p.monitor) p.monitor(p.monitor_ctx, its, nconv, eigr.data(), errest.data(), nv);
+    if (p.monitor) p.monitor(p.monitor_ctx, its, kconv, eigr.data(), errest.data(), nv);
   }
   // DSTruncate(nconv,TRUE); EPSComputeVectors (Schur: V <- V X).  Lanczos
   // leaves Ritz vectors in the leading columns already.

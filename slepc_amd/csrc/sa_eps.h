@@ -21,6 +21,7 @@ struct KsParams {
   int ksp_max_it = 10000;
   bool lanczos_local = false;  // EPSLanczosSetReorthog: local (else full)
   bool arnoldi_delayed = false;  // EPSArnoldiSetDelayed
+  bool lock = true;              // EPSKrylovSchurSetLocking
   int lanczos_partial = 0;     // EPSLanczosSetReorthog: 1 periodic, 2 partial, 3 selective, 4 delayed
   void (*monitor)(void *, int, int, const double *, const double *, int) = nullptr;
   void *monitor_ctx = nullptr;

@@ -200,6 +200,10 @@ int ks_twosided(const KsParams &p, const Mat &A, const Mat &AT, BV &V, BV &W, bo
       newl = ds.truncate_size_ts(k, nv, newl);
     }
     l = newl;
+    if (!p.lock && l > 0) {   // non-locking variant (ks-twosided.c:195)
+      l += k;
+      k = 0;
+    }
     tds += now_s() - td0;
 
     // V(:,nconv:k+l) = V Q, W(:,nconv:k+l) = W Z

@@ -504,6 +504,34 @@ def test_oracle_lanczos_delayed(orc, case):
         assert f["its"] == r["its"]
 
 
+@pytest.mark.parametrize("keep", [0.5, 0.2])
+@pytest.mark.parametrize("lock", [1, 0])
+@pytest.mark.parametrize("case", ["test2_LR", "test2_SR", "test2_interior", "test9_1"])
+def test_oracle_krylovschur_locking_restart(orc, case, lock, keep):
+    """Krylov-Schur with -eps_krylovschur_locking {0,1} (test2.c, test9.c
+    suffix 2 with -eps_balance none) and -eps_krylovschur_restart: the
+    values of test2_1.out (interior with harmonic extraction) and of
+    test9_1.out (Markov m = 15, nev 4, max_it 1500: 1, -1, 0.97137,
+    -0.97137)."""
+    kw = {}
+    if case.startswith("test2"):
+        A, nev, tol, mi, herm = _lap1d(orc, 30), 4, 1e-9, 0, True
+        which = {"test2_LR": "LR", "test2_SR": "SR", "test2_interior": "TM"}[case]
+        if which == "TM":
+            kw = dict(extraction="harmonic", target=2.1)
+        gold = {"LR": TEST2_LARGEST, "SR": TEST2_SMALLEST, "TM": TEST2_INTERIOR[30]}[which]
+    else:
+        A, nev, tol, mi, herm, which = orc.markov(15), 4, 1e-8, 1500, False, "LM"
+        gold = [1.0, -1.0, 0.97137, -0.97137]
+    r = orc.eps_solve(A, nev, 0, tol, which, herm, v0=orc.rand_uniform(0, A.n), max_it=mi, lock=lock, keep=keep,
+                      **kw)
+    assert r["nconv"] >= nev
+    got = list(np.round(r["eigr"][:nev], 5))
+    if case == "test9_1":   # 1 and -1 tie in magnitude: their order depends on rounding
+        got, gold = sorted(got), sorted(gold)
+    assert got == gold
+
+
 def test_oracle_arnoldi_delayed_test9_3(orc):
     """eps/tests/test9.c suffix 3: -eps_type arnoldi -eps_arnoldi_delayed
     -eps_largest_real -eps_nev 3 -eps_tol 1e-7 on the Markov model m = 15:

@@ -531,6 +531,52 @@ def test_lanczos_delayed_matches_oracle(sa, orc, case):
     assert np.allclose(vals, gold, atol=5e-6)
 
 
+@pytest.mark.parametrize("keep", [0.5, 0.2])
+@pytest.mark.parametrize("lock", [True, False])
+@pytest.mark.parametrize("case", ["test2_LR", "test2_interior", "test9_1"])
+def test_krylovschur_locking_restart_matches_oracle(sa, orc, case, lock, keep):
+    """EPSKrylovSchurSetLocking / SetRestart through the C ABI against the
+    oracle from the same start vector: eigenvalues to 1e-9, the printed
+    values (test2_1.out, test9_1.out) and the restart count."""
+    from test_cpu_oracle import _lap1d, TEST2_LARGEST, TEST2_INTERIOR
+    harm = False
+    if case.startswith("test2"):
+        Ao, nev, tol, mi, herm = _lap1d(orc, 30), 4, 1e-9, 0, True
+        which = "LR" if case == "test2_LR" else "TM"
+        harm = which == "TM"
+        gold = TEST2_LARGEST if which == "LR" else TEST2_INTERIOR[30]
+    else:
+        Ao, nev, tol, mi, herm, which = orc.markov(15), 4, 1e-8, 1500, False, "LM"
+        gold = [1.0, -1.0, 0.97137, -0.97137]
+    v0 = orc.rand_uniform(0, Ao.n)
+    E = sa.EPS()
+    E.setKrylovSchurLocking(lock)
+    E.setKrylovSchurRestart(keep)
+    assert E.getKrylovSchurLocking() == lock and abs(E.getKrylovSchurRestart() - keep) < 1e-15
+    with pytest.raises(Exception):
+        E.setKrylovSchurRestart(0.95)
+    E.setOperators(sa.Mat(sa.CSR(Ao.n, Ao.rowptr, Ao.col, Ao.val)))
+    E.setProblemType(herm)
+    E.setWhichEigenpairs(which)
+    if harm:
+        E.setTarget(2.1)
+        E.setExtraction("harmonic")
+    E.setDimensions(nev, 0)
+    E.setTolerances(tol, mi)
+    E.setInitialSpace(v0)
+    E.solve()
+    kw = dict(extraction="harmonic", target=2.1) if harm else {}
+    r = orc.eps_solve(Ao, nev, 0, tol, which, herm, v0=v0, max_it=mi, lock=1 if lock else 0, keep=keep, **kw)
+    assert E.getConverged() >= nev and r["nconv"] >= nev
+    vals = np.array([E.getEigenpair(i, False)[0] for i in range(nev)])
+    ref = r["eigr"][:nev]
+    if case == "test9_1":   # 1 and -1 tie in magnitude: their order depends on rounding
+        vals, ref, gold = np.sort(vals), np.sort(ref), sorted(gold)
+    assert np.max(np.abs(vals - ref) / np.abs(ref)) < 1e-9
+    assert list(np.round(vals, 5)) == list(gold)
+    assert abs(E.getIterationNumber() - r["its"]) <= 1
+
+
 def test_arnoldi_delayed_matches_oracle(sa, orc):
     """EPSArnoldi with delayed reorthogonalization (test9.c suffix 3: Markov
     m = 15, 3 largest real, tol 1e-7) through the C ABI against the oracle:
