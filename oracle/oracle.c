@@ -1402,8 +1402,10 @@ int orc_eps_solve(int n, const int32_t *rp, const int32_t *ci, const double *val
     if (ncv > nev + mpd) return -11;
     herm = 0;
   }
-  /* EPSLANCZOS, full reorthogonalization (lanczos.c:518-697): explicit
-     restart from the most wanted unconverged Ritz vector, DSHEP */
+  /* EPSLANCZOS (lanczos.c:518-697): explicit restart from the most wanted
+     unconverged Ritz vector, DSHEP; the sweep follows the reorthogonalization
+     type (ks_expand for full, ks_expand_local / _partial / _selective /
+     _delayed for the others) */
   const int lanc = o->solver == 2;
   if (lanc && (ncv > nev + mpd || !herm || harm)) return -11;
   if (lanc && o->lanczos_partial == 3 && !LP.stevr) return -10;

@@ -376,13 +376,12 @@ static int lz_sweep_delayed(BV &V, const std::function<int(const void *, void *)
   return 0;
 }
 
-// Local reorthogonalization (p.lanczos_local, -eps_lanczos_reorthog local):
-// with the default classical Gram-Schmidt the column mask of EPSLocalLanczos
-// (lanczos.c:89-118) is not applied (BVOrthogonalizeCGS1, bvorthog.c:91-132,
-// ignores it), so the sweep is the full one; the variant adds the restart
-// steps of lanczos.c:570-575 (repeated Ritz values), 633-660 (spurious Ritz
-// values: true residual of each converged vector through the operator op)
-// and 678-681 (reorthonormalised restart vector).  wtmp: one device vector.
+// Restart of EPSLanczos (lanczos.c:548-697).  Local reorthogonalization
+// (p.lanczos_local, -eps_lanczos_reorthog local; its sweep is sweep_local in
+// ks_solve) adds the restart steps of lanczos.c:570-575 (repeated Ritz
+// values), 633-660 (spurious Ritz values: true residual of each converged
+// vector through the operator op) and 678-681 (reorthonormalised restart
+// vector, MGS).  wtmp: one device vector.
 static int lanczos_restart(const KsParams &p, BV &V, const DenseProblem &ds, int &nconv, int nv, double beta,
                            bool breakdown, std::vector<double> &eigr, std::vector<double> &errest, double &anorm,
                            int its, int &reason, std::vector<double> &scratch, std::vector<double> &bnd,
@@ -516,9 +515,10 @@ int ks_solve(const KsParams &p, const Mat &A, BV &V, bool have_initial, KsResult
   // EPSARNOLDI (arnoldi.c:52-131, setup :31-50): explicit restart and
   // always a DSNHEP projected problem; ncv <= nev + mpd
   const bool arn = p.solver == 1;
-  // EPSLANCZOS with full reorthogonalization (lanczos.c:518-697, setup
-  // :36-78): explicit restart from the most wanted unconverged Ritz
-  // vector, DSHEP, converged pairs moved to the front and locked
+  // EPSLANCZOS (lanczos.c:518-697, setup :36-78): explicit restart from the
+  // most wanted unconverged Ritz vector, DSHEP, converged pairs moved to the
+  // front and locked; the sweep follows the reorthogonalization type (full:
+  // bv_sweep; local, periodic, partial, selective, delayed: the lz_ sweeps)
   const bool lanc = p.solver == 2;
   // Harmonic extraction (krylovschur.c:238-240,270-271,311-320; arnoldi.c:65,
   // 96-97): the projected matrix is translated by g b^T (DSTranslateHarmonic)
