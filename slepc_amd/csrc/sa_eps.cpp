@@ -1,7 +1,7 @@
 // Eigensolver driver: the restart loop of EPSSolve_KrylovSchur_Default
 // (src/eps/impls/krylov/krylovschur/krylovschur.c:227-348), and the explicit
-// restarts of EPSARNOLDI (arnoldi.c:52-131) and EPSLANCZOS with full
-// reorthogonalization (lanczos.c:518-697), over device-resident BV/Mat
+// restarts of EPSARNOLDI (arnoldi.c:52-131) and EPSLANCZOS with every
+// reorthogonalization type (lanczos.c:36-697), over device-resident BV/Mat
 // objects, with the projected problem on the host.
 #include "sa_eps.h"
 
@@ -73,11 +73,6 @@ int eps_start_vector(BV &V, int i, bool have_initial, double eta, int *breakdown
   return 0;
 }
 
-// The restart of EPSSolve_Lanczos with full reorthogonalization
-// (lanczos.c:555-690): residual bounds beta |y_m| + eps ||A||, the most
-// wanted unconverged Ritz vector as the next start vector (built in column
-// nv, then copied to the new column nconv), converged pairs moved to the
-// front, V(:,nconv:k) = V(:,nconv:nv) Y(nconv:nv, nconv:k), nconv = k.
 // PETSc's PETSC_SQRT_MACHINE_EPSILON for double: a decimal literal one ulp
 // above 2^-26, which the thresholds below compare against.
 constexpr double LZ_SQRT_EPS = 1.490116119384766e-08;
@@ -376,7 +371,11 @@ static int lz_sweep_delayed(BV &V, const std::function<int(const void *, void *)
   return 0;
 }
 
-// Restart of EPSLanczos (lanczos.c:548-697).  Local reorthogonalization
+// The restart of EPSSolve_Lanczos (lanczos.c:555-690): residual bounds
+// beta |y_m| + eps ||A||, the most wanted unconverged Ritz vector as the next
+// start vector (built in column nv, then copied to the new column nconv),
+// converged pairs moved to the front, V(:,nconv:k) = V(:,nconv:nv)
+// Y(nconv:nv, nconv:k), nconv = k.  Local reorthogonalization
 // (p.lanczos_local, -eps_lanczos_reorthog local; its sweep is sweep_local in
 // ks_solve) adds the restart steps of lanczos.c:570-575 (repeated Ritz
 // values), 633-660 (spurious Ritz values: true residual of each converged
