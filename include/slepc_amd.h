@@ -227,6 +227,17 @@ int sa_eps_set_st(sa_eps eps, int st_type, int shift_given, double shift);
 #define SA_PC_NONE 0
 #define SA_PC_JACOBI 1
 int sa_eps_set_st_ksp(sa_eps eps, int pc_type, double rtol, int max_it);
+/* EPSSetBalance (epsopts.c:1160-1215; EPSBuildBalance_Krylov,
+   epsdefault.c:443-504): balancing of non-Hermitian problems by a diagonal D
+   built from its iterations (its <= 0: 5) of random +-1 vectors, the update
+   cut off below cutoff ||A||_inf (cutoff <= 0: 1e-8).  The solve runs on
+   D A D^-1 and the eigenvectors returned are D^-1 y, normalised; ignored for
+   Hermitian problems, as in the reference.  FP64, one-sided solvers, no
+   spectral transformation, Ritz extraction. */
+#define SA_BALANCE_NONE 0
+#define SA_BALANCE_ONESIDE 1
+#define SA_BALANCE_TWOSIDE 2
+int sa_eps_set_balance(sa_eps eps, int balance, int its, double cutoff);
 /* EPSKrylovSchurSetRestart / GetRestart (krylovschur.c:350-410): the
    fraction keep in [0.1, 0.9] of the non-converged basis kept at restart
    (default 0.5).  EPSKrylovSchurSetLocking / GetLocking (krylovschur.c:

@@ -19,6 +19,7 @@
  *   EPSLanczos periodic/partial    lanczos/lanczos.c:260-441
  *   EPSLanczos selective           lanczos/lanczos.c:139-258 (LAPACK stevr)
  *   EPSLanczos delayed             epskrylov.c:24-114, lanczos/lanczos.c:495-512
+ *   EPSSetBalance                  src/eps/interface/epsdefault.c:128-139,443-504
  *   BVDot / BVMatMult              src/sys/classes/bv/interface/bvglobal.c:86-120, bvops.c:692-722
  *   BVOrthogonalize (gs/chol/svqb) src/sys/classes/bv/interface/bvorthog.c:489-767, bvlapack.c:136-330
  * The dense projected problems call host LAPACK exactly where the reference does
@@ -1359,6 +1360,40 @@ static void ks_expand(KSo *ks, int k, int *m, double *beta, int *breakdown)
   *breakdown = lindep;
 }
 
+/* EPSBuildBalance_Krylov (epsdefault.c:443-504): the diagonal D for which
+   D A D^-1 is balanced, from its random +-1 vectors z (here the oracle's
+   uniform generator, seed 0xBA1A + j, stands in for PetscRandom):
+   p = D A (z ./ D); oneside: D ./= |p|; twoside: r = (A^T (z .* D)) ./ D and
+   D .*= sqrt(|r ./ p|) where |p| > cutoff ||p_0||_inf and r != 0. */
+static void build_balance(int n, const int32_t *rp, const int32_t *ci, const double *val, int twoside, int its,
+                          double cutoff, double *D)
+{
+  double *z = (double *)malloc(sizeof(double) * (size_t)n), *r = (double *)malloc(sizeof(double) * (size_t)n);
+  double *p = (double *)malloc(sizeof(double) * (size_t)n), norma = 0.0;
+  for (int i = 0; i < n; i++) D[i] = 1.0;
+  for (int j = 0; j < its; j++) {
+    orc_rand_uniform(0xBA1AULL + (uint64_t)j, n, z);
+    for (int i = 0; i < n; i++) z[i] = z[i] < 0.5 ? -1.0 : 1.0;
+    for (int i = 0; i < n; i++) r[i] = z[i] / D[i];
+    orc_spmv(n, rp, ci, val, r, p);
+    for (int i = 0; i < n; i++) p[i] *= D[i];
+    if (twoside) {
+      if (j == 0) { norma = 0.0; for (int i = 0; i < n; i++) { p[i] = fabs(p[i]); if (p[i] > norma) norma = p[i]; } }
+      for (int i = 0; i < n; i++) z[i] *= D[i];
+      for (int i = 0; i < n; i++) r[i] = 0.0;
+      for (int row = 0; row < n; row++)                  /* r = A^T z */
+        for (int32_t q = rp[row]; q < rp[row + 1]; q++) r[ci[q]] += val[q] * z[row];
+      for (int i = 0; i < n; i++) r[i] /= D[i];
+    }
+    for (int i = 0; i < n; i++) {
+      if (twoside) {
+        if (fabs(p[i]) > cutoff * norma && r[i] != 0.0) D[i] *= sqrt(fabs(r[i] / p[i]));
+      } else if (p[i] != 0.0) D[i] /= fabs(p[i]);
+    }
+  }
+  free(z); free(r); free(p);
+}
+
 int orc_eps_solve(int n, const int32_t *rp, const int32_t *ci, const double *val, const orc_opts *o, const double *v0, orc_result *res)
 {
   if (!lp_load()) return -10;
@@ -1383,6 +1418,19 @@ int orc_eps_solve(int n, const int32_t *rp, const int32_t *ci, const double *val
   if (sinv && (which != ORC_TM && which != ORC_TR)) return -12;
   if (sinv && harm) return -13;
   cmp_sinv = sinv; cmp_sigma = o->sigma;
+  /* balancing (epssetup.c:399-405): non-Hermitian only; the balanced
+     operator is formed explicitly, B = D A D^-1 */
+  double *balD = NULL, *balval = NULL;
+  if (o->balance && !herm) {
+    if (sinv || harm) return -15;
+    balD = (double *)malloc(sizeof(double) * (size_t)n);
+    balval = (double *)malloc(sizeof(double) * (size_t)(rp[n] > 0 ? rp[n] : 1));
+    build_balance(n, rp, ci, val, o->balance == 2, o->balance_its > 0 ? o->balance_its : 5,
+                  o->balance_cutoff > 0 ? o->balance_cutoff : 1e-8, balD);
+    for (int row = 0; row < n; row++)
+      for (int32_t q = rp[row]; q < rp[row + 1]; q++) balval[q] = val[q] * balD[row] / balD[ci[q]];
+    val = balval;
+  }
   double *lu = NULL; int *piv = NULL;
   if (sinv) {
     if (!lp_load() || !LP.getrf || !LP.getrs) return -10;
@@ -1714,6 +1762,18 @@ int orc_eps_solve(int n, const int32_t *rp, const int32_t *ci, const double *val
       }
     }
   }
+  if (balD) {   /* EPSComputeVectors_Schur (epsdefault.c:128-139): x ./= D, then BVNormalize with eigi */
+    for (int i = 0; i < nconv; i++)
+      for (int q = 0; q < n; q++) ks.V[q + (int64_t)i * ks.ld] /= balD[q];
+    for (int i = 0; i < nconv; i++) {
+      double *a = ks.V + (int64_t)i * ks.ld;
+      if (eigi[i] != 0.0 && i + 1 < nconv) {
+        double *b = a + ks.ld, nr = orc_bv_norm2(n, a), ni = orc_bv_norm2(n, b), t = sqrt(nr * nr + ni * ni);
+        orc_bv_scale(n, a, 1.0 / t); orc_bv_scale(n, b, 1.0 / t);
+        i++;
+      } else orc_bv_scale(n, a, 1.0 / orc_bv_norm2(n, a));
+    }
+  }
   res->seconds = now_s() - t0;
   res->nconv = nconv; res->its = its; res->reason = reason; res->ncv_used = ncv;
   res->matvecs = ks.matvecs; res->orth_passes = ks.passes;
@@ -1730,7 +1790,7 @@ done:
 #ifdef _OPENMP
   omp_set_num_threads(saved_threads);
 #endif
-  free(gh); free(lu); free(piv);
+  free(gh); free(lu); free(piv); free(balD); free(balval);
   cmp_sinv = 0;
   free(AVs); free(lockd); free(ks.V); free(ks.buf); free(ks.which); free(ds.A); free(ds.Q); free(ds.X); free(ds.T); free(ds.work); free(ds.perm); free(ds.iwork);
   free(eigr); free(eigi); free(errest); free(perm);

@@ -83,6 +83,9 @@ typedef struct {
   double  sigma;       /* shift of the shift-and-invert operator */
   int     lanczos_local; /* Lanczos: 1 -eps_lanczos_reorthog local, 0 full */
   int     arnoldi_delayed; /* EPSArnoldiSetDelayed */
+  int     balance;       /* EPSSetBalance: 0 none, 1 oneside, 2 twoside */
+  int     balance_its;   /* 0: 5 */
+  double  balance_cutoff;  /* 0: 1e-8 */
   int     lanczos_partial; /* Lanczos: 1 periodic, 2 partial, 3 selective, 4 delayed (lanczos_local 0) */
 } orc_opts;
 

@@ -24,7 +24,8 @@ class Opts(C.Structure):
                 ("lock", C.c_int), ("nthreads", C.c_int), ("solver", C.c_int),
                 ("extraction", C.c_int), ("target", C.c_double),
                 ("st", C.c_int), ("sigma", C.c_double), ("lanczos_local", C.c_int),
-                ("arnoldi_delayed", C.c_int), ("lanczos_partial", C.c_int)]
+                ("arnoldi_delayed", C.c_int), ("balance", C.c_int), ("balance_its", C.c_int),
+                ("balance_cutoff", C.c_double), ("lanczos_partial", C.c_int)]
 
 
 class Result(C.Structure):
@@ -266,12 +267,13 @@ def arrow_tridiag(d, e):
 def eps_solve(A, nev, ncv=0, tol=1e-8, which="LM", hermitian=True, v0=None, max_it=0,
               mpd=0, nvec=None, nthreads=0, keep=0.5, lock=1, solver="krylovschur",
               extraction="ritz", target=0.0, twosided=False, st="shift", sigma=None,
-              reorthog="full"):
+              reorthog="full", balance="none"):
     o = Opts(hermitian=1 if hermitian else 0, which=WHICH[which], nev=nev, ncv=ncv, mpd=mpd,
              extraction={"ritz": 0, "harmonic": 1}[extraction], target=float(target),
              st={"shift": 0, "sinvert": 1}[st], sigma=float(target if sigma is None else sigma),
              lanczos_local=1 if reorthog == "local" else 0,
              arnoldi_delayed=1 if (solver == "arnoldi" and reorthog == "delayed") else 0,
+             balance={"none": 0, "oneside": 1, "twoside": 2}[balance],
              lanczos_partial={"full": 0, "local": 0, "periodic": 1, "partial": 2, "selective": 3,
                                "delayed": 4}[reorthog],
              max_it=max_it, tol=tol, keep=keep, eta=0.7071, lock=lock, nthreads=nthreads,

@@ -374,6 +374,11 @@ class EPS:
         Jacobi ("jacobi") or no ("none") preconditioner."""
         check(lib().sa_eps_set_st_ksp(self.h, {"none": 0, "jacobi": 1}[pc], float(rtol), int(max_it)))
 
+    def setBalance(self, balance="twoside", its=5, cutoff=1e-8):
+        """EPSSetBalance: "none", "oneside" or "twoside" (non-Hermitian problems)."""
+        check(lib().sa_eps_set_balance(self.h, {"none": 0, "oneside": 1, "twoside": 2}[balance], int(its),
+                                       float(cutoff)))
+
     def setKrylovSchurRestart(self, keep):
         """EPSKrylovSchurSetRestart: fraction of the basis kept at restart."""
         check(lib().sa_eps_krylovschur_set_restart(self.h, float(keep)))

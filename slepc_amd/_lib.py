@@ -101,6 +101,7 @@ SIGNATURES = {
     "sa_eps_set_target": (C.c_int, [_VP, C.c_double]),
     "sa_eps_set_extraction": (C.c_int, [_VP, C.c_int]),
     "sa_eps_set_two_sided": (C.c_int, [_VP, C.c_int]),
+    "sa_eps_set_balance": (C.c_int, [_VP, C.c_int, C.c_int, C.c_double]),
     "sa_eps_krylovschur_set_restart": (C.c_int, [_VP, C.c_double]),
     "sa_eps_krylovschur_get_restart": (C.c_int, [_VP, C.POINTER(C.c_double)]),
     "sa_eps_krylovschur_set_locking": (C.c_int, [_VP, C.c_int]),

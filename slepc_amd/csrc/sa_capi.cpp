@@ -80,7 +80,9 @@ struct sa_eps_s {
   BV V;                        // FP64 basis holding the solution
   bool twosided = false;       // EPSSetTwoSided
   bool shift_set = false;      // STSetShift given (else sigma = target, STSetDefaultShift)
-  sa_mat AT = nullptr;         // A^T, built at the first two-sided solve
+  sa_mat AT = nullptr;         // A^T, built at the first two-sided or two-side-balanced solve
+  int balance = SA_BALANCE_NONE, balance_its = 5;   // EPSSetBalance
+  double balance_cutoff = 1e-8;
   const void *AT_of = nullptr; // the operator AT was built from
   BV W;                        // left basis (two-sided)
   KsResult res, res32;
@@ -650,6 +652,62 @@ static int eps_setup_dims(sa_eps E, KsParams &p) {
   return 0;
 }
 
+}  // extern "C"
+
+// The operator's CSR arrays on the host.
+static int download_csr(const Mat &A, std::vector<int32_t> &rp, std::vector<int32_t> &ci, std::vector<double> &va) {
+  const int n = A.n;
+  const int64_t nnz = A.nnz;
+  SA_CHECK(A.prec == SA_FP64, "the operator must be FP64");
+  rp.resize(n + 1);
+  ci.resize(nnz);
+  va.resize(nnz);
+  SA_HIP(hipMemcpy(rp.data(), A.rowptr, sizeof(int32_t) * (n + 1), hipMemcpyDeviceToHost));
+  if (nnz) {
+    SA_HIP(hipMemcpy(ci.data(), A.col, sizeof(int32_t) * nnz, hipMemcpyDeviceToHost));
+    SA_HIP(hipMemcpy(va.data(), A.val, sizeof(double) * nnz, hipMemcpyDeviceToHost));
+  }
+  return 0;
+}
+
+// A^T as CSR (host transpose of the device arrays), built once per operator.
+static int ensure_transpose(sa_eps E) {
+  if (E->AT && E->AT_of == (const void *)E->A) return 0;
+  if (E->AT) sa_mat_destroy(E->AT);
+  E->AT = nullptr;
+  const Mat &A = E->A->A;
+  const int n = A.n;
+  const int64_t nnz = A.nnz;
+  std::vector<int32_t> rp, ci, trp(n + 1, 0), tci(nnz);
+  std::vector<double> va, tva(nnz);
+  SA_TRY(download_csr(A, rp, ci, va));
+  for (int64_t q = 0; q < nnz; q++) trp[ci[q] + 1]++;
+  for (int r = 0; r < n; r++) trp[r + 1] += trp[r];
+  std::vector<int32_t> next(trp.begin(), trp.end() - 1);
+  for (int r = 0; r < n; r++)          // rows in order: columns of A^T ascend
+    for (int32_t q = rp[r]; q < rp[r + 1]; q++) {
+      const int32_t dst = next[ci[q]]++;
+      tci[dst] = r;
+      tva[dst] = va[q];
+    }
+  SA_TRY(sa_mat_create_csr(n, nnz, trp.data(), tci.data(), tva.data(), SA_FP64, &E->AT));
+  E->AT_of = (const void *)E->A;
+  return 0;
+}
+
+extern "C" {
+
+int sa_eps_set_balance(sa_eps E, int balance, int its, double cutoff) {
+  SA_CHECK(E, "null EPS");
+  SA_CHECK(balance == SA_BALANCE_NONE || balance == SA_BALANCE_ONESIDE || balance == SA_BALANCE_TWOSIDE,
+           "unsupported balancing type (none, oneside and twoside are available)");
+  E->balance = balance;
+  E->balance_its = its > 0 ? its : 5;
+  E->balance_cutoff = cutoff > 0.0 ? cutoff : 1e-8;
+  E->solved = false;
+  return 0;
+}
+
 int sa_eps_solve(sa_eps E) {
   API_INIT();
   SA_CHECK(E && E->A, "EPS has no operator");
@@ -708,35 +766,37 @@ int sa_eps_solve(sa_eps E) {
            "shift-and-invert runs in FP64 with the one-sided solvers");
   if (E->twosided) {
     SA_CHECK(!E->mixed, "the two-sided solver runs in FP64 only");
-    if (!E->AT || E->AT_of != (const void *)E->A) {   // A^T as CSR (host transpose of the device arrays)
-      if (E->AT) sa_mat_destroy(E->AT);
-      E->AT = nullptr;
-      const int n = A.n;
-      const int64_t nnz = A.nnz;
-      std::vector<int32_t> rp(n + 1), ci(nnz), trp(n + 1, 0), tci(nnz);
-      std::vector<double> va(nnz), tva(nnz);
-      SA_HIP(hipMemcpy(rp.data(), A.rowptr, sizeof(int32_t) * (n + 1), hipMemcpyDeviceToHost));
-      if (nnz) {
-        SA_HIP(hipMemcpy(ci.data(), A.col, sizeof(int32_t) * nnz, hipMemcpyDeviceToHost));
-        SA_HIP(hipMemcpy(va.data(), A.val, sizeof(double) * nnz, hipMemcpyDeviceToHost));
-      }
-      for (int64_t q = 0; q < nnz; q++) trp[ci[q] + 1]++;
-      for (int r = 0; r < n; r++) trp[r + 1] += trp[r];
-      std::vector<int32_t> next(trp.begin(), trp.end() - 1);
-      for (int r = 0; r < n; r++)          // rows in order: columns of A^T ascend
-        for (int32_t q = rp[r]; q < rp[r + 1]; q++) {
-          const int32_t dst = next[ci[q]]++;
-          tci[dst] = r;
-          tva[dst] = va[q];
-        }
-      SA_TRY(sa_mat_create_csr(n, nnz, trp.data(), tci.data(), tva.data(), SA_FP64, &E->AT));
-      E->AT_of = (const void *)E->A;
-    }
+    SA_TRY(ensure_transpose(E));
     if (E->W.m != p.ncv + 1 || E->W.n != A.n) {
       bv_free(E->W);
       SA_TRY(bv_alloc(E->W, A.n, p.ncv + 1, SA_FP64));
     }
     SA_TRY(ks_twosided(p, A, E->AT->A, E->V, E->W, have_init, E->res));
+  } else if (E->balance != SA_BALANCE_NONE && !p.hermitian) {
+    // EPSSetUp (epssetup.c:399-405): D from EPSBuildBalance_Krylov, the
+    // solve on B = D A D^-1, eigenvectors x = D^-1 y normalised
+    // (epsdefault.c:128-139); errors are computed with A itself.
+    SA_CHECK(!E->mixed && p.st_type == SA_ST_SHIFT && p.extraction == 0,
+             "balancing runs with the FP64 one-sided solvers, no spectral transformation and Ritz extraction");
+    const bool two = E->balance == SA_BALANCE_TWOSIDE;
+    if (two) SA_TRY(ensure_transpose(E));
+    struct DevD {
+      double *p = nullptr;
+      sa_mat B = nullptr;
+      ~DevD() {
+        if (p) (void)hipFree(p);
+        if (B) sa_mat_destroy(B);
+      }
+    } bal;
+    SA_HIP(hipMalloc(&bal.p, sizeof(double) * (size_t)std::max(1, A.n)));
+    SA_TRY(eps_build_balance(A, two ? &E->AT->A : nullptr, two, E->balance_its, E->balance_cutoff, bal.p));
+    std::vector<int32_t> rp, ci;
+    std::vector<double> va;
+    SA_TRY(download_csr(A, rp, ci, va));
+    SA_TRY(sa_mat_create_csr(A.n, A.nnz, rp.data(), ci.data(), va.data(), SA_FP64, &bal.B));
+    SA_TRY(balance_scale_mat(bal.B->A, bal.p));
+    SA_TRY(ks_solve(p, bal.B->A, E->V, have_init, E->res));
+    SA_TRY(balance_unscale_vectors(E->V, bal.p, E->res.nconv, E->res.eigi.data()));
   } else {
     SA_TRY(ks_solve(p, A, E->V, have_init, E->res));
   }

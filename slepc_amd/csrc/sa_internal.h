@@ -155,6 +155,12 @@ int bv_orthonormalize(BV &V, int j, double eta, double *norm, int *lindep);
 // in that order), refinement IFNEEDED (sa_mgs.hip): coefficients in the
 // coefficient buffer column j, row j = the norm (0 on lindep); not scaled.
 int bv_mgs_column(BV &V, int j, const int *cols, int nc, double eta, double *norm, int *lindep);
+// Balancing (sa_balance.hip): D (device, A.n entries) from EPSBuildBalance_Krylov
+// (oneside, or twoside with AT = A^T); B's values scaled to D B D^-1 in place;
+// eigenvectors in V(:, 0:nconv) divided by D and normalised (pairs by eigi).
+int eps_build_balance(const Mat &A, const Mat *AT, bool twoside, int its, double cutoff, double *D);
+int balance_scale_mat(Mat &B, const double *D);
+int balance_unscale_vectors(BV &V, const double *D, int nconv, const double *eigi);
 // BVOrthogonalizeVec with MGS: the device vector w (W.n rows, W's precision)
 // against the columns 0..nc-1 of W, refinement IFNEEDED; not scaled.
 int bv_mgs_vec(BV &W, int nc, void *w, double eta, double *norm, int *lindep);

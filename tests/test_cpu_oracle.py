@@ -532,6 +532,25 @@ def test_oracle_krylovschur_locking_restart(orc, case, lock, keep):
     assert got == gold
 
 
+@pytest.mark.parametrize("lock", [1, 0])
+@pytest.mark.parametrize("balance", ["oneside", "twoside"])
+def test_oracle_balance_test9(orc, balance, lock):
+    """eps/tests/test9.c suffix 2: -eps_balance {oneside,twoside}
+    -eps_krylovschur_locking {0,1} -eps_nev 4 -eps_max_it 1500 on the Markov
+    model m = 15: the values of test9_1.out (the +-1 tie compared as a set),
+    and the returned vectors, divided by D and normalised, are eigenvectors
+    of the original matrix (true relative residual < 1e-7)."""
+    A = orc.markov(15)
+    r = orc.eps_solve(A, 4, 0, 1e-8, "LM", False, v0=orc.rand_uniform(0, A.n), max_it=1500, balance=balance,
+                      lock=lock, nvec=4)
+    assert r["nconv"] >= 4
+    assert sorted(np.round(r["eigr"][:4], 5)) == sorted([1.0, -1.0, 0.97137, -0.97137])
+    for i in range(4):
+        x = r["vecs"][:, i]
+        assert abs(np.linalg.norm(x) - 1.0) < 1e-12
+        assert orc.residual_rel(A, r["eigr"][i], 0.0, x) < 1e-7
+
+
 def test_oracle_arnoldi_delayed_test9_3(orc):
     """eps/tests/test9.c suffix 3: -eps_type arnoldi -eps_arnoldi_delayed
     -eps_largest_real -eps_nev 3 -eps_tol 1e-7 on the Markov model m = 15:

@@ -577,6 +577,40 @@ def test_krylovschur_locking_restart_matches_oracle(sa, orc, case, lock, keep):
     assert abs(E.getIterationNumber() - r["its"]) <= 1
 
 
+@pytest.mark.parametrize("lock", [True, False])
+@pytest.mark.parametrize("balance", ["oneside", "twoside"])
+def test_balance_matches_oracle(sa, orc, balance, lock):
+    """EPSSetBalance through the C ABI (test9.c suffix 2: Markov m = 15,
+    nev 4, max_it 1500) against the oracle's balancing: eigenvalues of
+    test9_1.out (the +-1 tie as a set), the oracle's values to 10 tol, and
+    eigenvectors of the original matrix (EPSComputeError with A, unit norm)."""
+    Ao = orc.markov(15)
+    v0 = orc.rand_uniform(0, Ao.n)
+    E = sa.EPS()
+    E.setOperators(sa.Mat(sa.CSR(Ao.n, Ao.rowptr, Ao.col, Ao.val)))
+    E.setProblemType(False)
+    E.setWhichEigenpairs("LM")
+    E.setBalance(balance)
+    E.setKrylovSchurLocking(lock)
+    E.setDimensions(4, 0)
+    E.setTolerances(1e-8, 1500)
+    E.setInitialSpace(v0)
+    E.solve()
+    r = orc.eps_solve(Ao, 4, 0, 1e-8, "LM", False, v0=v0, max_it=1500, balance=balance, lock=1 if lock else 0)
+    assert E.getConverged() >= 4 and r["nconv"] >= 4
+    vals = np.array([E.getEigenpair(i, False)[0] for i in range(4)])
+    # D is built from sums taken in another order than the oracle's, so the
+    # balanced operators differ in rounding and the values agree to the
+    # convergence tolerance (1e-8), not beyond
+    assert np.max(np.abs(np.sort(vals) - np.sort(r["eigr"][:4]))) < 1e-7
+    assert sorted(np.round(vals, 5)) == sorted([1.0, -1.0, 0.97137, -0.97137])
+    assert abs(E.getIterationNumber() - r["its"]) <= 2
+    for i in range(4):
+        x = E.getEigenpair(i)[2]
+        assert abs(np.linalg.norm(x) - 1.0) < 1e-12
+        assert E.computeError(i) < 1e-7
+
+
 def test_arnoldi_delayed_matches_oracle(sa, orc):
     """EPSArnoldi with delayed reorthogonalization (test9.c suffix 3: Markov
     m = 15, 3 largest real, tol 1e-7) through the C ABI against the oracle:
