@@ -718,6 +718,8 @@ int sa_eps_solve(sa_eps E) {
     bv_free(E->V);
     SA_TRY(bv_alloc(E->V, A.n, p.ncv + 1, SA_FP64));
   }
+  SA_CHECK(!(E->balance != SA_BALANCE_NONE && !p.hermitian && (E->mixed || E->twosided)),
+           "balancing runs with the FP64 one-sided solvers");
   E->st = sa_eps_stats{};
   E->res32 = KsResult();
   const double t0 = now();
@@ -776,8 +778,8 @@ int sa_eps_solve(sa_eps E) {
     // EPSSetUp (epssetup.c:399-405): D from EPSBuildBalance_Krylov, the
     // solve on B = D A D^-1, eigenvectors x = D^-1 y normalised
     // (epsdefault.c:128-139); errors are computed with A itself.
-    SA_CHECK(!E->mixed && p.st_type == SA_ST_SHIFT && p.extraction == 0,
-             "balancing runs with the FP64 one-sided solvers, no spectral transformation and Ritz extraction");
+    SA_CHECK(p.st_type == SA_ST_SHIFT && p.extraction == 0,
+             "balancing runs without a spectral transformation and with Ritz extraction");
     const bool two = E->balance == SA_BALANCE_TWOSIDE;
     if (two) SA_TRY(ensure_transpose(E));
     struct DevD {
