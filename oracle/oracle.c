@@ -1346,6 +1346,30 @@ static void ks_expand_delayed(KSo *ks, int k, int m, double *H, int ldh, double 
   free(u); free(t); free(lhh);
 }
 
+/* One local Lanczos sweep from v0 (test entry: exposes ks_expand_local):
+   V (n x (m+1)) gets the basis, alpha/beta (m entries) the tridiagonal.
+   Returns the number of steps taken. */
+int orc_lanczos_local_sweep(int n, const int32_t *rp, const int32_t *ci, const double *val, int m, const double *v0,
+                            double *V, double *alpha, double *beta)
+{
+  KSo ks = {0};
+  ks.n = n; ks.rp = rp; ks.ci = ci; ks.val = val; ks.m = m + 1; ks.ld = n; ks.eta = 0.7071; ks.mgs = 1;
+  ks.V = V;
+  ks.buf = (double *)calloc((size_t)(m + 1) * (m + 1), sizeof(double));
+  ks.which = (char *)calloc((size_t)m + 2, 1);
+  memcpy(V, v0, sizeof(double) * (size_t)n);
+  orc_bv_scale(n, V, 1.0 / orc_bv_norm2(n, V));
+  int mm = m, bd = 0;
+  double b = 0.0;
+  ks_expand_local(&ks, 0, &mm, &b, &bd);
+  for (int j = 0; j < mm; j++) {
+    alpha[j] = ks.buf[j + (int64_t)(j + 1) * ks.m];
+    beta[j] = ks.buf[j + 1 + (int64_t)(j + 1) * ks.m];
+  }
+  free(ks.buf); free(ks.which);
+  return mm;
+}
+
 static void ks_expand(KSo *ks, int k, int *m, double *beta, int *breakdown)
 {
   int lindep = 0;

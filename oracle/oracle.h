@@ -100,6 +100,8 @@ typedef struct {
   double  seconds;              /* wall time of the solve loop */
 } orc_result;
 
+int orc_lanczos_local_sweep(int n, const int32_t *rp, const int32_t *ci, const double *val, int m, const double *v0,
+                            double *V, double *alpha, double *beta);
 int orc_eps_solve(int n, const int32_t *rowptr, const int32_t *col, const double *val,
                   const orc_opts *o, const double *v0, orc_result *r);
 /* Two-sided Krylov-Schur (ks-twosided.c:126-241): as orc_eps_solve, plus the

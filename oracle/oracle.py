@@ -264,6 +264,20 @@ def arrow_tridiag(d, e):
 
 
 # ---------------------------------------------------------------- solve
+def lanczos_local_sweep(A, m, v0):
+    """One EPSLocalLanczos sweep (MGS against v_{j-1}, v_j only) from v0:
+    (V, alpha, beta) with V of shape (n, steps + 1)."""
+    V = np.zeros((m + 1, A.n))
+    al, be = np.zeros(m), np.zeros(m)
+    f = lib().orc_lanczos_local_sweep
+    f.restype = C.c_int
+    x0 = np.ascontiguousarray(v0, np.float64)
+    P = C.c_void_p
+    steps = f(A.n, P(A.rowptr.ctypes.data), P(A.col.ctypes.data), P(A.val.ctypes.data), m, P(x0.ctypes.data),
+              P(V.ctypes.data), P(al.ctypes.data), P(be.ctypes.data))
+    return V[:steps + 1].T.copy(), al[:steps], be[:steps]
+
+
 def eps_solve(A, nev, ncv=0, tol=1e-8, which="LM", hermitian=True, v0=None, max_it=0,
               mpd=0, nvec=None, nthreads=0, keep=0.5, lock=1, solver="krylovschur",
               extraction="ritz", target=0.0, twosided=False, st="shift", sigma=None,

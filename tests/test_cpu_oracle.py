@@ -460,6 +460,30 @@ def test_oracle_lanczos_local_test8(orc, case, reorthog):
     assert np.max(np.abs(r["eigr"][:nev] - ex)) < 1e-7 * np.max(np.abs(ex))
 
 
+def test_oracle_local_sweep_is_local(orc):
+    """The oracle's local Lanczos sweep (lanczos.c:89-118) on the 27-point
+    10^3 operator, 80 steps from a random vector: each new vector is
+    orthogonal to the two before it to rounding (the three-term recurrence),
+    the Lanczos relation A V_m = V_m T_m + beta v_{m+1} e_m^T holds, and,
+    as Ritz values converge, the basis loses global orthogonality (Paige):
+    a sweep that orthogonalised against every column would keep ||V^T V - I||
+    near 1e-15, so the last check fails if the mask is dropped."""
+    T = orc.stencil27(10, seed=1)
+    V, al, be = orc.lanczos_local_sweep(T, 80, orc.rand_uniform(3, T.n))
+    m = len(al)
+    assert m == 80
+    for j in range(1, m + 1):
+        for i in (j - 1, j - 2):
+            if i >= 0:
+                assert abs(V[:, j] @ V[:, i]) < 1e-12
+    A = T.to_dense()
+    Tm = np.diag(al) + np.diag(be[:-1], 1) + np.diag(be[:-1], -1)
+    R = A @ V[:, :m] - V[:, :m] @ Tm
+    R[:, m - 1] -= be[m - 1] * V[:, m]
+    assert np.abs(R).max() < 1e-10 * np.abs(A).max()
+    assert np.abs(V.T @ V - np.eye(m + 1)).max() > 1e-6
+
+
 def _tridiag_random(orc, n=30, seed=5):
     """eps/tests/test3.c's first matrix: tridiag(-1, d, -1) with a random
     diagonal d (here numpy's, seed 5: PetscRandom's stream is not available)."""
