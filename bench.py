@@ -45,11 +45,14 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
-def solve_one(sa, A64, A32, which, herm, nev, ncv, tol, v0, mixed, init=None, tol_req=None):
+def solve_one(sa, A64, A32, which, herm, nev, ncv, tol, v0, mixed, init=None, tol_req=None, trueres=True,
+              lock=True):
     """Returns (EPS, seconds, good pairs, worst residual).  init: optional
     callable producing the start vector; its cost (network inference and
-    prolongation) is part of the time-to-solution.  Pairs are checked against
-    tol_req (default tol) with the FP64 residual."""
+    prolongation) is part of the time-to-solution.  The convergence test
+    takes explicit residuals (EPSSetTrueResidual), so every converged pair
+    meets tol; pairs are checked again against tol_req (default tol) with the
+    FP64 residual of EPSComputeError."""
     t0 = time.perf_counter()
     v = init() if init is not None else None
     E = sa.EPS()
@@ -58,6 +61,8 @@ def solve_one(sa, A64, A32, which, herm, nev, ncv, tol, v0, mixed, init=None, to
     E.setWhichEigenpairs(which)
     E.setDimensions(nev, ncv)
     E.setTolerances(tol)
+    E.setTrueResidual(trueres)
+    E.setKrylovSchurLocking(lock)
     E.setInitialSpace(v0 if v is None else v)
     if mixed:
         E.setMixedPrecision(True, 1e-5)
@@ -262,15 +267,18 @@ def run_scale(sa):
     C64 = sa.Mat(Ch, sa.FP64)
     A64_nnz[0] = Ch.nnz
     gbs2, tb2, ms2, p2 = sweep_roofline(sa, C64, Ch.n, 64)
-    # non-normal operator: the true residual exceeds the Ritz estimate (the
-    # oracle shows the same), so the estimate tolerance is 1e-11 and pairs
-    # are counted at a true relative residual <= 1e-10
-    E, dt, good, w = solve_one(sa, C64, None, "LR", False, 20, 64, 1e-11, sa.rand_uniform(0, Ch.n), False,
-                               tol_req=1e-10)
+    # non-normal operator: the true residual exceeds the Ritz estimate, so
+    # the convergence test takes explicit residuals (EPSSetTrueResidual);
+    # without locking (EPSKrylovSchurSetLocking 0) the converged vectors keep
+    # being refined, whereas locked ones, frozen at residuals just below tol,
+    # leave the later pairs a residual floor above tol through the
+    # non-normal coupling (measured: 9 of 20 pairs after 150 restarts)
+    E, dt, good, w = solve_one(sa, C64, None, "LR", False, 20, 64, 1e-10, sa.rand_uniform(0, Ch.n), False,
+                               lock=False)
     st = E.getStats()
     scale["convdiff_128"] = {
         "workload": "convection-diffusion 7-pt 128^3 (n=2,097,152, nnz 14,581,760) NHEP, nev 20 largest real, "
-                    "ncv 64, estimate tol 1e-11, counted at true rel. residual 1e-10, plain FP64",
+                    "ncv 64, tol 1e-10 on true residuals, no locking, plain FP64",
         "eigenpairs_per_s": round(good / dt, 3), "seconds": round(dt, 3), "good_pairs": good,
         "worst_rel_residual": w, "restarts": st["its"], "matvecs": st["matvecs"],
         "gs_sweep_GBs": round(gbs2, 1), "gs_sweep_frac": round(gbs2 / HBM_PEAK_GBS, 4),

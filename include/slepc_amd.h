@@ -60,6 +60,18 @@ int sa_mat_get_info(sa_mat A, int *n, int64_t *nnz, int *prec);
 /* y = A x with x, y device vectors of the matrix precision (MatMult). */
 int sa_mat_mult(sa_mat A, const void *x_dev, void *y_dev);
 
+/* Residual norms of approximate eigenpairs, EPSComputeResidualNorm_Private
+ * (src/eps/interface/epssolve.c:666-718) with trans = FALSE and B = I:
+ * res[i] = ||A x_i - lambda_i x_i|| for the k device columns of X (the
+ * matrix precision, leading dimension ldx >= n), lambda_i = lr[i] + i li[i]
+ * (li may be NULL).  When li[i] != 0 (and not below |lr[i]| eps) columns i
+ * and i+1 hold the real and imaginary parts of the vector of lr[i] + i li[i]
+ * and res[i] = res[i+1] = sqrt(||.||_r^2 + ||.||_i^2); |lambda| <= eps drops
+ * the lambda terms.  One fused SpMV + axpy + norm kernel per 16 vectors; the
+ * partial sums live in device memory the library allocates per call. */
+int sa_mat_residual_norms(sa_mat A, int k, const void *X_dev, int64_t ldx, const double *lr_host,
+                          const double *li_host, double *res_host);
+
 /* ---- MatSetValues-style assembly and the stencil families ----------------
  * COO triplets -> CSR with ascending columns and INSERT_VALUES semantics
  * (the last value given for a duplicate (row,col) wins), as PETSc's
@@ -166,6 +178,25 @@ int sa_bv_mat_arnoldi(sa_bv V, sa_mat A, int k, int *m, double eta, double *beta
 int sa_bv_dot(sa_bv X, sa_bv Y, double *M_host, int ldm);
 int sa_bv_matmult(sa_bv X, sa_mat A, sa_bv Y);
 int sa_bv_orthogonalize(sa_bv V, int method, double *R_host, int ldr);
+/* BVMult (src/sys/classes/bv/interface/bvops.c:46-108; bvmat.c:17-36):
+   Y(:, Y.l:Y.k) = beta Y(:, Y.l:Y.k) + alpha X(:, X.l:X.k) Q(X.l:X.k, Y.l:Y.k)
+   with Q host column-major (ldq >= X.k); Q == NULL gives BVAXPY,
+   Y = beta Y + alpha X (equal numbers of active columns).  X != Y. */
+int sa_bv_mult(sa_bv Y, double alpha, double beta, sa_bv X, const double *Q_host, int ldq);
+/* BVCopy (bvmat.c:234-248): Y(:, Y.l + j) = X(:, X.l + j), j < X.k - X.l. */
+int sa_bv_copy(sa_bv X, sa_bv Y);
+/* BVResize (bvbasic.c:338-395; bvmat.c:262-284): m columns (< 128); copy != 0
+   keeps the first min(m, old m) columns; the active columns are clipped. */
+int sa_bv_resize(sa_bv V, int m, int copy);
+/* One pass of classical Gram-Schmidt of column j against the active columns
+   V(:, l:j) with one global reduction, BVOrthogonalizeCGS1
+   (src/sys/classes/bv/interface/bvorthog.c:91-132): the pass behind the
+   optional gramschmidt slot of struct _BVOps (bvimpl.h:53, called at
+   bvorthog.c:134 up to three times by the DGKS "if needed" loop).  c (host,
+   j-l entries) receives the pass's coefficients, h (host, or NULL) adds them
+   (BV_AddCoefficients), onorm = ||v|| before the pass, norm = the estimate
+   sqrt(onorm^2 - sum c^2), or the true norm when that is <= 0. */
+int sa_bv_gs_pass(sa_bv V, int j, double *h_host, double *c_host, double *onorm, double *norm);
 /* Gram-Schmidt passes (CGS1 + DGKS refinements) run on V so far. */
 int sa_bv_get_pass_count(sa_bv V, int64_t *passes);
 
@@ -288,6 +319,16 @@ int sa_eps_lanczos_get_reorthog(sa_eps eps, int *reorthog);
 int sa_eps_set_two_sided(sa_eps eps, int twosided);
 /* EPSGetLeftEigenvector: left eigenvector i in the order of sa_eps_get_eigenpair. */
 int sa_eps_get_left_eigenvector(sa_eps eps, int i, double *yr_host, double *yi_host);
+/* EPSSetTrueResidual / EPSGetTrueResidual (src/eps/interface/epsopts.c:
+   1333-1370): the convergence test of Krylov-Schur (also two-sided, for the
+   right residual) and Arnoldi takes the explicit residual
+   ||A x - lambda x|| / |lambda| of each candidate Ritz vector x = V X(:,k)
+   (src/eps/impls/krylov/epskrylov.c:225-265), lambda back-transformed under
+   shift-and-invert, instead of the Ritz estimate beta |y_last|.  Formed on
+   the device (restart GEMM kernel + fused SpMV residual).  Not with
+   EPSLANCZOS (the reference ignores it there) or balancing. */
+int sa_eps_set_true_residual(sa_eps eps, int trueres);
+int sa_eps_get_true_residual(sa_eps eps, int *trueres);
 int sa_eps_set_dimensions(sa_eps eps, int nev, int ncv, int mpd);
 int sa_eps_set_tolerances(sa_eps eps, double tol, int max_it);
 /* EPSSetInitialSpace (epssetup.c:613-626): nvec host vectors of length n,
@@ -314,6 +355,7 @@ typedef struct {
   int     its, its32, nconv, reason;
   double  seconds, seconds32, seconds_ds;
   double  bytes_spmv, bytes_bv, bytes_restart;
+  int64_t true_residuals;   /* explicit residuals taken by the convergence test */
 } sa_eps_stats;
 int sa_eps_get_stats(sa_eps eps, sa_eps_stats *st);
 

@@ -1053,6 +1053,23 @@ static int ks_start_vector(KSo *ks, int i, int nini, int *breakdown)
   return 0;
 }
 
+/* EPSSetTrueResidual branch of EPSKrylovConvergence (epskrylov.c:256-264):
+   EPSComputeRitzVector (epsdefault.c:386-398) x = V(:,0:nv) z, the imaginary
+   part V(:,0:nv) z2 for a pair, then ||A x - lambda x|| with the
+   back-transformed lambda. */
+double orc_residual_abs(int n, const int32_t *rp, const int32_t *ci, const double *val, double kr, double ki,
+                        const double *xr, const double *xi);
+static double ks_true_residual(const KSo *ks, int nv, const double *z, const double *z2, double re, double im)
+{
+  const int n = ks->n;
+  double *x = (double *)malloc(sizeof(double) * (size_t)n * 2);
+  orc_bv_multvec(n, nv, 1.0, ks->V, ks->ld, z, 0.0, x);
+  if (z2) orc_bv_multvec(n, nv, 1.0, ks->V, ks->ld, z2, 0.0, x + n);
+  const double r = orc_residual_abs(n, ks->rp, ks->ci, ks->val, re, z2 ? im : 0.0, x, z2 ? x + n : NULL);
+  free(x);
+  return r;
+}
+
 /* BVMatArnoldi / BVMatLanczos (bvkrylov.c:56-113,165-226) */
 /* y = Op x: A x, or (A - sigma I)^-1 x for STSINVERT with PREONLY + LU */
 static void ks_op(KSo *ks, const double *x, double *y)
@@ -1446,7 +1463,7 @@ int orc_eps_solve(int n, const int32_t *rp, const int32_t *ci, const double *val
      operator is formed explicitly, B = D A D^-1 */
   double *balD = NULL, *balval = NULL;
   if (o->balance && !herm) {
-    if (sinv || harm) return -15;
+    if (sinv || harm || o->trueres) return -15;
     balD = (double *)malloc(sizeof(double) * (size_t)n);
     balval = (double *)malloc(sizeof(double) * (size_t)(rp[n] > 0 ? rp[n] : 1));
     build_balance(n, rp, ci, val, o->balance == 2, o->balance_its > 0 ? o->balance_its : 5,
@@ -1678,7 +1695,14 @@ int orc_eps_solve(int n, const int32_t *rp, const int32_t *ci, const double *val
       int cplx = 0;
       if (herm) resnorm = fabs(ds.Q[nv - 1 + k * ld]);
       else if ((info = ds_vector_nhep_some(&ds, k, &resnorm, &cplx))) { rc = -50; goto done; }
-      resnorm *= beta * gamma;
+      if (o->trueres) {
+        if (o->st == 1) {                                    /* STBackTransform (sinvert.c:16-38) */
+          if (im == 0.0) re = 1.0 / re + o->sigma;
+          else { double t = re * re + im * im; re = re / t + o->sigma; im = -im / t; }
+        }
+        const double *z = herm ? ds.Q + (int64_t)k * ld : ds.X + (int64_t)k * ld;
+        resnorm = ks_true_residual(&ks, nv, z, cplx ? z + ld : NULL, re, im);
+      } else resnorm *= beta * gamma;
       double w = hypot(re, im);
       errest[k] = (w != 0.0) ? resnorm / w : 1.79e308;
       if (marker == -1 && errest[k] >= tol) marker = k;
@@ -2027,6 +2051,11 @@ int orc_eps_solve_twosided(int n, const int32_t *rp, const int32_t *ci, const do
       double rn, rl; int cplx = 0, cplx2 = 0;
       if (ds_vector_nhep_some(&ds, k, &rn, &cplx)) { rc = -50; goto done; }
       double lam = hypot(eigr[k], eigi[k]);
+      if (o->trueres) {                                      /* right residual from the Ritz vector */
+        rn = ks_true_residual(&ks, nv, ds.X + (int64_t)k * ld, cplx ? ds.X + (int64_t)(k + 1) * ld : NULL,
+                              eigr[k], eigi[k]);
+        errest[k] = lam != 0.0 ? rn / lam : 1.79e308;
+      } else
       errest[k] = lam != 0.0 ? rn * beta * norm / lam : 1.79e308;
       if (marker == -1 && errest[k] >= tol) marker = k;
       ds_swap_sides(&ds, &B, &Z, &Y);
@@ -2171,7 +2200,10 @@ done:
 }
 
 /* EPSComputeResidualNorm_Private + EPS_ERROR_RELATIVE (epssolve.c:666-815) */
-double orc_residual_rel(int n, const int32_t *rp, const int32_t *ci, const double *val, double kr, double ki,
+/* EPSComputeResidualNorm_Private (epssolve.c:666-718), trans = FALSE, B = I:
+   ||A x - lambda x|| with x = xr + i xi; the lambda terms are dropped when
+   |lambda| <= eps, and a pair whose imaginary part is below |kr| eps is real. */
+double orc_residual_abs(int n, const int32_t *rp, const int32_t *ci, const double *val, double kr, double ki,
                         const double *xr, const double *xi)
 {
   double *u = (double *)malloc(sizeof(double) * n);
@@ -2182,16 +2214,25 @@ double orc_residual_rel(int n, const int32_t *rp, const int32_t *ci, const doubl
     if (fabs(kr) > eps) for (int i = 0; i < n; i++) u[i] -= kr * xr[i];
     nr = orc_bv_norm2(n, u);
     free(u);
-    return nr / fabs(kr);
+    return nr;
   }
+  const int big = hypot(kr, ki) > eps;
   orc_spmv(n, rp, ci, val, xr, u);
-  for (int i = 0; i < n; i++) u[i] += -kr * xr[i] + ki * xi[i];
+  if (big) for (int i = 0; i < n; i++) u[i] += -kr * xr[i] + ki * xi[i];
   nr = orc_bv_norm2(n, u);
   orc_spmv(n, rp, ci, val, xi, u);
-  for (int i = 0; i < n; i++) u[i] += -kr * xi[i] - ki * xr[i];
+  if (big) for (int i = 0; i < n; i++) u[i] += -kr * xi[i] - ki * xr[i];
   ni = orc_bv_norm2(n, u);
   free(u);
-  return hypot(nr, ni) / hypot(kr, ki);
+  return hypot(nr, ni);
+}
+
+double orc_residual_rel(int n, const int32_t *rp, const int32_t *ci, const double *val, double kr, double ki,
+                        const double *xr, const double *xi)
+{
+  const double eps = 2.220446049250313e-16;
+  const int real = (ki == 0 || fabs(ki) < fabs(kr * eps) || !xi);
+  return orc_residual_abs(n, rp, ci, val, kr, ki, xr, xi) / (real ? fabs(kr) : hypot(kr, ki));
 }
 
 /* DSSolve + DSSort on a compact HEP problem (arrowhead rows l..k, tridiagonal

@@ -87,6 +87,8 @@ typedef struct {
   int     balance_its;   /* 0: 5 */
   double  balance_cutoff;  /* 0: 1e-8 */
   int     lanczos_partial; /* Lanczos: 1 periodic, 2 partial, 3 selective, 4 delayed (lanczos_local 0) */
+  int     trueres;         /* EPSSetTrueResidual: explicit residuals in the convergence test
+                              (Krylov-Schur, two-sided right residuals, Arnoldi) */
 } orc_opts;
 
 typedef struct {
@@ -109,6 +111,8 @@ int orc_eps_solve(int n, const int32_t *rowptr, const int32_t *col, const double
 int orc_eps_solve_twosided(int n, const int32_t *rowptr, const int32_t *col, const double *val,
                            const orc_opts *o, const double *v0, orc_result *r, double *lvecs);
 double orc_residual_rel(int n, const int32_t *rowptr, const int32_t *col, const double *val,
+                        double lr, double li, const double *xr, const double *xi);
+double orc_residual_abs(int n, const int32_t *rowptr, const int32_t *col, const double *val,
                         double lr, double li, const double *xr, const double *xi);
 
 #ifdef __cplusplus

@@ -25,7 +25,7 @@ class Opts(C.Structure):
                 ("extraction", C.c_int), ("target", C.c_double),
                 ("st", C.c_int), ("sigma", C.c_double), ("lanczos_local", C.c_int),
                 ("arnoldi_delayed", C.c_int), ("balance", C.c_int), ("balance_its", C.c_int),
-                ("balance_cutoff", C.c_double), ("lanczos_partial", C.c_int)]
+                ("balance_cutoff", C.c_double), ("lanczos_partial", C.c_int), ("trueres", C.c_int)]
 
 
 class Result(C.Structure):
@@ -281,7 +281,7 @@ def lanczos_local_sweep(A, m, v0):
 def eps_solve(A, nev, ncv=0, tol=1e-8, which="LM", hermitian=True, v0=None, max_it=0,
               mpd=0, nvec=None, nthreads=0, keep=0.5, lock=1, solver="krylovschur",
               extraction="ritz", target=0.0, twosided=False, st="shift", sigma=None,
-              reorthog="full", balance="none"):
+              reorthog="full", balance="none", trueres=False):
     o = Opts(hermitian=1 if hermitian else 0, which=WHICH[which], nev=nev, ncv=ncv, mpd=mpd,
              extraction={"ritz": 0, "harmonic": 1}[extraction], target=float(target),
              st={"shift": 0, "sinvert": 1}[st], sigma=float(target if sigma is None else sigma),
@@ -291,6 +291,7 @@ def eps_solve(A, nev, ncv=0, tol=1e-8, which="LM", hermitian=True, v0=None, max_
              lanczos_partial={"full": 0, "local": 0, "periodic": 1, "partial": 2, "selective": 3,
                                "delayed": 4}[reorthog],
              max_it=max_it, tol=tol, keep=keep, eta=0.7071, lock=lock, nthreads=nthreads,
+             trueres=1 if trueres else 0,
              solver={"krylovschur": 0, "arnoldi": 1, "lanczos": 2}[solver])
     ncv_eff = ncv if ncv else min(A.n, max(2 * nev, nev + 15))
     eigr = np.zeros(ncv_eff + 1)
@@ -324,6 +325,19 @@ def eps_solve(A, nev, ncv=0, tol=1e-8, which="LM", hermitian=True, v0=None, max_
     if twosided:
         out["lvecs"] = lvecs[:A.n * min(nc, nvec)].reshape((min(nc, nvec), A.n)).T.copy() if nvec else None
     return out
+
+
+def residual_abs(A, lr, li, xr, xi=None):
+    """||A x - lambda x|| (EPSComputeResidualNorm_Private, trans = FALSE)."""
+    xr = np.ascontiguousarray(xr, np.float64)
+    xip = None
+    if xi is not None:
+        xi = np.ascontiguousarray(xi, np.float64)
+        xip = xi.ctypes.data_as(C.c_void_p)
+    f = lib().orc_residual_abs
+    f.restype = C.c_double
+    f.argtypes = lib().orc_residual_rel.argtypes
+    return f(A.n, A.rowptr, A.col, A.val, lr, li, xr, xip)
 
 
 def residual_rel(A, lr, li, xr, xi=None):

@@ -196,6 +196,24 @@ class Mat:
         self.mult(xd, yd)
         return yd.to_host(dt, self.n)
 
+    def residual_norms(self, X, lr, li=None):
+        """||A x_i - lambda_i x_i|| for the columns of X (host array n x k,
+        uploaded; a complex pair occupies two columns, real then imaginary
+        part, with li[i] != 0), as EPSComputeResidualNorm_Private computes
+        them (sa_mat_residual_norms)."""
+        dt = np.float64 if self.prec == FP64 else np.float32
+        X = np.asarray(X, dt)
+        if X.ndim == 1:
+            X = X[:, None]
+        k = X.shape[1]
+        xd = DeviceArray.from_host(np.ascontiguousarray(X.T))
+        lr = np.ascontiguousarray(lr, np.float64)
+        li = None if li is None else np.ascontiguousarray(li, np.float64)
+        res = np.zeros(k)
+        check(lib().sa_mat_residual_norms(self.h, k, xd.ptr, self.n, lr.ctypes.data,
+                                          None if li is None else li.ctypes.data, res.ctypes.data))
+        return res
+
     def destroy(self):
         if self.h:
             lib().sa_mat_destroy(self.h)
@@ -378,6 +396,16 @@ class EPS:
         """EPSSetBalance: "none", "oneside" or "twoside" (non-Hermitian problems)."""
         check(lib().sa_eps_set_balance(self.h, {"none": 0, "oneside": 1, "twoside": 2}[balance], int(its),
                                        float(cutoff)))
+
+    def setTrueResidual(self, trueres=True):
+        """EPSSetTrueResidual: the convergence test takes explicit residuals
+        of the Ritz vectors (Krylov-Schur, Arnoldi)."""
+        check(lib().sa_eps_set_true_residual(self.h, 1 if trueres else 0))
+
+    def getTrueResidual(self):
+        v = C.c_int()
+        check(lib().sa_eps_get_true_residual(self.h, C.byref(v)))
+        return bool(v.value)
 
     def setKrylovSchurRestart(self, keep):
         """EPSKrylovSchurSetRestart: fraction of the basis kept at restart."""

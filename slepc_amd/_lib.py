@@ -27,7 +27,7 @@ class EpsStats(C.Structure):
                 ("gs_passes32", C.c_int64), ("its", C.c_int), ("its32", C.c_int), ("nconv", C.c_int),
                 ("reason", C.c_int), ("seconds", C.c_double), ("seconds32", C.c_double),
                 ("seconds_ds", C.c_double), ("bytes_spmv", C.c_double), ("bytes_bv", C.c_double),
-                ("bytes_restart", C.c_double)]
+                ("bytes_restart", C.c_double), ("true_residuals", C.c_int64)]
 
 
 _I32P = np.ctypeslib.ndpointer(np.int32, flags="C")
@@ -51,6 +51,7 @@ SIGNATURES = {
     "sa_mat_destroy": (C.c_int, [_VP]),
     "sa_mat_get_info": (C.c_int, [_VP, C.POINTER(C.c_int), C.POINTER(C.c_int64), C.POINTER(C.c_int)]),
     "sa_mat_mult": (C.c_int, [_VP, _VP, _VP]),
+    "sa_mat_residual_norms": (C.c_int, [_VP, C.c_int, _VP, C.c_int64, _VP, _VP, _VP]),
     "sa_csr_assemble": (C.c_int64, [C.c_int, C.c_int64, _I32P, _I32P, _F64P, _VP, _VP, _VP]),
     "sa_gen_lap2d": (C.c_int64, [C.c_int, C.c_int, C.c_double, C.c_uint64, _VP, _VP, _VP]),
     "sa_gen_convdiff3d": (C.c_int64, [C.c_int, C.c_int, C.c_int, _VP, _VP, _VP]),
@@ -102,6 +103,8 @@ SIGNATURES = {
     "sa_eps_set_extraction": (C.c_int, [_VP, C.c_int]),
     "sa_eps_set_two_sided": (C.c_int, [_VP, C.c_int]),
     "sa_eps_set_balance": (C.c_int, [_VP, C.c_int, C.c_int, C.c_double]),
+    "sa_eps_set_true_residual": (C.c_int, [_VP, C.c_int]),
+    "sa_eps_get_true_residual": (C.c_int, [_VP, C.POINTER(C.c_int)]),
     "sa_eps_krylovschur_set_restart": (C.c_int, [_VP, C.c_double]),
     "sa_eps_krylovschur_get_restart": (C.c_int, [_VP, C.POINTER(C.c_double)]),
     "sa_eps_krylovschur_set_locking": (C.c_int, [_VP, C.c_int]),

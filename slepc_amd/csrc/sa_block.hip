@@ -243,4 +243,69 @@ int launch_spmm(const Mat &A, const BV &X, int lx, const BV &Y, int ly, int ncol
   return 0;
 }
 
+// ---------------------------------------------------------------------------
+// BVMult (bvops.c:46-108 -> bvmat.c:17-36): Y(:, 0:N) = beta Y + alpha X(:, 0:K) Q
+// with Q (K x N, FP64, device) or, without Q, Y = beta Y + alpha X (N == K).
+// A workgroup owns 256 rows and 16 output columns (blockIdx.y): the Q slice is
+// staged in LDS, each thread accumulates its row's 16 outputs in FP64.  beta
+// == 0 ignores Y's old contents, as gemm does.
+// ---------------------------------------------------------------------------
+constexpr int BVM_NJ = 16;
+
+template <typename T>
+__global__ __launch_bounds__(BLK) void bvmult_kernel(int64_t n, const T *__restrict__ X, int64_t ldx, int K,
+                                                     T *__restrict__ Y, int64_t ldy, int N,
+                                                     const double *__restrict__ Q, int ldq, double alpha,
+                                                     double beta) {
+  __shared__ double qs[GS_MAXC * BVM_NJ];
+  const int j0 = blockIdx.y * BVM_NJ, nj = min(BVM_NJ, N - j0);
+  if (Q)
+    for (int idx = threadIdx.x; idx < K * BVM_NJ; idx += BLK) {
+      const int i = idx / BVM_NJ, jj = idx % BVM_NJ;
+      qs[idx] = jj < nj ? Q[i + (size_t)(j0 + jj) * ldq] : 0.0;
+    }
+  __syncthreads();
+  const int64_t r = (int64_t)blockIdx.x * BLK + threadIdx.x;
+  if (r >= n) return;
+  double acc[BVM_NJ];
+#pragma unroll
+  for (int jj = 0; jj < BVM_NJ; jj++) acc[jj] = 0.0;
+  if (Q) {
+    for (int i = 0; i < K; i++) {
+      const double x = (double)X[(size_t)i * ldx + r];
+#pragma unroll
+      for (int jj = 0; jj < BVM_NJ; jj++) acc[jj] += x * qs[i * BVM_NJ + jj];
+    }
+  } else {
+#pragma unroll
+    for (int jj = 0; jj < BVM_NJ; jj++)
+      if (jj < nj) acc[jj] = (double)X[(size_t)(j0 + jj) * ldx + r];
+  }
+#pragma unroll
+  for (int jj = 0; jj < BVM_NJ; jj++) {
+    if (jj < nj) {
+      T *y = Y + (size_t)(j0 + jj) * ldy + r;
+      *y = (T)(beta == 0.0 ? alpha * acc[jj] : beta * (double)*y + alpha * acc[jj]);
+    }
+  }
+}
+
+int launch_bvmult(const BV &X, int lx, int kx, BV &Y, int ly, int ky, const double *Q_dev, int ldq, double alpha,
+                  double beta, hipStream_t s) {
+  const int K = kx - lx, N = ky - ly;
+  SA_CHECK(K >= 0 && N >= 0 && K <= GS_MAXC, "BVMult: invalid active columns");
+  SA_CHECK(Q_dev || K == N, "BVMult without Q needs equal numbers of active columns");
+  SA_CHECK(X.n == Y.n && X.prec == Y.prec, "BVMult: X and Y differ in length or precision");
+  if (N == 0 || X.n == 0) return 0;
+  const dim3 grid((unsigned)((X.n + BLK - 1) / BLK), (unsigned)((N + BVM_NJ - 1) / BVM_NJ));
+  if (X.prec == SA_FP64)
+    hipLaunchKernelGGL(bvmult_kernel<double>, grid, dim3(BLK), 0, s, X.n, (const double *)col_ptr(X, lx), X.ld, K,
+                       (double *)col_ptr(Y, ly), Y.ld, N, Q_dev, ldq, alpha, beta);
+  else
+    hipLaunchKernelGGL(bvmult_kernel<float>, grid, dim3(BLK), 0, s, X.n, (const float *)col_ptr(X, lx), X.ld, K,
+                       (float *)col_ptr(Y, ly), Y.ld, N, Q_dev, ldq, alpha, beta);
+  SA_HIP(hipGetLastError());
+  return 0;
+}
+
 }  // namespace sa

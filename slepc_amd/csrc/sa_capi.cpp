@@ -88,6 +88,7 @@ struct sa_eps_s {
   KsResult res, res32;
   sa_eps_stats st{};
   bool solved = false;
+  double *rpart = nullptr;     // residual partial sums of EPSComputeError (allocated on first use)
 };
 
 #define API_INIT() SA_TRY(ensure_init())
@@ -440,6 +441,123 @@ int sa_bv_matmult(sa_bv X, sa_mat A, sa_bv Y) {
   return 0;
 }
 
+int sa_bv_mult(sa_bv Yb, double alpha, double beta, sa_bv Xb, const double *Q, int ldq) {
+  API_INIT();
+  SA_CHECK(Xb && Yb && Xb != Yb, "BVMult needs two distinct bases");
+  const BV &X = Xb->V;
+  BV &Y = Yb->V;
+  const int K = X.k - X.l, N = Y.k - Y.l;
+  SA_CHECK(Q == nullptr || ldq >= X.k, "leading dimension of Q smaller than the active columns of X");
+  if (N <= 0) return 0;
+  hipStream_t s = stream();
+  if (!Q) {
+    SA_TRY(launch_bvmult(X, X.l, X.k, Y, Y.l, Y.k, nullptr, 0, alpha, beta, s));
+    SA_HIP(hipStreamSynchronize(s));
+    return 0;
+  }
+  // Q(X.l:X.k, Y.l:Y.k) compacted to K x N on the device
+  std::vector<double> q((size_t)std::max(1, K) * N, 0.0);
+  for (int j = 0; j < N; j++)
+    for (int i = 0; i < K; i++) q[i + (size_t)j * K] = Q[(X.l + i) + (size_t)(Y.l + j) * ldq];
+  double *qd = nullptr;
+  SA_HIP(hipMalloc((void **)&qd, sizeof(double) * q.size()));
+  int rc = 0;
+  if (hipMemcpy(qd, q.data(), sizeof(double) * q.size(), hipMemcpyHostToDevice) != hipSuccess) {
+    set_error("hipMemcpy of Q failed in BVMult");
+    rc = 1;
+  }
+  if (!rc) rc = launch_bvmult(X, X.l, X.k, Y, Y.l, Y.k, qd, std::max(1, K), alpha, beta, s);
+  if (!rc && hipStreamSynchronize(s) != hipSuccess) {
+    set_error("HIP error in BVMult");
+    rc = 1;
+  }
+  (void)hipFree(qd);
+  return rc;
+}
+
+int sa_bv_copy(sa_bv Xb, sa_bv Yb) {
+  API_INIT();
+  SA_CHECK(Xb && Yb, "null argument");
+  const BV &X = Xb->V;
+  BV &Y = Yb->V;
+  const int nc = X.k - X.l;
+  SA_CHECK(X.n == Y.n && X.prec == Y.prec, "BVCopy: bases differ in length or precision");
+  SA_CHECK(Y.l + nc <= Y.m, "BVCopy: the target has too few columns");
+  if (nc <= 0 || X.n == 0) return 0;
+  const size_t es = prec_size(X.prec);
+  SA_HIP(hipMemcpy2DAsync(col_ptr(Y, Y.l), es * Y.ld, col_ptr(X, X.l), es * X.ld, es * X.n, nc,
+                          hipMemcpyDeviceToDevice, stream()));
+  SA_HIP(hipStreamSynchronize(stream()));
+  return 0;
+}
+
+int sa_bv_resize(sa_bv B, int m, int copy) {
+  API_INIT();
+  SA_CHECK(B && m > 0 && m < GS_MAXC, "invalid number of columns for BVResize");
+  BV &V = B->V;
+  if (V.m == m) return 0;
+  BV W;
+  int rc = bv_alloc(W, V.n, m, V.prec);
+  if (!rc && copy && V.n > 0) {
+    const size_t es = prec_size(V.prec);
+    if (hipMemcpy2D(W.data, es * W.ld, V.data, es * V.ld, es * V.n, std::min(m, V.m), hipMemcpyDeviceToDevice) !=
+        hipSuccess) {
+      set_error("hipMemcpy2D failed in BVResize");
+      rc = 1;
+    }
+  }
+  if (rc) {
+    bv_free(W);
+    return rc;
+  }
+  W.l = std::min(V.l, m);
+  W.k = std::min(V.k, m);
+  W.passes_done = V.passes_done;
+  bv_free(V);
+  V = W;
+  return 0;
+}
+
+// BVOrthogonalizeCGS1 (bvorthog.c:91-132): dots of column j with V(:, l:j+1)
+// in one device reduction, v -= V(:, l:j) c, ||v|| from the self dot and the
+// estimate ||v'|| = sqrt(||v||^2 - sum c^2) (the true norm when that is <= 0).
+int sa_bv_gs_pass(sa_bv B, int j, double *h, double *c, double *onorm, double *norm) {
+  API_INIT();
+  SA_CHECK(B && c, "null argument");
+  BV &V = B->V;
+  SA_CHECK(j >= V.l && j < V.m && j - V.l < GS_MAXC, "invalid column for a Gram-Schmidt pass");
+  hipStream_t s = stream();
+  const int nc = j - V.l;
+  SA_TRY(launch_dotvec(V, V.l, j + 1, col_ptr(V, j), V.dq, s));
+  std::vector<double> d(nc + 1);
+  SA_HIP(hipMemcpyAsync(d.data(), V.dq, sizeof(double) * (nc + 1), hipMemcpyDeviceToHost, s));
+  SA_HIP(hipStreamSynchronize(s));
+  SA_CHECK(d[nc] > -1e-8 * (1.0 + std::fabs(d[nc])), "Invalid inner product (negative squared norm)");
+  const double beta = d[nc] < 0.0 ? 0.0 : std::sqrt(d[nc]);   // BV_SafeSqrt
+  if (nc > 0) SA_TRY(launch_multvec(V, V.l, j, -1.0, V.dq, 1.0, col_ptr(V, j), s));
+  double sum = 0.0;
+  for (int i = 0; i < nc; i++) {
+    c[i] = d[i];
+    sum += d[i] * d[i];
+    if (h) h[i] += d[i];
+  }
+  if (onorm) *onorm = beta;
+  if (norm) {
+    double nn = beta * beta - sum;
+    if (nn <= 0.0) {
+      SA_TRY(launch_dot_self(V.prec, col_ptr(V, j), V.n, V.dq, V.ws.part, s));
+      SA_HIP(hipMemcpyAsync(&nn, V.dq, sizeof(double), hipMemcpyDeviceToHost, s));
+      SA_HIP(hipStreamSynchronize(s));
+      *norm = std::sqrt(nn);
+    } else {
+      *norm = std::sqrt(nn);
+    }
+  }
+  SA_HIP(hipStreamSynchronize(s));
+  V.passes_done++;
+  return 0;
+}
+
 int sa_bv_orthogonalize(sa_bv V, int method, double *R, int ldr) {
   API_INIT();
   SA_CHECK(V, "null argument");
@@ -462,6 +580,7 @@ int sa_eps_create(sa_eps *out) {
 
 int sa_eps_destroy(sa_eps E) {
   if (!E) return 0;
+  if (E->rpart) (void)hipFree(E->rpart);
   bv_free(E->V);
   bv_free(E->W);
   if (E->AT) sa_mat_destroy(E->AT);
@@ -474,6 +593,24 @@ int sa_eps_set_operators(sa_eps E, sa_mat A) {
   SA_CHECK(A->A.prec == SA_FP64, "the operator of EPS must be FP64 (use sa_eps_set_operators_fp32 for the bulk phase)");
   E->A = A;
   E->solved = false;
+  // a cached A^T belongs to the previous operator (an operator allocated later
+  // at the same address must not find it)
+  if (E->AT) sa_mat_destroy(E->AT);
+  E->AT = nullptr;
+  E->AT_of = nullptr;
+  return 0;
+}
+
+int sa_eps_set_true_residual(sa_eps E, int trueres) {
+  SA_CHECK(E, "null EPS");
+  E->p.trueres = trueres != 0;
+  E->solved = false;
+  return 0;
+}
+
+int sa_eps_get_true_residual(sa_eps E, int *trueres) {
+  SA_CHECK(E && trueres, "null argument");
+  *trueres = E->p.trueres ? 1 : 0;
   return 0;
 }
 
@@ -720,6 +857,8 @@ int sa_eps_solve(sa_eps E) {
   }
   SA_CHECK(!(E->balance != SA_BALANCE_NONE && !p.hermitian && (E->mixed || E->twosided)),
            "balancing runs with the FP64 one-sided solvers");
+  SA_CHECK(!(p.trueres && E->balance != SA_BALANCE_NONE && !p.hermitian),
+           "the true residual is not available with balancing");
   E->st = sa_eps_stats{};
   E->res32 = KsResult();
   const double t0 = now();
@@ -818,6 +957,7 @@ int sa_eps_solve(sa_eps E) {
   s.bytes_spmv = E->res.bytes_spmv + E->res32.bytes_spmv;
   s.bytes_bv = E->res.bytes_bv + E->res32.bytes_bv;
   s.bytes_restart = E->res.bytes_restart + E->res32.bytes_restart;
+  s.true_residuals = E->res.true_residuals + E->res32.true_residuals;
   return 0;
 }
 
@@ -887,86 +1027,53 @@ int sa_eps_get_error_estimate(sa_eps E, int i, double *errest) {
   return 0;
 }
 
-// EPSComputeError (epssolve.c:742-815): ||A x - lambda x|| / |lambda| on the device in FP64.
+// EPSComputeError (epssolve.c:742-815): ||A x - lambda x|| / |lambda| on the
+// device in FP64, by the fused SpMV residual kernel on the basis columns that
+// hold the vector (real part, then imaginary part for a complex pair).
 int sa_eps_compute_error(sa_eps E, int i, int type, double *error) {
   API_INIT();
   SA_CHECK(E && E->solved && error && i >= 0 && i < E->res.nconv, "eigenpair index out of range");
+  SA_CHECK(type == SA_ERROR_ABSOLUTE || type == SA_ERROR_RELATIVE, "unsupported error type");
   const KsResult &r = E->res;
   const int k = r.perm[i];
-  BV &V = E->V;
-  const Mat &A = E->A->A;
   hipStream_t s = stream();
-  double *tmp = nullptr, *out = nullptr;
-  SA_HIP(hipMalloc((void **)&tmp, sizeof(double) * std::max<int64_t>(1, V.n) * 2));
-  SA_HIP(hipMalloc((void **)&out, sizeof(double) * 2));
+  if (!E->rpart)
+    SA_HIP(hipMalloc((void **)&E->rpart, sizeof(double) * ((size_t)RESID_MAXVEC * RESID_MAXG + RESID_MAXVEC)));
+  double *part = E->rpart, *out = E->rpart + (size_t)RESID_MAXVEC * RESID_MAXG;
   const double kr = r.eigr[k], ki = r.eigi[k];
-  double res = 0.0;
-  int rc = 0;
-  if (ki == 0.0) {
-    rc = launch_spmv(A, col_ptr(V, k), tmp, s);
-    if (!rc) rc = launch_axpy_norm(SA_FP64, tmp, col_ptr(V, k), std::fabs(kr) > 2.220446049250313e-16 ? kr : 0.0,
-                                   V.n, V.ws.part, out, s);
-    double h = 0.0;
-    if (!rc && hipMemcpyAsync(&h, out, sizeof(double), hipMemcpyDeviceToHost, s) != hipSuccess) rc = 1;
-    if (!rc && hipStreamSynchronize(s) != hipSuccess) rc = 1;
-    res = std::sqrt(h);
-  } else {
-    // complex pair: columns (xr, xi) = (k, k+1) when ki > 0
-    std::vector<double> xr(V.n), xi(V.n), u(V.n);
-    rc = sa_eps_get_eigenpair(E, i, nullptr, nullptr, xr.data(), xi.data());
-    double kre = kr, kim = std::fabs(ki) * (ki > 0 ? 1 : -1);
-    double *dxr = tmp, *dxi = tmp + V.n;
-    double *dy = nullptr;
-    if (!rc && hipMalloc((void **)&dy, sizeof(double) * V.n) != hipSuccess) rc = 1;
-    if (!rc) rc = sa_memcpy_h2d(dxr, xr.data(), sizeof(double) * V.n);
-    if (!rc) rc = sa_memcpy_h2d(dxi, xi.data(), sizeof(double) * V.n);
-    double nr = 0.0, ni = 0.0;
-    if (!rc) {
-      rc = launch_spmv(A, dxr, dy, s);
-      std::vector<double> y(V.n);
-      if (!rc) rc = sa_memcpy_d2h(y.data(), dy, sizeof(double) * V.n);
-      for (int64_t q = 0; q < V.n; q++) { double t = y[q] - kre * xr[q] + kim * xi[q]; nr += t * t; }
-      if (!rc) rc = launch_spmv(A, dxi, dy, s);
-      if (!rc) rc = sa_memcpy_d2h(y.data(), dy, sizeof(double) * V.n);
-      for (int64_t q = 0; q < V.n; q++) { double t = y[q] - kre * xi[q] - kim * xr[q]; ni += t * t; }
-    }
-    if (dy) (void)hipFree(dy);
-    res = std::hypot(std::sqrt(nr), std::sqrt(ni));
-  }
+  // a pair lives in columns (c, c+1) with the positive imaginary part at c
+  const int c = (ki < 0.0) ? k - 1 : k;
+  const double lr[2] = {kr, kr}, li[2] = {std::fabs(ki), -std::fabs(ki)};
+  const int nx = ki != 0.0 ? 2 : 1;
+  double rn[2] = {0.0, 0.0};
+  SA_TRY(residual_norms(E->A->A, col_ptr(E->V, c), E->V.ld, nx, lr, ki != 0.0 ? li : nullptr, rn, part, out, s));
+  double res = rn[0];
   // two-sided: the left residual ||A^T y - conj(lambda) y|| too, and the
   // larger of the two (epssolve.c:777-782, EPSComputeResidualNorm_Private
-  // with trans = TRUE)
-  if (!rc && E->twosided) {
-    std::vector<double> yr(V.n), yi(V.n), u(V.n);
-    rc = basis_eigvec(E->W, r, i, yr.data(), ki != 0.0 ? yi.data() : nullptr);
-    double *dy = nullptr;
-    if (!rc && hipMalloc((void **)&dy, sizeof(double) * V.n) != hipSuccess) rc = 1;
-    const double eps = 2.220446049250313e-16;
-    double nr = 0.0, ni = 0.0;
-    if (!rc) rc = sa_memcpy_h2d(tmp, yr.data(), sizeof(double) * V.n);
-    if (!rc) rc = launch_spmv(E->AT->A, tmp, dy, s);
-    if (!rc) rc = sa_memcpy_d2h(u.data(), dy, sizeof(double) * V.n);
-    if (!rc) {
-      if (ki == 0.0) {
-        const double a = std::fabs(kr) > eps ? kr : 0.0;
-        for (int64_t q = 0; q < V.n; q++) { const double t = u[q] - a * yr[q]; nr += t * t; }
-      } else {
-        for (int64_t q = 0; q < V.n; q++) { const double t = u[q] - kr * yr[q] - ki * yi[q]; nr += t * t; }
-        rc = sa_memcpy_h2d(tmp, yi.data(), sizeof(double) * V.n);
-        if (!rc) rc = launch_spmv(E->AT->A, tmp, dy, s);
-        if (!rc) rc = sa_memcpy_d2h(u.data(), dy, sizeof(double) * V.n);
-        for (int64_t q = 0; q < V.n; q++) { const double t = u[q] - kr * yi[q] + ki * yr[q]; ni += t * t; }
-      }
-    }
-    if (dy) (void)hipFree(dy);
-    res = std::max(res, std::hypot(std::sqrt(nr), std::sqrt(ni)));
+  // with trans = TRUE: the conjugate value on A^T)
+  if (E->twosided) {
+    const double lil[2] = {-std::fabs(ki), std::fabs(ki)};
+    SA_TRY(residual_norms(E->AT->A, col_ptr(E->W, c), E->W.ld, nx, lr, ki != 0.0 ? lil : nullptr, rn, part, out, s));
+    res = std::max(res, rn[0]);
   }
-  (void)hipFree(tmp);
-  (void)hipFree(out);
-  if (rc) return rc;
   if (type == SA_ERROR_RELATIVE) res /= std::hypot(kr, ki);
   *error = res;
   return 0;
+}
+
+// EPSComputeResidualNorm_Private over k device columns (residual partials in
+// memory allocated for the call).
+int sa_mat_residual_norms(sa_mat M, int k, const void *X, int64_t ldx, const double *lr, const double *li,
+                          double *res) {
+  API_INIT();
+  SA_CHECK(M && X && lr && res && k >= 0, "null argument to sa_mat_residual_norms");
+  SA_CHECK(ldx >= M->A.n, "leading dimension of X smaller than the matrix size");
+  if (k == 0) return 0;
+  double *part = nullptr;
+  SA_HIP(hipMalloc((void **)&part, sizeof(double) * ((size_t)RESID_MAXVEC * RESID_MAXG + RESID_MAXVEC)));
+  const int rc = residual_norms(M->A, X, ldx, k, lr, li, res, part, part + (size_t)RESID_MAXVEC * RESID_MAXG, stream());
+  (void)hipFree(part);
+  return rc;
 }
 
 int sa_eps_get_stats(sa_eps E, sa_eps_stats *st) {

@@ -506,6 +506,21 @@ static int lanczos_restart(const KsParams &p, BV &V, const DenseProblem &ds, int
   return 0;
 }
 
+int ritz_true_residual(BV &V, const Mat &A, int nv, const double *Z, int ldz, int nx, double re, double im,
+                       void *work, double *part, double *resnorm) {
+  hipStream_t s = stream();
+  for (int c = 0; c < nx; c++) std::memcpy(V.hq + (size_t)c * ldz, Z + (size_t)c * ldz, sizeof(double) * nv);
+  SA_HIP(hipMemcpyAsync(V.dq, V.hq, sizeof(double) * (size_t)ldz * nx, hipMemcpyHostToDevice, s));
+  const size_t cb = prec_size(V.prec) * (size_t)V.ld;
+  for (int c = 0; c < nx; c++)
+    SA_TRY(launch_multvec(V, 0, nv, 1.0, V.dq + (size_t)c * ldz, 0.0, static_cast<char *>(work) + c * cb, s));
+  const double lr2[2] = {re, re}, li2[2] = {nx == 2 ? im : 0.0, nx == 2 ? -im : 0.0};
+  double rn[2] = {0.0, 0.0};
+  SA_TRY(residual_norms(A, work, V.ld, nx, lr2, li2, rn, part, part + (size_t)RESID_MAXVEC * RESID_MAXG, s));
+  *resnorm = rn[0];
+  return 0;
+}
+
 int ks_solve(const KsParams &p, const Mat &A, BV &V, bool have_initial, KsResult &r) {
   SA_CHECK(lapack_ready(), "no LAPACK library found for the projected problem");
   const int n = A.n, ncv = p.ncv, nev = p.nev, mpd = p.mpd, ld = ncv + 1;
@@ -563,6 +578,25 @@ int ks_solve(const KsParams &p, const Mat &A, BV &V, bool have_initial, KsResult
   } avh;
   BV &AV = avh.b;
   if (lanc && p.lanczos_partial == 3) SA_TRY(bv_alloc(AV, V.n, V.m, V.prec));
+  // EPSSetTrueResidual (epsopts.c:1333-1345) for Krylov-Schur and Arnoldi: the
+  // convergence test forms each candidate Ritz vector x = V(:,0:nv) X(:,k)
+  // (EPSComputeRitzVector, epsdefault.c:386-398) and takes ||A x - lambda x||
+  // with the back-transformed value (EPSComputeResidualNorm_Private), in
+  // place of beta |y_last| (epskrylov.c:225-265).  Work: two columns for the
+  // real and imaginary parts, the residual partials and their sums.
+  struct TrueWork {
+    void *x = nullptr;
+    double *part = nullptr;
+    ~TrueWork() {
+      if (x) (void)hipFree(x);
+      if (part) (void)hipFree(part);
+    }
+  } trw;
+  const bool trueres = p.trueres && !lanc;
+  if (trueres) {
+    SA_HIP(hipMalloc(&trw.x, prec_size(V.prec) * (size_t)V.ld * 2));
+    SA_HIP(hipMalloc((void **)&trw.part, sizeof(double) * ((size_t)RESID_MAXVEC * RESID_MAXG + RESID_MAXVEC)));
+  }
   // EPSLocalLanczos (lanczos.c:89-118): V(:,j+1) = Op V(:,j) orthogonalised
   // by MGS against the columns before k and V(:,j-1), V(:,j) only; the
   // coefficient buffer gets alpha_j (row j) and beta_j = the norm (row j+1,
@@ -721,8 +755,27 @@ int ks_solve(const KsParams &p, const Mat &A, BV &V, bool have_initial, KsResult
       int cplx = 0;
       info = ds.ritz_residual(k, &resnorm, &cplx);
       SA_CHECK(info == 0, "LAPACK failure in the projected problem (vectors)");
-      resnorm *= beta * gamma;
-      const double w = std::hypot(eigr[k], eigi[k]);
+      double re = eigr[k], im = eigi[k];
+      if (trueres) {
+        if (sinv) {   // STBackTransform (sinvert.c:16-38)
+          if (im == 0.0) {
+            re = 1.0 / re + p.st_shift;
+          } else {
+            const double t = re * re + im * im;
+            re = re / t + p.st_shift;
+            im = -im / t;
+          }
+        }
+        const double *Xk = dsherm ? ds.Q() + (size_t)k * ld : ds.X() + (size_t)k * ld;
+        const int nx = cplx ? 2 : 1;
+        SA_TRY(ritz_true_residual(V, A, nv, Xk, ld, nx, re, im, trw.x, trw.part, &resnorm));
+        r.true_residuals++;
+        r.bytes_bv += nx * ((double)nv + 1.0) * V.n * es;
+        r.bytes_spmv += nx * bsp;
+      } else {
+        resnorm *= beta * gamma;
+      }
+      const double w = std::hypot(re, im);
       errest[k] = (w != 0.0) ? resnorm / w : 1.79e308;
       if (marker == -1 && errest[k] >= p.tol) marker = k;
       if (cplx) {

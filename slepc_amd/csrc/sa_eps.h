@@ -22,6 +22,7 @@ struct KsParams {
   bool lanczos_local = false;  // EPSLanczosSetReorthog: local (else full)
   bool arnoldi_delayed = false;  // EPSArnoldiSetDelayed
   bool lock = true;              // EPSKrylovSchurSetLocking
+  bool trueres = false;          // EPSSetTrueResidual (Krylov-Schur and Arnoldi)
   int lanczos_partial = 0;     // EPSLanczosSetReorthog: 1 periodic, 2 partial, 3 selective, 4 delayed
   void (*monitor)(void *, int, int, const double *, const double *, int) = nullptr;
   void *monitor_ctx = nullptr;
@@ -33,6 +34,7 @@ struct KsResult {
   double seconds = 0.0, seconds_ds = 0.0;
   double bytes_spmv = 0.0, bytes_bv = 0.0, bytes_restart = 0.0;
   int64_t ksp_its = 0;             // BiCGStab iterations (shift-and-invert)
+  int64_t true_residuals = 0;      // explicit residuals of the convergence test
   std::vector<int> perm;           // sorted position -> column of V
   std::vector<double> eigr, eigi, errest;
 };
@@ -51,6 +53,14 @@ int ks_solve(const KsParams &p, const Mat &A, BV &V, bool have_initial, KsResult
 constexpr uint64_t EPS_SEED_RIGHT = 0x5EEDULL, EPS_SEED_LEFT = 0x1EF7ULL;
 int eps_start_vector(BV &V, int i, bool have_initial, double eta, int *breakdown, std::vector<double> &scratch,
                      uint64_t seed = EPS_SEED_RIGHT);
+
+// EPSComputeRitzVector + EPSComputeResidualNorm_Private (epsdefault.c:386-398,
+// epssolve.c:666-718): x = V(:,0:nv) Z(:,0) (and the imaginary part
+// V(:,0:nv) Z(:,1) when nx == 2) into work (two columns of V's leading
+// dimension), *resnorm = ||A x - (re + i im) x||.  part: RESID_MAXVEC *
+// RESID_MAXG + RESID_MAXVEC doubles.  Uses V's Q staging buffers.
+int ritz_true_residual(BV &V, const Mat &A, int nv, const double *Z, int ldz, int nx, double re, double im,
+                       void *work, double *part, double *resnorm);
 
 // Two-sided Krylov-Schur (ks-twosided.c:126-241): right basis V of A and left
 // basis W of A^T (AT), DSNHEPTS projected problem; on return the first nconv

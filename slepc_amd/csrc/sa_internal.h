@@ -113,8 +113,34 @@ int launch_axpy_norm(int prec, const void *Ax, const void *x, double lambda, int
 int launch_fill_random(int prec, void *x, int64_t n, uint64_t seed, hipStream_t s);
 // BVDot: M_dev(i + j*(ky-ly)) = Y(:,ly+i) . X(:,lx+j) (sa_block.hip); synchronises s
 int launch_bvdot(const BV &X, int lx, int kx, const BV &Y, int ly, int ky, double *M_dev, hipStream_t s);
+// BVMult: Y(:, ly:ky) = beta Y + alpha X(:, lx:kx) Q (Q_dev: (kx-lx) x (ky-ly),
+// leading dimension ldq) or, with Q_dev null, beta Y + alpha X (sa_block.hip)
+int launch_bvmult(const BV &X, int lx, int kx, BV &Y, int ly, int ky, const double *Q_dev, int ldq, double alpha,
+                  double beta, hipStream_t s);
 // BVMatMult: Y(:, ly+j) = A X(:, lx+j), j < ncols (sa_block.hip)
 int launch_spmm(const Mat &A, const BV &X, int lx, const BV &Y, int ly, int ncols, hipStream_t s);
+// Residual norms of approximate eigenpairs with the SpMV fused in
+// (sa_resid.hip): vector v of a launch is ||A X(:,xcol) - a X(:,xcol) + b X(:,zcol)||^2
+// (zcol < 0: no z term), X in the operator's precision with leading
+// dimension ldx.  part holds RESID_MAXVEC * RESID_MAXG doubles, out
+// RESID_MAXVEC; out[v] receives the squared norm of vector v.
+constexpr int RESID_MAXVEC = 16;
+constexpr int RESID_MAXG = 1024;
+struct ResidArgs {
+  int nvec;
+  int xcol[RESID_MAXVEC], zcol[RESID_MAXVEC];
+  double a[RESID_MAXVEC], b[RESID_MAXVEC];
+};
+int resid_grid(int64_t n, int lanes);
+int launch_resid_norms(const Mat &A, const void *X, int64_t ldx, const ResidArgs &args, double *part, double *out,
+                       hipStream_t s);
+// ||A x - lambda x|| (absolute) of k columns of X with eigenvalues lr + i li
+// by the rules of EPSComputeResidualNorm_Private (epssolve.c:666-718): when
+// li[c] != 0 (beyond |lr[c]| eps) columns c, c+1 hold the real and imaginary
+// parts of a pair and res[c] = res[c+1] = hypot of the two norms.  Blocks on
+// the stream; part/out as for launch_resid_norms.
+int residual_norms(const Mat &A, const void *X, int64_t ldx, int k, const double *lr, const double *li,
+                   double *res, double *part, double *out, hipStream_t s);
 // Shift-and-invert solves x = (A - sigma I)^-1 b (BiCGStab, FP64; optional
 // Jacobi preconditioning), workspace of 9 vectors + partials in one buffer.
 struct KspWork {

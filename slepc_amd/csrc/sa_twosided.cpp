@@ -100,6 +100,18 @@ int ks_twosided(const KsParams &p, const Mat &A, const Mat &AT, BV &V, BV &W, bo
     double *p;
     ~Free() { (void)hipFree(p); }
   } free_m{Mdev};
+  struct TrueWork {   // EPSSetTrueResidual: Ritz vector columns and residual partials
+    void *x = nullptr;
+    double *part = nullptr;
+    ~TrueWork() {
+      if (x) (void)hipFree(x);
+      if (part) (void)hipFree(part);
+    }
+  } trw;
+  if (p.trueres) {
+    SA_HIP(hipMalloc(&trw.x, sizeof(double) * (size_t)V.ld * 2));
+    SA_HIP(hipMalloc((void **)&trw.part, sizeof(double) * ((size_t)RESID_MAXVEC * RESID_MAXG + RESID_MAXVEC)));
+  }
   const double t0 = now_s();
   double tds = 0.0;
   r = KsResult();
@@ -179,7 +191,14 @@ int ks_twosided(const KsParams &p, const Mat &A, const Mat &AT, BV &V, BV &W, bo
       int cplx = 0, cplx2 = 0;
       SA_CHECK(ds.ritz_residual(k, &rn, &cplx) == 0, "LAPACK failure in the projected problem (vectors)");
       const double lam = std::hypot(eigr[k], eigi[k]);
-      errest[k] = lam != 0.0 ? rn * beta * norm / lam : 1.79e308;
+      if (p.trueres) {   // the right residual from the Ritz vector (epskrylov.c:256-264)
+        SA_TRY(ritz_true_residual(V, A, nv, ds.X() + (size_t)k * ld, ld, cplx ? 2 : 1, eigr[k], eigi[k], trw.x, trw.part,
+                                  &rn));
+        r.true_residuals++;
+        errest[k] = lam != 0.0 ? rn / lam : 1.79e308;
+      } else {
+        errest[k] = lam != 0.0 ? rn * beta * norm / lam : 1.79e308;
+      }
       if (marker == -1 && errest[k] >= p.tol) marker = k;
       SA_CHECK(ds.left_residual(k, &rl, &cplx2) == 0, "LAPACK failure in the projected problem (left vectors)");
       const double lerr = lam != 0.0 ? rl * betat * norm2 / lam : 1.79e308;

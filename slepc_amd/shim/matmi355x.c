@@ -61,6 +61,20 @@ static PetscErrorCode MatDestroy_AIJMI355X(Mat M)
   PetscFunctionReturn(PETSC_SUCCESS);
 }
 
+/* The library handle behind an aijmi355x matrix (used by BV type mi355x for
+   BVMatMult); an error for any other type. */
+PETSC_EXTERN PetscErrorCode MatMI355XGetHandle(Mat M, sa_mat *A)
+{
+  PetscBool match;
+
+  PetscFunctionBegin;
+  PetscCall(PetscObjectTypeCompare((PetscObject)M, "aijmi355x", &match));
+  PetscCheck(match, PETSC_COMM_SELF, PETSC_ERR_SUP, "BVMatMult of BV type mi355x needs a matrix of type aijmi355x");
+  *A = ((Mat_MI355X *)M->spptr)->A;
+  PetscCheck(*A, PETSC_COMM_SELF, PETSC_ERR_ORDER, "The aijmi355x matrix is not assembled");
+  PetscFunctionReturn(PETSC_SUCCESS);
+}
+
 PETSC_EXTERN PetscErrorCode MatCreate_AIJMI355X(Mat M)
 {
   Mat_MI355X *ctx;

@@ -758,3 +758,90 @@ def test_oracle_complex_pairs(orc, variant):
         else:
             assert orc.residual_rel(A, lr, 0.0, r["vecs"][:, i]) < 1e-8
             i += 1
+
+
+# ------------------------------------------------------------------ EPSSetTrueResidual
+def _brusselator(orc, N=30):
+    """test22.c's Brusselator wave model (a shell matrix there, assembled
+    here): A = [tau1 T + (beta-1) I, alpha^2 I; -beta I, tau2 T - alpha^2 I],
+    T = tridiag(1, -2, 1), tau_i = delta_i / (h L)^2, h = 1/(N+1)."""
+    alpha, beta, d1, d2, L = 2.0, 5.45, 0.008, 0.004, 0.51302
+    h = 1.0 / (N + 1)
+    t1, t2 = d1 / (h * L) ** 2, d2 / (h * L) ** 2
+    D = np.zeros((2 * N, 2 * N))
+    for i in range(N):
+        D[i, i] = -2 * t1 + beta - 1.0
+        D[N + i, N + i] = -2 * t2 - alpha * alpha
+        D[i, N + i] = alpha * alpha
+        D[N + i, i] = -beta
+        if i > 0:
+            D[i, i - 1], D[N + i, N + i - 1] = t1, t2
+        if i < N - 1:
+            D[i, i + 1], D[N + i, N + i + 1] = t1, t2
+    rows, cols = np.nonzero(D)
+    rp = np.zeros(2 * N + 1, np.int32)
+    np.add.at(rp, rows + 1, 1)
+    return orc.CSR(2 * N, np.cumsum(rp).astype(np.int32), cols.astype(np.int32), D[rows, cols].copy())
+
+
+TEST22_VALUES = sorted([(0.00019, 2.13938), (0.00019, -2.13938), (-0.67192, 2.52712), (-0.67192, -2.52712)])
+
+
+@pytest.mark.parametrize("trueres", [False, True])
+def test_oracle_test22_true_residual(orc, trueres):
+    """test22.c -eps_nev 4 -eps_true_residual {0,1} (test22_1.out and its
+    _alt orders): the two rightmost conjugate pairs of the Brusselator, and
+    every error estimate below 5 |lambda| tol as the test checks."""
+    A = _brusselator(orc)
+    r = orc.eps_solve(A, 4, 0, 1e-8, "LR", False, v0=orc.rand_uniform(0, A.n), trueres=trueres)
+    assert r["nconv"] >= 4
+    got = sorted((round(float(a), 5), round(float(b), 5)) for a, b in zip(r["eigr"][:4], r["eigi"][:4]))
+    assert got == TEST22_VALUES
+    assert np.all(r["errest"][:4] < 5 * np.hypot(r["eigr"][:4], r["eigi"][:4]) * 1e-8)
+
+
+def test_oracle_true_residual_test9_4(orc):
+    """test9.c suffix 4 (-eps_nev 4 -eps_true_residual) prints test9_1.out."""
+    A = orc.markov(15)
+    r = orc.eps_solve(A, 4, 0, 1e-8, "LM", False, v0=orc.rand_uniform(0, A.n), trueres=True)
+    assert list(np.round(r["eigr"][:4], 5)) == [1.0, -1.0, 0.97137, -0.97137]
+
+
+@pytest.mark.parametrize("case", ["markov15", "convdiff10", "lap2d_pot", "sinvert_markov"])
+def test_oracle_true_residual_is_the_residual_of_the_returned_vector(orc, case):
+    """With the true residual the error estimate of a returned pair is
+    ||A x - lambda x|| / |lambda| of the vector itself: recomputed here with
+    dense numpy products (no oracle code), it agrees to rounding, and every
+    pair meets the tolerance.  Without it the estimate is beta |y_last|,
+    which differs from the residual in the trailing digits (the check can
+    tell the two apart)."""
+    kw = {}
+    if case == "markov15":
+        A, nev, ncv, tol, which, herm = orc.markov(15), 4, 0, 1e-8, "LM", False
+    elif case == "convdiff10":
+        A, nev, ncv, tol, which, herm = orc.convdiff3d(10), 8, 32, 1e-10, "LR", False
+    elif case == "lap2d_pot":
+        A, nev, ncv, tol, which, herm = orc.lap2d_pot(30, 27, 2, 10.0), 8, 24, 1e-10, "LM", True
+    else:
+        A, nev, ncv, tol, which, herm = orc.markov(15), 3, 0, 1e-8, "TM", False
+        kw = dict(target=0.5, st="sinvert")
+    D = A.to_dense()
+    v0 = orc.rand_uniform(0, A.n)
+    diffs = {}
+    for tr in (True, False):
+        r = orc.eps_solve(A, nev, ncv, tol, which, herm, v0=v0, nvec=nev, trueres=tr, **kw)
+        assert r["nconv"] >= nev
+        rel = []
+        for i in range(nev):
+            assert r["eigi"][i] == 0.0
+            x, lam = r["vecs"][:, i], r["eigr"][i]
+            rel.append(np.linalg.norm(D @ x - lam * x) / abs(lam) / np.linalg.norm(x))
+        rel = np.array(rel)
+        # both sides are relative to |lambda|; rounding in a residual norm is
+        # about eps ||A|| ||x|| / |lambda|
+        diffs[tr] = np.abs(rel - r["errest"][:nev])
+        if tr:
+            assert np.all(rel < tol)
+            assert np.all(diffs[tr] < 1e-14)
+    if case in ("convdiff10", "lap2d_pot"):
+        assert diffs[False].max() > 1e-14

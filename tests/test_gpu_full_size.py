@@ -9,13 +9,15 @@ import pytest
 pytestmark = pytest.mark.gpu
 
 
-def _solve(sa, A64, nev, ncv, tol, which, herm, v0, A32=None, mixed=False):
+def _solve(sa, A64, nev, ncv, tol, which, herm, v0, A32=None, mixed=False, trueres=True, lock=True):
     E = sa.EPS()
     E.setOperators(A64, A32 if mixed else None)
     E.setProblemType(herm)
     E.setWhichEigenpairs(which)
     E.setDimensions(nev, ncv)
     E.setTolerances(tol)
+    E.setTrueResidual(trueres)
+    E.setKrylovSchurLocking(lock)
     E.setInitialSpace(v0)
     if mixed:
         E.setMixedPrecision(True, 1e-5)
@@ -28,17 +30,21 @@ def test_c2_convdiff_128_closed_form(sa):
     real, ncv 64: the spectrum is separable, 6 + sum_d 2 s_d cos(k_d pi/129)
     with s_d = sqrt((1 + 0.1 b_d)(1 - 0.1 b_d)), b = (1, 0.5, 0.25); the 20
     computed values are its 20 largest (all distinct), and every pair meets a
-    true relative residual of 1e-10 (estimate tolerance 1e-11: the operator is
-    non-normal).  The eigenvalues themselves are ill-conditioned: the
-    diagonal similarity that symmetrises the operator has condition number
-    (1.1/0.9)^63.5 ~ 3e5 along the first axis, so a 1e-11 residual allows
-    eigenvalue errors far above 1e-10 (5.6e-9 measured).  The closest two
+    true relative residual of 1e-10: the operator is non-normal (the true
+    residual runs above the Ritz estimate), so the convergence test takes
+    explicit residuals (EPSSetTrueResidual), without locking: locked vectors,
+    frozen at residuals just below tol, would leave the later pairs a
+    residual floor above tol through the non-normal coupling (9 of 20 pairs
+    after 150 restarts with locking).  The eigenvalues themselves are
+    ill-conditioned: the diagonal similarity that symmetrises the operator
+    has condition number (1.1/0.9)^63.5 ~ 3e5 along the first axis, so a
+    1e-10 residual allows eigenvalue errors far above 1e-10.  The closest two
     of the 21 largest closed-form values are 1.4e-7 apart (relative), so a
     2e-8 match still pins which eigenvalue each computed value is."""
     N = 128
     Ah = sa.convdiff3d(N)
     A = sa.Mat(Ah)
-    E = _solve(sa, A, 20, 64, 1e-11, "LR", False, sa.rand_uniform(0, Ah.n))
+    E = _solve(sa, A, 20, 64, 1e-10, "LR", False, sa.rand_uniform(0, Ah.n), lock=False)
     assert E.getConverged() >= 20
     s = np.sqrt((1 + 0.1 * np.array([1.0, 0.5, 0.25])) * (1 - 0.1 * np.array([1.0, 0.5, 0.25])))
     c = np.cos(np.arange(1, 12) * np.pi / (N + 1))
