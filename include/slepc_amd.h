@@ -71,6 +71,15 @@ int sa_mat_mult(sa_mat A, const void *x_dev, void *y_dev);
  * partial sums live in device memory the library allocates per call. */
 int sa_mat_residual_norms(sa_mat A, int k, const void *X_dev, int64_t ldx, const double *lr_host,
                           const double *li_host, double *res_host);
+/* The same computation without any allocation: no counterpart in the
+ * reference; for callers that cannot allocate device memory (code under HIP
+ * graph capture, serving loops, harnesses that forbid hipMalloc).  The
+ * partial sums live in a static device array of the library (131,200 bytes
+ * per GPU, held until the process ends), so at most 16 vectors (one launch)
+ * per call; more is an error.  Calls that overlap on one GPU share that
+ * array and race; nothing in it needs clearing between calls. */
+int sa_mat_residual_norms_noalloc(sa_mat A, int k, const void *X_dev, int64_t ldx, const double *lr_host,
+                                  const double *li_host, double *res_host);
 
 /* ---- MatSetValues-style assembly and the stencil families ----------------
  * COO triplets -> CSR with ascending columns and INSERT_VALUES semantics

@@ -108,6 +108,7 @@ int sa_init(int device) {
   SA_CHECK(std::string(prop.gcnArchName).rfind("gfx950", 0) == 0,
            std::string("libslepc_amd is built for gfx950, found ") + prop.gcnArchName);
   g_cus = prop.multiProcessorCount;
+  SA_TRY(noalloc_scratch_setup(device));
   if (t_stream) (void)hipStreamDestroy(t_stream);
   t_stream = nullptr;
   g_device = device;
@@ -1074,6 +1075,20 @@ int sa_mat_residual_norms(sa_mat M, int k, const void *X, int64_t ldx, const dou
   const int rc = residual_norms(M->A, X, ldx, k, lr, li, res, part, part + (size_t)RESID_MAXVEC * RESID_MAXG, stream());
   (void)hipFree(part);
   return rc;
+}
+
+// The same with the partial sums in the library's static device scratch: no
+// allocation; at most RESID_MAXVEC vectors (one launch) per call.
+int sa_mat_residual_norms_noalloc(sa_mat M, int k, const void *X, int64_t ldx, const double *lr, const double *li,
+                                  double *res) {
+  API_INIT();
+  SA_CHECK(M && X && lr && res && k >= 0, "null argument to sa_mat_residual_norms_noalloc");
+  SA_CHECK(ldx >= M->A.n, "leading dimension of X smaller than the matrix size");
+  SA_CHECK(k <= RESID_MAXVEC, "sa_mat_residual_norms_noalloc takes at most 16 vectors per call");
+  if (k == 0) return 0;
+  double *part = noalloc_scratch(g_device);
+  SA_CHECK(part != nullptr, "the residual scratch of this device is not set up (sa_init)");
+  return residual_norms(M->A, X, ldx, k, lr, li, res, part, part + (size_t)RESID_MAXVEC * RESID_MAXG, stream());
 }
 
 int sa_eps_get_stats(sa_eps E, sa_eps_stats *st) {

@@ -141,6 +141,10 @@ int launch_resid_norms(const Mat &A, const void *X, int64_t ldx, const ResidArgs
 // the stream; part/out as for launch_resid_norms.
 int residual_norms(const Mat &A, const void *X, int64_t ldx, int k, const double *lr, const double *li,
                    double *res, double *part, double *out, hipStream_t s);
+// Device address of the residual scratch array on a device (sa_resid.hip):
+// set up once per device by sa_init with that device current; null before.
+int noalloc_scratch_setup(int device);
+double *noalloc_scratch(int device);
 // Shift-and-invert solves x = (A - sigma I)^-1 b (BiCGStab, FP64; optional
 // Jacobi preconditioning), workspace of 9 vectors + partials in one buffer.
 struct KspWork {

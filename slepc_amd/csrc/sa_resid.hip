@@ -12,6 +12,7 @@
 // of each vector in a fixed order, so equal inputs give equal bits.
 #include <algorithm>
 #include <cmath>
+#include <mutex>
 
 #include "sa_device.h"
 #include "sa_internal.h"
@@ -154,6 +155,38 @@ int residual_norms(const Mat &A, const void *X, int64_t ldx, int k, const double
     i += 2;
   }
   return flush();
+}
+
+// ---- scratch of the no-allocation entry point ------------------------------
+// The partial sums and their totals of one launch (RESID_MAXVEC vectors of at
+// most RESID_MAXG partials: 131,200 bytes) in a file-scope device array, so
+// that sa_mat_residual_norms_noalloc allocates nothing.  Each GPU holds its
+// own copy for the life of the process; its device address is taken once
+// per device at initialisation (the array's name in host code is only a
+// host-side handle).  Every partial and total is written by the launch that
+// reads it, so nothing needs clearing between calls; calls that overlap on
+// one GPU share the array and race.
+constexpr size_t NOALLOC_DOUBLES = (size_t)RESID_MAXVEC * RESID_MAXG + RESID_MAXVEC;
+static __device__ __align__(256) double scratch[NOALLOC_DOUBLES];
+constexpr int NOALLOC_MAXDEV = 64;
+static double *g_scratch_dev[NOALLOC_MAXDEV] = {};
+static std::mutex g_scratch_mu;
+
+int noalloc_scratch_setup(int device) {
+  SA_CHECK(device >= 0 && device < NOALLOC_MAXDEV, "device index beyond the scratch table");
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  if (g_scratch_dev[device]) return 0;
+  void *p = nullptr;
+  SA_HIP(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+  SA_CHECK(p != nullptr, "no device address for the residual scratch");
+  g_scratch_dev[device] = static_cast<double *>(p);
+  return 0;
+}
+
+double *noalloc_scratch(int device) {
+  if (device < 0 || device >= NOALLOC_MAXDEV) return nullptr;
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  return g_scratch_dev[device];
 }
 
 }  // namespace sa
