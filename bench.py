@@ -326,8 +326,10 @@ def run_batched(sa):
     from slepc_amd.batch import Batch, uneven_laplacians
     csrs = uneven_laplacians(P=512, seed=2)
     B = Batch(csrs)
+    B.set_true_residual(True)
     out = {"workload": "512 Laplacians nx,ny in U{32..256} + diag 10 U[0,1) seed 2, nev 8 LM, ncv 24, "
-                       "tol 1e-10 (%d rows, %d nnz)" % (sum(A.n for A in csrs), sum(A.nnz for A in csrs))}
+                       "tol 1e-10 on true residuals (%d rows, %d nnz)" % (sum(A.n for A in csrs),
+                                                                         sum(A.nnz for A in csrs))}
     for name, mixed in (("plain_fp64", False), ("mixed", True)):
         t0 = time.perf_counter()
         B.solve(8, 24, 1e-10, "LM", mixed=mixed, seed=2)
@@ -353,7 +355,7 @@ def main():
     ap.add_argument("--steps", type=int, default=8)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--streams", type=int, default=STREAMS)
-    ap.add_argument("--cpu-restarts", type=int, default=30)
+    ap.add_argument("--cpu-max-it", type=int, default=2000, help="restart cap of the CPU baseline solves")
     ap.add_argument("--no-scale", action="store_true", help="skip the 128^3 sections")
     ap.add_argument("--no-batched", action="store_true", help="skip the 512-problem batched family")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
@@ -429,13 +431,16 @@ def main():
         E1, dt1, good1, w1 = solve_one(sa, M1, None, "LM", True, 10, 32, 1e-8, v1, False)
         A1 = orc.CSR(L1.n, L1.rowptr, L1.col, L1.val)
         nth = min(host_cores(), 16)
-        r1 = orc.eps_solve(A1, 10, 32, 1e-8, "LM", True, v0=v1, nthreads=nth, nvec=0)
-        r1s = orc.eps_solve(A1, 10, 32, 1e-8, "LM", True, v0=v1, nthreads=1, nvec=0)
+        orc.eps_solve(A1, 10, 32, 1e-8, "LM", True, v0=v1, nthreads=nth, nvec=0)          # warm-up
+        r1 = min((orc.eps_solve(A1, 10, 32, 1e-8, "LM", True, v0=v1, nthreads=nth, nvec=0) for _ in range(3)),
+                 key=lambda r: r["seconds"])
+        r1s = min((orc.eps_solve(A1, 10, 32, 1e-8, "LM", True, v0=v1, nthreads=1, nvec=0) for _ in range(3)),
+                  key=lambda r: r["seconds"])
         c1 = {"workload": "2-D 5-pt Laplacian 100x100 (n=10,000), HEP, nev 10 LM, ncv 32, tol 1e-8, start U[-1,1] "
                           "seed 0",
               "gpu_eigenpairs_per_s": round(good1 / dt1, 2), "gpu_seconds": round(dt1, 4), "gpu_pairs": good1,
               "gpu_restarts": E1.getIterationNumber(),
-              "cpu_oracle_seconds": round(r1["seconds"], 4), "cpu_oracle_threads": nth,
+              "cpu_oracle_seconds": round(r1["seconds"], 4), "cpu_oracle_threads": nth, "cpu_timing": "best of 3 after 1 warm-up",
               "cpu_oracle_seconds_1_thread": round(r1s["seconds"], 4), "cpu_oracle_restarts": r1["its"]}
         log("c1: %s" % json.dumps(c1))
 
@@ -456,19 +461,44 @@ def main():
     batched = run_batched(sa) if not args.no_batched else None
 
     # ---------------- CPU baseline (the oracle) on the headline problem
+    # One held-out problem solved completely by the oracle (the timed problem
+    # whose plain GPU solve has the fewest matvecs, so that the sample stays
+    # within tens of seconds): best of 3 after one warm-up on the host cores,
+    # plus one single-core solve; the plain GPU solve of the same problem is
+    # reported beside it.  The family figure scales the per-matvec time to
+    # the family's mean matvec count.
     cpu = None
     if not args.no_cpu:
         from oracle import oracle as orc
-        Ah = probs[args.warmup][1]
+        mvs = res["plain"]["matvecs"]
+        idx = int(np.argmin(mvs))
+        Ah = probs[args.warmup + idx][1]
         Ao = orc.CSR(Ah.n, Ah.rowptr, Ah.col, Ah.val)
         nth = min(host_cores(), 16)
-        r = orc.eps_solve(Ao, NEV, NCV, TOL, "LR", True, v0=v0, max_it=args.cpu_restarts, nthreads=nth, nvec=0)
-        per_mv = r["seconds"] / max(1, r["matvecs"])
-        full_mv = float(np.mean(res["plain"]["matvecs"]))
-        cpu = {"value": round(NEV / (per_mv * full_mv), 5), "unit": "eigenpairs/s", "cores": nth, "kind": "port",
-               "sample": "CPU oracle (FP64 Krylov-Schur restatement, OpenMP) on held-out problem %d for %d restarts "
-                         "(%d matvecs, %.2f s), scaled to the %d matvecs of the full plain solve"
-                         % (args.warmup, r["its"], r["matvecs"], r["seconds"], full_mv)}
+        kw = dict(v0=v0, max_it=args.cpu_max_it, nvec=0)
+        orc.eps_solve(Ao, NEV, NCV, TOL, "LR", True, nthreads=nth, **kw)               # warm-up
+        runs = [orc.eps_solve(Ao, NEV, NCV, TOL, "LR", True, nthreads=nth, trueres=True, **kw) for _ in range(3)]
+        best = min(runs, key=lambda r: r["seconds"])
+        one = orc.eps_solve(Ao, NEV, NCV, TOL, "LR", True, nthreads=1, trueres=True, **kw)
+        full_mv = float(np.mean(mvs))
+        gpu_s = res["plain"]["solve_s"][idx]
+        cpu = {"value": round(min(NEV, best["nconv"]) / best["seconds"], 5), "unit": "eigenpairs/s",
+               "cores": nth, "kind": "port",
+               "sample": "full solve of held-out problem %d (the timed problem with the fewest plain-GPU matvecs, "
+                         "%d) by the CPU oracle (FP64 Krylov-Schur restatement, OpenMP, true residuals), best of 3 "
+                         "after 1 warm-up: %.2f s, %d restarts, %d matvecs" % (args.warmup + idx, mvs[idx],
+                                                                            best["seconds"], best["its"],
+                                                                            best["matvecs"]),
+               "seconds_best_of_3": round(best["seconds"], 3),
+               "seconds_runs": [round(r["seconds"], 3) for r in runs],
+               "value_1_core": round(min(NEV, one["nconv"]) / one["seconds"], 5),
+               "seconds_1_core": round(one["seconds"], 3),
+               "gpu_plain_seconds_same_problem": gpu_s,
+               "family_estimate_eigenpairs_per_s": round(NEV / (best["seconds"] / max(1, best["matvecs"]) * full_mv),
+                                                         5),
+               "family_estimate_note": "per-matvec time of the full solve scaled to the family's mean %d matvecs"
+                                       % full_mv}
+        log("cpu baseline: %s" % json.dumps(cpu))
 
     m, p, pc = res["mixed"], res["plain"], res["plain_conc"]
     out = {

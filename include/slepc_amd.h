@@ -386,6 +386,10 @@ int sa_batch_destroy(sa_batch batch);
 int sa_batch_solve(sa_batch batch, int which, int nev, int ncv, double tol, int max_it,
                    int mixed, double tol32, const double *v0, uint64_t seed);
 int sa_batch_get_converged(sa_batch batch, int p, int *nconv, int *its, int *reason);
+/* EPSSetTrueResidual (epsopts.c:1333-1370) for every problem: the convergence
+   test takes the explicit residual of each candidate Ritz vector
+   (epskrylov.c:225-265), formed and evaluated for all problems at once. */
+int sa_batch_set_true_residual(sa_batch batch, int trueres);
 int sa_batch_get_eigenpair(sa_batch batch, int p, int i, double *re, double *x_host);
 /* err[p*nev + i] = ||A x - lambda x|| / |lambda| in FP64 on the device. */
 int sa_batch_compute_errors(sa_batch batch, int nev, double *err);

@@ -21,6 +21,7 @@ def _sig():
     L.sa_batch_compute_errors.argtypes = [vp, C.c_int, vp]
     L.sa_batch_get_stats.argtypes = [vp, vp, vp]
     L.sa_batch_set_sweep.argtypes = [vp, C.c_int]
+    L.sa_batch_set_true_residual.argtypes = [vp, C.c_int]
     L.sa_batch_get_sweep.argtypes = [vp, C.POINTER(C.c_int), C.POINTER(C.c_double), C.POINTER(C.c_double)]
     return L
 
@@ -56,6 +57,10 @@ class Batch:
         check(self.L.sa_batch_create(self.P, self.n.ctypes.data, rp.ctypes.data, ci.ctypes.data, va.ctypes.data,
                                      C.byref(self.h)))
         self.nev = 0
+
+    def set_true_residual(self, trueres=True):
+        """EPSSetTrueResidual for every problem of the batch."""
+        check(self.L.sa_batch_set_true_residual(self.h, 1 if trueres else 0))
 
     def solve(self, nev, ncv, tol, which="LM", mixed=False, tol32=1e-5, v0=None, seed=0, max_it=0):
         from . import WHICH

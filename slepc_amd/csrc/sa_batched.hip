@@ -263,6 +263,78 @@ __global__ void batched_scale_kernel(T *V, int64_t ldv, const int *chunk_prob, c
 }
 
 // ---------------------------------------------------------------------------
+// True residuals of candidate Ritz vectors (EPSSetTrueResidual,
+// epskrylov.c:225-265) for every problem at once: slot s of problem p holds
+// the coefficients z of one candidate (a column of the projected problem's
+// eigenvectors) and its eigenvalue.  batched_ritz_kernel forms x = V_p(:,0:nv_p) z
+// into column s of X (the problem's rows), batched_tres_kernel takes
+// ||A x - lambda x||^2 over each 64-row chunk of the block-diagonal operator
+// (the problem's rows only), and batched_tres_sum_kernel adds a problem's
+// chunk partials in a fixed order.
+// ---------------------------------------------------------------------------
+constexpr int TR_SLOTS = 4;
+
+template <typename T>
+__global__ __launch_bounds__(64) void batched_ritz_kernel(const T *V, int64_t ldv, const int *chunk_prob,
+                                                          const int *chunk_row, const int64_t *off, const int *n,
+                                                          const int *tnv, const int *tact, const double *Z, int m,
+                                                          T *X) {
+  const int g = blockIdx.x, sl = blockIdx.y, p = chunk_prob[g];
+  if (!tact[p * TR_SLOTS + sl]) return;
+  const int r = chunk_row[g] + (int)threadIdx.x;
+  if (r >= n[p]) return;
+  const double *z = Z + ((size_t)p * TR_SLOTS + sl) * m;
+  const T *vp = V + off[p] + r;
+  double acc = 0.0;
+  for (int i = 0; i < tnv[p]; i++) acc += (double)vp[(size_t)i * ldv] * z[i];
+  X[(size_t)sl * ldv + off[p] + r] = (T)acc;
+}
+
+template <typename T, int L>
+__global__ __launch_bounds__(256) void batched_tres_kernel(const int32_t *__restrict__ rp, const int32_t *__restrict__ ci,
+                                                           const T *__restrict__ val, const T *X, int64_t ldv,
+                                                           const int *chunk_prob, const int *chunk_row,
+                                                           const int64_t *off, const int *n, const int *tact,
+                                                           const double *lam, double *part, int nchunks) {
+  __shared__ double red[4];
+  const int g = blockIdx.x, sl = blockIdx.y, p = chunk_prob[g];
+  if (!tact[p * TR_SLOTS + sl]) return;   // uniform over the workgroup
+  const T *x = X + (size_t)sl * ldv;
+  const double lm = lam[p * TR_SLOTS + sl];
+  const int r0 = chunk_row[g], rows = min(64, n[p] - r0), lg = threadIdx.x % L;
+  double acc = 0.0;
+  for (int rr = threadIdx.x / L; rr < 64; rr += 256 / L) {
+    const int64_t row = off[p] + r0 + rr;
+    T sum = (T)0;
+    if (rr < rows)
+      for (int32_t q = rp[row] + lg; q < rp[row + 1]; q += L) sum += val[q] * x[ci[q]];
+#pragma unroll
+    for (int o = L / 2; o > 0; o >>= 1) sum += __shfl_xor(sum, o, L);
+    if (lg == 0 && rr < rows) {
+      const double rv = (double)sum - lm * (double)x[row];
+      acc += rv * rv;
+    }
+  }
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) acc += __shfl_xor(acc, o, 64);
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = acc;
+  __syncthreads();
+  if (threadIdx.x == 0) part[(size_t)sl * nchunks + g] = (red[0] + red[1]) + (red[2] + red[3]);
+}
+
+__global__ __launch_bounds__(64) void batched_tres_sum_kernel(const double *part, int nchunks, const int *pchunk0,
+                                                              const int *pnchunk, const int *tact, double *out) {
+  const int p = blockIdx.x, sl = blockIdx.y;
+  if (!tact[p * TR_SLOTS + sl]) return;
+  const double *pp = part + (size_t)sl * nchunks + pchunk0[p];
+  double acc = 0.0;
+  for (int i = threadIdx.x; i < pnchunk[p]; i += 64) acc += pp[i];
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) acc += __shfl_xor(acc, o, 64);
+  if (threadIdx.x == 0) out[p * TR_SLOTS + sl] = acc;
+}
+
+// ---------------------------------------------------------------------------
 // Lock-step ("wide") sweep.  Step j orthogonalises column c_p = k0[p]+1+j of
 // every listed problem p (c_p = k0[p] without SpMV); problems whose sweep is
 // shorter keep their state chain and skip the work.  One workgroup per active
@@ -377,6 +449,10 @@ struct BatchDev {
   double *wpart = nullptr, *wdots = nullptr;   // 2 x [nchunks][m], 2 x [P][m]
   int *cprob = nullptr, *crow = nullptr, *pc0 = nullptr, *pnc = nullptr;
   int wm = 0;                                  // basis width the wide buffers were sized for
+  // true residuals: candidate columns, coefficients, eigenvalues, partials
+  void *tx = nullptr;                          // TR_SLOTS columns of ldv
+  double *tz = nullptr, *tlam = nullptr, *tpart = nullptr, *tout = nullptr;
+  int *tnv = nullptr, *tact = nullptr, *pchunk0 = nullptr, *pnchunk = nullptr;
 };
 
 template <typename T, int CM, int NT>
@@ -531,6 +607,7 @@ using namespace sa;
 struct sa_batch_s {
   int P = 0;
   int sweep_req = 0;                    // 0 autotune, 1 persistent, 2 lock-step
+  int trueres = 0;                      // EPSSetTrueResidual for every problem
   int sweep_used = 0;                   // mode of the last solve's FP64 phase
   double ms_persistent = -1.0, ms_wide = -1.0;   // autotuner timings of one restart sweep
   int tuned_m[2] = {0, 0}, tuned_mode[2] = {0, 0};   // autotuner's choice per precision (FP64, FP32) and basis width
@@ -554,7 +631,8 @@ static void free_dev(BatchDev &B) {
   void *ptrs[] = {B.V,   B.rp,    B.ci,         B.val,        B.off,  B.n,    B.order, B.k0,  B.nv,
                   B.stop, B.passes, B.dbuf,     B.beta,       B.Q,    B.alpha, B.chunk_prob, B.chunk_row,
                   B.ml,  B.mk,    B.ms,         B.me,         B.src,  B.dst,  B.wst,   B.wpart, B.wdots,
-                  B.cprob, B.crow, B.pc0,       B.pnc};
+                  B.cprob, B.crow, B.pc0,       B.pnc,        B.tx,   B.tz,   B.tlam,  B.tpart, B.tout,
+                  B.tnv, B.tact, B.pchunk0, B.pnchunk};
   for (void *p : ptrs)
     if (p) (void)hipFree(p);
   B = BatchDev();
@@ -567,6 +645,8 @@ static int alloc_dev(sa_batch_s *b, BatchDev &B, int prec, int m) {
   else if (B.V) {
     (void)hipFree(B.V);
     B.V = nullptr;
+    if (B.tz) (void)hipFree(B.tz);   // sized by the basis width
+    B.tz = nullptr;
   }
   hipStream_t s = stream();
   B.P = b->P;
@@ -620,11 +700,77 @@ static int alloc_dev(sa_batch_s *b, BatchDev &B, int prec, int m) {
   return 0;
 }
 
+// Buffers of the true-residual rounds (allocated on the first solve that asks for them).
+static int ensure_tres(sa_batch_s *b, BatchDev &B) {
+  const int P = b->P, m = B.m;
+  hipStream_t s = stream();
+  if (!B.tx) {
+    SA_HIP(hipMalloc(&B.tx, prec_size(B.prec) * (size_t)B.ldv * TR_SLOTS));
+    SA_HIP(hipMalloc(&B.tlam, sizeof(double) * (size_t)P * TR_SLOTS));
+    SA_HIP(hipMalloc(&B.tout, sizeof(double) * (size_t)P * TR_SLOTS));
+    SA_HIP(hipMalloc(&B.tpart, sizeof(double) * (size_t)TR_SLOTS * std::max(1, B.nchunks)));
+    SA_HIP(hipMalloc(&B.tnv, sizeof(int) * P));
+    SA_HIP(hipMalloc(&B.tact, sizeof(int) * (size_t)P * TR_SLOTS));
+    SA_HIP(hipMalloc(&B.pchunk0, sizeof(int) * P));
+    SA_HIP(hipMalloc(&B.pnchunk, sizeof(int) * P));
+    std::vector<int> c0(P), nc(P);
+    int acc = 0;
+    for (int q = 0; q < P; q++) {    // chunks are numbered problem by problem
+      c0[q] = acc;
+      nc[q] = (b->n[q] + 63) / 64;
+      acc += nc[q];
+    }
+    SA_CHECK(acc == B.nchunks, "internal: chunk numbering");
+    SA_HIP(hipMemcpyAsync(B.pchunk0, c0.data(), sizeof(int) * P, hipMemcpyHostToDevice, s));
+    SA_HIP(hipMemcpyAsync(B.pnchunk, nc.data(), sizeof(int) * P, hipMemcpyHostToDevice, s));
+    SA_HIP(hipStreamSynchronize(s));
+  }
+  if (!B.tz) SA_HIP(hipMalloc(&B.tz, sizeof(double) * (size_t)P * TR_SLOTS * m));
+  return 0;
+}
+
+// One round: the candidates in Z/lam/act/nv (host, P x TR_SLOTS) -> squared
+// residual norms in out (host, P x TR_SLOTS).
+template <typename T>
+static void tres_launch(const BatchDev &B, hipStream_t s) {
+  const dim3 grid((unsigned)B.nchunks, TR_SLOTS);
+  hipLaunchKernelGGL(batched_ritz_kernel<T>, grid, dim3(64), 0, s, (const T *)B.V, B.ldv, B.chunk_prob, B.chunk_row,
+                     B.off, B.n, B.tnv, B.tact, B.tz, B.m, (T *)B.tx);
+  const T *val = (const T *)B.val;
+  switch (B.lanes) {
+    case 4: hipLaunchKernelGGL((batched_tres_kernel<T, 4>), grid, dim3(256), 0, s, B.rp, B.ci, val, (const T *)B.tx, B.ldv, B.chunk_prob, B.chunk_row, B.off, B.n, B.tact, B.tlam, B.tpart, B.nchunks); break;
+    case 8: hipLaunchKernelGGL((batched_tres_kernel<T, 8>), grid, dim3(256), 0, s, B.rp, B.ci, val, (const T *)B.tx, B.ldv, B.chunk_prob, B.chunk_row, B.off, B.n, B.tact, B.tlam, B.tpart, B.nchunks); break;
+    case 16: hipLaunchKernelGGL((batched_tres_kernel<T, 16>), grid, dim3(256), 0, s, B.rp, B.ci, val, (const T *)B.tx, B.ldv, B.chunk_prob, B.chunk_row, B.off, B.n, B.tact, B.tlam, B.tpart, B.nchunks); break;
+    default: hipLaunchKernelGGL((batched_tres_kernel<T, 32>), grid, dim3(256), 0, s, B.rp, B.ci, val, (const T *)B.tx, B.ldv, B.chunk_prob, B.chunk_row, B.off, B.n, B.tact, B.tlam, B.tpart, B.nchunks); break;
+  }
+  hipLaunchKernelGGL(batched_tres_sum_kernel, dim3((unsigned)B.P, TR_SLOTS), dim3(64), 0, s, B.tpart, B.nchunks,
+                     B.pchunk0, B.pnchunk, B.tact, B.tout);
+}
+
+static int tres_round(BatchDev &B, const std::vector<double> &Z, const std::vector<double> &lam,
+                      const std::vector<int> &act, const std::vector<int> &nv, std::vector<double> &out,
+                      hipStream_t s) {
+  const int P = B.P;
+  SA_HIP(hipMemcpyAsync(B.tz, Z.data(), sizeof(double) * (size_t)P * TR_SLOTS * B.m, hipMemcpyHostToDevice, s));
+  SA_HIP(hipMemcpyAsync(B.tlam, lam.data(), sizeof(double) * (size_t)P * TR_SLOTS, hipMemcpyHostToDevice, s));
+  SA_HIP(hipMemcpyAsync(B.tact, act.data(), sizeof(int) * (size_t)P * TR_SLOTS, hipMemcpyHostToDevice, s));
+  SA_HIP(hipMemcpyAsync(B.tnv, nv.data(), sizeof(int) * P, hipMemcpyHostToDevice, s));
+  if (B.prec == SA_FP64) tres_launch<double>(B, s);
+  else tres_launch<float>(B, s);
+  SA_HIP(hipGetLastError());
+  out.resize((size_t)P * TR_SLOTS);
+  SA_HIP(hipMemcpyAsync(out.data(), B.tout, sizeof(double) * (size_t)P * TR_SLOTS, hipMemcpyDeviceToHost, s));
+  SA_HIP(hipStreamSynchronize(s));
+  return 0;
+}
+
 namespace {
 struct ProbState {
   DenseProblem ds;
   std::vector<double> eigr, eigi, errest;
   int nconv = 0, l = 0, its = 0, reason = 0, k = 0, nv = 0, k0 = 0;
+  int tnext = 0;           // true residuals: next candidate index, or -1 once decided
+  int breakdown = 0;
   bool active = true, restart_random = false;
   double beta = 0.0;
   int stop = -1;
@@ -650,12 +796,15 @@ double nowb() { return std::chrono::duration<double>(std::chrono::steady_clock::
 // One batched Krylov-Schur solve (symmetric problems, Lanczos + DSHEP on each)
 // in basis precision prec; starts from column 0 of V.  Mirrors
 // EPSSolve_KrylovSchur_Default (krylovschur.c:227-348) per problem.
-static int batch_ks(sa_batch_s *b, BatchDev &B, int which, int nev, int ncv, double tol, int max_it,
+static int batch_ks(sa_batch_s *b, BatchDev &B, int which, int nev, int ncv, double tol, int max_it, bool trueres,
                     std::vector<ProbState> &ps, int64_t &matvecs, int64_t &passes_out, int &its_max, double &tds) {
   const int P = b->P, m = ncv + 1, ld = m, mpd = ncv;
   const double eta = 0.7071, keep = 0.5;
   hipStream_t s = stream();
   SA_CHECK(lapack_ready(), "no LAPACK library found for the projected problem");
+  if (trueres) SA_TRY(ensure_tres(b, B));
+  std::vector<double> tZ, tlam, tout;
+  std::vector<int> tact, tnvv;
   ps.assign(P, ProbState());
   for (int p = 0; p < P; p++) {
     ps[p].ds.allocate(ld, true, which);
@@ -764,6 +913,11 @@ static int batch_ks(sa_batch_s *b, BatchDev &B, int which, int nev, int ncv, dou
       if (S.ds.solve(S.eigr.data(), S.eigi.data())) { fail[ia] = 1; return; }
       S.ds.sort(S.eigr.data(), S.eigi.data());
       S.ds.update_extra_row();
+      S.breakdown = breakdown;
+      if (trueres) {          // decided by the device rounds below
+        S.tnext = S.nconv;
+        return;
+      }
       int k, marker = -1;
       for (k = S.nconv; k < nvp; k++) {
         double rn = 0.0;
@@ -776,6 +930,70 @@ static int batch_ks(sa_batch_s *b, BatchDev &B, int which, int nev, int ncv, dou
         if (marker != -1) break;
       }
       if (marker != -1) k = marker;
+      S.k = k;
+    });
+    for (size_t ia = 0; ia < act.size(); ia++) SA_CHECK(!fail[ia], "LAPACK failure in a batched projected problem");
+    if (trueres) {
+      // EPSKrylovConvergence with true residuals (epskrylov.c:225-265): per
+      // problem, the candidates from nconv on in order until the first whose
+      // explicit residual fails; a round evaluates up to TR_SLOTS candidates
+      // of every undecided problem (those whose estimate passes, and the
+      // first that fails, which usually ends the walk).
+      tZ.assign((size_t)P * TR_SLOTS * m, 0.0);
+      tlam.assign((size_t)P * TR_SLOTS, 0.0);
+      tact.assign((size_t)P * TR_SLOTS, 0);
+      tnvv.assign(P, 0);
+      while (true) {
+        bool any = false;
+        std::fill(tact.begin(), tact.end(), 0);
+        for (int p : act) {
+          ProbState &S = ps[p];
+          if (S.tnext < 0) continue;
+          if (S.tnext >= S.nv) {        // every candidate passed
+            S.k = S.nv;
+            S.tnext = -1;
+            continue;
+          }
+          tnvv[p] = S.nv;
+          for (int sl = 0; sl < TR_SLOTS && S.tnext + sl < S.nv; sl++) {
+            const int k = S.tnext + sl;
+            const double *Qk = S.ds.Q() + (size_t)k * ld;
+            std::memcpy(tZ.data() + ((size_t)p * TR_SLOTS + sl) * m, Qk, sizeof(double) * S.nv);
+            tlam[(size_t)p * TR_SLOTS + sl] = std::fabs(S.eigr[k]) > 2.220446049250313e-16 ? S.eigr[k] : 0.0;
+            tact[(size_t)p * TR_SLOTS + sl] = 1;
+            any = true;
+            double rn = 0.0;
+            int cp = 0;
+            S.ds.ritz_residual(k, &rn, &cp);
+            const double w = std::fabs(S.eigr[k]);
+            if ((w != 0.0 ? rn * beta[p] / w : 1.79e308) >= tol) break;   // the estimate fails: likely the end
+          }
+        }
+        if (!any) break;
+        SA_TRY(tres_round(B, tZ, tlam, tact, tnvv, tout, s));
+        for (int p : act) {
+          ProbState &S = ps[p];
+          if (S.tnext < 0) continue;
+          int sl = 0;
+          for (; sl < TR_SLOTS && tact[(size_t)p * TR_SLOTS + sl]; sl++) {
+            const int k = S.tnext + sl;
+            const double w = std::fabs(S.eigr[k]);
+            S.errest[k] = w != 0.0 ? std::sqrt(tout[(size_t)p * TR_SLOTS + sl]) / w : 1.79e308;
+            if (S.errest[k] >= tol) {
+              S.k = k;
+              S.tnext = -1;
+              break;
+            }
+          }
+          if (S.tnext >= 0) S.tnext += sl;
+        }
+      }
+    }
+    parallel_for((int)act.size(), [&](int ia) {
+      const int p = act[ia];
+      ProbState &S = ps[p];
+      const int nvp = S.nv, breakdown = S.breakdown;
+      const int k = S.k;
       if (k >= nev) S.reason = 1;
       else if (S.its >= max_it) S.reason = -1;
       int newl = 0;
@@ -972,7 +1190,8 @@ int sa_batch_solve(sa_batch b, int which, int nev, int ncv, double tol, int max_
     SA_HIP(hipMemcpyAsync(b->dev32.V, f.data(), sizeof(float) * b->nrows, hipMemcpyHostToDevice, s));
     SA_HIP(hipStreamSynchronize(s));
     const double t1 = nowb();
-    SA_TRY(batch_ks(b, b->dev32, which, nev, ncv, tol32, max_it, ps, b->matvecs32, b->passes32, b->its_max32, tds));
+    SA_TRY(batch_ks(b, b->dev32, which, nev, ncv, tol32, max_it, b->trueres != 0, ps, b->matvecs32, b->passes32,
+                    b->its_max32, tds));
     // polish seeds: sum of the first nev Ritz vectors of each problem, in FP64
     SA_TRY(launch_convert(SA_FP32, b->dev32.V, b->dev32.ldv, SA_FP64, b->dev64.V, b->dev64.ldv, b->nrows, m, s));
     std::vector<double> Qh((size_t)P * m * m, 0.0);
@@ -1000,7 +1219,8 @@ int sa_batch_solve(sa_batch b, int which, int nev, int ncv, double tol, int max_
     SA_HIP(hipMemcpyAsync(b->dev64.V, x0.data(), sizeof(double) * b->nrows, hipMemcpyHostToDevice, s));
     SA_HIP(hipStreamSynchronize(s));
   }
-  SA_TRY(batch_ks(b, b->dev64, which, nev, ncv, tol, max_it, ps, b->matvecs, b->passes, b->its_max, tds));
+  SA_TRY(batch_ks(b, b->dev64, which, nev, ncv, tol, max_it, b->trueres != 0, ps, b->matvecs, b->passes, b->its_max,
+                  tds));
   b->nconv.resize(P);
   b->its.resize(P);
   b->reason.resize(P);
@@ -1101,6 +1321,15 @@ int sa_batch_get_stats(sa_batch b, int64_t *counters6, double *seconds3) {
 // the first solve phase of each precision and basis width, keep the faster
 // for later solves of the batch), 1 one workgroup per problem, 2 lock-step.
 // Setting a mode forgets earlier autotuner choices.
+// EPSSetTrueResidual for every problem of the batch: the convergence test
+// takes explicit residuals of the candidate Ritz vectors, formed and
+// evaluated on the device for all problems at once.
+int sa_batch_set_true_residual(sa_batch b, int trueres) {
+  SA_CHECK(b, "null batch");
+  b->trueres = trueres != 0;
+  return 0;
+}
+
 int sa_batch_set_sweep(sa_batch b, int mode) {
   SA_CHECK(b && mode >= 0 && mode <= 2, "invalid sweep mode");
   b->sweep_req = mode;

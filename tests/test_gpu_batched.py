@@ -87,3 +87,31 @@ def test_batched_fv_family_ncv64_matches_single_solves(sa, orc):
         if p == 0:
             r = orc.eps_solve(orc.CSR(A.n, A.rowptr, A.col, A.val), 16, 64, 1e-10, "LR", True, v0=v0[p])
             assert np.allclose(vals, np.sort(r["eigr"][:16]), rtol=1e-10, atol=0)
+
+
+@pytest.mark.parametrize("sweep", ["persistent", "wide"])
+@pytest.mark.parametrize("mixed", [False, True])
+def test_batched_true_residual_matches_oracle(sa, orc, mixed, sweep):
+    """EPSSetTrueResidual for the whole batch (candidate Ritz vectors of all
+    problems formed and evaluated on the device at once): every problem
+    gives the oracle's values with its true-residual branch, the error
+    estimates are explicit residuals (equal to EPSComputeError of the
+    returned pair to rounding), and every pair meets the tolerance."""
+    from slepc_amd.batch import Batch
+    csrs = _problems(sa, orc)
+    v0 = [orc.rand_uniform(orc.sub_seed(9, p), A.n) for p, A in enumerate(csrs)]
+    B = Batch(csrs)
+    B.set_sweep(sweep)
+    B.set_true_residual(True)
+    B.solve(8, 24, 1e-10, "LM", mixed=mixed, v0=v0)
+    err = B.errors()
+    for p, A in enumerate(csrs):
+        nconv, its, reason = B.converged(p)
+        assert reason == 1 and nconv >= 8
+        Ao = orc.CSR(A.n, A.rowptr, A.col, A.val)
+        r = orc.eps_solve(Ao, 8, 24, 1e-10, "LM", True, v0=v0[p], trueres=True)
+        vals = np.array([B.eigenpair(p, i, False)[0] for i in range(8)])
+        assert np.allclose(np.sort(vals), np.sort(r["eigr"][:8]), rtol=1e-10, atol=0)
+        if not mixed:
+            assert abs(its - r["its"]) <= max(2, 0.2 * r["its"])
+        assert np.all(err[p] <= 1e-10), err[p]
