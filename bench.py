@@ -46,7 +46,7 @@ def log(*a):
 
 
 def solve_one(sa, A64, A32, which, herm, nev, ncv, tol, v0, mixed, init=None, tol_req=None, trueres=True,
-              lock=True):
+              lock=True, max_it=20000):
     """Returns (EPS, seconds, good pairs, worst residual).  init: optional
     callable producing the start vector; its cost (network inference and
     prolongation) is part of the time-to-solution.  The convergence test
@@ -60,7 +60,7 @@ def solve_one(sa, A64, A32, which, herm, nev, ncv, tol, v0, mixed, init=None, to
     E.setProblemType(herm)
     E.setWhichEigenpairs(which)
     E.setDimensions(nev, ncv)
-    E.setTolerances(tol)
+    E.setTolerances(tol, max_it)
     E.setTrueResidual(trueres)
     E.setKrylovSchurLocking(lock)
     E.setInitialSpace(v0 if v is None else v)
@@ -321,6 +321,24 @@ def run_scale(sa):
     return scale
 
 
+def run_learned_training(sa):
+    """Training of the warm-start network alone (2,000 coarse solves by the
+    batched solver, then 60 epochs of BF16 MFMA forward/backward/Adam in
+    batches of 64) with the GEMM rate of the training steps: FLOPs per step
+    6 B h dout + 6 B h^2 + 4 B h dpad (forward, dW and dX of the three layers)."""
+    from slepc_amd import learned
+    ws = learned.WarmStart.train(sa, log=log)
+    B, h, o, dp = 64, learned.HIDDEN, learned.NMODES * learned.COARSE ** 2, 32
+    flops = 6.0 * B * h * o + 6.0 * B * h * h + 4.0 * B * h * dp
+    out = {"n_train": ws.n_train, "coarse_solves_s": round(ws.seconds_solves, 3),
+           "epochs_s": round(ws.seconds_epochs, 3), "steps": ws.steps,
+           "ms_per_step": round(1e3 * ws.seconds_epochs / max(1, ws.steps), 3),
+           "gemm_tflops_per_s": round(flops * ws.steps / max(1e-9, ws.seconds_epochs) / 1e12, 2),
+           "loss_first_last": [round(ws.train_losses[0], 4), round(ws.train_losses[-1], 4)]}
+    log("learned training: %s" % json.dumps(out))
+    return out
+
+
 def run_batched(sa):
     """The 512-problem uneven family, plain FP64 and mixed (autotuned sweep)."""
     from slepc_amd.batch import Batch, uneven_laplacians
@@ -355,7 +373,7 @@ def main():
     ap.add_argument("--steps", type=int, default=8)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--streams", type=int, default=STREAMS)
-    ap.add_argument("--cpu-max-it", type=int, default=2000, help="restart cap of the CPU baseline solves")
+    ap.add_argument("--cpu-max-it", type=int, default=20000, help="restart cap of the CPU baseline solves")
     ap.add_argument("--no-scale", action="store_true", help="skip the 128^3 sections")
     ap.add_argument("--no-batched", action="store_true", help="skip the 512-problem batched family")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
@@ -365,7 +383,7 @@ def main():
     ap.add_argument("--sweep-case", default="all", choices=["all", "fv2d_256", "convdiff_128", "stencil27_128"],
                     help="operator of --sweep-only")
     ap.add_argument("--spmv", action="store_true", help="with --sweep-only: also time the standalone SpMV")
-    ap.add_argument("--only", choices=["scale", "batched"],
+    ap.add_argument("--only", choices=["scale", "batched", "learned"],
                     help="run one section alone and print its JSON (for profiling runs)")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
@@ -393,6 +411,9 @@ def main():
                 if name == "convdiff_128":
                     out[name]["block_ops"] = block_ops_roofline(sa, Ah)
         print(json.dumps(out))
+        return
+    if args.only == "learned":
+        print(json.dumps(run_learned_training(sa)))
         return
     if args.only:
         print(json.dumps(run_scale(sa) if args.only == "scale" else run_batched(sa)))

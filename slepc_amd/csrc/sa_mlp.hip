@@ -4,6 +4,7 @@
 #include <hip/hip_bf16.h>
 
 #include <algorithm>
+#include <cstdint>
 #include <cmath>
 #include <cstring>
 #include <vector>
@@ -20,8 +21,13 @@ __device__ __forceinline__ __bf16 to_bf16(float x) { return (__bf16)x; }
 // ---------------------------------------------------------------------------
 // C[M x N] (fp32, row-major, ldc) = sum_k A(i,k) B(k,j)  with generic strides
 //   A(i,k) = A[i*sai + k*sak], B(k,j) = B[k*sbk + j*sbj] (bf16)
-// Block tile 64x64, 4 waves of 32x32, K step 32 staged through LDS.
-// Epilogue: optional bias[j] (fp32), optional ReLU, optional bf16 copy, beta.
+// Block tile 64x64, 4 waves of 32x32, K step 32 staged through LDS.  When a
+// stride is 1 the tiles are staged with 16-byte loads (8 bf16 per thread per
+// tile): along k straight into the [row][k] LDS layout, along i or j with the
+// 8 values scattered into it.  Long K (the backward product dY W3, K = dout)
+// is split over gridDim.z slices whose FP32 partials a second kernel adds in a
+// fixed order; otherwise the epilogue runs in place: optional bias[j] (fp32),
+// optional ReLU, optional bf16 copy, beta.
 // ---------------------------------------------------------------------------
 struct GemmArgs {
   int M, N, K;
@@ -38,40 +44,60 @@ struct GemmArgs {
 };
 
 constexpr int GT = 64, GK = 32;
+typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
 
-__global__ __launch_bounds__(256) void gemm_bf16_kernel(GemmArgs g) {
+// staging modes: 0 generic, 1 contiguous along k, 2 contiguous along the row/column index
+template <int MODE>
+__device__ __forceinline__ void stage_tile(__bf16 (*S)[GK + 8], const __bf16 *P, long sr, long sk, int r0, int k0,
+                                           int R, int K, int t) {
+  if (MODE == 1) {            // 64 rows x 4 vectors of 8 along k
+    const int r = t >> 2, kv = (t & 3) * 8;
+    const int gr = r0 + r, gk = k0 + kv;
+    u32x4 v = (u32x4){0u, 0u, 0u, 0u};
+    if (gr < R && gk < K) v = *reinterpret_cast<const u32x4 *>(P + (long)gr * sr + gk);
+    *reinterpret_cast<u32x4 *>(&S[r][kv]) = v;
+  } else if (MODE == 2) {     // 32 k x 8 vectors of 8 along the row index
+    const int k = t >> 3, rv = (t & 7) * 8;
+    const int gr = r0 + rv, gk = k0 + k;
+    union { u32x4 v; __bf16 h[8]; } u;
+    u.v = (u32x4){0u, 0u, 0u, 0u};
+    if (gk < K && gr < R) u.v = *reinterpret_cast<const u32x4 *>(P + (long)gk * sk + gr);
+#pragma unroll
+    for (int q = 0; q < 8; q++) S[rv + q][k] = u.h[q];
+  } else {
+    for (int e = t; e < GT * GK; e += 256) {
+      const int r = e / GK, k = e % GK;
+      const int gr = r0 + r, gk = k0 + k;
+      S[r][k] = (gr < R && gk < K) ? P[(long)gr * sr + (long)gk * sk] : to_bf16(0.f);
+    }
+  }
+}
+
+template <int AM, int BM>
+__global__ __launch_bounds__(256) void gemm_bf16_kernel(GemmArgs g, int kchunk, float *part) {
   __shared__ __bf16 As[GT][GK + 8];   // [i][k]
   __shared__ __bf16 Bs[GT][GK + 8];   // [j][k]
   const int t = threadIdx.x, lane = t & 63, w = t >> 6;
   const int i0 = blockIdx.y * GT, j0 = blockIdx.x * GT;
   const int wi = (w >> 1) * 32, wj = (w & 1) * 32;
+  const int kb = blockIdx.z * kchunk, ke = min(g.K, kb + kchunk);
   f32x4 acc[2][2];
 #pragma unroll
   for (int a = 0; a < 2; a++)
 #pragma unroll
     for (int b = 0; b < 2; b++) acc[a][b] = (f32x4){0.f, 0.f, 0.f, 0.f};
-  for (int k0 = 0; k0 < g.K; k0 += GK) {
-    // stage 64x32 of A and 32x64 of B (8 elements per thread each)
-    for (int e = t; e < GT * GK; e += 256) {
-      const int r = e / GK, k = e % GK;
-      const int gi = i0 + r, gk = k0 + k;
-      As[r][k] = (gi < g.M && gk < g.K) ? g.A[(long)gi * g.sai + (long)gk * g.sak] : to_bf16(0.f);
-      const int gj = j0 + r;
-      Bs[r][k] = (gj < g.N && gk < g.K) ? g.B[(long)gk * g.sbk + (long)gj * g.sbj] : to_bf16(0.f);
-    }
+  for (int k0 = kb; k0 < ke; k0 += GK) {
+    stage_tile<AM>(As, g.A, g.sai, g.sak, i0, k0, g.M, ke, t);
+    stage_tile<BM>(Bs, g.B, g.sbj, g.sbk, j0, k0, g.N, ke, t);
     __syncthreads();
     // fragments: A lane l -> row l%16, k = 8*(l/16)..+7 ; B lane l -> col l%16, same k
     const int fr = lane & 15, fk = (lane >> 4) * 8;
 #pragma unroll
     for (int a = 0; a < 2; a++) {
-      bf16x8 av;
-#pragma unroll
-      for (int q = 0; q < 8; q++) av[q] = As[wi + a * 16 + fr][fk + q];
+      const bf16x8 av = *reinterpret_cast<const bf16x8 *>(&As[wi + a * 16 + fr][fk]);
 #pragma unroll
       for (int b = 0; b < 2; b++) {
-        bf16x8 bv;
-#pragma unroll
-        for (int q = 0; q < 8; q++) bv[q] = Bs[wj + b * 16 + fr][fk + q];
+        const bf16x8 bv = *reinterpret_cast<const bf16x8 *>(&Bs[wj + b * 16 + fr][fk]);
         acc[a][b] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(av, bv, acc[a][b], 0, 0, 0);
       }
     }
@@ -88,6 +114,10 @@ __global__ __launch_bounds__(256) void gemm_bf16_kernel(GemmArgs g) {
         const int j = j0 + wj + b * 16 + (lane & 15);
         if (i < g.M && j < g.N) {
           float x = acc[a][b][v];
+          if (part) {                 // split K: raw partial of slice z
+            part[((long)blockIdx.z * g.M + i) * g.N + j] = x;
+            continue;
+          }
           if (g.bias) x += g.bias[j];
           if (g.relu) x = fmaxf(x, 0.f);
           float *cp = g.C + (long)i * g.ldc + j;
@@ -98,10 +128,56 @@ __global__ __launch_bounds__(256) void gemm_bf16_kernel(GemmArgs g) {
       }
 }
 
-static int gemm(const GemmArgs &g, hipStream_t s) {
+// Sum of the split-K slices in slice order, then the epilogue.
+__global__ void gemm_splitk_sum_kernel(GemmArgs g, int nz, const float *part) {
+  const long e = (long)blockIdx.x * 256 + threadIdx.x;
+  if (e >= (long)g.M * g.N) return;
+  const int i = (int)(e / g.N), j = (int)(e % g.N);
+  float x = 0.f;
+  for (int z = 0; z < nz; z++) x += part[(long)z * g.M * g.N + e];
+  if (g.bias) x += g.bias[j];
+  if (g.relu) x = fmaxf(x, 0.f);
+  float *cp = g.C + (long)i * g.ldc + j;
+  if (g.beta != 0.f) x += g.beta * (*cp);
+  *cp = x;
+  if (g.Cb) g.Cb[(long)i * g.ldc + j] = to_bf16(x);
+}
+
+// staging mode of an operand: 1 when k is contiguous, 2 when the row/column
+// index is, with 16-byte aligned vectors; else 0
+static int stage_mode(const __bf16 *P, long sr, long sk, int R, int K) {
+  const bool al = (reinterpret_cast<uintptr_t>(P) & 15) == 0;
+  if (al && sk == 1 && sr % 8 == 0 && K % 8 == 0) return 1;
+  if (al && sr == 1 && sk % 8 == 0 && R % 8 == 0) return 2;
+  return 0;
+}
+
+template <int AM>
+static void gemm_launch_b(int bm, dim3 grid, const GemmArgs &g, int kchunk, float *part, hipStream_t s) {
+  if (bm == 1) hipLaunchKernelGGL((gemm_bf16_kernel<AM, 1>), grid, dim3(256), 0, s, g, kchunk, part);
+  else if (bm == 2) hipLaunchKernelGGL((gemm_bf16_kernel<AM, 2>), grid, dim3(256), 0, s, g, kchunk, part);
+  else hipLaunchKernelGGL((gemm_bf16_kernel<AM, 0>), grid, dim3(256), 0, s, g, kchunk, part);
+}
+
+// split: workspace of at least GEMM_SPLIT_MAX * M * N floats for products whose
+// output tile grid is small and K long, else null
+constexpr int GEMM_SPLIT_MAX = 64;
+static int gemm(const GemmArgs &g, hipStream_t s, float *split = nullptr) {
   if (g.M == 0 || g.N == 0) return 0;
-  dim3 grid((g.N + GT - 1) / GT, (g.M + GT - 1) / GT);
-  hipLaunchKernelGGL(gemm_bf16_kernel, grid, dim3(256), 0, s, g);
+  const int am = stage_mode(g.A, g.sai, g.sak, g.M, g.K), bm = stage_mode(g.B, g.sbj, g.sbk, g.N, g.K);
+  const int tiles = ((g.N + GT - 1) / GT) * ((g.M + GT - 1) / GT);
+  int nz = 1;
+  if (split && tiles < 256 && g.K >= 4096) nz = std::min(GEMM_SPLIT_MAX, std::max(1, 512 / tiles));
+  const int kchunk = ((g.K + nz - 1) / nz + GK - 1) / GK * GK;
+  nz = (g.K + kchunk - 1) / kchunk;
+  dim3 grid((g.N + GT - 1) / GT, (g.M + GT - 1) / GT, nz);
+  float *part = nz > 1 ? split : nullptr;
+  if (am == 1) gemm_launch_b<1>(bm, grid, g, kchunk, part, s);
+  else if (am == 2) gemm_launch_b<2>(bm, grid, g, kchunk, part, s);
+  else gemm_launch_b<0>(bm, grid, g, kchunk, part, s);
+  if (part)
+    hipLaunchKernelGGL(gemm_splitk_sum_kernel, dim3((unsigned)(((long)g.M * g.N + 255) / 256)), dim3(256), 0, s, g,
+                       nz, (const float *)part);
   SA_HIP(hipGetLastError());
   return 0;
 }
@@ -209,6 +285,7 @@ struct sa_mlp_s {
   float *H1 = nullptr, *H2 = nullptr, *Y = nullptr, *T = nullptr, *dY = nullptr, *dH2 = nullptr, *dH1 = nullptr,
         *Xf = nullptr;
   double *part = nullptr;
+  float *split = nullptr;     // split-K partials: GEMM_SPLIT_MAX x maxB x hid
   long step = 0;
 };
 
@@ -232,6 +309,8 @@ static int mlp_ensure_batch(sa_mlp net, int B) {
   SA_HIP(hipMalloc(&net->dY, sizeof(float) * B * o));
   SA_HIP(hipMalloc(&net->dH2, sizeof(float) * B * h));
   SA_HIP(hipMalloc(&net->dH1, sizeof(float) * B * h));
+  fr(net->split);
+  SA_HIP(hipMalloc(&net->split, sizeof(float) * GEMM_SPLIT_MAX * B * h));
   net->maxB = B;
   return 0;
 }
@@ -266,7 +345,7 @@ static int mlp_backward_dev(sa_mlp net, int B, const float *T_host, float *loss)
   hipLaunchKernelGGL(colsum_kernel, dim3(nblk(o)), dim3(256), 0, s, net->dY, B, (int)o, net->gb[2]);
   // dH2 = dY W3 (B x h): A(i,k) = dY[i*o + k], B(k,j) = W3[k*h + j]
   GemmArgs b2{B, (int)h, (int)o, net->dYb, o, 1, net->Wb[2], h, 1, net->dH2, h, nullptr, nullptr, 0, 0.f};
-  SA_TRY(gemm(b2, s));
+  SA_TRY(gemm(b2, s, net->split));
   hipLaunchKernelGGL(relu_back_kernel, dim3(nblk((long)B * h)), dim3(256), 0, s, net->dH2, net->H2, net->dZ2b, (long)B * h);
   hipLaunchKernelGGL(colsum_kernel, dim3(nblk(h)), dim3(256), 0, s, net->dH2, B, (int)h, net->gb[1]);
   // dW2 = dZ2^T H1 (h x h)
@@ -345,6 +424,7 @@ int sa_mlp_destroy(sa_mlp net) {
   fr(net->Xb); fr(net->H1b); fr(net->H2b); fr(net->dYb); fr(net->dZ2b); fr(net->dZ1b);
   fr(net->H1); fr(net->H2); fr(net->Y); fr(net->T); fr(net->dY); fr(net->dH2); fr(net->dH1); fr(net->Xf);
   fr(net->part);
+  fr(net->split);
   delete net;
   return 0;
 }

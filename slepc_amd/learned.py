@@ -164,7 +164,7 @@ class WarmStart:
         self.net, self.probes, self.scale = net, probes, scale
 
     @classmethod
-    def train(cls, sa, n_train=256, epochs=60, batch=64, lr=1e-3, seed=1, log=None):
+    def train(cls, sa, n_train=2000, epochs=60, batch=64, lr=1e-3, seed=1, log=None):
         t0 = time.perf_counter()
         W = probe_vectors()
         thetas, targets = [], []
@@ -187,12 +187,14 @@ class WarmStart:
                 idx = order[b0:b0 + batch]
                 losses.append(net.train_step(thetas[idx], T[idx], lr))
         t2 = time.perf_counter()
+        ws_steps = len(losses)
         if log:
             log("learned warm start: %d coarse solves in %.1f s, %d epochs in %.1f s, loss %.4f -> %.4f"
                 % (len(thetas), t1 - t0, epochs, t2 - t1, losses[0], losses[-1]))
         ws = cls(net, W, scale)
         ws.train_losses = losses
         ws.n_train = len(thetas)
+        ws.seconds_solves, ws.seconds_epochs, ws.steps = t1 - t0, t2 - t1, ws_steps
         return ws
 
     _cached = None

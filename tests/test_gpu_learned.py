@@ -92,45 +92,58 @@ def test_prolongation_matches_oracle(sa):
 
 def test_learned_start_gain_over_random_start(sa):
     """What the trained network gains over the method without it, on held-out
-    draws of the 128x128 family.  The network's start vector lies almost
-    entirely in the wanted 16-dimensional eigenspace (measured against the
-    converged FP64 eigenvectors of the same problem), where a random vector
-    has a fraction ~16/n of its energy; that is the gain this test asserts.
-    Krylov-Schur starts from that one vector and still decides convergence:
-    every pair meets 1e-10 either way.  On this family the matvec count is
-    set by the relative gaps (~1e-6) at the edge of the wanted cluster, which
-    the start vector does not change, so the matvec ratio stays near 1 and is
-    only checked not to regress beyond 25% on average."""
+    draws of the workload family at its full size (256 x 256, 16 pairs,
+    ncv 64, tol 1e-10 on true residuals).  The network's start vector lies
+    almost entirely in the wanted 16-dimensional eigenspace (measured against
+    the converged FP64 eigenvectors of the same problem), where a random
+    vector has a fraction ~16/n of its energy: that gain is asserted.
+    Krylov-Schur starts from that one vector and still decides convergence,
+    so every pair meets 1e-10 either way.  The time-to-solution ratio
+    random/learned is measured on the same problems: on this family the
+    matvec count is set by the relative gaps (~1e-6) at the edge of the
+    wanted cluster, which one start vector does not change: the ratios are
+    printed and asserted not to fall below 0.85 (no gain is claimed)."""
+    import time
     from slepc_amd.learned import WarmStart
-    ws = WarmStart.train(sa, n_train=96, epochs=40, batch=32, lr=1e-3)
+    ws = WarmStart.train(sa, n_train=256, epochs=40, batch=64, lr=1e-3)
     assert ws.train_losses[-1] < 0.5 * ws.train_losses[0]
-    fracs, ratios = {"random": [], "learned": []}, []
-    for s in range(3):
-        th = sa.fv_theta(5, s)
-        Ah = sa.fv2d(128, 128, th)
+    fracs, ratios, tratios = {"random": [], "learned": []}, [], []
+    for s in range(4):
+        th = sa.fv_theta(5, 20 + s)
+        Ah = sa.fv2d(256, 256, th)
         A = sa.Mat(Ah)
-        res = {}
-        starts = (("random", sa.rand_uniform(0, Ah.n)), ("learned", ws.initial_vector(th, 128, 128)))
-        for name, v0 in starts:
+        res, tt = {}, {}
+        starts = (("random", lambda: sa.rand_uniform(0, Ah.n)), ("learned", lambda: ws.initial_vector(th, 256, 256)))
+        vecs = {}
+        for name, make in starts:
+            t0 = time.perf_counter()
+            v0 = make()
             assert v0 is not None
             E = sa.EPS()
             E.setOperators(A)
             E.setProblemType(True)
             E.setWhichEigenpairs("LR")
             E.setDimensions(16, 64)
-            E.setTolerances(1e-10)
+            E.setTolerances(1e-10, 20000)      # the hardest draws need more than the default 2n/ncv restarts
+            E.setTrueResidual(True)
             E.setInitialSpace(v0)
             E.solve()
+            tt[name] = time.perf_counter() - t0
+            vecs[name] = v0
             assert E.getConverged() >= 16
             for i in range(16):
                 assert E.computeError(i) <= 1e-10
             res[name] = E.getStats()["matvecs"]
             if name == "random":
                 X = np.stack([E.getEigenpair(i)[2] for i in range(16)])   # orthonormal rows
-        for name, v0 in starts:
+        for name in ("random", "learned"):
+            v0 = vecs[name]
             fracs[name].append(np.linalg.norm(X @ v0) ** 2 / np.dot(v0, v0))
         ratios.append(res["random"] / res["learned"])
-    print("energy fraction in the wanted eigenspace:", fracs, "matvec ratio random/learned:", ratios)
+        tratios.append(tt["random"] / tt["learned"])
+    print("energy fraction in the wanted eigenspace:", fracs, "matvec ratio random/learned:", ratios,
+          "time ratio:", tratios)
     assert min(fracs["learned"]) > 0.5
     assert min(fracs["learned"]) > 100 * max(fracs["random"])
-    assert np.mean(ratios) > 0.8, ratios
+    assert np.mean(ratios) > 0.85, ratios
+    assert np.mean(tratios) > 0.85, tratios
