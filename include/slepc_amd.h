@@ -254,18 +254,25 @@ int sa_eps_set_extraction(sa_eps eps, int extraction);
    SA_ST_SINVERT (shift-and-invert, src/sys/classes/st/impls/sinvert: the
    operator is (A - sigma I)^-1, eigenvalues mapped back by sigma + 1/theta;
    needs which SA_TARGET_MAGNITUDE or SA_TARGET_REAL).  Without shift_given
-   sigma is the target (STSetDefaultShift).  The solves run on the device by
-   BiCGStab (sa_eps_set_st_ksp; the reference's default is a sparse direct
-   LU, PREONLY + LU). */
+   sigma is the target (STSetDefaultShift).  The solves run on the device,
+   by default as the reference's KSPPREONLY + PCLU (stsles.c:53-54): a banded
+   LU with partial pivoting of A - sigma I factored once per solve, or by
+   BiCGStab (sa_eps_set_st_ksp). */
 #define SA_ST_SHIFT 0
 #define SA_ST_SINVERT 1
 int sa_eps_set_st(sa_eps eps, int st_type, int shift_given, double shift);
-/* KSP of the spectral transformation: BiCGStab with SA_PC_JACOBI (default)
-   or SA_PC_NONE; rtol relative to ||b|| (default 1e-12), max_it (10000).
-   Jacobi needs the diagonal of A - sigma I bounded away from zero; a solve
-   that breaks down or does not converge ends the solve with an error. */
+/* KSP of the spectral transformation: SA_PC_LU (default) is KSPPREONLY +
+   PCLU, a banded LU on the device (dgbtf2/dgbtrs steps, one workgroup; the
+   band is taken from the CSR pattern and limited to 1 GiB of storage and
+   2e9 multiply-adds, beyond which the call errors); SA_PC_JACOBI or
+   SA_PC_NONE select BiCGStab with that preconditioner, rtol relative to
+   ||b|| (default 1e-12) and max_it (10000) — Jacobi needs the diagonal of
+   A - sigma I bounded away from zero, and a solve that breaks down or does
+   not converge ends the solve with an error (indefinite non-symmetric
+   A - sigma I can stagnate it; the LU does not). */
 #define SA_PC_NONE 0
 #define SA_PC_JACOBI 1
+#define SA_PC_LU 2
 int sa_eps_set_st_ksp(sa_eps eps, int pc_type, double rtol, int max_it);
 /* EPSSetBalance (epsopts.c:1160-1215; EPSBuildBalance_Krylov,
    epsdefault.c:443-504): balancing of non-Hermitian problems by a diagonal D

@@ -387,10 +387,11 @@ class EPS:
         check(lib().sa_eps_set_st(self.h, {"shift": 0, "sinvert": 1}[st_type], 0 if shift is None else 1,
                                   0.0 if shift is None else float(shift)))
 
-    def setSTKSP(self, pc="jacobi", rtol=1e-12, max_it=10000):
-        """Linear solver of the spectral transformation: BiCGStab with a
-        Jacobi ("jacobi") or no ("none") preconditioner."""
-        check(lib().sa_eps_set_st_ksp(self.h, {"none": 0, "jacobi": 1}[pc], float(rtol), int(max_it)))
+    def setSTKSP(self, pc="lu", rtol=1e-12, max_it=10000):
+        """Linear solver of the spectral transformation: "lu" (the default,
+        KSPPREONLY + PCLU: banded LU on the device) or BiCGStab with a Jacobi
+        ("jacobi") or no ("none") preconditioner."""
+        check(lib().sa_eps_set_st_ksp(self.h, {"none": 0, "jacobi": 1, "lu": 2}[pc], float(rtol), int(max_it)))
 
     def setBalance(self, balance="twoside", its=5, cutoff=1e-8):
         """EPSSetBalance: "none", "oneside" or "twoside" (non-Hermitian problems)."""

@@ -650,9 +650,10 @@ int sa_eps_set_st(sa_eps E, int st_type, int shift_given, double shift) {
 }
 
 int sa_eps_set_st_ksp(sa_eps E, int pc_type, double rtol, int max_it) {
-  SA_CHECK(E && (pc_type == SA_PC_NONE || pc_type == SA_PC_JACOBI), "unsupported preconditioner");
+  SA_CHECK(E && (pc_type == SA_PC_NONE || pc_type == SA_PC_JACOBI || pc_type == SA_PC_LU), "unsupported preconditioner");
   SA_CHECK(rtol > 0.0 && rtol < 1.0 && max_it > 0, "invalid linear solver tolerances");
   E->p.ksp_jacobi = pc_type == SA_PC_JACOBI;
+  E->p.ksp_direct = pc_type == SA_PC_LU;
   E->p.ksp_rtol = rtol;
   E->p.ksp_max_it = max_it;
   return 0;

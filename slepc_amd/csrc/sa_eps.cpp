@@ -557,7 +557,7 @@ int ks_solve(const KsParams &p, const Mat &A, BV &V, bool have_initial, KsResult
     KspWork &w;
     ~KspFree() { ksp_free(w); }
   } kw_free{kw};
-  if (sinv) SA_TRY(ksp_alloc(kw, A, p.st_shift, p.ksp_jacobi));
+  if (sinv) SA_TRY(ksp_alloc(kw, A, p.st_shift, p.ksp_jacobi, p.ksp_direct));
   // y = Op x: the operator of the spectral transformation (local Lanczos)
   std::function<int(const void *, void *)> op_apply = [&](const void *x, void *y) -> int {
     if (!sinv) return launch_spmv(A, x, y, stream());

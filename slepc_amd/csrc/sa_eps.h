@@ -16,6 +16,7 @@ struct KsParams {
   double target = 0.0;         // EPSSetTarget: comparison point of which 7/8 and the harmonic shift
   int st_type = 0;             // 0 shift (sigma = 0), 1 shift-and-invert (STSINVERT)
   double st_shift = 0.0;       // sigma of the shift-and-invert operator
+  bool ksp_direct = true;      // KSPPREONLY + PCLU (banded LU on the device); else BiCGStab
   bool ksp_jacobi = true;      // PCJACOBI for the BiCGStab solves (else none)
   double ksp_rtol = 1e-12;     // relative residual of each solve
   int ksp_max_it = 10000;

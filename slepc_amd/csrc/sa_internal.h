@@ -147,14 +147,21 @@ int noalloc_scratch_setup(int device);
 double *noalloc_scratch(int device);
 // Shift-and-invert solves x = (A - sigma I)^-1 b (BiCGStab, FP64; optional
 // Jacobi preconditioning), workspace of 9 vectors + partials in one buffer.
+// With direct, a banded LU of A - sigma I (KSPPREONLY + PCLU) factored on the
+// device at set-up, and each solve is its two substitutions.
 struct KspWork {
   double *buf = nullptr;
   int64_t n = 0;
   int G = 1;
   double sigma = 0.0;
   bool jacobi = true;
+  bool direct = false;
+  double *ab = nullptr;        // band storage, ldab x n
+  int *ipiv = nullptr;         // n pivots + info
+  int64_t ab_n = 0, ldab = 0;
+  int kl = 0, ku = 0;
 };
-int ksp_alloc(KspWork &w, const Mat &A, double sigma, bool jacobi);
+int ksp_alloc(KspWork &w, const Mat &A, double sigma, bool jacobi, bool direct);
 void ksp_free(KspWork &w);
 // rtol relative to ||b||; error when max_it is reached or on breakdown
 int ksp_solve(const Mat &A, KspWork &w, const double *b_dev, double *x_dev, double rtol, int max_it, int *its);

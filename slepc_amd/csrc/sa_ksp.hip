@@ -173,11 +173,191 @@ __global__ __launch_bounds__(BLK) void ksp_jacobi_kernel(int n, const int32_t *_
   dinv[i] = d != 0.0 ? 1.0 / d : 1.0;
 }
 
+// ---------------------------------------------------------------------------
+// Direct solves (KSPPREONLY + PCLU, the reference's default for STSINVERT,
+// stsles.c:53-54): banded LU with partial pivoting of A - sigma I in LAPACK's
+// band layout (dgbtf2/dgbtrs): element (r, c) at ab[(kv + r - c) + c * ldab],
+// kv = kl + ku, ldab = 2 kl + ku + 1, the rows above ku holding the fill-in.
+// One workgroup runs the column loop (the pivot search, the row swap, the
+// multipliers and the rank-1 update of the band each a parallel phase); the
+// solves run the two substitutions the same way.
+// ---------------------------------------------------------------------------
+constexpr int LU_T = 1024;
+
+__global__ void band_width_kernel(int n, const int32_t *__restrict__ rp, const int32_t *__restrict__ ci, int *klku) {
+  const int r = blockIdx.x * BLK + threadIdx.x;
+  if (r >= n) return;
+  int lo = 0, hi = 0;
+  for (int32_t q = rp[r]; q < rp[r + 1]; q++) {
+    lo = max(lo, r - ci[q]);
+    hi = max(hi, ci[q] - r);
+  }
+  atomicMax(klku, lo);
+  atomicMax(klku + 1, hi);
+}
+
+__global__ void band_fill_kernel(int n, const int32_t *__restrict__ rp, const int32_t *__restrict__ ci,
+                                 const double *__restrict__ val, double sigma, int kv, int64_t ldab, double *ab) {
+  const int r = blockIdx.x * BLK + threadIdx.x;
+  if (r >= n) return;
+  ab[kv + (int64_t)r * ldab] = -sigma;   // the shift, also where the pattern has no diagonal entry
+  for (int32_t q = rp[r]; q < rp[r + 1]; q++) {
+    const int c = ci[q];
+    ab[(kv + r - c) + (int64_t)c * ldab] = val[q] - (c == r ? sigma : 0.0);
+  }
+}
+
+// dgbtf2: info[0] = first zero pivot + 1, or 0
+__global__ __launch_bounds__(LU_T) void band_lu_kernel(int n, int kl, int ku, int64_t ldab, double *ab, int *ipiv,
+                                                       int *info) {
+  __shared__ double smax[LU_T / 64];
+  __shared__ int sidx[LU_T / 64];
+  __shared__ int s_jp, s_ju;
+  const int t = threadIdx.x, kv = kl + ku;
+  if (t == 0) { s_ju = 0; info[0] = 0; }
+  __syncthreads();
+  for (int j = 0; j < n; j++) {
+    double *col = ab + (int64_t)j * ldab;
+    const int km = min(kl, n - 1 - j);
+    // pivot: first index of the largest |a| among rows j..j+km (idamax)
+    double best = -1.0;
+    int bi = 0;
+    for (int i = t; i <= km; i += LU_T) {
+      const double a = fabs(col[kv + i]);
+      if (a > best) { best = a; bi = i; }
+    }
+    for (int o = 32; o > 0; o >>= 1) {
+      const double ob = __shfl_xor(best, o, 64);
+      const int oi = __shfl_xor(bi, o, 64);
+      if (ob > best || (ob == best && oi < bi)) { best = ob; bi = oi; }
+    }
+    if ((t & 63) == 0) { smax[t >> 6] = best; sidx[t >> 6] = bi; }
+    __syncthreads();
+    if (t == 0) {
+      double b = smax[0];
+      int i0 = sidx[0];
+      for (int w = 1; w < LU_T / 64; w++)
+        if (smax[w] > b || (smax[w] == b && sidx[w] < i0)) { b = smax[w]; i0 = sidx[w]; }
+      s_jp = i0;
+      ipiv[j] = j + i0;
+      if (col[kv + i0] != 0.0) s_ju = max(s_ju, min(j + ku + i0, n - 1));
+      else if (info[0] == 0) info[0] = j + 1;
+    }
+    __syncthreads();
+    const int jp = s_jp, ju = s_ju;
+    if (col[kv + jp] == 0.0) continue;          // uniform: singular column, nothing to eliminate
+    if (jp != 0)                                // swap rows j and j+jp over columns j..ju
+      for (int c = j + t; c <= ju; c += LU_T) {
+        double *a = ab + (kv + j - c) + (int64_t)c * ldab, *b = a + jp;
+        const double x = *a;
+        *a = *b;
+        *b = x;
+      }
+    __syncthreads();
+    const double rp_ = 1.0 / col[kv];
+    for (int i = 1 + t; i <= km; i += LU_T) col[kv + i] *= rp_;
+    __syncthreads();
+    const int nc = ju - j;                       // rank-1 update of rows j+1..j+km, columns j+1..ju
+    for (int e = t; e < nc * km; e += LU_T) {
+      const int cc = e / km, i = e % km + 1, c = j + 1 + cc;
+      double *cb = ab + (int64_t)c * ldab + (kv + j - c);
+      cb[i] -= col[kv + i] * cb[0];
+    }
+    __syncthreads();
+  }
+}
+
+// dgbtrs ('N') on one right-hand side held in x
+__global__ __launch_bounds__(LU_T) void band_solve_kernel(int n, int kl, int ku, int64_t ldab, const double *ab,
+                                                          const int *ipiv, double *x) {
+  const int t = threadIdx.x, kv = kl + ku;
+  for (int j = 0; j < n - 1; j++) {             // L: swaps and the multipliers of column j
+    const int l = ipiv[j];
+    if (t == 0 && l != j) {
+      const double tmp = x[l];
+      x[l] = x[j];
+      x[j] = tmp;
+    }
+    __syncthreads();
+    const int lm = min(kl, n - 1 - j);
+    const double xj = x[j];
+    const double *col = ab + (int64_t)j * ldab + kv;
+    for (int i = 1 + t; i <= lm; i += LU_T) x[j + i] -= col[i] * xj;
+    __syncthreads();
+  }
+  for (int j = n - 1; j >= 0; j--) {            // U, column by column
+    const double *col = ab + (int64_t)j * ldab;
+    __shared__ double s_xj;
+    if (t == 0) {
+      s_xj = x[j] / col[kv];
+      x[j] = s_xj;
+    }
+    __syncthreads();
+    const double xj = s_xj;
+    const int i0 = max(0, j - kv);
+    for (int i = i0 + t; i < j; i += LU_T) x[i] -= col[kv + i - j] * xj;
+    __syncthreads();
+  }
+}
+
 }  // namespace
 
-int ksp_alloc(KspWork &w, const Mat &A, double sigma, bool jacobi) {
+// Limits of the device banded LU: band storage and the work of the
+// one-workgroup factorization (n kl (kl + ku) multiply-adds).
+constexpr int64_t LU_MAX_BAND_DOUBLES = 1ll << 27;   // 1 GiB
+constexpr double LU_MAX_WORK = 2e9;
+
+static int band_lu_setup(KspWork &w, const Mat &A, double sigma) {
+  const int n = A.n;
+  hipStream_t s = stream();
+  int *klku = nullptr;
+  SA_HIP(hipMalloc(&klku, sizeof(int) * 4));
+  struct F { int *p; ~F() { (void)hipFree(p); } } fk{klku};
+  SA_HIP(hipMemsetAsync(klku, 0, sizeof(int) * 4, s));
+  const unsigned nb = (unsigned)std::max(1, (n + BLK - 1) / BLK);
+  hipLaunchKernelGGL(band_width_kernel, dim3(nb), dim3(BLK), 0, s, n, A.rowptr, A.col, klku);
+  int h[2] = {0, 0};
+  SA_HIP(hipMemcpyAsync(h, klku, sizeof(int) * 2, hipMemcpyDeviceToHost, s));
+  SA_HIP(hipStreamSynchronize(s));
+  const int kl = h[0], ku = h[1];
+  const int64_t ldab = 2 * (int64_t)kl + ku + 1;
+  SA_CHECK(ldab * n <= LU_MAX_BAND_DOUBLES && (double)n * kl * (kl + ku) <= LU_MAX_WORK,
+           "the band of A - sigma I is too wide for the device LU (use the BiCGStab solver, "
+           "sa_eps_set_st_ksp with SA_PC_JACOBI or SA_PC_NONE)");
+  if (w.ab_n != ldab * n) {
+    if (w.ab) (void)hipFree(w.ab);
+    if (w.ipiv) (void)hipFree(w.ipiv);
+    w.ab = nullptr;
+    w.ipiv = nullptr;
+    SA_HIP(hipMalloc(&w.ab, sizeof(double) * (size_t)std::max<int64_t>(1, ldab * n)));
+    SA_HIP(hipMalloc(&w.ipiv, sizeof(int) * (size_t)std::max(1, n) + sizeof(int)));
+    w.ab_n = ldab * n;
+  }
+  w.kl = kl;
+  w.ku = ku;
+  w.ldab = ldab;
+  SA_HIP(hipMemsetAsync(w.ab, 0, sizeof(double) * (size_t)std::max<int64_t>(1, ldab * n), s));
+  hipLaunchKernelGGL(band_fill_kernel, dim3(nb), dim3(BLK), 0, s, n, A.rowptr, A.col, (const double *)A.val, sigma,
+                     kl + ku, ldab, w.ab);
+  int *info = w.ipiv + n;
+  hipLaunchKernelGGL(band_lu_kernel, dim3(1), dim3(LU_T), 0, s, n, kl, ku, ldab, w.ab, w.ipiv, info);
+  SA_HIP(hipGetLastError());
+  int hinfo = 0;
+  SA_HIP(hipMemcpyAsync(&hinfo, info, sizeof(int), hipMemcpyDeviceToHost, s));
+  SA_HIP(hipStreamSynchronize(s));
+  SA_CHECK(hinfo == 0, "shift-and-invert: A - sigma I is exactly singular (zero pivot in the LU)");
+  return 0;
+}
+
+int ksp_alloc(KspWork &w, const Mat &A, double sigma, bool jacobi, bool direct) {
   SA_CHECK(A.prec == SA_FP64, "the shift-and-invert solves run in FP64");
   const int64_t n = A.n;
+  w.direct = direct;
+  if (direct) {
+    w.n = n;
+    w.sigma = sigma;
+    return band_lu_setup(w, A, sigma);
+  }
   if (w.n != n || !w.buf) {
     ksp_free(w);
     SA_HIP(hipMalloc(&w.buf, sizeof(double) * ((size_t)9 * std::max<int64_t>(n, 1) + 2 * KSP_MAXG + 8)));
@@ -196,14 +376,28 @@ int ksp_alloc(KspWork &w, const Mat &A, double sigma, bool jacobi) {
 
 void ksp_free(KspWork &w) {
   if (w.buf) (void)hipFree(w.buf);
+  if (w.ab) (void)hipFree(w.ab);
+  if (w.ipiv) (void)hipFree(w.ipiv);
   w.buf = nullptr;
+  w.ab = nullptr;
+  w.ipiv = nullptr;
+  w.ab_n = 0;
   w.n = 0;
 }
 
 int ksp_solve(const Mat &A, KspWork &w, const double *b, double *x, double rtol, int max_it, int *its_out) {
   const int64_t n = w.n;
-  SA_CHECK(n == A.n && w.buf, "solver workspace does not match the operator");
   hipStream_t s = stream();
+  if (w.direct) {        // PREONLY + LU: x = U^-1 L^-1 P b
+    SA_CHECK(n == A.n && w.ab, "solver workspace does not match the operator");
+    if (n == 0) return 0;
+    SA_HIP(hipMemcpyAsync(x, b, sizeof(double) * n, hipMemcpyDeviceToDevice, s));
+    hipLaunchKernelGGL(band_solve_kernel, dim3(1), dim3(LU_T), 0, s, (int)n, w.kl, w.ku, w.ldab, w.ab, w.ipiv, x);
+    SA_HIP(hipGetLastError());
+    if (its_out) *its_out = 1;
+    return 0;
+  }
+  SA_CHECK(n == A.n && w.buf, "solver workspace does not match the operator");
   double *r = w.buf, *rhat = r + n, *p = rhat + n, *v = p + n, *sv = v + n, *t = sv + n, *phat = t + n,
          *shat = phat + n, *dinv = w.jacobi ? shat + n : nullptr;
   double *part = w.buf + 9 * n, *scal = part + 2 * KSP_MAXG;

@@ -1035,3 +1035,71 @@ def test_mat_residual_norms_matches_oracle(sa, orc, prec):
     tol = 1e-12 if prec == "fp64" else 2e-6
     assert np.max(np.abs(res - ref) / ref) < tol
     assert res[3] == res[4] and res[17] == res[18]
+
+
+# ------------------------------------------------------------------ shift-and-invert, PREONLY + LU
+@pytest.mark.parametrize("solver,case", [("krylovschur", "markov15"), ("arnoldi", "markov15"),
+                                         ("lanczos", "test2"), ("krylovschur", "test2"),
+                                         ("krylovschur", "lap2d40"), ("krylovschur", "complex_inside"),
+                                         ("arnoldi", "complex_inside")])
+def test_sinvert_lu_matches_oracle(sa, orc, solver, case):
+    """Shift-and-invert with the default KSP, the reference's PREONLY + LU
+    (stsles.c:53-54): a banded LU of A - sigma I with partial pivoting
+    factored on the device, two substitutions per operator application.
+    Both sides solve exactly, so the GPU follows the oracle's trajectory:
+    eigenvalues in order to 1e-10, restart counts equal.  complex_inside:
+    the non-symmetric matrix with conjugate pairs at a target inside its
+    spectrum (A - 4 I indefinite), where BiCGStab stagnates."""
+    from test_cpu_oracle import _lap1d, _complex_csr, TEST2_INTERIOR
+    if case == "markov15":
+        Ao, herm, nev, tol, tgt = orc.markov(15), False, 3, 1e-8, 0.5
+    elif case == "test2":
+        Ao, herm, nev, tol, tgt = _lap1d(orc, 30), True, 4, 1e-9, 2.1
+    elif case == "lap2d40":
+        Ao, herm, nev, tol, tgt = orc.lap2d(40, 37), True, 6, 1e-10, 0.0
+    else:
+        Ao, herm, nev, tol, tgt = _complex_csr(orc), False, 4, 1e-10, 4.0
+    v0 = orc.rand_uniform(0, Ao.n)
+    E = sa.EPS()
+    E.setType(solver)
+    E.setOperators(sa.Mat(sa.CSR(Ao.n, Ao.rowptr, Ao.col, Ao.val)))
+    E.setProblemType(herm)
+    E.setWhichEigenpairs("TM")
+    E.setTarget(tgt)
+    E.setST("sinvert")
+    E.setDimensions(nev, 0)
+    E.setTolerances(tol, 500)
+    E.setInitialSpace(v0)
+    E.solve()
+    r = orc.eps_solve(Ao, nev, 0, tol, "TM", herm, v0=v0, target=tgt, st="sinvert", solver=solver, max_it=500)
+    assert E.getConverged() >= nev and r["nconv"] >= nev
+    pairs = [E.getEigenpair(i, False) for i in range(nev)]
+    vals = np.array([p[0] + 1j * p[1] for p in pairs])
+    ref = r["eigr"][:nev] + 1j * r["eigi"][:nev]
+    assert np.max(np.abs(vals - ref) / np.abs(ref)) < 1e-10
+    assert abs(E.getIterationNumber() - r["its"]) <= 1
+    if case == "markov15":
+        assert list(np.round(vals.real, 5)) == [0.49842, 0.51928, 0.47897]
+    if case == "test2":
+        assert np.allclose(vals.real, TEST2_INTERIOR[30], atol=5e-6)
+    if case == "complex_inside":
+        assert np.any(ref.imag != 0.0)
+    nrm = max(np.abs(Ao.val[Ao.rowptr[i]:Ao.rowptr[i + 1]]).sum() for i in range(Ao.n)) + abs(tgt)
+    for i in range(nev):
+        assert E.computeError(i) < 10 * tol * max(1.0, nrm / abs(vals[i]))
+
+
+def test_sinvert_lu_rejects_a_band_too_wide(sa):
+    """The 2-D Laplacian on 256 x 256 (band 256 on each side) exceeds the
+    device LU's work limit: the solve ends with an error naming BiCGStab as
+    the alternative, and no eigenpair is reported."""
+    A = sa.lap2d(256)
+    E = sa.EPS()
+    E.setOperators(sa.Mat(A))
+    E.setProblemType(True)
+    E.setWhichEigenpairs("TM")
+    E.setTarget(0.0)
+    E.setST("sinvert")
+    E.setDimensions(4, 0)
+    with pytest.raises(sa.SlepcAmdError, match="too wide"):
+        E.solve()
