@@ -55,6 +55,14 @@ int sa_timer_stop(double *ms);
  * ascending column indices per row; prec selects the stored value type. */
 int sa_mat_create_csr(int n, int64_t nnz, const int32_t *rowptr_host, const int32_t *col_host,
                       const double *val_host, int prec, sa_mat *A);
+/* Storage layout threshold: operators whose values and column indices take
+ * more than `bytes` are created for the stream SpMV and SpMM (row blocks of
+ * at most 2,048 nonzeros staged in LDS); smaller ones keep the
+ * vector-CSR kernel.  Default 64 MiB; returns the previous value; operators
+ * already created keep their layout.  No counterpart in the reference (a
+ * choice of storage format, as a MatType is); tests use 0 to run the stream
+ * kernels on small operators. */
+int64_t sa_set_spmv_stream_bytes(int64_t bytes);
 int sa_mat_destroy(sa_mat A);
 int sa_mat_get_info(sa_mat A, int *n, int64_t *nnz, int *prec);
 /* y = A x with x, y device vectors of the matrix precision (MatMult). */

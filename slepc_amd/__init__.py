@@ -177,6 +177,12 @@ class DeviceArray:
             pass
 
 
+def set_spmv_stream_bytes(nbytes):
+    """Operators whose values and indices exceed `nbytes` are created for the
+    stream SpMV (row blocks staged in LDS); returns the previous threshold."""
+    return int(lib().sa_set_spmv_stream_bytes(int(nbytes)))
+
+
 class Mat:
     """CSR operator in HBM (MATAIJ analogue); MatMult runs the HIP SpMV."""
 

@@ -161,11 +161,12 @@ int launch_bvdot(const BV &X, int lx, int kx, const BV &Y, int ly, int ky, doubl
 // BVMatMult: Y(:, j) = A X(:, j) for j < nc <= SPMM_NC.  L lanes per row;
 // each nonzero's value and column index are loaded once and applied to the
 // nc columns, so the operator is read once per SPMM_NC columns.  The sum of
-// each column runs in the order of the vector-CSR SpMV.
+// each column runs in the order of the vector-CSR SpMV.  Operators larger
+// than the caches take the stream layout below instead.
 // ---------------------------------------------------------------------------
 constexpr int SPMM_NC = 8;
 
-template <typename T, int L, bool NT>
+template <typename T, int L>
 __global__ __launch_bounds__(BLK) void spmm_kernel(int n, const int32_t *__restrict__ rp, const int32_t *__restrict__ ci,
                                                    const T *__restrict__ val, const T *__restrict__ X, int64_t ldx,
                                                    T *__restrict__ Y, int64_t ldy, int nc) {
@@ -177,8 +178,8 @@ __global__ __launch_bounds__(BLK) void spmm_kernel(int n, const int32_t *__restr
   if (row < n) {
     const int32_t pe = rp[row + 1];
     for (int32_t p = rp[row] + lg; p < pe; p += L) {
-      const T v = NT ? __builtin_nontemporal_load(val + p) : val[p];
-      const int32_t c = NT ? __builtin_nontemporal_load(ci + p) : ci[p];
+      const T v = val[p];
+      const int32_t c = ci[p];
 #pragma unroll
       for (int j = 0; j < SPMM_NC; j++)
         if (j < nc) s[j] += v * X[(size_t)j * ldx + c];
@@ -206,15 +207,121 @@ __global__ __launch_bounds__(BLK) void spmm_kernel(int n, const int32_t *__restr
   }
 }
 
-template <typename T, bool NT>
+// Stream SpMM for operators larger than the caches: a workgroup owns RB rows
+// (the stream SpMV's row blocks, at most SPMV_NZB nonzeros each) and stages
+// their values and column indices in LDS with coalesced non-temporal loads.
+// Thread t serves row t % RB for the columns t / RB + u * TPR (TPR = BLK / RB
+// threads per row), so a wave covers 64 consecutive rows of the same columns:
+// for a stencil its gathers of x read neighbouring addresses and its stores
+// are coalesced.  Each staged nonzero is read from LDS once per thread and
+// applied to all of the thread's columns.  Each column's sum is formed as the
+// stream SpMV forms it (TPR strided partial sums of rounded products, added in
+// the order of its shuffle tree), so a column of BVMatMult equals MatMult.
+// One row of the stream SpMM: nonzeros b..e-1 of (vs, cs) applied to NCT
+// columns, into TPR strided partial sums per column (nonzero p goes to sum
+// (p - b) % TPR, in order).  The nonzeros are taken TPR at a time: their
+// values and indices first (indices clamped to the row, so every load is in
+// bounds), then all their gathers of x (TPR * NCT = SPMM_NC loads in flight),
+// then the sums.  Batches of 2 or 4 times as many nonzeros measured slower at
+// 128^3 on the 27-point operator (more x in flight than the L2 holds).
+// Products are rounded before they are added, as in the stream SpMV.
+constexpr int SPMM_UNR = 1;
+
+template <typename T, int TPR, int NCT, typename VP, typename CP>
+__device__ __forceinline__ void spmm_row(VP vs, CP cs, int b, int e, const T *const (&xc)[NCT],
+                                         T (&acc)[NCT][TPR]) {
+#pragma clang fp contract(off)
+  constexpr int NB = TPR * SPMM_UNR;
+#pragma unroll
+  for (int u = 0; u < NCT; u++)
+#pragma unroll
+    for (int q = 0; q < TPR; q++) acc[u][q] = (T)0;
+  for (int p = b; p < e; p += NB) {
+    T v[NB], xv[NB][NCT];
+    int32_t c[NB];
+#pragma unroll
+    for (int k = 0; k < NB; k++) {
+      const int pk = min(p + k, e - 1);
+      v[k] = vs[pk];
+      c[k] = cs[pk];
+    }
+#pragma unroll
+    for (int k = 0; k < NB; k++)
+#pragma unroll
+      for (int u = 0; u < NCT; u++) xv[k][u] = xc[u][c[k]];
+#pragma unroll
+    for (int k = 0; k < NB; k++)
+      if (p + k < e)
+#pragma unroll
+        for (int u = 0; u < NCT; u++) acc[u][k % TPR] += v[k] * xv[k][u];
+  }
+}
+
+template <typename T, int RB>
+__global__ __launch_bounds__(BLK) void spmm_stream_kernel(int n, const int32_t *__restrict__ rp,
+                                                          const int32_t *__restrict__ ci, const T *__restrict__ val,
+                                                          const T *__restrict__ X, int64_t ldx, T *__restrict__ Y,
+                                                          int64_t ldy, int nc) {
+  constexpr int TPR = BLK / RB;
+  constexpr int NCT = SPMM_NC / TPR;   // columns per thread
+  static_assert(RB <= BLK && BLK % RB == 0 && SPMM_NC % TPR == 0, "row block");
+  __shared__ T sval[SPMV_NZB];
+  __shared__ int32_t scol[SPMV_NZB];
+  __shared__ int32_t srp[RB + 1];
+  const int t = threadIdx.x;
+  const int64_t r0 = (int64_t)xcd_block(blockIdx.x, gridDim.x) * RB;
+  const int rows = (int)min((int64_t)RB, (int64_t)n - r0);
+  for (int i = t; i <= rows; i += BLK) srp[i] = rp[r0 + i];
+  __syncthreads();
+  const int32_t p0 = srp[0], nz = srp[rows] - p0;
+  const bool staged = nz <= SPMV_NZB;    // uniform over the workgroup
+  if (staged)
+    for (int i = t; i < nz; i += BLK) {
+      sval[i] = __builtin_nontemporal_load(val + p0 + i);
+      scol[i] = __builtin_nontemporal_load(ci + p0 + i);
+    }
+  __syncthreads();
+  const int r = t % RB, jg = t / RB;
+  if (r >= rows || jg >= nc) return;
+  const int b = srp[r] - p0, e = srp[r + 1] - p0;
+  const T *xc[NCT];   // columns beyond nc repeat column nc - 1 (loaded, never stored)
+#pragma unroll
+  for (int u = 0; u < NCT; u++) xc[u] = X + (size_t)min(jg + u * TPR, nc - 1) * ldx;
+  T acc[NCT][TPR];
+  if (staged)
+    spmm_row<T, TPR, NCT>(sval, scol, b, e, xc, acc);
+  else
+    spmm_row<T, TPR, NCT>(val + p0, ci + p0, b, e, xc, acc);
+#pragma unroll
+  for (int u = 0; u < NCT; u++) {
+#pragma unroll
+    for (int h = TPR / 2; h > 0; h >>= 1)
+#pragma unroll
+      for (int q = 0; q < h; q++) acc[u][q] = acc[u][q] + acc[u][q + h];
+    if (jg + u * TPR < nc) Y[(size_t)(jg + u * TPR) * ldy + r0 + r] = acc[u][0];
+  }
+}
+
+template <typename T>
 static void spmm_t(const Mat &A, const T *X, int64_t ldx, T *Y, int64_t ldy, int nc, hipStream_t s) {
+  if (A.spmv_rb > 0) {   // operators larger than the caches: the stream layout
+    const int grid = (int)(((int64_t)A.n + A.spmv_rb - 1) / A.spmv_rb);
+    const T *val = static_cast<const T *>(A.val);
+    switch (A.spmv_rb) {
+      case 256: hipLaunchKernelGGL((spmm_stream_kernel<T, 256>), dim3(grid), dim3(BLK), 0, s, A.n, A.rowptr, A.col, val, X, ldx, Y, ldy, nc); break;
+      case 128: hipLaunchKernelGGL((spmm_stream_kernel<T, 128>), dim3(grid), dim3(BLK), 0, s, A.n, A.rowptr, A.col, val, X, ldx, Y, ldy, nc); break;
+      case 64: hipLaunchKernelGGL((spmm_stream_kernel<T, 64>), dim3(grid), dim3(BLK), 0, s, A.n, A.rowptr, A.col, val, X, ldx, Y, ldy, nc); break;
+      default: hipLaunchKernelGGL((spmm_stream_kernel<T, 32>), dim3(grid), dim3(BLK), 0, s, A.n, A.rowptr, A.col, val, X, ldx, Y, ldy, nc); break;
+    }
+    return;
+  }
   const int grid = (int)(((int64_t)A.n * A.lanes + BLK - 1) / BLK);
   const T *val = static_cast<const T *>(A.val);
   switch (A.lanes) {
-    case 4: hipLaunchKernelGGL((spmm_kernel<T, 4, NT>), dim3(grid), dim3(BLK), 0, s, A.n, A.rowptr, A.col, val, X, ldx, Y, ldy, nc); break;
-    case 8: hipLaunchKernelGGL((spmm_kernel<T, 8, NT>), dim3(grid), dim3(BLK), 0, s, A.n, A.rowptr, A.col, val, X, ldx, Y, ldy, nc); break;
-    case 16: hipLaunchKernelGGL((spmm_kernel<T, 16, NT>), dim3(grid), dim3(BLK), 0, s, A.n, A.rowptr, A.col, val, X, ldx, Y, ldy, nc); break;
-    default: hipLaunchKernelGGL((spmm_kernel<T, 32, NT>), dim3(grid), dim3(BLK), 0, s, A.n, A.rowptr, A.col, val, X, ldx, Y, ldy, nc); break;
+    case 4: hipLaunchKernelGGL((spmm_kernel<T, 4>), dim3(grid), dim3(BLK), 0, s, A.n, A.rowptr, A.col, val, X, ldx, Y, ldy, nc); break;
+    case 8: hipLaunchKernelGGL((spmm_kernel<T, 8>), dim3(grid), dim3(BLK), 0, s, A.n, A.rowptr, A.col, val, X, ldx, Y, ldy, nc); break;
+    case 16: hipLaunchKernelGGL((spmm_kernel<T, 16>), dim3(grid), dim3(BLK), 0, s, A.n, A.rowptr, A.col, val, X, ldx, Y, ldy, nc); break;
+    default: hipLaunchKernelGGL((spmm_kernel<T, 32>), dim3(grid), dim3(BLK), 0, s, A.n, A.rowptr, A.col, val, X, ldx, Y, ldy, nc); break;
   }
 }
 
@@ -224,19 +331,14 @@ int launch_spmm(const Mat &A, const BV &X, int lx, const BV &Y, int ly, int ncol
   if (A.n == 0) return 0;
   for (int j0 = 0; j0 < ncols; j0 += SPMM_NC) {
     const int nc = std::min(SPMM_NC, ncols - j0);
-    // an operator larger than the caches (spmv_rb set) is read once per pass:
-    // non-temporal value/index loads keep the x columns cached
-    const bool nt = A.spmv_rb > 0;
     if (X.prec == SA_FP64) {
       const double *xp = static_cast<const double *>(X.data) + (size_t)(lx + j0) * X.ld;
       double *yp = static_cast<double *>(Y.data) + (size_t)(ly + j0) * Y.ld;
-      if (nt) spmm_t<double, true>(A, xp, X.ld, yp, Y.ld, nc, s);
-      else spmm_t<double, false>(A, xp, X.ld, yp, Y.ld, nc, s);
+      spmm_t<double>(A, xp, X.ld, yp, Y.ld, nc, s);
     } else {
       const float *xp = static_cast<const float *>(X.data) + (size_t)(lx + j0) * X.ld;
       float *yp = static_cast<float *>(Y.data) + (size_t)(ly + j0) * Y.ld;
-      if (nt) spmm_t<float, true>(A, xp, X.ld, yp, Y.ld, nc, s);
-      else spmm_t<float, false>(A, xp, X.ld, yp, Y.ld, nc, s);
+      spmm_t<float>(A, xp, X.ld, yp, Y.ld, nc, s);
     }
   }
   SA_HIP(hipGetLastError());

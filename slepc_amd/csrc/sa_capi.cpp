@@ -56,8 +56,9 @@ int spmv_pick_rows(const int32_t *rp, int64_t n) {
   return best;
 }
 
-// Value/index bytes above which an operator uses the stream SpMV.
-constexpr int64_t SPMV_STREAM_BYTES = 64ll << 20;
+// Value/index bytes above which an operator uses the stream SpMV (settable
+// with sa_set_spmv_stream_bytes).
+std::atomic<int64_t> g_stream_bytes{64ll << 20};
 
 }  // namespace sa
 
@@ -171,7 +172,7 @@ int sa_mat_create_csr(int n, int64_t nnz, const int32_t *rp, const int32_t *ci, 
   // operators larger than the caches: the stream SpMV (small ones keep the
   // vector-CSR kernel, whose many short workgroups win when the operator is
   // cache-resident)
-  A.spmv_rb = nnz * (int64_t)(es + 4) > SPMV_STREAM_BYTES ? spmv_pick_rows(rp, n) : 0;
+  A.spmv_rb = nnz * (int64_t)(es + 4) > g_stream_bytes.load() ? spmv_pick_rows(rp, n) : 0;
   hipStream_t s = stream();
   if (hipMalloc(&A.rowptr, sizeof(int32_t) * (n + 1)) != hipSuccess ||
       hipMalloc(&A.col, sizeof(int32_t) * std::max<int64_t>(1, nnz)) != hipSuccess ||
@@ -195,6 +196,8 @@ int sa_mat_create_csr(int n, int64_t nnz, const int32_t *rp, const int32_t *ci, 
   *out = M;
   return 0;
 }
+
+int64_t sa_set_spmv_stream_bytes(int64_t bytes) { return g_stream_bytes.exchange(std::max<int64_t>(0, bytes)); }
 
 int sa_mat_destroy(sa_mat M) {
   if (!M) return 0;

@@ -12,6 +12,27 @@ constexpr int RBLK = 1024;      // threads of the one-workgroup reductions
 
 __device__ __forceinline__ double safe_sqrt(double a) { return a < 0.0 ? 0.0 : sqrt(a); }
 
+// A load that is non-temporal when NT (data read once, larger than the
+// caches: streaming it past the last-level cache keeps the caches for reused
+// data and reads faster) and ordinary otherwise.
+template <bool NT, typename T>
+__device__ __forceinline__ T ld_stream(const T *p) {
+  if constexpr (NT) return __builtin_nontemporal_load(p);
+  else return *p;
+}
+
+// The dispatcher hands workgroups to the 8 XCDs round-robin by index, so
+// consecutive blocks land on different L2s.  xcd_block maps the dispatch
+// index b of a G-block grid to a logical block such that each XCD takes one
+// contiguous range of about G/8 blocks (a bijection for any G): row-blocked
+// kernels whose rows share neighbours (stencil planes) then find them in
+// their own XCD's L2.
+constexpr int N_XCD = 8;
+__device__ __forceinline__ int xcd_block(int b, int G) {
+  const int q = G / N_XCD, r = G % N_XCD, x = b % N_XCD, i = b / N_XCD;
+  return x * q + min(x, r) + i;
+}
+
 // out[col] = sum_b part[col * G + b] for col < ncol.  Each wave reduces four
 // columns at a time (columns w, w+nw, w+2nw, w+3nw) with four loads per column
 // in flight, then a shuffle reduction; the order of the additions is fixed, so

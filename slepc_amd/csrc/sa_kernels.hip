@@ -44,13 +44,44 @@ __global__ __launch_bounds__(BLK) void spmv_kernel(int n, const int32_t *__restr
 
 // Stream CSR SpMV for operators larger than the caches: RB rows per
 // workgroup.  The workgroup's nonzeros (a contiguous range of val/col) are
-// loaded with every lane active, each thread issuing its SPMV_PT value/index
-// loads before the dependent gathers of x; the products meet in LDS and
-// BLK/RB threads per row add them (a strided partial sum per thread, then a
-// shuffle tree).  Values and indices are read once, so they are loaded
-// non-temporally and do not evict x from the caches.  A block whose nonzeros
-// do not fit the LDS buffer is computed straight from memory instead, so the
-// kernel is correct for any CSR; the host picks RB so that every block fits.
+// loaded with every lane active (SPMV_PT value/index loads per thread) and
+// staged in LDS.  Thread t then forms the strided partial sum q = t / RB of
+// row t % RB (TPR = BLK / RB partials per row: nonzeros q, q + TPR, ... of
+// the row, in order), so a wave covers 64 consecutive rows and its gathers
+// of x (for a stencil, the same neighbour of consecutive rows) coalesce.  The
+// partials are added in a fixed tree order through LDS.  Values and indices
+// are read once, so they are loaded non-temporally and do not evict x from
+// the caches.  A block whose nonzeros do not fit the LDS buffer is computed
+// straight from memory instead, so the kernel is correct for any CSR; the
+// host picks RB so that every block fits.
+
+// Partial sum over nonzeros b, b + TPR, ... < e of (vs, cs): four at a time,
+// their indices first (clamped into the row, so every load is in bounds),
+// then their gathers of x, then the additions in order.  Products are rounded
+// before they are added (no fused multiply-add), as the stream SpMM forms them.
+template <typename T, int TPR, typename VP, typename CP>
+__device__ __forceinline__ T spmv_row_part(VP vs, CP cs, int b, int e, const T *__restrict__ x) {
+#pragma clang fp contract(off)
+  constexpr int U = 4;
+  T s = (T)0;
+  for (int p = b; p < e; p += U * TPR) {
+    T v[U], xv[U];
+    int32_t c[U];
+#pragma unroll
+    for (int u = 0; u < U; u++) {
+      const int pk = min(p + u * TPR, e - 1);
+      v[u] = vs[pk];
+      c[u] = cs[pk];
+    }
+#pragma unroll
+    for (int u = 0; u < U; u++) xv[u] = x[c[u]];
+#pragma unroll
+    for (int u = 0; u < U; u++)
+      if (p + u * TPR < e) s += v[u] * xv[u];
+  }
+  return s;
+}
+
 template <typename T, int RB>
 __global__ __launch_bounds__(BLK) void spmv_stream_kernel(int n, const int32_t *__restrict__ rp,
                                                           const int32_t *__restrict__ ci,
@@ -60,17 +91,18 @@ __global__ __launch_bounds__(BLK) void spmv_stream_kernel(int n, const int32_t *
   static_assert(RB <= BLK && BLK % RB == 0, "rows per block must divide the workgroup");
   constexpr int NZB = BLK * SPMV_PT;
   static_assert(NZB == SPMV_NZB, "block capacity differs from the host's");
-  __shared__ T prod[NZB];
+  __shared__ T sval[NZB];
+  __shared__ int32_t scol[NZB];
   __shared__ int32_t srp[RB + 1];
+  __shared__ T part[TPR > 1 ? BLK : 1];
   const int t = threadIdx.x;
   const int64_t r0 = (int64_t)blockIdx.x * RB;
   const int rows = (int)min((int64_t)RB, (int64_t)n - r0);
   for (int i = t; i <= rows; i += BLK) srp[i] = rp[r0 + i];
   __syncthreads();
   const int32_t p0 = srp[0], nz = srp[rows] - p0;
-  const int r = t / TPR, q = t % TPR;
-  T s = (T)0;
-  if (nz <= NZB) {                       // uniform across the workgroup
+  const bool staged = nz <= NZB;         // uniform across the workgroup
+  if (staged) {
     T v[SPMV_PT];
     int32_t c[SPMV_PT];
 #pragma unroll
@@ -84,17 +116,35 @@ __global__ __launch_bounds__(BLK) void spmv_stream_kernel(int n, const int32_t *
 #pragma unroll
     for (int k = 0; k < SPMV_PT; k++) {
       const int i = k * BLK + t;
-      if (i < nz) prod[i] = v[k] * x[c[k]];
+      if (i < nz) {
+        sval[i] = v[k];
+        scol[i] = c[k];
+      }
     }
-    __syncthreads();
-    if (r < rows)
-      for (int p = srp[r] - p0 + q; p < srp[r + 1] - p0; p += TPR) s += prod[p];
-  } else if (r < rows) {
-    for (int32_t p = srp[r] + q; p < srp[r + 1]; p += TPR) s += val[p] * x[ci[p]];
   }
+  __syncthreads();
+  const int r = t % RB, q = t / RB;
+  T s = (T)0;
+  if (r < rows) {
+    const int b = srp[r] - p0 + q, e = srp[r + 1] - p0;
+    s = staged ? spmv_row_part<T, TPR>(sval, scol, b, e, x) : spmv_row_part<T, TPR>(val + p0, ci + p0, b, e, x);
+  }
+  if constexpr (TPR == 1) {
+    if (r < rows) y[r0 + r] = s;
+  } else {
+    part[t] = s;                         // part[q * RB + r]
+    __syncthreads();
+    if (q == 0 && r < rows) {
+      T a[TPR];
 #pragma unroll
-  for (int off = TPR / 2; off > 0; off >>= 1) s += __shfl_xor(s, off, TPR);
-  if (q == 0 && r < rows) y[r0 + r] = s;
+      for (int u = 0; u < TPR; u++) a[u] = part[u * RB + r];
+#pragma unroll
+      for (int h = TPR / 2; h > 0; h >>= 1)
+#pragma unroll
+        for (int u = 0; u < h; u++) a[u] = a[u] + a[u + h];
+      y[r0 + r] = a[0];
+    }
+  }
 }
 
 template <typename T>
@@ -283,12 +333,15 @@ __global__ __launch_bounds__(BLK) void multinplace_kernel(int64_t n, int64_t ld,
 // of a lane is column j = 4g+v (f32) or g+4v (f64) of the tile, g = lane/16,
 // and row i = lane%16, so a store writes 128-byte runs of a column.  Every
 // wave reads all K basis values of its rows before it writes them, and no
-// other wave touches those rows, so the product is safe in place.
+// other wave touches those rows, so the product is safe in place.  A basis
+// larger than the last-level cache (NT) is read with non-temporal loads, each
+// value being read once (1.2x faster at K = 32, n = 2.1M; at K = 96 the FP64
+// MFMA work, 2 K N flops per row, is as long as the memory time).
 constexpr int MIP_KB = 8;
 constexpr size_t MIP_QS_BYTES = 64 * 1024;
 constexpr int64_t MIP_GRID = 1024;             // workgroups (each stages Q once)
 
-template <typename T, int NTL>
+template <typename T, int NTL, bool NT>
 __global__ __launch_bounds__(BLK) void multinplace_q_kernel(int64_t n, int64_t ld, T *V, int l, int K, int s,
                                                             int N, const double *__restrict__ Q, int ldq) {
   typedef typename Vec4<T>::type V4;
@@ -317,14 +370,14 @@ __global__ __launch_bounds__(BLK) void multinplace_q_kernel(int64_t n, int64_t l
 #pragma unroll
     for (int u = 0; u < MIP_KB; u++) {
       const int kk = 4 * u + g;
-      av[u] = (rin && kk < K) ? src[(size_t)kk * ld] : (T)0;
+      av[u] = (rin && kk < K) ? ld_stream<NT>(src + (size_t)kk * ld) : (T)0;
     }
     for (int kb = 0; kb < nbatch; kb++) {
       if (kb + 1 < nbatch) {
 #pragma unroll
         for (int u = 0; u < MIP_KB; u++) {
           const int kk = 4 * ((kb + 1) * MIP_KB + u) + g;
-          an[u] = (rin && kk < K) ? src[(size_t)kk * ld] : (T)0;
+          an[u] = (rin && kk < K) ? ld_stream<NT>(src + (size_t)kk * ld) : (T)0;
         }
       }
 #pragma unroll
@@ -357,8 +410,12 @@ static void launch_mip_q(BV &V, int l, int K, int s_, int N, const double *Q_dev
   const int64_t nblk = (V.n + CH - 1) / CH;
   const int grid = (int)std::min<int64_t>(nblk, MIP_GRID);
   const size_t shm = (size_t)((K + 3) & ~3) * (NTL * 16 + 1) * sizeof(T);
-  hipLaunchKernelGGL((multinplace_q_kernel<T, NTL>), dim3(grid), dim3(BLK), shm, s, V.n, V.ld, (T *)V.data, l, K,
-                     s_, N, Q_dev, ldq);
+  if (V.n * (int64_t)K * (int64_t)sizeof(T) > GS_NT_BYTES)
+    hipLaunchKernelGGL((multinplace_q_kernel<T, NTL, true>), dim3(grid), dim3(BLK), shm, s, V.n, V.ld, (T *)V.data,
+                       l, K, s_, N, Q_dev, ldq);
+  else
+    hipLaunchKernelGGL((multinplace_q_kernel<T, NTL, false>), dim3(grid), dim3(BLK), shm, s, V.n, V.ld, (T *)V.data,
+                       l, K, s_, N, Q_dev, ldq);
 }
 
 template <typename T>

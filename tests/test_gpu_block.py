@@ -249,3 +249,69 @@ def test_bv_matmult_stencil27(sa, orc, prec):
     X.matmult(A, Y)
     ref = orc.bv_matmult(Ao, Xh)
     assert np.array_equal(Y.get_matrix(11), ref)
+
+
+def _random_csr(sa, n, lo, hi, seed):
+    """n rows with lo..hi nonzeros each (ragged), random ascending columns
+    (random gaps from a random start) and random values."""
+    rng = np.random.default_rng(seed)
+    lens = rng.integers(lo, hi + 1, n)
+    rp = np.zeros(n + 1, np.int64)
+    rp[1:] = np.cumsum(lens)
+    nnz = int(rp[-1])
+    gap = 1000
+    steps = np.cumsum(rng.integers(1, gap, nnz))
+    row = np.repeat(np.arange(n), lens)
+    within = steps - np.repeat(steps[rp[:-1]], lens)          # 0 at each row's first entry
+    start = rng.integers(0, n - hi * gap, n)
+    col = (start[row] + within).astype(np.int32)
+    rp = rp.astype(np.int32)
+    val = rng.standard_normal(nnz)
+    return sa.CSR(n, rp, col, val)
+
+
+@pytest.mark.parametrize("case", ["stencil27_80_fp64", "stencil27_80_fp32", "convdiff_96_fp64",
+                                  "ragged_12_fp64", "ragged_40_fp64"])
+def test_bv_matmult_stream_layout(sa, case):
+    """Operators whose values and indices exceed 64 MiB take the stream SpMM
+    (row blocks of 256, 128, 64 and 32 rows staged in LDS: 7-point, 9-16,
+    27-point and 33-48 nonzeros per row).  On real-valued data each of 11
+    columns (passes of 8 and 3) equals MatMult of that column bit for bit,
+    since both form a row's sum in the same order."""
+    prec = 1 if case.endswith("fp32") else 0
+    if case.startswith("stencil27"):
+        Ah = sa.stencil27(80, seed=3)
+    elif case.startswith("convdiff"):
+        Ah = sa.convdiff3d(96)
+    elif case.startswith("ragged_12"):
+        Ah = _random_csr(sa, 700_000, 9, 16, 5)
+    else:
+        Ah = _random_csr(sa, 200_000, 33, 48, 6)
+    es = 8 if prec == 0 else 4
+    assert Ah.nnz * (es + 4) > 64 << 20
+    n, nc = Ah.n, 11
+    A = sa.Mat(Ah, prec)
+    Xh = np.random.default_rng(7).standard_normal((n, nc))
+    X = _bv(sa, Xh, prec)
+    Y = sa.BV(n, nc, prec)
+    X.matmult(A, Y)
+    out = Y.get_matrix(nc)
+    dt = np.float64 if prec == 0 else np.float32
+    for j in range(nc):
+        assert np.array_equal(out[:, j], A.mult_host(Xh[:, j].astype(dt))), j
+
+
+def test_bv_matmult_stream_layout_oracle(sa, orc):
+    """The stream SpMM on the 27-point operator at 80^3 with integer-valued
+    diagonal and columns (every sum exact): equal to the oracle's BVMatMult."""
+    Ah = sa.stencil27(80, seed=1)
+    Ao = orc.stencil27(80, seed=1)
+    Ah.val = np.round(Ah.val)
+    Ao.val = np.round(Ao.val)
+    n = Ah.n
+    Xh = (np.arange(n * 9).reshape(n, 9, order="F") % 7 - 3).astype(np.float64)
+    A = sa.Mat(Ah)
+    X = _bv(sa, Xh)
+    Y = sa.BV(n, 9, 0)
+    X.matmult(A, Y)
+    assert np.array_equal(Y.get_matrix(9), orc.bv_matmult(Ao, Xh))

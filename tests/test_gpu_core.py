@@ -49,25 +49,75 @@ def _ragged(sa, n, lens, seed):
     return sa.CSR(n, rp, col, val)
 
 
+def _ragged_fast(sa, n, lens, seed):
+    """As _ragged for large n: ascending columns from random gaps after a
+    random start (vectorised)."""
+    rng = np.random.default_rng(seed)
+    rp = np.zeros(n + 1, np.int64)
+    rp[1:] = np.cumsum(lens)
+    nnz = int(rp[-1])
+    gap = max(2, n // (4 * int(max(lens))))
+    steps = np.cumsum(rng.integers(1, gap, nnz))
+    within = steps - np.repeat(steps[rp[:-1]], lens)
+    start = rng.integers(0, max(1, n - int(max(lens)) * gap), n)
+    col = (np.repeat(start, lens) + within).astype(np.int32)
+    val = rng.integers(-3, 4, size=nnz).astype(np.float64)
+    return sa.CSR(n, rp.astype(np.int32), col, val)
+
+
+def _stream_rows(rp, n):
+    """The row block the library picks for the stream SpMV (256, 128, 64 or
+    32 rows of at most 2,048 nonzeros, the largest giving 1,024 blocks; 0 when
+    none fits)."""
+    best = 0
+    for rb in (256, 128, 64, 32):
+        idx = np.minimum(np.arange(0, n + rb, rb), n)
+        if np.max(rp[idx[1:]] - rp[idx[:-1]]) > 2048:
+            continue
+        best = rb
+        if (n + rb - 1) // rb >= 1024:
+            break
+    return best
+
+
+@pytest.fixture
+def stream_layout(sa):
+    """Every operator created in the test takes the stream layout when a row
+    block fits (value/index threshold 0)."""
+    old = sa.set_spmv_stream_bytes(0)
+    yield
+    sa.set_spmv_stream_bytes(old)
+
+
 # Row-length patterns that select each block size of the stream SpMV (at most
 # 2,048 nonzeros per block of 256/128/64/32 rows) and the vector-CSR kernel
 # (no block size fits), with empty rows and sizes not divisible by the block.
-@pytest.mark.parametrize("case", ["rows256", "rows128", "rows64", "rows32", "vector", "one_long_row"])
+# "small" takes 32-row blocks (too few rows for larger ones to fill the GPU);
+# "large" takes the block size named, over thousands of blocks.  Integer
+# data: every sum is exact.
+STREAM_CASES = {"rows256": ((0, 8), 1001, 2_000_003, 256), "rows128": ((9, 15), 777, 600_001, 128),
+                "rows64": ((20, 30), 555, 300_007, 64), "rows32": ((40, 60), 333, 150_001, 32),
+                "vector": ((100, 120), 301, 301, 0), "one_long_row": ((0, 5), 2500, 2500, 0)}
+
+
+@pytest.mark.parametrize("size", ["small", "large"])
+@pytest.mark.parametrize("case", list(STREAM_CASES))
 @pytest.mark.parametrize("prec", ["fp64", "fp32"])
-def test_spmv_row_blocks_exact(sa, case, prec):
+def test_spmv_row_blocks_exact(sa, stream_layout, case, prec, size):
+    (lo, hi), n_small, n_large, rb_large = STREAM_CASES[case]
+    if size == "large" and n_large == n_small:
+        pytest.skip("the vector-CSR cases have one size")
+    n = n_small if size == "small" else n_large
     rng = np.random.default_rng(11)
-    n = {"rows256": 1001, "rows128": 777, "rows64": 555, "rows32": 333, "vector": 301, "one_long_row": 1500}[case]
-    lo, hi = {"rows256": (0, 8), "rows128": (9, 15), "rows64": (20, 30), "rows32": (40, 60), "vector": (100, 120),
-              "one_long_row": (0, 5)}[case]
     lens = rng.integers(lo, hi + 1, size=n)
     if case == "one_long_row":
-        lens[700] = 1400
-    Ah = _ragged(sa, n, lens, 5)
+        lens[700] = 2100                  # no block of any size holds this row
+    Ah = _ragged(sa, n, lens, 5) if n < 5000 else _ragged_fast(sa, n, lens, 5)
+    rb = _stream_rows(Ah.rowptr, n)
+    assert rb == (rb_large if size == "large" else (32 if rb_large else 0))
     x = (np.arange(n) % 7 - 3).astype(np.float64)
-    dense = np.zeros((n, n))
-    for r in range(n):
-        dense[r, Ah.col[Ah.rowptr[r]:Ah.rowptr[r + 1]]] = Ah.val[Ah.rowptr[r]:Ah.rowptr[r + 1]]
-    ref = dense @ x
+    row = np.repeat(np.arange(n), lens)
+    ref = np.bincount(row, weights=Ah.val * x[Ah.col], minlength=n)
     M = sa.Mat(Ah, sa.FP64 if prec == "fp64" else sa.FP32)
     y = M.mult_host(x if prec == "fp64" else x.astype(np.float32)).astype(np.float64)
     assert np.array_equal(y, ref)
@@ -192,9 +242,12 @@ def test_arnoldi_relation(sa, orc):
 
 # Shapes: ragged n/K/N in place (Q staged in LDS); more 64-row blocks than
 # workgroups (grid-stride loop); 64x64 (4 output tiles); a Q slice too large
-# for LDS (the tile-in-LDS kernel); K = 3, N = 1.
+# for LDS (the tile-in-LDS kernel); K = 3, N = 1; several blocks per
+# workgroup of two k batches each (the load pipeline crosses blocks); a
+# basis above 256 MB in FP64 (non-temporal loads).
 @pytest.mark.parametrize("shape", [(5000, 41, 3, 37, 5, 29), (70001, 20, 0, 20, 0, 10), (1000, 70, 0, 64, 0, 64),
-                                   (300, 127, 2, 122, 10, 110), (777, 10, 0, 3, 0, 1)])
+                                   (300, 127, 2, 122, 10, 110), (777, 10, 0, 3, 0, 1),
+                                   (300001, 45, 2, 42, 3, 23), (2_100_001, 21, 0, 21, 0, 9)])
 @pytest.mark.parametrize("prec", [0, 1])
 def test_multinplace_matches_oracle(sa, orc, prec, shape):
     n, m, l, k, s0, e0 = shape
