@@ -241,6 +241,7 @@ def run_family(sa, probs, v0, net, method, streams, warmup):
         st = E.getStats()
         return dict(good=good, worst=worst, dt=dt, mv=st["matvecs"] + st["matvecs32"], mv32=st["matvecs32"])
 
+    prev = sa.set_concurrent_solves(streams)   # partition hint for the bases of the concurrent solves
     for i in range(warmup):
         job(i)
     idxs = list(range(warmup, len(probs)))
@@ -251,6 +252,7 @@ def run_family(sa, probs, v0, net, method, streams, warmup):
         with cf.ThreadPoolExecutor(max_workers=streams) as ex:
             out = list(ex.map(job, idxs))
     wall = time.perf_counter() - t0
+    sa.set_concurrent_solves(prev)
     pairs = sum(o["good"] for o in out)
     r = dict(eps=pairs / wall, wall=wall, pairs=pairs, worst=max(o["worst"] for o in out),
              matvecs=[o["mv"] for o in out], matvecs32=[o["mv32"] for o in out],

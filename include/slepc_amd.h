@@ -63,6 +63,15 @@ int sa_mat_create_csr(int n, int64_t nnz, const int32_t *rowptr_host, const int3
  * choice of storage format, as a MatType is); tests use 0 to run the stream
  * kernels on small operators. */
 int64_t sa_set_spmv_stream_bytes(int64_t bytes);
+/* Partition hint: the number of solves the application keeps in flight at
+ * once on the device (one host thread each).  With 2 or more, bases that fit
+ * the last-level cache (<= 256 MiB) created afterwards give each
+ * Gram-Schmidt workgroup two 64-row chunks instead of one, so the slot
+ * kernels of concurrent solves share the CUs.  Default 1; returns the
+ * previous value.  No counterpart in the reference (PETSc runs one solve per
+ * process); results differ from a solve alone only in the rounding of the
+ * reductions. */
+int sa_set_concurrent_solves(int nsolves);
 int sa_mat_destroy(sa_mat A);
 int sa_mat_get_info(sa_mat A, int *n, int64_t *nnz, int *prec);
 /* y = A x with x, y device vectors of the matrix precision (MatMult). */
@@ -366,6 +375,32 @@ int sa_eps_solve(sa_eps eps);
 int sa_eps_get_converged(sa_eps eps, int *nconv);
 int sa_eps_get_iteration_number(sa_eps eps, int *its);
 int sa_eps_get_converged_reason(sa_eps eps, int *reason);
+/* Converged reasons (EPSConvergedReason, include/slepceps.h:179-185). */
+enum { SA_EPS_CONVERGED_TOL = 1, SA_EPS_CONVERGED_USER = 2, SA_EPS_DIVERGED_ITS = -1,
+       SA_EPS_DIVERGED_BREAKDOWN = -2, SA_EPS_CONVERGED_ITERATING = 0 };
+/* EPSMonitorSet (src/eps/interface/epsmon.c:96-122): adds a function called
+ * at the end of every outer iteration (EPSMonitor, epsmon.c:39-46, called at
+ * krylovschur.c:344 and the matching places of the other solvers) with the
+ * iteration number, the number of converged pairs, the Ritz values of the
+ * projected problem (eigr, eigi; with shift-and-invert these are theta, and
+ * lambda = sigma + 1/theta, as the reference's monitors back-transform them),
+ * the error estimates and their count nest.  At most 5 monitors; they run in
+ * the order set.  A nonzero return ends the solve with an error.  With mixed
+ * precision the monitors are called in both phases (the iteration count
+ * restarts at 1 in the FP64 phase).  EPSMonitorCancel: removes them all. */
+typedef int (*sa_eps_monitor_fn)(sa_eps eps, int its, int nconv, const double *eigr, const double *eigi,
+                                 const double *errest, int nest, void *ctx);
+int sa_eps_monitor_set(sa_eps eps, sa_eps_monitor_fn monitor, void *ctx);
+int sa_eps_monitor_cancel(sa_eps eps);
+/* EPSSetStoppingTestFunction (src/eps/interface/epsopts.c:863-885): replaces
+ * EPSStoppingBasic (src/eps/interface/epsdefault.c:290-302).  Called once per
+ * outer iteration after the convergence test (krylovschur.c:284) with the
+ * iteration number, max_it, the number of converged pairs and nev; it sets
+ * *reason (SA_EPS_CONVERGED_ITERATING to continue, any other value ends the
+ * iteration and becomes the converged reason).  A nonzero return ends the
+ * solve with an error.  NULL restores the basic test. */
+typedef int (*sa_eps_stopping_fn)(sa_eps eps, int its, int max_it, int nconv, int nev, int *reason, void *ctx);
+int sa_eps_set_stopping_test_function(sa_eps eps, sa_eps_stopping_fn stopping, void *ctx);
 /* Eigenpair i in the order EPSGetEigenpair returns them; vectors may be NULL. */
 int sa_eps_get_eigenpair(sa_eps eps, int i, double *re, double *im, double *xr_host, double *xi_host);
 int sa_eps_get_error_estimate(sa_eps eps, int i, double *errest);

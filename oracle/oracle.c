@@ -1665,6 +1665,7 @@ int orc_eps_solve(int n, const int32_t *rp, const int32_t *ci, const double *val
       for (int i = nconv; i < nv; i++) errest[i] = bnd[i];
       memcpy(eigr, lockd, sizeof(double) * (size_t)nconv);
       nconv = k;
+      if (res->trace_nconv && its <= res->trace_cap) { res->trace_nconv[its - 1] = k; res->trace_eig0[its - 1] = eigr[0]; }
       if (k >= nev) reason = 1;
       else if (its >= max_it) reason = -1;
       if (reason == 0) {                                     /* BVCopyColumn(V,nv,nconv) (:826) */
@@ -1715,6 +1716,7 @@ int orc_eps_solve(int n, const int32_t *rp, const int32_t *ci, const double *val
          (k+1)-th Schur vector is the next start vector */
       int e = k + 1 < nv ? k + 1 : nv;
       orc_bv_multinplace(n, ks.V, ks.ld, nconv, nv, nconv, e, ds.Q, ld);
+      if (res->trace_nconv && its <= res->trace_cap) { res->trace_nconv[its - 1] = k; res->trace_eig0[its - 1] = eigr[0]; }
       if (k >= nev) reason = 1;
       else if (its >= max_it) reason = -1;
       if (reason == 0 && breakdown) {                          /* arnoldi.c:118-125 */
@@ -1726,6 +1728,7 @@ int orc_eps_solve(int n, const int32_t *rp, const int32_t *ci, const double *val
       continue;
     }
     /* EPSStoppingBasic (epsdefault.c:290-302) */
+    if (res->trace_nconv && its <= res->trace_cap) { res->trace_nconv[its - 1] = k; res->trace_eig0[its - 1] = eigr[0]; }
     if (k >= nev) reason = 1;
     else if (its >= max_it) reason = -1;
     int newl;

@@ -100,6 +100,12 @@ typedef struct {
   int     nvec;                 /* number of vectors to return */
   int64_t matvecs, orth_passes; /* counters */
   double  seconds;              /* wall time of the solve loop */
+  /* per-iteration trace for the monitor comparison (EPSMonitor, epsmon.c:39-46),
+     optional: entry its-1 receives the converged count the stopping test saw
+     and the first Ritz value of that iteration, up to trace_cap iterations */
+  int     *trace_nconv;
+  double  *trace_eig0;
+  int     trace_cap;
 } orc_result;
 
 int orc_lanczos_local_sweep(int n, const int32_t *rp, const int32_t *ci, const double *val, int m, const double *v0,

@@ -34,7 +34,8 @@ class Result(C.Structure):
                 ("eigi", C.POINTER(C.c_double)), ("errest", C.POINTER(C.c_double)),
                 ("vecs", C.POINTER(C.c_double)), ("nvec", C.c_int),
                 ("matvecs", C.c_int64), ("orth_passes", C.c_int64),
-                ("seconds", C.c_double)]
+                ("seconds", C.c_double), ("trace_nconv", C.POINTER(C.c_int)),
+                ("trace_eig0", C.POINTER(C.c_double)), ("trace_cap", C.c_int)]
 
 
 def build():
@@ -281,7 +282,7 @@ def lanczos_local_sweep(A, m, v0):
 def eps_solve(A, nev, ncv=0, tol=1e-8, which="LM", hermitian=True, v0=None, max_it=0,
               mpd=0, nvec=None, nthreads=0, keep=0.5, lock=1, solver="krylovschur",
               extraction="ritz", target=0.0, twosided=False, st="shift", sigma=None,
-              reorthog="full", balance="none", trueres=False):
+              reorthog="full", balance="none", trueres=False, trace=0):
     o = Opts(hermitian=1 if hermitian else 0, which=WHICH[which], nev=nev, ncv=ncv, mpd=mpd,
              extraction={"ritz": 0, "harmonic": 1}[extraction], target=float(target),
              st={"shift": 0, "sinvert": 1}[st], sigma=float(target if sigma is None else sigma),
@@ -305,6 +306,12 @@ def eps_solve(A, nev, ncv=0, tol=1e-8, which="LM", hermitian=True, v0=None, max_
     r.errest = err.ctypes.data_as(C.POINTER(C.c_double))
     r.vecs = vecs.ctypes.data_as(C.POINTER(C.c_double)) if nvec else None
     r.nvec = nvec
+    tr_n = np.zeros(max(trace, 1), np.int32)
+    tr_e = np.zeros(max(trace, 1))
+    if trace:   # per-iteration (converged count, first Ritz value), the monitor's view
+        r.trace_nconv = tr_n.ctypes.data_as(C.POINTER(C.c_int))
+        r.trace_eig0 = tr_e.ctypes.data_as(C.POINTER(C.c_double))
+        r.trace_cap = trace
     v0p = None
     if v0 is not None:
         v0 = np.ascontiguousarray(v0, np.float64)
@@ -322,6 +329,10 @@ def eps_solve(A, nev, ncv=0, tol=1e-8, which="LM", hermitian=True, v0=None, max_
     out = dict(nconv=nc, its=r.its, reason=r.reason, eigr=eigr[:nc].copy(), eigi=eigi[:nc].copy(),
                errest=err[:nc].copy(), vecs=X, matvecs=r.matvecs, passes=r.orth_passes,
                seconds=r.seconds)
+    if trace:
+        m = min(r.its, trace)
+        out["trace_nconv"] = tr_n[:m].copy()
+        out["trace_eig0"] = tr_e[:m].copy()
     if twosided:
         out["lvecs"] = lvecs[:A.n * min(nc, nvec)].reshape((min(nc, nvec), A.n)).T.copy() if nvec else None
     return out

@@ -183,6 +183,13 @@ def set_spmv_stream_bytes(nbytes):
     return int(lib().sa_set_spmv_stream_bytes(int(nbytes)))
 
 
+def set_concurrent_solves(nsolves):
+    """Partition hint: number of solves kept in flight at once (one host
+    thread each); with 2 or more, cache-resident bases created afterwards use
+    half as many Gram-Schmidt workgroups.  Returns the previous value."""
+    return int(lib().sa_set_concurrent_solves(int(nsolves)))
+
+
 class Mat:
     """CSR operator in HBM (MATAIJ analogue); MatMult runs the HIP SpMV."""
 
@@ -357,6 +364,11 @@ class BV:
             pass
 
 
+# EPSConvergedReason (include/slepceps.h:179-185)
+REASON_CONVERGED_TOL, REASON_CONVERGED_USER, REASON_ITERATING = 1, 2, 0
+REASON_DIVERGED_ITS, REASON_DIVERGED_BREAKDOWN = -1, -2
+
+
 class EPS:
     """Krylov-Schur eigensolver (EPSKRYLOVSCHUR default variant)."""
 
@@ -501,6 +513,49 @@ class EPS:
         v = C.c_int()
         check(lib().sa_eps_get_converged_reason(self.h, C.byref(v)))
         return v.value
+
+    # EPSMonitorSet / EPSMonitorCancel (epsmon.c:96-135): fn(its, nconv, eigr,
+    # eigi, errest, nest) with numpy copies of the first nest values; return
+    # None or 0 to continue, anything else ends the solve with an error.
+    _MONITOR = C.CFUNCTYPE(C.c_int, C.c_void_p, C.c_int, C.c_int, C.POINTER(C.c_double), C.POINTER(C.c_double),
+                           C.POINTER(C.c_double), C.c_int, C.c_void_p)
+    _STOPPING = C.CFUNCTYPE(C.c_int, C.c_void_p, C.c_int, C.c_int, C.c_int, C.c_int, C.POINTER(C.c_int),
+                            C.c_void_p)
+
+    def setMonitor(self, fn):
+        def tramp(_eps, its, nconv, er, ei, err, nest, _ctx):
+            try:
+                k = max(nest, 0)
+                rc = fn(its, nconv, np.ctypeslib.as_array(er, (k,)).copy() if k else np.zeros(0),
+                        np.ctypeslib.as_array(ei, (k,)).copy() if k else np.zeros(0),
+                        np.ctypeslib.as_array(err, (k,)).copy() if k else np.zeros(0), nest)
+                return int(rc or 0)
+            except Exception:
+                return 1
+        cb = self._MONITOR(tramp)
+        self._callbacks = getattr(self, "_callbacks", []) + [cb]    # kept alive with the EPS
+        check(lib().sa_eps_monitor_set(self.h, C.cast(cb, C.c_void_p), None))
+
+    def cancelMonitor(self):
+        check(lib().sa_eps_monitor_cancel(self.h))
+
+    # EPSSetStoppingTestFunction (epsopts.c:863-885): fn(its, max_it, nconv,
+    # nev) returns the converged reason (REASON_ITERATING to continue); None
+    # restores EPSStoppingBasic.
+    def setStoppingTest(self, fn):
+        if fn is None:
+            self._stop_cb = None
+            check(lib().sa_eps_set_stopping_test_function(self.h, None, None))
+            return
+
+        def tramp(_eps, its, max_it, nconv, nev, reason, _ctx):
+            try:
+                reason[0] = int(fn(its, max_it, nconv, nev))
+                return 0
+            except Exception:
+                return 1
+        self._stop_cb = self._STOPPING(tramp)
+        check(lib().sa_eps_set_stopping_test_function(self.h, C.cast(self._stop_cb, C.c_void_p), None))
 
     def getEigenpair(self, i, vectors=True):
         re, im = C.c_double(), C.c_double()

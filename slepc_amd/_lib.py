@@ -49,6 +49,7 @@ SIGNATURES = {
     "sa_timer_stop": (C.c_int, [C.POINTER(C.c_double)]),
     "sa_mat_create_csr": (C.c_int, [C.c_int, C.c_int64, _I32P, _I32P, _F64P, C.c_int, C.POINTER(_VP)]),
     "sa_set_spmv_stream_bytes": (C.c_int64, [C.c_int64]),
+    "sa_set_concurrent_solves": (C.c_int, [C.c_int]),
     "sa_mat_destroy": (C.c_int, [_VP]),
     "sa_mat_get_info": (C.c_int, [_VP, C.POINTER(C.c_int), C.POINTER(C.c_int64), C.POINTER(C.c_int)]),
     "sa_mat_mult": (C.c_int, [_VP, _VP, _VP]),
@@ -125,6 +126,9 @@ SIGNATURES = {
     "sa_eps_get_converged": (C.c_int, [_VP, C.POINTER(C.c_int)]),
     "sa_eps_get_iteration_number": (C.c_int, [_VP, C.POINTER(C.c_int)]),
     "sa_eps_get_converged_reason": (C.c_int, [_VP, C.POINTER(C.c_int)]),
+    "sa_eps_monitor_set": (C.c_int, [_VP, _VP, _VP]),
+    "sa_eps_monitor_cancel": (C.c_int, [_VP]),
+    "sa_eps_set_stopping_test_function": (C.c_int, [_VP, _VP, _VP]),
     "sa_eps_get_eigenpair": (C.c_int, [_VP, C.c_int, C.POINTER(C.c_double), C.POINTER(C.c_double), _VP, _VP]),
     "sa_eps_get_error_estimate": (C.c_int, [_VP, C.c_int, C.POINTER(C.c_double)]),
     "sa_eps_compute_error": (C.c_int, [_VP, C.c_int, C.c_int, C.POINTER(C.c_double)]),

@@ -38,7 +38,7 @@ int bv_alloc(BV &V, int64_t n, int m, int prec) {
   // and at least gs_min_chunks() chunks each, so the reductions of the
   // partial sums stay short
   const bool big = V.ld * m * (int64_t)es > GS_NT_BYTES;   // the slot kernels' NT flag
-  const int64_t maxG = gs_max_blocks(big), minc = gs_min_chunks();
+  const int64_t maxG = gs_max_blocks(big), minc = gs_min_chunks(big);
   int64_t G = std::min(maxG, std::max<int64_t>(1, (chunks + minc - 1) / minc));
   const int64_t cpb = (chunks + G - 1) / G;
   G = (chunks + cpb - 1) / cpb;

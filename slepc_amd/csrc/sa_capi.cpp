@@ -59,6 +59,10 @@ int spmv_pick_rows(const int32_t *rp, int64_t n) {
 // Value/index bytes above which an operator uses the stream SpMV (settable
 // with sa_set_spmv_stream_bytes).
 std::atomic<int64_t> g_stream_bytes{64ll << 20};
+// Number of solves the application runs concurrently on the device (partition
+// hint for bases created afterwards, sa_set_concurrent_solves).
+std::atomic<int> g_concurrent{1};
+int concurrent_solves() { return g_concurrent.load(); }
 
 }  // namespace sa
 
@@ -90,6 +94,11 @@ struct sa_eps_s {
   sa_eps_stats st{};
   bool solved = false;
   double *rpart = nullptr;     // residual partial sums of EPSComputeError (allocated on first use)
+  // EPSMonitorSet (epsmon.c:96-122, at most MAXEPSMONITORS = 5) and
+  // EPSSetStoppingTestFunction (epsopts.c:863-885)
+  std::vector<std::pair<sa_eps_monitor_fn, void *>> monitors;
+  sa_eps_stopping_fn stopping = nullptr;
+  void *stopping_ctx = nullptr;
 };
 
 #define API_INIT() SA_TRY(ensure_init())
@@ -198,6 +207,7 @@ int sa_mat_create_csr(int n, int64_t nnz, const int32_t *rp, const int32_t *ci, 
 }
 
 int64_t sa_set_spmv_stream_bytes(int64_t bytes) { return g_stream_bytes.exchange(std::max<int64_t>(0, bytes)); }
+int sa_set_concurrent_solves(int nsolves) { return g_concurrent.exchange(std::max(1, nsolves)); }
 
 int sa_mat_destroy(sa_mat M) {
   if (!M) return 0;
@@ -781,9 +791,29 @@ int sa_eps_set_mixed(sa_eps E, int mixed, double tol32) {
 }
 
 // EPSSetUp + EPSSetDimensions_Default (epssetup.c:669-694, krylovschur.c:93-120)
+// Trampolines from the driver's hooks to the user's functions (EPSMonitor,
+// epsmon.c:39-46: every monitor in the order set).
+static int eps_monitor_hook(void *owner, int its, int nconv, const double *eigr, const double *eigi,
+                            const double *errest, int nest) {
+  sa_eps E = static_cast<sa_eps>(owner);
+  for (auto &m : E->monitors) {
+    const int rc = m.first(E, its, nconv, eigr, eigi, errest, nest, m.second);
+    if (rc) return rc;
+  }
+  return 0;
+}
+
+static int eps_stopping_hook(void *owner, int its, int max_it, int nconv, int nev, int *reason) {
+  sa_eps E = static_cast<sa_eps>(owner);
+  return E->stopping(E, its, max_it, nconv, nev, reason, E->stopping_ctx);
+}
+
 static int eps_setup_dims(sa_eps E, KsParams &p) {
   const int n = E->A->A.n;
   p = E->p;
+  p.hooks.owner = E;
+  p.hooks.monitor = E->monitors.empty() ? nullptr : eps_monitor_hook;
+  p.hooks.stopping = E->stopping ? eps_stopping_hook : nullptr;
   if (!E->ncv_set && !E->mpd_set) p.ncv = std::min(n, std::max(2 * p.nev, p.nev + 15));
   else if (!E->ncv_set) p.ncv = std::min(n, p.nev + p.mpd);
   if (!E->mpd_set) p.mpd = p.ncv;
@@ -981,6 +1011,26 @@ int sa_eps_get_iteration_number(sa_eps E, int *its) {
 int sa_eps_get_converged_reason(sa_eps E, int *reason) {
   SA_CHECK(E && reason && E->solved, "EPS not solved");
   *reason = E->res.reason;
+  return 0;
+}
+
+int sa_eps_monitor_set(sa_eps E, sa_eps_monitor_fn fn, void *ctx) {
+  SA_CHECK(E && fn, "null EPS or monitor");
+  SA_CHECK(E->monitors.size() < 5, "Too many EPS monitors set");
+  E->monitors.emplace_back(fn, ctx);
+  return 0;
+}
+
+int sa_eps_monitor_cancel(sa_eps E) {
+  SA_CHECK(E, "null EPS");
+  E->monitors.clear();
+  return 0;
+}
+
+int sa_eps_set_stopping_test_function(sa_eps E, sa_eps_stopping_fn fn, void *ctx) {
+  SA_CHECK(E, "null EPS");
+  E->stopping = fn;
+  E->stopping_ctx = ctx;
   return 0;
 }
 

@@ -23,6 +23,26 @@ double now() {
 }  // namespace
 
 // Algorithmic HBM bytes of one SpMV: values + column indices + row pointers + x + y.
+int eps_stopping(const KsParams &p, int its, int nconv, int nev, int *reason) {
+  if (p.hooks.stopping) {
+    int r = 0;
+    const int rc = p.hooks.stopping(p.hooks.owner, its, p.max_it, nconv, nev, &r);
+    SA_CHECK(rc == 0, "the stopping test returned an error code");
+    *reason = r;
+    return 0;
+  }
+  *reason = nconv >= nev ? 1 : (its >= p.max_it ? -1 : 0);
+  return 0;
+}
+
+int eps_monitor(const KsParams &p, int its, int nconv, const double *eigr, const double *eigi, const double *errest,
+                int nest) {
+  if (!p.hooks.monitor) return 0;
+  const int rc = p.hooks.monitor(p.hooks.owner, its, nconv, eigr, eigi, errest, nest);
+  SA_CHECK(rc == 0, "a monitor returned an error code");
+  return 0;
+}
+
 double spmv_bytes(const Mat &A) {
   const double s = (double)prec_size(A.prec);
   return (double)A.nnz * (s + 4.0) + 4.0 * (A.n + 1) + 2.0 * A.n * s;
@@ -483,8 +503,7 @@ static int lanczos_restart(const KsParams &p, BV &V, const DenseProblem &ds, int
   }
   for (int i = nconv; i < nv; i++) errest[i] = bnd[i];
   nconv = k;
-  if (k >= p.nev) reason = 1;
-  else if (its >= p.max_it) reason = -1;
+  SA_TRY(eps_stopping(p, its, k, p.nev, &reason));
   if (reason == 0) {   // BVCopyColumn(V,nv,nconv); a random vector after a breakdown
     SA_HIP(hipMemcpyAsync(col_ptr(V, nconv), col_ptr(V, nv), es * V.n, hipMemcpyDeviceToDevice, s));
     r.bytes_bv += 2.0 * V.n * es;
@@ -744,7 +763,7 @@ int ks_solve(const KsParams &p, const Mat &A, BV &V, bool have_initial, KsResult
                              bnd, conv, lperm, Yl, yrst, r, op_apply, wtmp.p));
       std::copy(locked.begin(), locked.end(), eigr.begin());
       tds += now() - td0;
-      if (p.monitor) p.monitor(p.monitor_ctx, its, nconv, eigr.data(), errest.data(), nv);
+      SA_TRY(eps_monitor(p, its, nconv, eigr.data(), eigi.data(), errest.data(), nv));
       continue;
     }
     ds.update_extra_row();
@@ -797,20 +816,18 @@ int ks_solve(const KsParams &p, const Mat &A, BV &V, bool have_initial, KsResult
         SA_HIP(hipStreamSynchronize(s));   // the staging buffer is rewritten next iteration
         r.bytes_restart += ((double)(nv - nconv) + (double)(e - nconv)) * V.n * es;
       }
-      if (k >= nev) reason = 1;
-      else if (its >= p.max_it) reason = -1;
+      SA_TRY(eps_stopping(p, its, k, nev, &reason));
       if (reason == 0 && breakdown) {    // arnoldi.c:118-125
         int bd = 0;
         SA_TRY(eps_start_vector(V, k, false, p.eta, &bd, scratch));
         if (bd) reason = -2;
       }
       nconv = k;
-      if (p.monitor) p.monitor(p.monitor_ctx, its, nconv, eigr.data(), errest.data(), nv);
+      SA_TRY(eps_monitor(p, its, nconv, eigr.data(), eigi.data(), errest.data(), nv));
       continue;
     }
-    // EPSStoppingBasic (epsdefault.c:290-302)
-    if (k >= nev) reason = 1;
-    else if (its >= p.max_it) reason = -1;
+    // the stopping test (krylovschur.c:284; EPSStoppingBasic by default)
+    SA_TRY(eps_stopping(p, its, k, nev, &reason));
     int newl = 0;
     if (!(reason != 0 || breakdown || k == nv)) {
       newl = std::max(1, (int)((nv - k) * p.keep));
@@ -858,7 +875,7 @@ int ks_solve(const KsParams &p, const Mat &A, BV &V, bool have_initial, KsResult
       r.bytes_bv += 2.0 * V.n * es;
     }
     nconv = k;
-    if (p.monitor) p.monitor(p.monitor_ctx, its, kconv, eigr.data(), errest.data(), nv);
+    SA_TRY(eps_monitor(p, its, kconv, eigr.data(), eigi.data(), errest.data(), nv));   // krylovschur.c:344
   }
   // DSTruncate(nconv,TRUE); EPSComputeVectors (Schur: V <- V X).  Lanczos
   // leaves Ritz vectors in the leading columns already.

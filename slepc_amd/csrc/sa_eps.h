@@ -6,6 +6,19 @@
 
 namespace sa {
 
+// Per-iteration hooks set through the C ABI: EPSMonitor (epsmon.c:39-46),
+// called at the end of every outer iteration with the Ritz values and error
+// estimates of that iteration, and the stopping test (EPSSetStoppingTestFunction,
+// epsopts.c; EPSStoppingBasic epsdefault.c:290-302 when unset).  owner is
+// passed back to the trampolines (the sa_eps handle).  A nonzero return from
+// either ends the solve with an error, as PetscCall does.
+struct KsHooks {
+  int (*monitor)(void *owner, int its, int nconv, const double *eigr, const double *eigi, const double *errest,
+                 int nest) = nullptr;
+  int (*stopping)(void *owner, int its, int max_it, int nconv, int nev, int *reason) = nullptr;
+  void *owner = nullptr;
+};
+
 struct KsParams {
   bool hermitian = true;
   int which = SA_LARGEST_MAGNITUDE;
@@ -25,8 +38,7 @@ struct KsParams {
   bool lock = true;              // EPSKrylovSchurSetLocking
   bool trueres = false;          // EPSSetTrueResidual (Krylov-Schur and Arnoldi)
   int lanczos_partial = 0;     // EPSLanczosSetReorthog: 1 periodic, 2 partial, 3 selective, 4 delayed
-  void (*monitor)(void *, int, int, const double *, const double *, int) = nullptr;
-  void *monitor_ctx = nullptr;
+  KsHooks hooks;
 };
 
 struct KsResult {
@@ -39,6 +51,14 @@ struct KsResult {
   std::vector<int> perm;           // sorted position -> column of V
   std::vector<double> eigr, eigi, errest;
 };
+
+// The stopping test of iteration its with nconv converged pairs: the hook,
+// or EPSStoppingBasic (epsdefault.c:290-302): 1 (converged) when nconv >=
+// nev, -1 (diverged, iterations) when its >= max_it, else 0.
+int eps_stopping(const KsParams &p, int its, int nconv, int nev, int *reason);
+// EPSMonitor: every monitor hook in turn (nothing when none is set).
+int eps_monitor(const KsParams &p, int its, int nconv, const double *eigr, const double *eigi, const double *errest,
+                int nest);
 
 double spmv_bytes(const Mat &A);
 int bv_upload_column(BV &V, int j, const double *x);

@@ -97,7 +97,8 @@ int launch_spmv(const Mat &A, const void *x, void *y, hipStream_t s);
 // One Arnoldi/orthonormalisation step for column c of V; spmv_from >= 0 first
 // computes V(:,c) = A V(:,spmv_from).  eta: DGKS threshold.
 int launch_gs_column(BV &V, const Mat *A, int c, double eta, hipStream_t s);
-int gs_min_chunks();        // fewest 64-row chunks per workgroup of the GS kernels
+int gs_min_chunks(bool big);   // fewest 64-row chunks per workgroup of the GS kernels (big: above GS_NT_BYTES)
+int concurrent_solves();      // the application's hint (sa_set_concurrent_solves)
 int gs_max_blocks(bool big);  // most workgroups of the GS kernels per basis (big: above GS_NT_BYTES)
 int launch_dotvec(BV &V, int l, int k, const void *x, double *q_dev, hipStream_t s);
 int launch_multvec(BV &V, int l, int k, double alpha, const double *q_dev, double beta, void *y,

@@ -211,8 +211,7 @@ int ks_twosided(const KsParams &p, const Mat &A, const Mat &AT, BV &V, BV &W, bo
       if (marker != -1) break;
     }
     if (marker != -1) k = marker;
-    if (k >= nev) reason = 1;
-    else if (its >= p.max_it) reason = -1;
+    SA_TRY(eps_stopping(p, its, k, nev, &reason));
     int newl = 0;
     if (!(reason != 0 || breakdown || k == nv)) {
       newl = std::max(1, (int)((nv - k) * p.keep));
@@ -280,7 +279,7 @@ int ks_twosided(const KsParams &p, const Mat &A, const Mat &AT, BV &V, BV &W, bo
         }
     }
     nconv = k;
-    if (p.monitor) p.monitor(p.monitor_ctx, its, nconv, eigr.data(), errest.data(), nv);
+    SA_TRY(eps_monitor(p, its, nconv, eigr.data(), eigi.data(), errest.data(), nv));
   }
 
   // DSTruncate(nconv,TRUE); EPSComputeVectors_Schur (epsdefault.c:105-169):

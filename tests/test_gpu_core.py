@@ -436,6 +436,49 @@ def test_concurrent_solves_match_sequential(sa):
         assert max(e2) <= 1e-10
 
 
+def test_concurrency_hint_partition(sa, orc):
+    """sa_set_concurrent_solves(4): bases created afterwards use two 64-row
+    chunks per Gram-Schmidt workgroup.  Concurrent solves under the hint give
+    the oracle's eigenvalues (to 1e-10 relative) with every pair at tol, and
+    the same bits as the hinted solves run one after the other."""
+    import concurrent.futures as cf
+    g = 96
+    hs = [sa.fv2d(g, g, sa.fv_theta(5, s)) for s in range(4)]
+    probs = [sa.Mat(h) for h in hs]
+    n = g * g
+    v0 = sa.rand_uniform(0, n)
+
+    def solve(i):
+        E = sa.EPS()
+        E.setOperators(probs[i])
+        E.setProblemType(True)
+        E.setWhichEigenpairs("LR")
+        E.setDimensions(8, 32)
+        E.setTolerances(1e-10, 20000)
+        E.setTrueResidual(True)
+        E.setInitialSpace(v0)
+        E.solve()
+        vals = [E.getEigenpair(j, vectors=False)[0] for j in range(E.getConverged())]
+        errs = [E.computeError(j) for j in range(min(8, E.getConverged()))]
+        return vals, E.getIterationNumber(), errs
+
+    prev = sa.set_concurrent_solves(4)
+    try:
+        seq = [solve(i) for i in range(len(probs))]
+        with cf.ThreadPoolExecutor(max_workers=4) as ex:
+            conc = list(ex.map(solve, range(len(probs))))
+    finally:
+        sa.set_concurrent_solves(prev)
+    for i, ((v1, it1, e1), (v2, it2, e2)) in enumerate(zip(seq, conc)):
+        assert it1 == it2 and np.array_equal(v1, v2)
+        assert len(e2) == 8 and max(e2) <= 1e-10
+        h = hs[i]
+        ref = orc.eps_solve(orc.CSR(h.n, h.rowptr, h.col, h.val), 8, 32, 1e-10, "LR", True, v0=v0, nvec=0,
+                            trueres=True, max_it=20000)
+        r = np.sort(ref["eigr"][:8])[::-1]
+        assert np.allclose(np.sort(v2[:8])[::-1], r, rtol=1e-10, atol=0)
+
+
 # ------------------------------------------------------------------ EPSARNOLDI
 @pytest.mark.parametrize("solver,case", [("arnoldi", "test2_LR"), ("arnoldi", "test2_SR"), ("arnoldi", "markov15"),
                                          ("arnoldi", "convdiff12"), ("lanczos", "test2_LR"), ("lanczos", "test2_SR"),
