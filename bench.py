@@ -350,9 +350,11 @@ def run_batched(sa):
     out = {"workload": "512 Laplacians nx,ny in U{32..256} + diag 10 U[0,1) seed 2, nev 8 LM, ncv 24, "
                        "tol 1e-10 on true residuals (%d rows, %d nnz)" % (sum(A.n for A in csrs),
                                                                          sum(A.nnz for A in csrs))}
-    for name, mixed in (("plain_fp64", False), ("mixed", True)):
+    def run(name, mixed, ncv=24, graphs=True):
+        B.set_graphs(graphs)
+        g0 = B.graph_stats()
         t0 = time.perf_counter()
-        B.solve(8, 24, 1e-10, "LM", mixed=mixed, seed=2)
+        B.solve(8, ncv, 1e-10, "LM", mixed=mixed, seed=2)
         err = B.errors()
         good = 0
         for p in range(len(csrs)):
@@ -360,10 +362,33 @@ def run_batched(sa):
             good += int(np.sum(err[p, :nc] <= 1e-10))
         dt = time.perf_counter() - t0
         st = B.stats()
+        g1 = B.graph_stats()
         out[name] = {"eigenpairs_per_s": round(good / dt, 1), "seconds": round(dt, 3), "good_pairs": good,
                      "worst_rel_residual": float(err.max()), "restart_rounds": st["rounds"],
                      "restart_rounds32": st["rounds32"], "host_ds_s": round(st["seconds_ds"], 3),
-                     "sweep": B.sweep_info()}
+                     "matvecs": st["matvecs"] + st["matvecs32"], "schedule": B.schedule(),
+                     "graph_captures": g1["captures"] - g0["captures"],
+                     "graph_replays": g1["replays"] - g0["replays"]}
+
+    # the first solve of each precision tunes the sweep schedule and captures
+    # the sweep graphs; it is reported apart (cold), the timed solves reuse both
+    run("plain_fp64_cold", False)
+    out["schedule_tuning"] = B.schedule()
+    run("plain_fp64", False)
+    run("plain_fp64_direct_launches", False, graphs=False)   # A/B of the captured sweeps
+    run("mixed_cold", True)
+    run("mixed", True)
+    # basis size per size class by the autotuner (the config fixes ncv 24; reported beside it)
+    t0 = time.perf_counter()
+    per = B.autotune_ncv(8, 1e-10, [16, 24, 32, 48], which="LM", nclass=4, nsample=16, seed=2)
+    tune_s = time.perf_counter() - t0
+    run("plain_fp64_autotuned_ncv_cold", False, ncv=0)
+    run("plain_fp64_autotuned_ncv", False, ncv=0)
+    tun = B.ncv_tuning()
+    out["ncv_autotuner"] = {"seconds": round(tune_s, 3), "candidates": [16, 24, 32, 48],
+                            "class_max_n": tun["class_hi"].tolist(),
+                            "seconds_per_pair": np.round(tun["scores"], 6).tolist(),
+                            "ncv_histogram": {int(k): int(v) for k, v in zip(*np.unique(per, return_counts=True))}}
     log("batched: %s" % json.dumps(out))
     B.destroy()
     return out

@@ -432,7 +432,9 @@ int sa_batch_create(int P, const int32_t *n, const int32_t *rowptr, const int32_
 int sa_batch_destroy(sa_batch batch);
 /* mixed != 0: FP32 bulk to tol32, then FP64 polish seeded with the sum of the
    FP32 Ritz vectors.  v0 optional (concatenated start vectors); NULL draws
-   U[-1,1) with seed sa_sub_seed(seed, p).  ncv <= 64. */
+   U[-1,1) with seed sa_sub_seed(seed, p).  ncv > 0: every problem uses it
+   (nev < ncv <= min(64, n)); ncv == 0: the per-problem choice of
+   sa_batch_autotune_ncv, else EPS's default max(2 nev, nev + 15). */
 int sa_batch_solve(sa_batch batch, int which, int nev, int ncv, double tol, int max_it,
                    int mixed, double tol32, const double *v0, uint64_t seed);
 int sa_batch_get_converged(sa_batch batch, int p, int *nconv, int *its, int *reason);
@@ -455,6 +457,42 @@ int sa_batch_get_stats(sa_batch batch, int64_t *counters6, double *seconds3);
    timings of one sweep (milliseconds, -1 when not timed). */
 int sa_batch_set_sweep(sa_batch batch, int mode);
 int sa_batch_get_sweep(sa_batch batch, int *mode, double *ms_persistent, double *ms_wide);
+/* The schedule autotuner (mode 0) times, on the first restart round, the
+   one-workgroup sweep (when ncv < 64) and three lock-step tile schedules:
+   the default (non-temporal tile loads for a basis above 256 MiB, SpMV lanes
+   per row from the mean row length), the other tile-load kind, and twice the
+   lanes; it reruns the fastest so the round's results are its own.  The
+   getter returns the last solve's schedule (kind 1 one workgroup per
+   problem, 2 lock-step; nt; lanes) and the four timings in milliseconds (-1:
+   not timed).  No counterpart in the reference. */
+int sa_batch_get_schedule(sa_batch batch, int *kind, int *nt, int *lanes, double *ms4);
+/* Lock-step restart sweeps replayed from HIP graphs (default on): each sweep
+   (state reset, 5 launches per step, export) is captured once per launch
+   signature (grids, LDS sizes, step count, schedule) and replayed; the
+   problem lists, k0, nv and chunk lists it reads are device buffers refreshed
+   before each replay.  At most 64 graphs per precision (least recently used
+   evicted).  The stats count captures, replays and cached graphs.  No
+   counterpart in the reference. */
+int sa_batch_set_graphs(sa_batch batch, int on);
+int sa_batch_get_graph_stats(sa_batch batch, int64_t *captures, int64_t *replays, int *cached);
+/* Basis-size autotuner: the problems, sorted by n, form nclass classes of
+   (nearly) equal count, and nsample problems spread over each class form one
+   stratified sample, solved as a sub-batch with every candidate ncv (nev <
+   cand <= 64; FP64, lock-step sweeps, this batch's true-residual setting,
+   start vectors U[-1,1) seed sa_sub_seed(seed, p)).  Each candidate's wall
+   time is split over the classes in proportion to their problems'
+   algorithmic bytes, and the candidate with the least time per converged
+   pair becomes the ncv of every problem of its class (ncv_out[p], may be
+   NULL), used by later sa_batch_solve calls with ncv == 0.  The tuning
+   getter returns the class bounds (largest n of each class) and the seconds
+   per pair of every (class, candidate), 1e300 where a candidate was not run
+   (ncv above the smallest sampled n) or left pairs unconverged.
+   sa_batch_get_ncv: the ncv of each problem in the last solve.  No
+   counterpart in the reference (EPSSetDimensions is set by hand there). */
+int sa_batch_autotune_ncv(sa_batch batch, int which, int nev, double tol, const int *cand, int ncand,
+                          int nclass, int nsample, uint64_t seed, int *ncv_out);
+int sa_batch_get_ncv_tuning(sa_batch batch, int *nclass, int *ncand, int *class_hi, double *scores);
+int sa_batch_get_ncv(sa_batch batch, int *ncv);
 
 /* ---- learned warm start: MLP in BF16 on MFMA ---------------------------------
  * theta (din <= 32) -> hidden -> hidden -> dout, ReLU, weights and activations

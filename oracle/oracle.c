@@ -405,7 +405,8 @@ void orc_bv_multvec(int64_t n, int k, double alpha, const double *V, int64_t ld,
       const double qc = q[c];
       for (int64_t r = r0; r < r1; r++) s[r - r0] += vc[r] * qc;
     }
-    for (int64_t r = r0; r < r1; r++) y[r] = beta * y[r] + alpha * s[r - r0];
+    /* beta == 0: y is not read (gemv semantics, bvblas.c:56-67) */
+    for (int64_t r = r0; r < r1; r++) y[r] = (beta == 0.0 ? 0.0 : beta * y[r]) + alpha * s[r - r0];
   }
 }
 

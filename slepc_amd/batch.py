@@ -23,6 +23,13 @@ def _sig():
     L.sa_batch_set_sweep.argtypes = [vp, C.c_int]
     L.sa_batch_set_true_residual.argtypes = [vp, C.c_int]
     L.sa_batch_get_sweep.argtypes = [vp, C.POINTER(C.c_int), C.POINTER(C.c_double), C.POINTER(C.c_double)]
+    L.sa_batch_get_schedule.argtypes = [vp, C.POINTER(C.c_int), C.POINTER(C.c_int), C.POINTER(C.c_int), vp]
+    L.sa_batch_set_graphs.argtypes = [vp, C.c_int]
+    L.sa_batch_get_graph_stats.argtypes = [vp, C.POINTER(C.c_int64), C.POINTER(C.c_int64), C.POINTER(C.c_int)]
+    L.sa_batch_autotune_ncv.argtypes = [vp, C.c_int, C.c_int, C.c_double, vp, C.c_int, C.c_int, C.c_int,
+                                        C.c_uint64, vp]
+    L.sa_batch_get_ncv_tuning.argtypes = [vp, C.POINTER(C.c_int), C.POINTER(C.c_int), vp, vp]
+    L.sa_batch_get_ncv.argtypes = [vp, vp]
     return L
 
 
@@ -104,6 +111,48 @@ class Batch:
         check(self.L.sa_batch_get_sweep(self.h, C.byref(m), C.byref(a), C.byref(b)))
         names = {v: k for k, v in SWEEP_MODES.items()}
         return dict(mode=names.get(m.value, "none"), ms_persistent=a.value, ms_wide=b.value)
+
+    def schedule(self):
+        """The last solve's sweep schedule and the autotuner's timings (ms) of the
+        candidates [one workgroup per problem, lock-step, other tile loads,
+        twice the SpMV lanes] (-1: not timed)."""
+        k, nt, la = C.c_int(), C.c_int(), C.c_int()
+        ms = np.zeros(4)
+        check(self.L.sa_batch_get_schedule(self.h, C.byref(k), C.byref(nt), C.byref(la), ms.ctypes.data))
+        return dict(kind={1: "persistent", 2: "wide"}.get(k.value, "none"), nt=nt.value, lanes=la.value,
+                    ms=[float(x) for x in ms])
+
+    def set_graphs(self, on=True):
+        """Replay the lock-step restart sweeps from captured HIP graphs."""
+        check(self.L.sa_batch_set_graphs(self.h, 1 if on else 0))
+
+    def graph_stats(self):
+        c, r, k = C.c_int64(), C.c_int64(), C.c_int()
+        check(self.L.sa_batch_get_graph_stats(self.h, C.byref(c), C.byref(r), C.byref(k)))
+        return dict(captures=c.value, replays=r.value, cached=k.value)
+
+    def autotune_ncv(self, nev, tol, cand, which="LM", nclass=4, nsample=4, seed=0):
+        """Basis-size autotuner per size class; returns the per-problem ncv
+        (used by solve(ncv=0))."""
+        from . import WHICH
+        c = np.ascontiguousarray(cand, np.int32)
+        out = np.zeros(self.P, np.int32)
+        check(self.L.sa_batch_autotune_ncv(self.h, WHICH[which], nev, tol, c.ctypes.data, len(c), nclass, nsample,
+                                           seed, out.ctypes.data))
+        return out
+
+    def ncv_tuning(self):
+        nc, nq = C.c_int(), C.c_int()
+        check(self.L.sa_batch_get_ncv_tuning(self.h, C.byref(nc), C.byref(nq), None, None))
+        hi = np.zeros(nc.value, np.int32)
+        sc = np.zeros(nc.value * nq.value)
+        check(self.L.sa_batch_get_ncv_tuning(self.h, C.byref(nc), C.byref(nq), hi.ctypes.data, sc.ctypes.data))
+        return dict(class_hi=hi, scores=sc.reshape(nc.value, nq.value))
+
+    def ncv(self):
+        out = np.zeros(self.P, np.int32)
+        check(self.L.sa_batch_get_ncv(self.h, out.ctypes.data))
+        return out
 
     def destroy(self):
         if self.h:

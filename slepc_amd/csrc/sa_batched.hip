@@ -432,6 +432,24 @@ __global__ void bw_export_kernel(const int *list, int nl, GsState *st, int *stop
 // ---------------------------------------------------------------------------
 // host side
 // ---------------------------------------------------------------------------
+// A lock-step sweep replayed from a HIP graph: the signature fixes every
+// launch of the sweep (grids, LDS sizes, step indices, schedule); the data it
+// reads (problem list, k0, nv, chunk lists) lives in device buffers whose
+// addresses the graph keeps, refreshed by copies before each launch.
+struct SweepGraph {
+  std::vector<int> sig;
+  hipGraphExec_t exec = nullptr;
+  uint64_t used = 0;
+};
+constexpr size_t SWEEP_GRAPHS_MAX = 256;
+
+// Sweep schedule: kind 1 one workgroup per problem (persistent sweep), 2
+// lock-step; for lock-step, nt selects non-temporal, wave-uniform tile loads
+// and lanes the lanes per row of the SpMV fused into slot 1.
+struct Sched {
+  int kind = 2, nt = 0, lanes = 8;
+};
+
 struct BatchDev {
   int P = 0, m = 0, prec = SA_FP64;
   int64_t ldv = 0, nrows = 0, nnz = 0;
@@ -453,7 +471,18 @@ struct BatchDev {
   void *tx = nullptr;                          // TR_SLOTS columns of ldv
   double *tz = nullptr, *tlam = nullptr, *tpart = nullptr, *tout = nullptr;
   int *tnv = nullptr, *tact = nullptr, *pchunk0 = nullptr, *pnchunk = nullptr;
+  int *passes_save = nullptr;                  // copy of passes while the autotuner reruns a sweep
+  // lock-step sweeps captured as HIP graphs, keyed by their launch signature
+  std::vector<SweepGraph> graphs;
+  bool graphs_on = true;
+  uint64_t graph_clock = 0, graph_captures = 0, graph_replays = 0;
 };
+
+static void clear_graphs(BatchDev &B) {
+  for (auto &g : B.graphs)
+    if (g.exec) (void)hipGraphExecDestroy(g.exec);
+  B.graphs.clear();
+}
 
 template <typename T, int CM, int NT>
 static void launch_sweep_t(const BatchDev &B, int nact, int spmv, double eta, hipStream_t s) {
@@ -505,16 +534,25 @@ static void launch_bw_slot(const BwArgs<T> &a, int lanes, bool nt, int grid, siz
 }
 
 template <typename T>
-static int wide_sweep_t(BatchDev &B, const std::vector<int> &list, const std::vector<int> &k0,
+static int wide_sweep_t(BatchDev &B, const Sched &sc, const std::vector<int> &list, const std::vector<int> &k0,
                         const std::vector<int> &nv, int nch, int spmv, double eta, hipStream_t s) {
   const int P = B.P, m = B.m, nl = (int)list.size();
-  const bool nt = B.ldv * m * (int64_t)sizeof(T) > GS_NT_BYTES;
+  const bool nt = sc.nt != 0;
   int steps = 1;
   if (spmv) {
     steps = 0;
     for (int p : list) steps = std::max(steps, nv[p] - k0[p]);
   }
-  hipLaunchKernelGGL(bw_reset_kernel, dim3((nl + 255) / 256), dim3(256), 0, s, B.order, nl, B.wst);
+  // largest column of each step over the problems still sweeping
+  std::vector<int> cmax(steps, 0);
+  for (int j = 0; j < steps; j++) {
+    for (int p : list) {
+      const int c = k0[p] + (spmv ? 1 + j : 0);
+      if (c <= nv[p]) cmax[j] = std::max(cmax[j], c);
+    }
+    SA_CHECK(gs_lds_bytes(cmax[j], sizeof(T)) + 4096 + 8 * BLK <= 65536,
+             "basis too wide for the LDS tile of the Gram-Schmidt kernel");
+  }
   double *pa = B.wpart, *pb = B.wpart + (size_t)B.nchunks * m;
   double *da = B.wdots, *db = B.wdots + (size_t)P * m;
   BwArgs<T> a;
@@ -522,41 +560,79 @@ static int wide_sweep_t(BatchDev &B, const std::vector<int> &list, const std::ve
   a.off = B.off; a.n = B.n; a.k0 = B.k0; a.nv = B.nv;
   a.cprob = B.cprob; a.crow = B.crow; a.pc0 = B.pc0; a.pnc = B.pnc;
   a.spmv = spmv; a.m = m; a.st = B.wst; a.dbuf = B.dbuf; a.eta = eta;
-  for (int j = 0; j < steps; j++) {
-    int cmax = 0;
-    for (int p : list) {
-      const int c = k0[p] + (spmv ? 1 + j : 0);
-      if (c <= nv[p]) cmax = std::max(cmax, c);
+  auto enqueue = [&]() {
+    hipLaunchKernelGGL(bw_reset_kernel, dim3((nl + 255) / 256), dim3(256), 0, s, B.order, nl, B.wst);
+    for (int j = 0; j < steps; j++) {
+      const size_t shm = gs_lds_bytes(cmax[j], sizeof(T));
+      a.j = j;
+      for (int slot = 1; slot <= 3; slot++) {
+        a.slot = slot;
+        a.part_out = slot == 1 ? pa : pb;
+        a.dots_in = slot == 2 ? da : db;
+        launch_bw_slot<T>(a, sc.lanes, nt, nch, shm, s);
+        if (slot < 3)
+          hipLaunchKernelGGL(bw_sum_kernel, dim3(cmax[j] + 1, nl), dim3(BLK), 0, s, (const int *)B.order,
+                             (const int *)B.k0, (const int *)B.nv, spmv, j, slot, (const GsState *)B.wst,
+                             (const double *)a.part_out, (const int *)B.pc0, (const int *)B.pnc, m,
+                             slot == 1 ? da : db);
+      }
     }
-    const size_t shm = gs_lds_bytes(cmax, sizeof(T));
-    SA_CHECK(shm + 4096 + 8 * BLK <= 65536, "basis too wide for the LDS tile of the Gram-Schmidt kernel");
-    a.j = j;
-    for (int slot = 1; slot <= 3; slot++) {
-      a.slot = slot;
-      a.part_out = slot == 1 ? pa : pb;
-      a.dots_in = slot == 2 ? da : db;
-      launch_bw_slot<T>(a, B.lanes, nt, nch, shm, s);
-      if (slot < 3)
-        hipLaunchKernelGGL(bw_sum_kernel, dim3(cmax + 1, nl), dim3(BLK), 0, s, (const int *)B.order,
-                           (const int *)B.k0, (const int *)B.nv, spmv, j, slot, (const GsState *)B.wst,
-                           (const double *)a.part_out, (const int *)B.pc0, (const int *)B.pnc, m,
-                           slot == 1 ? da : db);
-    }
+    hipLaunchKernelGGL(bw_export_kernel, dim3((nl + 255) / 256), dim3(256), 0, s, (const int *)B.order, nl, B.wst,
+                       B.stop, B.beta, B.passes);
+  };
+  if (!B.graphs_on) {
+    enqueue();
+    SA_HIP(hipGetLastError());
+    return 0;
   }
-  hipLaunchKernelGGL(bw_export_kernel, dim3((nl + 255) / 256), dim3(256), 0, s, (const int *)B.order, nl, B.wst,
-                     B.stop, B.beta, B.passes);
-  SA_HIP(hipGetLastError());
+  // replay the captured sweep of this launch signature, capturing it first
+  std::vector<int> sig = {spmv, sc.nt, sc.lanes, nch, nl, steps};
+  sig.insert(sig.end(), cmax.begin(), cmax.end());
+  SweepGraph *g = nullptr;
+  for (auto &e : B.graphs)
+    if (e.sig == sig) { g = &e; break; }
+  if (!g) {
+    SA_HIP(hipGetLastError());
+    SA_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+    enqueue();
+    const hipError_t le = hipGetLastError();
+    hipGraph_t graph = nullptr;
+    const hipError_t ec = hipStreamEndCapture(s, &graph);
+    if (le != hipSuccess || ec != hipSuccess) {
+      if (graph) (void)hipGraphDestroy(graph);
+      SA_HIP(le);
+      SA_HIP(ec);
+    }
+    hipGraphExec_t exec = nullptr;
+    const hipError_t ei = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(graph);
+    SA_HIP(ei);
+    if (B.graphs.size() >= SWEEP_GRAPHS_MAX) {   // evict the least recently used
+      auto lru = std::min_element(B.graphs.begin(), B.graphs.end(),
+                                  [](const SweepGraph &x, const SweepGraph &y) { return x.used < y.used; });
+      (void)hipGraphExecDestroy(lru->exec);
+      B.graphs.erase(lru);
+    }
+    B.graphs.push_back(SweepGraph{sig, exec, 0});
+    g = &B.graphs.back();
+    B.graph_captures++;
+  } else {
+    B.graph_replays++;
+  }
+  g->used = ++B.graph_clock;
+  SA_HIP(hipGraphLaunch(g->exec, s));
   return 0;
 }
 
 // Lock-step sweep of the listed problems; their k0/nv (host copies given) and
 // the list itself (B.order) are already on the device.  Buffers are allocated
 // on first use for the current basis width.
-static int wide_sweep(BatchDev &B, const std::vector<int> &nrows, const std::vector<int> &list,
+static int wide_sweep(BatchDev &B, const Sched &sc, const std::vector<int> &nrows, const std::vector<int> &list,
                       const std::vector<int> &k0, const std::vector<int> &nv, int spmv, double eta, hipStream_t s) {
   if (list.empty()) return 0;
   const int P = B.P, m = B.m;
   if (B.wm != m) {
+    clear_graphs(B);   // the captured sweeps point at the buffers replaced here
     for (double **q : {&B.wpart, &B.wdots})
       if (*q) { (void)hipFree(*q); *q = nullptr; }
     if (!B.wst) {
@@ -586,18 +662,29 @@ static int wide_sweep(BatchDev &B, const std::vector<int> &nrows, const std::vec
   SA_HIP(hipMemcpyAsync(B.crow, cr.data(), sizeof(int) * nch, hipMemcpyHostToDevice, s));
   SA_HIP(hipMemcpyAsync(B.pc0, pc0.data(), sizeof(int) * P, hipMemcpyHostToDevice, s));
   SA_HIP(hipMemcpyAsync(B.pnc, pnc.data(), sizeof(int) * P, hipMemcpyHostToDevice, s));
-  int rc = B.prec == SA_FP64 ? wide_sweep_t<double>(B, list, k0, nv, nch, spmv, eta, s)
-                             : wide_sweep_t<float>(B, list, k0, nv, nch, spmv, eta, s);
+  int rc = B.prec == SA_FP64 ? wide_sweep_t<double>(B, sc, list, k0, nv, nch, spmv, eta, s)
+                             : wide_sweep_t<float>(B, sc, list, k0, nv, nch, spmv, eta, s);
   // the host vectors must outlive the asynchronous copies
   SA_HIP(hipStreamSynchronize(s));
   return rc;
 }
 
-// Sweep modes: 1 one workgroup per problem (persistent), 2 lock-step (wide).
-static int run_sweep(BatchDev &B, int mode, const std::vector<int> &nrows, const std::vector<int> &list,
+// One sweep of the listed problems with schedule sc (kind 1: one workgroup per
+// problem, 2: lock-step).
+static int run_sweep(BatchDev &B, const Sched &sc, const std::vector<int> &nrows, const std::vector<int> &list,
                      const std::vector<int> &k0, const std::vector<int> &nv, int spmv, double eta, hipStream_t s) {
-  if (mode == 1) return launch_sweep(B, (int)list.size(), spmv, eta, s);
-  return wide_sweep(B, nrows, list, k0, nv, spmv, eta, s);
+  if (sc.kind == 1) return launch_sweep(B, (int)list.size(), spmv, eta, s);
+  return wide_sweep(B, sc, nrows, list, k0, nv, spmv, eta, s);
+}
+
+// The lock-step schedule without tuning: non-temporal tiles for a basis larger
+// than the last-level cache, lanes per row from the operator's mean row length.
+static Sched default_sched(const BatchDev &B) {
+  Sched sc;
+  sc.kind = 2;
+  sc.nt = B.ldv * B.m * (int64_t)prec_size(B.prec) > GS_NT_BYTES ? 1 : 0;
+  sc.lanes = B.lanes;
+  return sc;
 }
 
 }  // namespace sa
@@ -610,7 +697,15 @@ struct sa_batch_s {
   int trueres = 0;                      // EPSSetTrueResidual for every problem
   int sweep_used = 0;                   // mode of the last solve's FP64 phase
   double ms_persistent = -1.0, ms_wide = -1.0;   // autotuner timings of one restart sweep
-  int tuned_m[2] = {0, 0}, tuned_mode[2] = {0, 0};   // autotuner's choice per precision (FP64, FP32) and basis width
+  int tuned_m[2] = {0, 0};              // basis width the schedule autotuner ran at, per precision (FP64, FP32)
+  Sched tuned[2];                       // its choice
+  Sched sched_used;                     // schedule of the last solve's last phase
+  double tune_ms[4] = {-1, -1, -1, -1}; // candidate timings: persistent, lock-step, other NT, doubled lanes
+  std::vector<int> pncv;                // per-problem ncv chosen by sa_batch_autotune_ncv (empty: none)
+  std::vector<double> ncv_scores;       // its seconds per converged pair, [class][candidate]
+  std::vector<int> ncv_cand, class_hi;  // its candidates and the largest n of each size class
+  bool graphs = true;                   // capture the lock-step sweeps as HIP graphs
+  std::vector<int> ncv_used;            // ncv of each problem in the last solve
   std::vector<int> n;
   std::vector<int64_t> off;
   int64_t nrows = 0, nnz = 0;
@@ -628,11 +723,12 @@ struct sa_batch_s {
 };
 
 static void free_dev(BatchDev &B) {
+  clear_graphs(B);
   void *ptrs[] = {B.V,   B.rp,    B.ci,         B.val,        B.off,  B.n,    B.order, B.k0,  B.nv,
                   B.stop, B.passes, B.dbuf,     B.beta,       B.Q,    B.alpha, B.chunk_prob, B.chunk_row,
                   B.ml,  B.mk,    B.ms,         B.me,         B.src,  B.dst,  B.wst,   B.wpart, B.wdots,
                   B.cprob, B.crow, B.pc0,       B.pnc,        B.tx,   B.tz,   B.tlam,  B.tpart, B.tout,
-                  B.tnv, B.tact, B.pchunk0, B.pnchunk};
+                  B.tnv, B.tact, B.pchunk0, B.pnchunk, B.passes_save};
   for (void *p : ptrs)
     if (p) (void)hipFree(p);
   B = BatchDev();
@@ -643,6 +739,7 @@ static int alloc_dev(sa_batch_s *b, BatchDev &B, int prec, int m) {
   const bool keep_op = B.rp && B.prec == prec;
   if (!keep_op) free_dev(B);
   else if (B.V) {
+    clear_graphs(B);   // they point at the basis replaced here
     (void)hipFree(B.V);
     B.V = nullptr;
     if (B.tz) (void)hipFree(B.tz);   // sized by the basis width
@@ -676,7 +773,8 @@ static int alloc_dev(sa_batch_s *b, BatchDev &B, int prec, int m) {
   B.lanes = avg <= 4.5 ? 4 : (avg <= 9 ? 8 : (avg <= 18 ? 16 : 32));
   SA_HIP(hipMalloc(&B.off, sizeof(int64_t) * P));
   SA_HIP(hipMalloc(&B.n, sizeof(int) * P));
-  for (int **q : {&B.order, &B.k0, &B.nv, &B.stop, &B.passes, &B.ml, &B.mk, &B.ms, &B.me, &B.src, &B.dst})
+  for (int **q : {&B.order, &B.k0, &B.nv, &B.stop, &B.passes, &B.ml, &B.mk, &B.ms, &B.me, &B.src, &B.dst,
+                  &B.passes_save})
     SA_HIP(hipMalloc(q, sizeof(int) * P));
   SA_HIP(hipMalloc(&B.dbuf, sizeof(double) * (size_t)P * 128 * 128));
   SA_HIP(hipMalloc(&B.Q, sizeof(double) * (size_t)P * 128 * 128));
@@ -796,10 +894,18 @@ double nowb() { return std::chrono::duration<double>(std::chrono::steady_clock::
 // One batched Krylov-Schur solve (symmetric problems, Lanczos + DSHEP on each)
 // in basis precision prec; starts from column 0 of V.  Mirrors
 // EPSSolve_KrylovSchur_Default (krylovschur.c:227-348) per problem.
-static int batch_ks(sa_batch_s *b, BatchDev &B, int which, int nev, int ncv, double tol, int max_it, bool trueres,
-                    std::vector<ProbState> &ps, int64_t &matvecs, int64_t &passes_out, int &its_max, double &tds) {
-  const int P = b->P, m = ncv + 1, ld = m, mpd = ncv;
+// pncv: basis size of each problem (B.m = the largest + 1); subset: when
+// given, only the problems marked nonzero are solved (the others stay
+// inactive with no converged pairs); pbytes: when given, receives each
+// problem's algorithmic HBM bytes (two Gram-Schmidt passes per step, the
+// SpMV, the restart GEMM and the true-residual checks).
+static int batch_ks(sa_batch_s *b, BatchDev &B, int which, int nev, const std::vector<int> &pncv, double tol,
+                    int max_it, bool trueres, std::vector<ProbState> &ps, int64_t &matvecs, int64_t &passes_out,
+                    int &its_max, double &tds, const std::vector<char> *subset = nullptr,
+                    std::vector<double> *pbytes = nullptr) {
+  const int P = b->P, m = B.m, ld = m;
   const double eta = 0.7071, keep = 0.5;
+  B.graphs_on = b->graphs;
   hipStream_t s = stream();
   SA_CHECK(lapack_ready(), "no LAPACK library found for the projected problem");
   if (trueres) SA_TRY(ensure_tres(b, B));
@@ -812,40 +918,57 @@ static int batch_ks(sa_batch_s *b, BatchDev &B, int which, int nev, int ncv, dou
     ps[p].eigi.assign(ld, 0.0);
     ps[p].errest.assign(ld, 0.0);
   }
-  std::vector<int> order(P), k0(P), nv(P), stop(P), passes0(P), passes1(P), zero(P, 0);
+  std::vector<int> order, k0(P), nv(P), stop(P), passes0(P), passes1(P), zero(P, 0);
   std::vector<double> beta(P), dbuf, Qh((size_t)P * m * m, 0.0);
   std::vector<int> ml(P), mk(P), msv(P), mev(P), src(P), dst(P);
   // problems by decreasing size: the scheduler hands the largest sweeps out first
-  std::iota(order.begin(), order.end(), 0);
+  for (int p = 0; p < P; p++) {
+    if (subset && !(*subset)[p]) ps[p].active = false;
+    else order.push_back(p);
+  }
   std::stable_sort(order.begin(), order.end(), [&](int x, int y) { return b->n[x] > b->n[y]; });
+  SA_CHECK(!order.empty(), "no problem to solve");
   SA_HIP(hipMemcpyAsync(B.passes, zero.data(), sizeof(int) * P, hipMemcpyHostToDevice, s));
   // orthonormalise the start vectors (column 0): EPSGetStartVector
   for (int p = 0; p < P; p++) { k0[p] = 0; nv[p] = 0; stop[p] = -1; }
-  SA_HIP(hipMemcpyAsync(B.order, order.data(), sizeof(int) * P, hipMemcpyHostToDevice, s));
+  SA_HIP(hipMemcpyAsync(B.order, order.data(), sizeof(int) * order.size(), hipMemcpyHostToDevice, s));
   SA_HIP(hipMemcpyAsync(B.k0, k0.data(), sizeof(int) * P, hipMemcpyHostToDevice, s));
   SA_HIP(hipMemcpyAsync(B.nv, nv.data(), sizeof(int) * P, hipMemcpyHostToDevice, s));
   SA_HIP(hipMemcpyAsync(B.stop, stop.data(), sizeof(int) * P, hipMemcpyHostToDevice, s));
-  // sweep mode: requested, or decided by timing both on the first restart round
-  int mode = b->sweep_req;
-  SA_CHECK(mode != 1 || persistent_fits(B), "the one-workgroup-per-problem sweep handles ncv < 64");
-  if (mode == 0 && !persistent_fits(B)) mode = 2;
+  // sweep schedule: requested, tuned by an earlier solve of this precision
+  // and basis width, or decided by timing the candidates on the first
+  // restart round (autotuner)
   const int pi = B.prec == SA_FP64 ? 0 : 1;
-  if (mode == 0 && b->tuned_m[pi] == B.m) mode = b->tuned_mode[pi];   // tuned by an earlier solve
-  SA_TRY(run_sweep(B, mode == 2 ? 2 : 1, b->n, order, k0, nv, 0, eta, s));
+  Sched sc = default_sched(B);
+  bool tune = false;
+  if (b->sweep_req == 1) {
+    SA_CHECK(persistent_fits(B), "the one-workgroup-per-problem sweep handles ncv < 64");
+    sc.kind = 1;
+  } else if (b->sweep_req == 0) {
+    if (b->tuned_m[pi] == B.m) sc = b->tuned[pi];
+    else tune = true;
+  }
+  SA_TRY(run_sweep(B, sc, b->n, order, k0, nv, 0, eta, s));
   SA_HIP(hipMemcpyAsync(stop.data(), B.stop, sizeof(int) * P, hipMemcpyDeviceToHost, s));
   SA_HIP(hipStreamSynchronize(s));
-  for (int p = 0; p < P; p++) SA_CHECK(stop[p] < 0, "a start vector is zero");
+  for (int p : order) SA_CHECK(stop[p] < 0, "a start vector is zero");
   its_max = 0;
   matvecs = 0;
+  const double es = (double)prec_size(B.prec);
+  auto spmv_bytes_p = [&](int p) {
+    const double nz = (double)(b->rp[b->off[p] + b->n[p]] - b->rp[b->off[p]]);
+    return nz * (es + 4.0) + 4.0 * (b->n[p] + 1) + 2.0 * es * b->n[p];
+  };
+  if (pbytes) pbytes->assign(P, 0.0);
   while (true) {
     // ---- set up the sweep of every active problem
     std::vector<int> act;
-    for (int i = 0; i < P; i++) {
+    for (size_t i = 0; i < order.size(); i++) {
       const int p = order[i];
       ProbState &S = ps[p];
       if (!S.active) continue;
       S.its++;
-      S.nv = std::min(S.nconv + mpd, ncv);
+      S.nv = std::min(S.nconv + pncv[p], pncv[p]);
       S.k0 = S.nconv + S.l;
       S.ds.set_dimensions(S.nv, S.nconv, S.k0);
       act.push_back(p);
@@ -860,33 +983,57 @@ static int batch_ks(sa_batch_s *b, BatchDev &B, int which, int nev, int ncv, dou
     SA_HIP(hipMemcpyAsync(B.k0, k0.data(), sizeof(int) * P, hipMemcpyHostToDevice, s));
     SA_HIP(hipMemcpyAsync(B.nv, nv.data(), sizeof(int) * P, hipMemcpyHostToDevice, s));
     SA_HIP(hipMemcpyAsync(B.stop, stop.data(), sizeof(int) * P, hipMemcpyHostToDevice, s));
-    if (mode == 0) {
-      // autotune: the same sweep (it reads only columns <= k0) with both
-      // kernels, timed with events; the second run's results are kept
-      hipEvent_t ev[3];
+    if (tune) {
+      // autotune: the same sweep (it reads only columns <= k0 and rewrites
+      // the later ones) with each candidate schedule, timed with events; the
+      // fastest is run once more last so that the round's results are its
+      // own, and the pass counters are restored before that run
+      // candidates: 0 persistent, 1 lock-step, 2 lock-step with the other
+      // tile-load kind, 3 lock-step with twice the SpMV lanes per row
+      std::vector<Sched> cand;
+      std::vector<int> cslot;
+      if (persistent_fits(B)) { Sched c = sc; c.kind = 1; cand.push_back(c); cslot.push_back(0); }
+      cand.push_back(default_sched(B));
+      cslot.push_back(1);
+      { Sched c = default_sched(B); c.nt = 1 - c.nt; cand.push_back(c); cslot.push_back(2); }
+      if (B.lanes < 32) { Sched c = default_sched(B); c.lanes = 2 * B.lanes; cand.push_back(c); cslot.push_back(3); }
+      SA_HIP(hipMemcpyAsync(B.passes_save, B.passes, sizeof(int) * P, hipMemcpyDeviceToDevice, s));
+      std::vector<float> ms(cand.size(), 0.f);
+      B.graphs_on = false;   // time the kernels, not the first capture of a graph
+      hipEvent_t ev[2];
       for (auto &e : ev) SA_HIP(hipEventCreate(&e));
-      SA_HIP(hipEventRecord(ev[0], s));
-      SA_TRY(run_sweep(B, 1, b->n, act, k0, nv, 1, eta, s));
-      SA_HIP(hipEventRecord(ev[1], s));
-      SA_HIP(hipMemcpyAsync(B.stop, stop.data(), sizeof(int) * P, hipMemcpyHostToDevice, s));
-      SA_TRY(run_sweep(B, 2, b->n, act, k0, nv, 1, eta, s));
-      SA_HIP(hipEventRecord(ev[2], s));
-      SA_HIP(hipEventSynchronize(ev[2]));
-      float t1 = 0.f, t2 = 0.f;
-      SA_HIP(hipEventElapsedTime(&t1, ev[0], ev[1]));
-      SA_HIP(hipEventElapsedTime(&t2, ev[1], ev[2]));
+      for (size_t c = 0; c < cand.size(); c++) {
+        SA_HIP(hipMemcpyAsync(B.stop, stop.data(), sizeof(int) * P, hipMemcpyHostToDevice, s));
+        SA_HIP(hipEventRecord(ev[0], s));
+        SA_TRY(run_sweep(B, cand[c], b->n, act, k0, nv, 1, eta, s));
+        SA_HIP(hipEventRecord(ev[1], s));
+        SA_HIP(hipEventSynchronize(ev[1]));
+        SA_HIP(hipEventElapsedTime(&ms[c], ev[0], ev[1]));
+      }
       for (auto &e : ev) (void)hipEventDestroy(e);
-      b->ms_persistent = t1;
-      b->ms_wide = t2;
-      mode = t2 < t1 ? 2 : 1;
+      const size_t best = std::min_element(ms.begin(), ms.end()) - ms.begin();
+      for (int q = 0; q < 4; q++) b->tune_ms[q] = -1.0;
+      for (size_t c = 0; c < cand.size(); c++) b->tune_ms[cslot[c]] = ms[c];
+      b->ms_persistent = b->tune_ms[0];
+      b->ms_wide = b->tune_ms[1];
+      sc = cand[best];
+      B.graphs_on = b->graphs;
+      SA_HIP(hipMemcpyAsync(B.passes, B.passes_save, sizeof(int) * P, hipMemcpyDeviceToDevice, s));
+      SA_HIP(hipMemcpyAsync(B.stop, stop.data(), sizeof(int) * P, hipMemcpyHostToDevice, s));
+      SA_TRY(run_sweep(B, sc, b->n, act, k0, nv, 1, eta, s));
       b->tuned_m[pi] = B.m;
-      b->tuned_mode[pi] = mode;
+      b->tuned[pi] = sc;
+      tune = false;
     } else {
-      SA_TRY(run_sweep(B, mode, b->n, act, k0, nv, 1, eta, s));
+      SA_TRY(run_sweep(B, sc, b->n, act, k0, nv, 1, eta, s));
     }
-    b->sweep_used = mode;
+    b->sweep_used = sc.kind;
+    b->sched_used = sc;
+    // coefficients of the problems from the first to the last active one
+    const int plo = *std::min_element(act.begin(), act.end()), phi = *std::max_element(act.begin(), act.end());
     dbuf.resize((size_t)P * m * m);
-    SA_HIP(hipMemcpyAsync(dbuf.data(), B.dbuf, sizeof(double) * (size_t)P * m * m, hipMemcpyDeviceToHost, s));
+    SA_HIP(hipMemcpyAsync(dbuf.data() + (size_t)plo * m * m, B.dbuf + (size_t)plo * m * m,
+                          sizeof(double) * (size_t)(phi - plo + 1) * m * m, hipMemcpyDeviceToHost, s));
     SA_HIP(hipMemcpyAsync(beta.data(), B.beta, sizeof(double) * P, hipMemcpyDeviceToHost, s));
     SA_HIP(hipMemcpyAsync(stop.data(), B.stop, sizeof(int) * P, hipMemcpyDeviceToHost, s));
     SA_HIP(hipStreamSynchronize(s));
@@ -1021,8 +1168,18 @@ static int batch_ks(sa_batch_s *b, BatchDev &B, int which, int nev, int ncv, dou
     for (int p = 0; p < P; p++)
       if (!ps[p].active) { ml[p] = mk[p] = msv[p] = mev[p] = 0; src[p] = -1; dst[p] = 0; }
     for (int p : act) matvecs += ps[p].nv - ps[p].k0;
+    if (pbytes)
+      for (int p : act) {
+        const ProbState &S = ps[p];
+        double bb = 0.0;
+        for (int c = S.k0 + 1; c <= S.nv; c++) bb += 2.0 * (2.0 * c + 2.0) * es * b->n[p] + spmv_bytes_p(p);
+        bb += ((double)(S.nv - S.nconv) + (double)(S.k + S.l - S.nconv)) * es * b->n[p];
+        if (trueres) bb += (double)(S.k - S.nconv + 1) * (((double)S.nv + 1.0) * es * b->n[p] + spmv_bytes_p(p));
+        (*pbytes)[p] += bb;
+      }
     tds += nowb() - t0;
-    SA_HIP(hipMemcpyAsync(B.Q, Qh.data(), sizeof(double) * (size_t)P * m * m, hipMemcpyHostToDevice, s));
+    SA_HIP(hipMemcpyAsync(B.Q + (size_t)plo * m * m, Qh.data() + (size_t)plo * m * m,
+                          sizeof(double) * (size_t)(phi - plo + 1) * m * m, hipMemcpyHostToDevice, s));
     for (auto pr : {std::make_pair(B.ml, &ml), std::make_pair(B.mk, &mk), std::make_pair(B.ms, &msv),
                     std::make_pair(B.me, &mev), std::make_pair(B.src, &src), std::make_pair(B.dst, &dst)})
       SA_HIP(hipMemcpyAsync(pr.first, pr.second->data(), sizeof(int) * P, hipMemcpyHostToDevice, s));
@@ -1062,7 +1219,7 @@ static int batch_ks(sa_batch_s *b, BatchDev &B, int which, int nev, int ncv, dou
       SA_HIP(hipMemcpyAsync(B.order, rs.data(), sizeof(int) * rs.size(), hipMemcpyHostToDevice, s));
       SA_HIP(hipMemcpyAsync(B.k0, k0.data(), sizeof(int) * P, hipMemcpyHostToDevice, s));
       SA_HIP(hipMemcpyAsync(B.stop, st2.data(), sizeof(int) * P, hipMemcpyHostToDevice, s));
-      SA_TRY(run_sweep(B, mode, b->n, rs, k0, nv, 0, eta, s));
+      SA_TRY(run_sweep(B, sc, b->n, rs, k0, nv, 0, eta, s));
       SA_HIP(hipMemcpyAsync(st2.data(), B.stop, sizeof(int) * P, hipMemcpyDeviceToHost, s));
       SA_HIP(hipStreamSynchronize(s));
       for (int p : rs)
@@ -1163,8 +1320,17 @@ int sa_batch_destroy(sa_batch b) {
 int sa_batch_solve(sa_batch b, int which, int nev, int ncv, double tol, int max_it, int mixed, double tol32,
                    const double *v0, uint64_t seed) {
   SA_TRY(ensure_init());
-  SA_CHECK(b && nev > 0 && ncv > nev && ncv <= 64 && tol > 0, "invalid batch solve arguments (ncv <= 64)");
-  const int P = b->P, m = ncv + 1;
+  SA_CHECK(b && nev > 0 && ncv >= 0 && tol > 0, "invalid batch solve arguments");
+  const int P = b->P;
+  // basis size of each problem: ncv, or (ncv == 0) the autotuner's choice,
+  // or EPS's default max(2 nev, nev + 15) capped at n
+  std::vector<int> pncv(P);
+  for (int p = 0; p < P; p++) {
+    pncv[p] = ncv > 0 ? ncv : (!b->pncv.empty() ? b->pncv[p] : std::min(b->n[p], std::max(2 * nev, nev + 15)));
+    SA_CHECK(pncv[p] > nev && pncv[p] <= 64 && pncv[p] <= b->n[p], "invalid ncv: need nev < ncv <= min(64, n)");
+  }
+  b->ncv_used = pncv;
+  const int m = *std::max_element(pncv.begin(), pncv.end()) + 1;
   hipStream_t s = stream();
   if (max_it <= 0) max_it = 100000;
   const double t0 = nowb();
@@ -1190,7 +1356,7 @@ int sa_batch_solve(sa_batch b, int which, int nev, int ncv, double tol, int max_
     SA_HIP(hipMemcpyAsync(b->dev32.V, f.data(), sizeof(float) * b->nrows, hipMemcpyHostToDevice, s));
     SA_HIP(hipStreamSynchronize(s));
     const double t1 = nowb();
-    SA_TRY(batch_ks(b, b->dev32, which, nev, ncv, tol32, max_it, b->trueres != 0, ps, b->matvecs32, b->passes32,
+    SA_TRY(batch_ks(b, b->dev32, which, nev, pncv, tol32, max_it, b->trueres != 0, ps, b->matvecs32, b->passes32,
                     b->its_max32, tds));
     // polish seeds: sum of the first nev Ritz vectors of each problem, in FP64
     SA_TRY(launch_convert(SA_FP32, b->dev32.V, b->dev32.ldv, SA_FP64, b->dev64.V, b->dev64.ldv, b->nrows, m, s));
@@ -1219,7 +1385,7 @@ int sa_batch_solve(sa_batch b, int which, int nev, int ncv, double tol, int max_
     SA_HIP(hipMemcpyAsync(b->dev64.V, x0.data(), sizeof(double) * b->nrows, hipMemcpyHostToDevice, s));
     SA_HIP(hipStreamSynchronize(s));
   }
-  SA_TRY(batch_ks(b, b->dev64, which, nev, ncv, tol, max_it, b->trueres != 0, ps, b->matvecs, b->passes, b->its_max,
+  SA_TRY(batch_ks(b, b->dev64, which, nev, pncv, tol, max_it, b->trueres != 0, ps, b->matvecs, b->passes, b->its_max,
                   tds));
   b->nconv.resize(P);
   b->its.resize(P);
@@ -1342,6 +1508,158 @@ int sa_batch_get_sweep(sa_batch b, int *mode, double *ms_persistent, double *ms_
   if (mode) *mode = b->sweep_used;
   if (ms_persistent) *ms_persistent = b->ms_persistent;
   if (ms_wide) *ms_wide = b->ms_wide;
+  return 0;
+}
+
+int sa_batch_get_schedule(sa_batch b, int *kind, int *nt, int *lanes, double *ms4) {
+  SA_CHECK(b, "null batch");
+  if (kind) *kind = b->sched_used.kind;
+  if (nt) *nt = b->sched_used.nt;
+  if (lanes) *lanes = b->sched_used.lanes;
+  if (ms4)
+    for (int q = 0; q < 4; q++) ms4[q] = b->tune_ms[q];
+  return 0;
+}
+
+int sa_batch_set_graphs(sa_batch b, int on) {
+  SA_CHECK(b, "null batch");
+  b->graphs = on != 0;
+  return 0;
+}
+
+int sa_batch_get_graph_stats(sa_batch b, int64_t *captures, int64_t *replays, int *cached) {
+  SA_CHECK(b, "null batch");
+  const BatchDev *Bs[2] = {&b->dev64, &b->dev32};
+  int64_t c = 0, r = 0;
+  int k = 0;
+  for (const BatchDev *B : Bs) {
+    c += (int64_t)B->graph_captures;
+    r += (int64_t)B->graph_replays;
+    k += (int)B->graphs.size();
+  }
+  if (captures) *captures = c;
+  if (replays) *replays = r;
+  if (cached) *cached = k;
+  return 0;
+}
+
+int sa_batch_get_ncv(sa_batch b, int *ncv) {
+  SA_CHECK(b && ncv && b->solved, "batch not solved");
+  std::copy(b->ncv_used.begin(), b->ncv_used.end(), ncv);
+  return 0;
+}
+
+// Autotuner of the basis size: problems sorted by size are split into nclass
+// classes of (nearly) equal count and nsample problems spread over each class
+// form one stratified sample.  The sample is solved as a sub-batch with every
+// candidate ncv (FP64, lock-step sweeps, this batch's true-residual setting,
+// start vectors U[-1,1) with seed sa_sub_seed(seed, p)); each candidate's
+// wall time is split over the classes in proportion to their problems'
+// algorithmic bytes (lock-step kernels run all problems' chunks together, so
+// a problem's share of the time is its share of the traffic), and the
+// candidate with the least time per converged pair becomes the ncv of every
+// problem of the class.  Later sa_batch_solve calls with ncv == 0 use it.
+int sa_batch_autotune_ncv(sa_batch b, int which, int nev, double tol, const int *cand, int ncand, int nclass,
+                          int nsample, uint64_t seed, int *ncv_out) {
+  SA_TRY(ensure_init());
+  SA_CHECK(b && cand && ncand > 0 && ncand <= 16 && nclass > 0 && nsample > 0 && nev > 0 && tol > 0,
+           "invalid autotune arguments");
+  const int P = b->P;
+  nclass = std::min(nclass, P);
+  int maxc = 0;
+  for (int i = 0; i < ncand; i++) {
+    SA_CHECK(cand[i] > nev && cand[i] <= 64, "ncv candidates must satisfy nev < ncv <= 64");
+    maxc = std::max(maxc, cand[i]);
+  }
+  std::vector<int> idx(P);
+  std::iota(idx.begin(), idx.end(), 0);
+  std::stable_sort(idx.begin(), idx.end(), [&](int x, int y) { return b->n[x] < b->n[y]; });
+  // classes and the stratified sample
+  std::vector<int> cls(P, 0), hi(nclass, 0);
+  std::vector<char> sub(P, 0);
+  int nmin = INT32_MAX;
+  for (int c = 0; c < nclass; c++) {
+    const int lo = (int)((int64_t)c * P / nclass), up = (int)((int64_t)(c + 1) * P / nclass), cnt = up - lo;
+    hi[c] = b->n[idx[up - 1]];
+    for (int i = lo; i < up; i++) cls[idx[i]] = c;
+    const int ns = std::min(nsample, cnt);
+    for (int i = 0; i < ns; i++) {
+      const int p = idx[lo + (int)(((int64_t)2 * i + 1) * cnt / (2 * ns))];
+      sub[p] = 1;
+      nmin = std::min(nmin, b->n[p]);
+    }
+  }
+  hipStream_t s = stream();
+  SA_TRY(alloc_dev(b, b->dev64, SA_FP64, maxc + 1));
+  BatchDev &B = b->dev64;
+  std::vector<double> x0((size_t)b->nrows, 0.0);
+  for (int p = 0; p < P; p++) sa_rand_uniform(sa_sub_seed(seed, p), b->n[p], x0.data() + b->off[p]);
+  const int sweep_req = b->sweep_req;
+  const bool graphs = b->graphs;
+  b->sweep_req = 2;
+  b->graphs = false;   // time the kernels, not first captures
+  std::vector<double> scores((size_t)nclass * ncand, 1e300), pbytes;
+  std::vector<ProbState> ps;
+  int rc = 0;
+  for (int q = 0; q < ncand && !rc; q++) {
+    if (cand[q] > nmin) continue;
+    std::vector<int> pncv(P, cand[q]);
+    if (hipMemcpyAsync(B.V, x0.data(), sizeof(double) * b->nrows, hipMemcpyHostToDevice, s) != hipSuccess ||
+        hipStreamSynchronize(s) != hipSuccess) {
+      set_error("autotune: start vector upload failed");
+      rc = 1;
+      break;
+    }
+    int64_t mv = 0, pa = 0;
+    int itm = 0;
+    double tds = 0.0;
+    const double t0 = nowb();
+    rc = batch_ks(b, B, which, nev, pncv, tol, 100000, b->trueres != 0, ps, mv, pa, itm, tds, &sub, &pbytes);
+    const double dt = nowb() - t0;
+    if (rc) break;
+    double tot = 0.0;
+    std::vector<double> cb(nclass, 0.0);
+    std::vector<int> cpairs(nclass, 0), cwant(nclass, 0);
+    for (int p = 0; p < P; p++)
+      if (sub[p]) {
+        tot += pbytes[p];
+        cb[cls[p]] += pbytes[p];
+        cpairs[cls[p]] += std::min(nev, ps[p].nconv);
+        cwant[cls[p]] += nev;
+      }
+    for (int c = 0; c < nclass; c++)
+      if (cwant[c] > 0 && cpairs[c] == cwant[c] && tot > 0.0)
+        scores[(size_t)c * ncand + q] = dt * cb[c] / tot / cpairs[c];
+  }
+  b->sweep_req = sweep_req;
+  b->graphs = graphs;
+  if (rc) return rc;
+  b->pncv.assign(P, 0);
+  for (int c = 0; c < nclass; c++) {
+    int best = 0;
+    for (int q = 1; q < ncand; q++)
+      if (scores[(size_t)c * ncand + q] < scores[(size_t)c * ncand + best]) best = q;
+    SA_CHECK(scores[(size_t)c * ncand + best] < 1e300, "autotune: no candidate converged on a size class");
+    for (int p = 0; p < P; p++)
+      if (cls[p] == c) b->pncv[p] = std::min(cand[best], b->n[p]);
+  }
+  b->ncv_scores = scores;
+  b->ncv_cand.assign(cand, cand + ncand);
+  b->class_hi = hi;
+  if (ncv_out) std::copy(b->pncv.begin(), b->pncv.end(), ncv_out);
+  return 0;
+}
+
+// The autotuner's measurements: for class c (problems with n <= class_hi[c]),
+// scores[c * ncand + q] = seconds per converged pair with candidate q (1e300:
+// not run or not all pairs converged).
+int sa_batch_get_ncv_tuning(sa_batch b, int *nclass, int *ncand, int *class_hi, double *scores) {
+  SA_CHECK(b && !b->pncv.empty(), "the ncv autotuner has not run");
+  const int nc = (int)b->class_hi.size(), nq = (int)b->ncv_cand.size();
+  if (nclass) *nclass = nc;
+  if (ncand) *ncand = nq;
+  if (class_hi) std::copy(b->class_hi.begin(), b->class_hi.end(), class_hi);
+  if (scores) std::copy(b->ncv_scores.begin(), b->ncv_scores.end(), scores);
   return 0;
 }
 

@@ -244,7 +244,7 @@ __global__ __launch_bounds__(BLK) void multvec_kernel(int64_t n, int64_t ld, con
   if (r >= n) return;
   T s = (T)0;
   for (int i = 0; i < ncol; i++) s += V[(size_t)i * ld + r] * qs[i];
-  y[r] = (T)beta * y[r] + (T)alpha * s;
+  y[r] = (beta == 0.0 ? (T)0 : (T)beta * y[r]) + (T)alpha * s;   // beta 0: y is not read (gemv)
 }
 
 int launch_multvec(BV &V, int l, int k, double alpha, const double *q_dev, double beta, void *y,

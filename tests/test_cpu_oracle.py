@@ -845,3 +845,14 @@ def test_oracle_true_residual_is_the_residual_of_the_returned_vector(orc, case):
             assert np.all(diffs[tr] < 1e-14)
     if case in ("convdiff10", "lap2d_pot"):
         assert diffs[False].max() > 1e-14
+
+
+def test_oracle_multvec_beta_zero_does_not_read_y(orc):
+    """BVMultVec with beta = 0 (gemv semantics, bvblas.c:56-67): y is not read,
+    so NaN garbage in it cannot reach the result (the true-residual branch
+    forms its Ritz vector into uninitialised memory this way)."""
+    n, k = 1500, 5
+    V = np.random.default_rng(1).standard_normal((n, k))
+    q = np.arange(1.0, k + 1)
+    y = orc.bv_multvec(V, 2.0, q, 0.0, np.full(n, np.nan))
+    assert np.allclose(y, 2.0 * V @ q, rtol=1e-14, atol=1e-13)

@@ -160,6 +160,21 @@ def test_bv_test1_golden(sa):
         assert abs(X.norm() - g["normF_X"]) < 5e-5
 
 
+def test_bv_multvec_beta_zero_does_not_read_y(sa):
+    """BVMultVec with beta = 0 follows gemv (bvblas.c:56-67): y is not read, so
+    NaN in it cannot reach the result."""
+    n, k = 5000, 6
+    V = sa.BV(n, k)
+    X = np.random.default_rng(2).standard_normal((n, k))
+    V.set_matrix(X)
+    V.set_active_columns(0, k)
+    q = np.arange(1.0, k + 1)
+    y = sa.DeviceArray.from_host(np.full(n, np.nan))
+    V.multvec(2.0, 0.0, y.ptr, q)
+    got = y.to_host(np.float64, n)
+    assert np.allclose(got, 2.0 * X @ q, rtol=1e-13, atol=1e-12)
+
+
 def test_bv_orthogonality_level(sa):
     """test2.c criterion: CGS with refinement gives ||X^T X - I||_1 < 100 eps."""
     n, k = 20, 8
