@@ -169,6 +169,21 @@ int sa_bv_mat_mult_column(sa_bv V, sa_mat A, int j);
 /* BVOrthonormalizeColumn (bvorthog.c:380-427) with classical Gram-Schmidt and
    DGKS refinement "if needed" (bvorthog.c:91-217), eta as in BVSetOrthogonalization.
    Coefficients land in column j of the coefficient buffer (norm at row j). */
+/* BVSetOrthogonalization's refinement (src/sys/classes/bv/interface/bvbasic.c;
+ * BVOrthogonalizeGS bvorthog.c:172-199): SA_REFINE_IFNEEDED (default: DGKS,
+ * repeat while the estimated norm falls below eta times the norm before the
+ * pass, at most three passes), SA_REFINE_NEVER (one pass, then the true norm;
+ * lindep only for a zero norm), SA_REFINE_ALWAYS (two passes).  Applies to
+ * sa_bv_orthonormalize_column, sa_bv_mat_arnoldi and
+ * sa_bv_orthogonalize_some_column of this basis. */
+enum { SA_REFINE_IFNEEDED = 0, SA_REFINE_NEVER = 1, SA_REFINE_ALWAYS = 2 };
+int sa_bv_set_orthogonalization(sa_bv V, int refine);
+/* BVOrthogonalizeSomeColumn (src/sys/classes/bv/interface/bvorthog.c:317-378):
+ * modified Gram-Schmidt of column j against the columns i < j with which[i]
+ * != 0 (which: j host ints), with the basis's refinement; h (optional, host,
+ * j+1 entries) receives the coefficients with the norm at index j. */
+int sa_bv_orthogonalize_some_column(sa_bv V, int j, const int *which, double eta, double *h, double *norm,
+                                    int *lindep);
 int sa_bv_orthonormalize_column(sa_bv V, int j, double eta, double *norm, int *lindep);
 /* Copy columns [c0,c1) of the m x m coefficient buffer to the host (ld m). */
 int sa_bv_get_coefficients(sa_bv V, int c0, int c1, double *buf_host);
@@ -392,6 +407,12 @@ typedef int (*sa_eps_monitor_fn)(sa_eps eps, int its, int nconv, const double *e
                                  const double *errest, int nest, void *ctx);
 int sa_eps_monitor_set(sa_eps eps, sa_eps_monitor_fn monitor, void *ctx);
 int sa_eps_monitor_cancel(sa_eps eps);
+/* EPSGetBV + BVSetOrthogonalization for the solver's basis (-bv_orthog_refine,
+ * -bv_orthog_eta): refinement SA_REFINE_* and eta (<= 0 keeps 0.7071).  With
+ * SA_REFINE_NEVER the delayed variants (EPSArnoldiSetDelayed,
+ * -eps_lanczos_reorthog delayed) run EPSDelayedArnoldi1 (epskrylov.c:120-157)
+ * as arnoldi.c:84-87 and lanczos.c:500 select it. */
+int sa_eps_set_orthogonalization(sa_eps eps, int refine, double eta);
 /* EPSSetStoppingTestFunction (src/eps/interface/epsopts.c:863-885): replaces
  * EPSStoppingBasic (src/eps/interface/epsdefault.c:290-302).  Called once per
  * outer iteration after the convergence test (krylovschur.c:284) with the

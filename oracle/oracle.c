@@ -476,19 +476,34 @@ static void cgs1(int64_t n, double *V, int64_t ld, int j, double *buf, int m, do
   for (int i = 0; i < j; i++) h[i] += c[i];        /* BV_AddCoefficients */
 }
 
-static void orthogonalize_gs(int64_t n, double *V, int64_t ld, int j, double *buf, int m, double eta,
+/* refine (BVSetOrthogonalization, bvorthog.c:172-199): ORC_REFINE_IFNEEDED
+   repeats the pass while ||v'|| < eta ||v||, at most three passes;
+   ORC_REFINE_NEVER takes one pass and the true norm after it (lindep only
+   for a zero norm); ORC_REFINE_ALWAYS takes two passes, the second with the
+   norm estimate. */
+static void orthogonalize_gs(int64_t n, double *V, int64_t ld, int j, double *buf, int m, double eta, int refine,
                              double *norm, int *lindep, int64_t *passes)
 {
   double onrm = 0.0, nrm = 0.0;
   double *h = buf + (int64_t)j * m;
   for (int i = 0; i < j; i++) h[i] = 0.0;           /* BV_CleanCoefficients */
-  cgs1(n, V, ld, j, buf, m, &onrm, &nrm, passes);
-  int l = 1;
-  while (l < 3 && nrm != 0.0 && fabs(nrm) < eta * fabs(onrm)) {
-    l++;
+  if (refine == ORC_REFINE_NEVER) {
+    cgs1(n, V, ld, j, buf, m, NULL, NULL, passes);
+    nrm = orc_bv_norm2(n, V + (int64_t)j * ld);     /* BV_NormVecOrColumn */
+    *lindep = !(nrm != 0.0);
+  } else if (refine == ORC_REFINE_ALWAYS) {
+    cgs1(n, V, ld, j, buf, m, NULL, NULL, passes);
     cgs1(n, V, ld, j, buf, m, &onrm, &nrm, passes);
+    *lindep = !(nrm != 0.0 && fabs(nrm) >= eta * fabs(onrm));
+  } else {
+    cgs1(n, V, ld, j, buf, m, &onrm, &nrm, passes);
+    int l = 1;
+    while (l < 3 && nrm != 0.0 && fabs(nrm) < eta * fabs(onrm)) {
+      l++;
+      cgs1(n, V, ld, j, buf, m, &onrm, &nrm, passes);
+    }
+    *lindep = !(nrm != 0.0 && fabs(nrm) >= eta * fabs(onrm));
   }
-  *lindep = !(nrm != 0.0 && fabs(nrm) >= eta * fabs(onrm));
   *norm = nrm;
   h[j] = *lindep ? 0.0 : nrm;                       /* BV_SetValue(k,k,h,nrm) */
 }
@@ -514,23 +529,50 @@ static void mgs1(int64_t n, double *V, int64_t ld, int j, const char *which, dou
 /* BVOrthogonalizeGS (bvorthog.c:145-215) with BV_ORTHOG_MGS and refinement
    IFNEEDED: repeat while ||v|| < eta ||v_before||, at most three passes. */
 static void orthogonalize_mgs(int64_t n, double *V, int64_t ld, int j, const char *which, double *buf, int m,
-                              double eta, double *norm, int *lindep, int64_t *passes)
+                              double eta, int refine, double *norm, int *lindep, int64_t *passes)
 {
   double onrm = 0.0, nrm = 0.0;
   double *h = buf + (int64_t)j * m;
   for (int i = 0; i < j; i++) h[i] = 0.0;
-  mgs1(n, V, ld, j, which, h, &onrm, &nrm);
-  (*passes)++;
-  int l = 1;
-  while (l < 3 && nrm != 0.0 && fabs(nrm) < eta * fabs(onrm)) {
-    l++;
-    onrm = nrm;
-    mgs1(n, V, ld, j, which, h, NULL, &nrm);
+  if (refine == ORC_REFINE_NEVER) {                 /* bvorthog.c:187-193 */
+    mgs1(n, V, ld, j, which, h, NULL, NULL);
     (*passes)++;
+    nrm = orc_bv_norm2(n, V + (int64_t)j * ld);
+    *lindep = !(nrm != 0.0);
+  } else if (refine == ORC_REFINE_ALWAYS) {         /* bvorthog.c:195-199 */
+    mgs1(n, V, ld, j, which, h, NULL, NULL);
+    mgs1(n, V, ld, j, which, h, &onrm, &nrm);
+    (*passes) += 2;
+    *lindep = !(nrm != 0.0 && fabs(nrm) >= eta * fabs(onrm));
+  } else {
+    mgs1(n, V, ld, j, which, h, &onrm, &nrm);
+    (*passes)++;
+    int l = 1;
+    while (l < 3 && nrm != 0.0 && fabs(nrm) < eta * fabs(onrm)) {
+      l++;
+      onrm = nrm;
+      mgs1(n, V, ld, j, which, h, NULL, &nrm);
+      (*passes)++;
+    }
+    *lindep = !(nrm != 0.0 && fabs(nrm) >= eta * fabs(onrm));
   }
-  *lindep = !(nrm != 0.0 && fabs(nrm) >= eta * fabs(onrm));
   *norm = nrm;
   h[j] = *lindep ? 0.0 : nrm;
+}
+
+/* BVOrthogonalizeSomeColumn (bvorthog.c:317-378, MGS): column j against the
+   columns i < j with which[i] set, with refinement refine; coefficients into
+   column j of the m x m buffer, the norm at row j. */
+int orc_bv_orthogonalize_some_column(int64_t n, double *V, int64_t ld, int j, const char *which, double *buffer,
+                                     int m, int refine, double eta, double *norm, int *lindep)
+{
+  int64_t passes = 0;
+  double nrm = 0.0;
+  int ld_ = 0;
+  orthogonalize_mgs(n, V, ld, j, which, buffer, m, eta, refine, &nrm, &ld_, &passes);
+  if (norm) *norm = nrm;
+  if (lindep) *lindep = ld_;
+  return (int)passes;
 }
 
 int orc_bv_orthonormalize_column(int64_t n, double *V, int64_t ld, int j, double *buffer, int m, double eta,
@@ -539,7 +581,7 @@ int orc_bv_orthonormalize_column(int64_t n, double *V, int64_t ld, int j, double
   int64_t passes = 0;
   double nrm;
   int ld_ = 0;
-  orthogonalize_gs(n, V, ld, j, buffer, m, eta, &nrm, &ld_, &passes);
+  orthogonalize_gs(n, V, ld, j, buffer, m, eta, ORC_REFINE_IFNEEDED, &nrm, &ld_, &passes);
   if (nrm != 1.0 && nrm != 0.0) orc_bv_scale(n, V + (int64_t)j * ld, 1.0 / nrm);
   if (norm) *norm = nrm;
   if (lindep) *lindep = ld_;
@@ -1029,6 +1071,7 @@ typedef struct {
   const int *piv;
   int mgs;                       /* BV_ORTHOG_MGS (EPSLanczos other than full, lanczos.c:58-62) */
   char *which;                   /* column mask of the local Lanczos sweep, m entries */
+  int refine;                    /* BVSetOrthogonalization refinement: ORC_REFINE_* */
 } KSo;
 
 /* Orthonormalise column i against the columns before it (BVOrthonormalizeColumn,
@@ -1036,8 +1079,8 @@ typedef struct {
    norm is 0 or 1. */
 static void ks_orthonormalize(KSo *ks, int i, double *norm, int *lindep)
 {
-  if (ks->mgs) orthogonalize_mgs(ks->n, ks->V, ks->ld, i, NULL, ks->buf, ks->m, ks->eta, norm, lindep, &ks->passes);
-  else orthogonalize_gs(ks->n, ks->V, ks->ld, i, ks->buf, ks->m, ks->eta, norm, lindep, &ks->passes);
+  if (ks->mgs) orthogonalize_mgs(ks->n, ks->V, ks->ld, i, NULL, ks->buf, ks->m, ks->eta, ks->refine, norm, lindep, &ks->passes);
+  else orthogonalize_gs(ks->n, ks->V, ks->ld, i, ks->buf, ks->m, ks->eta, ks->refine, norm, lindep, &ks->passes);
 }
 
 /* EPSGetStartVector (epssolve.c:841-873) with a caller-provided vector for i==0
@@ -1099,7 +1142,7 @@ static void ks_expand_local(KSo *ks, int k, int *m, double *beta, int *breakdown
     which[j] = 1;
     if (j - 2 >= k) which[j - 2] = 0;
     double nrm;
-    orthogonalize_mgs(ks->n, ks->V, ks->ld, j + 1, which, ks->buf, ks->m, ks->eta, &nrm, &lindep, &ks->passes);
+    orthogonalize_mgs(ks->n, ks->V, ks->ld, j + 1, which, ks->buf, ks->m, ks->eta, ks->refine, &nrm, &lindep, &ks->passes);
     ks->buf[j + 1 + (int64_t)(j + 1) * ks->m] = nrm;
     *beta = nrm;
     if (lindep) { *m = j + 1; break; }
@@ -1183,12 +1226,12 @@ static void ks_expand_partial(KSo *ks, int k, int *M, double *betam, int *breakd
     ks_op(ks, ks->V + (int64_t)j * ks->ld, ks->V + (int64_t)(j + 1) * ks->ld);
     const double *h = ks->buf + (int64_t)(j + 1) * ks->m;
     if (fro) {
-      orthogonalize_mgs(n, ks->V, ks->ld, j + 1, NULL, ks->buf, ks->m, ks->eta, &norm, &lindep, &ks->passes);
+      orthogonalize_mgs(n, ks->V, ks->ld, j + 1, NULL, ks->buf, ks->m, ks->eta, ks->refine, &norm, &lindep, &ks->passes);
       alpha[j] = h[j];
     } else {
       which[j] = 1;
       if (j - 2 >= k) which[j - 2] = 0;
-      orthogonalize_mgs(n, ks->V, ks->ld, j + 1, which, ks->buf, ks->m, ks->eta, &norm, &lindep, &ks->passes);
+      orthogonalize_mgs(n, ks->V, ks->ld, j + 1, which, ks->buf, ks->m, ks->eta, ks->refine, &norm, &lindep, &ks->passes);
       alpha[j] = h[j];
       beta[j] = norm;
       if (estimate) {
@@ -1213,7 +1256,7 @@ static void ks_expand_partial(KSo *ks, int k, int *M, double *betam, int *breakd
           compute_int(which2 + k, omega, j - k, delta, eta);
           for (int i = 0; i < j - k; i++) if (which2[i + k]) omega[i] = eps1;
         }
-        orthogonalize_mgs(n, ks->V, ks->ld, j + 1, which2, ks->buf, ks->m, ks->eta, &norm, &lindep, &ks->passes);
+        orthogonalize_mgs(n, ks->V, ks->ld, j + 1, which2, ks->buf, ks->m, ks->eta, ks->refine, &norm, &lindep, &ks->passes);
       }
     }
     if (lindep || norm < n * anorm * meps) { *M = j + 1; break; }
@@ -1273,7 +1316,7 @@ static int ks_expand_selective(KSo *ks, int k, int *M, double *betam, int *break
     ks_op(ks, ks->V + (int64_t)j * ks->ld, vj1);
     which[j] = 1;
     if (j - 2 >= k) which[j - 2] = 0;
-    orthogonalize_mgs(n, ks->V, ks->ld, j + 1, which, ks->buf, ks->m, ks->eta, &norm, &lindep, &ks->passes);
+    orthogonalize_mgs(n, ks->V, ks->ld, j + 1, which, ks->buf, ks->m, ks->eta, ks->refine, &norm, &lindep, &ks->passes);
     alpha[j] = ks->buf[j + (int64_t)(j + 1) * ks->m];
     beta[j] = norm;
     if (lindep) { *M = j + 1; break; }
@@ -1364,6 +1407,38 @@ static void ks_expand_delayed(KSo *ks, int k, int m, double *H, int ldh, double 
   free(u); free(t); free(lhh);
 }
 
+/* EPSDelayedArnoldi1 (epskrylov.c:120-157; arnoldi.c:84-87 and lanczos.c:500
+   with refinement NEVER): EPSDelayedArnoldi without reorthogonalisation, the
+   normalisation alone delayed by one step.  At step j > k the dot products of
+   v_{j+1} with v_0..v_j are taken before v_j is normalised and rescaled by its
+   norm (by its square for the diagonal entry); v_j and v_{j+1} are scaled by
+   1/||v_j||, H(j,j-1) = ||v_j|| and beta = ||v_j||; v_{j+1} -= V(:,0:j+1) H(:,j).
+   The last column is left unnormalised and no breakdown is reported. */
+static void ks_expand_delayed1(KSo *ks, int k, int m, double *H, int ldh, double *betam)
+{
+  const int64_t N = ks->n, ld = ks->ld;
+  double *V = ks->V;
+#define COL(c) (V + (int64_t)(c) * ld)
+  for (int j = k; j < m; j++) {
+    ks_op(ks, COL(j), COL(j + 1));
+    double *Hj = H + (int64_t)ldh * j;
+    orc_bv_dotvec(N, j + 1, V, ld, COL(j + 1), Hj);                  /* BVDotColumn(V, j+1) */
+    if (j > k) {
+      double dot = 0.0;
+      for (int64_t r = 0; r < N; r++) dot += COL(j)[r] * COL(j)[r];    /* VecDot(vj, vj) */
+      const double norm = sqrt(dot);
+      orc_bv_scale(N, COL(j), 1.0 / norm);
+      H[(int64_t)ldh * (j - 1) + j] = norm;
+      for (int i = 0; i < j; i++) Hj[i] /= norm;
+      Hj[j] /= dot;
+      orc_bv_scale(N, COL(j + 1), 1.0 / norm);
+      *betam = norm;
+    }
+    orc_bv_multvec(N, j + 1, -1.0, V, ld, Hj, 1.0, COL(j + 1));      /* BVMultColumn(V, j+1) */
+  }
+#undef COL
+}
+
 /* One local Lanczos sweep from v0 (test entry: exposes ks_expand_local):
    V (n x (m+1)) gets the basis, alpha/beta (m entries) the tridiagonal.
    Returns the number of steps taken. */
@@ -1394,7 +1469,7 @@ static void ks_expand(KSo *ks, int k, int *m, double *beta, int *breakdown)
   for (int j = k; j < *m; j++) {
     ks_op(ks, ks->V + (int64_t)j * ks->ld, ks->V + (int64_t)(j + 1) * ks->ld);
     double nrm;
-    orthogonalize_gs(ks->n, ks->V, ks->ld, j + 1, ks->buf, ks->m, ks->eta, &nrm, &lindep, &ks->passes);
+    orthogonalize_gs(ks->n, ks->V, ks->ld, j + 1, ks->buf, ks->m, ks->eta, ks->refine, &nrm, &lindep, &ks->passes);
     if (nrm != 1.0 && nrm != 0.0) orc_bv_scale(ks->n, ks->V + (int64_t)(j + 1) * ks->ld, 1.0 / nrm);
     *beta = nrm;
     if (lindep) { *m = j + 1; break; }
@@ -1506,6 +1581,7 @@ int orc_eps_solve(int n, const int32_t *rp, const int32_t *ci, const double *val
   KSo ks = {0};
   ks.n = n; ks.rp = rp; ks.ci = ci; ks.val = val; ks.m = ncv + 1; ks.ld = n;
   ks.eta = o->eta > 0 ? o->eta : 0.7071;
+  ks.refine = o->refine;
   ks.lu = lu; ks.piv = piv;
   ks.mgs = lanc && (o->lanczos_local || o->lanczos_partial);
   /* selective: the Ritz vectors' basis AV (BVDuplicate of V, lanczos.c:63-71) */
@@ -1555,7 +1631,8 @@ int orc_eps_solve(int n, const int32_t *rp, const int32_t *ci, const double *val
     } else if (partial == 4) {                          /* lanczos.c:495-512 */
       if (nv - k0 < 2) { rc = -14; goto done; }
       double *Td = (double *)calloc((size_t)nv * nv, sizeof(double));
-      ks_expand_delayed(&ks, k0, nv, Td, nv, &beta);
+      if (o->refine == ORC_REFINE_NEVER) ks_expand_delayed1(&ks, k0, nv, Td, nv, &beta);   /* lanczos.c:500 */
+      else ks_expand_delayed(&ks, k0, nv, Td, nv, &beta);
       breakdown = 0;
       for (int i = k0; i < nv - 1; i++) { d[i] = Td[(int64_t)nv * i + i]; e[i] = Td[(int64_t)nv * i + i + 1]; }
       d[nv - 1] = Td[(int64_t)nv * (nv - 1) + nv - 1];
@@ -1564,7 +1641,8 @@ int orc_eps_solve(int n, const int32_t *rp, const int32_t *ci, const double *val
     } else if (partial) ks_expand_partial(&ks, k0, &nv, &beta, &breakdown, anorm, o->lanczos_partial == 1, d, e);
     else if (arn && o->arnoldi_delayed) {                /* arnoldi.c:88-91 */
       if (nv - k0 < 2) { rc = -14; goto done; }
-      ks_expand_delayed(&ks, k0, nv, ds.A, ld, &beta);
+      if (o->refine == ORC_REFINE_NEVER) ks_expand_delayed1(&ks, k0, nv, ds.A, ld, &beta);   /* arnoldi.c:84-87 */
+      else ks_expand_delayed(&ks, k0, nv, ds.A, ld, &beta);
       breakdown = 0;
     }
     else if (ks.mgs) ks_expand_local(&ks, k0, &nv, &beta, &breakdown);
@@ -1942,7 +2020,7 @@ int orc_eps_solve_twosided(int n, const int32_t *rp, const int32_t *ci, const do
   free(nxt);
   const int ld = ncv + 1;
   KSo ks = {0}, kw = {0};
-  ks.n = n; ks.rp = rp; ks.ci = ci; ks.val = val; ks.m = ncv + 1; ks.ld = n; ks.eta = o->eta > 0 ? o->eta : 0.7071;
+  ks.n = n; ks.rp = rp; ks.ci = ci; ks.val = val; ks.m = ncv + 1; ks.ld = n; ks.eta = o->eta > 0 ? o->eta : 0.7071; ks.refine = o->refine;
   kw = ks;
   kw.rp = trp; kw.ci = tci; kw.val = tva; kw.seed = 0x1EF7ULL;
   ks.V = (double *)calloc((size_t)n * ks.m, sizeof(double)); ks.buf = (double *)calloc((size_t)ks.m * ks.m, sizeof(double));
@@ -1968,7 +2046,7 @@ int orc_eps_solve_twosided(int n, const int32_t *rp, const int32_t *ci, const do
   if (v0) {
     memcpy(ks.V, v0, sizeof(double) * (size_t)n);
     double norm; int lindep;
-    orthogonalize_gs(n, ks.V, ks.ld, 0, ks.buf, ks.m, ks.eta, &norm, &lindep, &ks.passes);
+    orthogonalize_gs(n, ks.V, ks.ld, 0, ks.buf, ks.m, ks.eta, ks.refine, &norm, &lindep, &ks.passes);
     if (norm == 0.0 || lindep) { rc = -3; goto done; }
     orc_bv_scale(n, ks.V, 1.0 / norm);
     nini = 1;
@@ -2112,7 +2190,7 @@ int orc_eps_solve_twosided(int n, const int32_t *rp, const int32_t *ci, const do
       double *H[2] = {ds.A, B};
       for (int sd = 0; sd < 2; sd++) {
         double nrm; int lind;
-        orthogonalize_gs(n, side[sd]->V, n, kk, side[sd]->buf, side[sd]->m, side[sd]->eta, &nrm, &lind, &side[sd]->passes);
+        orthogonalize_gs(n, side[sd]->V, n, kk, side[sd]->buf, side[sd]->m, side[sd]->eta, side[sd]->refine, &nrm, &lind, &side[sd]->passes);
         orc_bv_scale(n, side[sd]->V + (int64_t)kk * n, 1.0 / nrm);
         const double *c = side[sd]->buf + (int64_t)kk * side[sd]->m;
         for (int j = nconv; j < kk; j++) {

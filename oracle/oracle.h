@@ -89,7 +89,14 @@ typedef struct {
   int     lanczos_partial; /* Lanczos: 1 periodic, 2 partial, 3 selective, 4 delayed (lanczos_local 0) */
   int     trueres;         /* EPSSetTrueResidual: explicit residuals in the convergence test
                               (Krylov-Schur, two-sided right residuals, Arnoldi) */
+  int     refine;          /* BVSetOrthogonalization refinement: ORC_REFINE_IFNEEDED (default),
+                              ORC_REFINE_NEVER, ORC_REFINE_ALWAYS */
 } orc_opts;
+enum { ORC_REFINE_IFNEEDED = 0, ORC_REFINE_NEVER = 1, ORC_REFINE_ALWAYS = 2 };
+/* BVOrthogonalizeSomeColumn (MGS against the columns with which[i] set) with a
+   refinement type; returns the number of passes. */
+int orc_bv_orthogonalize_some_column(int64_t n, double *V, int64_t ld, int j, const char *which, double *buffer,
+                                     int m, int refine, double eta, double *norm, int *lindep);
 
 typedef struct {
   int     nconv, its, reason;   /* reason: 1 converged, -1 max_it, -2 breakdown */

@@ -25,7 +25,10 @@ class Opts(C.Structure):
                 ("extraction", C.c_int), ("target", C.c_double),
                 ("st", C.c_int), ("sigma", C.c_double), ("lanczos_local", C.c_int),
                 ("arnoldi_delayed", C.c_int), ("balance", C.c_int), ("balance_its", C.c_int),
-                ("balance_cutoff", C.c_double), ("lanczos_partial", C.c_int), ("trueres", C.c_int)]
+                ("balance_cutoff", C.c_double), ("lanczos_partial", C.c_int), ("trueres", C.c_int),
+                ("refine", C.c_int)]
+
+REFINE = {"ifneeded": 0, "never": 1, "always": 2}
 
 
 class Result(C.Structure):
@@ -195,6 +198,27 @@ def bv_dotvec(V, x):
     return q
 
 
+def bv_orthogonalize_some_column(V, j, which, refine="ifneeded", eta=0.7071):
+    """BVOrthogonalizeSomeColumn (MGS against the columns i < j with which[i])
+    with a refinement type: returns (V, coefficients h[0:j+1], norm, lindep,
+    passes)."""
+    n, k = V.shape
+    Vf = np.asfortranarray(V, np.float64).copy(order="F")
+    m = k
+    buf = np.zeros(m * m)
+    w = np.zeros(max(k, 1), np.uint8)
+    w[:len(which)] = np.asarray(which, np.uint8)
+    nrm, lind = C.c_double(), C.c_int()
+    f = lib().orc_bv_orthogonalize_some_column
+    f.restype = C.c_int
+    f.argtypes = [C.c_int64, C.c_void_p, C.c_int64, C.c_int, C.c_void_p, C.c_void_p, C.c_int, C.c_int, C.c_double,
+                  C.POINTER(C.c_double), C.POINTER(C.c_int)]
+    passes = f(n, Vf.ctypes.data, n, j, w.ctypes.data, buf.ctypes.data, m, REFINE[refine], eta, C.byref(nrm),
+               C.byref(lind))
+    h = buf.reshape(m, m, order="F")[:j + 1, j].copy()
+    return Vf, h, nrm.value, lind.value, passes
+
+
 def bv_multvec(V, alpha, q, beta, y):
     n, k = V.shape
     Vf = np.asfortranarray(V).ravel(order="F")
@@ -282,7 +306,7 @@ def lanczos_local_sweep(A, m, v0):
 def eps_solve(A, nev, ncv=0, tol=1e-8, which="LM", hermitian=True, v0=None, max_it=0,
               mpd=0, nvec=None, nthreads=0, keep=0.5, lock=1, solver="krylovschur",
               extraction="ritz", target=0.0, twosided=False, st="shift", sigma=None,
-              reorthog="full", balance="none", trueres=False, trace=0):
+              reorthog="full", balance="none", trueres=False, trace=0, refine="ifneeded"):
     o = Opts(hermitian=1 if hermitian else 0, which=WHICH[which], nev=nev, ncv=ncv, mpd=mpd,
              extraction={"ritz": 0, "harmonic": 1}[extraction], target=float(target),
              st={"shift": 0, "sinvert": 1}[st], sigma=float(target if sigma is None else sigma),
@@ -292,7 +316,7 @@ def eps_solve(A, nev, ncv=0, tol=1e-8, which="LM", hermitian=True, v0=None, max_
              lanczos_partial={"full": 0, "local": 0, "periodic": 1, "partial": 2, "selective": 3,
                                "delayed": 4}[reorthog],
              max_it=max_it, tol=tol, keep=keep, eta=0.7071, lock=lock, nthreads=nthreads,
-             trueres=1 if trueres else 0,
+             trueres=1 if trueres else 0, refine=REFINE[refine],
              solver={"krylovschur": 0, "arnoldi": 1, "lanczos": 2}[solver])
     ncv_eff = ncv if ncv else min(A.n, max(2 * nev, nev + 15))
     eigr = np.zeros(ncv_eff + 1)

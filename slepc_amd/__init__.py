@@ -337,6 +337,21 @@ class BV:
     def mat_mult_column(self, A, j):
         check(lib().sa_bv_mat_mult_column(self.h, A.h, j))
 
+    def set_orthogonalization(self, refine="ifneeded"):
+        """BVSetOrthogonalization's refinement: "ifneeded", "never", "always"."""
+        check(lib().sa_bv_set_orthogonalization(self.h, REFINE[refine]))
+
+    def orthogonalize_some_column(self, j, which, eta=0.7071):
+        """BVOrthogonalizeSomeColumn (MGS against the columns i < j with
+        which[i]); returns (coefficients with the norm at index j, norm, lindep)."""
+        w = np.zeros(max(j, 1), np.int32)
+        w[:j] = np.asarray(which[:j], np.int32)
+        h = np.zeros(j + 1)
+        nrm, lind = C.c_double(), C.c_int()
+        check(lib().sa_bv_orthogonalize_some_column(self.h, j, w.ctypes.data, eta, h.ctypes.data, C.byref(nrm),
+                                                    C.byref(lind)))
+        return h, nrm.value, bool(lind.value)
+
     def orthonormalize_column(self, j, eta=0.7071):
         nrm, lind = C.c_double(), C.c_int()
         check(lib().sa_bv_orthonormalize_column(self.h, j, eta, C.byref(nrm), C.byref(lind)))
@@ -363,6 +378,9 @@ class BV:
         except Exception:
             pass
 
+
+# BVOrthogRefineType (BVSetOrthogonalization)
+REFINE = {"ifneeded": 0, "never": 1, "always": 2}
 
 # EPSConvergedReason (include/slepceps.h:179-185)
 REASON_CONVERGED_TOL, REASON_CONVERGED_USER, REASON_ITERATING = 1, 2, 0
@@ -466,6 +484,10 @@ class EPS:
         v = C.c_int()
         check(lib().sa_eps_lanczos_get_reorthog(self.h, C.byref(v)))
         return {0: "local", 1: "full", 2: "selective", 3: "periodic", 4: "partial", 5: "delayed"}[v.value]
+
+    def setOrthogonalization(self, refine="ifneeded", eta=0.0):
+        """EPSGetBV + BVSetOrthogonalization: refinement and eta (0 keeps 0.7071)."""
+        check(lib().sa_eps_set_orthogonalization(self.h, REFINE[refine], eta))
 
     def setTwoSided(self, twosided=True):
         """EPSSetTwoSided: also compute left eigenvectors (two-sided Krylov-Schur)."""

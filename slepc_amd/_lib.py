@@ -127,6 +127,10 @@ SIGNATURES = {
     "sa_eps_get_iteration_number": (C.c_int, [_VP, C.POINTER(C.c_int)]),
     "sa_eps_get_converged_reason": (C.c_int, [_VP, C.POINTER(C.c_int)]),
     "sa_eps_monitor_set": (C.c_int, [_VP, _VP, _VP]),
+    "sa_eps_set_orthogonalization": (C.c_int, [_VP, C.c_int, C.c_double]),
+    "sa_bv_set_orthogonalization": (C.c_int, [_VP, C.c_int]),
+    "sa_bv_orthogonalize_some_column": (C.c_int, [_VP, C.c_int, _VP, C.c_double, _VP, C.POINTER(C.c_double),
+                                                  C.POINTER(C.c_int)]),
     "sa_eps_monitor_cancel": (C.c_int, [_VP]),
     "sa_eps_set_stopping_test_function": (C.c_int, [_VP, _VP, _VP]),
     "sa_eps_get_eigenpair": (C.c_int, [_VP, C.c_int, C.POINTER(C.c_double), C.POINTER(C.c_double), _VP, _VP]),

@@ -395,6 +395,7 @@ __global__ __launch_bounds__(BLK) void bw_slot_kernel(BwArgs<T> b) {
   a.G = b.pnc[p];
   a.rpb = CH;
   a.eta = b.eta;
+  a.refine = GS_REFINE_IFNEEDED;
   const int64_t rb = b.crow[g];
   gs_slot<T, L, NT>(a, rb, min(a.n, rb + CH), bidx);
 }

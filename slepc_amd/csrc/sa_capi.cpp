@@ -406,6 +406,30 @@ int sa_bv_orthonormalize_column(sa_bv B, int j, double eta, double *norm, int *l
   return bv_orthonormalize(B->V, j, eta > 0 ? eta : 0.7071, norm, lindep);
 }
 
+int sa_bv_set_orthogonalization(sa_bv B, int refine) {
+  SA_CHECK(B && refine >= SA_REFINE_IFNEEDED && refine <= SA_REFINE_ALWAYS, "invalid refinement type");
+  B->V.refine = refine;
+  return 0;
+}
+
+int sa_bv_orthogonalize_some_column(sa_bv B, int j, const int *which, double eta, double *h, double *norm,
+                                    int *lindep) {
+  API_INIT();
+  SA_CHECK(B && which && j >= 0 && j < B->V.m, "invalid column");
+  std::vector<int> cols;
+  for (int i = 0; i < j; i++)
+    if (which[i]) cols.push_back(i);
+  double nrm = 0.0;
+  int ld = 0;
+  SA_TRY(bv_mgs_column(B->V, j, cols.data(), (int)cols.size(), eta > 0 ? eta : 0.7071, &nrm, &ld));
+  if (h) {
+    SA_HIP(hipMemcpy(h, B->V.dbuf + (size_t)j * B->V.m, sizeof(double) * (j + 1), hipMemcpyDeviceToHost));
+  }
+  if (norm) *norm = nrm;
+  if (lindep) *lindep = ld;
+  return 0;
+}
+
 int sa_bv_get_coefficients(sa_bv B, int c0, int c1, double *buf) {
   API_INIT();
   SA_CHECK(B && buf && c0 >= 0 && c1 <= B->V.m && c0 <= c1, "invalid coefficient range");
@@ -1011,6 +1035,13 @@ int sa_eps_get_iteration_number(sa_eps E, int *its) {
 int sa_eps_get_converged_reason(sa_eps E, int *reason) {
   SA_CHECK(E && reason && E->solved, "EPS not solved");
   *reason = E->res.reason;
+  return 0;
+}
+
+int sa_eps_set_orthogonalization(sa_eps E, int refine, double eta) {
+  SA_CHECK(E && refine >= SA_REFINE_IFNEEDED && refine <= SA_REFINE_ALWAYS, "invalid refinement type");
+  E->p.refine = refine;
+  if (eta > 0) E->p.eta = eta;
   return 0;
 }
 

@@ -308,6 +308,50 @@ static int lz_sweep_selective(BV &V, BV &AV, const std::function<int(const void 
   return 0;
 }
 
+// EPSDelayedArnoldi1 (epskrylov.c:120-157; arnoldi.c:84-87 and lanczos.c:500
+// with refinement NEVER): the delayed sweep without reorthogonalisation, the
+// normalisation alone delayed by one step.  At step j > k the dot products of
+// v_{j+1} with v_0..v_j and ||v_j||^2 are taken before v_j is normalised; the
+// coefficients are rescaled by ||v_j|| (the diagonal one by its square), v_j
+// and v_{j+1} scaled by 1/||v_j||, H(j,j-1) = beta = ||v_j||; then v_{j+1} -=
+// V(:,0:j+1) H(:,j).  The last column stays unnormalised and no breakdown is
+// reported, as in the reference.
+static int lz_sweep_delayed1(BV &V, const std::function<int(const void *, void *)> &op, int k, int m, double *H,
+                             int ldh, double *betam, KsResult &r) {
+  hipStream_t s = stream();
+  const double es = (double)prec_size(V.prec);
+  double *qd = V.dq;
+  for (int j = k; j < m; j++) {
+    SA_TRY(op(col_ptr(V, j), col_ptr(V, j + 1)));
+    double *Hj = H + (size_t)ldh * j;
+    SA_TRY(launch_dotvec(V, 0, j + 1, col_ptr(V, j + 1), qd, s));       // BVDotColumn(V, j+1)
+    SA_HIP(hipMemcpyAsync(Hj, qd, sizeof(double) * (j + 1), hipMemcpyDeviceToHost, s));
+    r.bytes_bv += (j + 2.0) * V.n * es;
+    double dot = 0.0;
+    if (j > k) {                                                          // VecDot(vj, vj)
+      SA_TRY(launch_dot_self(V.prec, col_ptr(V, j), V.n, V.ws.dots, V.ws.part, s));
+      SA_HIP(hipMemcpyAsync(&dot, V.ws.dots, sizeof(double), hipMemcpyDeviceToHost, s));
+      r.bytes_bv += V.n * es;
+    }
+    SA_HIP(hipStreamSynchronize(s));
+    if (j > k) {
+      const double norm = std::sqrt(dot);
+      SA_TRY(launch_scale(V.prec, col_ptr(V, j), V.n, 1.0 / norm, s));
+      H[(size_t)ldh * (j - 1) + j] = norm;
+      for (int i = 0; i < j; i++) Hj[i] /= norm;
+      Hj[j] /= dot;
+      SA_TRY(launch_scale(V.prec, col_ptr(V, j + 1), V.n, 1.0 / norm, s));
+      r.bytes_bv += 4.0 * V.n * es;
+      *betam = norm;
+    }
+    SA_HIP(hipMemcpyAsync(qd, Hj, sizeof(double) * (j + 1), hipMemcpyHostToDevice, s));   // BVMultColumn
+    SA_TRY(launch_multvec(V, 0, j + 1, -1.0, qd, 1.0, col_ptr(V, j + 1), s));
+    SA_HIP(hipStreamSynchronize(s));
+    r.bytes_bv += (j + 3.0) * V.n * es;
+  }
+  return 0;
+}
+
 // ---- EPSDelayedArnoldi (epskrylov.c:24-114; lanczos.c:495-512) ------------
 // One classical Gram-Schmidt pass per step whose normalisation is delayed by
 // one step and whose reorthogonalisation by two: at step j the dot products of
@@ -545,6 +589,8 @@ int ks_solve(const KsParams &p, const Mat &A, BV &V, bool have_initial, KsResult
   const int n = A.n, ncv = p.ncv, nev = p.nev, mpd = p.mpd, ld = ncv + 1;
   SA_CHECK(V.m == ncv + 1 && V.n == n && V.prec == A.prec, "basis does not match the operator/ncv");
   SA_CHECK(ncv + 1 < GS_MAXC, "ncv too large for the fused Gram-Schmidt kernels");
+  SA_CHECK(p.refine >= GS_REFINE_IFNEEDED && p.refine <= GS_REFINE_ALWAYS, "invalid refinement type");
+  V.refine = p.refine;   // BVSetOrthogonalization of the solver's basis
   // EPSARNOLDI (arnoldi.c:52-131, setup :31-50): explicit restart and
   // always a DSNHEP projected problem; ncv <= nev + mpd
   const bool arn = p.solver == 1;
@@ -691,8 +737,9 @@ int ks_solve(const KsParams &p, const Mat &A, BV &V, bool have_initial, KsResult
     } else if (partial && p.lanczos_partial == 4) {   // lanczos.c:495-512
       std::vector<double> Td((size_t)nv * nv, 0.0);
       const size_t vb = prec_size(V.prec) * (size_t)n;
-      SA_TRY(lz_sweep_delayed(V, op_apply, k0, nv, Td.data(), nv, &beta, wtmp.p, static_cast<char *>(wtmp.p) + vb,
-                              r));
+      if (p.refine == GS_REFINE_NEVER) SA_TRY(lz_sweep_delayed1(V, op_apply, k0, nv, Td.data(), nv, &beta, r));
+      else SA_TRY(lz_sweep_delayed(V, op_apply, k0, nv, Td.data(), nv, &beta, wtmp.p,
+                                   static_cast<char *>(wtmp.p) + vb, r));
       breakdown = 0;
       double *d = ds.d(), *e = ds.e();
       for (int i = k0; i < nv - 1; i++) {
@@ -702,8 +749,9 @@ int ks_solve(const KsParams &p, const Mat &A, BV &V, bool have_initial, KsResult
       d[nv - 1] = Td[(size_t)nv * (nv - 1) + nv - 1];
       e[nv - 1] = beta;
     } else if (arn && p.arnoldi_delayed) {   // arnoldi.c:88-91
-      SA_TRY(lz_sweep_delayed(V, op_apply, k0, nv, ds.A(), ld, &beta, wtmp.p,
-                              static_cast<char *>(wtmp.p) + prec_size(V.prec) * (size_t)n, r));
+      if (p.refine == GS_REFINE_NEVER) SA_TRY(lz_sweep_delayed1(V, op_apply, k0, nv, ds.A(), ld, &beta, r));
+      else SA_TRY(lz_sweep_delayed(V, op_apply, k0, nv, ds.A(), ld, &beta, wtmp.p,
+                                   static_cast<char *>(wtmp.p) + prec_size(V.prec) * (size_t)n, r));
       breakdown = 0;
     } else if (partial)
       SA_TRY(lz_sweep_partial(V, op_apply, k0, &nv, &beta, &breakdown, anorm, p.lanczos_partial == 1, ncv, p.eta,

@@ -38,6 +38,7 @@ struct KsParams {
   bool lock = true;              // EPSKrylovSchurSetLocking
   bool trueres = false;          // EPSSetTrueResidual (Krylov-Schur and Arnoldi)
   int lanczos_partial = 0;     // EPSLanczosSetReorthog: 1 periodic, 2 partial, 3 selective, 4 delayed
+  int refine = 0;              // BVSetOrthogonalization refinement of the basis: GS_REFINE_*
   KsHooks hooks;
 };
 

@@ -72,7 +72,7 @@ static int gs_launch(BV &V, const Mat *A, int c, double eta, hipStream_t s) {
   GsArgs<T> a{};
   a.n = V.n; a.ld = V.ld; a.V = static_cast<T *>(V.data); a.c = c;
   a.st = V.st; a.dbuf = V.dbuf; a.m = V.m; a.eta = eta;
-  a.G = V.G; a.rpb = V.rows_per_block;
+  a.G = V.G; a.rpb = V.rows_per_block; a.refine = V.refine;
   const size_t shm = gs_lds_bytes(c, sizeof(T));
   SA_CHECK(shm + 4096 + 8 * BLK <= 65536, "basis too wide for the LDS tile of the Gram-Schmidt kernel");
   double *pa = V.ws.part, *pb = V.ws.part + (size_t)V.G * (GS_MAXC + 1);

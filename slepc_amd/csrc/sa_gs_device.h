@@ -40,13 +40,19 @@ struct GsArgs {
   double *part_out;         // this slot's partial sums, [c+1][G]
   int G, rpb;
   double eta;
+  int refine;               // BVSetOrthogonalization refinement: GS_REFINE_*
 };
 
-// The rule of BVOrthogonalizeGS with refinement "if needed" (bvorthog.c:
-// 145-217) applied to the dots of one pass: dots[0:c] = V^T v, dots[c] =
-// v^T v, sumsq = sum dots[i]^2.  Updates S; returns the action for v.
-__device__ inline int gs_rule(GsState &S, const double *dots, double sumsq, int c, double eta, double *scale,
-                              int *record) {
+// The rule of BVOrthogonalizeGS (bvorthog.c:145-217) applied to the dots of
+// one pass: dots[0:c] = V^T v, dots[c] = v^T v, sumsq = sum dots[i]^2.
+// Refinement "if needed" repeats the pass while the estimated norm falls
+// below eta times the norm before it (at most three passes); "never" takes
+// one pass and then the true norm of the result (lindep only for a zero
+// norm); "always" takes two passes, the second with the norm estimate.  An
+// estimate that fails (<= 0) is replaced by the true norm after the update
+// (bvorthog.c:124-127).  Updates S; returns the action for v.
+__device__ inline int gs_rule(GsState &S, const double *dots, double sumsq, int c, double eta, int refine,
+                              double *scale, int *record) {
   const double self = dots[c];
   int fin = 0, act = ACT_MULT_DOTS;
   double nrm_f = 0.0, onrm_f = 0.0;
@@ -54,7 +60,7 @@ __device__ inline int gs_rule(GsState &S, const double *dots, double sumsq, int 
   if (S.pending) {                       // true norm of the previous update
     const double nrm = safe_sqrt(self);
     S.pending = 0;
-    if (!(nrm != 0.0 && nrm < eta * S.onrm && S.pass < 3)) {
+    if (refine != GS_REFINE_IFNEEDED || !(nrm != 0.0 && nrm < eta * S.onrm && S.pass < 3)) {
       fin = 1; nrm_f = nrm; onrm_f = S.onrm; act = ACT_SCALE_ONLY;
     }
   }
@@ -63,21 +69,28 @@ __device__ inline int gs_rule(GsState &S, const double *dots, double sumsq, int 
     S.passes_total++;
     *record = 1;
     const double onrm = safe_sqrt(self);
-    const double est = onrm * onrm - sumsq;
-    if (est <= 0.0) {                    // estimate failed: measure after the update
+    if (refine == GS_REFINE_NEVER) {     // the true norm after the update (bvorthog.c:187-193)
       S.pending = 1; S.onrm = onrm; act = ACT_MULT_DOTS;
+    } else if (refine == GS_REFINE_ALWAYS && S.pass == 1) {   // a second pass follows
+      act = ACT_MULT_DOTS;
     } else {
-      const double nrm = sqrt(est);
-      if (nrm != 0.0 && nrm < eta * onrm && S.pass < 3) {
-        S.onrm = onrm; act = ACT_MULT_DOTS;
+      const double est = onrm * onrm - sumsq;
+      if (est <= 0.0) {                  // estimate failed: measure after the update
+        S.pending = 1; S.onrm = onrm; act = ACT_MULT_DOTS;
       } else {
-        fin = 1; nrm_f = nrm; onrm_f = onrm; act = ACT_MULT_SCALE;
+        const double nrm = sqrt(est);
+        if (refine == GS_REFINE_IFNEEDED && nrm != 0.0 && nrm < eta * onrm && S.pass < 3) {
+          S.onrm = onrm; act = ACT_MULT_DOTS;
+        } else {
+          fin = 1; nrm_f = nrm; onrm_f = onrm; act = ACT_MULT_SCALE;
+        }
       }
     }
   }
   *scale = 1.0;
   if (fin) {                             // bvorthog.c:175-216 and 417-422
-    const int lind = !(nrm_f != 0.0 && fabs(nrm_f) >= eta * fabs(onrm_f));
+    const int lind = refine == GS_REFINE_NEVER ? !(nrm_f != 0.0)
+                                               : !(nrm_f != 0.0 && fabs(nrm_f) >= eta * fabs(onrm_f));
     S.lindep = lind;
     S.nrm = nrm_f;
     S.beta = nrm_f;
@@ -102,7 +115,7 @@ __device__ void gs_decide_block(const GsArgs<T> &a, const double *s_dots, GsStat
     GsState S = *s_st;
     double scale;
     int rec;
-    *s_act = gs_rule(S, s_dots, sumsq, c, a.eta, &scale, &rec);
+    *s_act = gs_rule(S, s_dots, sumsq, c, a.eta, a.refine, &scale, &rec);
     *s_st = S;
     *s_scale = scale;
     *s_rec = rec;

@@ -34,6 +34,8 @@ int num_cus();
 // Column c is orthogonalised against columns 0..c-1 by three kernels (slots
 // 1..3).  Slot s reads the state st[(s+1)%3] left by the previous kernel and
 // workgroup 0 writes st[s-1]; the state after a column is st[2].
+// BVSetOrthogonalization refinement types (BVOrthogRefineType)
+enum GsRefine : int { GS_REFINE_IFNEEDED = 0, GS_REFINE_NEVER = 1, GS_REFINE_ALWAYS = 2 };
 constexpr int GS_MAXC = 128;        // largest column index + 1 handled
 constexpr int GS_NSTATE = 3;
 constexpr int64_t GS_NT_BYTES = 256ll << 20;   // bases above this size use non-temporal tile loads
@@ -72,6 +74,7 @@ struct BV {
   double *hq = nullptr;        // pinned host staging
   int G = 0, rows_per_block = 0;
   int64_t passes_done = 0;     // host-side counter of orthogonalisation passes
+  int refine = 0;              // BVSetOrthogonalization refinement (0 if needed, 1 never, 2 always)
 };
 
 struct Mat {

@@ -110,21 +110,31 @@ static int mgs_pass(const BV &W, void *w, const int *cols, int nc, double *h, do
   return 0;
 }
 
-// BVOrthogonalizeGS with MGS and refinement IFNEEDED (bvorthog.c:175-185):
-// passes while ||w|| < eta ||w_before||, at most three; returns the final
-// norm and the lindep test of the last pass.
+// BVOrthogonalizeGS with MGS (bvorthog.c:145-217) and the basis's refinement:
+// IFNEEDED passes while ||w|| < eta ||w_before||, at most three; NEVER one
+// pass, lindep only for a zero norm; ALWAYS two passes, the second measured.
+// Returns the final norm and the lindep test.
 static int mgs_refined(const BV &W, void *w, const int *cols, int nc, double *h, double eta, double *norm,
                        int *lindep, int64_t *passes) {
   hipStream_t s = stream();
   double *sc = W.dq + (size_t)W.m * W.m;              // scalar slots past the staging area
   double *slot = sc, *nrm_d = sc + 1, *onrm_d = sc + 2;
   double hs[2];
+  if (W.refine == GS_REFINE_ALWAYS) {                 // bvorthog.c:195-199: a first pass without norms
+    SA_TRY(mgs_pass(W, w, cols, nc, h, slot, nullptr, nrm_d, nullptr, s));
+    (*passes)++;
+  }
   SA_TRY(mgs_pass(W, w, cols, nc, h, slot, onrm_d, nrm_d, nullptr, s));
   SA_HIP(hipMemcpyAsync(hs, nrm_d, 2 * sizeof(double), hipMemcpyDeviceToHost, s));
   SA_HIP(hipStreamSynchronize(s));
   (*passes)++;
   double nrm = hs[0], onrm = hs[1];
-  int l = 1;
+  if (W.refine == GS_REFINE_NEVER) {                  // bvorthog.c:187-193: lindep for a zero norm only
+    *norm = nrm;
+    *lindep = !(nrm != 0.0);
+    return 0;
+  }
+  int l = W.refine == GS_REFINE_ALWAYS ? 3 : 1;       // ALWAYS: no further pass
   while (l < 3 && nrm != 0.0 && std::fabs(nrm) < eta * std::fabs(onrm)) {
     l++;
     onrm = nrm;

@@ -86,6 +86,7 @@ int ks_twosided(const KsParams &p, const Mat &A, const Mat &AT, BV &V, BV &W, bo
   SA_CHECK(!p.hermitian, "Two-sided methods are not intended for symmetric problems");
   SA_CHECK(p.solver == 0 && p.extraction == 0, "two-sided runs with Krylov-Schur and Ritz extraction");
   SA_CHECK(ncv + 1 < GS_MAXC, "ncv too large for the fused Gram-Schmidt kernels");
+  V.refine = W.refine = p.refine;   // BVSetOrthogonalization of both bases
   hipStream_t s = stream();
   DenseProblem ds;
   ds.allocate(ld, false, p.which);
