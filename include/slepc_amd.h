@@ -306,6 +306,12 @@ int sa_eps_set_st(sa_eps eps, int st_type, int shift_given, double shift);
 #define SA_PC_JACOBI 1
 #define SA_PC_LU 2
 int sa_eps_set_st_ksp(sa_eps eps, int pc_type, double rtol, int max_it);
+/* EPSBuildBalance_Krylov (src/eps/interface/epsdefault.c:443-504) for an
+ * FP64 operator: the diagonal D (n host doubles) of D A D^-1 after its
+ * iterations, twoside with A^T; the +-1 vectors come from the library's
+ * generator (U[0,1) seed 0xBA1A + j, sign by < 0.5) in place of PetscRandom.
+ * cutoff <= 0 takes 1e-8.  Exposes what sa_eps_set_balance applies. */
+int sa_mat_build_balance(sa_mat A, int twoside, int its, double cutoff, double *D_host);
 /* EPSSetBalance (epsopts.c:1160-1215; EPSBuildBalance_Krylov,
    epsdefault.c:443-504): balancing of non-Hermitian problems by a diagonal D
    built from its iterations (its <= 0: 5) of random +-1 vectors, the update

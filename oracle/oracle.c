@@ -1479,7 +1479,8 @@ static void ks_expand(KSo *ks, int k, int *m, double *beta, int *breakdown)
 
 /* EPSBuildBalance_Krylov (epsdefault.c:443-504): the diagonal D for which
    D A D^-1 is balanced, from its random +-1 vectors z (here the oracle's
-   uniform generator, seed 0xBA1A + j, stands in for PetscRandom):
+   uniform generator, seed 0xBA1A + j, mapped to [0,1), stands in for
+   PetscRandom; a draw below 0.5 gives -1):
    p = D A (z ./ D); oneside: D ./= |p|; twoside: r = (A^T (z .* D)) ./ D and
    D .*= sqrt(|r ./ p|) where |p| > cutoff ||p_0||_inf and r != 0. */
 static void build_balance(int n, const int32_t *rp, const int32_t *ci, const double *val, int twoside, int its,
@@ -1490,7 +1491,8 @@ static void build_balance(int n, const int32_t *rp, const int32_t *ci, const dou
   for (int i = 0; i < n; i++) D[i] = 1.0;
   for (int j = 0; j < its; j++) {
     orc_rand_uniform(0xBA1AULL + (uint64_t)j, n, z);
-    for (int i = 0; i < n; i++) z[i] = z[i] < 0.5 ? -1.0 : 1.0;
+    /* a [0,1) draw (PetscRandom's default interval) from the U[-1,1) stream, signed at 0.5 */
+    for (int i = 0; i < n; i++) z[i] = 0.5 * (z[i] + 1.0) < 0.5 ? -1.0 : 1.0;
     for (int i = 0; i < n; i++) r[i] = z[i] / D[i];
     orc_spmv(n, rp, ci, val, r, p);
     for (int i = 0; i < n; i++) p[i] *= D[i];
@@ -1509,6 +1511,13 @@ static void build_balance(int n, const int32_t *rp, const int32_t *ci, const dou
     }
   }
   free(z); free(r); free(p);
+}
+
+int orc_build_balance(int n, const int32_t *rp, const int32_t *ci, const double *val, int twoside, int its,
+                      double cutoff, double *D)
+{
+  build_balance(n, rp, ci, val, twoside, its, cutoff, D);
+  return 0;
 }
 
 int orc_eps_solve(int n, const int32_t *rp, const int32_t *ci, const double *val, const orc_opts *o, const double *v0, orc_result *res)

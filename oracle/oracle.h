@@ -123,6 +123,9 @@ int orc_eps_solve(int n, const int32_t *rowptr, const int32_t *col, const double
    left eigenvectors (real parts) in lvecs [n * nvec], may be NULL. */
 int orc_eps_solve_twosided(int n, const int32_t *rowptr, const int32_t *col, const double *val,
                            const orc_opts *o, const double *v0, orc_result *r, double *lvecs);
+/* EPSBuildBalance_Krylov (epsdefault.c:443-504): D of the balanced operator. */
+int orc_build_balance(int n, const int32_t *rp, const int32_t *ci, const double *val, int twoside, int its,
+                      double cutoff, double *D);
 double orc_residual_rel(int n, const int32_t *rowptr, const int32_t *col, const double *val,
                         double lr, double li, const double *xr, const double *xi);
 double orc_residual_abs(int n, const int32_t *rowptr, const int32_t *col, const double *val,

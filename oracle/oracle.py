@@ -198,6 +198,19 @@ def bv_dotvec(V, x):
     return q
 
 
+def build_balance(A, twoside=False, its=5, cutoff=1e-8):
+    """EPSBuildBalance_Krylov: the diagonal D of D A D^-1."""
+    D = np.zeros(A.n)
+    f = lib().orc_build_balance
+    f.restype = C.c_int
+    f.argtypes = [C.c_int, C.c_void_p, C.c_void_p, C.c_void_p, C.c_int, C.c_int, C.c_double, C.c_void_p]
+    rp = np.ascontiguousarray(A.rowptr, np.int32)
+    ci = np.ascontiguousarray(A.col, np.int32)
+    va = np.ascontiguousarray(A.val, np.float64)
+    f(A.n, rp.ctypes.data, ci.ctypes.data, va.ctypes.data, 1 if twoside else 0, its, cutoff, D.ctypes.data)
+    return D
+
+
 def bv_orthogonalize_some_column(V, j, which, refine="ifneeded", eta=0.7071):
     """BVOrthogonalizeSomeColumn (MGS against the columns i < j with which[i])
     with a refinement type: returns (V, coefficients h[0:j+1], norm, lindep,

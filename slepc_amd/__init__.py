@@ -209,6 +209,12 @@ class Mat:
         self.mult(xd, yd)
         return yd.to_host(dt, self.n)
 
+    def build_balance(self, twoside=False, its=5, cutoff=1e-8):
+        """EPSBuildBalance_Krylov: the diagonal D of D A D^-1 (host array)."""
+        D = np.zeros(self.n)
+        check(lib().sa_mat_build_balance(self.h, 1 if twoside else 0, its, cutoff, D.ctypes.data))
+        return D
+
     def residual_norms(self, X, lr, li=None):
         """||A x_i - lambda_i x_i|| for the columns of X (host array n x k,
         uploaded; a complex pair occupies two columns, real then imaginary

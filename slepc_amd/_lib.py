@@ -50,6 +50,7 @@ SIGNATURES = {
     "sa_mat_create_csr": (C.c_int, [C.c_int, C.c_int64, _I32P, _I32P, _F64P, C.c_int, C.POINTER(_VP)]),
     "sa_set_spmv_stream_bytes": (C.c_int64, [C.c_int64]),
     "sa_set_concurrent_solves": (C.c_int, [C.c_int]),
+    "sa_mat_build_balance": (C.c_int, [_VP, C.c_int, C.c_int, C.c_double, _VP]),
     "sa_mat_destroy": (C.c_int, [_VP]),
     "sa_mat_get_info": (C.c_int, [_VP, C.POINTER(C.c_int), C.POINTER(C.c_int64), C.POINTER(C.c_int)]),
     "sa_mat_mult": (C.c_int, [_VP, _VP, _VP]),

@@ -93,7 +93,8 @@ int eps_build_balance(const Mat &A, const Mat *AT, bool twoside, int its, double
   double norma = 0.0;
   for (int j = 0; j < its; j++) {
     sa_rand_uniform(0xBA1AULL + (uint64_t)j, n, h.data());
-    for (int64_t i = 0; i < n; i++) h[i] = h[i] < 0.5 ? -1.0 : 1.0;
+    // a [0,1) draw (PetscRandom's default interval) from the U[-1,1) stream, signed at 0.5
+    for (int64_t i = 0; i < n; i++) h[i] = 0.5 * (h[i] + 1.0) < 0.5 ? -1.0 : 1.0;
     SA_HIP(hipMemcpyAsync(z, h.data(), sizeof(double) * n, hipMemcpyHostToDevice, s));
     hipLaunchKernelGGL(bal_div_kernel, dim3(nb), dim3(BLK), 0, s, z, D, r, n);   // r = z ./ D
     SA_TRY(launch_spmv(A, r, p, s));                                              // p = A r

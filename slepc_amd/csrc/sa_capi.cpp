@@ -867,11 +867,8 @@ static int download_csr(const Mat &A, std::vector<int32_t> &rp, std::vector<int3
 }
 
 // A^T as CSR (host transpose of the device arrays), built once per operator.
-static int ensure_transpose(sa_eps E) {
-  if (E->AT && E->AT_of == (const void *)E->A) return 0;
-  if (E->AT) sa_mat_destroy(E->AT);
-  E->AT = nullptr;
-  const Mat &A = E->A->A;
+// A^T of a device CSR operator as a new operator (columns ascending per row).
+static int build_transpose(const Mat &A, sa_mat *out) {
   const int n = A.n;
   const int64_t nnz = A.nnz;
   std::vector<int32_t> rp, ci, trp(n + 1, 0), tci(nnz);
@@ -886,12 +883,37 @@ static int ensure_transpose(sa_eps E) {
       tci[dst] = r;
       tva[dst] = va[q];
     }
-  SA_TRY(sa_mat_create_csr(n, nnz, trp.data(), tci.data(), tva.data(), SA_FP64, &E->AT));
+  return sa_mat_create_csr(n, nnz, trp.data(), tci.data(), tva.data(), SA_FP64, out);
+}
+
+static int ensure_transpose(sa_eps E) {
+  if (E->AT && E->AT_of == (const void *)E->A) return 0;
+  if (E->AT) sa_mat_destroy(E->AT);
+  E->AT = nullptr;
+  SA_TRY(build_transpose(E->A->A, &E->AT));
   E->AT_of = (const void *)E->A;
   return 0;
 }
 
 extern "C" {
+
+int sa_mat_build_balance(sa_mat M, int twoside, int its, double cutoff, double *D_host) {
+  API_INIT();
+  SA_CHECK(M && D_host && its > 0 && M->A.prec == SA_FP64, "invalid balance arguments (FP64 operator, its > 0)");
+  struct Hold {
+    sa_mat AT = nullptr;
+    double *D = nullptr;
+    ~Hold() {
+      if (AT) sa_mat_destroy(AT);
+      if (D) (void)hipFree(D);
+    }
+  } h;
+  if (twoside) SA_TRY(build_transpose(M->A, &h.AT));
+  SA_HIP(hipMalloc(&h.D, sizeof(double) * (size_t)std::max(1, M->A.n)));
+  SA_TRY(eps_build_balance(M->A, twoside ? &h.AT->A : nullptr, twoside != 0, its, cutoff > 0 ? cutoff : 1e-8, h.D));
+  SA_HIP(hipMemcpy(D_host, h.D, sizeof(double) * (size_t)M->A.n, hipMemcpyDeviceToHost));
+  return 0;
+}
 
 int sa_eps_set_balance(sa_eps E, int balance, int its, double cutoff) {
   SA_CHECK(E, "null EPS");

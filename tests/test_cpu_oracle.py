@@ -944,3 +944,29 @@ def test_oracle_lanczos_delayed1_test2(orc):
     assert list(np.round(r["eigr"][:nc], 5)) == TEST2_LARGEST[:nc]
     ex = np.sort(2 - 2 * np.cos(np.arange(1, 31) * np.pi / 31))[::-1][:nc]
     assert np.max(np.abs(r["eigr"][:nc] - ex)) < 1e-8
+
+
+@pytest.mark.parametrize("twoside", [False, True])
+def test_oracle_balance_one_iteration_closed_form(orc, twoside):
+    """EPSBuildBalance_Krylov (epsdefault.c:443-504) after one iteration from
+    D = I and the +-1 vector z: oneside D = 1 / |A z|; twoside D = sqrt(|A^T z|
+    / |A z|) where |A z| exceeds cutoff ||A z||_inf.  z is drawn as the
+    oracle does (U[-1,1) seed 0xBA1A mapped to [0,1), -1 below 0.5), so a
+    dropped transpose, a missing square root or a wrong sign split fails."""
+    A = orc.markov(15)
+    Ad = A.to_dense()
+    u = orc.rand_uniform(0xBA1A, A.n)
+    z = np.where(0.5 * (u + 1.0) < 0.5, -1.0, 1.0)
+    assert 0.35 < np.mean(z < 0) < 0.65                  # an even split of signs
+    p = np.abs(Ad @ z)
+    if twoside:
+        r = np.abs(Ad.T @ z)
+        ok = (p > 1e-8 * p.max()) & (r != 0)
+        ref = np.where(ok, np.sqrt(r / np.where(ok, p, 1.0)), 1.0)
+    else:
+        # rows of the Markov matrix where A z cancels to rounding level (their
+        # D is 1/rounding, order-dependent) are left out
+        ok = p > 1e-8 * p.max()
+        ref = np.where(ok, 1.0 / np.where(ok, p, 1.0), 1.0)
+    D = orc.build_balance(A, twoside, its=1)
+    assert np.allclose(D[ok], ref[ok], rtol=1e-13, atol=0) and ok.sum() > 0.8 * A.n

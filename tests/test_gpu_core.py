@@ -732,6 +732,34 @@ def test_balance_matches_oracle(sa, orc, balance, lock):
         assert E.computeError(i) < 1e-7
 
 
+@pytest.mark.parametrize("twoside", [False, True])
+@pytest.mark.parametrize("case", ["markov15", "convdiff8"])
+def test_balance_vector_matches_oracle(sa, orc, case, twoside):
+    """The balancing diagonal D itself (sa_mat_build_balance,
+    EPSBuildBalance_Krylov epsdefault.c:443-504): the device D equals the
+    oracle's from the same +-1 vectors after 5 iterations (to rounding of the
+    SpMV sums), and after one iteration equals the closed form 1/|A z|
+    (oneside) or sqrt(|A^T z| / |A z|) (twoside).  B = D A D^-1 is similar to
+    A for any D, so the solve's eigenvalues alone cannot catch a wrong D."""
+    Ao = orc.markov(15) if case == "markov15" else orc.convdiff3d(8)
+    M = sa.Mat(sa.CSR(Ao.n, Ao.rowptr, Ao.col, Ao.val))
+    D = M.build_balance(twoside, its=5)
+    Do = orc.build_balance(Ao, twoside, its=5)
+    good = np.abs(Do) < 1e8                     # rows whose |A z| cancelled to rounding are left out
+    assert good.sum() > 0.8 * Ao.n
+    assert np.allclose(D[good], Do[good], rtol=1e-11, atol=0)
+    D1 = M.build_balance(twoside, its=1)
+    Ad = Ao.to_dense()
+    u = orc.rand_uniform(0xBA1A, Ao.n)
+    z = np.where(0.5 * (u + 1.0) < 0.5, -1.0, 1.0)
+    p = np.abs(Ad @ z)
+    rt = np.abs(Ad.T @ z)
+    ok = (p > 1e-8 * p.max()) & ((rt > 1e-8 * rt.max()) if twoside else True)
+    ref = np.sqrt(rt / np.where(ok, p, 1.0)) if twoside else 1.0 / np.where(ok, p, 1.0)
+    assert ok.sum() > 0.8 * Ao.n
+    assert np.allclose(D1[ok], ref[ok], rtol=1e-13, atol=0)
+
+
 def test_arnoldi_delayed_matches_oracle(sa, orc):
     """EPSArnoldi with delayed reorthogonalization (test9.c suffix 3: Markov
     m = 15, 3 largest real, tol 1e-7) through the C ABI against the oracle:
