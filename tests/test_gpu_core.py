@@ -100,13 +100,14 @@ STREAM_CASES = {"rows256": ((0, 8), 1001, 2_000_003, 256), "rows128": ((9, 15), 
                 "vector": ((100, 120), 301, 301, 0), "one_long_row": ((0, 5), 2500, 2500, 0)}
 
 
-@pytest.mark.parametrize("size", ["small", "large"])
-@pytest.mark.parametrize("case", list(STREAM_CASES))
+# the vector-CSR cases have one size
+CASE_SIZES = [(c, s) for c, v in STREAM_CASES.items() for s in (("small", "large") if v[1] != v[2] else ("small",))]
+
+
+@pytest.mark.parametrize("case,size", CASE_SIZES)
 @pytest.mark.parametrize("prec", ["fp64", "fp32"])
 def test_spmv_row_blocks_exact(sa, stream_layout, case, prec, size):
     (lo, hi), n_small, n_large, rb_large = STREAM_CASES[case]
-    if size == "large" and n_large == n_small:
-        pytest.skip("the vector-CSR cases have one size")
     n = n_small if size == "small" else n_large
     rng = np.random.default_rng(11)
     lens = rng.integers(lo, hi + 1, size=n)
