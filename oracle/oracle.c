@@ -1528,9 +1528,12 @@ int orc_eps_solve(int n, const int32_t *rp, const int32_t *ci, const double *val
   if (o->nthreads > 0) omp_set_num_threads(o->nthreads);
 #endif
   int nev = o->nev > 0 ? o->nev : 1;
+  if (nev > n) nev = n;                       /* small-problem safeguard, epssetup.c:342-345 */
   int ncv = o->ncv > 0 ? o->ncv : (nev + 15 > 2 * nev ? nev + 15 : 2 * nev);
   if (ncv > n) ncv = n;
   int mpd = o->mpd > 0 ? o->mpd : ncv;
+  /* EPSSetDimensions_Default (epssetup.c:679) and krylovschur.c:88 */
+  if (!(ncv >= nev + 1 || (ncv == nev && ncv == n)) || ncv > nev + mpd) return -16;
   int max_it = o->max_it > 0 ? o->max_it : (100 > 2 * n / ncv ? 100 : 2 * n / ncv);
   double tol = o->tol > 0 ? o->tol : 1e-8, keep = o->keep > 0 ? o->keep : 0.5;
   int herm = o->hermitian, which = o->which ? o->which : ORC_LM, lock = o->lock;
@@ -2008,9 +2011,12 @@ int orc_eps_solve_twosided(int n, const int32_t *rp, const int32_t *ci, const do
 {
   if (!lp_load() || !LP.getrf || !LP.getrs) return -10;
   int nev = o->nev > 0 ? o->nev : 1;
+  if (nev > n) nev = n;                       /* small-problem safeguard, epssetup.c:342-345 */
   int ncv = o->ncv > 0 ? o->ncv : (nev + 15 > 2 * nev ? nev + 15 : 2 * nev);
   if (ncv > n) ncv = n;
   int mpd = o->mpd > 0 ? o->mpd : ncv;
+  /* EPSSetDimensions_Default (epssetup.c:679) and krylovschur.c:88 */
+  if (!(ncv >= nev + 1 || (ncv == nev && ncv == n)) || ncv > nev + mpd) return -16;
   int max_it = o->max_it > 0 ? o->max_it : (100 > 2 * n / ncv ? 100 : 2 * n / ncv);
   double tol = o->tol > 0 ? o->tol : 1e-8, keep = o->keep > 0 ? o->keep : 0.5;
   if (o->hermitian || o->solver != 0 || o->extraction != 0) return -11;

@@ -117,6 +117,9 @@ typedef struct {
 
 int orc_lanczos_local_sweep(int n, const int32_t *rp, const int32_t *ci, const double *val, int m, const double *v0,
                             double *V, double *alpha, double *beta);
+/* Returns 0, or a negative code: -16 for dimensions the reference rejects at
+   setup (ncv < nev+1 unless ncv = nev = n, or ncv > nev+mpd), after nev and
+   ncv are clamped to n as its small-problem safeguard does. */
 int orc_eps_solve(int n, const int32_t *rowptr, const int32_t *col, const double *val,
                   const orc_opts *o, const double *v0, orc_result *r);
 /* Two-sided Krylov-Schur (ks-twosided.c:126-241): as orc_eps_solve, plus the

@@ -838,11 +838,15 @@ static int eps_setup_dims(sa_eps E, KsParams &p) {
   p.hooks.owner = E;
   p.hooks.monitor = E->monitors.empty() ? nullptr : eps_monitor_hook;
   p.hooks.stopping = E->stopping ? eps_stopping_hook : nullptr;
+  // safeguard for small problems (epssetup.c:342-345): nev and a set ncv at most n
+  if (p.nev > n) p.nev = n;
+  if (E->ncv_set && p.ncv > n) p.ncv = n;
   if (!E->ncv_set && !E->mpd_set) p.ncv = std::min(n, std::max(2 * p.nev, p.nev + 15));
   else if (!E->ncv_set) p.ncv = std::min(n, p.nev + p.mpd);
   if (!E->mpd_set) p.mpd = p.ncv;
   SA_CHECK(p.ncv >= p.nev + 1 || (p.ncv == p.nev && p.ncv == n), "The value of ncv must be at least nev+1");
   SA_CHECK(p.ncv <= p.nev + p.mpd, "The value of ncv must not be larger than nev+mpd");
+  SA_CHECK(E->nini <= p.ncv, "The number of initial vectors is larger than ncv");   // epssetup.c:423
   if (!E->maxit_set) p.max_it = std::max(100, 2 * n / p.ncv);
   if (p.st_type == SA_ST_SINVERT && !E->shift_set) p.st_shift = p.target;   // STSetDefaultShift
   return 0;

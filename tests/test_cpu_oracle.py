@@ -970,3 +970,47 @@ def test_oracle_balance_one_iteration_closed_form(orc, twoside):
         ref = np.where(ok, 1.0 / np.where(ok, p, 1.0), 1.0)
     D = orc.build_balance(A, twoside, its=1)
     assert np.allclose(D[ok], ref[ok], rtol=1e-13, atol=0) and ok.sum() > 0.8 * A.n
+
+
+def _dense_csr(orc, A):
+    n = A.shape[0]
+    rp = np.arange(0, n * n + 1, n, dtype=np.int32)
+    ci = np.tile(np.arange(n, dtype=np.int32), n)
+    return orc.CSR(n, rp, ci, np.ascontiguousarray(A, dtype=np.float64).ravel())
+
+
+@pytest.mark.parametrize("n", [1, 2, 3, 5, 8])
+def test_oracle_small_problems_every_eigenvalue(orc, n):
+    """The small-problem safeguard of EPSSetUp (epssetup.c:342-345): nev and a
+    set ncv above n are clamped to n, EPSSetDimensions_Default then accepts
+    ncv = nev = n (epssetup.c:679), and the Arnoldi sweep breaks down once the
+    Krylov space is the whole space (beta = 0, every estimate zero), so one
+    restart returns all n eigenvalues of the dense matrix (numpy's eigvalsh /
+    eigvals), conjugate pairs included."""
+    rng = np.random.default_rng(20 + n)
+    B = rng.standard_normal((n, n))
+    v0 = orc.rand_uniform(0, n)
+    S = B + B.T
+    w = np.linalg.eigvalsh(S)
+    for nev, ncv in ((n, 0), (n + 3, 0), (max(1, n - 1), 50)):
+        r = orc.eps_solve(_dense_csr(orc, S), nev, ncv, 1e-10, "LM", True, v0=v0)
+        assert r["nconv"] == n and r["its"] == 1 and r["reason"] > 0
+        assert np.allclose(np.sort(r["eigr"]), w, rtol=0, atol=1e-13 * np.abs(w).max())
+    r = orc.eps_solve(_dense_csr(orc, B), n, 0, 1e-10, "LM", False, v0=v0)
+    ev = np.linalg.eigvals(B)
+    got = np.sort_complex(r["eigr"][:n] + 1j * r["eigi"][:n])
+    assert r["nconv"] == n and np.allclose(got, np.sort_complex(ev), rtol=0, atol=1e-12 * np.abs(ev).max())
+
+
+def test_oracle_rejects_dimensions_as_the_reference():
+    """EPSSetDimensions_Default (epssetup.c:679): ncv = nev on a larger
+    operator is rejected; krylovschur.c:88: ncv above nev + mpd is rejected."""
+    from oracle import oracle as orc
+    A = orc.lap2d(10)
+    v0 = orc.rand_uniform(0, A.n)
+    with pytest.raises(RuntimeError, match="rc=-16"):
+        orc.eps_solve(A, 10, 10, 1e-8, "LM", True, v0=v0)
+    with pytest.raises(RuntimeError, match="rc=-16"):
+        orc.eps_solve(A, 10, 30, 1e-8, "LM", True, v0=v0, mpd=15)
+    r = orc.eps_solve(A, 10, 11, 1e-8, "LM", True, v0=v0, max_it=5000)   # ncv = nev + 1 is accepted
+    assert r["nconv"] >= 10

@@ -1,0 +1,133 @@
+"""Setup checks and small problems of the EPS interface, as the reference
+sets them up.  EPSSetDimensions rejects nev <= 0 (epsopts.c:409).  At setup
+the small-problem safeguard clamps nev and a set ncv to n (epssetup.c:342-345);
+EPSSetDimensions_Default then requires ncv >= nev+1 unless ncv = nev = n
+(epssetup.c:679), Krylov-Schur requires ncv <= nev+mpd (krylovschur.c:88), and
+no more initial vectors than ncv are accepted (epssetup.c:423).  Operators of
+1 to 8 rows run the whole path: the Arnoldi sweep breaks down once the Krylov
+space is the whole space, and one restart returns every eigenvalue of the
+dense matrix, for each solver, in FP64 and mixed precision, one- and
+two-sided."""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+SIZES = [1, 2, 3, 5, 8]
+
+
+def _dense(sa, orc, A):
+    n = A.shape[0]
+    rp = np.arange(0, n * n + 1, n, dtype=np.int32)
+    ci = np.tile(np.arange(n, dtype=np.int32), n)
+    val = np.ascontiguousarray(A, dtype=np.float64).ravel()
+    return sa.CSR(n, rp, ci, val), orc.CSR(n, rp, ci, val.copy())
+
+
+def _eps(sa, Ah, herm, nev, ncv, solver="krylovschur", mixed=False, twosided=False, v0=None):
+    E = sa.EPS()
+    E.setType(solver)
+    E.setOperators(sa.Mat(Ah, sa.FP64), sa.Mat(Ah, sa.FP32) if mixed else None)
+    E.setProblemType(herm)
+    E.setWhichEigenpairs("LM")
+    E.setDimensions(nev, ncv)
+    E.setTolerances(1e-10, 0)
+    if mixed:
+        E.setMixedPrecision(True, 1e-5)
+    if twosided:
+        E.setTwoSided(True)
+    if v0 is not None:
+        E.setInitialSpace(v0)
+    return E
+
+
+@pytest.mark.parametrize("solver", ["krylovschur", "arnoldi", "lanczos"])
+@pytest.mark.parametrize("n", SIZES)
+def test_small_symmetric_every_eigenvalue(sa, orc, n, solver):
+    """nev = n, nev = n + 3 (clamped to n) and nev = n - 1 with ncv 50
+    (clamped to n): every eigenvalue of the dense symmetric matrix, equal to
+    the oracle's and to numpy's eigvalsh, in one restart, each pair within
+    the tolerance."""
+    rng = np.random.default_rng(20 + n)
+    B = rng.standard_normal((n, n))
+    S = B + B.T
+    Ah, Ao = _dense(sa, orc, S)
+    v0 = orc.rand_uniform(0, n)
+    w = np.linalg.eigvalsh(S)
+    for nev, ncv in ((n, 0), (n + 3, 0), (max(1, n - 1), 50)):
+        E = _eps(sa, Ah, True, nev, ncv, solver, v0=v0)
+        E.solve()
+        r = orc.eps_solve(Ao, nev, ncv, 1e-10, "LM", True, v0=v0, solver=solver)
+        assert E.getConverged() == n == r["nconv"] and E.getIterationNumber() == r["its"] == 1
+        vals = np.sort([E.getEigenpair(i, False)[0] for i in range(n)])
+        assert np.allclose(vals, w, rtol=0, atol=1e-13 * np.abs(w).max())
+        assert np.allclose(vals, np.sort(r["eigr"]), rtol=0, atol=1e-13 * np.abs(w).max())
+        for i in range(n):
+            assert E.computeError(i) <= 1e-10
+
+
+@pytest.mark.parametrize("twosided", [False, True])
+@pytest.mark.parametrize("n", SIZES)
+def test_small_nonsymmetric_every_eigenvalue(sa, orc, n, twosided):
+    """Non-symmetric dense matrices (complex conjugate pairs from n = 3 on):
+    every eigenvalue equals numpy's eigvals and the oracle's, one- and
+    two-sided; the two-sided solve also returns left vectors within the
+    tolerance."""
+    rng = np.random.default_rng(40 + n)
+    B = rng.standard_normal((n, n))
+    Ah, Ao = _dense(sa, orc, B)
+    v0 = orc.rand_uniform(0, n)
+    E = _eps(sa, Ah, False, n, 0, twosided=twosided, v0=v0)
+    E.solve()
+    r = orc.eps_solve(Ao, n, 0, 1e-10, "LM", False, v0=v0, twosided=twosided)
+    assert E.getConverged() == n == r["nconv"]
+    got = np.sort_complex(np.array([complex(*E.getEigenpair(i, False)[:2]) for i in range(n)]))
+    ev = np.sort_complex(np.linalg.eigvals(B))
+    scale = np.abs(ev).max()
+    assert np.allclose(got, ev, rtol=0, atol=1e-12 * scale)
+    assert np.allclose(got, np.sort_complex(r["eigr"] + 1j * r["eigi"]), rtol=0, atol=1e-12 * scale)
+    for i in range(n):
+        assert E.computeError(i) <= 1e-10
+
+
+def test_small_mixed_precision(sa, orc):
+    """Mixed FP32 -> FP64 on an 8-row symmetric matrix: the FP32 phase and the
+    FP64 polish both stop at the breakdown, and every eigenvalue is exact to
+    FP64 rounding."""
+    rng = np.random.default_rng(7)
+    B = rng.standard_normal((8, 8))
+    S = B + B.T
+    Ah, _ = _dense(sa, orc, S)
+    E = _eps(sa, Ah, True, 8, 0, mixed=True, v0=orc.rand_uniform(0, 8))
+    E.solve()
+    assert E.getConverged() == 8
+    vals = np.sort([E.getEigenpair(i, False)[0] for i in range(8)])
+    w = np.linalg.eigvalsh(S)
+    assert np.allclose(vals, w, rtol=0, atol=1e-13 * np.abs(w).max())
+
+
+def test_setup_rejects_what_the_reference_rejects(sa, orc):
+    """nev <= 0, ncv = nev on a larger operator, ncv > nev + mpd and more
+    initial vectors than ncv are errors with the reference's messages; the
+    same EPS then solves once the dimensions are valid."""
+    Ah = sa.lap2d(10)
+    E = sa.EPS()
+    E.setOperators(sa.Mat(Ah))
+    E.setProblemType(True)
+    with pytest.raises(RuntimeError, match="nev must be positive"):
+        E.setDimensions(0, 10)
+    E.setDimensions(10, 10)
+    with pytest.raises(RuntimeError, match="ncv must be at least nev\\+1"):
+        E.solve()
+    E.setDimensions(10, 30, 15)
+    with pytest.raises(RuntimeError, match="must not be larger than nev\\+mpd"):
+        E.solve()
+    E.setDimensions(4, 10)
+    E.setInitialSpace(orc.rand_uniform(3, 12 * Ah.n).reshape(12, Ah.n))
+    with pytest.raises(RuntimeError, match="initial vectors is larger than ncv"):
+        E.solve()
+    E.setInitialSpace(orc.rand_uniform(0, Ah.n))
+    E.solve()
+    assert E.getConverged() >= 4
+    for i in range(4):
+        assert E.computeError(i) <= 1e-8
