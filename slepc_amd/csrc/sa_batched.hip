@@ -1261,21 +1261,36 @@ int sa_batch_create(int P, const int32_t *n, const int32_t *rowptr, const int32_
                     sa_batch *out) {
   SA_TRY(ensure_init());
   SA_CHECK(P > 0 && n && rowptr && col && val && out, "invalid batch arguments");
+  // every problem's CSR is checked as sa_mat_create_csr checks one operator,
+  // before anything is allocated
+  int64_t rows = 0, nnz = 0, rp_off = 0;
+  for (int p = 0; p < P; p++) {
+    SA_CHECK(n[p] > 0, "empty problem in batch");
+    const int32_t *rpp = rowptr + rp_off;
+    SA_CHECK(rpp[0] == 0, "row pointer of a batch problem must start at 0");
+    for (int r = 0; r < n[p]; r++) {
+      SA_CHECK(rpp[r + 1] >= rpp[r], "row pointer not monotone in batch");
+      for (int32_t q = rpp[r]; q < rpp[r + 1]; q++) {
+        SA_CHECK(col[nnz + q] >= 0 && col[nnz + q] < n[p], "column index out of range in batch");
+        SA_CHECK(q == rpp[r] || col[nnz + q] > col[nnz + q - 1], "column indices must ascend within a row");
+      }
+    }
+    rows += (n[p] + 63) / 64 * 64;
+    nnz += rpp[n[p]];
+    rp_off += n[p] + 1;
+  }
+  SA_CHECK(nnz < INT32_MAX && rows < INT32_MAX, "batch too large for int32 CSR indices");
   auto *b = new sa_batch_s();
   b->P = P;
   if (const char *e = std::getenv("SLEPC_AMD_BATCH_SWEEP"))
     b->sweep_req = !std::strcmp(e, "persistent") ? 1 : (!std::strcmp(e, "wide") ? 2 : 0);
   b->n.assign(n, n + P);
   b->off.resize(P);
-  int64_t rows = 0, nnz = 0, rp_off = 0;
-  for (int p = 0; p < P; p++) {
-    SA_CHECK(n[p] > 0, "empty problem in batch");
-    b->off[p] = rows;
-    rows += (n[p] + 63) / 64 * 64;
-    nnz += rowptr[rp_off + n[p]];
-    rp_off += n[p] + 1;
+  int64_t r0 = 0;
+  for (int p = 0; p < P; p++) {   // problems start at 64-row boundaries
+    b->off[p] = r0;
+    r0 += (n[p] + 63) / 64 * 64;
   }
-  SA_CHECK(nnz < INT32_MAX && rows < INT32_MAX, "batch too large for int32 CSR indices");
   b->nrows = rows;
   b->nnz = nnz;
   b->rp.assign(rows + 1, 0);
@@ -1289,7 +1304,6 @@ int sa_batch_create(int P, const int32_t *n, const int32_t *rowptr, const int32_
     for (int r = 0; r < n[p]; r++) {
       b->rp[base + r] = (int32_t)z;
       for (int32_t q = rpp[r]; q < rpp[r + 1]; q++) {
-        SA_CHECK(col[vo + q] >= 0 && col[vo + q] < n[p], "column index out of range in batch");
         b->ci[z] = (int32_t)(base + col[vo + q]);
         b->val[z] = val[vo + q];
         z++;

@@ -131,3 +131,120 @@ def test_setup_rejects_what_the_reference_rejects(sa, orc):
     assert E.getConverged() >= 4
     for i in range(4):
         assert E.computeError(i) <= 1e-8
+
+
+def _csr4():
+    rp = np.array([0, 2, 4, 6, 8], np.int32)
+    ci = np.array([0, 1, 1, 2, 2, 3, 0, 3], np.int32)
+    return rp, ci, np.arange(1.0, 9.0)
+
+
+@pytest.mark.parametrize("bad,msg", [("col_range", "column index out of range"),
+                                     ("col_negative", "column index out of range"),
+                                     ("col_order", "must ascend within a row"),
+                                     ("rowptr_order", "row pointer not monotone"),
+                                     ("rowptr_nnz", "row pointer does not match nnz"),
+                                     ("rowptr_start", "row pointer does not match nnz")])
+def test_csr_rejected_before_upload(sa, bad, msg):
+    """MatSetValues / MatAssemblyEnd reject what sa_mat_create_csr rejects on
+    the host, before anything reaches the device: column indices outside
+    [0, n), columns not ascending within a row (the SpMV kernels and the
+    transpose assume sorted rows), row pointers that decrease or do not
+    span [0, nnz].  A valid operator created afterwards multiplies exactly."""
+    from types import SimpleNamespace
+    rp, ci, val = _csr4()
+    nnz = 8
+    if bad == "col_range":
+        ci[5] = 4
+    elif bad == "col_negative":
+        ci[0] = -1
+    elif bad == "col_order":
+        ci[2], ci[3] = 2, 1
+    elif bad == "rowptr_order":
+        rp[2] = 1
+    elif bad == "rowptr_nnz":
+        nnz = 7
+    else:
+        rp[0] = 1
+    with pytest.raises(RuntimeError, match=msg):
+        sa.Mat(SimpleNamespace(n=4, rowptr=rp, col=ci, val=val, nnz=nnz))
+    rp, ci, val = _csr4()
+    M = sa.Mat(sa.CSR(4, rp, ci, val))
+    x = np.array([1.0, -2.0, 3.0, 5.0])
+    A = np.zeros((4, 4))
+    for r in range(4):
+        A[r, ci[rp[r]:rp[r + 1]]] = val[rp[r]:rp[r + 1]]
+    assert np.array_equal(M.mult_host(x), A @ x)
+
+
+def test_bv_arguments_rejected(sa):
+    """BV calls with columns outside the basis or an inverted active range
+    are errors (BVSetActiveColumns, BVGetColumn argument checks), and the
+    basis is usable afterwards."""
+    V = sa.BV(1000, 8)
+    with pytest.raises(RuntimeError, match="invalid active columns"):
+        V.set_active_columns(3, 2)
+    with pytest.raises(RuntimeError, match="invalid active columns"):
+        V.set_active_columns(0, 9)
+    with pytest.raises(RuntimeError, match="invalid column"):
+        V.norm_column(8)
+    with pytest.raises(RuntimeError, match="invalid column"):
+        V.copy_column(0, -1)
+    with pytest.raises(RuntimeError, match="too many columns"):
+        sa.BV(1000, 128)
+    x = np.linspace(-1.0, 1.0, 1000)
+    V.set_column(2, x)
+    assert abs(V.norm_column(2) - np.linalg.norm(x)) < 1e-13 * np.linalg.norm(x)
+
+
+@pytest.mark.parametrize("bad,msg", [("col_range", "column index out of range in batch"),
+                                     ("col_negative", "column index out of range in batch"),
+                                     ("col_order", "must ascend within a row"),
+                                     ("rowptr_order", "row pointer not monotone in batch"),
+                                     ("rowptr_start", "must start at 0")])
+def test_batch_csr_rejected(sa, bad, msg):
+    """The batched solver checks every problem's CSR as a single operator is
+    checked, before anything is allocated or uploaded (here the second of two
+    problems is malformed)."""
+    from types import SimpleNamespace
+    from slepc_amd.batch import Batch
+    rp, ci, val = _csr4()
+    if bad == "col_range":
+        ci[5] = 4
+    elif bad == "col_negative":
+        ci[0] = -1
+    elif bad == "col_order":
+        ci[2], ci[3] = 2, 1
+    elif bad == "rowptr_order":
+        rp[2] = 1
+    else:
+        rp[0] = 1
+    with pytest.raises(RuntimeError, match=msg):
+        Batch([sa.lap2d(6), SimpleNamespace(n=4, rowptr=rp, col=ci, val=val)])
+
+
+def test_batch_dimensions_and_small_problems(sa, orc):
+    """sa_batch_solve needs nev < ncv <= min(64, n) for every problem (the
+    lock-step kernels hold at most 64 columns; ncv = nev is rejected as
+    EPSSetDimensions_Default rejects it); a batch of a 16-row and a 64-row
+    problem then solves, each problem equal to the oracle run alone."""
+    from slepc_amd.batch import Batch
+    csrs = [sa.lap2d(4, pot_scale=10.0, seed=1), sa.lap2d(8, pot_scale=10.0, seed=2)]
+    B = Batch(csrs)
+    with pytest.raises(RuntimeError, match="invalid ncv"):
+        B.solve(8, 24, 1e-10, "LM")          # ncv > n of the first problem
+    with pytest.raises(RuntimeError, match="invalid ncv"):
+        B.solve(8, 8, 1e-10, "LM")
+    v0 = [orc.rand_uniform(orc.sub_seed(5, p), A.n) for p, A in enumerate(csrs)]
+    B.solve(4, 12, 1e-10, "LM", v0=v0)
+    err = B.errors()
+    for p, A in enumerate(csrs):
+        nconv, its, reason = B.converged(p)
+        assert reason == 1 and nconv >= 4
+        Ao = orc.CSR(A.n, A.rowptr, A.col, A.val)
+        r = orc.eps_solve(Ao, 4, 12, 1e-10, "LM", True, v0=v0[p])
+        vals = np.sort([B.eigenpair(p, i, False)[0] for i in range(4)])
+        assert np.allclose(vals, np.sort(r["eigr"][:4]), rtol=1e-10, atol=0)
+        w = np.linalg.eigvalsh(Ao.to_dense())
+        assert np.allclose(np.sort(vals), np.sort(w[np.argsort(-np.abs(w))[:4]]), rtol=1e-10, atol=0)
+        assert np.all(err[p] <= 1e-10)
