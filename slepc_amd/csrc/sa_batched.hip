@@ -735,7 +735,18 @@ static void free_dev(BatchDev &B) {
   B = BatchDev();
 }
 
+static int alloc_dev_impl(sa_batch_s *b, BatchDev &B, int prec, int m);
+
+// Device state of one precision for a basis of m columns.  A failed
+// allocation releases everything, so a later call starts from scratch
+// instead of finding a basis of the right width with its operator missing.
 static int alloc_dev(sa_batch_s *b, BatchDev &B, int prec, int m) {
+  const int rc = alloc_dev_impl(b, B, prec, m);
+  if (rc) free_dev(B);
+  return rc;
+}
+
+static int alloc_dev_impl(sa_batch_s *b, BatchDev &B, int prec, int m) {
   if (B.V && B.m == m && B.prec == prec) return 0;
   const bool keep_op = B.rp && B.prec == prec;
   if (!keep_op) free_dev(B);

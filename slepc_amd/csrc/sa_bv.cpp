@@ -14,7 +14,17 @@ namespace sa {
 
 int launch_gs_reset(BV &V, hipStream_t s);
 
+static int bv_alloc_impl(BV &V, int64_t n, int m, int prec);
+
+// A basis of m columns of length n.  A failed allocation leaves V empty
+// (BV()), so no caller can mistake it for a basis of the requested size.
 int bv_alloc(BV &V, int64_t n, int m, int prec) {
+  const int rc = bv_alloc_impl(V, n, m, prec);
+  if (rc) bv_free(V);
+  return rc;
+}
+
+static int bv_alloc_impl(BV &V, int64_t n, int m, int prec) {
   SA_CHECK(n >= 0 && m > 0, "invalid BV sizes");
   SA_CHECK(prec == SA_FP64 || prec == SA_FP32, "invalid precision");
   V.n = n;

@@ -39,7 +39,12 @@ hipStream_t stream() {
 }
 int num_cus() { return g_cus; }
 
+// Called on entry to every C ABI function that touches the device.  It also
+// clears the runtime's last-error state, which a HIP error handled in an
+// earlier call (a failed allocation, say) leaves set, so that the launch
+// checks of this call report only this call's errors.
 int ensure_init() {
+  (void)hipGetLastError();
   if (g_device >= 0 && stream()) return 0;
   return sa_init(0);
 }
@@ -170,7 +175,8 @@ int sa_mat_create_csr(int n, int64_t nnz, const int32_t *rp, const int32_t *ci, 
       SA_CHECK(p == rp[r] || ci[p] > ci[p - 1], "column indices must ascend within a row");
     }
   }
-  auto *M = new sa_mat_s();
+  BuildGuard<sa_mat, sa_mat_destroy> guard{new sa_mat_s()};
+  sa_mat M = guard.h;
   Mat &A = M->A;
   A.n = n;
   A.nnz = nnz;
@@ -186,7 +192,6 @@ int sa_mat_create_csr(int n, int64_t nnz, const int32_t *rp, const int32_t *ci, 
   if (hipMalloc(&A.rowptr, sizeof(int32_t) * (n + 1)) != hipSuccess ||
       hipMalloc(&A.col, sizeof(int32_t) * std::max<int64_t>(1, nnz)) != hipSuccess ||
       hipMalloc(&A.val, es * std::max<int64_t>(1, nnz)) != hipSuccess) {
-    sa_mat_destroy(M);
     set_error("hipMalloc failed for the CSR arrays");
     return 1;
   }
@@ -202,7 +207,7 @@ int sa_mat_create_csr(int n, int64_t nnz, const int32_t *rp, const int32_t *ci, 
     }
   }
   SA_HIP(hipStreamSynchronize(s));
-  *out = M;
+  *out = guard.release();
   return 0;
 }
 

@@ -11,11 +11,14 @@ namespace sa {
 
 // ---- errors ---------------------------------------------------------------
 void set_error(const std::string &msg);
+// A failed HIP call is reported once: the runtime's last-error state is cleared
+// too, or the next launch check (hipGetLastError) would report it again.
 #define SA_HIP(call)                                                              \
   do {                                                                            \
     hipError_t e_ = (call);                                                       \
     if (e_ != hipSuccess) {                                                       \
       ::sa::set_error(std::string(#call) + ": " + hipGetErrorString(e_));         \
+      (void)hipGetLastError();                                                    \
       return 1;                                                                   \
     }                                                                             \
   } while (0)
@@ -25,6 +28,15 @@ void set_error(const std::string &msg);
   } while (0)
 #define SA_TRY(call)                                                              \
   do { int r_ = (call); if (r_) return r_; } while (0)
+
+// Owns an object under construction: destroys it if an error path returns
+// before release() hands it to the caller.
+template <typename H, int (*Destroy)(H)>
+struct BuildGuard {
+  H h;
+  ~BuildGuard() { if (h) Destroy(h); }
+  H release() { H r = h; h = nullptr; return r; }
+};
 
 hipStream_t stream();
 int ensure_init();

@@ -291,7 +291,13 @@ struct sa_mlp_s {
 
 static int mlp_ensure_batch(sa_mlp net, int B) {
   if (B <= net->maxB) return 0;
-  auto fr = [](void *p) { if (p) (void)hipFree(p); };
+  // freed pointers are cleared, so a failed allocation below leaves nothing
+  // that sa_mlp_destroy would free twice
+  auto fr = [](auto *&p) {
+    if (p) (void)hipFree(p);
+    p = nullptr;
+  };
+  net->maxB = 0;
   fr(net->Xb); fr(net->H1b); fr(net->H2b); fr(net->dYb); fr(net->dZ2b); fr(net->dZ1b);
   fr(net->H1); fr(net->H2); fr(net->Y); fr(net->T); fr(net->dY); fr(net->dH2); fr(net->dH1); fr(net->Xf);
   const long h = net->hid, o = net->dout;
@@ -373,7 +379,8 @@ extern "C" {
 int sa_mlp_create(int din, int hidden, int dout, uint64_t seed, sa_mlp *out) {
   SA_TRY(ensure_init());
   SA_CHECK(out && din > 0 && din <= 32 && hidden > 0 && dout > 0, "invalid MLP sizes (din <= 32)");
-  auto *net = new sa_mlp_s();
+  BuildGuard<sa_mlp, sa_mlp_destroy> guard{new sa_mlp_s()};
+  sa_mlp net = guard.h;
   net->din = din;
   net->hid = hidden;
   net->dout = dout;
@@ -410,7 +417,7 @@ int sa_mlp_create(int din, int hidden, int dout, uint64_t seed, sa_mlp *out) {
     SA_HIP(hipStreamSynchronize(s));
   }
   SA_HIP(hipMalloc(&net->part, sizeof(double) * 512));
-  *out = net;
+  *out = guard.release();
   return 0;
 }
 

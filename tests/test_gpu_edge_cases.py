@@ -248,3 +248,22 @@ def test_batch_dimensions_and_small_problems(sa, orc):
         w = np.linalg.eigvalsh(Ao.to_dense())
         assert np.allclose(np.sort(vals), np.sort(w[np.argsort(-np.abs(w))[:4]]), rtol=1e-10, atol=0)
         assert np.all(err[p] <= 1e-10)
+
+
+def test_failed_allocation_reported_and_library_usable(sa):
+    """A basis too large for the device (32 TiB) fails in hipMalloc: the call
+    returns the error, leaves no half-built basis behind, and later bases
+    and solves work."""
+    with pytest.raises(RuntimeError, match="hipMalloc"):
+        sa.BV(1 << 40, 4)
+    V = sa.BV(5000, 4)
+    x = np.cos(np.arange(5000.0))
+    V.set_column(1, x)
+    assert abs(V.norm_column(1) - np.linalg.norm(x)) < 1e-13 * np.linalg.norm(x)
+    E = sa.EPS()
+    E.setOperators(sa.Mat(sa.lap2d(20)))
+    E.setProblemType(True)
+    E.setDimensions(4, 12)
+    E.setTolerances(1e-10, 5000)
+    E.solve()
+    assert E.getConverged() >= 4
