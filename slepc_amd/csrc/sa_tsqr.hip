@@ -3,9 +3,12 @@
 // 510-560 (geqrf + orgqr on each process, a binary tree of QR factorizations
 // of stacked R factors across processes).
 //
-// On one GPU the tree runs over 256-row tiles instead of processes:
-//   level 0: every workgroup computes the Householder QR of one 256 x k tile
-//            of V(:, l:k) (one row per thread, held in registers), writes the
+// On one GPU the tree runs over tiles of rows instead of processes (256-row
+// tiles factored by a workgroup, one row per thread, for 33..64 columns;
+// tiles of 128 rows (17..32 columns) or 256 rows (up to 16) factored by a
+// single wave, 2 or 4 rows per lane, below):
+//   level 0: every tile's Householder QR of its rows
+//            of V(:, l:k) (held in registers) is computed; the kernel writes the
 //            Householder vectors over the tile and its k x k R into the
 //            stacked matrix of level 1;
 //   level i: the same kernel on the stacked R factors, until one tile is left,
@@ -27,6 +30,7 @@
 // rounding at no extra pass over V.
 #include <algorithm>
 #include <cmath>
+#include <cstdint>
 #include <vector>
 
 #include "sa_device.h"
@@ -255,6 +259,250 @@ __global__ __launch_bounds__(BLK) void tsqr_form_kernel(const T *Y, int64_t ldy,
   }
 }
 
+// ---- one wave per tile (k <= 32) ----------------------------------------------
+// A wave factors a tile of 64 RPL rows on its own: lane L holds rows L, L+64,
+// ..., L + 64 (RPL-1) of the tile (each column load is coalesced), so every
+// reduction is a wave reduction and no Householder step waits at a workgroup
+// barrier.  The 4 waves of a workgroup take 4 consecutive tiles.  The steps
+// are those of the workgroup kernels above (dgeqr2 / dorg2r order, the same
+// windows); only the order of the additions inside a reduction differs.
+
+// A value that is uniform across the wave, read from lane l (l uniform).
+template <typename T>
+__device__ __forceinline__ T ts_readlane(T v, int l) {
+  if constexpr (sizeof(T) == 8) {
+    const uint64_t u = __builtin_bit_cast(uint64_t, v);
+    const uint32_t lo = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)u, l);
+    const uint32_t hi = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)(u >> 32), l);
+    return __builtin_bit_cast(T, ((uint64_t)hi << 32) | lo);
+  } else {
+    return __builtin_bit_cast(T, __builtin_amdgcn_readlane(__builtin_bit_cast(int, v), l));
+  }
+}
+
+// Lane l returns sum over the tile of v * x[c] for column c = OFF + (l & (KS-1)):
+// each lane first adds its RPL rows, then the wave reduce-scatters as ts_dots.
+template <typename T, int K, int RPL, int OFF>
+__device__ __forceinline__ T tw_dots(const T (&x)[RPL][K], const T (&v)[RPL], int lane) {
+  constexpr int KS = K - OFF, H = KS / 2;
+  static_assert(KS >= 2 && (KS & (KS - 1)) == 0 && KS <= 64, "window must be a power of two up to 64");
+  T p[H];
+  {
+    const bool hi = (lane & H) != 0;
+#pragma unroll
+    for (int i = 0; i < H; i++) {
+      T a = v[0] * x[0][OFF + i], b = v[0] * x[0][OFF + H + i];
+#pragma unroll
+      for (int r = 1; r < RPL; r++) {
+        a += v[r] * x[r][OFF + i];
+        b += v[r] * x[r][OFF + H + i];
+      }
+      p[i] = ts_keep_add(a, b, hi, H);
+    }
+  }
+#pragma unroll
+  for (int m = H / 2; m >= 1; m >>= 1) {
+    const bool hi = (lane & m) != 0;
+#pragma unroll
+    for (int i = 0; i < m; i++) p[i] = ts_keep_add(p[i], p[i + m], hi, m);
+  }
+#pragma unroll
+  for (int m = KS; m < 64; m <<= 1) p[0] += __shfl_xor(p[0], m, 64);
+  return p[0];
+}
+
+template <typename T, int K, int RPL, int OFF>
+__device__ __forceinline__ void tw_factor_phase(T (&x)[RPL][K], int j0, int j1, int lane, T &tau_l) {
+#pragma nounroll
+  for (int j = j0; j < j1; j++) {
+    T xj[RPL];
+    T s = 0;
+#pragma unroll
+    for (int r = 0; r < RPL; r++) {
+      T e = x[r][OFF];
+#pragma unroll
+      for (int c = OFF + 1; c < K; c++) e = c == j ? x[r][c] : e;
+      xj[r] = e;
+      s += (lane + 64 * r > j) ? e * e : (T)0;
+    }
+    const T sigma = ts_wave_sum(s);
+    const T alpha = ts_readlane(xj[0], j);       // row j: lane j, first row slot (j < K <= 64)
+    T tau = 0;
+    if (sigma != (T)0) {                         // else H = I, as dlarfg does for a zero column
+      const T nrm = sqrt(alpha * alpha + sigma);
+      const T beta = alpha >= (T)0 ? -nrm : nrm;
+      tau = (beta - alpha) / beta;
+      const T scal = (T)1 / (alpha - beta);
+      T v[RPL];
+#pragma unroll
+      for (int r = 0; r < RPL; r++) {
+        const int row = lane + 64 * r;
+        v[r] = row > j ? xj[r] * scal : (row == j ? (T)1 : (T)0);
+      }
+      const T ws = tw_dots<T, K, RPL, OFF>(x, v, lane);
+#pragma unroll
+      for (int c = OFF; c < K; c++) {
+        const T w = ts_readlane(ws, c - OFF);
+        if (c > j) {
+#pragma unroll
+          for (int r = 0; r < RPL; r++) x[r][c] -= (tau * v[r]) * w;
+        }
+      }
+#pragma unroll
+      for (int r = 0; r < RPL; r++) {
+        const int row = lane + 64 * r;
+        const T nj = row > j ? v[r] : (row == j ? beta : xj[r]);
+#pragma unroll
+        for (int c = OFF; c < K; c++)
+          if (c == j) x[r][c] = nj;
+      }
+    }
+    if (lane == j) tau_l = tau;
+  }
+}
+
+template <typename T, int K, int RPL, int OFF>
+__device__ __forceinline__ void tw_factor_all(T (&x)[RPL][K], int kk, int lane, T &tau_l) {
+  constexpr int KS = K - OFF, JE = KS == 8 ? K : OFF + KS / 2;
+  tw_factor_phase<T, K, RPL, OFF>(x, OFF, min(kk, JE), lane, tau_l);
+  if constexpr (KS > 8) tw_factor_all<T, K, RPL, JE>(x, kk, lane, tau_l);
+}
+
+// tsqr_factor_kernel with one wave per tile: G tiles of 64 RPL rows.
+template <typename T, int K, int RPL>
+__global__ __launch_bounds__(BLK) void tsqr_factor_wave_kernel(const T *A, int64_t lda, int64_t rows, int k, T *Y,
+                                                               int64_t ldy, T *__restrict__ tau, T *__restrict__ R,
+                                                               int64_t ldr, int64_t G) {
+  constexpr int TR = 64 * RPL;
+  static_assert(K <= 64, "row j of a tile must sit in lane j");
+  const int lane = threadIdx.x & 63;
+  const int64_t b = (int64_t)blockIdx.x * (BLK / 64) + __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  if (b >= G) return;                           // the whole wave: no barrier follows
+  const int64_t r0 = b * TR;
+  const int nb = (int)min((int64_t)TR, rows - r0);
+  const int kk = min(k, nb);
+  T x[RPL][K];
+#pragma unroll
+  for (int r = 0; r < RPL; r++) {
+    const int row = lane + 64 * r;
+#pragma unroll
+    for (int c = 0; c < K; c++) x[r][c] = (c < k && row < nb) ? A[r0 + row + (int64_t)c * lda] : (T)0;
+  }
+  T tau_l = 0;
+  tw_factor_all<T, K, RPL, 0>(x, kk, lane, tau_l);
+  if (Y) {
+#pragma unroll
+    for (int r = 0; r < RPL; r++) {
+      const int row = lane + 64 * r;
+      if (row < nb) {
+#pragma unroll
+        for (int c = 0; c < K; c++)
+          if (c < k) Y[r0 + row + (int64_t)c * ldy] = x[r][c];
+      }
+    }
+  }
+  if (lane < k) {                               // rows 0..k-1 of the tile: first row slot
+    const int64_t rr = b * k + lane;
+#pragma unroll
+    for (int c = 0; c < K; c++)
+      if (c < k) R[rr + (int64_t)c * ldr] = (lane < nb && c >= lane) ? x[0][c] : (T)0;
+    tau[rr] = lane < kk ? tau_l : (T)0;
+  }
+}
+
+// Applies H_j for j1 > j >= j0 (descending) to the rows q of the tile; the
+// Householder vector of step j is read one step ahead.
+template <typename T, int K, int RPL, int OFF>
+__device__ __forceinline__ void tw_form_phase(T (&q)[RPL][K], int j1, int j0, int lane, const T *ytile, int64_t ldy,
+                                              int nb, const T *tau_b) {
+  T yn[RPL];
+  if (j1 - 1 >= j0) {
+#pragma unroll
+    for (int r = 0; r < RPL; r++) {
+      const int row = lane + 64 * r;
+      yn[r] = row < nb ? ytile[row + (int64_t)(j1 - 1) * ldy] : (T)0;
+    }
+  }
+#pragma nounroll
+  for (int j = j1 - 1; j >= j0; j--) {
+    T y[RPL];
+#pragma unroll
+    for (int r = 0; r < RPL; r++) y[r] = yn[r];
+    if (j - 1 >= j0) {
+#pragma unroll
+      for (int r = 0; r < RPL; r++) {
+        const int row = lane + 64 * r;
+        yn[r] = row < nb ? ytile[row + (int64_t)(j - 1) * ldy] : (T)0;
+      }
+    }
+    const T tj = tau_b[j];
+    if (tj == (T)0) continue;
+    T v[RPL];
+#pragma unroll
+    for (int r = 0; r < RPL; r++) {
+      const int row = lane + 64 * r;
+      v[r] = row > j ? y[r] : (row == j ? (T)1 : (T)0);
+    }
+    const T ws = tw_dots<T, K, RPL, OFF>(q, v, lane);
+#pragma unroll
+    for (int c = OFF; c < K; c++) {
+      const T w = ts_readlane(ws, c - OFF);
+#pragma unroll
+      for (int r = 0; r < RPL; r++) q[r][c] -= (tj * v[r]) * w;
+    }
+  }
+}
+
+template <typename T, int K, int RPL, int OFF>
+__device__ __forceinline__ void tw_form_all(T (&q)[RPL][K], int kk, int lane, const T *ytile, int64_t ldy, int nb,
+                                            const T *tau_b) {
+  constexpr int KS = K - OFF, JE = KS == 8 ? K : OFF + KS / 2;
+  if constexpr (KS > 8) tw_form_all<T, K, RPL, JE>(q, kk, lane, ytile, ldy, nb, tau_b);
+  tw_form_phase<T, K, RPL, OFF>(q, min(kk, JE), OFF, lane, ytile, ldy, nb, tau_b);
+}
+
+// tsqr_form_kernel with one wave per tile: G tiles of 64 RPL rows.
+template <typename T, int K, int RPL>
+__global__ __launch_bounds__(BLK) void tsqr_form_wave_kernel(const T *Y, int64_t ldy, int64_t rows, int k,
+                                                             const T *__restrict__ tau, const T *__restrict__ C,
+                                                             int64_t ldc, T *out, int64_t ldo, const T *__restrict__ sgn,
+                                                             int64_t G) {
+  constexpr int TR = 64 * RPL;
+  const int lane = threadIdx.x & 63;
+  const int64_t b = (int64_t)blockIdx.x * (BLK / 64) + __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  if (b >= G) return;
+  const int64_t r0 = b * TR;
+  const int nb = (int)min((int64_t)TR, rows - r0);
+  const int kk = min(k, nb);
+  const T *ytile = Y + r0;
+  const T *tau_b = tau + b * k;
+  T q[RPL][K];
+  if (C) {
+    // rows of C_b beyond the tile's rows meet zero rows of its R
+#pragma unroll
+    for (int r = 0; r < RPL; r++)
+#pragma unroll
+      for (int c = 0; c < K; c++)
+        q[r][c] = (r == 0 && lane < kk && c < k) ? C[b * k + lane + (int64_t)c * ldc] : (T)0;
+    tw_form_phase<T, K, RPL, 0>(q, kk, 0, lane, ytile, ldy, nb, tau_b);
+  } else {
+#pragma unroll
+    for (int r = 0; r < RPL; r++)
+#pragma unroll
+      for (int c = 0; c < K; c++) q[r][c] = (lane + 64 * r == c) ? (T)1 : (T)0;
+    tw_form_all<T, K, RPL, 0>(q, kk, lane, ytile, ldy, nb, tau_b);
+  }
+#pragma unroll
+  for (int r = 0; r < RPL; r++) {
+    const int row = lane + 64 * r;
+    if (row < nb) {
+#pragma unroll
+      for (int c = 0; c < K; c++)
+        if (c < k) out[r0 + row + (int64_t)c * ldo] = sgn ? q[r][c] * sgn[c] : q[r][c];
+    }
+  }
+}
+
 // ---- host side ----------------------------------------------------------------
 
 void tsqr_signs(const double *Qtop, const double *R, int64_t n, int k, double *f) {
@@ -294,8 +542,26 @@ struct TsLevel {
   T *tau;
 };
 
+// Up to 32 columns the tree uses the one-wave tile kernels (128-row tiles
+// for 17..32 columns, 256-row tiles below: RPL rows per lane in registers);
+// wider blocks use the workgroup kernels on 256-row tiles.
 template <typename T, int K>
 int tsqr_t(BV &V, int l, int k, bool formq, double *R_host, double *f, hipStream_t s) {
+  constexpr int RPL = K <= 16 ? 4 : 2;
+  constexpr int TR = K <= 32 ? 64 * RPL : TS_ROWS;
+  auto factor = [&](const T *A, int64_t lda, int64_t rows, int G, T *Y, int64_t ldy, T *tau, T *R, int64_t ldr) {
+    if constexpr (K <= 32)
+      tsqr_factor_wave_kernel<T, K, RPL><<<(G + 3) / 4, BLK, 0, s>>>(A, lda, rows, k - l, Y, ldy, tau, R, ldr, G);
+    else
+      tsqr_factor_kernel<T, K><<<G, BLK, 0, s>>>(A, lda, rows, k - l, Y, ldy, tau, R, ldr);
+  };
+  auto form = [&](const T *Y, int64_t ldy, int64_t rows, int G, const T *tau, const T *C, int64_t ldc, T *out,
+                  int64_t ldo, const T *sg) {
+    if constexpr (K <= 32)
+      tsqr_form_wave_kernel<T, K, RPL><<<(G + 3) / 4, BLK, 0, s>>>(Y, ldy, rows, k - l, tau, C, ldc, out, ldo, sg, G);
+    else
+      tsqr_form_kernel<T, K><<<G, BLK, 0, s>>>(Y, ldy, rows, k - l, tau, C, ldc, out, ldo, sg);
+  };
   const int nk = k - l;
   const int64_t n = V.n;
   // level sizes: rows_{i+1} = tiles_i * nk until one tile is left
@@ -303,7 +569,7 @@ int tsqr_t(BV &V, int l, int k, bool formq, double *R_host, double *f, hipStream
   size_t need = 0;
   auto al = [](size_t e) { return (e + 63) / 64 * 64; };
   for (int64_t rows = n;;) {
-    const int64_t G = (rows + TS_ROWS - 1) / TS_ROWS;
+    const int64_t G = (rows + TR - 1) / TR;
     SA_CHECK(G <= (int64_t)1 << 30, "TSQR: too many tiles");
     L.push_back({nullptr, 0, rows, (int)G, nullptr});
     need += al((size_t)G * nk);                             // tau
@@ -311,7 +577,7 @@ int tsqr_t(BV &V, int l, int k, bool formq, double *R_host, double *f, hipStream
     rows = G * nk;
     need += al((size_t)rows * nk);                          // next level's matrix
   }
-  need += al((size_t)nk * nk) + al((size_t)TS_ROWS * nk) + al(nk);   // R, probe tile, signs
+  need += al((size_t)nk * nk) + al((size_t)TR * nk) + al(nk);   // R, probe tile, signs
   thread_local TsWork W;
   if (W.bytes < need * sizeof(T)) {
     if (W.buf) (void)hipFree(W.buf);
@@ -335,14 +601,13 @@ int tsqr_t(BV &V, int l, int k, bool formq, double *R_host, double *f, hipStream
   T *Rtop = p;
   p += al((size_t)nk * nk);
   T *probe = p;
-  p += al((size_t)TS_ROWS * nk);
+  p += al((size_t)TR * nk);
   T *sgn = p;
   const int top = (int)L.size() - 1;
   for (int i = 0; i <= top; i++) {
     T *Rout = i == top ? Rtop : L[i + 1].M;
     const int64_t ldr = i == top ? nk : L[i + 1].ld;
-    tsqr_factor_kernel<T, K><<<L[i].G, BLK, 0, s>>>(L[i].M, L[i].ld, L[i].rows, nk, formq ? L[i].M : nullptr,
-                                                    L[i].ld, L[i].tau, Rout, ldr);
+    factor(L[i].M, L[i].ld, L[i].rows, L[i].G, formq ? L[i].M : nullptr, L[i].ld, L[i].tau, Rout, ldr);
   }
   SA_HIP(hipGetLastError());
   std::vector<T> buf((size_t)nk * nk);
@@ -351,23 +616,21 @@ int tsqr_t(BV &V, int l, int k, bool formq, double *R_host, double *f, hipStream
   for (size_t e = 0; e < buf.size(); e++) R_host[e] = (double)buf[e];
   if (!formq) return 0;
   for (int i = top; i >= 1; i--)
-    tsqr_form_kernel<T, K><<<L[i].G, BLK, 0, s>>>(L[i].M, L[i].ld, L[i].rows, nk, L[i].tau,
-                                                  i == top ? nullptr : L[i + 1].M, i == top ? 0 : L[i + 1].ld, L[i].M,
-                                                  L[i].ld, nullptr);
+    form(L[i].M, L[i].ld, L[i].rows, L[i].G, L[i].tau, i == top ? nullptr : L[i + 1].M, i == top ? 0 : L[i + 1].ld,
+         L[i].M, L[i].ld, nullptr);
   const T *C0 = top == 0 ? nullptr : L[1].M;
   const int64_t ldc0 = top == 0 ? 0 : L[1].ld;
   // the first nk rows of Q (tile 0 holds them: n >= nk) for the signs
-  tsqr_form_kernel<T, K><<<1, BLK, 0, s>>>(L[0].M, L[0].ld, L[0].rows, nk, L[0].tau, C0, ldc0, probe, TS_ROWS, nullptr);
+  form(L[0].M, L[0].ld, L[0].rows, 1, L[0].tau, C0, ldc0, probe, TR, nullptr);
   SA_HIP(hipGetLastError());
-  SA_HIP(hipMemcpy2DAsync(buf.data(), sizeof(T) * nk, probe, sizeof(T) * TS_ROWS, sizeof(T) * nk, nk,
+  SA_HIP(hipMemcpy2DAsync(buf.data(), sizeof(T) * nk, probe, sizeof(T) * TR, sizeof(T) * nk, nk,
                           hipMemcpyDeviceToHost, s));
   SA_HIP(hipStreamSynchronize(s));
   std::vector<double> Qtop(buf.begin(), buf.end());
   tsqr_signs(Qtop.data(), R_host, n, nk, f);
   for (int j = 0; j < nk; j++) buf[j] = (T)f[j];
   SA_HIP(hipMemcpyAsync(sgn, buf.data(), sizeof(T) * nk, hipMemcpyHostToDevice, s));
-  tsqr_form_kernel<T, K><<<L[0].G, BLK, 0, s>>>(L[0].M, L[0].ld, L[0].rows, nk, L[0].tau, C0, ldc0, L[0].M, L[0].ld,
-                                                sgn);
+  form(L[0].M, L[0].ld, L[0].rows, L[0].G, L[0].tau, C0, ldc0, L[0].M, L[0].ld, sgn);
   SA_HIP(hipGetLastError());
   SA_HIP(hipStreamSynchronize(s));
   return 0;
