@@ -306,6 +306,17 @@ int sa_eps_set_st(sa_eps eps, int st_type, int shift_given, double shift);
 #define SA_PC_JACOBI 1
 #define SA_PC_LU 2
 int sa_eps_set_st_ksp(sa_eps eps, int pc_type, double rtol, int max_it);
+/* KSP type of the iterative solves (with SA_PC_JACOBI or SA_PC_NONE):
+   SA_KSP_BCGS (BiCGStab, the default here) or SA_KSP_GMRES (GMRES(restart),
+   right preconditioned, classical Gram-Schmidt without refinement, the
+   convergence test on the recurrence residual: KSPGMRES with
+   KSPGMRESSetRestart, PETSc's default KSP type with restart 30;
+   src/ksp/ksp/impls/gmres in PETSc, selected by STGetKSP + KSPSetType at
+   the reference's st/interface/stsles.c).  restart in [1, 107] (the FP64
+   Gram-Schmidt kernels' widest basis is 108 columns). */
+#define SA_KSP_BCGS 0
+#define SA_KSP_GMRES 1
+int sa_eps_set_st_ksp_type(sa_eps eps, int ksp_type, int restart);
 /* EPSBuildBalance_Krylov (src/eps/interface/epsdefault.c:443-504) for an
  * FP64 operator: the diagonal D (n host doubles) of D A D^-1 after its
  * iterations, twoside with A^T; the +-1 vectors come from the library's
@@ -442,6 +453,7 @@ typedef struct {
   double  seconds, seconds32, seconds_ds;
   double  bytes_spmv, bytes_bv, bytes_restart;
   int64_t true_residuals;   /* explicit residuals taken by the convergence test */
+  int64_t ksp_its;          /* iterations of the shift-and-invert linear solves (KSPGetTotalIterations) */
 } sa_eps_stats;
 int sa_eps_get_stats(sa_eps eps, sa_eps_stats *st);
 

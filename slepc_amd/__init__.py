@@ -435,6 +435,11 @@ class EPS:
         ("jacobi") or no ("none") preconditioner."""
         check(lib().sa_eps_set_st_ksp(self.h, {"none": 0, "jacobi": 1, "lu": 2}[pc], float(rtol), int(max_it)))
 
+    def setSTKSPType(self, ksp_type="bcgs", restart=30):
+        """KSPSetType of the iterative solves: "bcgs" (BiCGStab) or "gmres"
+        (GMRES(restart), right preconditioned, as KSPGMRES)."""
+        check(lib().sa_eps_set_st_ksp_type(self.h, {"bcgs": 0, "gmres": 1}[ksp_type], int(restart)))
+
     def setBalance(self, balance="twoside", its=5, cutoff=1e-8):
         """EPSSetBalance: "none", "oneside" or "twoside" (non-Hermitian problems)."""
         check(lib().sa_eps_set_balance(self.h, {"none": 0, "oneside": 1, "twoside": 2}[balance], int(its),

@@ -27,7 +27,8 @@ class EpsStats(C.Structure):
                 ("gs_passes32", C.c_int64), ("its", C.c_int), ("its32", C.c_int), ("nconv", C.c_int),
                 ("reason", C.c_int), ("seconds", C.c_double), ("seconds32", C.c_double),
                 ("seconds_ds", C.c_double), ("bytes_spmv", C.c_double), ("bytes_bv", C.c_double),
-                ("bytes_restart", C.c_double), ("true_residuals", C.c_int64)]
+                ("bytes_restart", C.c_double), ("true_residuals", C.c_int64),
+                ("ksp_its", C.c_int64)]
 
 
 _I32P = np.ctypeslib.ndpointer(np.int32, flags="C")
@@ -118,6 +119,7 @@ SIGNATURES = {
     "sa_eps_lanczos_get_reorthog": (C.c_int, [_VP, C.POINTER(C.c_int)]),
     "sa_eps_set_st": (C.c_int, [_VP, C.c_int, C.c_int, C.c_double]),
     "sa_eps_set_st_ksp": (C.c_int, [_VP, C.c_int, C.c_double, C.c_int]),
+    "sa_eps_set_st_ksp_type": (C.c_int, [_VP, C.c_int, C.c_int]),
     "sa_eps_get_left_eigenvector": (C.c_int, [_VP, C.c_int, _VP, _VP]),
     "sa_eps_set_dimensions": (C.c_int, [_VP, C.c_int, C.c_int, C.c_int]),
     "sa_eps_set_tolerances": (C.c_int, [_VP, C.c_double, C.c_int]),

@@ -12,7 +12,6 @@
 
 namespace sa {
 
-int launch_gs_reset(BV &V, hipStream_t s);
 
 static int bv_alloc_impl(BV &V, int64_t n, int m, int prec);
 

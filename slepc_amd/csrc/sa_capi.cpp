@@ -701,6 +701,16 @@ int sa_eps_set_st_ksp(sa_eps E, int pc_type, double rtol, int max_it) {
   return 0;
 }
 
+int sa_eps_set_st_ksp_type(sa_eps E, int ksp_type, int restart) {
+  SA_CHECK(E && (ksp_type == SA_KSP_BCGS || ksp_type == SA_KSP_GMRES), "unsupported KSP type");
+  // the FP64 Gram-Schmidt tile holds columns 0..107: gs_lds_bytes(107, 8) + 6 KB fits 64 KB
+  SA_CHECK(ksp_type != SA_KSP_GMRES || (restart >= 1 && restart <= 107), "GMRES restart must be in [1, 107]");
+  E->p.ksp_gmres = ksp_type == SA_KSP_GMRES;
+  if (ksp_type == SA_KSP_GMRES) E->p.ksp_restart = restart;
+  E->solved = false;
+  return 0;
+}
+
 int sa_eps_krylovschur_set_restart(sa_eps E, double keep) {
   SA_CHECK(E, "null EPS");
   SA_CHECK(keep >= 0.1 && keep <= 0.9, "The keep argument must be in the range [0.1,0.9]");
@@ -1048,6 +1058,7 @@ int sa_eps_solve(sa_eps E) {
   s.bytes_bv = E->res.bytes_bv + E->res32.bytes_bv;
   s.bytes_restart = E->res.bytes_restart + E->res32.bytes_restart;
   s.true_residuals = E->res.true_residuals + E->res32.true_residuals;
+  s.ksp_its = E->res.ksp_its + E->res32.ksp_its;
   return 0;
 }
 

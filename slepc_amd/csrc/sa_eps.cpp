@@ -622,12 +622,14 @@ int ks_solve(const KsParams &p, const Mat &A, BV &V, bool have_initial, KsResult
     KspWork &w;
     ~KspFree() { ksp_free(w); }
   } kw_free{kw};
-  if (sinv) SA_TRY(ksp_alloc(kw, A, p.st_shift, p.ksp_jacobi, p.ksp_direct));
+  if (sinv) SA_TRY(ksp_alloc(kw, A, p.st_shift, p.ksp_jacobi, p.ksp_direct, p.ksp_gmres, p.ksp_restart));
   // y = Op x: the operator of the spectral transformation (local Lanczos)
   std::function<int(const void *, void *)> op_apply = [&](const void *x, void *y) -> int {
     if (!sinv) return launch_spmv(A, x, y, stream());
     int its = 0;
-    return ksp_solve(A, kw, (const double *)x, (double *)y, p.ksp_rtol, p.ksp_max_it, &its);
+    const int rc = ksp_solve(A, kw, (const double *)x, (double *)y, p.ksp_rtol, p.ksp_max_it, &its);
+    r.ksp_its += its;
+    return rc;
   };
   struct DevBuf {
     void *p = nullptr;

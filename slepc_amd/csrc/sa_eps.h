@@ -31,6 +31,8 @@ struct KsParams {
   double st_shift = 0.0;       // sigma of the shift-and-invert operator
   bool ksp_direct = true;      // KSPPREONLY + PCLU (banded LU on the device); else BiCGStab
   bool ksp_jacobi = true;      // PCJACOBI for the BiCGStab solves (else none)
+  bool ksp_gmres = false;      // GMRES(ksp_restart) instead of BiCGStab for the iterative solves
+  int ksp_restart = 30;
   double ksp_rtol = 1e-12;     // relative residual of each solve
   int ksp_max_it = 10000;
   bool lanczos_local = false;  // EPSLanczosSetReorthog: local (else full)
@@ -47,7 +49,7 @@ struct KsResult {
   int64_t matvecs = 0, gs_passes = 0;
   double seconds = 0.0, seconds_ds = 0.0;
   double bytes_spmv = 0.0, bytes_bv = 0.0, bytes_restart = 0.0;
-  int64_t ksp_its = 0;             // BiCGStab iterations (shift-and-invert)
+  int64_t ksp_its = 0;             // linear-solver iterations (shift-and-invert)
   int64_t true_residuals = 0;      // explicit residuals of the convergence test
   std::vector<int> perm;           // sorted position -> column of V
   std::vector<double> eigr, eigi, errest;

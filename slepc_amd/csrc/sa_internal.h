@@ -112,6 +112,8 @@ int launch_spmv(const Mat &A, const void *x, void *y, hipStream_t s);
 // One Arnoldi/orthonormalisation step for column c of V; spmv_from >= 0 first
 // computes V(:,c) = A V(:,spmv_from).  eta: DGKS threshold.
 int launch_gs_column(BV &V, const Mat *A, int c, double eta, hipStream_t s);
+// Clears the Gram-Schmidt state machine of V before a sweep.
+int launch_gs_reset(BV &V, hipStream_t s);
 int gs_min_chunks(bool big);   // fewest 64-row chunks per workgroup of the GS kernels (big: above GS_NT_BYTES)
 int concurrent_solves();      // the application's hint (sa_set_concurrent_solves)
 int gs_max_blocks(bool big);  // most workgroups of the GS kernels per basis (big: above GS_NT_BYTES)
@@ -172,12 +174,15 @@ struct KspWork {
   double sigma = 0.0;
   bool jacobi = true;
   bool direct = false;
+  bool gmres = false;          // GMRES(restart) instead of BiCGStab for the iterative solves
+  int restart = 30;
+  BV gv;                       // GMRES Arnoldi basis, restart + 1 columns
   double *ab = nullptr;        // band storage, ldab x n
   int *ipiv = nullptr;         // n pivots + info
   int64_t ab_n = 0, ldab = 0;
   int kl = 0, ku = 0;
 };
-int ksp_alloc(KspWork &w, const Mat &A, double sigma, bool jacobi, bool direct);
+int ksp_alloc(KspWork &w, const Mat &A, double sigma, bool jacobi, bool direct, bool gmres = false, int restart = 30);
 void ksp_free(KspWork &w);
 // rtol relative to ||b||; error when max_it is reached or on breakdown
 int ksp_solve(const Mat &A, KspWork &w, const double *b_dev, double *x_dev, double rtol, int max_it, int *its);

@@ -1,9 +1,10 @@
 // Linear solves for the shift-and-invert spectral transformation
 // (STSINVERT: src/sys/classes/st/impls/sinvert/sinvert.c, STMatSolve in
-// st/interface/stsles.c): x = (A - sigma I)^-1 b by BiCGStab with right
-// Jacobi (or no) preconditioning, in FP64, every scalar kept in device
-// memory so that one iteration is a chain of stream-ordered kernels with no
-// host round trip; the host reads the residual norm every KSP_CHECK
+// st/interface/stsles.c): x = (A - sigma I)^-1 b by banded LU (the default,
+// PREONLY + LU), by BiCGStab or by GMRES(m), the iterative ones with right
+// Jacobi (or no) preconditioning, in FP64.  BiCGStab keeps every scalar in
+// device memory so that one iteration is a chain of stream-ordered kernels
+// with no host round trip; the host reads the residual norm every KSP_CHECK
 // iterations.  Dot products are per-workgroup partials summed by one
 // workgroup in a fixed order (deterministic).
 #include <algorithm>
@@ -171,6 +172,47 @@ __global__ __launch_bounds__(BLK) void ksp_jacobi_kernel(int n, const int32_t *_
   for (int32_t q = rp[i]; q < rp[i + 1]; q++)
     if (ci[q] == i) d += val[q];
   dinv[i] = d != 0.0 ? 1.0 / d : 1.0;
+}
+
+// ---------------------------------------------------------------------------
+// GMRES(m) with right preconditioning (KSPGMRES, PETSc's default KSP type,
+// restart 30): the Arnoldi basis is a BV orthonormalised by the fused
+// Gram-Schmidt kernels with classical Gram-Schmidt and refinement never (the
+// defaults of KSPGMRES), which leave the Hessenberg column in the coefficient
+// buffer; these kernels do the rest of each step.
+// ---------------------------------------------------------------------------
+
+// dst = dinv .* src (src copied when dinv is null)
+__global__ __launch_bounds__(BLK) void gm_prec_kernel(int64_t n, const double *__restrict__ dinv,
+                                                      const double *__restrict__ src, double *__restrict__ dst) {
+  const int64_t i = (int64_t)blockIdx.x * BLK + threadIdx.x;
+  if (i >= n) return;
+  dst[i] = dinv ? dinv[i] * src[i] : src[i];
+}
+
+// y -= sigma z
+__global__ __launch_bounds__(BLK) void gm_shift_kernel(int64_t n, double *y, const double *__restrict__ z,
+                                                       double sigma) {
+  const int64_t i = (int64_t)blockIdx.x * BLK + threadIdx.x;
+  if (i >= n) return;
+  y[i] -= sigma * z[i];
+}
+
+// r = b - (t - sigma x), t = A x
+__global__ __launch_bounds__(BLK) void gm_resid_kernel(int64_t n, const double *__restrict__ b,
+                                                       const double *__restrict__ t, const double *__restrict__ x,
+                                                       double sigma, double *__restrict__ r) {
+  const int64_t i = (int64_t)blockIdx.x * BLK + threadIdx.x;
+  if (i >= n) return;
+  r[i] = b[i] - (t[i] - sigma * x[i]);
+}
+
+// x += dinv .* z (z when dinv is null)
+__global__ __launch_bounds__(BLK) void gm_update_kernel(int64_t n, double *x, const double *__restrict__ dinv,
+                                                        const double *__restrict__ z) {
+  const int64_t i = (int64_t)blockIdx.x * BLK + threadIdx.x;
+  if (i >= n) return;
+  x[i] += dinv ? dinv[i] * z[i] : z[i];
 }
 
 // ---------------------------------------------------------------------------
@@ -349,10 +391,13 @@ static int band_lu_setup(KspWork &w, const Mat &A, double sigma) {
   return 0;
 }
 
-int ksp_alloc(KspWork &w, const Mat &A, double sigma, bool jacobi, bool direct) {
+int ksp_alloc(KspWork &w, const Mat &A, double sigma, bool jacobi, bool direct, bool gmres, int restart) {
   SA_CHECK(A.prec == SA_FP64, "the shift-and-invert solves run in FP64");
+  SA_CHECK(!gmres || (restart >= 1 && restart <= 107), "GMRES restart must be in [1, 107]");
   const int64_t n = A.n;
   w.direct = direct;
+  w.gmres = gmres && !direct;
+  w.restart = restart;
   if (direct) {
     w.n = n;
     w.sigma = sigma;
@@ -371,10 +416,16 @@ int ksp_alloc(KspWork &w, const Mat &A, double sigma, bool jacobi, bool direct) 
                        A.rowptr, A.col, (const double *)A.val, sigma, w.buf + 8 * n);
     SA_HIP(hipGetLastError());
   }
+  if (w.gmres && (w.gv.m != restart + 1 || w.gv.n != n)) {
+    bv_free(w.gv);
+    SA_TRY(bv_alloc(w.gv, n, restart + 1, SA_FP64));
+    w.gv.refine = GS_REFINE_NEVER;   // KSP_GMRES_CGS_REFINE_NEVER, PETSc's default for KSPGMRES
+  }
   return 0;
 }
 
 void ksp_free(KspWork &w) {
+  bv_free(w.gv);
   if (w.buf) (void)hipFree(w.buf);
   if (w.ab) (void)hipFree(w.ab);
   if (w.ipiv) (void)hipFree(w.ipiv);
@@ -385,9 +436,117 @@ void ksp_free(KspWork &w) {
   w.n = 0;
 }
 
+namespace {
+
+// GMRES(m), right preconditioned: each cycle starts from r = b - (A - sigma I) x
+// in column 0 of the basis; step j builds column j+1 = (A - sigma I) M^-1 v_j,
+// orthonormalises it (the Hessenberg column in the coefficient buffer), and
+// Givens rotations on the host turn the least-squares residual into the
+// estimate |g_{j+1}|, tested against rtol ||b|| (KSPConvergedDefault); the
+// cycle ends by x += M^-1 V y.  A zero norm (exact breakdown) ends the solve.
+int gmres_solve(const Mat &A, KspWork &w, const double *b, double *x, double rtol, int max_it, int *its_out) {
+  const int64_t n = w.n;
+  hipStream_t s = stream();
+  BV &V = w.gv;
+  const int m = w.restart;
+  SA_CHECK(V.data && V.m == m + 1 && V.n == n, "GMRES workspace does not match the operator");
+  double *t = w.buf, *z = w.buf + n, *dinv = w.jacobi ? w.buf + 8 * n : nullptr;
+  double *part = w.buf + 9 * n, *spare = part + 2 * KSP_MAXG + S_NSCAL;
+  const int G = w.G;
+  const unsigned nb = (unsigned)std::max<int64_t>(1, (n + BLK - 1) / BLK);
+  const double sigma = w.sigma;
+  hipLaunchKernelGGL(ksp_dot2_kernel, dim3(G), dim3(BLK), 0, s, n, b, b, nullptr, nullptr, part, G);
+  hipLaunchKernelGGL(ksp_scalar_kernel, dim3(1), dim3(BLK), 0, s, part, G, 3, spare);
+  SA_HIP(hipGetLastError());
+  double hb = 0.0;
+  SA_HIP(hipMemcpyAsync(&hb, spare, sizeof(double), hipMemcpyDeviceToHost, s));
+  SA_HIP(hipMemsetAsync(x, 0, sizeof(double) * n, s));
+  SA_HIP(hipStreamSynchronize(s));
+  const double bnorm = std::sqrt(std::max(0.0, hb)), target = rtol * bnorm;
+  int its = 0;
+  if (bnorm == 0.0) {
+    if (its_out) *its_out = 0;
+    return 0;
+  }
+  std::vector<double> H((size_t)m * m), cs(m), sn(m), g(m + 1), h(m + 2), y(m);
+  for (bool first = true;; first = false) {
+    double *v0 = (double *)col_ptr(V, 0);
+    if (first) {
+      SA_HIP(hipMemcpyAsync(v0, b, sizeof(double) * n, hipMemcpyDeviceToDevice, s));
+    } else {
+      SA_TRY(launch_spmv(A, x, t, s));
+      hipLaunchKernelGGL(gm_resid_kernel, dim3(nb), dim3(BLK), 0, s, n, b, t, x, sigma, v0);
+    }
+    SA_TRY(launch_gs_reset(V, s));
+    SA_TRY(launch_gs_column(V, nullptr, 0, 0.7071, s));
+    SA_HIP(hipMemcpyAsync(V.st_host, V.st + 2, sizeof(GsState), hipMemcpyDeviceToHost, s));
+    SA_HIP(hipStreamSynchronize(s));
+    const double beta = V.st_host->nrm;
+    SA_CHECK(std::isfinite(beta), "shift-and-invert: GMRES residual is not finite in (A - sigma I) x = b");
+    if (beta <= target) break;                         // the true residual meets the test
+    SA_CHECK(its < max_it, "shift-and-invert: the linear solver did not converge within max_it iterations");
+    std::fill(g.begin(), g.end(), 0.0);
+    g[0] = beta;
+    int k = 0;                                         // columns of V in this cycle's update
+    bool done = false;
+    for (int j = 0; j < m && its < max_it; j++) {
+      its++;
+      double *vj = (double *)col_ptr(V, j), *vn = (double *)col_ptr(V, j + 1);
+      hipLaunchKernelGGL(gm_prec_kernel, dim3(nb), dim3(BLK), 0, s, n, dinv, vj, z);
+      SA_TRY(launch_spmv(A, z, vn, s));
+      hipLaunchKernelGGL(gm_shift_kernel, dim3(nb), dim3(BLK), 0, s, n, vn, z, sigma);
+      SA_TRY(launch_gs_column(V, nullptr, j + 1, 0.7071, s));
+      SA_HIP(hipMemcpyAsync(h.data(), V.dbuf + (size_t)(j + 1) * V.m, sizeof(double) * (j + 2),
+                            hipMemcpyDeviceToHost, s));
+      SA_HIP(hipMemcpyAsync(V.st_host, V.st + 2, sizeof(GsState), hipMemcpyDeviceToHost, s));
+      SA_HIP(hipStreamSynchronize(s));
+      const bool stop = V.st_host->stop != 0;
+      if (stop) h[j + 1] = 0.0;                        // the Krylov space is invariant
+      for (int i = 0; i < j; i++) {                    // earlier rotations
+        const double a = cs[i] * h[i] + sn[i] * h[i + 1];
+        h[i + 1] = -sn[i] * h[i] + cs[i] * h[i + 1];
+        h[i] = a;
+      }
+      const double r = std::hypot(h[j], h[j + 1]);
+      SA_CHECK(r != 0.0, "shift-and-invert: GMRES breakdown in (A - sigma I) x = b (singular system)");
+      cs[j] = h[j] / r;
+      sn[j] = h[j + 1] / r;
+      h[j] = r;
+      g[j + 1] = -sn[j] * g[j];
+      g[j] = cs[j] * g[j];
+      for (int i = 0; i <= j; i++) H[i + (size_t)j * m] = h[i];
+      k = j + 1;
+      if (std::fabs(g[j + 1]) <= target || stop) {
+        done = true;
+        break;
+      }
+    }
+    for (int i = k - 1; i >= 0; i--) {                // H(0:k,0:k) y = g(0:k)
+      double v = g[i];
+      for (int c = i + 1; c < k; c++) v -= H[i + (size_t)c * m] * y[c];
+      y[i] = v / H[i + (size_t)i * m];
+    }
+    SA_HIP(hipMemcpyAsync(V.dq, y.data(), sizeof(double) * k, hipMemcpyHostToDevice, s));
+    SA_TRY(launch_multvec(V, 0, k, 1.0, V.dq, 0.0, z, s));
+    hipLaunchKernelGGL(gm_update_kernel, dim3(nb), dim3(BLK), 0, s, n, x, dinv, z);
+    SA_HIP(hipGetLastError());
+    SA_HIP(hipStreamSynchronize(s));                   // y is rewritten by the next cycle
+    if (done) break;
+  }
+  if (its_out) *its_out = its;
+  return 0;
+}
+
+}  // namespace
+
 int ksp_solve(const Mat &A, KspWork &w, const double *b, double *x, double rtol, int max_it, int *its_out) {
   const int64_t n = w.n;
   hipStream_t s = stream();
+  if (w.gmres && !w.direct) {
+    SA_CHECK(n == A.n && w.buf, "solver workspace does not match the operator");
+    if (n == 0) return 0;
+    return gmres_solve(A, w, b, x, rtol, max_it, its_out);
+  }
   if (w.direct) {        // PREONLY + LU: x = U^-1 L^-1 P b
     SA_CHECK(n == A.n && w.ab, "solver workspace does not match the operator");
     if (n == 0) return 0;

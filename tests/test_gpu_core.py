@@ -969,6 +969,88 @@ def test_sinvert_matches_oracle(sa, orc, solver, case):
         assert E.computeError(i) < 10 * tol * max(1.0, nrm / abs(vals[i]))
 
 
+@pytest.mark.parametrize("solver,case", [("krylovschur", "markov15"), ("arnoldi", "markov15"),
+                                         ("lanczos", "test2"), ("krylovschur", "lap2d40")])
+def test_sinvert_gmres_matches_oracle(sa, orc, solver, case):
+    """Shift-and-invert with GMRES(m) and Jacobi (KSPGMRES, PETSc's default
+    KSP type: right preconditioning, classical Gram-Schmidt without
+    refinement, rtol 1e-13) against the oracle's exact solves: eigenvalues in
+    order to 1e-9 and residuals, as for BiCGStab.  Restarted GMRES can stall
+    on an indefinite A - sigma I (a numpy GMRES(30) on the Markov matrix at
+    0.5 stays at a relative residual of 0.56 from the second cycle on, while
+    GMRES(100) converges in 184 iterations), so the Markov case runs m = 100
+    and the 1-D Laplacian m = n = 30;
+    the positive definite 2-D Laplacian takes the default m = 30."""
+    from test_cpu_oracle import _lap1d, TEST2_INTERIOR
+    if case == "markov15":
+        Ao, herm, nev, tol, tgt, m = orc.markov(15), False, 3, 1e-8, 0.5, 100
+    elif case == "test2":
+        Ao, herm, nev, tol, tgt, m = _lap1d(orc, 30), True, 4, 1e-9, 2.1, 30
+    else:
+        Ao, herm, nev, tol, tgt, m = orc.lap2d(40, 37), True, 6, 1e-10, 0.0, 30
+    v0 = orc.rand_uniform(0, Ao.n)
+    E = sa.EPS()
+    E.setType(solver)
+    E.setOperators(sa.Mat(sa.CSR(Ao.n, Ao.rowptr, Ao.col, Ao.val)))
+    E.setProblemType(herm)
+    E.setWhichEigenpairs("TM")
+    E.setTarget(tgt)
+    E.setST("sinvert")
+    E.setSTKSP("jacobi", 1e-13, 20000)
+    E.setSTKSPType("gmres", m)
+    E.setDimensions(nev, 0)
+    E.setTolerances(tol, 500)
+    E.setInitialSpace(v0)
+    E.solve()
+    r = orc.eps_solve(Ao, nev, 0, tol, "TM", herm, v0=v0, target=tgt, st="sinvert", solver=solver, max_it=500)
+    assert E.getConverged() >= nev and r["nconv"] >= nev
+    assert E.getStats()["ksp_its"] > 0
+    pairs = [E.getEigenpair(i, False) for i in range(nev)]
+    vals = np.array([p[0] + 1j * p[1] for p in pairs])
+    ref = r["eigr"][:nev] + 1j * r["eigi"][:nev]
+    assert np.max(np.abs(vals - ref) / np.abs(ref)) < 1e-9
+    if case == "markov15":
+        assert list(np.round(vals.real, 5)) == [0.49842, 0.51928, 0.47897]
+    if case == "test2":
+        assert np.allclose(vals.real, TEST2_INTERIOR[30], atol=5e-6)
+    nrm = max(np.abs(Ao.val[Ao.rowptr[i]:Ao.rowptr[i + 1]]).sum() for i in range(Ao.n)) + abs(tgt)
+    for i in range(nev):
+        assert E.computeError(i) < 10 * tol * max(1.0, nrm / abs(vals[i]))
+
+
+def test_sinvert_gmres_restart_and_arguments(sa, orc):
+    """A shorter GMRES restart takes more iterations for the same values (the
+    2-D Laplacian at target 0, 6 smallest); restart outside [1, 126] and an
+    unknown KSP type are rejected."""
+    Ao = orc.lap2d(40, 37)
+    v0 = orc.rand_uniform(0, Ao.n)
+    out = {}
+    for restart in (5, 30):
+        E = sa.EPS()
+        E.setOperators(sa.Mat(sa.CSR(Ao.n, Ao.rowptr, Ao.col, Ao.val)))
+        E.setProblemType(True)
+        E.setWhichEigenpairs("TM")
+        E.setTarget(0.0)
+        E.setST("sinvert")
+        E.setSTKSP("jacobi", 1e-13, 50000)
+        E.setSTKSPType("gmres", restart)
+        E.setDimensions(6, 0)
+        E.setTolerances(1e-10, 500)
+        E.setInitialSpace(v0)
+        E.solve()
+        assert E.getConverged() >= 6
+        out[restart] = (np.array([E.getEigenpair(i, False)[0] for i in range(6)]), E.getStats()["ksp_its"])
+    assert np.allclose(out[5][0], out[30][0], rtol=1e-9, atol=0)
+    assert out[5][1] > out[30][1] > 0
+    E = sa.EPS()
+    with pytest.raises(sa.SlepcAmdError, match="restart"):
+        E.setSTKSPType("gmres", 108)
+    with pytest.raises(sa.SlepcAmdError, match="restart"):
+        E.setSTKSPType("gmres", 0)
+    with pytest.raises(sa.SlepcAmdError, match="unsupported KSP type"):
+        sa.check(sa.lib().sa_eps_set_st_ksp_type(E.h, 7, 30))
+
+
 def test_sinvert_laplacian_smallest(sa, orc):
     """The 2-D Laplacian on a 100 x 97 grid (C1's stencil and size, with
     distinct eigenvalues) with shift-and-invert at target 0: the 10 smallest
