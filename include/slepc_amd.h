@@ -394,6 +394,9 @@ int sa_eps_get_left_eigenvector(sa_eps eps, int i, double *yr_host, double *yi_h
    EPSLANCZOS (the reference ignores it there) or balancing. */
 int sa_eps_set_true_residual(sa_eps eps, int trueres);
 int sa_eps_get_true_residual(sa_eps eps, int *trueres);
+/* EPSSetDimensions (epsopts.c:401-425): ncv or mpd <= 0 take the defaults of
+   EPSSetDimensions_Default at solve time.  ncv is at most 107 (the FP64
+   Gram-Schmidt kernels hold 108 basis columns in their LDS tile). */
 int sa_eps_set_dimensions(sa_eps eps, int nev, int ncv, int mpd);
 int sa_eps_set_tolerances(sa_eps eps, double tol, int max_it);
 /* EPSSetInitialSpace (epssetup.c:613-626): nvec host vectors of length n,

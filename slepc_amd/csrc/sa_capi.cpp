@@ -703,8 +703,7 @@ int sa_eps_set_st_ksp(sa_eps E, int pc_type, double rtol, int max_it) {
 
 int sa_eps_set_st_ksp_type(sa_eps E, int ksp_type, int restart) {
   SA_CHECK(E && (ksp_type == SA_KSP_BCGS || ksp_type == SA_KSP_GMRES), "unsupported KSP type");
-  // the FP64 Gram-Schmidt tile holds columns 0..107: gs_lds_bytes(107, 8) + 6 KB fits 64 KB
-  SA_CHECK(ksp_type != SA_KSP_GMRES || (restart >= 1 && restart <= 107), "GMRES restart must be in [1, 107]");
+  SA_CHECK(ksp_type != SA_KSP_GMRES || (restart >= 1 && restart < GS_MAXC_FP64), "GMRES restart must be in [1, 107]");
   E->p.ksp_gmres = ksp_type == SA_KSP_GMRES;
   if (ksp_type == SA_KSP_GMRES) E->p.ksp_restart = restart;
   E->solved = false;
@@ -862,6 +861,7 @@ static int eps_setup_dims(sa_eps E, KsParams &p) {
   SA_CHECK(p.ncv >= p.nev + 1 || (p.ncv == p.nev && p.ncv == n), "The value of ncv must be at least nev+1");
   SA_CHECK(p.ncv <= p.nev + p.mpd, "The value of ncv must not be larger than nev+mpd");
   SA_CHECK(E->nini <= p.ncv, "The number of initial vectors is larger than ncv");   // epssetup.c:423
+  SA_CHECK(p.ncv < GS_MAXC_FP64, "ncv above 107 is beyond the FP64 Gram-Schmidt kernels (their LDS tile holds 108 columns)");
   if (!E->maxit_set) p.max_it = std::max(100, 2 * n / p.ncv);
   if (p.st_type == SA_ST_SINVERT && !E->shift_set) p.st_shift = p.target;   // STSetDefaultShift
   return 0;

@@ -108,8 +108,9 @@ def test_small_mixed_precision(sa, orc):
 
 def test_setup_rejects_what_the_reference_rejects(sa, orc):
     """nev <= 0, ncv = nev on a larger operator, ncv > nev + mpd and more
-    initial vectors than ncv are errors with the reference's messages; the
-    same EPS then solves once the dimensions are valid."""
+    initial vectors than ncv are errors with the reference's messages (and
+    ncv above the FP64 kernels' 107 columns an error of its own); the same
+    EPS then solves once the dimensions are valid."""
     Ah = sa.lap2d(10)
     E = sa.EPS()
     E.setOperators(sa.Mat(Ah))
@@ -131,6 +132,12 @@ def test_setup_rejects_what_the_reference_rejects(sa, orc):
     assert E.getConverged() >= 4
     for i in range(4):
         assert E.computeError(i) <= 1e-8
+    E2 = sa.EPS()                                 # 144 rows: ncv 108 is not clamped
+    E2.setOperators(sa.Mat(sa.lap2d(12)))
+    E2.setProblemType(True)
+    E2.setDimensions(10, 108)
+    with pytest.raises(RuntimeError, match="ncv above 107"):
+        E2.solve()
 
 
 def _csr4():
