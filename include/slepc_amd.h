@@ -312,8 +312,7 @@ int sa_eps_set_st_ksp(sa_eps eps, int pc_type, double rtol, int max_it);
    convergence test on the recurrence residual: KSPGMRES with
    KSPGMRESSetRestart, PETSc's default KSP type with restart 30;
    src/ksp/ksp/impls/gmres in PETSc, selected by STGetKSP + KSPSetType at
-   the reference's st/interface/stsles.c).  restart in [1, 107] (the FP64
-   Gram-Schmidt kernels' widest basis is 108 columns). */
+   the reference's st/interface/stsles.c).  restart in [1, 126]. */
 #define SA_KSP_BCGS 0
 #define SA_KSP_GMRES 1
 int sa_eps_set_st_ksp_type(sa_eps eps, int ksp_type, int restart);
@@ -395,8 +394,9 @@ int sa_eps_get_left_eigenvector(sa_eps eps, int i, double *yr_host, double *yi_h
 int sa_eps_set_true_residual(sa_eps eps, int trueres);
 int sa_eps_get_true_residual(sa_eps eps, int *trueres);
 /* EPSSetDimensions (epsopts.c:401-425): ncv or mpd <= 0 take the defaults of
-   EPSSetDimensions_Default at solve time.  ncv is at most 107 (the FP64
-   Gram-Schmidt kernels hold 108 basis columns in their LDS tile). */
+   EPSSetDimensions_Default at solve time.  ncv is at most 126 (the fused
+   Gram-Schmidt kernels handle 127 basis columns; an FP64 tile wider than 108
+   columns takes more than 64 KB of LDS, which gfx950 allows a workgroup). */
 int sa_eps_set_dimensions(sa_eps eps, int nev, int ncv, int mpd);
 int sa_eps_set_tolerances(sa_eps eps, double tol, int max_it);
 /* EPSSetInitialSpace (epssetup.c:613-626): nvec host vectors of length n,

@@ -20,6 +20,7 @@ thread_local std::string g_err;
 thread_local hipStream_t t_stream = nullptr;   // one stream per host thread
 std::atomic<int> g_device{-1};
 int g_cus = 0;
+int g_lds_max = 65536;   // LDS bytes a workgroup may use (160 KB on gfx950)
 std::mutex g_init_mu;
 double now() {
   return std::chrono::duration<double>(std::chrono::steady_clock::now().time_since_epoch()).count();
@@ -38,6 +39,7 @@ hipStream_t stream() {
   return t_stream;
 }
 int num_cus() { return g_cus; }
+int lds_max_bytes() { return g_lds_max; }
 
 // Called on entry to every C ABI function that touches the device.  It also
 // clears the runtime's last-error state, which a HIP error handled in an
@@ -123,6 +125,7 @@ int sa_init(int device) {
   SA_CHECK(std::string(prop.gcnArchName).rfind("gfx950", 0) == 0,
            std::string("libslepc_amd is built for gfx950, found ") + prop.gcnArchName);
   g_cus = prop.multiProcessorCount;
+  g_lds_max = (int)prop.sharedMemPerBlock;
   SA_TRY(noalloc_scratch_setup(device));
   if (t_stream) (void)hipStreamDestroy(t_stream);
   t_stream = nullptr;
@@ -703,7 +706,7 @@ int sa_eps_set_st_ksp(sa_eps E, int pc_type, double rtol, int max_it) {
 
 int sa_eps_set_st_ksp_type(sa_eps E, int ksp_type, int restart) {
   SA_CHECK(E && (ksp_type == SA_KSP_BCGS || ksp_type == SA_KSP_GMRES), "unsupported KSP type");
-  SA_CHECK(ksp_type != SA_KSP_GMRES || (restart >= 1 && restart < GS_MAXC_FP64), "GMRES restart must be in [1, 107]");
+  SA_CHECK(ksp_type != SA_KSP_GMRES || (restart >= 1 && restart <= GS_MAXC - 2), "GMRES restart must be in [1, 126]");
   E->p.ksp_gmres = ksp_type == SA_KSP_GMRES;
   if (ksp_type == SA_KSP_GMRES) E->p.ksp_restart = restart;
   E->solved = false;
@@ -861,7 +864,7 @@ static int eps_setup_dims(sa_eps E, KsParams &p) {
   SA_CHECK(p.ncv >= p.nev + 1 || (p.ncv == p.nev && p.ncv == n), "The value of ncv must be at least nev+1");
   SA_CHECK(p.ncv <= p.nev + p.mpd, "The value of ncv must not be larger than nev+mpd");
   SA_CHECK(E->nini <= p.ncv, "The number of initial vectors is larger than ncv");   // epssetup.c:423
-  SA_CHECK(p.ncv < GS_MAXC_FP64, "ncv above 107 is beyond the FP64 Gram-Schmidt kernels (their LDS tile holds 108 columns)");
+  SA_CHECK(p.ncv <= GS_MAXC - 2, "ncv above 126 is beyond the fused Gram-Schmidt kernels (127 basis columns)");
   if (!E->maxit_set) p.max_it = std::max(100, 2 * n / p.ncv);
   if (p.st_type == SA_ST_SINVERT && !E->shift_set) p.st_shift = p.target;   // STSetDefaultShift
   return 0;

@@ -4,6 +4,7 @@
 // launch sequence.  Every decision of the reference's rules
 // (bvorthog.c:91-217, 380-427) is taken on the device.
 #include <cstdlib>
+#include <mutex>
 
 #include "sa_gs_device.h"
 
@@ -13,6 +14,23 @@ template <typename T, int L, bool NT>
 __global__ __launch_bounds__(BLK) void gs_kernel(GsArgs<T> a) {
   const int64_t rb = (int64_t)blockIdx.x * a.rpb;
   gs_slot<T, L, NT>(a, rb, min(a.n, rb + a.rpb), (int)blockIdx.x);
+}
+
+// Bases wide enough that the FP64 tile exceeds 64 KB of LDS (beyond column
+// 107) need the kernels' dynamic-LDS limit raised to what the device allows
+// a workgroup (160 KB on gfx950); done once per instantiation.
+template <typename T, bool NT>
+static int allow_wide_lds() {
+  static std::once_flag once;
+  static hipError_t err = hipSuccess;
+  std::call_once(once, [] {
+    const int bytes = lds_max_bytes() - 8192;   // room for the kernels' static LDS
+    for (const void *f : {(const void *)gs_kernel<T, 4, NT>, (const void *)gs_kernel<T, 8, NT>,
+                          (const void *)gs_kernel<T, 16, NT>, (const void *)gs_kernel<T, 32, NT>})
+      if (err == hipSuccess) err = hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, bytes);
+  });
+  SA_HIP(err);
+  return 0;
 }
 
 template <typename T, bool NT>
@@ -74,7 +92,9 @@ static int gs_launch(BV &V, const Mat *A, int c, double eta, hipStream_t s) {
   a.st = V.st; a.dbuf = V.dbuf; a.m = V.m; a.eta = eta;
   a.G = V.G; a.rpb = V.rows_per_block; a.refine = V.refine;
   const size_t shm = gs_lds_bytes(c, sizeof(T));
-  SA_CHECK(shm + 4096 + 8 * BLK <= 65536, "basis too wide for the LDS tile of the Gram-Schmidt kernel");
+  SA_CHECK(shm + 4096 + 8 * BLK <= (size_t)lds_max_bytes(), "basis too wide for the LDS tile of the Gram-Schmidt kernel");
+  const bool nt = (int64_t)V.ld * V.m * (int64_t)sizeof(T) > GS_NT_BYTES;
+  if (shm + 4096 + 8 * BLK > 65536) SA_TRY((nt ? allow_wide_lds<T, true>() : allow_wide_lds<T, false>()));
   double *pa = V.ws.part, *pb = V.ws.part + (size_t)V.G * (GS_MAXC + 1);
   double *da = V.ws.dots, *db = V.ws.dots + (GS_MAXC + 1);
   // Operators larger than the caches run the stream SpMV as a kernel of its
@@ -84,7 +104,6 @@ static int gs_launch(BV &V, const Mat *A, int c, double eta, hipStream_t s) {
   const bool fused = A && A->spmv_rb == 0;
   if (A && !fused) SA_TRY(launch_spmv(*A, a.V + (size_t)(c - 1) * a.ld, a.V + (size_t)c * a.ld, s));
   const int lanes = fused ? A->lanes : 8;
-  const bool nt = (int64_t)V.ld * V.m * (int64_t)sizeof(T) > GS_NT_BYTES;
   for (int slot = 1; slot <= 3; slot++) {
     a.slot = slot;
     const bool spmv = slot == 1 && fused;

@@ -41,6 +41,7 @@ struct BuildGuard {
 hipStream_t stream();
 int ensure_init();
 int num_cus();
+int lds_max_bytes();   // LDS a workgroup may allocate (device attribute)
 
 // ---- Gram-Schmidt step machine ---------------------------------------------
 // Column c is orthogonalised against columns 0..c-1 by three kernels (slots
@@ -49,7 +50,6 @@ int num_cus();
 // BVSetOrthogonalization refinement types (BVOrthogRefineType)
 enum GsRefine : int { GS_REFINE_IFNEEDED = 0, GS_REFINE_NEVER = 1, GS_REFINE_ALWAYS = 2 };
 constexpr int GS_MAXC = 128;        // largest column index + 1 handled
-constexpr int GS_MAXC_FP64 = 108;   // FP64: the LDS tile holds columns 0..107 (gs_lds_bytes + 6 KB <= 64 KB)
 constexpr int GS_NSTATE = 3;
 constexpr int64_t GS_NT_BYTES = 256ll << 20;   // bases above this size use non-temporal tile loads
 constexpr int SPMV_PT = 8;          // nonzeros per thread of the stream SpMV

@@ -393,7 +393,7 @@ static int band_lu_setup(KspWork &w, const Mat &A, double sigma) {
 
 int ksp_alloc(KspWork &w, const Mat &A, double sigma, bool jacobi, bool direct, bool gmres, int restart) {
   SA_CHECK(A.prec == SA_FP64, "the shift-and-invert solves run in FP64");
-  SA_CHECK(!gmres || (restart >= 1 && restart < GS_MAXC_FP64), "GMRES restart must be in [1, 107]");
+  SA_CHECK(!gmres || (restart >= 1 && restart <= GS_MAXC - 2), "GMRES restart must be in [1, 126]");
   const int64_t n = A.n;
   w.direct = direct;
   w.gmres = gmres && !direct;

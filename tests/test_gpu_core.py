@@ -1044,7 +1044,7 @@ def test_sinvert_gmres_restart_and_arguments(sa, orc):
     assert out[5][1] > out[30][1] > 0
     E = sa.EPS()
     with pytest.raises(sa.SlepcAmdError, match="restart"):
-        E.setSTKSPType("gmres", 108)
+        E.setSTKSPType("gmres", 127)
     with pytest.raises(sa.SlepcAmdError, match="restart"):
         E.setSTKSPType("gmres", 0)
     with pytest.raises(sa.SlepcAmdError, match="unsupported KSP type"):

@@ -109,7 +109,7 @@ def test_small_mixed_precision(sa, orc):
 def test_setup_rejects_what_the_reference_rejects(sa, orc):
     """nev <= 0, ncv = nev on a larger operator, ncv > nev + mpd and more
     initial vectors than ncv are errors with the reference's messages (and
-    ncv above the FP64 kernels' 107 columns an error of its own); the same
+    ncv above the fused kernels' 126 an error of its own); the same
     EPS then solves once the dimensions are valid."""
     Ah = sa.lap2d(10)
     E = sa.EPS()
@@ -132,11 +132,11 @@ def test_setup_rejects_what_the_reference_rejects(sa, orc):
     assert E.getConverged() >= 4
     for i in range(4):
         assert E.computeError(i) <= 1e-8
-    E2 = sa.EPS()                                 # 144 rows: ncv 108 is not clamped
+    E2 = sa.EPS()                                 # 144 rows: ncv 127 is not clamped
     E2.setOperators(sa.Mat(sa.lap2d(12)))
     E2.setProblemType(True)
-    E2.setDimensions(10, 108)
-    with pytest.raises(RuntimeError, match="ncv above 107"):
+    E2.setDimensions(10, 127)
+    with pytest.raises(RuntimeError, match="ncv above 126"):
         E2.solve()
 
 
@@ -274,3 +274,47 @@ def test_failed_allocation_reported_and_library_usable(sa):
     E.setTolerances(1e-10, 5000)
     E.solve()
     assert E.getConverged() >= 4
+
+
+def test_wide_fp64_basis_beyond_64kb_lds(sa, orc):
+    """FP64 bases wider than 108 columns need more than 64 KB of LDS in the
+    Gram-Schmidt slot kernels (the tile of column c takes (66 c + 320) x 8
+    bytes), which gfx950 allows a workgroup (160 KB).  A 127-column basis
+    orthonormalised column by column equals the oracle to rounding, and a
+    Krylov-Schur solve with ncv 120 returns the oracle's values and the
+    closed-form spectrum of the 2-D Laplacian."""
+    n, k = 3000, 127
+    X0 = np.random.default_rng(12).standard_normal((n, k))
+    V = sa.BV(n, k, sa.FP64)
+    V.set_matrix(X0)
+    Vo = X0.copy(order="F")
+    for j in range(k):
+        nrm, lind = V.orthonormalize_column(j)
+        Vo, nrmo, lindo, ho = orc.bv_orthonormalize_column(Vo, j)
+        assert not lind and not lindo and abs(nrm - nrmo) <= 1e-12 * abs(nrmo)
+        if j >= 100:
+            h = V.coefficients(j, j + 1)[:j + 1, 0]
+            assert np.max(np.abs(h - ho)) <= 1e-12 * np.abs(ho).max()
+    Q = V.get_matrix(k)
+    assert np.max(np.abs(Q - Vo)) < 1e-11
+    assert np.linalg.norm(Q.T @ Q - np.eye(k), 1) < 100 * 2.2e-16 * k
+    nx, ny = 30, 31
+    Ah = sa.lap2d(nx, ny)
+    Ao = orc.CSR(Ah.n, Ah.rowptr, Ah.col, Ah.val)
+    v0 = orc.rand_uniform(0, Ah.n)
+    E = sa.EPS()
+    E.setOperators(sa.Mat(Ah))
+    E.setProblemType(True)
+    E.setDimensions(40, 120)
+    E.setTolerances(1e-10, 1000)
+    E.setInitialSpace(v0)
+    E.solve()
+    r = orc.eps_solve(Ao, 40, 120, 1e-10, "LM", True, v0=v0, max_it=1000)
+    assert E.getConverged() >= 40 and r["nconv"] >= 40
+    vals = np.sort([E.getEigenpair(i, False)[0] for i in range(40)])
+    assert np.allclose(vals, np.sort(r["eigr"][:40]), rtol=1e-10, atol=0)
+    i, j = np.meshgrid(np.arange(1, nx + 1), np.arange(1, ny + 1))
+    w = np.sort((4 - 2 * np.cos(i * np.pi / (nx + 1)) - 2 * np.cos(j * np.pi / (ny + 1))).ravel())
+    assert np.allclose(vals, w[-40:], rtol=1e-10, atol=0)
+    for q in range(40):
+        assert E.computeError(q) <= 1e-9
