@@ -31,7 +31,7 @@ typedef struct sa_mlp_s *sa_mlp;   /* learned warm-start network (BF16 MFMA)  */
 enum { SA_FP64 = 0, SA_FP32 = 1 };
 enum { SA_LARGEST_MAGNITUDE = 1, SA_SMALLEST_MAGNITUDE = 2, SA_LARGEST_REAL = 3, SA_SMALLEST_REAL = 4,
        SA_LARGEST_IMAGINARY = 5, SA_SMALLEST_IMAGINARY = 6, SA_TARGET_MAGNITUDE = 7, SA_TARGET_REAL = 8 };
-enum { SA_ERROR_ABSOLUTE = 0, SA_ERROR_RELATIVE = 1 };
+enum { SA_ERROR_ABSOLUTE = 0, SA_ERROR_RELATIVE = 1, SA_ERROR_BACKWARD = 2 };
 
 /* ---- library ----------------------------------------------------------- */
 /* SlepcInitialize analogue: selects the device and creates the stream. */
@@ -442,6 +442,24 @@ int sa_eps_set_orthogonalization(sa_eps eps, int refine, double eta);
  * solve with an error.  NULL restores the basic test. */
 typedef int (*sa_eps_stopping_fn)(sa_eps eps, int its, int max_it, int nconv, int nev, int *reason, void *ctx);
 int sa_eps_set_stopping_test_function(sa_eps eps, sa_eps_stopping_fn stopping, void *ctx);
+/* EPSSetConvergenceTest (epsopts.c:796-810): the error estimate of a pair
+   with eigenvalue lambda and residual norm r is r (SA_CONV_ABS), r / |lambda|
+   (SA_CONV_REL, the default) or r / (||A||_inf + |lambda| ||B||_inf)
+   (SA_CONV_NORM; ||A|| is computed at the solve, and ||B|| stays 0 for a
+   standard problem unless a backward error was computed before, as in the
+   reference, epssetup.c:351-359 and epssolve.c:794-809).  Under
+   shift-and-invert SA_CONV_NORM takes the back-transformed lambda
+   (epskrylov.c:245). */
+enum { SA_CONV_ABS = 0, SA_CONV_REL = 1, SA_CONV_NORM = 2, SA_CONV_USER = 3 };
+int sa_eps_set_convergence_test(sa_eps eps, int conv);
+/* EPSSetConvergenceTestFunction (epsopts.c:751-770): *errest from the
+   eigenvalue (eigr, eigi) and the residual norm res; selects SA_CONV_USER.
+   A nonzero return ends the solve with an error.  NULL restores SA_CONV_REL. */
+typedef int (*sa_eps_converged_fn)(sa_eps eps, double eigr, double eigi, double res, double *errest, void *ctx);
+int sa_eps_set_convergence_test_function(sa_eps eps, sa_eps_converged_fn fn, void *ctx);
+/* The matrix norms of SA_CONV_NORM and of backward errors, ||A||_inf and
+   ||B||_inf (EPSView's "computed matrix norms"); 0 until computed. */
+int sa_eps_get_matrix_norms(sa_eps eps, double *nrma, double *nrmb);
 /* Eigenpair i in the order EPSGetEigenpair returns them; vectors may be NULL. */
 int sa_eps_get_eigenpair(sa_eps eps, int i, double *re, double *im, double *xr_host, double *xi_host);
 int sa_eps_get_error_estimate(sa_eps eps, int i, double *errest);

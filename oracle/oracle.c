@@ -1520,6 +1520,29 @@ int orc_build_balance(int n, const int32_t *rp, const int32_t *ci, const double 
   return 0;
 }
 
+/* EPSConvergedRelative / Absolute / Norm (epsdefault.c:224-256); for a
+   standard problem ||B|| stays 0 (epssetup.c:351-359 computes it only with
+   two matrices), so the norm test is ||r|| / ||A||_inf. */
+static double conv_errest(int conv, double nrma, double re, double im, double res)
+{
+  const double w = hypot(re, im);
+  if (conv == 1) return res;
+  if (conv == 2) return res / (nrma + w * 0.0);
+  return w != 0.0 ? res / w : 1.79e308;
+}
+
+/* MatNorm(A, NORM_INFINITY): the largest absolute row sum */
+static double csr_norm_inf(int n, const int32_t *rp, const double *val)
+{
+  double m = 0.0;
+  for (int r = 0; r < n; r++) {
+    double t = 0.0;
+    for (int32_t q = rp[r]; q < rp[r + 1]; q++) t += fabs(val[q]);
+    if (t > m) m = t;
+  }
+  return m;
+}
+
 int orc_eps_solve(int n, const int32_t *rp, const int32_t *ci, const double *val, const orc_opts *o, const double *v0, orc_result *res)
 {
   if (!lp_load()) return -10;
@@ -1536,6 +1559,7 @@ int orc_eps_solve(int n, const int32_t *rp, const int32_t *ci, const double *val
   if (!(ncv >= nev + 1 || (ncv == nev && ncv == n)) || ncv > nev + mpd) return -16;
   int max_it = o->max_it > 0 ? o->max_it : (100 > 2 * n / ncv ? 100 : 2 * n / ncv);
   double tol = o->tol > 0 ? o->tol : 1e-8, keep = o->keep > 0 ? o->keep : 0.5;
+  const double nrma = o->conv == 2 ? csr_norm_inf(n, rp, val) : 0.0;   /* epssetup.c:351-355 */
   int herm = o->hermitian, which = o->which ? o->which : ORC_LM, lock = o->lock;
   /* harmonic extraction (krylovschur.c:238-240; arnoldi.c:65): NHEP projected problem */
   const int harm = o->extraction == 1;
@@ -1686,7 +1710,7 @@ int orc_eps_solve(int n, const int32_t *rp, const int32_t *ci, const double *val
       for (int i = nconv; i < nv; i++) if (fabs(eigr[i]) > anorm) anorm = fabs(eigr[i]);
       for (int i = nconv; i < nv; i++) {                     /* beta |y_m| + eps ||A||, relative */
         double res = beta * fabs(Y[nv - 1 + (int64_t)i * ld]) + 2.220446049250313e-16 * anorm;
-        bnd[i] = eigr[i] != 0.0 ? res / fabs(eigr[i]) : 1.79e308;
+        bnd[i] = conv_errest(o->conv, nrma, eigr[i], 0.0, res);
         cv[i] = bnd[i] < tol ? 'C' : 'N';
       }
       if (o->lanczos_local)                                  /* purge repeated Ritz values (:570-575) */
@@ -1734,7 +1758,7 @@ int orc_eps_solve(int n, const int32_t *rp, const int32_t *ci, const double *val
           ks_op(&ks, vi, w);                                 /* STApply: the operator of the ST */
           for (int64_t q = 0; q < n; q++) w[q] -= eigr[i] * vi[q];
           double nr = orc_bv_norm2(n, w);
-          bnd[i] = eigr[i] != 0.0 ? nr / fabs(eigr[i]) : 1.79e308;
+          bnd[i] = conv_errest(o->conv, nrma, eigr[i], 0.0, nr);
           if (bnd[i] >= tol) cv[i] = 'S';
         }
         int i;
@@ -1780,23 +1804,24 @@ int orc_eps_solve(int n, const int32_t *rp, const int32_t *ci, const double *val
       if ((info = ds_sort_nhep(&ds, eigr, eigi))) { rc = -40; goto done; }
       ds_update_extra_row_nhep(&ds);
     }
-    /* EPSKrylovConvergence (epskrylov.c:207-295), relative criterion */
+    /* EPSKrylovConvergence (epskrylov.c:207-295) */
     int k, marker = -1;
     for (k = nconv; k < nv; k++) {
       double re = eigr[k], im = eigi[k], resnorm;
       int cplx = 0;
       if (herm) resnorm = fabs(ds.Q[nv - 1 + k * ld]);
       else if ((info = ds_vector_nhep_some(&ds, k, &resnorm, &cplx))) { rc = -50; goto done; }
+      /* STBackTransform (sinvert.c:16-38) for the true residual and the norm
+         test; the other tests take the transformed value (epskrylov.c:245-251) */
+      if (o->st == 1 && (o->trueres || o->conv == 2)) {
+        if (im == 0.0) re = 1.0 / re + o->sigma;
+        else { double t = re * re + im * im; re = re / t + o->sigma; im = -im / t; }
+      }
       if (o->trueres) {
-        if (o->st == 1) {                                    /* STBackTransform (sinvert.c:16-38) */
-          if (im == 0.0) re = 1.0 / re + o->sigma;
-          else { double t = re * re + im * im; re = re / t + o->sigma; im = -im / t; }
-        }
         const double *z = herm ? ds.Q + (int64_t)k * ld : ds.X + (int64_t)k * ld;
         resnorm = ks_true_residual(&ks, nv, z, cplx ? z + ld : NULL, re, im);
       } else resnorm *= beta * gamma;
-      double w = hypot(re, im);
-      errest[k] = (w != 0.0) ? resnorm / w : 1.79e308;
+      errest[k] = conv_errest(o->conv, nrma, re, im, resnorm);
       if (marker == -1 && errest[k] >= tol) marker = k;
       if (cplx) { errest[k + 1] = errest[k]; k++; }
       if (marker != -1) break;
@@ -2019,6 +2044,7 @@ int orc_eps_solve_twosided(int n, const int32_t *rp, const int32_t *ci, const do
   if (!(ncv >= nev + 1 || (ncv == nev && ncv == n)) || ncv > nev + mpd) return -16;
   int max_it = o->max_it > 0 ? o->max_it : (100 > 2 * n / ncv ? 100 : 2 * n / ncv);
   double tol = o->tol > 0 ? o->tol : 1e-8, keep = o->keep > 0 ? o->keep : 0.5;
+  const double nrma = o->conv == 2 ? csr_norm_inf(n, rp, val) : 0.0;   /* epssetup.c:351-355 */
   if (o->hermitian || o->solver != 0 || o->extraction != 0) return -11;
   const int which = o->which ? o->which : ORC_LM;
   cmp_target = o->target;
@@ -2147,19 +2173,17 @@ int orc_eps_solve_twosided(int n, const int32_t *rp, const int32_t *ci, const do
     for (k = nconv; k < nv; k++) {
       double rn, rl; int cplx = 0, cplx2 = 0;
       if (ds_vector_nhep_some(&ds, k, &rn, &cplx)) { rc = -50; goto done; }
-      double lam = hypot(eigr[k], eigi[k]);
-      if (o->trueres) {                                      /* right residual from the Ritz vector */
+      if (o->trueres)                                        /* right residual from the Ritz vector */
         rn = ks_true_residual(&ks, nv, ds.X + (int64_t)k * ld, cplx ? ds.X + (int64_t)(k + 1) * ld : NULL,
                               eigr[k], eigi[k]);
-        errest[k] = lam != 0.0 ? rn / lam : 1.79e308;
-      } else
-      errest[k] = lam != 0.0 ? rn * beta * norm / lam : 1.79e308;
+      else rn = rn * beta * norm;
+      errest[k] = conv_errest(o->conv, nrma, eigr[k], eigi[k], rn);
       if (marker == -1 && errest[k] >= tol) marker = k;
       ds_swap_sides(&ds, &B, &Z, &Y);
       info = ds_vector_nhep_some(&ds, k, &rl, &cplx2);
       ds_swap_sides(&ds, &B, &Z, &Y);
       if (info) { rc = -51; goto done; }
-      double le = lam != 0.0 ? rl * betat * norm2 / lam : 1.79e308;
+      double le = conv_errest(o->conv, nrma, eigr[k], eigi[k], rl * betat * norm2);
       if (le > errest[k]) errest[k] = le;
       if (marker == -1 && le >= tol) marker = k;
       if (cplx) { errest[k + 1] = errest[k]; k++; }

@@ -91,6 +91,7 @@ typedef struct {
                               (Krylov-Schur, two-sided right residuals, Arnoldi) */
   int     refine;          /* BVSetOrthogonalization refinement: ORC_REFINE_IFNEEDED (default),
                               ORC_REFINE_NEVER, ORC_REFINE_ALWAYS */
+  int     conv;            /* EPSSetConvergenceTest: 0 relative (default), 1 absolute, 2 norm-relative */
 } orc_opts;
 enum { ORC_REFINE_IFNEEDED = 0, ORC_REFINE_NEVER = 1, ORC_REFINE_ALWAYS = 2 };
 /* BVOrthogonalizeSomeColumn (MGS against the columns with which[i] set) with a

@@ -26,7 +26,7 @@ class Opts(C.Structure):
                 ("st", C.c_int), ("sigma", C.c_double), ("lanczos_local", C.c_int),
                 ("arnoldi_delayed", C.c_int), ("balance", C.c_int), ("balance_its", C.c_int),
                 ("balance_cutoff", C.c_double), ("lanczos_partial", C.c_int), ("trueres", C.c_int),
-                ("refine", C.c_int)]
+                ("refine", C.c_int), ("conv", C.c_int)]
 
 REFINE = {"ifneeded": 0, "never": 1, "always": 2}
 
@@ -319,7 +319,7 @@ def lanczos_local_sweep(A, m, v0):
 def eps_solve(A, nev, ncv=0, tol=1e-8, which="LM", hermitian=True, v0=None, max_it=0,
               mpd=0, nvec=None, nthreads=0, keep=0.5, lock=1, solver="krylovschur",
               extraction="ritz", target=0.0, twosided=False, st="shift", sigma=None,
-              reorthog="full", balance="none", trueres=False, trace=0, refine="ifneeded"):
+              reorthog="full", balance="none", trueres=False, trace=0, refine="ifneeded", conv="rel"):
     o = Opts(hermitian=1 if hermitian else 0, which=WHICH[which], nev=nev, ncv=ncv, mpd=mpd,
              extraction={"ritz": 0, "harmonic": 1}[extraction], target=float(target),
              st={"shift": 0, "sinvert": 1}[st], sigma=float(target if sigma is None else sigma),
@@ -329,7 +329,7 @@ def eps_solve(A, nev, ncv=0, tol=1e-8, which="LM", hermitian=True, v0=None, max_
              lanczos_partial={"full": 0, "local": 0, "periodic": 1, "partial": 2, "selective": 3,
                                "delayed": 4}[reorthog],
              max_it=max_it, tol=tol, keep=keep, eta=0.7071, lock=lock, nthreads=nthreads,
-             trueres=1 if trueres else 0, refine=REFINE[refine],
+             trueres=1 if trueres else 0, refine=REFINE[refine], conv={"rel": 0, "abs": 1, "norm": 2}[conv],
              solver={"krylovschur": 0, "arnoldi": 1, "lanczos": 2}[solver])
     ncv_eff = ncv if ncv else min(A.n, max(2 * nev, nev + 15))
     eigr = np.zeros(ncv_eff + 1)

@@ -12,7 +12,7 @@ import numpy as np
 
 from . import _lib
 from ._lib import (FP32, FP64, LARGEST_MAGNITUDE, LARGEST_REAL, SMALLEST_MAGNITUDE, SMALLEST_REAL,
-                   ERROR_ABSOLUTE, ERROR_RELATIVE, SlepcAmdError, check, lib)
+                   ERROR_ABSOLUTE, ERROR_RELATIVE, ERROR_BACKWARD, SlepcAmdError, check, lib)
 
 __all__ = ["Mat", "BV", "EPS", "CSR", "init", "lap2d", "convdiff3d", "stencil27", "fv2d", "markov",
            "fv_theta", "rand_uniform", "FP32", "FP64", "SlepcAmdError"]
@@ -554,6 +554,38 @@ class EPS:
                            C.POINTER(C.c_double), C.c_int, C.c_void_p)
     _STOPPING = C.CFUNCTYPE(C.c_int, C.c_void_p, C.c_int, C.c_int, C.c_int, C.c_int, C.POINTER(C.c_int),
                             C.c_void_p)
+    _CONVERGED = C.CFUNCTYPE(C.c_int, C.c_void_p, C.c_double, C.c_double, C.c_double, C.POINTER(C.c_double),
+                             C.c_void_p)
+    CONV = {"abs": 0, "rel": 1, "norm": 2}
+
+    def setConvergenceTest(self, conv):
+        """EPSSetConvergenceTest: "abs" (||r||), "rel" (||r|| / |lambda|, the
+        default) or "norm" (||r|| / (||A||_inf + |lambda| ||B||))."""
+        check(lib().sa_eps_set_convergence_test(self.h, self.CONV[conv]))
+
+    def setConvergenceTestFunction(self, fn):
+        """EPSSetConvergenceTestFunction: fn(eigr, eigi, res) returns the error
+        estimate; None restores the relative test."""
+        if fn is None:
+            self._conv_cb = None
+            check(lib().sa_eps_set_convergence_test_function(self.h, None, None))
+            return
+
+        def tramp(_eps, er, ei, res, errest, _ctx):
+            try:
+                errest[0] = float(fn(er, ei, res))
+                return 0
+            except Exception:
+                return 1
+        self._conv_cb = self._CONVERGED(tramp)
+        check(lib().sa_eps_set_convergence_test_function(self.h, C.cast(self._conv_cb, C.c_void_p), None))
+
+    def getMatrixNorms(self):
+        """(||A||_inf, ||B||_inf) as computed for the norm test or a backward
+        error; 0 until computed."""
+        a, b = C.c_double(), C.c_double()
+        check(lib().sa_eps_get_matrix_norms(self.h, C.byref(a), C.byref(b)))
+        return a.value, b.value
 
     def setMonitor(self, fn):
         def tramp(_eps, its, nconv, er, ei, err, nest, _ctx):

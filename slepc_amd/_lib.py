@@ -15,7 +15,7 @@ _LIB = None
 
 FP64, FP32 = 0, 1
 LARGEST_MAGNITUDE, SMALLEST_MAGNITUDE, LARGEST_REAL, SMALLEST_REAL = 1, 2, 3, 4
-ERROR_ABSOLUTE, ERROR_RELATIVE = 0, 1
+ERROR_ABSOLUTE, ERROR_RELATIVE, ERROR_BACKWARD = 0, 1, 2
 
 
 class SlepcAmdError(RuntimeError):
@@ -136,6 +136,9 @@ SIGNATURES = {
                                                   C.POINTER(C.c_int)]),
     "sa_eps_monitor_cancel": (C.c_int, [_VP]),
     "sa_eps_set_stopping_test_function": (C.c_int, [_VP, _VP, _VP]),
+    "sa_eps_set_convergence_test": (C.c_int, [_VP, C.c_int]),
+    "sa_eps_set_convergence_test_function": (C.c_int, [_VP, _VP, _VP]),
+    "sa_eps_get_matrix_norms": (C.c_int, [_VP, C.POINTER(C.c_double), C.POINTER(C.c_double)]),
     "sa_eps_get_eigenpair": (C.c_int, [_VP, C.c_int, C.POINTER(C.c_double), C.POINTER(C.c_double), _VP, _VP]),
     "sa_eps_get_error_estimate": (C.c_int, [_VP, C.c_int, C.POINTER(C.c_double)]),
     "sa_eps_compute_error": (C.c_int, [_VP, C.c_int, C.c_int, C.POINTER(C.c_double)]),

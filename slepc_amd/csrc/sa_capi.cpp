@@ -106,6 +106,10 @@ struct sa_eps_s {
   std::vector<std::pair<sa_eps_monitor_fn, void *>> monitors;
   sa_eps_stopping_fn stopping = nullptr;
   void *stopping_ctx = nullptr;
+  // EPSSetConvergenceTestFunction (epsopts.c:751-770)
+  sa_eps_converged_fn converged = nullptr;
+  void *converged_ctx = nullptr;
+  double nrma = 0.0, nrmb = 0.0;   // ||A||_inf, ||B||_inf (0 until computed; reset with the operators)
 };
 
 #define API_INIT() SA_TRY(ensure_init())
@@ -639,6 +643,7 @@ int sa_eps_set_operators(sa_eps E, sa_mat A) {
   SA_CHECK(A->A.prec == SA_FP64, "the operator of EPS must be FP64 (use sa_eps_set_operators_fp32 for the bulk phase)");
   E->A = A;
   E->solved = false;
+  E->nrma = E->nrmb = 0.0;   // epssetup.c:491-492
   // a cached A^T belongs to the previous operator (an operator allocated later
   // at the same address must not find it)
   if (E->AT) sa_mat_destroy(E->AT);
@@ -849,12 +854,23 @@ static int eps_stopping_hook(void *owner, int its, int max_it, int nconv, int ne
   return E->stopping(E, its, max_it, nconv, nev, reason, E->stopping_ctx);
 }
 
+static int eps_converged_hook(void *owner, double re, double im, double res, double *errest) {
+  sa_eps E = static_cast<sa_eps>(owner);
+  return E->converged(E, re, im, res, errest, E->converged_ctx);
+}
+
 static int eps_setup_dims(sa_eps E, KsParams &p) {
   const int n = E->A->A.n;
   p = E->p;
   p.hooks.owner = E;
   p.hooks.monitor = E->monitors.empty() ? nullptr : eps_monitor_hook;
   p.hooks.stopping = E->stopping ? eps_stopping_hook : nullptr;
+  p.hooks.converged = E->converged ? eps_converged_hook : nullptr;
+  // initialization of the matrix norms (epssetup.c:351-359): ||A||_inf for
+  // the norm-relative test; ||B|| is not computed for a standard problem
+  if (p.conv == SA_CONV_NORM && E->nrma == 0.0) SA_TRY(mat_norm_inf(E->A->A, &E->nrma));
+  p.nrma = E->nrma;
+  p.nrmb = E->nrmb;
   // safeguard for small problems (epssetup.c:342-345): nev and a set ncv at most n
   if (p.nev > n) p.nev = n;
   if (E->ncv_set && p.ncv > n) p.ncv = n;
@@ -1164,7 +1180,8 @@ int sa_eps_get_error_estimate(sa_eps E, int i, double *errest) {
 int sa_eps_compute_error(sa_eps E, int i, int type, double *error) {
   API_INIT();
   SA_CHECK(E && E->solved && error && i >= 0 && i < E->res.nconv, "eigenpair index out of range");
-  SA_CHECK(type == SA_ERROR_ABSOLUTE || type == SA_ERROR_RELATIVE, "unsupported error type");
+  SA_CHECK(type == SA_ERROR_ABSOLUTE || type == SA_ERROR_RELATIVE || type == SA_ERROR_BACKWARD,
+           "unsupported error type");
   const KsResult &r = E->res;
   const int k = r.perm[i];
   hipStream_t s = stream();
@@ -1188,7 +1205,36 @@ int sa_eps_compute_error(sa_eps E, int i, int type, double *error) {
     res = std::max(res, rn[0]);
   }
   if (type == SA_ERROR_RELATIVE) res /= std::hypot(kr, ki);
+  if (type == SA_ERROR_BACKWARD) {   // epssolve.c:790-809: ||r|| / (||A|| + |lambda| ||B||), B = I here
+    if (E->nrma == 0.0) SA_TRY(mat_norm_inf(E->A->A, &E->nrma));
+    E->nrmb = 1.0;
+    res /= E->nrma + std::hypot(kr, ki) * E->nrmb;
+  }
   *error = res;
+  return 0;
+}
+
+int sa_eps_set_convergence_test(sa_eps E, int conv) {
+  SA_CHECK(E && conv >= SA_CONV_ABS && conv <= SA_CONV_USER, "invalid convergence test");
+  SA_CHECK(conv != SA_CONV_USER || E->converged, "Must call EPSSetConvergenceTestFunction() first");
+  E->p.conv = conv;
+  E->solved = false;
+  return 0;
+}
+
+int sa_eps_set_convergence_test_function(sa_eps E, sa_eps_converged_fn fn, void *ctx) {
+  SA_CHECK(E, "null EPS");
+  E->converged = fn;
+  E->converged_ctx = fn ? ctx : nullptr;
+  E->p.conv = fn ? SA_CONV_USER : SA_CONV_REL;
+  E->solved = false;
+  return 0;
+}
+
+int sa_eps_get_matrix_norms(sa_eps E, double *nrma, double *nrmb) {
+  SA_CHECK(E, "null EPS");
+  if (nrma) *nrma = E->nrma;
+  if (nrmb) *nrmb = E->nrmb;
   return 0;
 }
 

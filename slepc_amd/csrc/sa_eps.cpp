@@ -23,6 +23,21 @@ double now() {
 }  // namespace
 
 // Algorithmic HBM bytes of one SpMV: values + column indices + row pointers + x + y.
+int eps_converged(const KsParams &p, double re, double im, double res, double *errest) {
+  const double w = std::hypot(re, im);
+  switch (p.conv) {
+    case SA_CONV_ABS: *errest = res; return 0;
+    case SA_CONV_NORM: *errest = res / (p.nrma + w * p.nrmb); return 0;
+    case SA_CONV_USER: {
+      SA_CHECK(p.hooks.converged, "the user convergence test is not set");
+      const int rc = p.hooks.converged(p.hooks.owner, re, im, res, errest);
+      SA_CHECK(rc == 0, "the convergence test returned an error code");
+      return 0;
+    }
+    default: *errest = w != 0.0 ? res / w : 1.79e308; return 0;
+  }
+}
+
 int eps_stopping(const KsParams &p, int its, int nconv, int nev, int *reason) {
   if (p.hooks.stopping) {
     int r = 0;
@@ -460,7 +475,7 @@ static int lanczos_restart(const KsParams &p, BV &V, const DenseProblem &ds, int
   for (int i = nconv; i < nv; i++) anorm = std::max(anorm, std::fabs(eigr[i]));
   for (int i = nconv; i < nv; i++) {
     const double res = beta * std::fabs(Y[nv - 1 + (size_t)i * ld]) + DBL_EPSILON * anorm;
-    bnd[i] = eigr[i] != 0.0 ? res / std::fabs(eigr[i]) : 1.79e308;   // EPSConvergedRelative
+    SA_TRY(eps_converged(p, eigr[i], 0.0, res, &bnd[i]));   // (*eps->converged), lanczos.c:564
     conv[i] = bnd[i] < p.tol ? 'C' : 'N';
   }
   if (p.lanczos_local)   // purge repeated Ritz values
@@ -526,7 +541,7 @@ static int lanczos_restart(const KsParams &p, BV &V, const DenseProblem &ds, int
       SA_HIP(hipMemcpyAsync(&h, V.ws.dots, sizeof(double), hipMemcpyDeviceToHost, s));
       SA_HIP(hipStreamSynchronize(s));
       const double nr = std::sqrt(h);
-      bnd[i] = eigr[i] != 0.0 ? nr / std::fabs(eigr[i]) : 1.79e308;
+      SA_TRY(eps_converged(p, eigr[i], 0.0, nr, &bnd[i]));   // lanczos.c:644
       if (bnd[i] >= p.tol) conv[i] = 'S';
     }
     int i;
@@ -825,16 +840,19 @@ int ks_solve(const KsParams &p, const Mat &A, BV &V, bool have_initial, KsResult
       info = ds.ritz_residual(k, &resnorm, &cplx);
       SA_CHECK(info == 0, "LAPACK failure in the projected problem (vectors)");
       double re = eigr[k], im = eigi[k];
-      if (trueres) {
-        if (sinv) {   // STBackTransform (sinvert.c:16-38)
-          if (im == 0.0) {
-            re = 1.0 / re + p.st_shift;
-          } else {
-            const double t = re * re + im * im;
-            re = re / t + p.st_shift;
-            im = -im / t;
-          }
+      // STBackTransform (sinvert.c:16-38) for the true residual and for the
+      // norm-relative test; the other tests take the transformed value
+      // (epskrylov.c:245-251)
+      if (sinv && (trueres || p.conv == SA_CONV_NORM)) {
+        if (im == 0.0) {
+          re = 1.0 / re + p.st_shift;
+        } else {
+          const double t = re * re + im * im;
+          re = re / t + p.st_shift;
+          im = -im / t;
         }
+      }
+      if (trueres) {
         const double *Xk = dsherm ? ds.Q() + (size_t)k * ld : ds.X() + (size_t)k * ld;
         const int nx = cplx ? 2 : 1;
         SA_TRY(ritz_true_residual(V, A, nv, Xk, ld, nx, re, im, trw.x, trw.part, &resnorm));
@@ -844,8 +862,7 @@ int ks_solve(const KsParams &p, const Mat &A, BV &V, bool have_initial, KsResult
       } else {
         resnorm *= beta * gamma;
       }
-      const double w = std::hypot(re, im);
-      errest[k] = (w != 0.0) ? resnorm / w : 1.79e308;
+      SA_TRY(eps_converged(p, re, im, resnorm, &errest[k]));
       if (marker == -1 && errest[k] >= p.tol) marker = k;
       if (cplx) {
         errest[k + 1] = errest[k];

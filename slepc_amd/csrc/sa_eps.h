@@ -16,6 +16,8 @@ struct KsHooks {
   int (*monitor)(void *owner, int its, int nconv, const double *eigr, const double *eigi, const double *errest,
                  int nest) = nullptr;
   int (*stopping)(void *owner, int its, int max_it, int nconv, int nev, int *reason) = nullptr;
+  // EPSSetConvergenceTestFunction: errest from the eigenvalue and residual norm
+  int (*converged)(void *owner, double re, double im, double res, double *errest) = nullptr;
   void *owner = nullptr;
 };
 
@@ -24,6 +26,8 @@ struct KsParams {
   int which = SA_LARGEST_MAGNITUDE;
   int nev = 1, ncv = 16, mpd = 16, max_it = 100;
   double tol = 1e-8, keep = 0.5, eta = 0.7071;
+  int conv = SA_CONV_REL;       // EPSSetConvergenceTest: SA_CONV_ABS, _REL, _NORM, _USER
+  double nrma = 0.0, nrmb = 0.0;  // ||A||_inf and ||B||_inf of EPS_CONV_NORM (nrmb 0 unless set)
   int solver = 0;              // 0 Krylov-Schur, 1 Arnoldi with explicit restart (EPSARNOLDI), 2 Lanczos
   int extraction = 0;          // 0 Ritz, 1 harmonic Ritz (EPS_HARMONIC)
   double target = 0.0;         // EPSSetTarget: comparison point of which 7/8 and the harmonic shift
@@ -55,6 +59,10 @@ struct KsResult {
   std::vector<double> eigr, eigi, errest;
 };
 
+// The error estimate of a pair from its eigenvalue (re, im) and residual norm
+// res by the convergence test of p (EPSConvergedAbsolute / Relative / Norm,
+// epsdefault.c:224-256, or the user's function).
+int eps_converged(const KsParams &p, double re, double im, double res, double *errest);
 // The stopping test of iteration its with nconv converged pairs: the hook,
 // or EPSStoppingBasic (epsdefault.c:290-302): 1 (converged) when nconv >=
 // nev, -1 (diverged, iterations) when its >= max_it, else 0.

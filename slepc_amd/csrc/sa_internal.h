@@ -110,6 +110,8 @@ int spmv_pick_rows(const int32_t *rp, int64_t n);
 
 // ---- kernel launchers (sa_kernels.hip) -------------------------------------
 int launch_spmv(const Mat &A, const void *x, void *y, hipStream_t s);
+// MatNorm(A, NORM_INFINITY): the largest absolute row sum, computed on the device.
+int mat_norm_inf(const Mat &A, double *out);
 // One Arnoldi/orthonormalisation step for column c of V; spmv_from >= 0 first
 // computes V(:,c) = A V(:,spmv_from).  eta: DGKS threshold.
 int launch_gs_column(BV &V, const Mat *A, int c, double eta, hipStream_t s);

@@ -3,6 +3,7 @@
 // GEMM on MFMA.  Wave size 64; 256-thread workgroups.  The fused Gram-Schmidt
 // step of the Arnoldi sweep lives in sa_gs.hip.
 #include <algorithm>
+#include <vector>
 #include <cmath>
 
 #include "sa_device.h"
@@ -170,6 +171,48 @@ static int spmv_t(const Mat &A, const T *x, T *y, hipStream_t s) {
     default: hipLaunchKernelGGL((spmv_kernel<T, 32>), dim3(grid), dim3(BLK), 0, s, A.n, A.rowptr, A.col, (const T *)A.val, x, y); break;
   }
   SA_HIP(hipGetLastError());
+  return 0;
+}
+
+// MatNorm(A, NORM_INFINITY): part[b] = the largest absolute row sum of the
+// rows that workgroup b strides over; the host takes the largest partial.
+template <typename T>
+__global__ __launch_bounds__(BLK) void row_abs_max_kernel(int n, const int32_t *__restrict__ rp,
+                                                          const T *__restrict__ val, double *part) {
+  __shared__ double sh[BLK / 64];
+  double m = 0.0;
+  for (int i = blockIdx.x * BLK + threadIdx.x; i < n; i += gridDim.x * BLK) {
+    double r = 0.0;
+    for (int32_t q = rp[i]; q < rp[i + 1]; q++) r += fabs((double)val[q]);
+    m = fmax(m, r);
+  }
+  for (int o = 32; o > 0; o >>= 1) m = fmax(m, __shfl_xor(m, o, 64));
+  if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = m;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    for (int w = 1; w < BLK / 64; w++) m = fmax(m, sh[w]);
+    part[blockIdx.x] = fmax(m, sh[0]);
+  }
+}
+
+int mat_norm_inf(const Mat &A, double *out) {
+  *out = 0.0;
+  if (A.n == 0) return 0;
+  const int G = (int)std::min<int64_t>(1024, ((int64_t)A.n + BLK - 1) / BLK);
+  hipStream_t s = stream();
+  double *part = nullptr;
+  SA_HIP(hipMalloc(&part, sizeof(double) * G));
+  if (A.prec == SA_FP64)
+    hipLaunchKernelGGL(row_abs_max_kernel<double>, dim3(G), dim3(BLK), 0, s, A.n, A.rowptr, (const double *)A.val, part);
+  else
+    hipLaunchKernelGGL(row_abs_max_kernel<float>, dim3(G), dim3(BLK), 0, s, A.n, A.rowptr, (const float *)A.val, part);
+  std::vector<double> h(G);
+  hipError_t e = hipGetLastError();
+  if (e == hipSuccess) e = hipMemcpyAsync(h.data(), part, sizeof(double) * G, hipMemcpyDeviceToHost, s);
+  if (e == hipSuccess) e = hipStreamSynchronize(s);
+  (void)hipFree(part);
+  SA_HIP(e);
+  for (double v : h) *out = std::max(*out, v);
   return 0;
 }
 

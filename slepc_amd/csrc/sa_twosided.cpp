@@ -191,18 +191,18 @@ int ks_twosided(const KsParams &p, const Mat &A, const Mat &AT, BV &V, BV &W, bo
       double rn = 0.0, rl = 0.0;
       int cplx = 0, cplx2 = 0;
       SA_CHECK(ds.ritz_residual(k, &rn, &cplx) == 0, "LAPACK failure in the projected problem (vectors)");
-      const double lam = std::hypot(eigr[k], eigi[k]);
       if (p.trueres) {   // the right residual from the Ritz vector (epskrylov.c:256-264)
         SA_TRY(ritz_true_residual(V, A, nv, ds.X() + (size_t)k * ld, ld, cplx ? 2 : 1, eigr[k], eigi[k], trw.x, trw.part,
                                   &rn));
         r.true_residuals++;
-        errest[k] = lam != 0.0 ? rn / lam : 1.79e308;
       } else {
-        errest[k] = lam != 0.0 ? rn * beta * norm / lam : 1.79e308;
+        rn = rn * beta * norm;
       }
+      SA_TRY(eps_converged(p, eigr[k], eigi[k], rn, &errest[k]));
       if (marker == -1 && errest[k] >= p.tol) marker = k;
       SA_CHECK(ds.left_residual(k, &rl, &cplx2) == 0, "LAPACK failure in the projected problem (left vectors)");
-      const double lerr = lam != 0.0 ? rl * betat * norm2 / lam : 1.79e308;
+      double lerr = 0.0;
+      SA_TRY(eps_converged(p, eigr[k], eigi[k], rl * betat * norm2, &lerr));
       errest[k] = std::max(errest[k], lerr);
       if (marker == -1 && lerr >= p.tol) marker = k;
       if (cplx) {

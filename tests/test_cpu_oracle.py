@@ -1014,3 +1014,40 @@ def test_oracle_rejects_dimensions_as_the_reference():
         orc.eps_solve(A, 10, 30, 1e-8, "LM", True, v0=v0, mpd=15)
     r = orc.eps_solve(A, 10, 11, 1e-8, "LM", True, v0=v0, max_it=5000)   # ncv = nev + 1 is accepted
     assert r["nconv"] >= 10
+
+
+def _scaled_lap(orc, m, scale):
+    A = orc.lap2d(m)
+    return orc.CSR(A.n, A.rowptr, A.col, A.val * scale)
+
+
+@pytest.mark.parametrize("conv,which", [("abs", "LM"), ("rel", "LM"), ("norm", "SM")])
+def test_oracle_convergence_criteria(orc, conv, which):
+    """EPSConvergedAbsolute / Relative / Norm (epsdefault.c:224-256) with true
+    residuals on the 2-D Laplacian scaled by 1e-4 (|lambda| and ||A||_inf =
+    8e-4 far from 1), tol 1e-8: every estimate equals the explicit residual
+    ||A x - lambda x|| (dense numpy) divided by 1, |lambda| or ||A||_inf, and
+    meets tol.  The absolute test accepts pairs whose relative residual is
+    about 1e-5, and the norm test on the smallest eigenvalues accepts
+    relative residuals far above 1e-8: a criterion that divided by the wrong
+    quantity fails one of these."""
+    A = _scaled_lap(orc, 20, 1e-4)
+    Ad = A.to_dense()
+    nrma = np.max(np.abs(Ad).sum(axis=1))
+    v0 = orc.rand_uniform(0, A.n)
+    r = orc.eps_solve(A, 4, 20, 1e-8, which, True, v0=v0, trueres=True, conv=conv, max_it=5000, nvec=4)
+    assert r["nconv"] >= 4
+    rel = []
+    for i in range(4):
+        lam, x = r["eigr"][i], r["vecs"][:, i] / np.linalg.norm(r["vecs"][:, i])
+        res = np.linalg.norm(Ad @ x - lam * x)
+        denom = {"abs": 1.0, "rel": abs(lam), "norm": nrma}[conv]
+        assert r["errest"][i] <= 1e-8
+        assert abs(r["errest"][i] - res / denom) <= 1e-4 * res / denom
+        rel.append(res / abs(lam))
+    if conv == "abs":
+        assert max(rel) > 1e-6
+    elif conv == "norm":
+        assert max(rel) > 1e-7
+    else:
+        assert max(rel) <= 1e-8

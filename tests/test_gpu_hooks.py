@@ -109,3 +109,103 @@ def test_user_stopping_test(sa, orc, case):
     E.setStoppingTest(None)
     E.solve()
     assert E.getConvergedReason() == sa.REASON_CONVERGED_TOL and E.getConverged() >= nev
+
+
+@pytest.mark.parametrize("conv,which", [("abs", "LM"), ("rel", "LM"), ("norm", "SM")])
+def test_convergence_criteria_match_oracle(sa, orc, conv, which):
+    """EPSSetConvergenceTest on the 2-D Laplacian scaled by 1e-4 with true
+    residuals, tol 1e-8: the device solve returns the oracle's values and
+    restart count, and each error estimate equals its explicit residual
+    (EPSComputeError, absolute) divided by 1, |lambda| or ||A||_inf."""
+    from test_cpu_oracle import _scaled_lap
+    Ao = _scaled_lap(orc, 20, 1e-4)
+    v0 = orc.rand_uniform(0, Ao.n)
+    E = sa.EPS()
+    E.setOperators(sa.Mat(sa.CSR(Ao.n, Ao.rowptr, Ao.col, Ao.val)))
+    E.setProblemType(True)
+    E.setWhichEigenpairs(which)
+    E.setDimensions(4, 20)
+    E.setTolerances(1e-8, 5000)
+    E.setTrueResidual(True)
+    E.setConvergenceTest(conv)
+    E.setInitialSpace(v0)
+    E.solve()
+    r = orc.eps_solve(Ao, 4, 20, 1e-8, which, True, v0=v0, trueres=True, conv=conv, max_it=5000)
+    assert E.getConverged() >= 4 and r["nconv"] >= 4
+    assert abs(E.getIterationNumber() - r["its"]) <= 1
+    vals = np.array([E.getEigenpair(i, False)[0] for i in range(4)])
+    assert np.allclose(vals, r["eigr"][:4], rtol=1e-9, atol=0)
+    nrma = np.max(np.abs(Ao.to_dense()).sum(axis=1))
+    nA, nB = E.getMatrixNorms()
+    assert nB == 0.0 and abs(nA - (nrma if conv == "norm" else 0.0)) <= 1e-15 * nrma
+    for i in range(4):
+        denom = {"abs": 1.0, "rel": abs(vals[i]), "norm": nrma}[conv]
+        est = E.getErrorEstimate(i)
+        assert est <= 1e-8
+        assert abs(est - E.computeError(i, sa.ERROR_ABSOLUTE) / denom) <= 1e-3 * est
+
+
+def test_convergence_norm_test30_and_backward_error(sa):
+    """eps/tests/test30.c suffix 1 (-eps_conv_norm, test30_1.out): the diagonal
+    matrix 1..20, 4 eigenvalues closest to 4.8 print 5.00000, 4.00000,
+    6.00000, 3.00000 and the computed norm(A) is 20 (here with
+    shift-and-invert; the reference test runs GD).  ||B|| stays 0 for the
+    standard problem until a backward error is computed, which sets it to 1
+    (epssolve.c:801): EPS_ERROR_BACKWARD = ||r|| / (||A|| + |lambda|)."""
+    n = 20
+    Ah = sa.CSR(n, np.arange(n + 1, dtype=np.int32), np.arange(n, dtype=np.int32), np.arange(1.0, n + 1))
+    E = sa.EPS()
+    E.setOperators(sa.Mat(Ah))
+    E.setProblemType(True)
+    E.setWhichEigenpairs("TM")
+    E.setTarget(4.8)
+    E.setST("sinvert")
+    E.setDimensions(4, 0)
+    E.setConvergenceTest("norm")
+    E.solve()
+    assert E.getConverged() >= 4
+    vals = [E.getEigenpair(i, False)[0] for i in range(4)]
+    assert ["%.5f" % v for v in vals] == ["5.00000", "4.00000", "6.00000", "3.00000"]
+    assert E.getMatrixNorms() == (20.0, 0.0)
+    for i in range(4):
+        a = E.computeError(i, sa.ERROR_ABSOLUTE)
+        b = E.computeError(i, sa.ERROR_BACKWARD)
+        assert b == a / (20.0 + abs(vals[i]))
+    assert E.getMatrixNorms() == (20.0, 1.0)
+
+
+def test_user_convergence_function(sa, orc):
+    """EPSSetConvergenceTestFunction: a user function computing the relative
+    estimate gives the default test's values and restart count bit for bit;
+    a function that fails ends the solve with an error; None restores the
+    relative test."""
+    Ao = orc.lap2d(20)
+    v0 = orc.rand_uniform(0, Ao.n)
+
+    def run(fn):
+        E = sa.EPS()
+        E.setOperators(sa.Mat(sa.CSR(Ao.n, Ao.rowptr, Ao.col, Ao.val)))
+        E.setProblemType(True)
+        E.setDimensions(6, 20)
+        E.setTolerances(1e-10, 0)
+        E.setInitialSpace(v0)
+        if fn != "default":
+            E.setConvergenceTestFunction(fn)
+        E.solve()
+        return [E.getEigenpair(i, False)[0] for i in range(6)], E.getIterationNumber()
+
+    v1, i1 = run("default")
+    calls = []
+
+    def rel(er, ei, res):
+        calls.append(res)
+        return res / np.hypot(er, ei)
+    v2, i2 = run(rel)
+    assert v1 == v2 and i1 == i2 and len(calls) > 0
+    with pytest.raises(sa.SlepcAmdError, match="convergence test returned an error"):
+        run(lambda er, ei, res: 1 / 0)
+    v3, i3 = run(None)
+    assert v3 == v1 and i3 == i1
+    E = sa.EPS()
+    with pytest.raises(sa.SlepcAmdError, match="EPSSetConvergenceTestFunction"):
+        sa.check(sa.lib().sa_eps_set_convergence_test(E.h, 3))
