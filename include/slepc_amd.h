@@ -30,7 +30,8 @@ typedef struct sa_mlp_s *sa_mlp;   /* learned warm-start network (BF16 MFMA)  */
 
 enum { SA_FP64 = 0, SA_FP32 = 1 };
 enum { SA_LARGEST_MAGNITUDE = 1, SA_SMALLEST_MAGNITUDE = 2, SA_LARGEST_REAL = 3, SA_SMALLEST_REAL = 4,
-       SA_LARGEST_IMAGINARY = 5, SA_SMALLEST_IMAGINARY = 6, SA_TARGET_MAGNITUDE = 7, SA_TARGET_REAL = 8 };
+       SA_LARGEST_IMAGINARY = 5, SA_SMALLEST_IMAGINARY = 6, SA_TARGET_MAGNITUDE = 7, SA_TARGET_REAL = 8,
+       SA_WHICH_USER = 11 };
 enum { SA_ERROR_ABSOLUTE = 0, SA_ERROR_RELATIVE = 1, SA_ERROR_BACKWARD = 2 };
 
 /* ---- library ----------------------------------------------------------- */
@@ -460,6 +461,14 @@ int sa_eps_set_convergence_test_function(sa_eps eps, sa_eps_converged_fn fn, voi
 /* The matrix norms of SA_CONV_NORM and of backward errors, ||A||_inf and
    ||B||_inf (EPSView's "computed matrix norms"); 0 until computed. */
 int sa_eps_get_matrix_norms(sa_eps eps, double *nrma, double *nrmb);
+/* EPSSetEigenvalueComparison (epsopts.c:675-684): the order of the eigenvalues is the
+   user's: cmp(ar, ai, br, bi, &res, ctx) sets res < 0 when a goes before b,
+   > 0 when after, 0 when equal (under shift-and-invert the values are the
+   back-transformed ones, SlepcSCCompare, slepcsc.c:40-63); selects
+   SA_WHICH_USER.  A nonzero return ends the solve with an error.  NULL
+   restores SA_LARGEST_MAGNITUDE. */
+typedef int (*sa_eps_compare_fn)(double ar, double ai, double br, double bi, int *res, void *ctx);
+int sa_eps_set_eigenvalue_comparison(sa_eps eps, sa_eps_compare_fn cmp, void *ctx);
 /* Eigenpair i in the order EPSGetEigenpair returns them; vectors may be NULL. */
 int sa_eps_get_eigenpair(sa_eps eps, int i, double *re, double *im, double *xr_host, double *xi_host);
 int sa_eps_get_error_estimate(sa_eps eps, int i, double *errest);

@@ -748,10 +748,20 @@ typedef struct {
 
 /* SlepcCompare* (slepcsc.c:152-267): result>0 means (ar,ai) goes after (br,bi);
    the target ones compare distances to the target. */
+/* EPSSetEigenvalueComparison: the user's comparison for ORC_USER */
+static orc_compare_fn user_cmp;
+static void *user_cmp_ctx;
+void orc_set_user_comparison(orc_compare_fn fn, void *ctx) { user_cmp = fn; user_cmp_ctx = ctx; }
+
 static int sc_compare_t(int which, double ar, double ai, double br, double bi, double target)
 {
   double a, b;
   switch (which) {
+    case ORC_USER: {
+      int r = 0;
+      if (user_cmp) user_cmp(ar, ai, br, bi, &r, user_cmp_ctx);
+      return r > 0 ? 1 : (r < 0 ? -1 : 0);
+    }
     case ORC_LM: a = hypot(ar, ai); b = hypot(br, bi); return a < b ? 1 : (a > b ? -1 : 0);
     case ORC_SM: a = hypot(ar, ai); b = hypot(br, bi); return a > b ? 1 : (a < b ? -1 : 0);
     case ORC_LR: a = ar; b = br; return a < b ? 1 : (a > b ? -1 : 0);
@@ -1568,7 +1578,7 @@ int orc_eps_solve(int n, const int32_t *rp, const int32_t *ci, const double *val
   double *gh = (double *)calloc(ncv + 2, sizeof(double)), gamma = 1.0;
   /* shift-and-invert (sinvert.c, epssetup.c:119): dense LU of A - sigma I */
   const int sinv = o->st == 1;
-  if (sinv && (which != ORC_TM && which != ORC_TR)) return -12;
+  if (sinv && (which != ORC_TM && which != ORC_TR && which != ORC_USER)) return -12;
   if (sinv && harm) return -13;
   cmp_sinv = sinv; cmp_sigma = o->sigma;
   /* balancing (epssetup.c:399-405): non-Hermitian only; the balanced

@@ -65,7 +65,10 @@ int  orc_ds_hep_compact(int n, int l, int k, const double *d, const double *e, i
                         double *wr, double *Q);
 
 /* ---- Krylov-Schur and Arnoldi solves ----------------------------------- */
-enum { ORC_LM = 1, ORC_SM = 2, ORC_LR = 3, ORC_SR = 4, ORC_LI = 5, ORC_SI = 6, ORC_TM = 7, ORC_TR = 8 };
+enum { ORC_LM = 1, ORC_SM = 2, ORC_LR = 3, ORC_SR = 4, ORC_LI = 5, ORC_SI = 6, ORC_TM = 7, ORC_TR = 8, ORC_USER = 11 };
+/* EPSSetEigenvalueComparison: the comparison for ORC_USER (res < 0: a first). */
+typedef int (*orc_compare_fn)(double ar, double ai, double br, double bi, int *res, void *ctx);
+void orc_set_user_comparison(orc_compare_fn fn, void *ctx);
 typedef struct {
   int     hermitian;   /* 1: Lanczos + DSHEP, 0: Arnoldi + DSNHEP */
   int     which;       /* ORC_LM ... */

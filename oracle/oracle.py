@@ -14,7 +14,7 @@ _HERE = os.path.dirname(os.path.abspath(__file__))
 _LIB = None
 
 LM, SM, LR, SR = 1, 2, 3, 4
-WHICH = {"LM": LM, "SM": SM, "LR": LR, "SR": SR, "LI": 5, "SI": 6, "TM": 7, "TR": 8}
+WHICH = {"LM": LM, "SM": SM, "LR": LR, "SR": SR, "LI": 5, "SI": 6, "TM": 7, "TR": 8, "USER": 11}
 
 
 class Opts(C.Structure):
@@ -314,6 +314,24 @@ def lanczos_local_sweep(A, m, v0):
     steps = f(A.n, P(A.rowptr.ctypes.data), P(A.col.ctypes.data), P(A.val.ctypes.data), m, P(x0.ctypes.data),
               P(V.ctypes.data), P(al.ctypes.data), P(be.ctypes.data))
     return V[:steps + 1].T.copy(), al[:steps], be[:steps]
+
+
+_CMPFN = C.CFUNCTYPE(C.c_int, C.c_double, C.c_double, C.c_double, C.c_double, C.POINTER(C.c_int), C.c_void_p)
+
+
+def eps_solve_user_order(A, nev, cmp, **kw):
+    """eps_solve with which = USER: cmp(ar, ai, br, bi) returns < 0 when a goes
+    first (EPSSetEigenvalueComparison)."""
+    def tramp(ar, ai, br, bi, res, _ctx):
+        res[0] = int(cmp(ar, ai, br, bi))
+        return 0
+    cb = _CMPFN(tramp)
+    lib().orc_set_user_comparison.argtypes = [_CMPFN, C.c_void_p]
+    lib().orc_set_user_comparison(cb, None)
+    try:
+        return eps_solve(A, nev, which="USER", **kw)
+    finally:
+        lib().orc_set_user_comparison(_CMPFN(), None)
 
 
 def eps_solve(A, nev, ncv=0, tol=1e-8, which="LM", hermitian=True, v0=None, max_it=0,

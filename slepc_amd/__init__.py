@@ -580,6 +580,26 @@ class EPS:
         self._conv_cb = self._CONVERGED(tramp)
         check(lib().sa_eps_set_convergence_test_function(self.h, C.cast(self._conv_cb, C.c_void_p), None))
 
+    _COMPARE = C.CFUNCTYPE(C.c_int, C.c_double, C.c_double, C.c_double, C.c_double, C.POINTER(C.c_int),
+                           C.c_void_p)
+
+    def setEigenvalueComparison(self, fn):
+        """EPSSetEigenvalueComparison: fn(ar, ai, br, bi) < 0 when a goes first
+        (which becomes EPS_WHICH_USER); None restores largest magnitude."""
+        if fn is None:
+            self._cmp_cb = None
+            check(lib().sa_eps_set_eigenvalue_comparison(self.h, None, None))
+            return
+
+        def tramp(ar, ai, br, bi, res, _ctx):
+            try:
+                res[0] = int(fn(ar, ai, br, bi))
+                return 0
+            except Exception:
+                return 1
+        self._cmp_cb = self._COMPARE(tramp)
+        check(lib().sa_eps_set_eigenvalue_comparison(self.h, C.cast(self._cmp_cb, C.c_void_p), None))
+
     def getMatrixNorms(self):
         """(||A||_inf, ||B||_inf) as computed for the norm test or a backward
         error; 0 until computed."""

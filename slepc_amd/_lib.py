@@ -139,6 +139,7 @@ SIGNATURES = {
     "sa_eps_set_convergence_test": (C.c_int, [_VP, C.c_int]),
     "sa_eps_set_convergence_test_function": (C.c_int, [_VP, _VP, _VP]),
     "sa_eps_get_matrix_norms": (C.c_int, [_VP, C.POINTER(C.c_double), C.POINTER(C.c_double)]),
+    "sa_eps_set_eigenvalue_comparison": (C.c_int, [_VP, _VP, _VP]),
     "sa_eps_get_eigenpair": (C.c_int, [_VP, C.c_int, C.POINTER(C.c_double), C.POINTER(C.c_double), _VP, _VP]),
     "sa_eps_get_error_estimate": (C.c_int, [_VP, C.c_int, C.POINTER(C.c_double)]),
     "sa_eps_compute_error": (C.c_int, [_VP, C.c_int, C.c_int, C.POINTER(C.c_double)]),

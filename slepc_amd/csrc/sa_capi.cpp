@@ -109,6 +109,7 @@ struct sa_eps_s {
   // EPSSetConvergenceTestFunction (epsopts.c:751-770)
   sa_eps_converged_fn converged = nullptr;
   void *converged_ctx = nullptr;
+  ScUser cmp;                      // EPSSetEigenvalueComparison (EPS_WHICH_USER)
   double nrma = 0.0, nrmb = 0.0;   // ||A||_inf, ||B||_inf (0 until computed; reset with the operators)
 };
 
@@ -684,8 +685,18 @@ int sa_eps_set_type(sa_eps E, int type) {
 }
 
 int sa_eps_set_which(sa_eps E, int which) {
-  SA_CHECK(E && which >= SA_LARGEST_MAGNITUDE && which <= SA_TARGET_REAL, "invalid which");
+  SA_CHECK(E && ((which >= SA_LARGEST_MAGNITUDE && which <= SA_TARGET_REAL) || which == SA_WHICH_USER),
+           "invalid which");
   E->p.which = which;
+  return 0;
+}
+
+int sa_eps_set_eigenvalue_comparison(sa_eps E, sa_eps_compare_fn cmp, void *ctx) {
+  SA_CHECK(E, "null EPS");
+  E->cmp.fn = cmp;
+  E->cmp.ctx = cmp ? ctx : nullptr;
+  E->p.which = cmp ? SA_WHICH_USER : SA_LARGEST_MAGNITUDE;
+  E->solved = false;
   return 0;
 }
 
@@ -964,9 +975,29 @@ int sa_eps_set_balance(sa_eps E, int balance, int its, double cutoff) {
   return 0;
 }
 
+static int eps_solve_impl(sa_eps E);
+
 int sa_eps_solve(sa_eps E) {
   API_INIT();
   SA_CHECK(E && E->A, "EPS has no operator");
+  SA_CHECK(E->p.which != SA_WHICH_USER || E->cmp.fn, "EPS_WHICH_USER needs EPSSetEigenvalueComparison");
+  // the user's comparison serves the sorts of this solve on this thread
+  E->cmp.err = 0;
+  struct Install {
+    ScUser *prev;
+    explicit Install(ScUser *u) : prev(t_sc_user) { t_sc_user = u; }
+    ~Install() { t_sc_user = prev; }
+  } install(E->p.which == SA_WHICH_USER ? &E->cmp : nullptr);
+  const int rc = eps_solve_impl(E);
+  if (E->cmp.err) {
+    E->solved = false;
+    set_error("the eigenvalue comparison returned an error code");
+    return 2;
+  }
+  return rc;
+}
+
+static int eps_solve_impl(sa_eps E) {
   KsParams p;
   SA_TRY(eps_setup_dims(E, p));
   const Mat &A = E->A->A;

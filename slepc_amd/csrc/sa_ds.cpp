@@ -181,9 +181,17 @@ int la_syev_lower(int n, double *a, int lda, double *w) {
   return info;
 }
 
+thread_local ScUser *t_sc_user = nullptr;
+
 int sc_compare(int which, double ar, double ai, double br, double bi, double target) {
   double a, b;
   switch (which) {
+    case 11: {   // SlepcSCCompare with the user's comparison (slepcsc.c:40-63)
+      if (!t_sc_user || !t_sc_user->fn) return 0;
+      int r = 0;
+      if (t_sc_user->fn(ar, ai, br, bi, &r, t_sc_user->ctx)) t_sc_user->err = 1;
+      return r > 0 ? 1 : (r < 0 ? -1 : 0);
+    }
     case 1: a = std::hypot(ar, ai); b = std::hypot(br, bi); return a < b ? 1 : (a > b ? -1 : 0);
     case 2: a = std::hypot(ar, ai); b = std::hypot(br, bi); return a > b ? 1 : (a < b ? -1 : 0);
     case 3: return ar < br ? 1 : (ar > br ? -1 : 0);

@@ -25,6 +25,14 @@ enum DsState { DS_RAW = 0, DS_INTERMEDIATE = 1, DS_CONDENSED = 2, DS_TRUNCATED =
 // SlepcCompare* (src/sys/slepcsc.c:152-267): >0 when (ar,ai) must follow (br,bi).
 // which as EPSWhich: 1 LM, 2 SM, 3 LR, 4 SR, 5 LI, 6 SI, 7 target magnitude,
 // 8 target real (distance to target).
+// EPSSetEigenvalueComparison (EPS_WHICH_USER, which = 11): the user comparator
+// of the solve running on this thread; a nonzero return of fn sets err.
+struct ScUser {
+  int (*fn)(double ar, double ai, double br, double bi, int *res, void *ctx) = nullptr;
+  void *ctx = nullptr;
+  int err = 0;
+};
+extern thread_local ScUser *t_sc_user;
 int sc_compare(int which, double ar, double ai, double br, double bi, double target = 0.0);
 
 class DenseProblem {

@@ -629,8 +629,8 @@ int ks_solve(const KsParams &p, const Mat &A, BV &V, bool have_initial, KsResult
   // is a solve with A - sigma I; the DS compares back-transformed values and
   // the eigenvalues are mapped back (EPSComputeValues) before the final sort.
   const bool sinv = p.st_type == 1;
-  SA_CHECK(!sinv || (p.which == SA_TARGET_MAGNITUDE || p.which == SA_TARGET_REAL),
-           "Shift-and-invert requires a target 'which' (target magnitude or target real)");
+  SA_CHECK(!sinv || (p.which == SA_TARGET_MAGNITUDE || p.which == SA_TARGET_REAL || p.which == SA_WHICH_USER),
+           "Shift-and-invert requires a target 'which' (target magnitude, target real or user)");
   SA_CHECK(!(sinv && harm), "Cannot use a spectral transformation combined with harmonic extraction");
   KspWork kw;
   struct KspFree {

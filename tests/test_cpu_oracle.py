@@ -1051,3 +1051,26 @@ def test_oracle_convergence_criteria(orc, conv, which):
         assert max(rel) > 1e-7
     else:
         assert max(rel) <= 1e-8
+
+
+def _right_of(t):
+    """test11.c's MyEigenSort: eigenvalues to the right of t first, then the
+    closest to t."""
+    def cmp(ar, ai, br, bi):
+        ra, rb = t < ar, t < br
+        if ra == rb:
+            da, db = np.hypot(ar - t, ai), np.hypot(br - t, bi)
+            return -1 if da < db else (1 if da > db else 0)
+        return -1 if ra else 1
+    return cmp
+
+
+def test_oracle_user_comparison_test11(orc):
+    """eps/tests/test11.c suffix 1 (Krylov-Schur, -st_type sinvert, shift 0.5,
+    start vector of ones, the user's order "closest to the right of 0.5"):
+    test11_1.out prints 0.51928, 0.55740, 0.57028, 0.57143."""
+    A = orc.markov(15)
+    r = orc.eps_solve_user_order(A, 4, _right_of(0.5), tol=1e-10, hermitian=False, v0=np.ones(A.n),
+                                 st="sinvert", target=0.5)
+    assert r["nconv"] >= 4
+    assert ["%.5f" % v for v in r["eigr"][:4]] == ["0.51928", "0.55740", "0.57028", "0.57143"]

@@ -209,3 +209,60 @@ def test_user_convergence_function(sa, orc):
     E = sa.EPS()
     with pytest.raises(sa.SlepcAmdError, match="EPSSetConvergenceTestFunction"):
         sa.check(sa.lib().sa_eps_set_convergence_test(E.h, 3))
+
+
+@pytest.mark.parametrize("solver", ["krylovschur", "arnoldi"])
+def test_user_comparison_test11(sa, orc, solver):
+    """EPSSetEigenvalueComparison, eps/tests/test11.c suffix 1 (-eps_type
+    {krylovschur, arnoldi} -st_type sinvert, shift 0.5, ones as start
+    vector): the eigenvalues closest to the right of 0.5 print 0.51928,
+    0.55740, 0.57028, 0.57143 (test11_1.out), equal to the oracle run with the
+    same comparison in values and restarts."""
+    from test_cpu_oracle import _right_of
+    Ao = orc.markov(15)
+    E = sa.EPS()
+    E.setType(solver)
+    E.setOperators(sa.Mat(sa.CSR(Ao.n, Ao.rowptr, Ao.col, Ao.val)))
+    E.setProblemType(False)
+    E.setDimensions(4, 0)
+    E.setTolerances(1e-10, 0)
+    E.setTarget(0.5)
+    E.setST("sinvert", 0.5)
+    E.setEigenvalueComparison(_right_of(0.5))
+    E.setInitialSpace(np.ones(Ao.n))
+    E.solve()
+    r = orc.eps_solve_user_order(Ao, 4, _right_of(0.5), tol=1e-10, hermitian=False, v0=np.ones(Ao.n),
+                                 st="sinvert", target=0.5, solver=solver)
+    assert E.getConverged() >= 4
+    vals = [E.getEigenpair(i, False)[0] for i in range(4)]
+    assert ["%.5f" % v for v in vals] == ["0.51928", "0.55740", "0.57028", "0.57143"]
+    assert np.allclose(vals, r["eigr"][:4], rtol=1e-10, atol=0)
+    assert abs(E.getIterationNumber() - r["its"]) <= 1
+
+
+def test_user_comparison_as_largest_magnitude_and_errors(sa, orc):
+    """A comparison that orders by decreasing magnitude reproduces which = LM
+    bit for bit; a comparison that fails ends the solve with an error; None
+    restores largest magnitude."""
+    Ao = orc.markov(15)
+    v0 = orc.rand_uniform(0, Ao.n)
+
+    def run(cmp):
+        E = sa.EPS()
+        E.setOperators(sa.Mat(sa.CSR(Ao.n, Ao.rowptr, Ao.col, Ao.val)))
+        E.setProblemType(False)
+        E.setDimensions(4, 0)
+        E.setTolerances(1e-10, 0)
+        if cmp != "LM":
+            E.setEigenvalueComparison(cmp)
+        E.setInitialSpace(v0)
+        E.solve()
+        return [E.getEigenpair(i, False)[:2] for i in range(4)], E.getIterationNumber()
+
+    def lm(ar, ai, br, bi):
+        a, b = np.hypot(ar, ai), np.hypot(br, bi)
+        return 1 if a < b else (-1 if a > b else 0)
+    assert run(lm) == run("LM")
+    with pytest.raises(sa.SlepcAmdError, match="comparison returned an error"):
+        run(lambda ar, ai, br, bi: 1 / 0)
+    assert run(None) == run("LM")
