@@ -266,3 +266,38 @@ def test_user_comparison_as_largest_magnitude_and_errors(sa, orc):
     with pytest.raises(sa.SlepcAmdError, match="comparison returned an error"):
         run(lambda ar, ai, br, bi: 1 / 0)
     assert run(None) == run("LM")
+
+
+def test_user_comparisons_on_concurrent_threads(sa, orc):
+    """Each host thread's solve uses its own comparison (the comparison is
+    installed per thread for the duration of a solve): four concurrent
+    solves, two ordering by largest magnitude and two by smallest real part,
+    each give the same values and restart count as the same solve run alone."""
+    import concurrent.futures as cf
+    Ao = orc.lap2d(20)
+    v0 = orc.rand_uniform(0, Ao.n)
+
+    def lm(ar, ai, br, bi):
+        a, b = np.hypot(ar, ai), np.hypot(br, bi)
+        return 1 if a < b else (-1 if a > b else 0)
+
+    def sr(ar, ai, br, bi):
+        return 1 if ar > br else (-1 if ar < br else 0)
+
+    def run(cmp):
+        E = sa.EPS()
+        E.setOperators(sa.Mat(sa.CSR(Ao.n, Ao.rowptr, Ao.col, Ao.val)))
+        E.setProblemType(True)
+        E.setDimensions(4, 16)
+        E.setTolerances(1e-10, 5000)
+        E.setEigenvalueComparison(cmp)
+        E.setInitialSpace(v0)
+        E.solve()
+        return [E.getEigenpair(i, False)[0] for i in range(4)], E.getIterationNumber()
+
+    alone = {"lm": run(lm), "sr": run(sr)}
+    assert alone["lm"][0][0] > 7.9 and alone["sr"][0][0] < 0.05
+    with cf.ThreadPoolExecutor(max_workers=4) as ex:
+        futs = [(k, ex.submit(run, f)) for k, f in (("lm", lm), ("sr", sr), ("lm", lm), ("sr", sr))]
+        for k, f in futs:
+            assert f.result() == alone[k]
