@@ -404,6 +404,14 @@ int sa_eps_set_tolerances(sa_eps eps, double tol, int max_it);
    stored consecutively.  As in the reference, Krylov-Schur starts from the
    first one after orthonormalisation. */
 int sa_eps_set_initial_space(sa_eps eps, int nvec, const double *vecs_host);
+/* EPSSetDeflationSpace (epssetup.c:544-583): nvec host vectors of length n,
+   stored consecutively.  The solve seeks eigenpairs in their orthogonal
+   complement: they are orthonormalized in order (a linearly dependent one is
+   dropped, BVInsertVecs) and every Krylov vector is kept orthogonal to them.
+   As in the reference the space is removed at the end of each solve, so it is
+   set before every solve.  Krylov-Schur and Arnoldi, FP64, one-sided; up to
+   128 vectors. */
+int sa_eps_set_deflation_space(sa_eps eps, int nvec, const double *vecs_host);
 /* Precision: SA_FP64 plain, or mixed (bulk FP32 to tol32, then FP64 polish
    seeded with the FP32 Ritz vectors). */
 int sa_eps_set_mixed(sa_eps eps, int mixed, double tol32);

@@ -526,6 +526,15 @@ class EPS:
             V = np.ascontiguousarray(V.T)
         check(lib().sa_eps_set_initial_space(self.h, V.shape[0], V.ravel()))
 
+    def setDeflationSpace(self, vecs):
+        """EPSSetDeflationSpace (src/eps/interface/epssetup.c:544-583): the
+        solve looks for eigenpairs orthogonal to these vectors (rows, or
+        columns of an n x k array).  The space is removed by the next solve."""
+        V = np.ascontiguousarray(np.atleast_2d(np.asarray(vecs, np.float64)))
+        if V.shape[1] != self.A.n:
+            V = np.ascontiguousarray(V.T)
+        check(lib().sa_eps_set_deflation_space(self.h, V.shape[0], V.ravel()))
+
     def setMixedPrecision(self, mixed=True, tol32=1e-5):
         check(lib().sa_eps_set_mixed(self.h, 1 if mixed else 0, tol32))
 

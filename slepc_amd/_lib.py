@@ -124,6 +124,7 @@ SIGNATURES = {
     "sa_eps_set_dimensions": (C.c_int, [_VP, C.c_int, C.c_int, C.c_int]),
     "sa_eps_set_tolerances": (C.c_int, [_VP, C.c_double, C.c_int]),
     "sa_eps_set_initial_space": (C.c_int, [_VP, C.c_int, _F64P]),
+    "sa_eps_set_deflation_space": (C.c_int, [_VP, C.c_int, _F64P]),
     "sa_eps_set_mixed": (C.c_int, [_VP, C.c_int, C.c_double]),
     "sa_eps_solve": (C.c_int, [_VP]),
     "sa_eps_get_converged": (C.c_int, [_VP, C.POINTER(C.c_int)]),

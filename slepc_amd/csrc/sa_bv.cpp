@@ -135,6 +135,65 @@ int bv_sweep_sinvert(BV &V, const Mat &A, KspWork &w, double rtol, int max_it, i
   return 0;
 }
 
+// x -= C C^T x with classical Gram-Schmidt, `passes` times.  The reference
+// keeps the deflation space as constraint columns of V (BVInsertConstraints,
+// epssetup.c:412-418) that every BVOrthogonalizeColumn runs against together
+// with the basis; one pass leaves components of order eps ||x|| /
+// ||(I - CC^T) x||, the second removes them.
+int bv_project_out(BV &C, int nc, void *x, hipStream_t s, int passes) {
+  if (nc <= 0) return 0;
+  SA_CHECK(C.prec == SA_FP64 && nc <= C.m, "invalid deflation space");
+  for (int pass = 0; pass < passes; pass++) {
+    SA_TRY(launch_dotvec(C, 0, nc, x, C.dq, s));
+    SA_TRY(launch_multvec(C, 0, nc, -1.0, C.dq, 1.0, x, s));
+  }
+  return 0;
+}
+
+// BVMatArnoldi (bvkrylov.c:56-113) with constraints: each new column is the
+// operator applied to the previous one, orthogonalized against the deflation
+// space and then against V(:, 0:j+1) by the Gram-Schmidt slots (no SpMV in
+// them).  The slots subtract V h, and V carries deflation components of
+// order eps; divided by the new column's norm beta they would grow from
+// column to column when beta is small, so one more pass against the space
+// follows the slots (the reference's joint refinement over constraints and
+// basis does the same).  With w the operator is the shift-and-invert solve,
+// as in bv_sweep_sinvert.
+int bv_sweep_deflated(BV &V, const Mat &A, KspWork *w, double rtol, int max_it, BV &C, int nc, int k, int *m,
+                      double eta, double *beta, int *breakdown, int64_t *ksp_its) {
+  SA_CHECK(k >= 0 && *m > k && *m < V.m, "invalid Arnoldi range");
+  SA_CHECK(A.prec == SA_FP64 && V.prec == SA_FP64 && A.n == V.n && C.n == V.n,
+           "the deflation space needs an FP64 operator and basis of its length");
+  hipStream_t s = stream();
+  SA_TRY(launch_gs_reset(V, s));
+  int stop = 0;
+  for (int j = k; j < *m && !stop; j++) {
+    if (w) {
+      int its = 0;
+      SA_TRY(ksp_solve(A, *w, (const double *)col_ptr(V, j), (double *)col_ptr(V, j + 1), rtol, max_it, &its));
+      if (ksp_its) *ksp_its += its;
+    } else {
+      SA_TRY(launch_spmv(A, col_ptr(V, j), col_ptr(V, j + 1), s));
+    }
+    SA_TRY(bv_project_out(C, nc, col_ptr(V, j + 1), s));
+    SA_TRY(launch_gs_column(V, nullptr, j + 1, eta, s));
+    SA_TRY(bv_project_out(C, nc, col_ptr(V, j + 1), s, 1));
+    SA_HIP(hipMemcpyAsync(V.st_host, V.st + 2, sizeof(GsState), hipMemcpyDeviceToHost, s));
+    SA_HIP(hipStreamSynchronize(s));
+    stop = V.st_host->stop;
+  }
+  const GsState &st = *V.st_host;
+  if (st.stop) {
+    *m = st.stop_col;
+    *breakdown = 1;
+  } else {
+    *breakdown = 0;
+  }
+  *beta = st.beta;
+  V.passes_done = st.passes_total;
+  return 0;
+}
+
 // BVOrthonormalizeColumn(V,j,PETSC_FALSE,&norm,&lindep) (bvorthog.c:380-427)
 int bv_orthonormalize(BV &V, int j, double eta, double *norm, int *lindep) {
   hipStream_t s = stream();

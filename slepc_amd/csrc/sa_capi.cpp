@@ -87,6 +87,10 @@ struct sa_eps_s {
   bool ncv_set = false, mpd_set = false, maxit_set = false;
   std::vector<double> init;   // initial space, nini x n
   int nini = 0;
+  std::vector<double> defl;   // deflation space, nds x n (EPSSetDeflationSpace), cleared by each solve
+  int nds = 0;
+  int64_t defl_n = 0;
+  BV D;                       // its orthonormalized columns on the device
   bool mixed = false;
   double tol32 = 1e-5;
   BV V;                        // FP64 basis holding the solution
@@ -634,6 +638,7 @@ int sa_eps_destroy(sa_eps E) {
   if (E->rpart) (void)hipFree(E->rpart);
   bv_free(E->V);
   bv_free(E->W);
+  bv_free(E->D);
   if (E->AT) sa_mat_destroy(E->AT);
   delete E;
   return 0;
@@ -840,6 +845,17 @@ int sa_eps_set_initial_space(sa_eps E, int nvec, const double *vecs) {
   return 0;
 }
 
+int sa_eps_set_deflation_space(sa_eps E, int nvec, const double *vecs) {
+  SA_CHECK(E && E->A, "set the operator first");
+  SA_CHECK(nvec >= 0 && (nvec == 0 || vecs), "invalid deflation space");   // epssetup.c:575
+  SA_CHECK(nvec <= GS_MAXC, "a deflation space of at most 128 vectors is supported");
+  const int64_t n = E->A->A.n;
+  E->defl.assign(vecs, vecs + (size_t)nvec * n);
+  E->nds = nvec;
+  E->defl_n = n;
+  return 0;
+}
+
 int sa_eps_set_mixed(sa_eps E, int mixed, double tol32) {
   SA_CHECK(E, "null EPS");
   E->mixed = mixed != 0;
@@ -989,6 +1005,9 @@ int sa_eps_solve(sa_eps E) {
     ~Install() { t_sc_user = prev; }
   } install(E->p.which == SA_WHICH_USER ? &E->cmp : nullptr);
   const int rc = eps_solve_impl(E);
+  // EPSSolve removes the deflation space at its end (epssolve.c:201-205)
+  E->defl.clear();
+  E->nds = 0;
   if (E->cmp.err) {
     E->solved = false;
     set_error("the eigenvalue comparison returned an error code");
@@ -1001,6 +1020,32 @@ static int eps_solve_impl(sa_eps E) {
   KsParams p;
   SA_TRY(eps_setup_dims(E, p));
   const Mat &A = E->A->A;
+  if (E->nds > 0) {
+    // EPSSetUp (epssetup.c:412-418): BVInsertConstraints orthonormalizes the
+    // deflation vectors in order and drops each one that is linearly
+    // dependent on those before it (BVInsertVecs, bvfunc.c:359-374).
+    SA_CHECK(E->defl_n == A.n, "the deflation vectors do not have the operator's length");
+    SA_CHECK(!E->mixed && !E->twosided && p.solver != 2 && !p.arnoldi_delayed &&
+                 !(E->balance != SA_BALANCE_NONE && !p.hermitian),
+             "the deflation space runs with one-sided FP64 Krylov-Schur or Arnoldi (no delayed "
+             "reorthogonalization, no balancing)");
+    if (E->D.m != E->nds || E->D.n != A.n) {
+      bv_free(E->D);
+      SA_TRY(bv_alloc(E->D, A.n, E->nds, SA_FP64));
+    }
+    int kd = 0;
+    for (int i = 0; i < E->nds; i++) {
+      SA_TRY(bv_upload_column(E->D, kd, E->defl.data() + (size_t)i * A.n));
+      double norm = 0.0;
+      int lindep = 0;
+      SA_TRY(bv_orthonormalize(E->D, kd, p.eta, &norm, &lindep));
+      if (!(norm == 0.0 || lindep)) kd++;
+    }
+    if (kd > 0) {
+      p.defl = &E->D;
+      p.ndefl = kd;
+    }
+  }
   if (E->V.m != p.ncv + 1 || E->V.n != A.n || E->V.prec != SA_FP64) {
     bv_free(E->V);
     SA_TRY(bv_alloc(E->V, A.n, p.ncv + 1, SA_FP64));

@@ -94,15 +94,17 @@ int bv_download_column(BV &V, int j, double *x) {
 // EPSGetStartVector (epssolve.c:841-873): column i is a deterministic random
 // vector unless an initial vector is already in place (i == 0 && nini > 0).
 int eps_start_vector(BV &V, int i, bool have_initial, double eta, int *breakdown, std::vector<double> &scratch,
-                     uint64_t seed) {
+                     uint64_t seed, BV *defl, int ndefl) {
   if (i > 0 || !have_initial) {
     scratch.resize(V.n);
     sa_rand_uniform(seed + (uint64_t)i, V.n, scratch.data());
     SA_TRY(bv_upload_column(V, i, scratch.data()));
   }
+  if (defl) SA_TRY(bv_project_out(*defl, ndefl, col_ptr(V, i), stream()));   // the constraints come first
   double norm = 0.0;
   int lindep = 0;
   SA_TRY(bv_orthonormalize(V, i, eta, &norm, &lindep));
+  if (defl) SA_TRY(bv_project_out(*defl, ndefl, col_ptr(V, i), stream(), 1));   // as after each Arnoldi column
   if (breakdown) *breakdown = lindep;
   else SA_CHECK(!(lindep || norm == 0.0), "Initial vector is zero or belongs to the deflation space");
   return 0;
@@ -577,7 +579,7 @@ static int lanczos_restart(const KsParams &p, BV &V, const DenseProblem &ds, int
     }
     if (breakdown) {
       int bd = 0;
-      SA_TRY(eps_start_vector(V, nconv, false, p.eta, &bd, scratch));
+      SA_TRY(eps_start_vector(V, nconv, false, p.eta, &bd, scratch, EPS_SEED_RIGHT, p.defl, p.ndefl));
       if (bd) reason = -2;
     }
   }
@@ -709,6 +711,9 @@ int ks_solve(const KsParams &p, const Mat &A, BV &V, bool have_initial, KsResult
     return 0;
   };
   auto sweep = [&](int k0, int *nv, double *beta, int *bd) -> int {
+    if (p.defl)
+      return bv_sweep_deflated(V, A, sinv ? &kw : nullptr, p.ksp_rtol, p.ksp_max_it, *p.defl, p.ndefl, k0, nv, p.eta,
+                               beta, bd, &r.ksp_its);
     if (lanc && p.lanczos_local) return sweep_local(k0, nv, beta, bd);
     if (!sinv) return bv_sweep(V, A, k0, nv, p.eta, beta, bd);
     return bv_sweep_sinvert(V, A, kw, p.ksp_rtol, p.ksp_max_it, k0, nv, p.eta, beta, bd, &r.ksp_its);
@@ -732,11 +737,12 @@ int ks_solve(const KsParams &p, const Mat &A, BV &V, bool have_initial, KsResult
   if (have_initial) {  // BVInsertVecs(V,0,&k,IS,TRUE) (bvfunc.c:331-374)
     double norm = 0.0;
     int lindep = 0;
+    if (p.defl) SA_TRY(bv_project_out(*p.defl, p.ndefl, col_ptr(V, 0), s));
     SA_TRY(bv_orthonormalize(V, 0, p.eta, &norm, &lindep));
     SA_CHECK(!(norm == 0.0 || lindep), "initial vector is linearly dependent");
     r.bytes_bv += 3.0 * V.n * es;
   }
-  SA_TRY(eps_start_vector(V, 0, have_initial, p.eta, nullptr, scratch));
+  SA_TRY(eps_start_vector(V, 0, have_initial, p.eta, nullptr, scratch, EPS_SEED_RIGHT, p.defl, p.ndefl));
   r.bytes_bv += 3.0 * V.n * es;
 
   int l = 0, nconv = 0, its = 0, reason = 0;
@@ -886,7 +892,7 @@ int ks_solve(const KsParams &p, const Mat &A, BV &V, bool have_initial, KsResult
       SA_TRY(eps_stopping(p, its, k, nev, &reason));
       if (reason == 0 && breakdown) {    // arnoldi.c:118-125
         int bd = 0;
-        SA_TRY(eps_start_vector(V, k, false, p.eta, &bd, scratch));
+        SA_TRY(eps_start_vector(V, k, false, p.eta, &bd, scratch, EPS_SEED_RIGHT, p.defl, p.ndefl));
         if (bd) reason = -2;
       }
       nconv = k;
@@ -911,7 +917,7 @@ int ks_solve(const KsParams &p, const Mat &A, BV &V, bool have_initial, KsResult
       if (breakdown || k == nv) {
         if (k < nev) {
           int bd = 0;
-          SA_TRY(eps_start_vector(V, k, false, p.eta, &bd, scratch));
+          SA_TRY(eps_start_vector(V, k, false, p.eta, &bd, scratch, EPS_SEED_RIGHT, p.defl, p.ndefl));
           if (bd) reason = -2;
         }
       } else {

@@ -45,6 +45,8 @@ struct KsParams {
   bool trueres = false;          // EPSSetTrueResidual (Krylov-Schur and Arnoldi)
   int lanczos_partial = 0;     // EPSLanczosSetReorthog: 1 periodic, 2 partial, 3 selective, 4 delayed
   int refine = 0;              // BVSetOrthogonalization refinement of the basis: GS_REFINE_*
+  BV *defl = nullptr;          // EPSSetDeflationSpace: ndefl orthonormal columns the basis is kept orthogonal to
+  int ndefl = 0;
   KsHooks hooks;
 };
 
@@ -84,7 +86,7 @@ int ks_solve(const KsParams &p, const Mat &A, BV &V, bool have_initial, KsResult
 // seed + i, orthonormalised against columns 0..i-1.
 constexpr uint64_t EPS_SEED_RIGHT = 0x5EEDULL, EPS_SEED_LEFT = 0x1EF7ULL;
 int eps_start_vector(BV &V, int i, bool have_initial, double eta, int *breakdown, std::vector<double> &scratch,
-                     uint64_t seed = EPS_SEED_RIGHT);
+                     uint64_t seed = EPS_SEED_RIGHT, BV *defl = nullptr, int ndefl = 0);
 
 // EPSComputeRitzVector + EPSComputeResidualNorm_Private (epsdefault.c:386-398,
 // epssolve.c:666-718): x = V(:,0:nv) Z(:,0) (and the imaginary part

@@ -192,6 +192,13 @@ int ksp_solve(const Mat &A, KspWork &w, const double *b_dev, double *x_dev, doub
 
 int bv_sweep_sinvert(BV &V, const Mat &A, KspWork &w, double rtol, int max_it, int k, int *m, double eta,
                      double *beta, int *breakdown, int64_t *ksp_its);
+// x -= C(:,0:nc) (C(:,0:nc)^T x), `passes` times; C has orthonormal columns.
+int bv_project_out(BV &C, int nc, void *x, hipStream_t s, int passes = 2);
+// The Arnoldi sweep with a deflation space C (nc columns): the operator
+// (SpMV, or the shift-and-invert solve when w is given), C projected out,
+// then the Gram-Schmidt slots without SpMV.
+int bv_sweep_deflated(BV &V, const Mat &A, KspWork *w, double rtol, int max_it, BV &C, int nc, int k, int *m,
+                      double eta, double *beta, int *breakdown, int64_t *ksp_its);
 
 // TSQR of V(:, l:k) (k - l <= 64, V.n >= k - l): R_host ((k-l) x (k-l)) gets
 // the R of the tile tree; with formq, V(:, l:k) is overwritten by Q times the
